@@ -1,0 +1,153 @@
+"""Benchmark of the Joseph forward + matched adjoint on one MI355X.
+
+Workload (the 3D cone-beam drop-in case): 256^3 FP32 modified Shepp-Logan volume, 360 poses
+over 360 degrees (LinSpaced, both ends), 384x384 detector, source-centre 25600, centre-
+detector 1280. One step = one forward projection plus one back projection of the whole
+volume/sinogram, inputs resident in HBM.
+
+Prints ONE JSON line:
+  value       whole-job Joseph samples per second, (S_fwd + S_adj) * steps / wall time, Gsamples/s
+  roofline    dominant kernel: algorithmic bytes per launch (16 B per sample = 4 FP32 taps, plus
+              4 B per ray) / its mean duration from HIP events on the launch stream, against
+              MI355X's 8 TB/s HBM
+  cpu_baseline  the CPU restatement (oracle/, OpenMP over rays) on a bounded sample of the same
+              workload (a subset of the poses), on this host's cores
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "Joseph fwd+adj Gsamples/s & %HBM roofline; CG wall time to fixed rel. error"
+HBM_PEAK_GBS = 8000.0
+WORKLOADS = {
+    "cone3d_256": dict(dim=3, vol=(256, 256, 256), views=360, det=(384, 384), s2c=25600.0, c2d=1280.0,
+                       text="3D cone-beam Joseph fwd+adj: 256^3 f32 Shepp-Logan, 360 poses over 360 deg, "
+                            "384x384 detector, s2c 25600, c2d 1280"),
+    "fan2d_256": dict(dim=2, vol=(256, 256), views=180, det=(384,), s2c=25600.0, c2d=1280.0,
+                      text="2D fan-beam Joseph fwd+adj: 256^2 f32 Shepp-Logan, 180 poses over 360 deg, "
+                           "384 detector, s2c 25600, c2d 1280"),
+}
+
+
+def parse():
+    p = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--workload", default="cone3d_256", choices=sorted(WORKLOADS))
+    p.add_argument("--cpu-poses", type=int, default=24,
+                   help="poses of the bounded CPU sample (evenly spread over the trajectory)")
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    return p.parse_args()
+
+
+def cpu_baseline(wl, x_np, n_poses):
+    """Times the oracle (FP32, elsa's own position rule) on n_poses of the trajectory."""
+    import oracle
+
+    g = oracle.circle_geometry(wl["dim"], wl["vol"], wl["views"], 360, wl["s2c"], wl["c2d"], det=wl["det"])
+    idx = np.linspace(0, wl["views"] - 1, n_poses).round().astype(int)
+    sub = oracle.Geom(g.dim, g.vol, g.det, g.P[idx], g.Pinv[idx], g.C[idx], g.ext[idx], g.sdd[idx])
+    S = oracle.joseph_sample_count(sub)
+    threads = len(os.sched_getaffinity(0))
+    threads = min(threads, int(os.environ.get("OMP_NUM_THREADS", threads)))
+    t0 = time.perf_counter()
+    y = oracle.joseph(sub, x_np, nthreads=threads)
+    oracle.joseph(sub, y, adjoint=True, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": round(2 * S / dt / 1e9, 5), "unit": "Gsamples/s", "cores": threads, "kind": "port",
+            "sample": f"{n_poses} of {wl['views']} poses of the same workload (Joseph fwd+adj, {2 * S} samples, "
+                      f"{dt:.1f} s), oracle/elsa_oracle.cpp with OpenMP over rays"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on exactly one GPU (got --gpus {a.gpus}); the projectors are single-GPU here",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    import elsa_amd as E
+    from elsa_amd.phantoms import modified_shepp_logan
+
+    wl = WORKLOADS[a.workload]
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    vd = E.VolumeDescriptor(list(wl["vol"]))
+    dd = E.CircleTrajectoryGenerator.createTrajectory(wl["views"], vd, 360, wl["s2c"], wl["c2d"],
+                                                      detectorSize=list(wl["det"]))
+    op = E.JosephsMethodCUDA(vd, dd)
+    S = op.sample_count()
+    R = int(np.prod(wl["det"])) * wl["views"]
+    x_np = modified_shepp_logan(wl["vol"][::-1])
+    x = torch.from_numpy(x_np).to(dev)
+    sino = torch.empty(dd.shape, dtype=torch.float32, device=dev)
+    back = torch.empty(vd.shape, dtype=torch.float32, device=dev)
+
+    for _ in range(a.warmup):
+        op.apply(x, sino)
+        op.applyAdjoint(sino, back)
+    torch.cuda.synchronize()
+
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True),
+            torch.cuda.Event(enable_timing=True)) for _ in range(a.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for e0, e1, e2 in evs:
+        e0.record()
+        op.apply(x, sino)
+        e1.record()
+        op.applyAdjoint(sino, back)
+        e2.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+
+    t_fwd = float(np.mean([e0.elapsed_time(e1) for e0, e1, _ in evs])) * 1e-3
+    t_adj = float(np.mean([e1.elapsed_time(e2) for _, e1, e2 in evs])) * 1e-3
+    tap_bytes = 16 if wl["dim"] == 3 else 8
+    alg_bytes = tap_bytes * S + 4 * R
+    name, t_dom = ("joseph_adj_kernel", t_adj) if t_adj >= t_fwd else ("joseph_fwd_kernel", t_fwd)
+    achieved = alg_bytes / t_dom / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"{name}_{a.workload}_traffic.json")
+    if os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
+
+    out = {
+        "metric": METRIC,
+        "value": round(2 * S * a.steps / wall / 1e9, 3),
+        "unit": "Gsamples/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(wall / a.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (modified Shepp-Logan phantom)",
+        "config": {"workload": wl["text"], "volume": list(wl["vol"]), "poses": wl["views"],
+                   "detector": list(wl["det"]), "samples_per_pass": S, "rays": R,
+                   "fwd_ms": round(t_fwd * 1e3, 3), "adj_ms": round(t_adj * 1e3, 3)},
+        "roofline": {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes_per_launch": alg_bytes},
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(wl, x_np, a.cpu_poses)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

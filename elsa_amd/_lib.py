@@ -1,0 +1,83 @@
+"""ctypes binding of libelsahip.so (include/elsa_hip.h).
+
+The library is loaded from this package directory only. If it is missing the import of the
+projector classes fails with an ImportError naming the build command: there is no fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libelsahip.so")
+
+EHP_OK, EHP_ERR_ARGUMENT, EHP_ERR_HIP, EHP_ERR_ALLOC, EHP_ERR_UNSUPPORTED = range(5)
+EHP_F32, EHP_F64 = 0, 1
+
+# every symbol include/elsa_hip.h declares
+EXPORTS = (
+    "ehp_last_error", "ehp_version", "ehp_geometry_create", "ehp_geometry_destroy",
+    "ehp_joseph_create", "ehp_siddon_create", "ehp_voxel_create", "ehp_destroy", "ehp_forward",
+    "ehp_adjoint", "ehp_op_sizes", "ehp_sample_count", "ehp_joseph_ray_info", "ehp_geometry_pose",
+    "ehp_circle_trajectory", "ehp_blob_lut", "ehp_bspline_lut",
+)
+
+_lib = None
+
+
+class EhpError(RuntimeError):
+    pass
+
+
+def load():
+    """Load libelsahip.so and declare the C signatures (no GPU call happens here)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m elsa_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i64, c_int, c_f, c_d = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_float, ctypes.c_double
+    fp = ctypes.POINTER(ctypes.c_float)
+    i64p = ctypes.POINTER(ctypes.c_int64)
+    L.ehp_last_error.restype = ctypes.c_char_p
+    L.ehp_last_error.argtypes = []
+    L.ehp_version.restype = ctypes.c_char_p
+    L.ehp_version.argtypes = []
+    L.ehp_geometry_create.argtypes = [c_int, i64, i64p, i64p, fp, fp, fp, fp, fp, ctypes.POINTER(vp)]
+    L.ehp_geometry_destroy.argtypes = [vp]
+    L.ehp_geometry_destroy.restype = None
+    L.ehp_joseph_create.argtypes = [vp, c_int, c_int, ctypes.POINTER(vp)]
+    L.ehp_siddon_create.argtypes = [vp, c_int, ctypes.POINTER(vp)]
+    L.ehp_voxel_create.argtypes = [vp, c_int, vp, c_int, c_d, ctypes.POINTER(vp)]
+    L.ehp_destroy.argtypes = [vp]
+    L.ehp_destroy.restype = None
+    L.ehp_forward.argtypes = [vp, vp, vp, vp]
+    L.ehp_adjoint.argtypes = [vp, vp, vp, vp]
+    L.ehp_op_sizes.argtypes = [vp, i64p, i64p]
+    L.ehp_sample_count.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
+    L.ehp_joseph_ray_info.argtypes = [vp, vp, vp]
+    L.ehp_geometry_pose.argtypes = [c_int, c_f, c_f, fp, fp, fp, fp, c_f, c_f, c_f, fp, fp, fp, fp, fp, fp, fp]
+    L.ehp_circle_trajectory.argtypes = [c_int, i64, i64p, c_f, c_f, c_f, i64p, i64p, fp, fp, fp, fp, fp]
+    L.ehp_blob_lut.argtypes = [c_int, c_d, c_d, i64, c_int, vp, ctypes.POINTER(c_d)]
+    L.ehp_bspline_lut.argtypes = [c_int, i64, i64, c_int, vp, ctypes.POINTER(c_d)]
+    for name in EXPORTS:
+        if name not in ("ehp_last_error", "ehp_version", "ehp_geometry_destroy", "ehp_destroy"):
+            getattr(L, name).restype = c_int
+    _lib = L
+    return L
+
+
+def check(rc: int, what: str = "") -> None:
+    """Raise the Python counterpart of elsa's exception for a failing status code."""
+    if rc == EHP_OK:
+        return
+    msg = load().ehp_last_error().decode(errors="replace")
+    text = f"{what}: {msg}" if what else msg
+    if rc == EHP_ERR_ARGUMENT:
+        raise ValueError(text)
+    if rc == EHP_ERR_ALLOC:
+        raise MemoryError(text)
+    if rc == EHP_ERR_UNSUPPORTED:
+        raise NotImplementedError(text)
+    raise EhpError(text)

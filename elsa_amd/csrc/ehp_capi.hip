@@ -1,0 +1,387 @@
+// ehp_capi.hip -- the extern "C" boundary of libelsahip.so (declared in include/elsa_hip.h).
+//
+// Owns the device state elsa's GPU projector classes build in their constructors (per-pose
+// geometry, the basis-function table) and dispatches apply/applyAdjoint to the kernels.
+// Errors come back as status codes with a per-thread message instead of elsa's exceptions
+// (elsa/projectors_cuda/JosephsMethodCUDA.cpp:17-69, SiddonsMethodCUDA.cpp:17-30).
+#include "elsa_hip.h"
+#include "ehp_kernels.hpp"
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+struct ehp_geometry {
+    int dim;
+    int64_t nposes;
+    int vol[3];
+    int det[2];
+    std::vector<ehp::PoseGeom> host;
+};
+
+enum op_kind { KIND_JOSEPH = 0, KIND_SIDDON = 1, KIND_VOXEL = 2 };
+
+struct ehp_op {
+    int kind;
+    int dtype;
+    int device;
+    ehp::Dims dims;
+    ehp::PoseGeom* d_geo = nullptr;
+    void* d_lut = nullptr;
+    int n_lut = 0;
+    float support = 0.0f;
+    uint64_t samples = 0;
+};
+
+namespace
+{
+    thread_local std::string g_last_error;
+
+    int fail(int code, const char* fmt, ...)
+    {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        std::vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        g_last_error = buf;
+        return code;
+    }
+
+    int hip_fail(hipError_t e, const char* what)
+    {
+        return fail(EHP_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+    }
+
+    bool valid_dtype(int dtype) { return dtype == EHP_F32 || dtype == EHP_F64; }
+
+    size_t dtype_size(int dtype) { return dtype == EHP_F64 ? sizeof(double) : sizeof(float); }
+
+    ehp::Dims make_dims(const ehp_geometry* g)
+    {
+        ehp::Dims d{};
+        d.dim = g->dim;
+        for (int i = 0; i < 3; ++i)
+            d.vol[i] = i < g->dim ? g->vol[i] : 1;
+        d.det[0] = g->det[0];
+        d.det[1] = g->dim == 3 ? g->det[1] : 1;
+        d.nposes = (int) g->nposes;
+        d.strides[0] = 1;
+        for (int i = 1; i < 3; ++i)
+            d.strides[i] = d.strides[i - 1] * d.vol[i - 1];
+        d.nvox = (int64_t) d.vol[0] * d.vol[1] * d.vol[2];
+        d.pix_per_pose = (int64_t) d.det[0] * d.det[1];
+        d.nrays = d.pix_per_pose * d.nposes;
+        return d;
+    }
+
+    int new_op(const ehp_geometry* g, int kind, int dtype, ehp_op** out)
+    {
+        if (!out)
+            return fail(EHP_ERR_ARGUMENT, "output handle is NULL");
+        *out = nullptr;
+        if (!g)
+            return fail(EHP_ERR_ARGUMENT, "geometry is NULL");
+        if (!valid_dtype(dtype))
+            return fail(EHP_ERR_ARGUMENT, "unsupported dtype %d", dtype);
+        ehp_op* op = new ehp_op();
+        op->kind = kind;
+        op->dtype = dtype;
+        op->dims = make_dims(g);
+        hipError_t e = hipGetDevice(&op->device);
+        if (e != hipSuccess) {
+            delete op;
+            return hip_fail(e, "hipGetDevice");
+        }
+        const size_t bytes = g->host.size() * sizeof(ehp::PoseGeom);
+        e = hipMalloc((void**) &op->d_geo, bytes);
+        if (e != hipSuccess) {
+            delete op;
+            return fail(EHP_ERR_ALLOC, "hipMalloc of %zu bytes of geometry failed: %s", bytes, hipGetErrorString(e));
+        }
+        e = hipMemcpy(op->d_geo, g->host.data(), bytes, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            hipFree(op->d_geo);
+            delete op;
+            return hip_fail(e, "geometry upload");
+        }
+        *out = op;
+        return EHP_OK;
+    }
+
+    // Sample count of one pass, computed once on the device.
+    int count_samples(ehp_op* op)
+    {
+        unsigned long long* d_total = nullptr;
+        hipError_t e = hipMalloc((void**) &d_total, sizeof(unsigned long long));
+        if (e != hipSuccess)
+            return fail(EHP_ERR_ALLOC, "hipMalloc of the sample counter failed: %s", hipGetErrorString(e));
+        e = hipMemset(d_total, 0, sizeof(unsigned long long));
+        if (e == hipSuccess) {
+            if (op->kind == KIND_JOSEPH)
+                e = ehp::joseph_info(op->d_geo, op->dims, nullptr, nullptr, d_total, nullptr);
+            else if (op->kind == KIND_SIDDON)
+                e = ehp::siddon_count(op->d_geo, op->dims, d_total, nullptr);
+            else
+                e = ehp::voxel_count(op->d_geo, op->dims, op->support, d_total, nullptr);
+        }
+        unsigned long long h = 0;
+        if (e == hipSuccess)
+            e = hipMemcpy(&h, d_total, sizeof(h), hipMemcpyDeviceToHost);
+        hipFree(d_total);
+        if (e != hipSuccess)
+            return hip_fail(e, "sample count");
+        op->samples = h;
+        return EHP_OK;
+    }
+} // namespace
+
+extern "C" {
+
+const char* ehp_last_error(void) { return g_last_error.c_str(); }
+
+const char* ehp_version(void) { return "elsa_amd 0.1 (gfx950)"; }
+
+int ehp_geometry_create(int dim, int64_t nposes, const int64_t* vol_dims, const int64_t* det_dims,
+                        const float* P, const float* Pinv, const float* C, const float* ext,
+                        const float* sdd, ehp_geometry** out)
+{
+    if (!out)
+        return fail(EHP_ERR_ARGUMENT, "output handle is NULL");
+    *out = nullptr;
+    if (dim != 2 && dim != 3)
+        return fail(EHP_ERR_ARGUMENT, "only 2D and 3D geometries are supported (got %d)", dim);
+    if (nposes <= 0)
+        return fail(EHP_ERR_ARGUMENT, "geometry list was empty");
+    if (nposes > 65535)
+        return fail(EHP_ERR_UNSUPPORTED, "at most 65535 poses per operator (got %lld)", (long long) nposes);
+    if (!vol_dims || !det_dims || !P || !Pinv || !C || !ext || !sdd)
+        return fail(EHP_ERR_ARGUMENT, "NULL geometry array");
+    ehp_geometry* g = new ehp_geometry();
+    g->dim = dim;
+    g->nposes = nposes;
+    int64_t nvox = 1, npix = 1;
+    for (int i = 0; i < 3; ++i)
+        g->vol[i] = 1;
+    g->det[0] = g->det[1] = 1;
+    for (int i = 0; i < dim; ++i) {
+        if (vol_dims[i] <= 0 || vol_dims[i] > (1 << 20)) {
+            delete g;
+            return fail(EHP_ERR_ARGUMENT, "invalid volume size %lld along axis %d", (long long) vol_dims[i], i);
+        }
+        g->vol[i] = (int) vol_dims[i];
+        nvox *= vol_dims[i];
+    }
+    for (int i = 0; i < dim - 1; ++i) {
+        if (det_dims[i] <= 0 || det_dims[i] > (1 << 20)) {
+            delete g;
+            return fail(EHP_ERR_ARGUMENT, "invalid detector size %lld along axis %d", (long long) det_dims[i], i);
+        }
+        g->det[i] = (int) det_dims[i];
+        npix *= det_dims[i];
+    }
+    if (nvox >= (int64_t(1) << 40) || npix * nposes >= (int64_t(1) << 40)) {
+        delete g;
+        return fail(EHP_ERR_UNSUPPORTED, "problem too large");
+    }
+    g->host.resize((size_t) nposes);
+    const int nm = dim * (dim + 1);
+    for (int64_t p = 0; p < nposes; ++p) {
+        ehp::PoseGeom& pg = g->host[(size_t) p];
+        std::memset(&pg, 0, sizeof(pg));
+        for (int i = 0; i < nm; ++i) {
+            pg.P[i] = P[p * nm + i];
+            pg.Pinv[i] = Pinv[p * nm + i];
+            pg.ext[i] = ext[p * nm + i];
+        }
+        for (int i = 0; i < dim; ++i)
+            pg.C[i] = C[p * dim + i];
+        pg.sdd = sdd[p];
+    }
+    *out = g;
+    return EHP_OK;
+}
+
+void ehp_geometry_destroy(ehp_geometry* g) { delete g; }
+
+int ehp_joseph_create(const ehp_geometry* g, int dtype, int flags, ehp_op** out)
+{
+    if (flags != 0)
+        return fail(EHP_ERR_ARGUMENT, "Joseph flags must be 0 (got %d)", flags);
+    int rc = new_op(g, KIND_JOSEPH, dtype, out);
+    if (rc != EHP_OK)
+        return rc;
+    rc = count_samples(*out);
+    if (rc != EHP_OK) {
+        ehp_destroy(*out);
+        *out = nullptr;
+    }
+    return rc;
+}
+
+int ehp_siddon_create(const ehp_geometry* g, int dtype, ehp_op** out)
+{
+    int rc = new_op(g, KIND_SIDDON, dtype, out);
+    if (rc != EHP_OK)
+        return rc;
+    rc = count_samples(*out);
+    if (rc != EHP_OK) {
+        ehp_destroy(*out);
+        *out = nullptr;
+    }
+    return rc;
+}
+
+int ehp_voxel_create(const ehp_geometry* g, int dtype, const void* lut, int n_lut, double support,
+                     ehp_op** out)
+{
+    if (!lut || n_lut < 2)
+        return fail(EHP_ERR_ARGUMENT, "the basis-function table needs at least 2 entries");
+    if (!(support > 0.0))
+        return fail(EHP_ERR_ARGUMENT, "the basis-function support must be positive");
+    int rc = new_op(g, KIND_VOXEL, dtype, out);
+    if (rc != EHP_OK)
+        return rc;
+    ehp_op* op = *out;
+    op->n_lut = n_lut;
+    op->support = (float) support;
+    const size_t bytes = (size_t) n_lut * dtype_size(dtype);
+    hipError_t e = hipMalloc(&op->d_lut, bytes);
+    if (e != hipSuccess) {
+        ehp_destroy(op);
+        *out = nullptr;
+        return fail(EHP_ERR_ALLOC, "hipMalloc of the basis-function table failed: %s", hipGetErrorString(e));
+    }
+    e = hipMemcpy(op->d_lut, lut, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        ehp_destroy(op);
+        *out = nullptr;
+        return hip_fail(e, "basis-function table upload");
+    }
+    rc = count_samples(op);
+    if (rc != EHP_OK) {
+        ehp_destroy(op);
+        *out = nullptr;
+    }
+    return rc;
+}
+
+void ehp_destroy(ehp_op* op)
+{
+    if (!op)
+        return;
+    int cur = -1;
+    hipGetDevice(&cur);
+    if (cur != op->device)
+        hipSetDevice(op->device);
+    if (op->d_geo)
+        hipFree(op->d_geo);
+    if (op->d_lut)
+        hipFree(op->d_lut);
+    if (cur >= 0 && cur != op->device)
+        hipSetDevice(cur);
+    delete op;
+}
+
+int ehp_op_sizes(const ehp_op* op, int64_t* n_vol, int64_t* n_sino)
+{
+    if (!op)
+        return fail(EHP_ERR_ARGUMENT, "operator is NULL");
+    if (n_vol)
+        *n_vol = op->dims.nvox;
+    if (n_sino)
+        *n_sino = op->dims.nrays;
+    return EHP_OK;
+}
+
+int ehp_sample_count(const ehp_op* op, uint64_t* samples)
+{
+    if (!op || !samples)
+        return fail(EHP_ERR_ARGUMENT, "NULL argument");
+    *samples = op->samples;
+    return EHP_OK;
+}
+
+static int apply(ehp_op* op, const void* in, void* out, void* stream, bool adjoint)
+{
+    if (!op)
+        return fail(EHP_ERR_ARGUMENT, "operator is NULL");
+    if (!in || !out)
+        return fail(EHP_ERR_ARGUMENT, "NULL data pointer");
+    if (in == out)
+        return fail(EHP_ERR_ARGUMENT, "input and output must not alias");
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess)
+        return hip_fail(e, "hipGetDevice");
+    if (cur != op->device)
+        return fail(EHP_ERR_ARGUMENT, "operator lives on device %d, current device is %d", op->device, cur);
+    hipStream_t s = (hipStream_t) stream;
+    const ehp::Dims& dm = op->dims;
+    if (op->dtype == EHP_F32) {
+        const float* x = (const float*) in;
+        float* y = (float*) out;
+        if (op->kind == KIND_JOSEPH)
+            e = adjoint ? ehp::joseph_adjoint<float>(op->d_geo, dm, x, y, s) : ehp::joseph_forward<float>(op->d_geo, dm, x, y, s);
+        else if (op->kind == KIND_SIDDON)
+            e = adjoint ? ehp::siddon_adjoint<float>(op->d_geo, dm, x, y, s) : ehp::siddon_forward<float>(op->d_geo, dm, x, y, s);
+        else
+            e = adjoint ? ehp::voxel_adjoint<float>(op->d_geo, dm, (const float*) op->d_lut, op->n_lut, op->support, x, y, s)
+                        : ehp::voxel_forward<float>(op->d_geo, dm, (const float*) op->d_lut, op->n_lut, op->support, x, y, s);
+    } else {
+        const double* x = (const double*) in;
+        double* y = (double*) out;
+        if (op->kind == KIND_JOSEPH)
+            e = adjoint ? ehp::joseph_adjoint<double>(op->d_geo, dm, x, y, s) : ehp::joseph_forward<double>(op->d_geo, dm, x, y, s);
+        else if (op->kind == KIND_SIDDON)
+            e = adjoint ? ehp::siddon_adjoint<double>(op->d_geo, dm, x, y, s) : ehp::siddon_forward<double>(op->d_geo, dm, x, y, s);
+        else
+            e = adjoint ? ehp::voxel_adjoint<double>(op->d_geo, dm, (const double*) op->d_lut, op->n_lut, op->support, x, y, s)
+                        : ehp::voxel_forward<double>(op->d_geo, dm, (const double*) op->d_lut, op->n_lut, op->support, x, y, s);
+    }
+    if (e != hipSuccess)
+        return hip_fail(e, adjoint ? "adjoint launch" : "forward launch");
+    return EHP_OK;
+}
+
+int ehp_forward(ehp_op* op, const void* d_vol, void* d_sino, void* stream)
+{
+    return apply(op, d_vol, d_sino, stream, false);
+}
+
+int ehp_adjoint(ehp_op* op, const void* d_sino, void* d_vol, void* stream)
+{
+    return apply(op, d_sino, d_vol, stream, true);
+}
+
+int ehp_joseph_ray_info(const ehp_op* op, int32_t* axis, int64_t* nsteps)
+{
+    if (!op || !axis || !nsteps)
+        return fail(EHP_ERR_ARGUMENT, "NULL argument");
+    if (op->kind != KIND_JOSEPH)
+        return fail(EHP_ERR_ARGUMENT, "not a Joseph operator");
+    const int64_t n = op->dims.nrays;
+    int32_t* d_axis = nullptr;
+    int64_t* d_steps = nullptr;
+    hipError_t e = hipMalloc((void**) &d_axis, n * sizeof(int32_t));
+    if (e == hipSuccess)
+        e = hipMalloc((void**) &d_steps, n * sizeof(int64_t));
+    if (e != hipSuccess) {
+        hipFree(d_axis);
+        return fail(EHP_ERR_ALLOC, "hipMalloc for ray info failed: %s", hipGetErrorString(e));
+    }
+    e = ehp::joseph_info(op->d_geo, op->dims, d_axis, d_steps, nullptr, nullptr);
+    if (e == hipSuccess)
+        e = hipMemcpy(axis, d_axis, n * sizeof(int32_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess)
+        e = hipMemcpy(nsteps, d_steps, n * sizeof(int64_t), hipMemcpyDeviceToHost);
+    hipFree(d_axis);
+    hipFree(d_steps);
+    if (e != hipSuccess)
+        return hip_fail(e, "ray info");
+    return EHP_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,246 @@
+// ehp_device.hpp -- device-side geometry shared by the Joseph, Siddon and voxel kernels.
+//
+// Everything here is evaluated in FP32 (elsa's real_t) with the operation order of elsa's
+// CPU code and without contraction (the library is built with -ffp-contract=off), so the
+// per-ray traversal state -- driving axis, first sample and sample count -- is the same on
+// the GPU as in the CPU restatement. Where a fused multiply-add is meant it is written.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ehp
+{
+    // Per-pose geometry as kernels read it. One 64-float record per pose so a wavefront can
+    // read it with scalar loads. Column-major blocks as in elsa (Eigen).
+    struct PoseGeom {
+        float Pinv[12]; // (dim+1) x dim
+        float C[4];     // camera centre
+        float P[12];    // dim x (dim+1)
+        float ext[12];  // dim x (dim+1)
+        float sdd;
+        float pad[23];
+    };
+    static_assert(sizeof(PoseGeom) == 64 * sizeof(float), "PoseGeom must be 256 bytes");
+
+    struct Dims {
+        int dim;
+        int vol[3];    // voxels per axis (1 for unused)
+        int det[2];    // detector pixels (1 for unused)
+        int nposes;
+        int64_t strides[3];
+        int64_t nvox;
+        int64_t pix_per_pose;
+        int64_t nrays;
+    };
+
+    __host__ __device__ inline float minNum(float x, float y)
+    {
+        return (y != y) ? x : (x < y ? x : y);
+    }
+    __host__ __device__ inline float maxNum(float x, float y)
+    {
+        return (y != y) ? x : (x > y ? x : y);
+    }
+    __host__ __device__ inline float sgnf(float v)
+    {
+        return v > 0.0f ? 1.0f : (v < 0.0f ? -1.0f : 0.0f);
+    }
+
+    // Ray through the centre of pixel (u, v): origin C, direction normalize(Pinv h).
+    // elsa/core/Descriptors/DetectorDescriptor.cpp:90-128.
+    template <int D>
+    __device__ inline void make_ray(const PoseGeom& g, int u, int v, float* o, float* d)
+    {
+        float h[3];
+        h[0] = (float) u + 0.5f;
+        if (D == 3) {
+            h[1] = (float) v + 0.5f;
+            h[2] = 1.0f;
+        } else {
+            h[1] = 1.0f;
+        }
+        float y[3];
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            float acc = g.Pinv[i] * h[0];
+#pragma unroll
+            for (int j = 1; j < D; ++j)
+                acc = acc + g.Pinv[i + j * (D + 1)] * h[j];
+            y[i] = acc;
+        }
+        float n2 = y[0] * y[0];
+#pragma unroll
+        for (int i = 1; i < D; ++i)
+            n2 = n2 + y[i] * y[i];
+        const float n = sqrtf(n2);
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            d[i] = n2 > 0.0f ? y[i] / n : y[i];
+            o[i] = g.C[i];
+        }
+    }
+
+    // Slab test against [0, vol) with elsa's NaN handling and hit rule.
+    // elsa/projectors/Intersection.cpp:7-46.
+    template <int D>
+    __device__ inline bool intersect(const float* o, const float* d, const int* vol, float& tmin_out,
+                                     float& tmax_out)
+    {
+        float invDir = 1.0f / d[0];
+        float t1 = (0.0f - o[0]) * invDir;
+        float t2 = ((float) vol[0] - o[0]) * invDir;
+        float tmin = invDir >= 0.0f ? t1 : t2;
+        float tmax = invDir >= 0.0f ? t2 : t1;
+#pragma unroll
+        for (int i = 1; i < D; ++i) {
+            invDir = 1.0f / d[i];
+            t1 = (0.0f - o[i]) * invDir;
+            t2 = ((float) vol[i] - o[i]) * invDir;
+            tmin = maxNum(tmin, invDir >= 0.0f ? t1 : t2);
+            tmax = minNum(tmax, invDir >= 0.0f ? t2 : t1);
+        }
+        if (tmax == 0.0f && tmin == 0.0f)
+            return false;
+        if (tmax >= maxNum(tmin, 0.0f)) {
+            tmin_out = tmin;
+            tmax_out = tmax;
+            return true;
+        }
+        return false;
+    }
+
+    // Traversal state of elsa's DrivingDirectionTraversal after construction
+    // (elsa/projectors/DrivingDirectionTraversal.cpp:6-57): driving axis, first sample
+    // position (snapped to a voxel-centre plane), step (unit along the driving axis),
+    // number of samples and the length of one step.
+    struct JRay {
+        int axis;
+        int nsteps;
+        float pos0[3];
+        float step[3];
+        float len;
+    };
+
+    template <int D>
+    __device__ inline void joseph_setup(const float* o, const float* d, const int* vol, JRay& jr)
+    {
+        int a = 0;
+        float m = fabsf(d[0]);
+#pragma unroll
+        for (int i = 1; i < D; ++i)
+            if (fabsf(d[i]) > m) {
+                m = fabsf(d[i]);
+                a = i;
+            }
+        jr.axis = a;
+        jr.nsteps = 0;
+        jr.len = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            jr.pos0[i] = 0.0f;
+            jr.step[i] = 0.0f;
+        }
+        float tmin, tmax;
+        if (!intersect<D>(o, d, vol, tmin, tmax))
+            return;
+        float cur[3], ex[3];
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            const float hi = (float) vol[i];
+            cur[i] = o[i] + tmin * d[i];
+            cur[i] = cur[i] < 0.0f ? 0.0f : cur[i];
+            cur[i] = cur[i] > hi ? hi : cur[i];
+            ex[i] = o[i] + tmax * d[i];
+            ex[i] = ex[i] < 0.0f ? 0.0f : ex[i];
+            ex[i] = ex[i] > hi ? hi : ex[i];
+        }
+        const float ad = fabsf(d[a]);
+        float step[3];
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+            step[i] = d[i] / ad;
+        const float dist = cur[a] - floorf(cur[a]);
+        const float s = sgnf(step[a]);
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+            cur[i] = cur[i] + step[i] * (0.5f - dist) * s;
+        cur[a] = floorf(cur[a]) + 0.5f;
+        jr.nsteps = (int) ceilf(fabsf(ex[a] - cur[a]));
+        const float n2 = (D == 3) ? step[0] * step[0] + (step[1] * step[1] + step[2] * step[2])
+                                  : step[0] * step[0] + step[1] * step[1];
+        jr.len = sqrtf(n2);
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            jr.pos0[i] = cur[i];
+            jr.step[i] = step[i];
+        }
+    }
+
+    // Bilinear weights in the plane normal to the driving axis for sample s (position
+    // pos0 + s*step, one FMA per transverse axis), as getLinearInterpolationWeights
+    // (JosephsMethod.cpp:37-52) evaluates them.
+    struct JSample {
+        int fl[3];
+        int cl[3];
+        float w[3];
+        float cw[3];
+    };
+
+    template <int D>
+    __device__ inline void joseph_sample(const JRay& jr, int s, JSample& js)
+    {
+        const float fs = (float) s;
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            const float p = (i == jr.axis) ? jr.pos0[i] + fs * jr.step[i]
+                                           : __builtin_fmaf(fs, jr.step[i], jr.pos0[i]);
+            const float f = floorf(p - 0.5f);
+            js.fl[i] = (int) f;
+            js.cl[i] = (int) ceilf(p - 0.5f);
+            js.cw[i] = (p - f) - 0.5f;
+            js.w[i] = 1.0f - js.cw[i];
+        }
+        js.w[jr.axis] = 1.0f;
+        js.cw[jr.axis] = 1.0f;
+    }
+
+    // The taps of one sample in elsa's order (doInterpolation, JosephsMethod.cpp:60-97):
+    // 2D (F0,C1)(C0,F1); 3D (F0,F1,F2)(F0,C1,C2)(C0,F1,C2)(C0,C1,F2). A tap outside the
+    // volume gets weight 0 and a clamped index.
+    template <int D, class Fn>
+    __device__ inline void joseph_taps(const JSample& js, const int* vol, const int64_t* strides, Fn fn)
+    {
+        auto tap = [&](int c0, int c1, int c2, float wt) {
+            const int c[3] = {c0, c1, c2};
+            bool in = true;
+            int64_t lin = 0;
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+                in = in && (c[i] >= 0) && (c[i] < vol[i]);
+                const int cc = c[i] < 0 ? 0 : (c[i] > vol[i] - 1 ? vol[i] - 1 : c[i]);
+                lin += (int64_t) cc * strides[i];
+            }
+            fn(lin, in ? wt : 0.0f);
+        };
+        if (D == 2) {
+            tap(js.fl[0], js.cl[1], 0, js.w[0] * js.cw[1]);
+            tap(js.cl[0], js.fl[1], 0, js.cw[0] * js.w[1]);
+        } else {
+            tap(js.fl[0], js.fl[1], js.fl[2], js.w[0] * js.w[1] * js.w[2]);
+            tap(js.fl[0], js.cl[1], js.cl[2], js.w[0] * js.cw[1] * js.cw[2]);
+            tap(js.cl[0], js.fl[1], js.cl[2], js.cw[0] * js.w[1] * js.cw[2]);
+            tap(js.cl[0], js.cl[1], js.fl[2], js.cw[0] * js.cw[1] * js.w[2]);
+        }
+    }
+
+    // Hardware float/double atomic add (global_atomic_add_f32 / _f64 on gfx950).
+    __device__ inline void atomic_add(float* p, float v)
+    {
+        unsafeAtomicAdd(p, v);
+    }
+    __device__ inline void atomic_add(double* p, double v)
+    {
+        unsafeAtomicAdd(p, v);
+    }
+} // namespace ehp

@@ -1,0 +1,530 @@
+// ehp_kernels.hip -- HIP kernels of the X-ray projectors for gfx950.
+//
+// Joseph (forward / matched adjoint), Siddon (forward / adjoint) and the voxel-driven
+// blob / B-spline projector. The discretizations are those of elsa's CPU projectors:
+//   Joseph  elsa/projectors/JosephsMethod.cpp:140-198 + DrivingDirectionTraversal.cpp:6-106
+//   Siddon  elsa/projectors/SiddonsMethod.cpp:76-124 + TraverseAABB.cpp:6-184
+//   voxel   elsa/projectors/VoxelComputation.h:40-185 + Luts.hpp:60-78
+// The Joseph kernels place sample s of a ray at pos0 + s*step (one FMA per transverse axis)
+// instead of accumulating the step, so the forward and the adjoint see identical weights.
+#include "ehp_kernels.hpp"
+
+namespace ehp
+{
+    // ---------------------------------------------------------------------------------
+    // fill
+
+    template <class T>
+    __global__ void __launch_bounds__(256) fill_kernel(T* __restrict__ p, int64_t n, T value)
+    {
+        const int64_t stride = (int64_t) gridDim.x * blockDim.x;
+        for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+            p[i] = value;
+    }
+
+    template <class T>
+    hipError_t launch_fill(T* p, int64_t n, T value, hipStream_t s)
+    {
+        if (n <= 0)
+            return hipSuccess;
+        int64_t blocks = (n + 255) / 256;
+        if (blocks > 8192)
+            blocks = 8192;
+        hipLaunchKernelGGL(fill_kernel<T>, dim3((unsigned) blocks), dim3(256), 0, s, p, n, value);
+        return hipGetLastError();
+    }
+
+    // ---------------------------------------------------------------------------------
+    // Joseph: one thread per ray (pixel of one pose). Grid: x = pixel block, y = pose.
+
+    template <int D, class T>
+    __global__ void __launch_bounds__(256)
+        joseph_fwd_kernel(const PoseGeom* __restrict__ geo, const Dims dims, const T* __restrict__ vol,
+                          T* __restrict__ sino)
+    {
+        const int pose = blockIdx.y;
+        const int64_t pix = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+        if (pix >= dims.pix_per_pose)
+            return;
+        const int u = (int) (pix % dims.det[0]);
+        const int v = (int) (pix / dims.det[0]);
+        float o[3], d[3];
+        make_ray<D>(geo[pose], u, v, o, d);
+        JRay jr;
+        joseph_setup<D>(o, d, dims.vol, jr);
+        T acc = T(0);
+        for (int s = 0; s < jr.nsteps; ++s) {
+            JSample js;
+            joseph_sample<D>(jr, s, js);
+            joseph_taps<D>(js, dims.vol, dims.strides,
+                           [&](int64_t lin, float wt) { acc += vol[lin] * (T) wt; });
+        }
+        sino[(int64_t) pose * dims.pix_per_pose + pix] = acc * (T) jr.len;
+    }
+
+    template <int D, class T>
+    __global__ void __launch_bounds__(256)
+        joseph_adj_kernel(const PoseGeom* __restrict__ geo, const Dims dims, const T* __restrict__ sino,
+                          T* __restrict__ vol)
+    {
+        const int pose = blockIdx.y;
+        const int64_t pix = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+        if (pix >= dims.pix_per_pose)
+            return;
+        const T yv = sino[(int64_t) pose * dims.pix_per_pose + pix];
+        if (yv == T(0))
+            return;
+        const int u = (int) (pix % dims.det[0]);
+        const int v = (int) (pix / dims.det[0]);
+        float o[3], d[3];
+        make_ray<D>(geo[pose], u, v, o, d);
+        JRay jr;
+        joseph_setup<D>(o, d, dims.vol, jr);
+        const T yl = yv * (T) jr.len;
+        for (int s = 0; s < jr.nsteps; ++s) {
+            JSample js;
+            joseph_sample<D>(jr, s, js);
+            joseph_taps<D>(js, dims.vol, dims.strides, [&](int64_t lin, float wt) {
+                if (wt != 0.0f)
+                    atomic_add(vol + lin, yl * (T) wt);
+            });
+        }
+    }
+
+    // Per-ray traversal state (axis, sample count) and the total sample count.
+    template <int D>
+    __global__ void __launch_bounds__(256)
+        joseph_info_kernel(const PoseGeom* __restrict__ geo, const Dims dims, int32_t* __restrict__ axis,
+                           int64_t* __restrict__ nsteps, unsigned long long* __restrict__ total)
+    {
+        const int pose = blockIdx.y;
+        const int64_t pix = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+        int n = 0;
+        if (pix < dims.pix_per_pose) {
+            const int u = (int) (pix % dims.det[0]);
+            const int v = (int) (pix / dims.det[0]);
+            float o[3], d[3];
+            make_ray<D>(geo[pose], u, v, o, d);
+            JRay jr;
+            joseph_setup<D>(o, d, dims.vol, jr);
+            n = jr.nsteps;
+            const int64_t r = (int64_t) pose * dims.pix_per_pose + pix;
+            if (axis)
+                axis[r] = jr.axis;
+            if (nsteps)
+                nsteps[r] = jr.nsteps;
+        }
+        if (total) {
+            unsigned long long c = (unsigned long long) n;
+            for (int off = 32; off > 0; off >>= 1)
+                c += __shfl_down(c, off, 64);
+            if ((threadIdx.x & 63) == 0 && c)
+                atomicAdd(total, c);
+        }
+    }
+
+    // ---------------------------------------------------------------------------------
+    // Siddon: one thread per ray; the traversal of TraverseAABB (ties step several axes).
+
+    template <int D, bool ADJ, bool COUNT, class T>
+    __device__ inline unsigned long long siddon_ray(const float* o, const float* d, const Dims& dims,
+                                                    const T* in, T* out, int64_t ray)
+    {
+        T acc = T(0);
+        unsigned long long crossings = 0;
+        float tmin, tmax;
+        bool inbox = intersect<D>(o, d, dims.vol, tmin, tmax);
+        const T yv = (ADJ && !COUNT) ? in[ray] : T(0);
+        if (inbox) {
+            float entry[3], cur[3], tdelta[3], Tn[3];
+            int stepdir[3];
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+                const float hi = (float) dims.vol[i];
+                entry[i] = o[i] + tmin * d[i];
+                entry[i] = entry[i] < 0.0f ? 0.0f : entry[i];
+                entry[i] = entry[i] > hi ? hi : entry[i];
+                stepdir[i] = (int) sgnf(d[i]);
+            }
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+                float lower = floorf(entry[i]);
+                if (lower == entry[i] && stepdir[i] < 0)
+                    lower = lower - 1.0f;
+                cur[i] = lower;
+                if (cur[i] >= (float) dims.vol[i] || cur[i] < 0.0f)
+                    inbox = false;
+            }
+            const float EPS = 1.1920928955078125e-07f;
+            const float MAXF = 3.4028234663852886e+38f;
+            int64_t idx = 0;
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+                tdelta[i] = fabsf(d[i]) > EPS ? (float) stepdir[i] / d[i] : MAXF;
+                Tn[i] = fabsf(d[i]) > EPS ? ((d[i] > 0.0f ? cur[i] + 1.0f : cur[i]) - entry[i]) / d[i] : MAXF;
+                idx += dims.strides[i] * (int64_t) cur[i];
+            }
+            float tExit = 0.0f;
+            while (inbox) {
+                const float tEntry = tExit;
+                float tm = Tn[0];
+#pragma unroll
+                for (int i = 1; i < D; ++i)
+                    tm = fminf(tm, Tn[i]);
+                tExit = tm;
+                bool mask[3];
+#pragma unroll
+                for (int i = 0; i < D; ++i) {
+                    bool mi = true;
+#pragma unroll
+                    for (int j = 0; j < D; ++j)
+                        if (j != i)
+                            mi = mi && (Tn[i] <= Tn[j]);
+                    mask[i] = mi;
+                }
+                int64_t nidx = idx;
+                bool nin = true;
+#pragma unroll
+                for (int i = 0; i < D; ++i) {
+                    if (mask[i]) {
+                        cur[i] += (float) stepdir[i];
+                        Tn[i] += tdelta[i];
+                        nidx += (int64_t) stepdir[i] * dims.strides[i];
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < D; ++i)
+                    nin = nin && (cur[i] < (float) dims.vol[i]) && (cur[i] >= 0.0f);
+                const float w = tExit - tEntry;
+                if (COUNT) {
+                    ++crossings;
+                } else if (ADJ) {
+                    if (w != 0.0f)
+                        atomic_add(out + idx, yv * (T) w);
+                } else {
+                    acc += in[idx] * (T) w;
+                }
+                idx = nidx;
+                inbox = nin;
+            }
+        }
+        if (!ADJ && !COUNT)
+            out[ray] = acc;
+        return crossings;
+    }
+
+    template <int D, bool ADJ, bool COUNT, class T>
+    __global__ void __launch_bounds__(256)
+        siddon_kernel(const PoseGeom* __restrict__ geo, const Dims dims, const T* __restrict__ in,
+                      T* __restrict__ out, unsigned long long* count)
+    {
+        const int pose = blockIdx.y;
+        const int64_t pix = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+        const bool valid = pix < dims.pix_per_pose;
+        if (!COUNT && !valid)
+            return;
+        unsigned long long c = 0;
+        if (valid) {
+            const int u = (int) (pix % dims.det[0]);
+            const int v = (int) (pix / dims.det[0]);
+            float o[3], d[3];
+            make_ray<D>(geo[pose], u, v, o, d);
+            c = siddon_ray<D, ADJ, COUNT, T>(o, d, dims, in, out, (int64_t) pose * dims.pix_per_pose + pix);
+        }
+        if (COUNT) {
+            for (int off = 32; off > 0; off >>= 1)
+                c += __shfl_down(c, off, 64);
+            if ((threadIdx.x & 63) == 0 && c)
+                atomicAdd(count, c);
+        }
+    }
+
+    // ---------------------------------------------------------------------------------
+    // Voxel-driven blob / B-spline projector.
+
+    template <class T>
+    __device__ inline T lut_eval(const T* __restrict__ lut, int n, float support, float distance)
+    {
+        const float index = (distance / support) * (float) (n - 1);
+        if (index < 0.0f || index > (float) (n - 1))
+            return T(0);
+        const int a = (int) floorf(index);
+        const int b = (int) ceilf(index);
+        const T fa = lut[a];
+        const T fb = lut[b];
+        const T t = (T) (index - (float) a);
+        return t * fb + (T(1) - t) * fa;
+    }
+
+    // Footprint of voxel (x, y, z) in one pose (VoxelComputation.h:40-107, full rectangle).
+    struct Footprint {
+        float center[2];
+        float scaling;
+        int lo[2];
+        int hi[2];
+    };
+
+    template <int D>
+    __device__ inline void footprint(const PoseGeom& g, const Dims& dims, int x, int y, int z, float radius,
+                                     Footprint& fp)
+    {
+        float hv[4];
+        hv[0] = (float) x + 0.5f;
+        hv[1] = (float) y + 0.5f;
+        hv[2] = (float) z + 0.5f;
+        hv[D] = 1.0f;
+        float pc[3], ce[3];
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            float acc = g.P[i] * hv[0];
+            float acc2 = g.ext[i] * hv[0];
+#pragma unroll
+            for (int j = 1; j <= D; ++j) {
+                acc = acc + g.P[i + j * D] * hv[j];
+                acc2 = acc2 + g.ext[i + j * D] * hv[j];
+            }
+            pc[i] = acc;
+            ce[i] = acc2;
+        }
+#pragma unroll
+        for (int i = 0; i < D - 1; ++i)
+            fp.center[i] = pc[i] / pc[D - 1] - 0.5f;
+        float n2 = ce[0] * ce[0];
+#pragma unroll
+        for (int i = 1; i < D; ++i)
+            n2 = n2 + ce[i] * ce[i];
+        fp.scaling = g.sdd / sqrtf(n2);
+        const float rod = fp.scaling * radius;
+#pragma unroll
+        for (int i = 0; i < D - 1; ++i) {
+            const int l = (int) ceilf(fp.center[i] - rod);
+            const int h = (int) floorf(fp.center[i] + rod);
+            fp.lo[i] = l < 0 ? 0 : l;
+            fp.hi[i] = h > dims.det[i] - 1 ? dims.det[i] - 1 : h;
+        }
+        if (D == 2) {
+            fp.lo[1] = 0;
+            fp.hi[1] = 0;
+        }
+    }
+
+    template <int D, class T, class Fn>
+    __device__ inline void visit(const Footprint& fp, const Dims& dims, Fn fn)
+    {
+        for (int j = fp.lo[1]; j <= fp.hi[1]; ++j)
+            for (int i = fp.lo[0]; i <= fp.hi[0]; ++i) {
+                const float d0 = (fp.center[0] - (float) i) / fp.scaling;
+                float dist;
+                if (D == 2) {
+                    dist = sqrtf(d0 * d0);
+                } else {
+                    const float d1 = (fp.center[1] - (float) j) / fp.scaling;
+                    dist = sqrtf(d0 * d0 + d1 * d1);
+                }
+                fn((int64_t) j * dims.det[0] + i, dist);
+            }
+    }
+
+    // Adjoint: one thread per voxel gathers over all poses (no atomics).
+    template <int D, class T>
+    __global__ void __launch_bounds__(256)
+        voxel_adj_kernel(const PoseGeom* __restrict__ geo, const Dims dims, const T* __restrict__ lut, int n_lut,
+                         float support, const T* __restrict__ sino, T* __restrict__ vol)
+    {
+        const int64_t vox = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+        if (vox >= dims.nvox)
+            return;
+        const int x = (int) (vox % dims.vol[0]);
+        const int y = (int) ((vox / dims.vol[0]) % dims.vol[1]);
+        const int z = (int) (vox / ((int64_t) dims.vol[0] * dims.vol[1]));
+        T acc = T(0);
+        for (int p = 0; p < dims.nposes; ++p) {
+            Footprint fp;
+            footprint<D>(geo[p], dims, x, y, z, support, fp);
+            const T* sp = sino + (int64_t) p * dims.pix_per_pose;
+            visit<D, T>(fp, dims, [&](int64_t pix, float dist) { acc += lut_eval<T>(lut, n_lut, support, dist) * sp[pix]; });
+        }
+        vol[vox] = acc;
+    }
+
+    // Forward: one thread per (voxel, pose) scatters into the detector.
+    template <int D, class T>
+    __global__ void __launch_bounds__(256)
+        voxel_fwd_kernel(const PoseGeom* __restrict__ geo, const Dims dims, const T* __restrict__ lut, int n_lut,
+                         float support, const T* __restrict__ vol, T* __restrict__ sino,
+                         unsigned long long* count)
+    {
+        const int p = blockIdx.y;
+        const int64_t vox = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+        unsigned long long taps = 0;
+        if (vox < dims.nvox) {
+            const int x = (int) (vox % dims.vol[0]);
+            const int y = (int) ((vox / dims.vol[0]) % dims.vol[1]);
+            const int z = (int) (vox / ((int64_t) dims.vol[0] * dims.vol[1]));
+            Footprint fp;
+            footprint<D>(geo[p], dims, x, y, z, support, fp);
+            if (count) {
+                const int nu = fp.hi[0] - fp.lo[0] + 1;
+                const int nv = fp.hi[1] - fp.lo[1] + 1;
+                taps = (nu > 0 && nv > 0) ? (unsigned long long) nu * nv : 0;
+            } else {
+                const T xv = vol[vox];
+                if (xv != T(0)) {
+                    T* sp = sino + (int64_t) p * dims.pix_per_pose;
+                    visit<D, T>(fp, dims, [&](int64_t pix, float dist) {
+                        const T w = lut_eval<T>(lut, n_lut, support, dist);
+                        if (w != T(0))
+                            atomic_add(sp + pix, xv * w);
+                    });
+                }
+            }
+        }
+        if (count) {
+            for (int off = 32; off > 0; off >>= 1)
+                taps += __shfl_down(taps, off, 64);
+            if ((threadIdx.x & 63) == 0 && taps)
+                atomicAdd(count, taps);
+        }
+    }
+
+    // ---------------------------------------------------------------------------------
+    // launchers
+
+    static dim3 ray_grid(const Dims& dims)
+    {
+        return dim3((unsigned) ((dims.pix_per_pose + 255) / 256), (unsigned) dims.nposes);
+    }
+
+    template <class T>
+    hipError_t joseph_forward(const PoseGeom* geo, const Dims& dims, const T* vol, T* sino, hipStream_t s)
+    {
+        if (dims.dim == 2)
+            hipLaunchKernelGGL((joseph_fwd_kernel<2, T>), ray_grid(dims), dim3(256), 0, s, geo, dims, vol, sino);
+        else
+            hipLaunchKernelGGL((joseph_fwd_kernel<3, T>), ray_grid(dims), dim3(256), 0, s, geo, dims, vol, sino);
+        return hipGetLastError();
+    }
+
+    template <class T>
+    hipError_t joseph_adjoint(const PoseGeom* geo, const Dims& dims, const T* sino, T* vol, hipStream_t s)
+    {
+        hipError_t e = launch_fill<T>(vol, dims.nvox, T(0), s);
+        if (e != hipSuccess)
+            return e;
+        if (dims.dim == 2)
+            hipLaunchKernelGGL((joseph_adj_kernel<2, T>), ray_grid(dims), dim3(256), 0, s, geo, dims, sino, vol);
+        else
+            hipLaunchKernelGGL((joseph_adj_kernel<3, T>), ray_grid(dims), dim3(256), 0, s, geo, dims, sino, vol);
+        return hipGetLastError();
+    }
+
+    hipError_t joseph_info(const PoseGeom* geo, const Dims& dims, int32_t* axis, int64_t* nsteps,
+                           unsigned long long* total, hipStream_t s)
+    {
+        if (dims.dim == 2)
+            hipLaunchKernelGGL((joseph_info_kernel<2>), ray_grid(dims), dim3(256), 0, s, geo, dims, axis, nsteps, total);
+        else
+            hipLaunchKernelGGL((joseph_info_kernel<3>), ray_grid(dims), dim3(256), 0, s, geo, dims, axis, nsteps, total);
+        return hipGetLastError();
+    }
+
+    template <class T>
+    hipError_t siddon_forward(const PoseGeom* geo, const Dims& dims, const T* vol, T* sino, hipStream_t s)
+    {
+        if (dims.dim == 2)
+            hipLaunchKernelGGL((siddon_kernel<2, false, false, T>), ray_grid(dims), dim3(256), 0, s, geo, dims, vol, sino,
+                               nullptr);
+        else
+            hipLaunchKernelGGL((siddon_kernel<3, false, false, T>), ray_grid(dims), dim3(256), 0, s, geo, dims, vol, sino,
+                               nullptr);
+        return hipGetLastError();
+    }
+
+    template <class T>
+    hipError_t siddon_adjoint(const PoseGeom* geo, const Dims& dims, const T* sino, T* vol, hipStream_t s)
+    {
+        hipError_t e = launch_fill<T>(vol, dims.nvox, T(0), s);
+        if (e != hipSuccess)
+            return e;
+        if (dims.dim == 2)
+            hipLaunchKernelGGL((siddon_kernel<2, true, false, T>), ray_grid(dims), dim3(256), 0, s, geo, dims, sino, vol,
+                               nullptr);
+        else
+            hipLaunchKernelGGL((siddon_kernel<3, true, false, T>), ray_grid(dims), dim3(256), 0, s, geo, dims, sino, vol,
+                               nullptr);
+        return hipGetLastError();
+    }
+
+    hipError_t siddon_count(const PoseGeom* geo, const Dims& dims, unsigned long long* total, hipStream_t s)
+    {
+        if (dims.dim == 2)
+            hipLaunchKernelGGL((siddon_kernel<2, false, true, float>), ray_grid(dims), dim3(256), 0, s, geo, dims,
+                               (const float*) nullptr, (float*) nullptr, total);
+        else
+            hipLaunchKernelGGL((siddon_kernel<3, false, true, float>), ray_grid(dims), dim3(256), 0, s, geo, dims,
+                               (const float*) nullptr, (float*) nullptr, total);
+        return hipGetLastError();
+    }
+
+    static dim3 vox_pose_grid(const Dims& dims)
+    {
+        return dim3((unsigned) ((dims.nvox + 255) / 256), (unsigned) dims.nposes);
+    }
+
+    template <class T>
+    hipError_t voxel_forward(const PoseGeom* geo, const Dims& dims, const T* lut, int n_lut, float support,
+                             const T* vol, T* sino, hipStream_t s)
+    {
+        hipError_t e = launch_fill<T>(sino, dims.nrays, T(0), s);
+        if (e != hipSuccess)
+            return e;
+        if (dims.dim == 2)
+            hipLaunchKernelGGL((voxel_fwd_kernel<2, T>), vox_pose_grid(dims), dim3(256), 0, s, geo, dims, lut, n_lut,
+                               support, vol, sino, nullptr);
+        else
+            hipLaunchKernelGGL((voxel_fwd_kernel<3, T>), vox_pose_grid(dims), dim3(256), 0, s, geo, dims, lut, n_lut,
+                               support, vol, sino, nullptr);
+        return hipGetLastError();
+    }
+
+    template <class T>
+    hipError_t voxel_adjoint(const PoseGeom* geo, const Dims& dims, const T* lut, int n_lut, float support,
+                             const T* sino, T* vol, hipStream_t s)
+    {
+        const dim3 grid((unsigned) ((dims.nvox + 255) / 256));
+        if (dims.dim == 2)
+            hipLaunchKernelGGL((voxel_adj_kernel<2, T>), grid, dim3(256), 0, s, geo, dims, lut, n_lut, support, sino,
+                               vol);
+        else
+            hipLaunchKernelGGL((voxel_adj_kernel<3, T>), grid, dim3(256), 0, s, geo, dims, lut, n_lut, support, sino,
+                               vol);
+        return hipGetLastError();
+    }
+
+    hipError_t voxel_count(const PoseGeom* geo, const Dims& dims, float support, unsigned long long* total,
+                           hipStream_t s)
+    {
+        if (dims.dim == 2)
+            hipLaunchKernelGGL((voxel_fwd_kernel<2, float>), vox_pose_grid(dims), dim3(256), 0, s, geo, dims,
+                               (const float*) nullptr, 0, support, (const float*) nullptr, (float*) nullptr, total);
+        else
+            hipLaunchKernelGGL((voxel_fwd_kernel<3, float>), vox_pose_grid(dims), dim3(256), 0, s, geo, dims,
+                               (const float*) nullptr, 0, support, (const float*) nullptr, (float*) nullptr, total);
+        return hipGetLastError();
+    }
+
+#define EHP_INSTANTIATE(T)                                                                                         \
+    template hipError_t launch_fill<T>(T*, int64_t, T, hipStream_t);                                             \
+    template hipError_t joseph_forward<T>(const PoseGeom*, const Dims&, const T*, T*, hipStream_t);              \
+    template hipError_t joseph_adjoint<T>(const PoseGeom*, const Dims&, const T*, T*, hipStream_t);              \
+    template hipError_t siddon_forward<T>(const PoseGeom*, const Dims&, const T*, T*, hipStream_t);              \
+    template hipError_t siddon_adjoint<T>(const PoseGeom*, const Dims&, const T*, T*, hipStream_t);              \
+    template hipError_t voxel_forward<T>(const PoseGeom*, const Dims&, const T*, int, float, const T*, T*,        \
+                                         hipStream_t);                                                           \
+    template hipError_t voxel_adjoint<T>(const PoseGeom*, const Dims&, const T*, int, float, const T*, T*,        \
+                                         hipStream_t);
+
+    EHP_INSTANTIATE(float)
+    EHP_INSTANTIATE(double)
+#undef EHP_INSTANTIATE
+} // namespace ehp

@@ -1,0 +1,32 @@
+// ehp_kernels.hpp -- launchers of the projector kernels (all asynchronous on stream s).
+#pragma once
+
+#include "ehp_device.hpp"
+
+namespace ehp
+{
+    template <class T>
+    hipError_t launch_fill(T* p, int64_t n, T value, hipStream_t s);
+
+    template <class T>
+    hipError_t joseph_forward(const PoseGeom* geo, const Dims& dims, const T* vol, T* sino, hipStream_t s);
+    template <class T>
+    hipError_t joseph_adjoint(const PoseGeom* geo, const Dims& dims, const T* sino, T* vol, hipStream_t s);
+    hipError_t joseph_info(const PoseGeom* geo, const Dims& dims, int32_t* axis, int64_t* nsteps,
+                           unsigned long long* total, hipStream_t s);
+
+    template <class T>
+    hipError_t siddon_forward(const PoseGeom* geo, const Dims& dims, const T* vol, T* sino, hipStream_t s);
+    template <class T>
+    hipError_t siddon_adjoint(const PoseGeom* geo, const Dims& dims, const T* sino, T* vol, hipStream_t s);
+    hipError_t siddon_count(const PoseGeom* geo, const Dims& dims, unsigned long long* total, hipStream_t s);
+
+    template <class T>
+    hipError_t voxel_forward(const PoseGeom* geo, const Dims& dims, const T* lut, int n_lut, float support,
+                             const T* vol, T* sino, hipStream_t s);
+    template <class T>
+    hipError_t voxel_adjoint(const PoseGeom* geo, const Dims& dims, const T* lut, int n_lut, float support,
+                             const T* sino, T* vol, hipStream_t s);
+    hipError_t voxel_count(const PoseGeom* geo, const Dims& dims, float support, unsigned long long* total,
+                           hipStream_t s);
+} // namespace ehp

@@ -1,0 +1,237 @@
+"""GPU X-ray projectors with elsa's class names and LinearOperator surface.
+
+  JosephsMethodCUDA(domainDescriptor, rangeDescriptor, fast=True)
+      elsa/projectors_cuda/JosephsMethodCUDA.h:78-79, pyelsa bind_projectors_cuda.cpp:30-44
+  SiddonsMethodCUDA(domainDescriptor, rangeDescriptor)
+      elsa/projectors_cuda/SiddonsMethodCUDA.h:71-72
+  BlobVoxelProjectorCUDA(domainDescriptor, rangeDescriptor, radius=2, alpha=10.83, order=2)
+      elsa/projectors_cuda/VoxelProjectorCUDA.h:39-43
+  BSplineVoxelProjectorCUDA(domainDescriptor, rangeDescriptor, order=2)
+      elsa/projectors_cuda/VoxelProjectorCUDA.h:82-84
+
+apply(x[, out]) / applyAdjoint(y[, out]) follow elsa::LinearOperator
+(elsa/core/LinearOperator.h:75-111): the result is overwritten. Data are torch tensors on the
+GPU (the projection runs on torch's current stream) or numpy arrays (copied to the GPU and
+back). Volume arrays have shape ([z,] y, x) and sinograms (poses, [v,] u): C order of elsa's
+x-fastest layout. Every projection runs a HIP kernel of libelsahip.so; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .descriptors import DetectorDescriptor, VolumeDescriptor
+
+_DT = {np.dtype(np.float32): _lib.EHP_F32, np.dtype(np.float64): _lib.EHP_F64}
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+class LinearOperator:
+    """apply / applyAdjoint over a domain and a range descriptor."""
+
+    def __init__(self, domainDescriptor, rangeDescriptor):
+        self._domain = domainDescriptor
+        self._range = rangeDescriptor
+
+    def getDomainDescriptor(self):
+        return self._domain
+
+    def getRangeDescriptor(self):
+        return self._range
+
+    def apply(self, x, out=None):
+        return self._run(x, out, adjoint=False)
+
+    def applyAdjoint(self, y, out=None):
+        return self._run(y, out, adjoint=True)
+
+    def _run(self, x, out, adjoint):
+        raise NotImplementedError
+
+
+class _HipProjector(LinearOperator):
+    """Holds one ehp_op handle; the device geometry lives as long as the object."""
+
+    _dtype = np.dtype(np.float32)
+
+    def __init__(self, domainDescriptor: VolumeDescriptor, rangeDescriptor: DetectorDescriptor, dtype=np.float32):
+        super().__init__(domainDescriptor, rangeDescriptor)
+        dim = domainDescriptor.getNumberOfDimensions()
+        if dim not in (2, 3):
+            raise ValueError(f"{type(self).__name__}: only supporting 2d/3d operations")
+        if dim != rangeDescriptor.getNumberOfDimensions():
+            raise ValueError(f"{type(self).__name__}: domain and range dimension need to match")
+        if rangeDescriptor.getNumberOfGeometryPoses() == 0:
+            raise ValueError(f"{type(self).__name__}: geometry list was empty")
+        self._dtype = np.dtype(dtype)
+        if self._dtype not in _DT:
+            raise ValueError(f"unsupported dtype {dtype}")
+        self._op = ctypes.c_void_p()
+        self._L = _lib.load()
+        P, Pinv, C, ext, sdd = rangeDescriptor.pose_arrays()
+        vol = domainDescriptor.getNumberOfCoefficientsPerDimension()
+        det = rangeDescriptor.getNumberOfCoefficientsPerDimension()[: dim - 1].copy()
+        geo = ctypes.c_void_p()
+        fp = ctypes.POINTER(ctypes.c_float)
+        i64p = ctypes.POINTER(ctypes.c_int64)
+        _lib.check(self._L.ehp_geometry_create(dim, rangeDescriptor.getNumberOfGeometryPoses(), vol.ctypes.data_as(i64p),
+                                               det.ctypes.data_as(i64p), P.ctypes.data_as(fp), Pinv.ctypes.data_as(fp),
+                                               C.ctypes.data_as(fp), ext.ctypes.data_as(fp), sdd.ctypes.data_as(fp),
+                                               ctypes.byref(geo)), type(self).__name__)
+        try:
+            self._create(geo)
+        finally:
+            self._L.ehp_geometry_destroy(geo)
+        nv, ns = ctypes.c_int64(), ctypes.c_int64()
+        _lib.check(self._L.ehp_op_sizes(self._op, ctypes.byref(nv), ctypes.byref(ns)))
+        self._n_vol, self._n_sino = nv.value, ns.value
+
+    def _create(self, geo):
+        raise NotImplementedError
+
+    def __del__(self):
+        op = getattr(self, "_op", None)
+        if op is not None and op.value:
+            self._L.ehp_destroy(op)
+            self._op = None
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        return self._op
+
+    def sample_count(self) -> int:
+        """Exact number of samples (Joseph), voxel crossings (Siddon) or footprint taps
+        (voxel projectors) of one pass."""
+        s = ctypes.c_uint64()
+        _lib.check(self._L.ehp_sample_count(self._op, ctypes.byref(s)))
+        return int(s.value)
+
+    def _run(self, x, out, adjoint):
+        torch = _torch()
+        n_in, n_out = (self._n_sino, self._n_vol) if adjoint else (self._n_vol, self._n_sino)
+        shape_out = self._domain.shape if adjoint else self._range.shape
+        to_numpy = isinstance(x, np.ndarray)
+        tdt = torch.float32 if self._dtype == np.float32 else torch.float64
+        if to_numpy:
+            xt = torch.from_numpy(np.ascontiguousarray(x, dtype=self._dtype)).cuda()
+        else:
+            if not x.is_cuda:
+                raise ValueError("input tensor must live on the GPU")
+            if x.dtype != tdt:
+                raise ValueError(f"input dtype {x.dtype} does not match the operator's {tdt}")
+            xt = x.contiguous()
+        if xt.numel() != n_in:
+            raise ValueError(f"input has {xt.numel()} elements, expected {n_in}")
+        if out is None or to_numpy:
+            ot = torch.empty(shape_out, dtype=tdt, device=xt.device)
+        else:
+            ot = out
+            if not (ot.is_cuda and ot.dtype == tdt and ot.is_contiguous() and ot.numel() == n_out):
+                raise ValueError("output tensor must be a contiguous GPU tensor of the range size and dtype")
+        stream = ctypes.c_void_p(torch.cuda.current_stream(xt.device).cuda_stream)
+        f = self._L.ehp_adjoint if adjoint else self._L.ehp_forward
+        _lib.check(f(self._op, ctypes.c_void_p(xt.data_ptr()), ctypes.c_void_p(ot.data_ptr()), stream),
+                   "applyAdjoint" if adjoint else "apply")
+        if to_numpy:
+            return ot.cpu().numpy()
+        return ot
+
+
+class JosephsMethodCUDA(_HipProjector):
+    """Joseph's method with the discretization of elsa's CPU JosephsMethod
+    (JosephsMethod.cpp:140-198) and its matched adjoint. ``fast`` is accepted for
+    interface compatibility; the adjoint is always the matched one."""
+
+    def __init__(self, domainDescriptor, rangeDescriptor, fast: bool = True, dtype=np.float32):
+        self.fast = bool(fast)
+        super().__init__(domainDescriptor, rangeDescriptor, dtype)
+
+    def _create(self, geo):
+        _lib.check(self._L.ehp_joseph_create(geo, _DT[self._dtype], 0, ctypes.byref(self._op)), "JosephsMethodCUDA")
+
+    def ray_info(self):
+        """Driving axis and sample count of every ray as the device computes them."""
+        axis = np.empty(self._n_sino, np.int32)
+        nsteps = np.empty(self._n_sino, np.int64)
+        _lib.check(self._L.ehp_joseph_ray_info(self._op, axis.ctypes.data_as(ctypes.c_void_p),
+                                               nsteps.ctypes.data_as(ctypes.c_void_p)))
+        return axis, nsteps
+
+
+class SiddonsMethodCUDA(_HipProjector):
+    """Siddon's method (traversal of TraverseAABB.cpp:6-184)."""
+
+    def __init__(self, domainDescriptor, rangeDescriptor, dtype=np.float32):
+        super().__init__(domainDescriptor, rangeDescriptor, dtype)
+
+    def _create(self, geo):
+        _lib.check(self._L.ehp_siddon_create(geo, _DT[self._dtype], ctypes.byref(self._op)), "SiddonsMethodCUDA")
+
+
+def blob_lut(radius=2.0, alpha=10.83, order=2, n=100, dtype=np.float32):
+    """Table of elsa's ProjectedBlob<data_t, n> (Blobs.h:249-317) and its support."""
+    out = np.empty(n, dtype)
+    sup = ctypes.c_double()
+    _lib.check(_lib.load().ehp_blob_lut(_DT[np.dtype(dtype)], radius, alpha, order, n,
+                                        out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(sup)), "blob_lut")
+    return out, sup.value
+
+
+def bspline_lut(dim, order=2, n=100, dtype=np.float32):
+    """Table of elsa's ProjectedBSpline<data_t, n> (BSplines.h:132-208) and its support."""
+    out = np.empty(n, dtype)
+    sup = ctypes.c_double()
+    _lib.check(_lib.load().ehp_bspline_lut(_DT[np.dtype(dtype)], dim, order, n,
+                                           out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(sup)), "bspline_lut")
+    return out, sup.value
+
+
+class _VoxelProjector(_HipProjector):
+    def _create(self, geo):
+        _lib.check(self._L.ehp_voxel_create(geo, _DT[self._dtype], self.lut.ctypes.data_as(ctypes.c_void_p),
+                                            int(self.lut.shape[0]), float(self.support), ctypes.byref(self._op)),
+                   type(self).__name__)
+
+
+class BlobVoxelProjectorCUDA(_VoxelProjector):
+    """Voxel-driven projector with Kaiser-Bessel blobs (elsa BlobVoxelProjectorCUDA)."""
+
+    def __init__(self, domainDescriptor, rangeDescriptor, radius=2.0, alpha=10.83, order=2, dtype=np.float32):
+        self.radius, self.alpha, self.order = radius, alpha, int(order)
+        self.lut, self.support = blob_lut(radius, alpha, order, 100, dtype)
+        super().__init__(domainDescriptor, rangeDescriptor, dtype)
+
+
+class BSplineVoxelProjectorCUDA(_VoxelProjector):
+    """Voxel-driven projector with B-splines (elsa BSplineVoxelProjectorCUDA)."""
+
+    def __init__(self, domainDescriptor, rangeDescriptor, order=2, dtype=np.float32):
+        self.order = int(order)
+        self.lut, self.support = bspline_lut(domainDescriptor.getNumberOfDimensions(), order, 100, dtype)
+        super().__init__(domainDescriptor, rangeDescriptor, dtype)
+
+
+# pyelsa exposes float and double variants with f / d suffixes and un-suffixed float aliases
+def _variant(cls, dtype, name):
+    def init(self, *args, **kwargs):
+        kwargs.setdefault("dtype", dtype)
+        cls.__init__(self, *args, **kwargs)
+
+    return type(name, (cls,), {"__init__": init, "__doc__": cls.__doc__})
+
+
+JosephsMethodCUDAf = _variant(JosephsMethodCUDA, np.float32, "JosephsMethodCUDAf")
+JosephsMethodCUDAd = _variant(JosephsMethodCUDA, np.float64, "JosephsMethodCUDAd")
+SiddonsMethodCUDAf = _variant(SiddonsMethodCUDA, np.float32, "SiddonsMethodCUDAf")
+SiddonsMethodCUDAd = _variant(SiddonsMethodCUDA, np.float64, "SiddonsMethodCUDAd")
+BlobVoxelProjectorCUDAf = _variant(BlobVoxelProjectorCUDA, np.float32, "BlobVoxelProjectorCUDAf")
+BlobVoxelProjectorCUDAd = _variant(BlobVoxelProjectorCUDA, np.float64, "BlobVoxelProjectorCUDAd")
+BSplineVoxelProjectorCUDAf = _variant(BSplineVoxelProjectorCUDA, np.float32, "BSplineVoxelProjectorCUDAf")
+BSplineVoxelProjectorCUDAd = _variant(BSplineVoxelProjectorCUDA, np.float64, "BSplineVoxelProjectorCUDAd")

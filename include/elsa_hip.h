@@ -1,0 +1,140 @@
+/*
+ * elsa_hip.h -- C-ABI of libelsahip.so, the MI355X (gfx950) X-ray projectors.
+ *
+ * These entry points replace the static C++ launcher members that elsa's GPU projector
+ * classes call (not extern "C" in elsa):
+ *   TraverseJosephsCUDA<T,dim>::traverseForward / traverseAdjoint
+ *       elsa/projectors_cuda/projector_kernels/TraverseJosephsCUDA.cuh:61-65, :89-93
+ *   TraverseSiddonsCUDA<T,dim>::traverseForward / traverseAdjoint
+ *       elsa/projectors_cuda/projector_kernels/TraverseSiddonsCUDA.cuh:60-89
+ *   ProjectVoxelsCUDA<T,dim,adjoint,type>::project
+ *       elsa/projectors_cuda/projector_kernels/ProjectVoxelsCUDA.cuh:49-52
+ * together with the per-operator device state that the host classes set up in their
+ * constructors (JosephsMethodCUDA.cpp:9-92, SiddonsMethodCUDA.cpp:13-59,
+ * VoxelProjectorCUDA.cpp:16-54 with VoxelCUDAHelper.cuh:18-48).
+ *
+ * Conventions
+ *  - Every call returns EHP_OK (0) on success or an EHP_ERR_* code; the message of the
+ *    last failure on the calling thread is ehp_last_error().
+ *  - Geometry is elsa's real_t = float, matrices column-major as Eigen stores them:
+ *      P    dim x (dim+1)   projection matrix          Geometry::getProjectionMatrix()
+ *      Pinv (dim+1) x dim   inverse projection matrix  Geometry::getInverseProjectionMatrix()
+ *      C    dim             camera centre              Geometry::getCameraCenter()
+ *      ext  dim x (dim+1)   extrinsic matrix           Geometry::getExtrinsicMatrix()
+ *      sdd  1               source-detector distance   Geometry::getSourceDetectorDistance()
+ *    one block per pose, poses contiguous.
+ *  - Data layout is elsa's (DataDescriptor.cpp:25-28): volume x fastest, then y (then z);
+ *    sinogram u fastest, then v (3D), then the pose index.
+ *  - Data pointers are device pointers to the first element of the container (a view's
+ *    begin pointer, never the parent storage). Outputs are overwritten, never accumulated.
+ *  - Calls are asynchronous on the given HIP stream (hipStream_t passed as void*; NULL is
+ *    the default stream). Inputs must stay valid until the stream reaches the call.
+ *  - Host-only helpers (ehp_geometry_pose_*, ehp_*_lut) never touch the GPU.
+ */
+#ifndef ELSA_HIP_H
+#define ELSA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum ehp_status {
+    EHP_OK = 0,
+    EHP_ERR_ARGUMENT = 1,    /* elsa: InvalidArgumentError / std::invalid_argument */
+    EHP_ERR_HIP = 2,         /* a HIP runtime call failed (elsa: std::logic_error) */
+    EHP_ERR_ALLOC = 3,       /* device allocation failed (elsa: std::bad_alloc) */
+    EHP_ERR_UNSUPPORTED = 4  /* valid request this build does not implement */
+};
+
+enum ehp_dtype {
+    EHP_F32 = 0,  /* data_t = float  */
+    EHP_F64 = 1   /* data_t = double */
+};
+
+typedef struct ehp_geometry ehp_geometry; /* device-resident per-pose matrices      */
+typedef struct ehp_op ehp_op;             /* one projector (Joseph / Siddon / voxel) */
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* ehp_last_error(void);
+
+/* Library version string. */
+const char* ehp_version(void);
+
+/* Upload the geometry of a trajectory (elsa: the ctor-time upload of JosephsMethodCUDA.cpp:
+ * 33-91 and transferGeometries, VoxelCUDAHelper.cuh:18-48). vol_dims has dim entries,
+ * det_dims dim-1 entries. All pointer arguments are host memory. */
+int ehp_geometry_create(int dim, int64_t nposes, const int64_t* vol_dims, const int64_t* det_dims,
+                        const float* P, const float* Pinv, const float* C, const float* ext,
+                        const float* sdd, ehp_geometry** out);
+void ehp_geometry_destroy(ehp_geometry* g);
+
+/* Joseph's method (elsa: JosephsMethodCUDA(volume, detector, fast), JosephsMethodCUDA.h:78-79,
+ * with the discretization of the CPU JosephsMethod, JosephsMethod.cpp:140-198). The adjoint
+ * is always the matched one; flags are reserved and must be 0. */
+int ehp_joseph_create(const ehp_geometry* g, int dtype, int flags, ehp_op** out);
+
+/* Siddon's method (elsa: SiddonsMethodCUDA(volume, detector), SiddonsMethodCUDA.h:71-72,
+ * traversal of TraverseAABB.cpp:6-184). */
+int ehp_siddon_create(const ehp_geometry* g, int dtype, ehp_op** out);
+
+/* Voxel-driven projector with a radially symmetric basis function sampled in a table of
+ * n_lut values over [0, support] (elsa: BlobVoxelProjectorCUDA / BSplineVoxelProjectorCUDA,
+ * VoxelProjectorCUDA.h:39-43, :82-84; the table is ProjectedBlob::get_lut() or
+ * ProjectedBSpline::get_lut(), Blobs.h:249-317, BSplines.h:132-208). lut is host memory of
+ * the op's dtype. */
+int ehp_voxel_create(const ehp_geometry* g, int dtype, const void* lut, int n_lut, double support,
+                     ehp_op** out);
+
+void ehp_destroy(ehp_op* op);
+
+/* Forward projection Ax of the volume d_vol into d_sino (elsa: LinearOperator::apply). */
+int ehp_forward(ehp_op* op, const void* d_vol, void* d_sino, void* stream);
+
+/* Back projection A^T y of d_sino into d_vol (elsa: LinearOperator::applyAdjoint). */
+int ehp_adjoint(ehp_op* op, const void* d_sino, void* d_vol, void* stream);
+
+/* Number of elements of the domain (volume) and range (sinogram). */
+int ehp_op_sizes(const ehp_op* op, int64_t* n_vol, int64_t* n_sino);
+
+/* Joseph: exact number of samples (ray x driving-plane crossings inside the volume) of one
+ * pass, the unit of the roofline. Siddon: number of voxel crossings. Voxel projectors:
+ * number of footprint taps. Computed on the device at create time. */
+int ehp_sample_count(const ehp_op* op, uint64_t* samples);
+
+/* Per-ray Joseph traversal state as the kernels compute it (for parity checks): driving
+ * axis and number of samples of every ray, host memory of n_sino entries each. */
+int ehp_joseph_ray_info(const ehp_op* op, int32_t* axis, int64_t* nsteps);
+
+/* ---- host-only helpers (no GPU) -------------------------------------------------------- */
+
+/* One pose of elsa's Geometry (elsa/core/Geometry.cpp:12-51 for dim 2, :53-109 for dim 3):
+ * source-to-centre s2c, centre-to-detector c2d, volume spacing and origin (dim), detector
+ * spacing and origin (dim-1), rotation angles in radians (dim 2: gamma only), principal point
+ * offset (dim-1) and centre-of-rotation offset (dim). Writes P, Pinv, C, ext, sdd as above. */
+int ehp_geometry_pose(int dim, float s2c, float c2d, const float* vol_spacing,
+                      const float* vol_origin, const float* det_spacing, const float* det_origin,
+                      float gamma, float beta, float alpha, const float* principal_offset,
+                      const float* rotation_offset, float* P, float* Pinv, float* C, float* ext,
+                      float* sdd);
+
+/* Circular trajectory of elsa's CircleTrajectoryGenerator::createTrajectory
+ * (CircleTrajectoryGenerator.cpp:9-24): nposes angles LinSpaced over [0, arc_degrees]
+ * (both ends included), detector size det_dims (or floor(N*sqrt 2) per axis when det_dims is
+ * NULL; the chosen size is written back to det_out), volume spacing 1. Writes nposes blocks. */
+int ehp_circle_trajectory(int dim, int64_t nposes, const int64_t* vol_dims, float arc_degrees,
+                          float s2c, float c2d, const int64_t* det_dims, int64_t* det_out,
+                          float* P, float* Pinv, float* C, float* ext, float* sdd);
+
+/* LUT of elsa's ProjectedBlob<data_t, n> (Blobs.h:249-317) and ProjectedBSpline<data_t, n>
+ * (BSplines.h:132-208), in the given dtype; *support receives the table's support. */
+int ehp_blob_lut(int dtype, double radius, double alpha, int64_t order, int n, void* out,
+                 double* support);
+int ehp_bspline_lut(int dtype, int64_t dim, int64_t order, int n, void* out, double* support);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ELSA_HIP_H */

@@ -1,0 +1,318 @@
+"""CPU oracle for the elsa X-ray projectors -- TEST INFRASTRUCTURE ONLY.
+
+A ctypes front end over ``oracle/liboracle.so`` (built from ``oracle/elsa_oracle.cpp`` by
+``oracle/Makefile``). It restates elsa's CPU projectors and geometry in FP32 (elsa's own
+precision) and FP64. Only ``tests/``, the smoke check and the ``cpu_baseline`` leg of
+``bench.py`` use it; the product package ``elsa_amd`` never imports it.
+
+Every function names the reference routine it follows (paths relative to the elsa tree).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+JOSEPH, SIDDON = 0, 1
+MODE_ELSA, MODE_DIRECT = 0, 1
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+
+
+def build() -> str:
+    """Compile liboracle.so with the committed Makefile (g++, no GPU needed)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        build()
+    L = ctypes.CDLL(_LIB_PATH)
+    c_int, c_i64, c_f, c_d = ctypes.c_int, ctypes.c_int64, ctypes.c_float, ctypes.c_double
+    L.orc_xray_f32.argtypes = [c_int, c_int, c_int, c_int, _i64p, _i64p, c_i64, _f32p, _f32p, _f32p, _f32p, c_int]
+    L.orc_xray_f64.argtypes = [c_int, c_int, c_int, c_int, _i64p, _i64p, c_i64, _f64p, _f64p, _f64p, _f64p, c_int]
+    L.orc_voxel_f32.argtypes = [c_int, c_int, _i64p, _i64p, c_i64, _f32p, _f32p, _f32p, _f32p, c_int, c_f,
+                                _f32p, _f32p, c_int]
+    L.orc_voxel_f64.argtypes = [c_int, c_int, _i64p, _i64p, c_i64, _f64p, _f64p, _f64p, _f64p, c_int, c_d,
+                                _f64p, _f64p, c_int]
+    vp = ctypes.c_void_p
+    L.orc_joseph_rays_f32.argtypes = [c_int, _i64p, _i64p, c_i64, _f32p, _f32p, vp, vp, vp, vp, vp]
+    L.orc_joseph_rays_f64.argtypes = [c_int, _i64p, _i64p, c_i64, _f64p, _f64p, vp, vp, vp, vp, vp]
+    L.orc_make_ray_f32.argtypes = [c_int, _f32p, _f32p, c_i64, c_i64, _f32p, _f32p]
+    L.orc_intersect_f32.argtypes = [c_int, _f32p, _f32p, _i64p, ctypes.POINTER(c_f), ctypes.POINTER(c_f)]
+    L.orc_intersect_f32.restype = c_int
+    L.orc_geometry2d_f32.argtypes = [c_f, c_f, c_f, _f32p, _f32p, c_f, c_f, c_f, _f32p, _f32p, _f32p, _f32p,
+                                     _f32p, ctypes.POINTER(c_f)]
+    L.orc_geometry2d_f64.argtypes = [c_d, c_d, c_d, _f64p, _f64p, c_d, c_d, c_d, _f64p, _f64p, _f64p, _f64p,
+                                     _f64p, ctypes.POINTER(c_d)]
+    L.orc_geometry3d_f32.argtypes = [c_f, c_f, _f32p, _f32p, _f32p, _f32p, c_f, c_f, c_f, _f32p, _f32p, _f32p,
+                                     _f32p, _f32p, _f32p, ctypes.POINTER(c_f)]
+    L.orc_geometry3d_f64.argtypes = [c_d, c_d, _f64p, _f64p, _f64p, _f64p, c_d, c_d, c_d, _f64p, _f64p, _f64p,
+                                     _f64p, _f64p, _f64p, ctypes.POINTER(c_d)]
+    L.orc_bessi.argtypes = [c_i64, c_d]
+    L.orc_bessi.restype = c_d
+    L.orc_blob_evaluate_f64.argtypes = [c_d, c_d, c_d, c_i64]
+    L.orc_blob_evaluate_f64.restype = c_d
+    L.orc_blob_projected_f64.argtypes = [c_d, c_d, c_d, c_i64]
+    L.orc_blob_projected_f64.restype = c_d
+    L.orc_blob_lut_f32.argtypes = [c_f, c_f, c_i64, c_int, _f32p]
+    L.orc_blob_lut_f64.argtypes = [c_d, c_d, c_i64, c_int, _f64p]
+    L.orc_bspline_lut_f32.argtypes = [c_i64, c_i64, c_int, _f32p]
+    L.orc_bspline_lut_f64.argtypes = [c_i64, c_i64, c_int, _f64p]
+    L.orc_lut_eval_f64.argtypes = [_f64p, c_int, c_d, c_d]
+    L.orc_lut_eval_f64.restype = c_d
+    L.orc_lut_eval_f32.argtypes = [_f32p, c_int, c_f, c_f]
+    L.orc_lut_eval_f32.restype = c_f
+    _lib = L
+    return L
+
+
+# --------------------------------------------------------------------------------------
+# Geometry (elsa/core/Geometry.cpp) and the circular trajectory
+# (elsa/generators/CircleTrajectoryGenerator.cpp:9-24, BaseCircleTrajectoryGenerator.cpp:12-74,
+# TrajectoryGenerator.cpp:9-47).
+
+
+@dataclass
+class Geom:
+    """Per-view matrices of a trajectory, column-major blocks as Eigen stores them."""
+
+    dim: int
+    vol: tuple          # voxels per axis, x fastest
+    det: tuple          # detector pixels (u[, v])
+    P: np.ndarray       # nviews x dim*(dim+1)
+    Pinv: np.ndarray    # nviews x (dim+1)*dim
+    C: np.ndarray       # nviews x dim
+    ext: np.ndarray     # nviews x dim*(dim+1)
+    sdd: np.ndarray     # nviews
+
+    @property
+    def nviews(self) -> int:
+        return int(self.sdd.shape[0])
+
+    def astype(self, dt) -> "Geom":
+        return Geom(self.dim, self.vol, self.det, *(np.ascontiguousarray(a, dtype=dt) for a in
+                                                     (self.P, self.Pinv, self.C, self.ext, self.sdd)))
+
+    @property
+    def sino_shape(self):
+        """numpy shape of the sinogram (slowest first): (views[, v], u)."""
+        return (self.nviews,) + tuple(reversed(self.det))
+
+    @property
+    def vol_shape(self):
+        """numpy shape of the volume (slowest first): ([z,] y, x)."""
+        return tuple(reversed(self.vol))
+
+
+def geometry_pose(dim, s2c, c2d, vol, det, angle_rad=0.0, beta=0.0, alpha=0.0, vol_spacing=None,
+                  det_spacing=None, ppo=None, cor=None, dtype=np.float32):
+    """One pose of elsa's Geometry (Geometry.cpp:12-109); origins at the data centres."""
+    L = lib()
+    dt = np.dtype(dtype)
+    sfx = "f32" if dt == np.float32 else "f64"
+    ct = ctypes.c_float if dt == np.float32 else ctypes.c_double
+    vsp = np.ones(dim, dt) if vol_spacing is None else np.asarray(vol_spacing, dt)
+    vorg = (dt.type(0.5) * (np.asarray(vol, dt) * vsp)).astype(dt)
+    ssp = np.ones(dim - 1, dt) if det_spacing is None else np.asarray(det_spacing, dt)
+    sorg = (dt.type(0.5) * (np.asarray(det, dt) * ssp)).astype(dt)
+    cor = np.zeros(dim, dt) if cor is None else np.asarray(cor, dt)
+    P = np.zeros(dim * (dim + 1), dt)
+    Pinv = np.zeros(dim * (dim + 1), dt)
+    C = np.zeros(dim, dt)
+    ext = np.zeros(dim * (dim + 1), dt)
+    sdd = ct()
+    if dim == 2:
+        ppo = 0.0 if ppo is None else float(np.asarray(ppo).reshape(-1)[0])
+        getattr(L, f"orc_geometry2d_{sfx}")(s2c, c2d, angle_rad, vsp, vorg, ssp[0], sorg[0], ppo, cor, P, Pinv, C,
+                                            ext, ctypes.byref(sdd))
+    else:
+        ppo = np.zeros(2, dt) if ppo is None else np.asarray(ppo, dt)
+        getattr(L, f"orc_geometry3d_{sfx}")(s2c, c2d, vsp, vorg, ssp, sorg, angle_rad, beta, alpha, ppo, cor, P, Pinv,
+                                            C, ext, ctypes.byref(sdd))
+    return P, Pinv, C, ext, dt.type(sdd.value)
+
+
+def linspaced_f32(n, high):
+    """Eigen 3.4 LinSpaced(n, 0, high) in float: low + i*step, last element exactly high."""
+    if n == 1:
+        return np.array([high], np.float32)
+    step = np.float32(np.float32(high) / np.float32(n - 1))
+    out = np.arange(n, dtype=np.float32) * step
+    out[-1] = np.float32(high)
+    return out
+
+
+def circle_geometry(dim, vol, nviews, arc_deg, s2c, c2d, det=None, dtype=np.float32, angles_deg=None):
+    """Circular trajectory of elsa's CircleTrajectoryGenerator; detector defaults to floor(N*sqrt(2))."""
+    vol = tuple(int(v) for v in vol)
+    if det is None:
+        det = tuple(int(np.float32(v) * np.float32(np.sqrt(np.float32(2.0)))) for v in vol[:dim - 1])
+    det = tuple(int(d) for d in det)
+    if angles_deg is None:
+        thetas = linspaced_f32(nviews, arc_deg)
+    else:
+        thetas = np.asarray(angles_deg, np.float32)
+    dt = np.dtype(dtype)
+    Ps, Pis, Cs, Es, Ss = [], [], [], [], []
+    pi_f = np.float32(np.pi)
+    for th in thetas:
+        if dt == np.float32:
+            ang = np.float32(np.float32(th * pi_f) / np.float32(180))
+        else:
+            ang = float(th) * np.pi / 180.0
+        P, Pi, C, E, sdd = geometry_pose(dim, s2c, c2d, vol, det, angle_rad=ang, dtype=dt)
+        Ps.append(P); Pis.append(Pi); Cs.append(C); Es.append(E); Ss.append(sdd)
+    return Geom(dim, vol, det, np.stack(Ps), np.stack(Pis), np.stack(Cs), np.stack(Es), np.array(Ss, dt))
+
+
+def geom_from_poses(dim, vol, det, poses, dtype=np.float32):
+    """Stack explicit poses (tuples returned by geometry_pose) into a Geom."""
+    dt = np.dtype(dtype)
+    cols = list(zip(*poses))
+    return Geom(dim, tuple(vol), tuple(det), *(np.ascontiguousarray(np.stack(c) if i < 4 else np.array(c), dtype=dt)
+                                               for i, c in enumerate(cols)))
+
+
+# --------------------------------------------------------------------------------------
+# Projectors
+
+
+def _dims(g: Geom):
+    vol = np.array(list(g.vol) + [1] * (3 - g.dim), np.int64)
+    det = np.array(list(g.det) + [1] * (2 - len(g.det)), np.int64)
+    return vol, det
+
+
+def xray(kind, g: Geom, data, adjoint=False, mode=MODE_ELSA, nthreads=0):
+    """Joseph (kind=JOSEPH; JosephsMethod.cpp:140-198) or Siddon (kind=SIDDON;
+    SiddonsMethod.cpp:76-124) forward (adjoint=False) or back projection."""
+    L = lib()
+    dt = np.float64 if g.Pinv.dtype == np.float64 else np.float32
+    f = L.orc_xray_f64 if dt == np.float64 else L.orc_xray_f32
+    vol, det = _dims(g)
+    data = np.ascontiguousarray(data, dtype=dt)
+    out = np.empty(g.vol_shape if adjoint else g.sino_shape, dt)
+    rc = f(kind, g.dim, int(adjoint), mode, vol, det, g.nviews, np.ascontiguousarray(g.Pinv),
+           np.ascontiguousarray(g.C), data.reshape(-1), out.reshape(-1), nthreads)
+    assert rc == 0
+    return out
+
+
+def joseph(g, data, adjoint=False, mode=MODE_ELSA, nthreads=0):
+    return xray(JOSEPH, g, data, adjoint, mode, nthreads)
+
+
+def siddon(g, data, adjoint=False, nthreads=0):
+    return xray(SIDDON, g, data, adjoint, MODE_ELSA, nthreads)
+
+
+def voxel(g: Geom, data, lut, support, adjoint=False, nthreads=0):
+    """Blob / B-spline voxel projector (VoxelComputation.h:40-185, full-rectangle footprint)."""
+    L = lib()
+    dt = np.float64 if g.P.dtype == np.float64 else np.float32
+    f = L.orc_voxel_f64 if dt == np.float64 else L.orc_voxel_f32
+    vol, det = _dims(g)
+    data = np.ascontiguousarray(data, dtype=dt)
+    lut = np.ascontiguousarray(lut, dtype=dt)
+    out = np.empty(g.vol_shape if adjoint else g.sino_shape, dt)
+    rc = f(g.dim, int(adjoint), vol, det, g.nviews, np.ascontiguousarray(g.P), np.ascontiguousarray(g.ext),
+           np.ascontiguousarray(g.sdd), lut, lut.shape[0], support, data.reshape(-1), out.reshape(-1), nthreads)
+    assert rc == 0
+    return out
+
+
+def joseph_rays(g: Geom):
+    """Per-ray DrivingDirectionTraversal state: (axis, nsteps, pos0, step, len)."""
+    L = lib()
+    dt = np.float64 if g.Pinv.dtype == np.float64 else np.float32
+    f = L.orc_joseph_rays_f64 if dt == np.float64 else L.orc_joseph_rays_f32
+    vol, det = _dims(g)
+    n = g.nviews * int(np.prod(g.det))
+    axis = np.empty(n, np.int32)
+    nsteps = np.empty(n, np.int64)
+    pos0 = np.empty((n, 3), dt)
+    step = np.empty((n, 3), dt)
+    ln = np.empty(n, dt)
+    f(g.dim, vol, det, g.nviews, np.ascontiguousarray(g.Pinv), np.ascontiguousarray(g.C),
+      axis.ctypes.data, nsteps.ctypes.data, pos0.ctypes.data, step.ctypes.data, ln.ctypes.data)
+    return axis, nsteps, pos0, step, ln
+
+
+def joseph_sample_count(g: Geom) -> int:
+    return int(joseph_rays(g)[1].sum())
+
+
+def make_ray(g: Geom, view, u, v=0):
+    L = lib()
+    o = np.zeros(3, np.float32)
+    d = np.zeros(3, np.float32)
+    L.orc_make_ray_f32(g.dim, np.ascontiguousarray(g.Pinv[view], np.float32),
+                       np.ascontiguousarray(g.C[view], np.float32), u, v, o, d)
+    return o[:g.dim], d[:g.dim]
+
+
+def intersect(o, d, vol):
+    """elsa intersectRay (Intersection.cpp:35-46); returns (tmin, tmax) or None."""
+    L = lib()
+    o = np.ascontiguousarray(np.pad(np.asarray(o, np.float32), (0, 3 - len(o))))
+    dd = np.ascontiguousarray(np.pad(np.asarray(d, np.float32), (0, 3 - len(d))))
+    tmin, tmax = ctypes.c_float(), ctypes.c_float()
+    hit = L.orc_intersect_f32(len(vol), o, dd, np.array(list(vol) + [1] * (3 - len(vol)), np.int64),
+                              ctypes.byref(tmin), ctypes.byref(tmax))
+    return (tmin.value, tmax.value) if hit else None
+
+
+# --------------------------------------------------------------------------------------
+# Basis functions (Blobs.h, BSplines.h, Luts.hpp, Bessel.cpp)
+
+
+def bessi(m, x):
+    return lib().orc_bessi(m, x)
+
+
+def blob_evaluate(r, a=2.0, alpha=10.83, m=2):
+    return lib().orc_blob_evaluate_f64(r, a, alpha, m)
+
+
+def blob_projected(s, a=2.0, alpha=10.83, m=2):
+    return lib().orc_blob_projected_f64(s, a, alpha, m)
+
+
+def blob_lut(radius=2.0, alpha=10.83, order=2, N=100, dtype=np.float32):
+    out = np.empty(N, dtype)
+    if np.dtype(dtype) == np.float32:
+        lib().orc_blob_lut_f32(radius, alpha, order, N, out)
+    else:
+        lib().orc_blob_lut_f64(radius, alpha, order, N, out)
+    return out
+
+
+def bspline_lut(dim, order=2, N=100, dtype=np.float32):
+    out = np.empty(N, dtype)
+    if np.dtype(dtype) == np.float32:
+        lib().orc_bspline_lut_f32(dim, order, N, out)
+    else:
+        lib().orc_bspline_lut_f64(dim, order, N, out)
+    return out
+
+
+def lut_eval(lut, support, distance):
+    lut = np.ascontiguousarray(lut)
+    if lut.dtype == np.float64:
+        return lib().orc_lut_eval_f64(lut, lut.shape[0], support, distance)
+    return lib().orc_lut_eval_f32(lut, lut.shape[0], support, distance)

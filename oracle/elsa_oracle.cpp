@@ -1,0 +1,1112 @@
+// elsa_oracle.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// A CPU restatement of elsa's CPU X-ray projectors and of the geometry that feeds
+// them. It is the checker for the HIP projectors in elsa_amd: only tests/, the smoke
+// check and the cpu_baseline leg of bench.py load it, and never as the thing that is
+// measured or shipped. The product library (libelsahip.so) does not link it.
+//
+// Every routine is templated on the geometry/traversal type R and the data type T:
+//   *_f32 : R = float,  T = float   (elsa's own precision: real_t is float,
+//                                    elsa/core/elsaDefines.h:20)
+//   *_f64 : R = double, T = double  (an FP64 oracle with double geometry; elsa has none)
+// Built with -ffp-contract=off so that no product/sum pair is fused behind our back;
+// where a fused multiply-add is part of the algorithm it is written as std::fma.
+//
+// Parity pins (what checks each function; see tests/):
+//   geometry          closed forms P*Pinv = I, camera centre, KAT matrices from
+//                     elsa/core/tests/test_Geometry.cpp
+//   intersect_ray     KATs of elsa/projectors/tests/test_Intersection.cpp
+//   joseph_*          KATs of elsa/projectors/tests/test_JosephsMethod.cpp, linearity,
+//                     dot-product adjointness, FP64 brute force on tiny grids
+//   siddon_*          KATs of elsa/projectors/tests/test_SiddonsMethod.cpp and
+//                     elsa/projectors_cuda/tests/test_SiddonsMethodCUDA.cpp, exact
+//                     chord length sum, adjointness
+//   voxel_*           centre-voxel blob(0) KAT and blob(d/s) back-projection weights of
+//                     elsa/projectors_cuda/tests/test_VoxelProjectorCUDA.cpp, adjointness
+//   LUTs / Bessel     golden vectors of elsa/projectors/tests/test_Luts.cpp, Simpson
+//                     integration of the blob profile, scipy.special.iv
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace orc
+{
+    using i64 = int64_t;
+
+    // elsa/projectors/Intersection.h:26-45 (NaN-suppressing min/max)
+    template <class R>
+    inline R minNum(R x, R y)
+    {
+        return (y != y) ? x : (x < y ? x : y);
+    }
+    template <class R>
+    inline R maxNum(R x, R y)
+    {
+        return (y != y) ? x : (x > y ? x : y);
+    }
+
+    template <class R>
+    inline R sgn(R v)
+    {
+        return v > R(0) ? R(1) : (v < R(0) ? R(-1) : R(0));
+    }
+
+    // elsa/projectors/Intersection.cpp:7-46: slab test against the box [0, dims).
+    template <class R, int D>
+    bool intersect_ray(const R* o, const R* d, const i64* dims, R& tmin_out, R& tmax_out)
+    {
+        R invDir = R(1) / d[0];
+        R t1 = (R(0) - o[0]) * invDir;
+        R t2 = (R(dims[0]) - o[0]) * invDir;
+        R tmin = invDir >= 0 ? t1 : t2;
+        R tmax = invDir >= 0 ? t2 : t1;
+        for (int i = 1; i < D; ++i) {
+            invDir = R(1) / d[i];
+            t1 = (R(0) - o[i]) * invDir;
+            t2 = (R(dims[i]) - o[i]) * invDir;
+            tmin = maxNum(tmin, invDir >= 0 ? t1 : t2);
+            tmax = minNum(tmax, invDir >= 0 ? t2 : t1);
+        }
+        if (tmax == R(0) && tmin == R(0))
+            return false;
+        if (tmax >= maxNum(tmin, R(0))) {
+            tmin_out = tmin;
+            tmax_out = tmax;
+            return true;
+        }
+        return false;
+    }
+
+    // elsa/core/Descriptors/DetectorDescriptor.cpp:90-128: ray through the centre of
+    // detector pixel (u, v) of one pose. Pinv is (D+1) x D, column-major; the origin is
+    // the camera centre C. The direction is Pinv * [u+.5, (v+.5,) 1], head(D), normalised
+    // (division by the square root of the sequential sum of squares, as Eigen's
+    // normalized() on a dynamic vector does).
+    template <class R, int D>
+    void make_ray(const R* Pinv, const R* C, i64 u, i64 v, R* o, R* d)
+    {
+        R h[3];
+        h[0] = R(u) + R(0.5);
+        if (D == 3) {
+            h[1] = R(v) + R(0.5);
+            h[2] = R(1);
+        } else {
+            h[1] = R(1);
+        }
+        R y[3];
+        for (int i = 0; i < D; ++i) {
+            R acc = Pinv[i] * h[0];
+            for (int j = 1; j < D; ++j)
+                acc = acc + Pinv[i + j * (D + 1)] * h[j];
+            y[i] = acc;
+        }
+        R n2 = y[0] * y[0];
+        for (int i = 1; i < D; ++i)
+            n2 = n2 + y[i] * y[i];
+        const R n = std::sqrt(n2);
+        for (int i = 0; i < D; ++i) {
+            d[i] = n2 > R(0) ? y[i] / n : y[i];
+            o[i] = C[i];
+        }
+    }
+
+    // State of elsa's DrivingDirectionTraversal after construction
+    // (elsa/projectors/DrivingDirectionTraversal.cpp:6-57).
+    template <class R>
+    struct JRay {
+        int axis;
+        i64 nsteps;
+        R pos0[3];
+        R step[3];
+        R len;
+    };
+
+    template <class R, int D>
+    void joseph_setup(const R* o, const R* d, const i64* dims, JRay<R>& jr)
+    {
+        // driving direction: first index of max |d| (Eigen maxCoeff keeps the first)
+        int a = 0;
+        R m = std::abs(d[0]);
+        for (int i = 1; i < D; ++i)
+            if (std::abs(d[i]) > m) {
+                m = std::abs(d[i]);
+                a = i;
+            }
+        jr.axis = a;
+        jr.nsteps = 0;
+        jr.len = R(0);
+        for (int i = 0; i < 3; ++i) {
+            jr.pos0[i] = R(0);
+            jr.step[i] = R(0);
+        }
+        R tmin, tmax;
+        if (!intersect_ray<R, D>(o, d, dims, tmin, tmax))
+            return;
+        R cur[3], ex[3];
+        for (int i = 0; i < D; ++i) {
+            const R hi = R(dims[i]);
+            cur[i] = o[i] + tmin * d[i];
+            cur[i] = cur[i] < R(0) ? R(0) : cur[i];
+            cur[i] = cur[i] > hi ? hi : cur[i];
+            ex[i] = o[i] + tmax * d[i];
+            ex[i] = ex[i] < R(0) ? R(0) : ex[i];
+            ex[i] = ex[i] > hi ? hi : ex[i];
+        }
+        const R ad = std::abs(d[a]);
+        R step[3];
+        for (int i = 0; i < D; ++i)
+            step[i] = d[i] / ad;
+        const R dist = cur[a] - std::floor(cur[a]);
+        const R s = sgn(step[a]);
+        for (int i = 0; i < D; ++i)
+            cur[i] = cur[i] + step[i] * (R(0.5) - dist) * s;
+        cur[a] = std::floor(cur[a]) + R(0.5);
+        jr.nsteps = static_cast<i64>(std::ceil(std::abs(ex[a] - cur[a])));
+        // fixed-size Eigen norm: x0^2 + (x1^2 + x2^2) for 3D, x0^2 + x1^2 for 2D
+        R n2 = (D == 3) ? step[0] * step[0] + (step[1] * step[1] + step[2] * step[2])
+                        : step[0] * step[0] + step[1] * step[1];
+        jr.len = std::sqrt(n2);
+        for (int i = 0; i < D; ++i) {
+            jr.pos0[i] = cur[i];
+            jr.step[i] = step[i];
+        }
+    }
+
+    enum { MODE_ELSA = 0, MODE_DIRECT = 1 };
+
+    // elsa/projectors/JosephsMethod.cpp:140-198 with getLinearInterpolationWeights
+    // (:37-52) and doInterpolation (:60-97). MODE_ELSA advances the position by repeated
+    // addition of the step, exactly as DrivingDirectionTraversal::updateTraverse
+    // (DrivingDirectionTraversal.cpp:59-64). MODE_DIRECT evaluates sample s as
+    // pos0 + s*step with one fused multiply-add per transverse axis, which is what the
+    // HIP kernels compute.
+    template <class R, class T, int D, bool ADJ>
+    void joseph_ray(const JRay<R>& jr, const i64* dims, const i64* strides, int mode,
+                    const T* in, T* out, i64 ir)
+    {
+        T acc = T(0);
+        const int a = jr.axis;
+        const T len = static_cast<T>(jr.len);
+        R pos[3] = {jr.pos0[0], jr.pos0[1], jr.pos0[2]};
+        for (i64 s = 0; s < jr.nsteps; ++s) {
+            if (mode == MODE_DIRECT) {
+                for (int i = 0; i < D; ++i)
+                    pos[i] = (i == a) ? jr.pos0[i] + R(s) * jr.step[i]
+                                      : std::fma(R(s), jr.step[i], jr.pos0[i]);
+            }
+            i64 fl[3], cl[3];
+            R w[3], cw[3];
+            for (int i = 0; i < D; ++i) {
+                fl[i] = static_cast<i64>(std::floor(pos[i] - R(0.5)));
+                cl[i] = static_cast<i64>(std::ceil(pos[i] - R(0.5)));
+                cw[i] = (pos[i] - R(fl[i])) - R(0.5);
+                w[i] = R(1) - cw[i];
+            }
+            w[a] = R(1);
+            cw[a] = R(1);
+
+            auto tap = [&](const i64* c, R wt) {
+                bool inbox = true;
+                i64 lin = 0;
+                for (int i = 0; i < D; ++i) {
+                    inbox = inbox && (c[i] >= 0 && c[i] < dims[i]);
+                    const i64 cc = std::max<i64>(0, std::min<i64>(c[i], dims[i] - 1));
+                    lin += cc * strides[i];
+                }
+                const R weight = R(inbox ? 1 : 0) * wt;
+                if (ADJ) {
+#pragma omp atomic
+                    out[lin] += in[ir] * len * static_cast<T>(weight);
+                } else {
+                    acc += in[lin] * len * static_cast<T>(weight);
+                }
+            };
+            if (D == 2) {
+                const i64 c0[2] = {fl[0], cl[1]};
+                tap(c0, w[0] * cw[1]);
+                const i64 c1[2] = {cl[0], fl[1]};
+                tap(c1, cw[0] * w[1]);
+            } else {
+                const i64 c0[3] = {fl[0], fl[1], fl[2]};
+                tap(c0, w[0] * w[1] * w[2]);
+                const i64 c1[3] = {fl[0], cl[1], cl[2]};
+                tap(c1, w[0] * cw[1] * cw[2]);
+                const i64 c2[3] = {cl[0], fl[1], cl[2]};
+                tap(c2, cw[0] * w[1] * cw[2]);
+                const i64 c3[3] = {cl[0], cl[1], fl[2]};
+                tap(c3, cw[0] * cw[1] * w[2]);
+            }
+            if (mode == MODE_ELSA)
+                for (int i = 0; i < D; ++i)
+                    pos[i] += jr.step[i];
+        }
+        if (!ADJ)
+            out[ir] = acc;
+    }
+
+    struct Shape {
+        int dim;
+        i64 vol[3];
+        i64 det[2];
+        i64 nviews;
+        i64 strides[3];
+        i64 nvox;
+        i64 pix_per_view;
+        i64 nrays;
+    };
+
+    inline Shape make_shape(int dim, const i64* vol, const i64* det, i64 nviews)
+    {
+        Shape s{};
+        s.dim = dim;
+        for (int i = 0; i < 3; ++i)
+            s.vol[i] = i < dim ? vol[i] : 1;
+        s.det[0] = det[0];
+        s.det[1] = dim == 3 ? det[1] : 1;
+        s.nviews = nviews;
+        s.strides[0] = 1;
+        for (int i = 1; i < 3; ++i)
+            s.strides[i] = s.strides[i - 1] * s.vol[i - 1];
+        s.nvox = s.vol[0] * s.vol[1] * s.vol[2];
+        s.pix_per_view = s.det[0] * s.det[1];
+        s.nrays = s.pix_per_view * nviews;
+        return s;
+    }
+
+    template <class R, int D>
+    inline void ray_of_index(const Shape& sh, const R* Pinv, const R* C, i64 ir, R* o, R* d)
+    {
+        const i64 view = ir / sh.pix_per_view;
+        const i64 rem = ir - view * sh.pix_per_view;
+        const i64 u = rem % sh.det[0];
+        const i64 v = rem / sh.det[0];
+        make_ray<R, D>(Pinv + view * (D + 1) * D, C + view * D, u, v, o, d);
+    }
+
+    inline void set_threads(int nthreads)
+    {
+#ifdef _OPENMP
+        if (nthreads > 0)
+            omp_set_num_threads(nthreads);
+#else
+        (void) nthreads;
+#endif
+    }
+
+    template <class R, class T, int D, bool ADJ>
+    void joseph_all(const Shape& sh, const R* Pinv, const R* C, int mode, const T* in, T* out)
+    {
+        if (ADJ)
+            std::fill(out, out + sh.nvox, T(0));
+#pragma omp parallel for schedule(dynamic, 256)
+        for (i64 ir = 0; ir < sh.nrays; ++ir) {
+            R o[3], d[3];
+            ray_of_index<R, D>(sh, Pinv, C, ir, o, d);
+            JRay<R> jr;
+            joseph_setup<R, D>(o, d, sh.vol, jr);
+            joseph_ray<R, T, D, ADJ>(jr, sh.vol, sh.strides, mode, in, out, ir);
+        }
+    }
+
+    // elsa/projectors/TraverseAABB.cpp:6-184 driven by SiddonsMethod::doTraverseRay*
+    // (elsa/projectors/SiddonsMethod.cpp:76-124). Ties in T step several axes at once
+    // (calcMask, TraverseAABB.cpp:59-69). The length of each segment is tExit - tEntry with
+    // tExit starting at 0 (TraverseAABB.h: _tExit{0.0}).
+    template <class R, class T, int D, bool ADJ>
+    void siddon_ray(const R* o, const R* d, const i64* dims, const i64* strides, const T* in,
+                    T* out, i64 ir)
+    {
+        T acc = T(0);
+        R tmin, tmax;
+        bool inbox = intersect_ray<R, D>(o, d, dims, tmin, tmax);
+        if (inbox) {
+            R entry[3], cur[3], tdelta[3], T_[3];
+            i64 stepdir[3];
+            for (int i = 0; i < D; ++i) {
+                const R hi = R(dims[i]);
+                entry[i] = o[i] + tmin * d[i];
+                entry[i] = entry[i] < R(0) ? R(0) : entry[i];
+                entry[i] = entry[i] > hi ? hi : entry[i];
+                stepdir[i] = static_cast<i64>(sgn(d[i]));
+            }
+            for (int i = 0; i < D; ++i) {
+                R lower = std::floor(entry[i]);
+                if (lower == entry[i] && stepdir[i] < 0)
+                    lower = lower - R(1);
+                cur[i] = lower;
+                if (cur[i] >= R(dims[i]) || cur[i] < R(0))
+                    inbox = false;
+            }
+            const R EPS = std::numeric_limits<R>::epsilon();
+            const R MAX = std::numeric_limits<R>::max();
+            i64 idx = 0;
+            for (int i = 0; i < D; ++i) {
+                tdelta[i] = std::abs(d[i]) > EPS ? R(stepdir[i]) / d[i] : MAX;
+                T_[i] = std::abs(d[i]) > EPS ? ((d[i] > R(0) ? cur[i] + R(1) : cur[i]) - entry[i]) / d[i]
+                                             : MAX;
+                idx += strides[i] * static_cast<i64>(cur[i]);
+            }
+            R tExit = R(0);
+            while (inbox) {
+                const R tEntry = tExit;
+                R tm = T_[0];
+                for (int i = 1; i < D; ++i)
+                    tm = std::min(tm, T_[i]);
+                tExit = tm;
+                bool mask[3];
+                for (int i = 0; i < D; ++i) {
+                    bool mi = true;
+                    for (int j = 0; j < D; ++j)
+                        if (j != i)
+                            mi = mi && (T_[i] <= T_[j]);
+                    mask[i] = mi;
+                }
+                i64 nidx = idx;
+                bool nin = true;
+                for (int i = 0; i < D; ++i) {
+                    if (mask[i]) {
+                        cur[i] += R(stepdir[i]);
+                        T_[i] += tdelta[i];
+                        nidx += stepdir[i] * strides[i];
+                    }
+                }
+                for (int i = 0; i < D; ++i)
+                    nin = nin && (cur[i] < R(dims[i])) && (cur[i] >= R(0));
+                const R wgt = tExit - tEntry;
+                if (ADJ) {
+#pragma omp atomic
+                    out[idx] += in[ir] * static_cast<T>(wgt);
+                } else {
+                    acc += in[idx] * static_cast<T>(wgt);
+                }
+                idx = nidx;
+                inbox = nin;
+            }
+        }
+        if (!ADJ)
+            out[ir] = acc;
+    }
+
+    template <class R, class T, int D, bool ADJ>
+    void siddon_all(const Shape& sh, const R* Pinv, const R* C, const T* in, T* out)
+    {
+        if (ADJ)
+            std::fill(out, out + sh.nvox, T(0));
+#pragma omp parallel for schedule(dynamic, 256)
+        for (i64 ir = 0; ir < sh.nrays; ++ir) {
+            R o[3], d[3];
+            ray_of_index<R, D>(sh, Pinv, C, ir, o, d);
+            siddon_ray<R, T, D, ADJ>(o, d, sh.vol, sh.strides, in, out, ir);
+        }
+    }
+
+    // elsa/projectors/Luts.hpp:60-78 (linear interpolation in a table of N samples on
+    // [0, support]; zero outside).
+    template <class R, class T>
+    inline T lut_eval(const T* lut, int N, R support, R distance)
+    {
+        const R index = (distance / support) * R(N - 1);
+        if (index < R(0) || index > R(N - 1))
+            return T(0);
+        const i64 a = static_cast<i64>(std::floor(index));
+        const i64 b = static_cast<i64>(std::ceil(index));
+        const T fa = lut[a];
+        const T fb = lut[b];
+        const T t = static_cast<T>(index - static_cast<R>(a));
+        return t * fb + (T(1) - t) * fa;
+    }
+
+    // elsa/projectors/VoxelComputation.h:40-107 (visitDetector). The footprint is the full
+    // rectangle [ceil(c - r*s), floor(c + r*s)] in u (and v), clipped to the detector.
+    // elsa's 3D loop (VoxelComputation.h:90-105) walks a transposed, sheared grid whenever
+    // the u and v extents differ; this restatement visits the rectangle it describes.
+    template <class R, int D, class Fn>
+    inline void visit_footprint(const Shape& sh, const R* P, const R* ext, R sdd, R radius,
+                                i64 vox, i64 view, Fn apply)
+    {
+        i64 c[3];
+        c[0] = vox % sh.vol[0];
+        c[1] = (vox / sh.vol[0]) % sh.vol[1];
+        c[2] = vox / (sh.vol[0] * sh.vol[1]);
+        R hv[4];
+        for (int i = 0; i < D; ++i)
+            hv[i] = R(c[i]) + R(0.5);
+        hv[D] = R(1);
+        // P and ext are D x (D+1), column-major
+        R pc[3], ce[3];
+        for (int i = 0; i < D; ++i) {
+            R acc = P[i] * hv[0];
+            R acc2 = ext[i] * hv[0];
+            for (int j = 1; j <= D; ++j) {
+                acc = acc + P[i + j * D] * hv[j];
+                acc2 = acc2 + ext[i + j * D] * hv[j];
+            }
+            pc[i] = acc;
+            ce[i] = acc2;
+        }
+        R center[2];
+        for (int i = 0; i < D - 1; ++i)
+            center[i] = pc[i] / pc[D - 1] - R(0.5);
+        R n2 = ce[0] * ce[0];
+        for (int i = 1; i < D; ++i)
+            n2 = n2 + ce[i] * ce[i];
+        const R scaling = sdd / std::sqrt(n2);
+        const R rod = scaling * radius;
+        i64 lo[2], hi[2];
+        for (int i = 0; i < D - 1; ++i) {
+            lo[i] = std::max<i64>(0, static_cast<i64>(std::ceil(center[i] - rod)));
+            hi[i] = std::min<i64>(sh.det[i] - 1, static_cast<i64>(std::floor(center[i] + rod)));
+        }
+        const i64 base = view * sh.pix_per_view;
+        if (D == 2) {
+            for (i64 i = lo[0]; i <= hi[0]; ++i) {
+                const R dd = (center[0] - R(i)) / scaling;
+                apply(base + i, std::sqrt(dd * dd));
+            }
+        } else {
+            for (i64 j = lo[1]; j <= hi[1]; ++j)
+                for (i64 i = lo[0]; i <= hi[0]; ++i) {
+                    const R d0 = (center[0] - R(i)) / scaling;
+                    const R d1 = (center[1] - R(j)) / scaling;
+                    apply(base + j * sh.det[0] + i, std::sqrt(d0 * d0 + d1 * d1));
+                }
+        }
+    }
+
+    // Forward: views in parallel, voxels in order (VoxelComputation.h:109-146).
+    // Adjoint: voxels in parallel, views in order (VoxelComputation.h:148-185).
+    template <class R, class T, int D, bool ADJ>
+    void voxel_all(const Shape& sh, const R* P, const R* ext, const R* sdd, const T* lut, int N,
+                   R support, const T* in, T* out)
+    {
+        const int pstride = D * (D + 1);
+        if (!ADJ) {
+            std::fill(out, out + sh.nrays, T(0));
+#pragma omp parallel for schedule(dynamic, 1)
+            for (i64 g = 0; g < sh.nviews; ++g) {
+                for (i64 vox = 0; vox < sh.nvox; ++vox) {
+                    const T xv = in[vox];
+                    visit_footprint<R, D>(sh, P + g * pstride, ext + g * pstride, sdd[g], support,
+                                          vox, g, [&](i64 pix, R dist) {
+                                              out[pix] += xv * lut_eval<R, T>(lut, N, support, dist);
+                                          });
+                }
+            }
+        } else {
+#pragma omp parallel for schedule(dynamic, 64)
+            for (i64 vox = 0; vox < sh.nvox; ++vox) {
+                T acc = T(0);
+                for (i64 g = 0; g < sh.nviews; ++g) {
+                    visit_footprint<R, D>(sh, P + g * pstride, ext + g * pstride, sdd[g], support,
+                                          vox, g, [&](i64 pix, R dist) {
+                                              acc += lut_eval<R, T>(lut, N, support, dist) * in[pix];
+                                          });
+                }
+                out[vox] = acc;
+            }
+        }
+    }
+
+    // ---------------------------------------------------------------------------------
+    // Bessel functions, elsa/core/Utilities/Bessel.cpp:7-136 (polynomial fits of
+    // Abramowitz & Stegun 9.8.1-9.8.4, upward recurrence for orders 2-4, Miller's downward
+    // recurrence above).
+    inline double bessi0(double x)
+    {
+        const double ax = std::abs(x);
+        if (ax < 3.75) {
+            const double y = std::pow(x / 3.75, 2);
+            return 1.0
+                   + y * (3.5156229
+                          + y * (3.0899424
+                                 + y * (1.2067492 + y * (0.2659732 + y * (0.360768e-1 + y * 0.45813e-2)))));
+        }
+        const double y = 3.75 / ax;
+        const double p = 0.39894228
+                         + y * (0.1328592e-1
+                                + y * (0.225319e-2
+                                       + y * (-0.157565e-2
+                                              + y * (0.916281e-2
+                                                     + y * (-0.2057706e-1
+                                                            + y * (0.2635537e-1
+                                                                   + y * (-0.1647633e-1 + y * 0.392377e-2)))))));
+        return (std::exp(ax) / std::sqrt(ax)) * p;
+    }
+
+    inline double bessi1(double x)
+    {
+        const double ax = std::abs(x);
+        double result;
+        if (ax < 3.75) {
+            const double y = std::pow(x / 3.75, 2);
+            result = ax
+                     * (0.5
+                        + y * (0.87890594
+                               + y * (0.51498869
+                                      + y * (0.15084934 + y * (0.2658733e-1 + y * (0.301532e-2 + y * 0.32411e-3))))));
+        } else {
+            const double y = 3.75 / ax;
+            const double p = 0.39894228
+                             + y * (-0.3988024e-1
+                                    + y * (-0.362018e-2
+                                           + y * (0.163801e-2
+                                                  + y * (-0.1031555e-1
+                                                         + y * (0.2282967e-1
+                                                                + y * (-0.2895312e-1
+                                                                       + y * (0.1787654e-1 - y * 0.420059e-2)))))));
+            result = p * (std::exp(ax) / std::sqrt(ax));
+        }
+        return x < 0.0 ? -result : result;
+    }
+
+    inline double bessi2(double x) { return x == 0 ? 0 : bessi0(x) - (2.0 / x) * bessi1(x); }
+    inline double bessi3(double x) { return x == 0 ? 0 : bessi1(x) - (4.0 / x) * bessi2(x); }
+    inline double bessi4(double x) { return x == 0 ? 0 : bessi2(x) - (6.0 / x) * bessi3(x); }
+
+    inline double bessi(i64 m, double x)
+    {
+        switch (m) {
+        case 0: return bessi0(x);
+        case 1: return bessi1(x);
+        case 2: return bessi2(x);
+        case 3: return bessi3(x);
+        case 4: return bessi4(x);
+        default: break;
+        }
+        const double ACC = 40.0, BIGNO = 1.0e10, BIGNI = 1.0e-10;
+        if (x == 0.0)
+            return 0.0;
+        const double tox = 2.0 / std::abs(x);
+        double result = 0.0, bip = 0.0, bi = 1.0;
+        for (i64 j = 2 * (m + static_cast<i64>(std::sqrt(ACC * static_cast<double>(m)))); j > 0; --j) {
+            const double bim = bip + static_cast<double>(j) * tox * bi;
+            bip = bi;
+            bi = bim;
+            if (std::abs(bi) > BIGNO) {
+                result *= BIGNI;
+                bi *= BIGNI;
+                bip *= BIGNI;
+            }
+            if (j == m)
+                result = bip;
+        }
+        result *= bessi0(x) / bi;
+        return ((x < 0.0) && ((m % 2) == 0)) ? -result : result;
+    }
+
+    // elsa/projectors/Blobs.h:17-30: the Kaiser-Bessel blob profile.
+    template <class T>
+    T blob_evaluate(T r, T a, T alpha, i64 m)
+    {
+        const T w = T(1) - std::pow(r / a, T(2));
+        if (w >= 0) {
+            const T Im1 = static_cast<T>(bessi(m, alpha));
+            const T arg = std::sqrt(w);
+            const T Im2 = static_cast<T>(bessi(m, alpha * arg));
+            return (Im2 / Im1) * static_cast<T>(std::pow(arg, m));
+        }
+        return T(0);
+    }
+
+    // elsa/projectors/Blobs.h:67-98: closed-form line integral of the blob (orders 0-2).
+    template <class T>
+    T blob_projected(T s, T a, T alpha, i64 m)
+    {
+        const T sda = s / a;
+        const T sdas = std::pow(sda, 2.f);
+        const T w = 1.f - sdas;
+        if (w > 1.0e-10) {
+            const T arg = alpha * std::sqrt(w);
+            if (m == 0)
+                return (2.f * a / alpha) * std::sinh(arg) / static_cast<T>(bessi0(alpha));
+            if (m == 1)
+                return (2.f * a / alpha) * std::sqrt(w) * (std::cosh(arg) - std::sinh(arg) / arg)
+                       / static_cast<T>(bessi1(alpha));
+            if (m == 2)
+                return (2.f * a / alpha) * w
+                       * ((3.f / (arg * arg) + 1.f) * std::sinh(arg) - (3.f / arg) * std::cosh(arg))
+                       / static_cast<T>(bessi2(alpha));
+            return std::numeric_limits<T>::quiet_NaN();
+        }
+        return T(0);
+    }
+
+    inline i64 factorial(i64 n) { return (n == 1 || n == 0) ? 1 : factorial(n - 1) * n; }
+    inline i64 binom(i64 n, i64 k)
+    {
+        return (k > n) ? 0
+               : (k == 0 || k == n) ? 1
+               : (k == 1 || k == n - 1) ? n
+               : (k + k < n) ? (binom(n - 1, k - 1) * n) / k
+                             : (binom(n - 1, k) * n) / (n - k);
+    }
+
+    // elsa/projectors/BSplines.h:22-38: centred cardinal B-spline of order m. The power
+    // terms promote to double (std::pow with an integer exponent), the sum stays in T.
+    template <class T>
+    T bspline1d(T x, i64 m)
+    {
+        const T xs = x + static_cast<T>(m + 1) / T(2);
+        T res = 0;
+        for (int n = 0; n <= m + 1; ++n) {
+            const T nf = static_cast<T>(n);
+            const T d = xs - nf;
+            const T tmp1 = d == 0 ? T(0) : (d < 0 ? T(0) : T(1));
+            const double tmp2 = std::pow(static_cast<double>(d), static_cast<int>(m));
+            const T tmp3 = static_cast<T>(binom(m + 1, n));
+            const double tmp4 = std::pow(-1.0, n) / static_cast<double>(static_cast<T>(factorial(m)));
+            res += static_cast<T>(tmp1 * tmp2 * tmp3 * tmp4);
+        }
+        return res;
+    }
+
+    // elsa/projectors/BSplines.h:72-81 (nd_bspline_centered) as used by ProjectedBSpline
+    // with dim - 1 (BSplines.h:153-159).
+    template <class T>
+    T bspline_projected(T x, i64 m, i64 dim)
+    {
+        T res = bspline1d<T>(x, m);
+        for (i64 i = 1; i < dim - 1; ++i)
+            res *= bspline1d<T>(T(0), m);
+        return res;
+    }
+
+    // elsa/projectors/Luts.hpp:14-29: N samples at step*i, step = support / (N - 1).
+    template <class T, class Gen>
+    void make_lut(T support, int N, T* out, Gen gen)
+    {
+        const T step = support / static_cast<T>(N - 1);
+        for (int i = 0; i < N; ++i)
+            out[i] = gen(step * static_cast<T>(i));
+    }
+
+    // ---------------------------------------------------------------------------------
+    // Geometry, elsa/core/Geometry.cpp:12-109 (constructors) and :225-270 (buildMatrices).
+    // Matrices are column-major, as Eigen stores them.
+    template <class R>
+    struct Mat {
+        int r, c;
+        R v[16];
+        R& operator()(int i, int j) { return v[i + j * r]; }
+        R operator()(int i, int j) const { return v[i + j * r]; }
+    };
+
+    template <class R>
+    Mat<R> zeros(int r, int c)
+    {
+        Mat<R> m;
+        m.r = r;
+        m.c = c;
+        for (int i = 0; i < 16; ++i)
+            m.v[i] = R(0);
+        return m;
+    }
+
+    template <class R>
+    Mat<R> eye(int r, int c)
+    {
+        Mat<R> m = zeros<R>(r, c);
+        for (int i = 0; i < std::min(r, c); ++i)
+            m(i, i) = R(1);
+        return m;
+    }
+
+    template <class R>
+    Mat<R> mul(const Mat<R>& A, const Mat<R>& B)
+    {
+        Mat<R> C = zeros<R>(A.r, B.c);
+        for (int i = 0; i < A.r; ++i)
+            for (int j = 0; j < B.c; ++j) {
+                R acc = A(i, 0) * B(0, j);
+                for (int k = 1; k < A.c; ++k)
+                    acc = acc + A(i, k) * B(k, j);
+                C(i, j) = acc;
+            }
+        return C;
+    }
+
+    // Solve the d x d system M x = b in double with partial pivoting (the camera centre;
+    // elsa uses colPivHouseholderQr, which agrees to rounding).
+    inline void solve(int d, const double* M, const double* b, double* x)
+    {
+        double A[3][4];
+        for (int i = 0; i < d; ++i) {
+            for (int j = 0; j < d; ++j)
+                A[i][j] = M[i + j * d];
+            A[i][d] = b[i];
+        }
+        for (int k = 0; k < d; ++k) {
+            int p = k;
+            for (int i = k + 1; i < d; ++i)
+                if (std::abs(A[i][k]) > std::abs(A[p][k]))
+                    p = i;
+            for (int j = 0; j <= d; ++j)
+                std::swap(A[k][j], A[p][j]);
+            for (int i = k + 1; i < d; ++i) {
+                const double f = A[i][k] / A[k][k];
+                for (int j = k; j <= d; ++j)
+                    A[i][j] -= f * A[k][j];
+            }
+        }
+        for (int i = d - 1; i >= 0; --i) {
+            double s = A[i][d];
+            for (int j = i + 1; j < d; ++j)
+                s -= A[i][j] * x[j];
+            x[i] = s / A[i][i];
+        }
+    }
+
+    template <class R>
+    void build_matrices(int d, const Mat<R>& Rm, const Mat<R>& t, const Mat<R>& K, const Mat<R>& S,
+                        R* P_out, R* Pinv_out, R* C_out, R* ext_out)
+    {
+        Mat<R> Rt = zeros<R>(d + 1, d + 1);
+        for (int i = 0; i < d; ++i) {
+            for (int j = 0; j < d; ++j)
+                Rt(i, j) = Rm(i, j);
+            Rt(i, d) = t(i, 0);
+        }
+        Rt(d, d) = R(1);
+        const Mat<R> Id = eye<R>(d, d + 1);
+        const Mat<R> extFull = mul(Rt, S);
+        const Mat<R> P = mul(mul(K, Id), extFull);
+        // camera centre C = -P[:, :d]^-1 P[:, d]
+        double M[9], b[3], x[3];
+        for (int i = 0; i < d; ++i) {
+            for (int j = 0; j < d; ++j)
+                M[i + j * d] = static_cast<double>(P(i, j));
+            b[i] = static_cast<double>(P(i, d));
+        }
+        solve(d, M, b, x);
+        // Pinv = S^-1 [R^T | -R^T t] [I; 0] K^-1
+        Mat<R> Sinv = zeros<R>(d + 1, d + 1);
+        for (int i = 0; i <= d; ++i)
+            Sinv(i, i) = R(1) / S(i, i);
+        Mat<R> Kinv = eye<R>(d, d);
+        Kinv(0, 0) = R(1) / K(0, 0);
+        Kinv(0, d - 1) = -K(0, d - 1) / K(0, 0);
+        if (d == 3) {
+            Kinv(1, 1) = R(1) / K(1, 1);
+            Kinv(1, d - 1) = -K(1, d - 1) / K(1, 1);
+        }
+        Mat<R> Rtinv = zeros<R>(d + 1, d + 1);
+        Mat<R> mRT = zeros<R>(d, d);
+        for (int i = 0; i < d; ++i)
+            for (int j = 0; j < d; ++j) {
+                Rtinv(i, j) = Rm(j, i);
+                mRT(i, j) = -Rm(j, i);
+            }
+        const Mat<R> mRTt = mul(mRT, t);
+        for (int i = 0; i < d; ++i)
+            Rtinv(i, d) = mRTt(i, 0);
+        Rtinv(d, d) = R(1);
+        const Mat<R> Idinv = eye<R>(d + 1, d);
+        const Mat<R> Pinv = mul(mul(mul(Sinv, Rtinv), Idinv), Kinv);
+        for (int i = 0; i < d * (d + 1); ++i) {
+            P_out[i] = P.v[i];
+            ext_out[i] = extFull(i % d, i / d);
+            Pinv_out[i] = Pinv.v[i];
+        }
+        for (int i = 0; i < d; ++i)
+            C_out[i] = static_cast<R>(-x[i]);
+    }
+
+    // elsa/core/Geometry.cpp:12-51 (2D) with buildMatrices.
+    template <class R>
+    void geometry_2d(R s2c, R c2d, R angle, const R* vsp, const R* vorg, R ssp, R sorg, R ppo,
+                     const R* cor, R* P, R* Pinv, R* C, R* ext, R* sdd)
+    {
+        Mat<R> Rm = zeros<R>(2, 2);
+        const R c = std::cos(angle), s = std::sin(angle);
+        Rm(0, 0) = c;
+        Rm(0, 1) = -s;
+        Rm(1, 0) = s;
+        Rm(1, 1) = c;
+        Mat<R> S = eye<R>(3, 3);
+        S(0, 0) = vsp[0];
+        S(1, 1) = vsp[1];
+        Mat<R> v = zeros<R>(2, 1);
+        v(0, 0) = -cor[0] - vorg[0];
+        v(1, 0) = -cor[1] - vorg[1];
+        Mat<R> t = mul(Rm, v);
+        t(1, 0) += s2c;
+        Mat<R> K = eye<R>(2, 2);
+        K(0, 0) = (s2c + c2d) / ssp;
+        K(0, 1) = sorg / ssp + ppo;
+        build_matrices<R>(2, Rm, t, K, S, P, Pinv, C, ext);
+        *sdd = s2c + c2d;
+    }
+
+    // elsa/core/Geometry.cpp:53-109 (3D, YZY Euler angles) with buildMatrices.
+    template <class R>
+    void geometry_3d(R s2c, R c2d, const R* vsp, const R* vorg, const R* ssp, const R* sorg,
+                     R gamma, R beta, R alpha, const R* ppo, const R* cor, R* P, R* Pinv, R* C,
+                     R* ext, R* sdd)
+    {
+        const R ca = std::cos(alpha), sa = std::sin(alpha);
+        const R cb = std::cos(beta), sb = std::sin(beta);
+        const R cg = std::cos(gamma), sg = std::sin(gamma);
+        Mat<R> A = zeros<R>(3, 3), B = zeros<R>(3, 3), G = zeros<R>(3, 3);
+        G(0, 0) = cg; G(0, 2) = sg; G(1, 1) = 1; G(2, 0) = -sg; G(2, 2) = cg;
+        B(0, 0) = cb; B(0, 1) = -sb; B(1, 0) = sb; B(1, 1) = cb; B(2, 2) = 1;
+        A(0, 0) = ca; A(0, 2) = sa; A(1, 1) = 1; A(2, 0) = -sa; A(2, 2) = ca;
+        const Mat<R> Rm = mul(mul(G, B), A);
+        Mat<R> S = eye<R>(4, 4);
+        for (int i = 0; i < 3; ++i)
+            S(i, i) = vsp[i];
+        Mat<R> v = zeros<R>(3, 1);
+        for (int i = 0; i < 3; ++i)
+            v(i, 0) = -cor[i] - vorg[i];
+        Mat<R> t = mul(Rm, v);
+        t(2, 0) += s2c;
+        Mat<R> K = eye<R>(3, 3);
+        K(0, 0) = (s2c + c2d) / ssp[0];
+        K(0, 2) = sorg[0] / ssp[0] + ppo[0];
+        K(1, 1) = (s2c + c2d) / ssp[1];
+        K(1, 2) = sorg[1] / ssp[1] + ppo[1];
+        build_matrices<R>(3, Rm, t, K, S, P, Pinv, C, ext);
+        *sdd = s2c + c2d;
+    }
+
+    template <class R, class T>
+    int dispatch_projector(int kind, int dim, int adjoint, int mode, const i64* vol, const i64* det,
+                           i64 nviews, const R* Pinv, const R* C, const T* in, T* out)
+    {
+        if (dim != 2 && dim != 3)
+            return -1;
+        const Shape sh = make_shape(dim, vol, det, nviews);
+        if (kind == 0) {
+            if (dim == 2)
+                adjoint ? joseph_all<R, T, 2, true>(sh, Pinv, C, mode, in, out)
+                        : joseph_all<R, T, 2, false>(sh, Pinv, C, mode, in, out);
+            else
+                adjoint ? joseph_all<R, T, 3, true>(sh, Pinv, C, mode, in, out)
+                        : joseph_all<R, T, 3, false>(sh, Pinv, C, mode, in, out);
+        } else {
+            if (dim == 2)
+                adjoint ? siddon_all<R, T, 2, true>(sh, Pinv, C, in, out)
+                        : siddon_all<R, T, 2, false>(sh, Pinv, C, in, out);
+            else
+                adjoint ? siddon_all<R, T, 3, true>(sh, Pinv, C, in, out)
+                        : siddon_all<R, T, 3, false>(sh, Pinv, C, in, out);
+        }
+        return 0;
+    }
+
+    template <class R, class T>
+    int dispatch_voxel(int dim, int adjoint, const i64* vol, const i64* det, i64 nviews,
+                       const R* P, const R* ext, const R* sdd, const T* lut, int N, R support,
+                       const T* in, T* out)
+    {
+        if (dim != 2 && dim != 3)
+            return -1;
+        const Shape sh = make_shape(dim, vol, det, nviews);
+        if (dim == 2)
+            adjoint ? voxel_all<R, T, 2, true>(sh, P, ext, sdd, lut, N, support, in, out)
+                    : voxel_all<R, T, 2, false>(sh, P, ext, sdd, lut, N, support, in, out);
+        else
+            adjoint ? voxel_all<R, T, 3, true>(sh, P, ext, sdd, lut, N, support, in, out)
+                    : voxel_all<R, T, 3, false>(sh, P, ext, sdd, lut, N, support, in, out);
+        return 0;
+    }
+
+    template <class R>
+    void joseph_rays(int dim, const i64* vol, const i64* det, i64 nviews, const R* Pinv,
+                     const R* C, int32_t* axis, int64_t* nsteps, R* pos0, R* step, R* len)
+    {
+        const Shape sh = make_shape(dim, vol, det, nviews);
+#pragma omp parallel for schedule(static)
+        for (i64 ir = 0; ir < sh.nrays; ++ir) {
+            R o[3], d[3];
+            JRay<R> jr;
+            if (dim == 2) {
+                ray_of_index<R, 2>(sh, Pinv, C, ir, o, d);
+                joseph_setup<R, 2>(o, d, sh.vol, jr);
+            } else {
+                ray_of_index<R, 3>(sh, Pinv, C, ir, o, d);
+                joseph_setup<R, 3>(o, d, sh.vol, jr);
+            }
+            if (axis)
+                axis[ir] = jr.axis;
+            if (nsteps)
+                nsteps[ir] = jr.nsteps;
+            for (int i = 0; i < 3; ++i) {
+                if (pos0)
+                    pos0[ir * 3 + i] = jr.pos0[i];
+                if (step)
+                    step[ir * 3 + i] = jr.step[i];
+            }
+            if (len)
+                len[ir] = jr.len;
+        }
+    }
+} // namespace orc
+
+extern "C" {
+
+// Projectors. kind: 0 = Joseph, 1 = Siddon. mode (Joseph only): 0 = elsa's repeated
+// addition, 1 = pos0 + s*step. Pinv: nviews blocks of (dim+1) x dim, C: nviews x dim.
+// Volume x fastest; sinogram [u][v][view] with u fastest. Returns 0 on success.
+int orc_xray_f32(int kind, int dim, int adjoint, int mode, const int64_t* vol, const int64_t* det,
+                 int64_t nviews, const float* Pinv, const float* C, const float* in, float* out,
+                 int nthreads)
+{
+    orc::set_threads(nthreads);
+    return orc::dispatch_projector<float, float>(kind, dim, adjoint, mode, vol, det, nviews, Pinv,
+                                                 C, in, out);
+}
+
+int orc_xray_f64(int kind, int dim, int adjoint, int mode, const int64_t* vol, const int64_t* det,
+                 int64_t nviews, const double* Pinv, const double* C, const double* in,
+                 double* out, int nthreads)
+{
+    orc::set_threads(nthreads);
+    return orc::dispatch_projector<double, double>(kind, dim, adjoint, mode, vol, det, nviews,
+                                                   Pinv, C, in, out);
+}
+
+// Voxel projector (blob or B-spline LUT). P, ext: nviews blocks of dim x (dim+1).
+int orc_voxel_f32(int dim, int adjoint, const int64_t* vol, const int64_t* det, int64_t nviews,
+                  const float* P, const float* ext, const float* sdd, const float* lut, int N,
+                  float support, const float* in, float* out, int nthreads)
+{
+    orc::set_threads(nthreads);
+    return orc::dispatch_voxel<float, float>(dim, adjoint, vol, det, nviews, P, ext, sdd, lut, N,
+                                             support, in, out);
+}
+
+int orc_voxel_f64(int dim, int adjoint, const int64_t* vol, const int64_t* det, int64_t nviews,
+                  const double* P, const double* ext, const double* sdd, const double* lut, int N,
+                  double support, const double* in, double* out, int nthreads)
+{
+    orc::set_threads(nthreads);
+    return orc::dispatch_voxel<double, double>(dim, adjoint, vol, det, nviews, P, ext, sdd, lut,
+                                               N, support, in, out);
+}
+
+// Per-ray Joseph traversal state (driving axis, number of samples, snapped start,
+// step, step length); any output pointer may be null.
+void orc_joseph_rays_f32(int dim, const int64_t* vol, const int64_t* det, int64_t nviews,
+                         const float* Pinv, const float* C, int32_t* axis, int64_t* nsteps,
+                         float* pos0, float* step, float* len)
+{
+    orc::joseph_rays<float>(dim, vol, det, nviews, Pinv, C, axis, nsteps, pos0, step, len);
+}
+
+void orc_joseph_rays_f64(int dim, const int64_t* vol, const int64_t* det, int64_t nviews,
+                         const double* Pinv, const double* C, int32_t* axis, int64_t* nsteps,
+                         double* pos0, double* step, double* len)
+{
+    orc::joseph_rays<double>(dim, vol, det, nviews, Pinv, C, axis, nsteps, pos0, step, len);
+}
+
+// Ray (origin, unit direction) through pixel (u, v) for one pose.
+void orc_make_ray_f32(int dim, const float* Pinv, const float* C, int64_t u, int64_t v, float* o,
+                      float* d)
+{
+    dim == 2 ? orc::make_ray<float, 2>(Pinv, C, u, v, o, d)
+             : orc::make_ray<float, 3>(Pinv, C, u, v, o, d);
+}
+
+int orc_intersect_f32(int dim, const float* o, const float* d, const int64_t* vol, float* tmin,
+                      float* tmax)
+{
+    return dim == 2 ? orc::intersect_ray<float, 2>(o, d, vol, *tmin, *tmax)
+                    : orc::intersect_ray<float, 3>(o, d, vol, *tmin, *tmax);
+}
+
+// Geometry of one pose (elsa/core/Geometry.cpp). Outputs column-major P (dim x dim+1),
+// Pinv (dim+1 x dim), C (dim), ext (dim x dim+1) and the source-detector distance.
+void orc_geometry2d_f32(float s2c, float c2d, float angle, const float* vsp, const float* vorg,
+                        float ssp, float sorg, float ppo, const float* cor, float* P, float* Pinv,
+                        float* C, float* ext, float* sdd)
+{
+    orc::geometry_2d<float>(s2c, c2d, angle, vsp, vorg, ssp, sorg, ppo, cor, P, Pinv, C, ext, sdd);
+}
+
+void orc_geometry2d_f64(double s2c, double c2d, double angle, const double* vsp,
+                        const double* vorg, double ssp, double sorg, double ppo, const double* cor,
+                        double* P, double* Pinv, double* C, double* ext, double* sdd)
+{
+    orc::geometry_2d<double>(s2c, c2d, angle, vsp, vorg, ssp, sorg, ppo, cor, P, Pinv, C, ext,
+                             sdd);
+}
+
+void orc_geometry3d_f32(float s2c, float c2d, const float* vsp, const float* vorg,
+                        const float* ssp, const float* sorg, float gamma, float beta, float alpha,
+                        const float* ppo, const float* cor, float* P, float* Pinv, float* C,
+                        float* ext, float* sdd)
+{
+    orc::geometry_3d<float>(s2c, c2d, vsp, vorg, ssp, sorg, gamma, beta, alpha, ppo, cor, P, Pinv,
+                            C, ext, sdd);
+}
+
+void orc_geometry3d_f64(double s2c, double c2d, const double* vsp, const double* vorg,
+                        const double* ssp, const double* sorg, double gamma, double beta,
+                        double alpha, const double* ppo, const double* cor, double* P,
+                        double* Pinv, double* C, double* ext, double* sdd)
+{
+    orc::geometry_3d<double>(s2c, c2d, vsp, vorg, ssp, sorg, gamma, beta, alpha, ppo, cor, P,
+                             Pinv, C, ext, sdd);
+}
+
+double orc_bessi(int64_t m, double x) { return orc::bessi(m, x); }
+
+double orc_blob_evaluate_f64(double r, double a, double alpha, int64_t m)
+{
+    return orc::blob_evaluate<double>(r, a, alpha, m);
+}
+
+double orc_blob_projected_f64(double s, double a, double alpha, int64_t m)
+{
+    return orc::blob_projected<double>(s, a, alpha, m);
+}
+
+// LUTs of elsa's ProjectedBlob / ProjectedBSpline (N samples over [0, support]).
+void orc_blob_lut_f32(float radius, float alpha, int64_t order, int N, float* out)
+{
+    orc::make_lut<float>(radius, N, out, [&](float s) {
+        return orc::blob_projected<float>(s, radius, alpha, order);
+    });
+}
+
+void orc_blob_lut_f64(double radius, double alpha, int64_t order, int N, double* out)
+{
+    orc::make_lut<double>(radius, N, out, [&](double s) {
+        return orc::blob_projected<double>(s, radius, alpha, order);
+    });
+}
+
+void orc_bspline_lut_f32(int64_t dim, int64_t order, int N, float* out)
+{
+    const float support = static_cast<float>((order + 1) * 0.5);
+    orc::make_lut<float>(support, N, out,
+                         [&](float s) { return orc::bspline_projected<float>(s, order, dim); });
+}
+
+void orc_bspline_lut_f64(int64_t dim, int64_t order, int N, double* out)
+{
+    const double support = (order + 1) * 0.5;
+    orc::make_lut<double>(support, N, out,
+                          [&](double s) { return orc::bspline_projected<double>(s, order, dim); });
+}
+
+// Lut::operator() of elsa/projectors/Luts.hpp:60-78 with a double distance.
+double orc_lut_eval_f64(const double* lut, int N, double support, double distance)
+{
+    return orc::lut_eval<double, double>(lut, N, support, distance);
+}
+
+float orc_lut_eval_f32(const float* lut, int N, float support, float distance)
+{
+    return orc::lut_eval<float, float>(lut, N, support, distance);
+}
+
+} // extern "C"

@@ -1,0 +1,63 @@
+"""Extract the golden vectors held by elsa's own tests into tests/golden/*.json.
+
+Run once in a container that has the reference tree (REF defaults to /root/reference):
+    python tests/golden/extract_reference_vectors.py
+It copies numbers only (expected outputs of the reference tests), never source text:
+  luts.json    elsa/projectors/tests/test_Luts.cpp  ProjectedBlobLut (a=2, alpha=10.83, m=2,
+               101 entries) and ProjectedBSplineLut (order 3, dim 2, 101 entries)
+  bessel.json  elsa/core/tests/test_Bessel.cpp  I_0..I_4 on LinSpaced(size, 0, 4)
+"""
+import json
+import os
+import re
+import sys
+
+REF = os.environ.get("REF", "/root/reference")
+OUT = os.path.dirname(os.path.abspath(__file__))
+NUM = r"[-+]?(?:\d+\.\d*|\.\d+|\d+)(?:[eE][-+]?\d+)?"
+
+
+def numbers(block: str):
+    return [float(x) for x in re.findall(NUM, block)]
+
+
+def brace_block(text: str, start: int) -> str:
+    i = text.index("{", start)
+    depth = 0
+    for j in range(i, len(text)):
+        if text[j] == "{":
+            depth += 1
+        elif text[j] == "}":
+            depth -= 1
+            if depth == 0:
+                return text[i + 1:j]
+    raise ValueError("unbalanced braces")
+
+
+def main():
+    luts_src = open(os.path.join(REF, "elsa/projectors/tests/test_Luts.cpp")).read()
+    blob_at = luts_src.index('TEST_CASE_TEMPLATE("ProjectedBlobLut: "')
+    blob = numbers(brace_block(luts_src, luts_src.index("expected", blob_at)))
+    bsp_at = luts_src.index('TEST_CASE_TEMPLATE("ProjectedBSplineLut: "')
+    bsp = numbers(brace_block(luts_src, luts_src.index("expected", bsp_at)))
+    assert len(blob) == 101 and len(bsp) == 101, (len(blob), len(bsp))
+    json.dump({
+        "source": "elsa/projectors/tests/test_Luts.cpp",
+        "projected_blob": {"radius": 2.0, "alpha": 10.83, "order": 2, "N": 101, "distance": "i/50",
+                           "checked": "i < 99", "epsilon": 1e-6, "expected": blob},
+        "projected_bspline": {"dim": 2, "order": 3, "N": 101, "distance": "i/50", "checked": "i < 101",
+                              "epsilon": 1e-6, "expected": bsp},
+    }, open(os.path.join(OUT, "luts.json"), "w"), indent=1)
+
+    bes_src = open(os.path.join(REF, "elsa/core/tests/test_Bessel.cpp")).read()
+    tables = {}
+    for m in range(5):
+        at = bes_src.index(f"std::vector<T> get_bessi{m}()\n{{")
+        tables[f"I{m}"] = numbers(brace_block(bes_src, bes_src.index("return", at)))
+    json.dump({"source": "elsa/core/tests/test_Bessel.cpp", "x": "LinSpaced(size, 0, 4)", **tables},
+              open(os.path.join(OUT, "bessel.json"), "w"), indent=1)
+    print("wrote", os.listdir(OUT))
+
+
+if __name__ == "__main__":
+    sys.exit(main())

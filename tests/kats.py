@@ -1,0 +1,409 @@
+"""Known-answer tests transcribed from elsa's projector tests, run against any back end.
+
+Sources (paths relative to the elsa tree):
+  Joseph  elsa/projectors/tests/test_JosephsMethod.cpp
+  Siddon  elsa/projectors/tests/test_SiddonsMethod.cpp and
+          elsa/projectors_cuda/tests/test_SiddonsMethodCUDA.cpp
+  voxel   elsa/projectors_cuda/tests/test_VoxelProjectorCUDA.cpp
+Volumes are indexed vol[z, y, x] here (elsa's volume(x, y, z) with x fastest).
+Tolerances are those of the reference tests: doctest Approx (relative 100*FLT_EPSILON on
+scale 1), isApprox (||a-b|| <= prec*min(||a||,||b||), prec 1e-5 or the test's epsilon 0.01)
+and isCwiseApprox (elementwise relative 0.01).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+PI = np.float32(np.pi)
+EPS = 0.01  # elsa test_routines/testHelpers.h:20
+
+
+def approx(a, b, eps=100 * np.finfo(np.float32).eps):
+    """doctest::Approx(b).epsilon(eps) == a."""
+    a, b = float(a), float(b)
+    return abs(a - b) < eps * (1.0 + max(abs(a), abs(b)))
+
+
+def is_approx(x, y, prec=1e-5):
+    """elsa isApprox (testHelpers.cpp:20-34)."""
+    x = np.asarray(x, np.float64).ravel()
+    y = np.asarray(y, np.float64).ravel()
+    return np.linalg.norm(x - y) <= prec * min(np.linalg.norm(x), np.linalg.norm(y))
+
+
+def is_zero(x, eps=EPS):
+    """isApprox against an all-zero container: holds only for an exactly zero result."""
+    return bool(np.all(np.asarray(x) == 0))
+
+
+def cwise_approx(x, y, eps=EPS):
+    x = np.asarray(x, np.float64).ravel()
+    y = np.asarray(y, np.float64).ravel()
+    return all(approx(a, b, eps) for a, b in zip(x, y))
+
+
+def pose(s2c, c2d, gamma=0.0, beta=0.0, alpha=0.0, ppo=None, cor=None):
+    return {"s2c": float(s2c), "c2d": float(c2d), "gamma": float(gamma), "beta": float(beta), "alpha": float(alpha),
+            "ppo": ppo, "cor": cor}
+
+
+def deg(d):
+    """Degree{d}.to_radian() in float: d * pi_t / 180."""
+    return float(np.float32(np.float32(np.float32(d) * PI) / np.float32(180)))
+
+
+# --------------------------------------------------------------------------------------
+# Joseph, test_JosephsMethod.cpp
+
+
+def joseph_single_ray_2d(B):
+    """test_JosephsMethod.cpp:17-202: 5x5 ones, one ray, s2c 100, c2d 5."""
+    x = np.ones((5, 5))
+    col = np.zeros((5, 5)); col[:, 2] = 5
+    row = np.zeros((5, 5)); row[2, :] = 5
+    diag = np.diag(np.full(5, 10.0))
+    for angle, expect, cmp, tol in [(0.0, 5, col, None), (deg(180), 5, col, 1e-5), (deg(90), 5, row, 1e-5),
+                                    (deg(270), 5, row, 1e-5), (deg(45), None, diag, EPS), (deg(225), None, diag, EPS)]:
+        g = B.setup((5, 5), (1,), [pose(100, 5, angle)])
+        y = B.joseph(g, x)
+        if expect is not None:
+            assert approx(y.ravel()[0], expect), (angle, y)
+        back = B.joseph(g, y, adjoint=True)
+        assert (cwise_approx(back, cmp) if tol is None else is_approx(back, cmp, tol)), (angle, back)
+
+
+def joseph_four_rays_2d(B):
+    """test_JosephsMethod.cpp:205-265: poses 0, 90, 180, 270 degrees."""
+    g = B.setup((5, 5), (1,), [pose(100, 5, deg(a)) for a in (0, 90, 180, 270)])
+    y = B.joseph(g, np.ones((5, 5)))
+    back = B.joseph(g, y, adjoint=True)
+    cmp = np.array([0, 0, 10, 0, 0, 0, 0, 10, 0, 0, 10, 10, 20, 10, 10, 0, 0, 10, 0, 0, 0, 0, 10, 0, 0],
+                   np.float64).reshape(5, 5)
+    assert is_approx(back, cmp), back
+
+
+def joseph_rays_missing(B):
+    """test_JosephsMethod.cpp:438-640: rays beside the volume give zero both ways."""
+    n = 5
+    cases2d = [(0.0, (n, 0)), (0.0, (-n, 0)), (float(PI / 2), (0, n)), (float(PI / 2), (0, -n))]
+    for ang, cor in cases2d:
+        g = B.setup((n, n), (1,), [pose(20 * n, n, ang, cor=cor)])
+        assert is_zero(B.joseph(g, np.ones((n, n)))), (ang, cor)
+        assert is_zero(B.joseph(g, np.ones((1, 1)), adjoint=True)), (ang, cor)
+    h = float(PI / 2)
+    beta = [0, 0, 0, 0, 0, 0, h, h, h]
+    gamma = [0, 0, 0, h, h, h, h, h, h]
+    ox = [n, 0, n, 0, 0, 0, n, 0, n]
+    oy = [0, n, n, n, 0, n, 0, 0, 0]
+    oz = [0, 0, 0, 0, n, n, 0, n, n]
+    for i in range(9):
+        g = B.setup((n, n, n), (1, 1), [pose(20 * n, n, gamma[i], beta[i], 0.0, ppo=(0, 0),
+                                              cor=(-ox[i], -oy[i], -oz[i]))])
+        assert is_zero(B.joseph(g, np.ones((n, n, n)))), i
+        assert is_zero(B.joseph(g, np.ones((1, 1, 1)), adjoint=True)), i
+
+
+def joseph_axis_aligned_2d(B):
+    """test_JosephsMethod.cpp:642-828: axis-aligned rays through and beside pixel centres."""
+    n = 5
+    angles = [0.0, float(PI / 2), float(PI), float(3 * PI / 2)]
+    bp = [np.zeros((n, n)), np.zeros((n, n))]
+    bp[0][:, 2] = 1
+    bp[1][2, :] = 1
+    for i, a in enumerate(angles):
+        g = B.setup((n, n), (1,), [pose(20 * n, n, a)])
+        for j in range(n):
+            v = np.zeros((n, n))
+            if i % 2 == 0:
+                v[j, n // 2] = 1
+            else:
+                v[n // 2, j] = 1
+            y = B.joseph(g, v)
+            assert approx(y.ravel()[0], 1), (i, j, y)
+        assert is_approx(B.joseph(g, y, adjoint=True), bp[i % 2]), i
+    offx = [0.25, 0.0, 0.25, 0.0]
+    offy = [0.0, 0.25, 0.0, 0.25]
+    bp[0] = np.array([0, 0.25, 0.75, 0, 0] * 5, np.float64).reshape(n, n)
+    bp[1] = np.array([0] * 5 + [0.25] * 5 + [0.75] * 5 + [0] * 10, np.float64).reshape(n, n)
+    for i, a in enumerate(angles):
+        g = B.setup((n, n), (1,), [pose(n * 2000, n, a, ppo=0.0, cor=(-offx[i], -offy[i]))])
+        for j in range(n):
+            v = np.zeros((n, n))
+            if i % 2 == 0:
+                v[j, n // 2] = 1
+            else:
+                v[n // 2, j] = 1
+            assert approx(B.joseph(g, v).ravel()[0], 0.75), (i, j)
+        back = B.joseph(g, np.ones((1, 1)), adjoint=True)
+        assert is_approx(back, bp[i % 2]), (i, back)
+    # rays along the right / left volume boundary
+    right = np.zeros((n, n)); right[:, n - 1] = 0.5
+    left = np.zeros((n, n)); left[:, 0] = 0.5
+    for cor, col, expect in [((n * 0.5, 0), n - 1, right), ((-n / 2.0, 0), 0, left)]:
+        g = B.setup((n, n), (1,), [pose(n * 2000, n, 0.0, ppo=0.0, cor=cor)])
+        for j in range(n):
+            v = np.zeros((n, n)); v[j, col] = 1
+            assert approx(B.joseph(g, v).ravel()[0], 0.5), (cor, j)
+        assert is_approx(B.joseph(g, np.ones((1, 1)), adjoint=True), expect), cor
+
+
+def joseph_non_square_2d(B):
+    """test_JosephsMethod.cpp:830-883: 5x2 volume, ray through the short dimension."""
+    g = B.setup((5, 2), (1,), [pose(100, 5, 0.0, ppo=0.0, cor=(1.6, 0))])
+    v = np.zeros((2, 5)); v[0, 4] = 1; v[1, 4] = 1
+    assert approx(B.joseph(g, v).ravel()[0], 1.2)
+    exp = np.zeros((2, 5)); exp[:, 4] = 0.6; exp[:, 3] = 0.4
+    assert is_approx(B.joseph(g, np.ones((1, 1)), adjoint=True), exp)
+
+
+def joseph_single_ray_3d(B):
+    """test_JosephsMethod.cpp:885-1090: axis-aligned rays in a 3^3 volume, centred and offset."""
+    n = 3
+    h = float(PI / 2)
+    beta = [0.0, 0.0, 0.0, 0.0, h, float(3 * PI / 2)]
+    gamma = [0.0, float(PI), h, float(3 * PI / 2), h, float(3 * PI / 2)]
+
+    def placed(i, j):
+        v = np.zeros((n, n, n))
+        if i // 2 == 0:
+            v[j, 1, 1] = 1      # volume(1, 1, j)
+        elif i // 2 == 1:
+            v[1, 1, j] = 1      # volume(j, 1, 1)
+        else:
+            v[1, j, 1] = 1      # volume(1, j, 1)
+        return v
+
+    bp = [np.zeros(27) for _ in range(3)]
+    bp[2][[10, 13, 16]] = 1
+    bp[1][[12, 13, 14]] = 1
+    bp[0][[4, 13, 22]] = 1
+    for i in range(6):
+        g = B.setup((n, n, n), (1, 1), [pose(20 * n, n, gamma[i], beta[i])])
+        for j in range(n):
+            y = B.joseph(g, placed(i, j))
+            assert approx(y.ravel()[0], 1.0), (i, j, y)
+        assert is_approx(B.joseph(g, y, adjoint=True), bp[i // 2].reshape(n, n, n)), i
+    ox = [0.25, 0.25, 0.0, 0.0, 0.0, 0.0]
+    oy = [0.0, 0.0, 0.25, 0.25, 0.0, 0.0]
+    oz = [0.0, 0.0, 0.0, 0.0, 0.25, 0.25]
+    bp[2] = np.array([0, .25, 0, 0, .25, 0, 0, .25, 0, 0, .75, 0, 0, .75, 0, 0, .75, 0] + [0] * 9, np.float64)
+    bp[1] = np.array([0] * 9 + [.25, .25, .25, .75, .75, .75, 0, 0, 0] + [0] * 9, np.float64)
+    bp[0] = np.array([0, 0, 0, .25, .75, 0, 0, 0, 0] * 3, np.float64)
+    for i in range(6):
+        g = B.setup((n, n, n), (1, 1), [pose(n * 2000, n, gamma[i], beta[i], ppo=(0, 0), cor=(-ox[i], -oy[i], -oz[i]))])
+        for j in range(n):
+            assert approx(B.joseph(g, placed(i, j)).ravel()[0], 0.75), (i, j)
+        assert is_approx(B.joseph(g, np.ones((1, 1, 1)), adjoint=True), bp[i // 2].reshape(n, n, n)), i
+
+
+def joseph_angle_30_2d(B):
+    """test_JosephsMethod.cpp:1296-1391: -30 degree ray through the centre and through a corner."""
+    n = 4
+    g = B.setup((n, n), (1,), [pose(20 * n, n, float(-PI / 6))])
+    w = float(np.float32(2 / np.sqrt(np.float32(3))))
+    v = np.ones((n, n))
+    for (xx, yy) in [(3, 0), (1, 1), (1, 2), (1, 3), (2, 0), (2, 1), (2, 2), (0, 3)]:
+        v[yy, xx] = 0
+    assert is_zero(B.joseph(g, v))
+    for (xx, yy) in [(3, 0), (1, 1), (1, 2), (1, 3)]:
+        v[yy, xx] = 1
+    assert approx(B.joseph(g, v).ravel()[0], 2 * w)
+    s3 = math.sqrt(3.0)
+    exp = np.array([0, 0, (3 - s3) / 2, (s3 - 1) / 2, 0, (s3 - 1) / (2 * s3), (s3 + 1) / (2 * s3), 0, 0,
+                    (s3 + 1) / (2 * s3), (s3 - 1) / (2 * s3), 0, (s3 - 1) / 2, (3 - s3) / 2, 0, 0]) * w
+    assert is_approx(B.joseph(g, np.ones((1, 1)), adjoint=True), exp.reshape(n, n))
+
+
+def joseph_corner_ray_2d(B):
+    """test_JosephsMethod.cpp:1353-1391: -30 degree ray that only clips pixel (0, 0). The ray
+    leaves the volume exactly on the plane y = 0.5 of the one sample, so whether that sample
+    exists (numSteps = ceil(|exit_y - 0.5|)) hangs on the last bit of the camera centre; elsa
+    computes that with a float QR solve, this build with a double solve rounded to float, and
+    in FP32 the exit lands exactly on 0.5 (no sample). FP64 takes the reference's branch."""
+    n = 4
+    g = B.setup((n, n), (1,), [pose(20 * n, n, float(-PI / 6), ppo=0.0, cor=(-2 - math.sqrt(3.0) / 2, 0))])
+    v = np.ones((n, n)); v[0, 0] = 0
+    assert is_zero(B.joseph(g, v))
+    v[0, 0] = 1
+    assert approx(B.joseph(g, v).ravel()[0], 1 / math.sqrt(3.0), EPS)
+    exp = np.zeros(16); exp[0] = 1 / math.sqrt(3.0)
+    assert is_approx(B.joseph(g, np.ones((1, 1)), adjoint=True), exp.reshape(n, n))
+
+
+JOSEPH_KATS = [joseph_single_ray_2d, joseph_four_rays_2d, joseph_rays_missing, joseph_axis_aligned_2d,
+               joseph_non_square_2d, joseph_single_ray_3d, joseph_angle_30_2d]
+# cases whose expected value depends on an exact floating-point tie (see the docstring)
+JOSEPH_KATS_FP64_ONLY = [joseph_corner_ray_2d]
+
+
+# --------------------------------------------------------------------------------------
+# Siddon, elsa/projectors_cuda/tests/test_SiddonsMethodCUDA.cpp (shared CPU/GPU KATs)
+
+
+def siddon_axis_aligned_2d(B):
+    """test_SiddonsMethodCUDA.cpp:284-447."""
+    n = 5
+    bp = [np.zeros((n, n)), np.zeros((n, n))]
+    bp[0][:, 2] = 1
+    bp[1][2, :] = 1
+    for i, a in enumerate((0, 90, 180, 270)):
+        g = B.setup((n, n), (1,), [pose(20 * n, n, deg(a))])
+        for j in range(n):
+            v = np.zeros((n, n))
+            if i % 2 == 0:
+                v[j, n // 2] = 1
+            else:
+                v[n // 2, j] = 1
+            y = B.siddon(g, v)
+            assert approx(y.ravel()[0], 1), (a, j, y)
+        assert is_approx(B.siddon(g, y, adjoint=True), bp[i % 2]), a
+    # along the left voxel boundary of the centre column: the higher index wins
+    g = B.setup((n, n), (1,), [pose(n * 2000, n, 0.0, ppo=0.0, cor=(-0.5, 0))])
+    for j in range(n):
+        v = np.zeros((n, n)); v[j, n // 2] = 1
+        assert approx(B.siddon(g, v).ravel()[0], 1.0), j
+    assert is_approx(B.siddon(g, np.ones((1, 1)), adjoint=True), bp[0])
+    # along the right volume boundary: ignored
+    g = B.setup((n, n), (1,), [pose(n * 2000, n, 0.0, ppo=0.0, cor=(n * 0.5, 0))])
+    assert approx(B.siddon(g, np.ones((n, n))).ravel()[0], 0)
+    assert is_zero(B.siddon(g, np.ones((1, 1)), adjoint=True))
+    # along the left volume boundary: the first column
+    g = B.setup((n, n), (1,), [pose(n * 2000, n, 0.0, ppo=0.0, cor=(-n * 0.5, 0))])
+    left = np.zeros((n, n)); left[:, 0] = 1
+    for j in range(n):
+        v = np.zeros((n, n)); v[j, 0] = 1
+        assert approx(B.siddon(g, v).ravel()[0], 1), j
+    assert is_approx(B.siddon(g, np.ones((1, 1)), adjoint=True), left)
+
+
+def siddon_angle_30_2d(B):
+    """test_SiddonsMethodCUDA.cpp:478-597: -30 degrees through the centre and entering through
+    the right border (exact intersection lengths)."""
+    n = 4
+    s3 = math.sqrt(3.0)
+    g = B.setup((n, n), (1,), [pose(20 * n, n, deg(-30))])
+    v = np.zeros((n, n))
+    v[0, 3] = 1; v[0, 2] = 2; v[1, 2] = 3
+    assert approx(B.siddon(g, v).ravel()[0], 2 * s3 + 2)
+    v = np.zeros((n, n))
+    v[2, 1] = 3; v[3, 1] = 2; v[3, 0] = 1
+    assert approx(B.siddon(g, v).ravel()[0], 2 * s3 + 2)
+    exp = np.array([0, 0, 2 - 2 / s3, 4 / s3 - 2, 0, 0, 2 / s3, 0, 0, 2 / s3, 0, 0, 4 / s3 - 2, 2 - 2 / s3, 0, 0])
+    assert is_approx(B.siddon(g, np.ones((1, 1)), adjoint=True), exp.reshape(n, n), EPS)
+    g = B.setup((n, n), (1,), [pose(20 * n, n, deg(-30), ppo=0.0, cor=(float(np.sqrt(np.float32(3))), 0))])
+    v = np.ones((n, n))
+    for (xx, yy) in [(3, 1), (3, 2), (3, 3), (2, 3)]:
+        v[yy, xx] = 0
+    assert is_zero(B.siddon(g, v))
+    for (xx, yy), val in zip([(3, 1), (3, 2), (3, 3), (2, 3)], (4, 3, 2, 1)):
+        v[yy, xx] = val
+    assert approx(B.siddon(g, v).ravel()[0], 14 - 4 * s3)
+    exp = np.array([0, 0, 0, 0, 0, 0, 0, 4 - 2 * s3, 0, 0, 0, 2 / s3, 0, 0, 2 - 2 / s3, 4 / s3 - 2])
+    assert is_approx(B.siddon(g, np.ones((1, 1)), adjoint=True), exp.reshape(n, n), EPS)
+
+
+def siddon_multiple_rays_2d(B):
+    """test_SiddonsMethodCUDA.cpp:928-1000: four axis-aligned poses."""
+    n = 5
+    g = B.setup((n, n), (1,), [pose(20 * n, n, deg(a)) for a in (0, 90, 180, 270)])
+    v = np.zeros((n, n))
+    v[:, n // 2] = 1
+    v[n // 2, :] = 1
+    y = B.siddon(g, v)
+    assert all(approx(s, 5.0) for s in y.ravel()), y
+    cmp = np.array([0, 0, 10, 0, 0, 0, 0, 10, 0, 0, 10, 10, 20, 10, 10, 0, 0, 10, 0, 0, 0, 0, 10, 0, 0],
+                   np.float64).reshape(n, n)
+    assert is_approx(B.siddon(g, y, adjoint=True), cmp)
+
+
+def siddon_axis_aligned_3d(B):
+    """test_SiddonsMethodCUDA.cpp:1005-1120: empty inputs and axis-aligned rays in 3^3."""
+    n = 3
+    g = B.setup((n, n, n), (1, 1), [pose(20 * n, n, 0.0)])
+    assert is_zero(B.siddon(g, np.zeros((n, n, n))))
+    assert is_zero(B.siddon(g, np.zeros((1, 1, 1)), adjoint=True))
+    h = float(PI / 2)
+    beta = [0.0, 0.0, 0.0, 0.0, h, float(3 * PI / 2)]
+    gamma = [0.0, float(PI), h, float(3 * PI / 2), h, float(3 * PI / 2)]
+    bp = [np.zeros(27) for _ in range(3)]
+    bp[0][[4, 13, 22]] = 1
+    bp[1][[12, 13, 14]] = 1
+    bp[2][[10, 13, 16]] = 1
+    for i in range(6):
+        g = B.setup((n, n, n), (1, 1), [pose(20 * n, n, gamma[i], beta[i])])
+        for j in range(n):
+            v = np.zeros((n, n, n))
+            if i // 2 == 0:
+                v[j, 1, 1] = 1
+            elif i // 2 == 1:
+                v[1, 1, j] = 1
+            else:
+                v[1, j, 1] = 1
+            y = B.siddon(g, v)
+            assert approx(y.ravel()[0], 1.0), (i, j, y)
+        assert is_approx(B.siddon(g, y, adjoint=True), bp[i // 2].reshape(n, n, n)), i
+
+
+SIDDON_KATS = [siddon_axis_aligned_2d, siddon_angle_30_2d, siddon_multiple_rays_2d, siddon_axis_aligned_3d]
+
+
+# --------------------------------------------------------------------------------------
+# Voxel projectors, elsa/projectors_cuda/tests/test_VoxelProjectorCUDA.cpp
+
+
+def _blob0(B):
+    import oracle
+
+    return float(oracle.blob_lut(2.0, 10.83, 2, 100, B.dtype)[0])
+
+
+def voxel_centre_voxel_3d(B):
+    """test_VoxelProjectorCUDA.cpp:96-139: the centre voxel of 5^3 projects to blob(0) on a
+    one-pixel detector for poses Gamma = 0, 4, 8, ... radians (epsilon 0.005)."""
+    w0 = _blob0(B)
+    x = np.zeros((5, 5, 5)); x[2, 2, 2] = 1
+    for i in range(0, 360, 4):
+        g = B.setup((5, 5, 5), (1, 1), [pose(100, 5, float(i))])
+        y = B.voxel(g, x, "blob")
+        assert approx(w0, y.ravel()[0], 0.005), (i, y)
+
+
+def voxel_centre_voxel_2d(B):
+    """test_VoxelProjectorCUDA.cpp:141-190 (unit spacing): centre voxel -> blob(0)."""
+    w0 = _blob0(B)
+    x = np.zeros((5, 5)); x[2, 2] = 1
+    for i in range(0, 360, 4):
+        g = B.setup((5, 5), (1,), [pose(100, 5, deg(i))])
+        y = B.voxel(g, x, "blob")
+        assert approx(w0, y.ravel()[0]), (i, y)
+
+
+def voxel_backward_2d(B):
+    """test_VoxelProjectorCUDA.cpp:945-1010: back projection of a single detector value: the
+    central column gets blob(0), the neighbouring columns blob(d/s) of their projected
+    distance (projectAndScaleVoxelOnDetector, DetectorDescriptor.cpp:130-151)."""
+    import oracle
+
+    g = B.setup((5, 5), (1,), [pose(100, 5, 0.0)])
+    x = B.voxel(g, np.ones((1, 1)), "blob", adjoint=True)
+    lut = oracle.blob_lut(2.0, 10.83, 2, 100, np.float64)
+    w0 = _blob0(B)
+    for s in range(5):
+        assert approx(x[s, 2], w0), (s, x[s, 2])
+    og = oracle.geom_from_poses(2, (5, 5), (1,), [oracle.geometry_pose(2, 100.0, 5.0, (5, 5), (1,), 0.0,
+                                                                       dtype=np.float64)], np.float64)
+    P = og.P[0].reshape(3, 2).T
+    E = og.ext[0].reshape(3, 2).T
+    for col in (1, 3):
+        for s in range(5):
+            hv = np.array([col + 0.5, s + 0.5, 1.0])
+            pc = P @ hv
+            det = pc[0] / pc[1] - 0.5
+            scaling = og.sdd[0] / np.linalg.norm(E @ hv)
+            w = oracle.lut_eval(lut, 2.0, abs(det / scaling))
+            assert approx(x[s, col], w, 1e-4), (col, s, x[s, col], w)
+
+
+VOXEL_KATS = [voxel_centre_voxel_3d, voxel_centre_voxel_2d, voxel_backward_2d]

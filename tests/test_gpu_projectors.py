@@ -1,0 +1,244 @@
+"""GPU parity of the HIP projectors (through elsa_amd and the C-ABI) against the oracle.
+
+Bars:
+  * integer work (per-ray driving axis and sample counts, total sample counts): bit-exact;
+  * Siddon forward and voxel back projection (same arithmetic, same summation order as the
+    CPU restatement): bit-exact in FP32;
+  * Joseph against the oracle evaluating sample positions the same way (pos0 + s*step):
+    rel. L2 <= 1e-6 (only the summation order differs);
+  * Joseph against elsa's own rule (repeated addition of the step): rel. L2 <= 2e-5, and the
+    GPU's error against the FP64 oracle is no larger than elsa-FP32's;
+  * atomically accumulated back projections (Joseph, Siddon) and the voxel forward:
+    rel. L2 <= 1e-6 against the oracle;
+  * matched adjoints: |<Ax,y> - <x,A^T y>| / (|Ax| |y|) <= 1e-6 in FP32, 1e-12 in FP64.
+"""
+import numpy as np
+import pytest
+
+import kats
+import oracle
+from backends import GpuBackend, geom_to_descriptors
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    import torch
+
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    import elsa_amd  # noqa: F401  (loads libelsahip.so; fails loudly when missing)
+
+
+# --------------------------------------------------------------------------------------
+# known-answer tests of the reference, through the C-ABI
+
+
+@pytest.mark.parametrize("kat", kats.JOSEPH_KATS, ids=lambda k: k.__name__)
+@pytest.mark.parametrize("dt", [np.float32, np.float64], ids=["f32", "f64"])
+def test_joseph_kats_gpu(kat, dt):
+    kat(GpuBackend(dt))
+
+
+@pytest.mark.parametrize("kat", kats.SIDDON_KATS, ids=lambda k: k.__name__)
+@pytest.mark.parametrize("dt", [np.float32, np.float64], ids=["f32", "f64"])
+def test_siddon_kats_gpu(kat, dt):
+    kat(GpuBackend(dt))
+
+
+@pytest.mark.parametrize("kat", kats.VOXEL_KATS, ids=lambda k: k.__name__)
+@pytest.mark.parametrize("dt", [np.float32, np.float64], ids=["f32", "f64"])
+def test_voxel_kats_gpu(kat, dt):
+    kat(GpuBackend(dt))
+
+
+# --------------------------------------------------------------------------------------
+# parity on seeded inputs
+
+CASES = {
+    # 2D fan beam of the 2D CPU reference workload: 256^2, 180 poses over 360 deg, det 384
+    "fan2d_256": dict(dim=2, vol=(256, 256), views=180, det=(384,), s2c=25600.0, c2d=1280.0),
+    # cone beam at a reduced size with the same magnification as the 3D drop-in workload
+    "cone3d_64": dict(dim=3, vol=(64, 64, 64), views=90, det=(96, 96), s2c=6400.0, c2d=320.0),
+    # strongly divergent cone with a non-cubic volume and ragged detector
+    "cone3d_wide": dict(dim=3, vol=(40, 24, 33), views=13, det=(61, 29), s2c=60.0, c2d=40.0),
+    # a small fan with a detector wider than the fan (many rays miss)
+    "fan2d_wide": dict(dim=2, vol=(31, 17), views=7, det=(80,), s2c=40.0, c2d=30.0),
+}
+
+
+def make_case(name, dtype=np.float32):
+    c = CASES[name]
+    return oracle.circle_geometry(c["dim"], c["vol"], c["views"], 360, c["s2c"], c["c2d"], det=c["det"], dtype=dtype)
+
+
+def phantom(shape, seed, dtype=np.float32):
+    rng = np.random.default_rng(seed)
+    x = rng.random(shape)
+    # plus a smooth part so that interpolation weights matter
+    grids = np.meshgrid(*[np.linspace(-1, 1, s) for s in shape], indexing="ij")
+    r2 = sum(g * g for g in grids)
+    return (x * 0.3 + (r2 < 0.6)).astype(dtype)
+
+
+@pytest.fixture(scope="module")
+def ops():
+    import elsa_amd as E
+
+    cache = {}
+
+    def get(cls, name, dtype=np.float32, **kw):
+        key = (cls, name, np.dtype(dtype).name, tuple(sorted(kw.items())))
+        if key not in cache:
+            g32 = make_case(name, np.float32)
+            vd, dd = geom_to_descriptors(g32)
+            cache[key] = getattr(E, cls)(vd, dd, dtype=dtype, **kw)
+        return cache[key]
+
+    return get
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_joseph_ray_state_bit_exact(name, ops):
+    """Driving axis and sample count of every ray equal the oracle's (integer parity)."""
+    g = make_case(name)
+    op = ops("JosephsMethodCUDA", name)
+    axis, nsteps = op.ray_info()
+    oaxis, onsteps, _, _, _ = oracle.joseph_rays(g)
+    np.testing.assert_array_equal(nsteps, onsteps)
+    np.testing.assert_array_equal(axis, oaxis)
+    assert op.sample_count() == int(onsteps.sum())
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_joseph_forward_parity(name, ops):
+    g = make_case(name)
+    x = phantom(g.vol_shape, 1)
+    gpu = ops("JosephsMethodCUDA", name).apply(x)
+    direct = oracle.joseph(g, x, mode=oracle.MODE_DIRECT)
+    elsa = oracle.joseph(g, x, mode=oracle.MODE_ELSA)
+    ref64 = oracle.joseph(make_case(name, np.float64), x.astype(np.float64), mode=oracle.MODE_DIRECT)
+    assert rel_l2(gpu, direct) <= 1e-6, rel_l2(gpu, direct)
+    assert rel_l2(gpu, elsa) <= 2e-5, rel_l2(gpu, elsa)
+    # error budget against FP64: the GPU is at least as close as elsa's FP32 arithmetic
+    assert rel_l2(gpu, ref64) <= max(1.05 * rel_l2(elsa, ref64), 1e-6), (rel_l2(gpu, ref64), rel_l2(elsa, ref64))
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_joseph_adjoint_parity(name, ops):
+    g = make_case(name)
+    y = phantom(g.sino_shape, 2)
+    gpu = ops("JosephsMethodCUDA", name).applyAdjoint(y)
+    direct = oracle.joseph(g, y, adjoint=True, mode=oracle.MODE_DIRECT)
+    elsa = oracle.joseph(g, y, adjoint=True, mode=oracle.MODE_ELSA)
+    assert rel_l2(gpu, direct) <= 1e-6, rel_l2(gpu, direct)
+    assert rel_l2(gpu, elsa) <= 2e-5, rel_l2(gpu, elsa)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_siddon_parity(name, ops):
+    g = make_case(name)
+    x = phantom(g.vol_shape, 3)
+    y = phantom(g.sino_shape, 4)
+    op = ops("SiddonsMethodCUDA", name)
+    fwd = op.apply(x)
+    np.testing.assert_array_equal(fwd, oracle.siddon(g, x))  # same arithmetic, same order
+    adj = op.applyAdjoint(y)
+    assert rel_l2(adj, oracle.siddon(g, y, adjoint=True)) <= 1e-6
+
+
+@pytest.mark.parametrize("name", ["fan2d_256", "cone3d_wide", "fan2d_wide"])
+@pytest.mark.parametrize("kind", ["blob", "bspline"])
+def test_voxel_parity(name, kind, ops):
+    g = make_case(name)
+    x = phantom(g.vol_shape, 5)
+    y = phantom(g.sino_shape, 6)
+    if kind == "blob":
+        op = ops("BlobVoxelProjectorCUDA", name)
+        lut, sup = oracle.blob_lut(2.0, 10.83, 2, 100, np.float32), 2.0
+    else:
+        op = ops("BSplineVoxelProjectorCUDA", name)
+        lut, sup = oracle.bspline_lut(g.dim, 2, 100, np.float32), 1.5
+    np.testing.assert_array_equal(op.applyAdjoint(y), oracle.voxel(g, y, lut, sup, adjoint=True))
+    assert rel_l2(op.apply(x), oracle.voxel(g, x, lut, sup)) <= 1e-6
+
+
+@pytest.mark.parametrize("cls", ["JosephsMethodCUDA", "SiddonsMethodCUDA", "BlobVoxelProjectorCUDA"])
+@pytest.mark.parametrize("name", ["fan2d_256", "cone3d_64", "cone3d_wide"])
+@pytest.mark.parametrize("dt", [np.float32, np.float64], ids=["f32", "f64"])
+def test_dot_product_adjointness(cls, name, dt, ops):
+    g = make_case(name)
+    op = ops(cls, name, dtype=dt)
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(g.vol_shape).astype(dt)
+    y = rng.standard_normal(g.sino_shape).astype(dt)
+    ax = op.apply(x).astype(np.float64)
+    aty = op.applyAdjoint(y).astype(np.float64)
+    lhs = np.vdot(ax, y.astype(np.float64))
+    rhs = np.vdot(x.astype(np.float64), aty)
+    tol = 1e-6 if dt == np.float32 else 1e-12
+    assert abs(lhs - rhs) <= tol * np.linalg.norm(ax) * np.linalg.norm(y), (lhs, rhs)
+
+
+@pytest.mark.parametrize("name", ["cone3d_64", "fan2d_wide"])
+def test_fp64_data_projectors(name, ops):
+    """data_t = double keeps elsa's FP32 geometry (real_t is float) and accumulates in double:
+    against the FP32-geometry oracle run on the same data, only the data precision differs."""
+    g = make_case(name)
+    x = phantom(g.vol_shape, 8, np.float64)
+    gpu = ops("JosephsMethodCUDA", name, dtype=np.float64).apply(x)
+    ref = oracle.joseph(g, x.astype(np.float32), mode=oracle.MODE_DIRECT)
+    assert rel_l2(gpu, ref) <= 1e-6
+    y = phantom(g.sino_shape, 9, np.float64)
+    gpu_b = ops("SiddonsMethodCUDA", name, dtype=np.float64).applyAdjoint(y)
+    ref_b = oracle.siddon(g, y.astype(np.float32), adjoint=True)
+    assert rel_l2(gpu_b, ref_b) <= 1e-6
+
+
+def test_outputs_overwritten_not_accumulated(ops):
+    """elsa's projectors overwrite the output (TraverseJosephsCUDA.cu:69, SiddonsMethodCUDA.cpp:75)."""
+    import torch
+
+    name = "cone3d_wide"
+    g = make_case(name)
+    x = torch.from_numpy(phantom(g.vol_shape, 9)).cuda()
+    for cls in ("JosephsMethodCUDA", "SiddonsMethodCUDA", "BlobVoxelProjectorCUDA"):
+        op = ops(cls, name)
+        out = torch.full(g.sino_shape, 7.0, device="cuda")
+        op.apply(x, out)
+        ref = op.apply(x)
+        assert torch.equal(out, ref) or float((out - ref).abs().max()) <= 1e-5 * float(ref.abs().max())
+        vol = torch.full(g.vol_shape, -3.0, device="cuda")
+        op.applyAdjoint(ref, vol)
+        ref_v = op.applyAdjoint(ref)
+        assert float((vol - ref_v).abs().max()) <= 1e-5 * float(ref_v.abs().max())
+
+
+def test_views_of_a_larger_buffer(ops):
+    """Input and output may be views at an offset inside a bigger allocation (the begin
+    pointer is used, never the parent storage)."""
+    import torch
+
+    name = "fan2d_wide"
+    g = make_case(name)
+    op = ops("JosephsMethodCUDA", name)
+    x = phantom(g.vol_shape, 10)
+    n_sino = int(np.prod(g.sino_shape))
+    big = torch.zeros(3 * n_sino, device="cuda")
+    view = big[n_sino:2 * n_sino].view(g.sino_shape)
+    op.apply(torch.from_numpy(x).cuda(), view)
+    torch.cuda.synchronize()
+    assert float(big[:n_sino].abs().sum()) == 0.0 and float(big[2 * n_sino:].abs().sum()) == 0.0
+    np.testing.assert_allclose(view.cpu().numpy(), op.apply(x), rtol=0, atol=0)
+
+
+def test_size_mismatch_raises(ops):
+    op = ops("JosephsMethodCUDA", "fan2d_wide")
+    with pytest.raises(ValueError):
+        op.apply(np.zeros((5, 5), np.float32))

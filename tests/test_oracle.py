@@ -1,0 +1,329 @@
+"""Pins the CPU oracle before it is trusted as the checker.
+
+Each check is chosen so that a plausible slip in the restatement (a dropped term, a swapped
+index, a wrong sign) fails it: golden vectors of the reference's own tests, its known-answer
+tests, and what the mathematics fixes (adjointness, linearity, closed forms, brute force).
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import kats
+import oracle
+from backends import OracleBackend
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# --------------------------------------------------------------------------------------
+# golden vectors
+
+
+def test_projected_blob_lut_golden():
+    """elsa/projectors/tests/test_Luts.cpp:125-247 (ProjectedBlob<data_t, 101>, rel 1e-6)."""
+    gold = json.load(open(os.path.join(GOLDEN, "luts.json")))["projected_blob"]
+    for dt in (np.float32, np.float64):
+        lut = oracle.blob_lut(2.0, 10.83, 2, 101, dt)
+        for i in range(99):
+            got = oracle.lut_eval(lut, 2.0, i / 50.0) if dt == np.float64 else oracle.lut_eval(lut, 2.0, np.float32(i / 50.0))
+            assert kats.approx(got, gold["expected"][i], 1e-6), (dt, i, got, gold["expected"][i])
+
+
+def test_projected_bspline_lut_golden():
+    """elsa/projectors/tests/test_Luts.cpp:249-306 (ProjectedBSpline order 3, dim 2)."""
+    gold = json.load(open(os.path.join(GOLDEN, "luts.json")))["projected_bspline"]
+    for dt in (np.float32, np.float64):
+        lut = oracle.bspline_lut(2, 3, 101, dt)
+        for i in range(101):
+            got = oracle.lut_eval(lut, 2.0, i / 50.0) if dt == np.float64 else oracle.lut_eval(lut, 2.0, np.float32(i / 50.0))
+            assert kats.approx(got, gold["expected"][i], 1e-6), (dt, i, got)
+
+
+def test_bessel_golden():
+    """elsa/core/tests/test_Bessel.cpp:29-89 (std::cyl_bessel_i tables, doctest Approx)."""
+    gold = json.load(open(os.path.join(GOLDEN, "bessel.json")))
+    for m in range(5):
+        exp = gold[f"I{m}"]
+        x = np.linspace(0.0, 4.0, len(exp))
+        for xi, e in zip(x, exp):
+            assert kats.approx(oracle.bessi(m, float(xi)), e), (m, xi)
+
+
+def test_bessel_vs_scipy_and_recurrence():
+    """I_0, I_1 are the A&S 9.8.1-9.8.4 fits (relative error below 1e-6 of scipy); higher
+    orders come from upward recurrence (orders 2-4) or Miller's algorithm, which elsa accepts
+    to doctest Approx, i.e. 1.2e-5 on scale 1 (cancellation at small x)."""
+    sp = pytest.importorskip("scipy.special")
+    for m in range(0, 7):
+        for x in (0.3, 1.0, 3.7, 3.8, 7.5, 10.83):
+            got, ref = oracle.bessi(m, x), sp.iv(m, x)
+            if m < 2:
+                assert got == pytest.approx(ref, rel=1e-6), (m, x)
+            else:
+                assert kats.approx(got, ref), (m, x, got, ref)
+
+
+def test_blob_profile_vs_scipy():
+    """Blob profile of elsa/projectors/Blobs.h:17-30 against the definition with scipy's I_m
+    (the formula elsa's test_Blobs.cpp:112-122 generated its tables from)."""
+    sp = pytest.importorskip("scipy.special")
+    for a in (1.0, 2.0, 3.0):
+        for alpha in (2.5, 5.0, 7.5, 10.83):
+            for m in (0, 1, 2, 3):
+                for r in np.linspace(0, a * 1.05, 23):
+                    arg = math.sqrt(max(0.0, 1 - (r / a) ** 2))
+                    ref = 0.0 if r > a else sp.iv(m, alpha * arg) / sp.iv(m, alpha) * arg ** m
+                    assert oracle.blob_evaluate(float(r), a, alpha, m) == pytest.approx(ref, rel=1e-5, abs=1e-7)
+
+
+def test_projected_blob_is_line_integral_of_profile():
+    """Closed form of Blobs.h:67-98 against numerical integration of the profile along the
+    line (elsa test_Blobs.cpp:48-120 does the same with Simpson's rule)."""
+    from scipy.integrate import quad
+
+    for a in (1.0, 2.0, 3.0):
+        for alpha in (2.5, 10.83):
+            for m in (0, 1, 2):
+                for s in np.linspace(0, a * 0.98, 9):
+                    L = math.sqrt(a * a - s * s)
+                    num = 2 * quad(lambda t: oracle.blob_evaluate(math.sqrt(t * t + s * s), a, alpha, m), 0, L)[0]
+                    assert oracle.blob_projected(float(s), a, alpha, m) == pytest.approx(num, rel=1e-4, abs=1e-9)
+
+
+def test_bspline_partition_of_unity():
+    """Cardinal B-splines sum to one over integer shifts; a wrong binomial or sign breaks it.
+    (Off the knots: the order-0 box is closed at +-1/2 in elsa's definition.)"""
+    for order in range(0, 5):
+        lut = oracle.bspline_lut(2, order, 2001, np.float64)
+        sup = (order + 1) * 0.5
+        for x in np.linspace(0, 1, 8):
+            tot = sum(oracle.lut_eval(lut, sup, abs(x - k)) for k in range(-4, 5))
+            assert tot == pytest.approx(1.0, abs=2e-3), (order, x)
+
+
+# --------------------------------------------------------------------------------------
+# geometry
+
+
+def _closed_form_pose(dim, s2c, c2d, vol, det, gamma):
+    """P, Pinv, C from the closed forms of the circle geometry, in double."""
+    if dim == 2:
+        c, s = math.cos(gamma), math.sin(gamma)
+        R = np.array([[c, -s], [s, c]])
+    else:
+        c, s = math.cos(gamma), math.sin(gamma)
+        R = np.array([[c, 0, s], [0, 1, 0], [-s, 0, c]])
+    org = 0.5 * np.asarray(vol, float)
+    t = R @ (-org)
+    t[-1] += s2c
+    K = np.eye(dim)
+    for i in range(dim - 1):
+        K[i, i] = s2c + c2d
+        K[i, -1] = 0.5 * det[i]
+    P = K @ np.hstack([R, t[:, None]])
+    C = -R.T @ t
+    return P, C
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_geometry_closed_forms(dim):
+    vol = (16,) * dim
+    det = (23,) * (dim - 1)
+    for gamma in (0.0, 0.3, 1.7, -2.9):
+        P, Pinv, C, ext, sdd = oracle.geometry_pose(dim, 250.0, 40.0, vol, det, gamma, dtype=np.float64)
+        Pm = P.reshape(dim + 1, dim).T
+        Pi = Pinv.reshape(dim, dim + 1).T
+        Pc, Cc = _closed_form_pose(dim, 250.0, 40.0, vol, det, gamma)
+        np.testing.assert_allclose(Pm, Pc, rtol=1e-12, atol=1e-9)
+        np.testing.assert_allclose(C, Cc, rtol=1e-12, atol=1e-9)
+        # Pinv inverts the 3x3 block: P[:, :d] @ Pinv[:d, :] = I
+        np.testing.assert_allclose(Pm[:, :dim] @ Pi[:dim, :], np.eye(dim), atol=1e-12)
+        # the camera centre projects to the homogeneous zero
+        np.testing.assert_allclose(Pm @ np.append(C, 1.0), 0, atol=1e-9)
+        assert sdd == 290.0
+        # FP32 geometry agrees with FP64 to float rounding
+        P32, Pinv32, C32, _, _ = oracle.geometry_pose(dim, 250.0, 40.0, vol, det, gamma, dtype=np.float32)
+        np.testing.assert_allclose(P32, P, rtol=1e-5, atol=1e-4)
+        np.testing.assert_allclose(C32, C, rtol=1e-6, atol=1e-4)
+
+
+def test_geometry_kat_3d_rotations():
+    """elsa/core/tests/test_Geometry.cpp: with gamma = pi/2 the source sits on +x at s2c
+    (rotation about y), the principal ray runs along -x."""
+    P, Pinv, C, ext, sdd = oracle.geometry_pose(3, 10.0, 5.0, (4, 4, 4), (3, 3), float(np.pi / 2), dtype=np.float64)
+    # volume centre is (2, 2, 2); the camera centre lies s2c away from it
+    assert np.linalg.norm(C - 2.0) == pytest.approx(10.0)
+    o, d = C, (Pinv.reshape(3, 4).T @ np.array([1.5, 1.5, 1.0]))[:3]
+    d = d / np.linalg.norm(d)
+    # the central pixel's ray passes through the volume centre
+    t = np.dot(np.array([2.0, 2.0, 2.0]) - o, d)
+    assert np.linalg.norm(o + t * d - 2.0) < 1e-9
+
+
+def test_circle_trajectory_linspaced_inclusive():
+    """LinSpaced(n, 0, arc) includes both ends (CircleTrajectoryGenerator.cpp:16-19):
+    for arc 360 the first and last poses coincide."""
+    g = oracle.circle_geometry(2, (8, 8), 5, 360, 100.0, 20.0)
+    np.testing.assert_allclose(g.C[0], g.C[-1], atol=1e-4)
+    th = oracle.linspaced_f32(5, 360)
+    assert list(th) == [0.0, 90.0, 180.0, 270.0, 360.0]
+    # default detector floor(N*sqrt(2))
+    assert oracle.circle_geometry(3, (256, 256, 256), 2, 180, 100.0, 20.0).det == (362, 362)
+
+
+def test_intersection_kats():
+    """elsa/projectors/tests/test_Intersection.cpp:21-150 (corners and edges of boxes)."""
+    r2 = math.sqrt(0.5)
+    assert oracle.intersect([-3, -3], [r2, r2], (1, 0)) is not None
+    assert oracle.intersect([1, 2], [r2, -r2], (1, 0)) is None
+    assert oracle.intersect([3, -2], [-r2, r2], (1, 0)) is not None
+    assert oracle.intersect([3, 1], [-r2, -r2], (1, 0)) is None
+    assert oracle.intersect([132, 30], [-1, 0], (30, 31)) is not None
+    assert oracle.intersect([132, 30], [-1, 0], (40, 30)) is not None   # along the upper edge
+    assert oracle.intersect([132, 30], [-1, 0], (40, 29)) is None
+    assert oracle.intersect([30, -35], [0, 1], (31, 30)) is not None
+    assert oracle.intersect([30, -35], [0, 1], (30, 40)) is not None
+    assert oracle.intersect([30, -35], [0, 1], (29, 40)) is None
+    assert oracle.intersect([1.5, -10], [0, 1], (0, 1)) is None
+    assert oracle.intersect([1.5, -10], [0, 1], (1, 1)) is None
+    assert oracle.intersect([1.5, -10], [0, 1], (2, 1)) is not None
+    # entry/exit parameters of an axis ray
+    assert oracle.intersect([-1.0, 0.5], [1.0, 0.0], (2, 2)) == pytest.approx((1.0, 3.0))
+    assert oracle.intersect([1.0, 1.0], [1.0, 0.0], (2, 2)) == pytest.approx((-1.0, 1.0))
+
+
+# --------------------------------------------------------------------------------------
+# Joseph / Siddon / voxel on the oracle
+
+
+@pytest.mark.parametrize("kat", kats.JOSEPH_KATS, ids=lambda k: k.__name__)
+@pytest.mark.parametrize("mode", [oracle.MODE_ELSA, oracle.MODE_DIRECT], ids=["elsa", "direct"])
+@pytest.mark.parametrize("dt", [np.float32, np.float64], ids=["f32", "f64"])
+def test_joseph_kats_oracle(kat, mode, dt):
+    kat(OracleBackend(dt, mode))
+
+
+@pytest.mark.parametrize("kat", kats.JOSEPH_KATS_FP64_ONLY, ids=lambda k: k.__name__)
+def test_joseph_tie_kats_oracle(kat):
+    kat(OracleBackend(np.float64))
+    with pytest.raises(AssertionError):
+        kat(OracleBackend(np.float32))  # the exit lands exactly on the sample plane in FP32
+
+
+@pytest.mark.parametrize("kat", kats.SIDDON_KATS, ids=lambda k: k.__name__)
+@pytest.mark.parametrize("dt", [np.float32, np.float64], ids=["f32", "f64"])
+def test_siddon_kats_oracle(kat, dt):
+    kat(OracleBackend(dt))
+
+
+@pytest.mark.parametrize("kat", kats.VOXEL_KATS, ids=lambda k: k.__name__)
+@pytest.mark.parametrize("dt", [np.float32, np.float64], ids=["f32", "f64"])
+def test_voxel_kats_oracle(kat, dt):
+    kat(OracleBackend(dt))
+
+
+def _rand_geom(dim, n, nviews, det, dtype, seed=0, s2c=None):
+    rng = np.random.default_rng(seed)
+    angles = np.sort(rng.uniform(0, 360, nviews)).astype(np.float32)
+    return oracle.circle_geometry(dim, (n,) * dim, nviews, 360, s2c or 20.0 * n, 2.0 * n, det=det, dtype=dtype,
+                                  angles_deg=angles)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("kind", ["joseph", "siddon", "blob"])
+def test_adjointness_fp64(dim, kind):
+    """<Ax, y> = <x, A^T y> to FP64 rounding: a transposed tap or a dropped weight breaks it."""
+    n = 12 if dim == 3 else 24
+    det = (17, 15) if dim == 3 else (31,)
+    g = _rand_geom(dim, n, 7, det, np.float64, seed=3)
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal(g.vol_shape)
+    y = rng.standard_normal(g.sino_shape)
+    if kind == "joseph":
+        A = lambda v: oracle.joseph(g, v)
+        At = lambda v: oracle.joseph(g, v, adjoint=True)
+    elif kind == "siddon":
+        A = lambda v: oracle.siddon(g, v)
+        At = lambda v: oracle.siddon(g, v, adjoint=True)
+    else:
+        lut = oracle.blob_lut(2.0, 10.83, 2, 100, np.float64)
+        A = lambda v: oracle.voxel(g, v, lut, 2.0)
+        At = lambda v: oracle.voxel(g, v, lut, 2.0, adjoint=True)
+    lhs = np.vdot(A(x), y)
+    rhs = np.vdot(x, At(y))
+    assert abs(lhs - rhs) <= 1e-12 * np.linalg.norm(A(x)) * np.linalg.norm(y) * 10
+
+
+def test_joseph_linearity_and_modes():
+    g = _rand_geom(3, 10, 5, (15, 13), np.float32, seed=5)
+    rng = np.random.default_rng(6)
+    a = rng.standard_normal(g.vol_shape).astype(np.float32)
+    b = rng.standard_normal(g.vol_shape).astype(np.float32)
+    lhs = oracle.joseph(g, 2 * a + 3 * b)
+    rhs = 2 * oracle.joseph(g, a) + 3 * oracle.joseph(g, b)
+    np.testing.assert_allclose(lhs, rhs, rtol=1e-4, atol=1e-4)
+    # the two position rules agree to rounding of the accumulated step
+    e = oracle.joseph(g, a, mode=oracle.MODE_ELSA)
+    d = oracle.joseph(g, a, mode=oracle.MODE_DIRECT)
+    assert np.linalg.norm(e - d) <= 1e-5 * np.linalg.norm(e)
+
+
+def test_joseph_constant_volume_chord():
+    """A constant volume projects to value x chord length up to the half-voxel boundary
+    sampling: rays parallel to an axis through the box centre see exactly N samples."""
+    n = 16
+    g = oracle.geom_from_poses(2, (n, n), (1,), [oracle.geometry_pose(2, 1e5, 10.0, (n, n), (1,), 0.0,
+                                                                      dtype=np.float64)], np.float64)
+    y = oracle.joseph(g, np.full(g.vol_shape, 2.0))
+    assert y.ravel()[0] == pytest.approx(2.0 * n, rel=1e-9)
+
+
+def test_siddon_lengths_sum_to_chord():
+    """Siddon: the intersection lengths of a ray sum to its chord through the box."""
+    n = 9
+    g = _rand_geom(3, n, 6, (13, 11), np.float64, seed=8)
+    ones = oracle.siddon(g, np.ones(g.vol_shape))
+    axis, nsteps, pos0, step, ln = oracle.joseph_rays(g)
+    for r in np.random.default_rng(9).choice(ones.size, 40, replace=False):
+        view, rem = divmod(int(r), 13 * 11)
+        v, u = divmod(rem, 13)
+        o, d = oracle.make_ray(g.astype(np.float32), view, u, v)
+        hit = oracle.intersect(o, d, (n, n, n))
+        chord = 0.0 if hit is None else max(0.0, hit[1] - max(hit[0], 0.0))
+        assert ones.ravel()[r] == pytest.approx(chord, abs=2e-4), r
+
+
+def test_joseph_fp64_brute_force_tiny():
+    """FP64 brute force: the forward value of each ray equals len * sum over the planes of the
+    bilinear interpolation, recomputed here independently from the ray parameters."""
+    n = 6
+    g = _rand_geom(3, n, 3, (7, 5), np.float64, seed=10, s2c=30.0)
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal(g.vol_shape)
+    y = oracle.joseph(g, x, mode=oracle.MODE_DIRECT).ravel()
+    axis, nsteps, pos0, step, ln = oracle.joseph_rays(g)
+
+    def val(c):
+        if all(0 <= c[i] < n for i in range(3)):
+            return x[c[2], c[1], c[0]]
+        return 0.0
+
+    for r in range(y.size):
+        acc = 0.0
+        a = axis[r]
+        bs = [i for i in range(3) if i != a]
+        for s in range(nsteps[r]):
+            p = pos0[r] + s * step[r]
+            f = [int(math.floor(p[i] - 0.5)) for i in range(3)]
+            t = [p[i] - 0.5 - f[i] for i in range(3)]
+            for d1 in (0, 1):
+                for d2 in (0, 1):
+                    c = list(f)
+                    c[bs[0]] += d1
+                    c[bs[1]] += d2
+                    w = (t[bs[0]] if d1 else 1 - t[bs[0]]) * (t[bs[1]] if d2 else 1 - t[bs[1]])
+                    acc += w * val(c)
+        assert y[r] == pytest.approx(acc * ln[r], rel=1e-9, abs=1e-9), r
