@@ -45,9 +45,11 @@ def parse():
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--workload", default="cone3d_256", choices=sorted(WORKLOADS))
-    p.add_argument("--cpu-poses", type=int, default=24,
+    p.add_argument("--cpu-poses", type=int, default=240,
                    help="poses of the bounded CPU sample (evenly spread over the trajectory)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--kernels", default="tiled", choices=["tiled", "simple"],
+                   help="Joseph kernels: LDS-tiled back projection (default) or one thread per ray")
     return p.parse_args()
 
 
@@ -87,7 +89,7 @@ def main():
     vd = E.VolumeDescriptor(list(wl["vol"]))
     dd = E.CircleTrajectoryGenerator.createTrajectory(wl["views"], vd, 360, wl["s2c"], wl["c2d"],
                                                       detectorSize=list(wl["det"]))
-    op = E.JosephsMethodCUDA(vd, dd)
+    op = E.JosephsMethodCUDA(vd, dd, kernels=a.kernels)
     S = op.sample_count()
     R = int(np.prod(wl["det"])) * wl["views"]
     x_np = modified_shepp_logan(wl["vol"][::-1])
@@ -117,7 +119,8 @@ def main():
     t_adj = float(np.mean([e1.elapsed_time(e2) for _, e1, e2 in evs])) * 1e-3
     tap_bytes = 16 if wl["dim"] == 3 else 8
     alg_bytes = tap_bytes * S + 4 * R
-    name, t_dom = ("joseph_adj_kernel", t_adj) if t_adj >= t_fwd else ("joseph_fwd_kernel", t_fwd)
+    adj_name = "joseph_adj_tiled_kernel" if a.kernels == "tiled" else "joseph_adj_kernel"
+    name, t_dom = (adj_name, t_adj) if t_adj >= t_fwd else ("joseph_fwd_kernel", t_fwd)
     achieved = alg_bytes / t_dom / 1e9
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"{name}_{a.workload}_traffic.json")
@@ -137,7 +140,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (modified Shepp-Logan phantom)",
-        "config": {"workload": wl["text"], "volume": list(wl["vol"]), "poses": wl["views"],
+        "config": {"workload": wl["text"], "kernels": a.kernels, "volume": list(wl["vol"]), "poses": wl["views"],
                    "detector": list(wl["det"]), "samples_per_pass": S, "rays": R,
                    "fwd_ms": round(t_fwd * 1e3, 3), "adj_ms": round(t_adj * 1e3, 3)},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

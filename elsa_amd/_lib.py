@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(HERE, "libelsahip.so")
 
 EHP_OK, EHP_ERR_ARGUMENT, EHP_ERR_HIP, EHP_ERR_ALLOC, EHP_ERR_UNSUPPORTED = range(5)
 EHP_F32, EHP_F64 = 0, 1
+EHP_JOSEPH_SIMPLE_KERNELS = 1
 
 # every symbol include/elsa_hip.h declares
 EXPORTS = (

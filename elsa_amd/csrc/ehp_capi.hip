@@ -33,6 +33,11 @@ struct ehp_op {
     int n_lut = 0;
     float support = 0.0f;
     uint64_t samples = 0;
+    // Joseph, FP32: LDS-tiled back projection
+    bool tiled = false;
+    int* d_pose_axes = nullptr;
+    int axes_present = 0;
+    ehp::TileCfg tile{};
 };
 
 namespace
@@ -206,14 +211,52 @@ int ehp_geometry_create(int dim, int64_t nposes, const int64_t* vol_dims, const 
 
 void ehp_geometry_destroy(ehp_geometry* g) { delete g; }
 
+namespace
+{
+    // Tile shape of the LDS-privatised Joseph back projection, shrunk until it fits the
+    // device's LDS per workgroup.
+    ehp::TileCfg default_tile(const ehp::Dims& dims, int max_lds)
+    {
+        ehp::TileCfg tc = dims.dim == 3 ? ehp::TileCfg{32, 15, 15} : ehp::TileCfg{64, 125, 1};
+        while (tc.LA > 1 && (int) ehp::joseph_tile_lds_bytes(dims, tc) > max_lds)
+            tc.LA /= 2;
+        return tc;
+    }
+
+    int setup_tiled(ehp_op* op)
+    {
+        const int n = op->dims.nposes;
+        hipError_t e = hipMalloc((void**) &op->d_pose_axes, n * sizeof(int));
+        if (e != hipSuccess)
+            return fail(EHP_ERR_ALLOC, "hipMalloc of the pose axis table failed: %s", hipGetErrorString(e));
+        e = ehp::joseph_pose_axes(op->d_geo, op->dims, op->d_pose_axes, nullptr);
+        std::vector<int> h(n);
+        if (e == hipSuccess)
+            e = hipMemcpy(h.data(), op->d_pose_axes, n * sizeof(int), hipMemcpyDeviceToHost);
+        if (e != hipSuccess)
+            return hip_fail(e, "pose axis table");
+        op->axes_present = 0;
+        for (int v : h)
+            op->axes_present |= v;
+        int max_lds = 65536;
+        if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, op->device) != hipSuccess)
+            max_lds = 65536;
+        op->tile = default_tile(op->dims, max_lds);
+        op->tiled = true;
+        return EHP_OK;
+    }
+} // namespace
+
 int ehp_joseph_create(const ehp_geometry* g, int dtype, int flags, ehp_op** out)
 {
-    if (flags != 0)
-        return fail(EHP_ERR_ARGUMENT, "Joseph flags must be 0 (got %d)", flags);
+    if (flags & ~EHP_JOSEPH_SIMPLE_KERNELS)
+        return fail(EHP_ERR_ARGUMENT, "unknown Joseph flags 0x%x", flags);
     int rc = new_op(g, KIND_JOSEPH, dtype, out);
     if (rc != EHP_OK)
         return rc;
     rc = count_samples(*out);
+    if (rc == EHP_OK && dtype == EHP_F32 && !(flags & EHP_JOSEPH_SIMPLE_KERNELS))
+        rc = setup_tiled(*out);
     if (rc != EHP_OK) {
         ehp_destroy(*out);
         *out = nullptr;
@@ -280,6 +323,8 @@ void ehp_destroy(ehp_op* op)
         hipFree(op->d_geo);
     if (op->d_lut)
         hipFree(op->d_lut);
+    if (op->d_pose_axes)
+        hipFree(op->d_pose_axes);
     if (cur >= 0 && cur != op->device)
         hipSetDevice(cur);
     delete op;
@@ -323,7 +368,9 @@ static int apply(ehp_op* op, const void* in, void* out, void* stream, bool adjoi
     if (op->dtype == EHP_F32) {
         const float* x = (const float*) in;
         float* y = (float*) out;
-        if (op->kind == KIND_JOSEPH)
+        if (op->kind == KIND_JOSEPH && adjoint && op->tiled)
+            e = ehp::joseph_adjoint_tiled(op->d_geo, dm, op->d_pose_axes, op->axes_present, op->tile, x, y, s);
+        else if (op->kind == KIND_JOSEPH)
             e = adjoint ? ehp::joseph_adjoint<float>(op->d_geo, dm, x, y, s) : ehp::joseph_forward<float>(op->d_geo, dm, x, y, s);
         else if (op->kind == KIND_SIDDON)
             e = adjoint ? ehp::siddon_adjoint<float>(op->d_geo, dm, x, y, s) : ehp::siddon_forward<float>(op->d_geo, dm, x, y, s);

@@ -6,8 +6,16 @@
 // the GPU as in the CPU restatement. Where a fused multiply-add is meant it is written.
 #pragma once
 
-#include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define EHP_HD __host__ __device__
+#else
+// plain C++ build of the geometry (CPU checks of the kernel index logic)
+#include <math.h>
+#define EHP_HD
+#endif
 
 namespace ehp
 {
@@ -34,15 +42,15 @@ namespace ehp
         int64_t nrays;
     };
 
-    __host__ __device__ inline float minNum(float x, float y)
+    EHP_HD inline float minNum(float x, float y)
     {
         return (y != y) ? x : (x < y ? x : y);
     }
-    __host__ __device__ inline float maxNum(float x, float y)
+    EHP_HD inline float maxNum(float x, float y)
     {
         return (y != y) ? x : (x > y ? x : y);
     }
-    __host__ __device__ inline float sgnf(float v)
+    EHP_HD inline float sgnf(float v)
     {
         return v > 0.0f ? 1.0f : (v < 0.0f ? -1.0f : 0.0f);
     }
@@ -50,7 +58,7 @@ namespace ehp
     // Ray through the centre of pixel (u, v): origin C, direction normalize(Pinv h).
     // elsa/core/Descriptors/DetectorDescriptor.cpp:90-128.
     template <int D>
-    __device__ inline void make_ray(const PoseGeom& g, int u, int v, float* o, float* d)
+    EHP_HD inline void make_ray(const PoseGeom& g, int u, int v, float* o, float* d)
     {
         float h[3];
         h[0] = (float) u + 0.5f;
@@ -84,7 +92,7 @@ namespace ehp
     // Slab test against [0, vol) with elsa's NaN handling and hit rule.
     // elsa/projectors/Intersection.cpp:7-46.
     template <int D>
-    __device__ inline bool intersect(const float* o, const float* d, const int* vol, float& tmin_out,
+    EHP_HD inline bool intersect(const float* o, const float* d, const int* vol, float& tmin_out,
                                      float& tmax_out)
     {
         float invDir = 1.0f / d[0];
@@ -123,7 +131,7 @@ namespace ehp
     };
 
     template <int D>
-    __device__ inline void joseph_setup(const float* o, const float* d, const int* vol, JRay& jr)
+    EHP_HD inline void joseph_setup(const float* o, const float* d, const int* vol, JRay& jr)
     {
         int a = 0;
         float m = fabsf(d[0]);
@@ -188,7 +196,7 @@ namespace ehp
     };
 
     template <int D>
-    __device__ inline void joseph_sample(const JRay& jr, int s, JSample& js)
+    EHP_HD inline void joseph_sample(const JRay& jr, int s, JSample& js)
     {
         const float fs = (float) s;
 #pragma unroll
@@ -209,7 +217,7 @@ namespace ehp
     // 2D (F0,C1)(C0,F1); 3D (F0,F1,F2)(F0,C1,C2)(C0,F1,C2)(C0,C1,F2). A tap outside the
     // volume gets weight 0 and a clamped index.
     template <int D, class Fn>
-    __device__ inline void joseph_taps(const JSample& js, const int* vol, const int64_t* strides, Fn fn)
+    EHP_HD inline void joseph_taps(const JSample& js, const int* vol, const int64_t* strides, Fn fn)
     {
         auto tap = [&](int c0, int c1, int c2, float wt) {
             const int c[3] = {c0, c1, c2};
@@ -234,6 +242,7 @@ namespace ehp
         }
     }
 
+#if defined(__HIPCC__)
     // Hardware float/double atomic add (global_atomic_add_f32 / _f64 on gfx950).
     __device__ inline void atomic_add(float* p, float v)
     {
@@ -243,4 +252,5 @@ namespace ehp
     {
         unsafeAtomicAdd(p, v);
     }
+#endif
 } // namespace ehp

@@ -165,7 +165,10 @@ namespace ehp
                 idx += dims.strides[i] * (int64_t) cur[i];
             }
             float tExit = 0.0f;
-            while (inbox) {
+            // a ray crosses at most vol[0] + vol[1] + vol[2] voxels; the cap only matters for
+            // degenerate (non-finite) rays, which would otherwise never leave the loop
+            int guard = dims.vol[0] + dims.vol[1] + dims.vol[2] + 2;
+            while (inbox && guard-- > 0) {
                 const float tEntry = tExit;
                 float tm = Tn[0];
 #pragma unroll
@@ -526,5 +529,6 @@ namespace ehp
 
     EHP_INSTANTIATE(float)
     EHP_INSTANTIATE(double)
+    template hipError_t launch_fill<int>(int*, int64_t, int, hipStream_t);
 #undef EHP_INSTANTIATE
 } // namespace ehp

@@ -149,12 +149,17 @@ class JosephsMethodCUDA(_HipProjector):
     (JosephsMethod.cpp:140-198) and its matched adjoint. ``fast`` is accepted for
     interface compatibility; the adjoint is always the matched one."""
 
-    def __init__(self, domainDescriptor, rangeDescriptor, fast: bool = True, dtype=np.float32):
+    def __init__(self, domainDescriptor, rangeDescriptor, fast: bool = True, dtype=np.float32, kernels: str = "tiled"):
         self.fast = bool(fast)
+        if kernels not in ("tiled", "simple"):
+            raise ValueError("kernels must be 'tiled' or 'simple'")
+        self.kernels = kernels
         super().__init__(domainDescriptor, rangeDescriptor, dtype)
 
     def _create(self, geo):
-        _lib.check(self._L.ehp_joseph_create(geo, _DT[self._dtype], 0, ctypes.byref(self._op)), "JosephsMethodCUDA")
+        flags = _lib.EHP_JOSEPH_SIMPLE_KERNELS if self.kernels == "simple" else 0
+        _lib.check(self._L.ehp_joseph_create(geo, _DT[self._dtype], flags, ctypes.byref(self._op)),
+                   "JosephsMethodCUDA")
 
     def ray_info(self):
         """Driving axis and sample count of every ray as the device computes them."""

@@ -48,6 +48,12 @@ enum ehp_status {
     EHP_ERR_UNSUPPORTED = 4  /* valid request this build does not implement */
 };
 
+/* flags of ehp_joseph_create */
+enum ehp_joseph_flags {
+    EHP_JOSEPH_SIMPLE_KERNELS = 1 /* one thread per ray in both directions (atomic back
+                                     projection); the reference point for the tiled kernels */
+};
+
 enum ehp_dtype {
     EHP_F32 = 0,  /* data_t = float  */
     EHP_F64 = 1   /* data_t = double */
@@ -72,7 +78,8 @@ void ehp_geometry_destroy(ehp_geometry* g);
 
 /* Joseph's method (elsa: JosephsMethodCUDA(volume, detector, fast), JosephsMethodCUDA.h:78-79,
  * with the discretization of the CPU JosephsMethod, JosephsMethod.cpp:140-198). The adjoint
- * is always the matched one; flags are reserved and must be 0. */
+ * is always the matched one (elsa's fast=false); elsa's unmatched fast=true back projection
+ * has no counterpart. flags: 0 or EHP_JOSEPH_SIMPLE_KERNELS. */
 int ehp_joseph_create(const ehp_geometry* g, int dtype, int flags, ehp_op** out);
 
 /* Siddon's method (elsa: SiddonsMethodCUDA(volume, detector), SiddonsMethodCUDA.h:71-72,

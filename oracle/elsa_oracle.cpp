@@ -354,7 +354,9 @@ namespace orc
                 idx += strides[i] * static_cast<i64>(cur[i]);
             }
             R tExit = R(0);
-            while (inbox) {
+            // at most dims[0] + dims[1] + dims[2] crossings; the cap only stops non-finite rays
+            i64 guard = dims[0] + (D > 1 ? dims[1] : 0) + (D > 2 ? dims[2] : 0) + 2;
+            while (inbox && guard-- > 0) {
                 const R tEntry = tExit;
                 R tm = T_[0];
                 for (int i = 1; i < D; ++i)

@@ -6,8 +6,10 @@ Bars:
     CPU restatement): bit-exact in FP32;
   * Joseph against the oracle evaluating sample positions the same way (pos0 + s*step):
     rel. L2 <= 1e-6 (only the summation order differs);
-  * Joseph against elsa's own rule (repeated addition of the step): rel. L2 <= 2e-5, and the
-    GPU's error against the FP64 oracle is no larger than elsa-FP32's;
+  * Joseph against elsa's own rule (repeated addition of the step): rel. L2 <= 1e-4 (the
+    accumulated step drifts by up to ~1e-4 voxel after 256 steps in FP32; measured at most
+    5.4e-5 on these cases), and the GPU's error against the FP64 oracle is no larger than
+    elsa-FP32's;
   * atomically accumulated back projections (Joseph, Siddon) and the voxel forward:
     rel. L2 <= 1e-6 against the oracle;
   * matched adjoints: |<Ax,y> - <x,A^T y>| / (|Ax| |y|) <= 1e-6 in FP32, 1e-12 in FP64.
@@ -125,7 +127,7 @@ def test_joseph_forward_parity(name, ops):
     elsa = oracle.joseph(g, x, mode=oracle.MODE_ELSA)
     ref64 = oracle.joseph(make_case(name, np.float64), x.astype(np.float64), mode=oracle.MODE_DIRECT)
     assert rel_l2(gpu, direct) <= 1e-6, rel_l2(gpu, direct)
-    assert rel_l2(gpu, elsa) <= 2e-5, rel_l2(gpu, elsa)
+    assert rel_l2(gpu, elsa) <= 1e-4, rel_l2(gpu, elsa)
     # error budget against FP64: the GPU is at least as close as elsa's FP32 arithmetic
     assert rel_l2(gpu, ref64) <= max(1.05 * rel_l2(elsa, ref64), 1e-6), (rel_l2(gpu, ref64), rel_l2(elsa, ref64))
 
@@ -137,8 +139,10 @@ def test_joseph_adjoint_parity(name, ops):
     gpu = ops("JosephsMethodCUDA", name).applyAdjoint(y)
     direct = oracle.joseph(g, y, adjoint=True, mode=oracle.MODE_DIRECT)
     elsa = oracle.joseph(g, y, adjoint=True, mode=oracle.MODE_ELSA)
+    ref64 = oracle.joseph(make_case(name, np.float64), y.astype(np.float64), adjoint=True, mode=oracle.MODE_DIRECT)
     assert rel_l2(gpu, direct) <= 1e-6, rel_l2(gpu, direct)
-    assert rel_l2(gpu, elsa) <= 2e-5, rel_l2(gpu, elsa)
+    assert rel_l2(gpu, elsa) <= 1e-4, rel_l2(gpu, elsa)
+    assert rel_l2(gpu, ref64) <= max(1.05 * rel_l2(elsa, ref64), 1e-6), (rel_l2(gpu, ref64), rel_l2(elsa, ref64))
 
 
 @pytest.mark.parametrize("name", list(CASES))
@@ -236,6 +240,20 @@ def test_views_of_a_larger_buffer(ops):
     torch.cuda.synchronize()
     assert float(big[:n_sino].abs().sum()) == 0.0 and float(big[2 * n_sino:].abs().sum()) == 0.0
     np.testing.assert_allclose(view.cpu().numpy(), op.apply(x), rtol=0, atol=0)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_joseph_tiled_equals_simple_kernels(name, ops):
+    """The LDS-tiled back projection and the one-thread-per-ray atomic one compute the same
+    sums of the same bit-identical terms."""
+    g = make_case(name)
+    y = phantom(g.sino_shape, 11)
+    tiled = ops("JosephsMethodCUDA", name).applyAdjoint(y)
+    simple = ops("JosephsMethodCUDA", name, kernels="simple").applyAdjoint(y)
+    assert rel_l2(tiled, simple) <= 1e-6
+    x = phantom(g.vol_shape, 12)
+    np.testing.assert_array_equal(ops("JosephsMethodCUDA", name).apply(x),
+                                  ops("JosephsMethodCUDA", name, kernels="simple").apply(x))
 
 
 def test_size_mismatch_raises(ops):
