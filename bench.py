@@ -50,6 +50,7 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--kernels", default="tiled", choices=["tiled", "simple"],
                    help="Joseph kernels: LDS-tiled back projection (default) or one thread per ray")
+    p.add_argument("--tile", default="", help="LA,TB,TC of the tiled back projection (default: built-in)")
     return p.parse_args()
 
 
@@ -90,6 +91,9 @@ def main():
     dd = E.CircleTrajectoryGenerator.createTrajectory(wl["views"], vd, 360, wl["s2c"], wl["c2d"],
                                                       detectorSize=list(wl["det"]))
     op = E.JosephsMethodCUDA(vd, dd, kernels=a.kernels)
+    tile = None
+    if a.kernels == "tiled":
+        tile = op.tile(*[int(v) for v in a.tile.split(",")]) if a.tile else op.tile()
     S = op.sample_count()
     R = int(np.prod(wl["det"])) * wl["views"]
     x_np = modified_shepp_logan(wl["vol"][::-1])
@@ -140,7 +144,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (modified Shepp-Logan phantom)",
-        "config": {"workload": wl["text"], "kernels": a.kernels, "volume": list(wl["vol"]), "poses": wl["views"],
+        "config": {"workload": wl["text"], "kernels": a.kernels, "tile": tile, "volume": list(wl["vol"]), "poses": wl["views"],
                    "detector": list(wl["det"]), "samples_per_pass": S, "rays": R,
                    "fwd_ms": round(t_fwd * 1e3, 3), "adj_ms": round(t_adj * 1e3, 3)},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

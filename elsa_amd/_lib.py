@@ -19,7 +19,7 @@ EHP_JOSEPH_SIMPLE_KERNELS = 1
 EXPORTS = (
     "ehp_last_error", "ehp_version", "ehp_geometry_create", "ehp_geometry_destroy",
     "ehp_joseph_create", "ehp_siddon_create", "ehp_voxel_create", "ehp_destroy", "ehp_forward",
-    "ehp_adjoint", "ehp_op_sizes", "ehp_sample_count", "ehp_joseph_ray_info", "ehp_geometry_pose",
+    "ehp_adjoint", "ehp_op_sizes", "ehp_sample_count", "ehp_joseph_tile", "ehp_joseph_ray_info", "ehp_geometry_pose",
     "ehp_circle_trajectory", "ehp_blob_lut", "ehp_bspline_lut",
 )
 
@@ -58,6 +58,8 @@ def load():
     L.ehp_op_sizes.argtypes = [vp, i64p, i64p]
     L.ehp_sample_count.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
     L.ehp_joseph_ray_info.argtypes = [vp, vp, vp]
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.ehp_joseph_tile.argtypes = [vp, c_int, c_int, c_int, ip, ip, ip]
     L.ehp_geometry_pose.argtypes = [c_int, c_f, c_f, fp, fp, fp, fp, c_f, c_f, c_f, fp, fp, fp, fp, fp, fp, fp]
     L.ehp_circle_trajectory.argtypes = [c_int, i64, i64p, c_f, c_f, c_f, i64p, i64p, fp, fp, fp, fp, fp]
     L.ehp_blob_lut.argtypes = [c_int, c_d, c_d, i64, c_int, vp, ctypes.POINTER(c_d)]

@@ -38,6 +38,12 @@ struct ehp_op {
     int* d_pose_axes = nullptr;
     int axes_present = 0;
     ehp::TileCfg tile{};
+    ehp::RayRec* d_table = nullptr;   // per-ray traversal state
+    unsigned int* d_maxbits = nullptr; // max|y| of the current back projection
+    float* d_scale = nullptr;          // fixed-point scale q, 1/q
+    double slot_bound = 0.0;
+    int batch = ehp::kPoseBatch;
+    std::vector<ehp::PoseGeom> host_geo;
 };
 
 namespace
@@ -108,10 +114,11 @@ namespace
         }
         e = hipMemcpy(op->d_geo, g->host.data(), bytes, hipMemcpyHostToDevice);
         if (e != hipSuccess) {
-            hipFree(op->d_geo);
+            (void) hipFree(op->d_geo);
             delete op;
             return hip_fail(e, "geometry upload");
         }
+        op->host_geo = g->host;
         *out = op;
         return EHP_OK;
     }
@@ -135,7 +142,7 @@ namespace
         unsigned long long h = 0;
         if (e == hipSuccess)
             e = hipMemcpy(&h, d_total, sizeof(h), hipMemcpyDeviceToHost);
-        hipFree(d_total);
+        (void) hipFree(d_total);
         if (e != hipSuccess)
             return hip_fail(e, "sample count");
         op->samples = h;
@@ -217,7 +224,7 @@ namespace
     // device's LDS per workgroup.
     ehp::TileCfg default_tile(const ehp::Dims& dims, int max_lds)
     {
-        ehp::TileCfg tc = dims.dim == 3 ? ehp::TileCfg{32, 15, 15} : ehp::TileCfg{64, 125, 1};
+        ehp::TileCfg tc = dims.dim == 3 ? ehp::TileCfg{16, 15, 15} : ehp::TileCfg{64, 125, 1};
         while (tc.LA > 1 && (int) ehp::joseph_tile_lds_bytes(dims, tc) > max_lds)
             tc.LA /= 2;
         return tc;
@@ -225,23 +232,40 @@ namespace
 
     int setup_tiled(ehp_op* op)
     {
-        const int n = op->dims.nposes;
-        hipError_t e = hipMalloc((void**) &op->d_pose_axes, n * sizeof(int));
+        const ehp::Dims& dm = op->dims;
+        // the bound keeps the 64-bit fixed-point sums in range with a resolution of at least
+        // 2^-30 max|y|
+        const double bound = ehp::fixed_point_slot_bound(dm, op->host_geo.data());
+        if (!(bound > 0.0) || bound > 4294967296.0)
+            return EHP_OK; // keep the one-thread-per-ray kernels
+        op->slot_bound = bound;
+        const int n = dm.nposes;
+        const size_t tbytes = (size_t) dm.nrays * sizeof(ehp::RayRec);
+        hipError_t e = hipMalloc((void**) &op->d_table, tbytes);
         if (e != hipSuccess)
-            return fail(EHP_ERR_ALLOC, "hipMalloc of the pose axis table failed: %s", hipGetErrorString(e));
-        e = ehp::joseph_pose_axes(op->d_geo, op->dims, op->d_pose_axes, nullptr);
+            return fail(EHP_ERR_ALLOC, "hipMalloc of the %zu-byte ray table failed: %s", tbytes, hipGetErrorString(e));
+        e = hipMalloc((void**) &op->d_pose_axes, n * sizeof(int));
+        if (e == hipSuccess)
+            e = hipMalloc((void**) &op->d_maxbits, sizeof(unsigned int));
+        if (e == hipSuccess)
+            e = hipMalloc((void**) &op->d_scale, 2 * sizeof(float));
+        if (e != hipSuccess)
+            return fail(EHP_ERR_ALLOC, "hipMalloc of the tiled back projection state failed: %s", hipGetErrorString(e));
+        e = ehp::joseph_ray_table(op->d_geo, dm, op->d_table, nullptr);
+        if (e == hipSuccess)
+            e = ehp::joseph_pose_axes(op->d_table, dm, op->d_pose_axes, nullptr);
         std::vector<int> h(n);
         if (e == hipSuccess)
             e = hipMemcpy(h.data(), op->d_pose_axes, n * sizeof(int), hipMemcpyDeviceToHost);
         if (e != hipSuccess)
-            return hip_fail(e, "pose axis table");
+            return hip_fail(e, "ray table");
         op->axes_present = 0;
         for (int v : h)
             op->axes_present |= v;
         int max_lds = 65536;
         if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, op->device) != hipSuccess)
             max_lds = 65536;
-        op->tile = default_tile(op->dims, max_lds);
+        op->tile = default_tile(dm, max_lds);
         op->tiled = true;
         return EHP_OK;
     }
@@ -318,15 +342,13 @@ void ehp_destroy(ehp_op* op)
     int cur = -1;
     hipGetDevice(&cur);
     if (cur != op->device)
-        hipSetDevice(op->device);
-    if (op->d_geo)
-        hipFree(op->d_geo);
-    if (op->d_lut)
-        hipFree(op->d_lut);
-    if (op->d_pose_axes)
-        hipFree(op->d_pose_axes);
+        (void) hipSetDevice(op->device);
+    for (void* p : {(void*) op->d_geo, op->d_lut, (void*) op->d_pose_axes, (void*) op->d_table, (void*) op->d_maxbits,
+                    (void*) op->d_scale})
+        if (p)
+            (void) hipFree(p);
     if (cur >= 0 && cur != op->device)
-        hipSetDevice(cur);
+        (void) hipSetDevice(cur);
     delete op;
 }
 
@@ -369,7 +391,8 @@ static int apply(ehp_op* op, const void* in, void* out, void* stream, bool adjoi
         const float* x = (const float*) in;
         float* y = (float*) out;
         if (op->kind == KIND_JOSEPH && adjoint && op->tiled)
-            e = ehp::joseph_adjoint_tiled(op->d_geo, dm, op->d_pose_axes, op->axes_present, op->tile, x, y, s);
+            e = ehp::joseph_adjoint_tiled(op->d_geo, op->d_table, dm, op->d_pose_axes, op->axes_present, op->tile,
+                                          op->batch, op->slot_bound, op->d_maxbits, op->d_scale, x, y, s);
         else if (op->kind == KIND_JOSEPH)
             e = adjoint ? ehp::joseph_adjoint<float>(op->d_geo, dm, x, y, s) : ehp::joseph_forward<float>(op->d_geo, dm, x, y, s);
         else if (op->kind == KIND_SIDDON)
@@ -403,6 +426,33 @@ int ehp_adjoint(ehp_op* op, const void* d_sino, void* d_vol, void* stream)
     return apply(op, d_sino, d_vol, stream, true);
 }
 
+int ehp_joseph_tile(ehp_op* op, int la, int tb, int tc, int* la_out, int* tb_out, int* tc_out)
+{
+    if (!op)
+        return fail(EHP_ERR_ARGUMENT, "operator is NULL");
+    if (op->kind != KIND_JOSEPH || !op->tiled)
+        return fail(EHP_ERR_ARGUMENT, "operator does not use the tiled Joseph kernels");
+    if (la > 0) {
+        if (tb <= 0 || (op->dims.dim == 3 && tc <= 0))
+            return fail(EHP_ERR_ARGUMENT, "tile extents must be positive");
+        ehp::TileCfg t{la, tb, op->dims.dim == 3 ? tc : 1};
+        int max_lds = 65536;
+        if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, op->device) != hipSuccess)
+            max_lds = 65536;
+        const size_t need = ehp::joseph_tile_lds_bytes(op->dims, t);
+        if ((int64_t) need > max_lds)
+            return fail(EHP_ERR_ARGUMENT, "tile needs %zu bytes of LDS, the device allows %d", need, max_lds);
+        op->tile = t;
+    }
+    if (la_out)
+        *la_out = op->tile.LA;
+    if (tb_out)
+        *tb_out = op->tile.TB;
+    if (tc_out)
+        *tc_out = op->tile.TC;
+    return EHP_OK;
+}
+
 int ehp_joseph_ray_info(const ehp_op* op, int32_t* axis, int64_t* nsteps)
 {
     if (!op || !axis || !nsteps)
@@ -416,7 +466,7 @@ int ehp_joseph_ray_info(const ehp_op* op, int32_t* axis, int64_t* nsteps)
     if (e == hipSuccess)
         e = hipMalloc((void**) &d_steps, n * sizeof(int64_t));
     if (e != hipSuccess) {
-        hipFree(d_axis);
+        (void) hipFree(d_axis);
         return fail(EHP_ERR_ALLOC, "hipMalloc for ray info failed: %s", hipGetErrorString(e));
     }
     e = ehp::joseph_info(op->d_geo, op->dims, d_axis, d_steps, nullptr, nullptr);
@@ -424,8 +474,8 @@ int ehp_joseph_ray_info(const ehp_op* op, int32_t* axis, int64_t* nsteps)
         e = hipMemcpy(axis, d_axis, n * sizeof(int32_t), hipMemcpyDeviceToHost);
     if (e == hipSuccess)
         e = hipMemcpy(nsteps, d_steps, n * sizeof(int64_t), hipMemcpyDeviceToHost);
-    hipFree(d_axis);
-    hipFree(d_steps);
+    (void) hipFree(d_axis);
+    (void) hipFree(d_steps);
     if (e != hipSuccess)
         return hip_fail(e, "ray info");
     return EHP_OK;

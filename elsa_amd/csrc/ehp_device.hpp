@@ -19,6 +19,20 @@
 
 namespace ehp
 {
+    // Approximate reciprocal (one rounding error of a few ulp at most) for quantities that are
+    // only bounds; exact quotients keep the division operator.
+    EHP_HD inline float ehp_rcp(float x)
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        return __builtin_amdgcn_rcpf(x);
+#else
+        return 1.0f / x;
+#endif
+    }
+} // namespace ehp
+
+namespace ehp
+{
     // Per-pose geometry as kernels read it. One 64-float record per pose so a wavefront can
     // read it with scalar loads. Column-major blocks as in elsa (Eigen).
     struct PoseGeom {
@@ -130,17 +144,27 @@ namespace ehp
         float len;
     };
 
+    // v[a] for a runtime a without dynamic indexing (keeps small arrays in registers).
+    template <int D>
+    EHP_HD inline float pick(const float* v, int a)
+    {
+        return (D == 3 && a == 2) ? v[D - 1] : (a == 1 ? v[1] : v[0]);
+    }
+
     template <int D>
     EHP_HD inline void joseph_setup(const float* o, const float* d, const int* vol, JRay& jr)
     {
+        // driving axis: first index of the largest |d| (Eigen's maxCoeff keeps the first)
         int a = 0;
         float m = fabsf(d[0]);
-#pragma unroll
-        for (int i = 1; i < D; ++i)
-            if (fabsf(d[i]) > m) {
-                m = fabsf(d[i]);
-                a = i;
-            }
+        if (fabsf(d[1]) > m) {
+            m = fabsf(d[1]);
+            a = 1;
+        }
+        if (D == 3 && fabsf(d[D - 1]) > m) {
+            m = fabsf(d[D - 1]);
+            a = 2;
+        }
         jr.axis = a;
         jr.nsteps = 0;
         jr.len = 0.0f;
@@ -163,19 +187,23 @@ namespace ehp
             ex[i] = ex[i] < 0.0f ? 0.0f : ex[i];
             ex[i] = ex[i] > hi ? hi : ex[i];
         }
-        const float ad = fabsf(d[a]);
+        const float ad = m; // |d[a]|
         float step[3];
 #pragma unroll
         for (int i = 0; i < D; ++i)
             step[i] = d[i] / ad;
-        const float dist = cur[a] - floorf(cur[a]);
-        const float s = sgnf(step[a]);
+        const float cur_a = pick<D>(cur, a);
+        const float dist = cur_a - floorf(cur_a);
+        const float s = sgnf(pick<D>(step, a));
 #pragma unroll
         for (int i = 0; i < D; ++i)
             cur[i] = cur[i] + step[i] * (0.5f - dist) * s;
-        cur[a] = floorf(cur[a]) + 0.5f;
-        jr.nsteps = (int) ceilf(fabsf(ex[a] - cur[a]));
-        const float n2 = (D == 3) ? step[0] * step[0] + (step[1] * step[1] + step[2] * step[2])
+        const float snapped = floorf(pick<D>(cur, a)) + 0.5f;
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+            cur[i] = (i == a) ? snapped : cur[i];
+        jr.nsteps = (int) ceilf(fabsf(pick<D>(ex, a) - snapped));
+        const float n2 = (D == 3) ? step[0] * step[0] + (step[1] * step[1] + step[D - 1] * step[D - 1])
                                   : step[0] * step[0] + step[1] * step[1];
         jr.len = sqrtf(n2);
 #pragma unroll
@@ -187,7 +215,8 @@ namespace ehp
 
     // Bilinear weights in the plane normal to the driving axis for sample s (position
     // pos0 + s*step, one FMA per transverse axis), as getLinearInterpolationWeights
-    // (JosephsMethod.cpp:37-52) evaluates them.
+    // (JosephsMethod.cpp:37-52) evaluates them. A >= 0 fixes the driving axis at compile
+    // time; A = -1 reads it from the ray.
     struct JSample {
         int fl[3];
         int cl[3];
@@ -195,22 +224,21 @@ namespace ehp
         float cw[3];
     };
 
-    template <int D>
+    template <int D, int A = -1>
     EHP_HD inline void joseph_sample(const JRay& jr, int s, JSample& js)
     {
+        const int ax = A >= 0 ? A : jr.axis;
         const float fs = (float) s;
 #pragma unroll
         for (int i = 0; i < D; ++i) {
-            const float p = (i == jr.axis) ? jr.pos0[i] + fs * jr.step[i]
-                                           : __builtin_fmaf(fs, jr.step[i], jr.pos0[i]);
+            const float p = (i == ax) ? jr.pos0[i] + fs * jr.step[i] : __builtin_fmaf(fs, jr.step[i], jr.pos0[i]);
             const float f = floorf(p - 0.5f);
             js.fl[i] = (int) f;
             js.cl[i] = (int) ceilf(p - 0.5f);
-            js.cw[i] = (p - f) - 0.5f;
-            js.w[i] = 1.0f - js.cw[i];
+            const float cw = (p - f) - 0.5f;
+            js.cw[i] = (i == ax) ? 1.0f : cw;
+            js.w[i] = (i == ax) ? 1.0f : 1.0f - cw;
         }
-        js.w[jr.axis] = 1.0f;
-        js.cw[jr.axis] = 1.0f;
     }
 
     // The taps of one sample in elsa's order (doInterpolation, JosephsMethod.cpp:60-97):

@@ -16,30 +16,68 @@ namespace ehp
         int TC;
     };
 
+    // Compact traversal state of one ray, built once per operator (joseph_ray_table): the
+    // position of sample 0 and the step along the two transverse axes b1 < b2 of the ray's own
+    // driving axis, the number of samples, the driving axis, the sign of its step, the plane
+    // of sample 0 and the step length. Sample s lies on plane P0 + s*sign at transverse
+    // position fma(s, step, pos0) -- the same arithmetic as joseph_sample.
+    struct alignas(16) RayRec {
+        float p1, p2, s1, s2;
+        int nsteps;
+        int meta; // driving axis in bits 0-1, bit 2 set when the step along it is negative
+        int P0;
+        float len;
+    };
+    static_assert(sizeof(RayRec) == 32, "RayRec must be 32 bytes");
+
+    template <int D>
+    EHP_HD inline RayRec make_ray_rec(const JRay& jr)
+    {
+        const int a = jr.axis;
+        RayRec r;
+        float p[3] = {jr.pos0[0], jr.pos0[1], jr.pos0[2]};
+        float st[3] = {jr.step[0], jr.step[1], jr.step[2]};
+        // transverse axes of a: (1, 2), (0, 2), (0, 1) in 3D; 1 - a in 2D
+        r.p1 = (a == 0) ? p[1] : p[0];
+        r.s1 = (a == 0) ? st[1] : st[0];
+        r.p2 = (D == 3) ? ((a == 2) ? p[1] : p[D - 1]) : 0.0f;
+        r.s2 = (D == 3) ? ((a == 2) ? st[1] : st[D - 1]) : 0.0f;
+        r.nsteps = jr.nsteps;
+        r.meta = a | ((pick<D>(st, a) < 0.0f) ? 4 : 0);
+        r.P0 = (int) (pick<D>(p, a) - 0.5f);
+        r.len = jr.len;
+        return r;
+    }
+
     // One tile of one pass: planes [A0, A1) along the driving axis a, transverse voxel ranges
     // [B0, B1) along b1 and [C0, C1) along b2 (b1 < b2). The LDS block holds the planes of
     // the tile and the transverse ranges [B0 - 1, B0 + TB] x [C0 - 1, C0 + TC] (one slot of
     // halo on either side), b1 fastest: slot(pa, fc, fb) = (pa * PC + fc) * PB + fb.
     struct TileBox {
-        int a, b1, b2;
         int A0, A1, B0, B1, C0, C1;
         int PB, PC;
         int nslots;
     };
 
-    template <int D>
-    EHP_HD inline TileBox make_tile(const Dims& dims, const TileCfg& tc, int a, int ia, int ib, int ic)
+    // The transverse axes b1 < b2 of driving axis A (compile-time, so that per-axis arrays are
+    // indexed statically and stay in registers).
+    template <int D, int A>
+    struct TileAxes {
+        static constexpr int b1 = (D == 2) ? 1 - A : (A == 0 ? 1 : 0);
+        static constexpr int b2 = (D == 2) ? 0 : (A == 2 ? 1 : 2);
+    };
+
+    template <int D, int A>
+    EHP_HD inline TileBox make_tile(const Dims& dims, const TileCfg& tc, int ia, int ib, int ic)
     {
+        constexpr int b1 = TileAxes<D, A>::b1, b2 = TileAxes<D, A>::b2;
         TileBox t;
-        t.a = a;
-        t.b1 = (D == 2) ? 1 - a : (a == 0 ? 1 : 0);
-        t.b2 = (D == 2) ? 0 : (a == 2 ? 1 : 2);
         t.A0 = ia * tc.LA;
-        t.A1 = dims.vol[a] < t.A0 + tc.LA ? dims.vol[a] : t.A0 + tc.LA;
+        t.A1 = dims.vol[A] < t.A0 + tc.LA ? dims.vol[A] : t.A0 + tc.LA;
         t.B0 = ib * tc.TB;
-        t.B1 = dims.vol[t.b1] < t.B0 + tc.TB ? dims.vol[t.b1] : t.B0 + tc.TB;
+        t.B1 = dims.vol[b1] < t.B0 + tc.TB ? dims.vol[b1] : t.B0 + tc.TB;
         t.C0 = (D == 3) ? ic * tc.TC : 0;
-        t.C1 = (D == 3) ? (dims.vol[t.b2] < t.C0 + tc.TC ? dims.vol[t.b2] : t.C0 + tc.TC) : 1;
+        t.C1 = (D == 3) ? (dims.vol[b2] < t.C0 + tc.TC ? dims.vol[b2] : t.C0 + tc.TC) : 1;
         t.PB = tc.TB + 2;
         t.PC = (D == 3) ? tc.TC + 2 : 1;
         t.nslots = tc.LA * t.PC * t.PB;
@@ -50,18 +88,19 @@ namespace ehp
     // of [A0, A1] x [B0 - 1, B1 + 1] x [C0 - 1, C1 + 1], widened by one pixel. A point X lies on
     // the ray of pixel u when P X (homogeneous) maps to u + 0.5. Falls back to the whole
     // detector when a corner is not in front of the source.
-    template <int D>
+    template <int D, int A>
     EHP_HD inline bool tile_bbox(const PoseGeom& g, const TileBox& t, const Dims& dims, int& u0, int& u1, int& v0,
                                  int& v1)
     {
+        constexpr int b1 = TileAxes<D, A>::b1, b2 = TileAxes<D, A>::b2;
         float lo[3], hi[3];
-        lo[t.a] = (float) t.A0;
-        hi[t.a] = (float) t.A1;
-        lo[t.b1] = (float) (t.B0 - 1);
-        hi[t.b1] = (float) (t.B1 + 1);
+        lo[A] = (float) t.A0;
+        hi[A] = (float) t.A1;
+        lo[b1] = (float) (t.B0 - 1);
+        hi[b1] = (float) (t.B1 + 1);
         if (D == 3) {
-            lo[t.b2] = (float) (t.C0 - 1);
-            hi[t.b2] = (float) (t.C1 + 1);
+            lo[b2] = (float) (t.C0 - 1);
+            hi[b2] = (float) (t.C1 + 1);
         }
         float umin = 3.0e38f, umax = -3.0e38f, vmin = 3.0e38f, vmax = -3.0e38f;
         bool front = true;
@@ -78,11 +117,13 @@ namespace ehp
             }
             const float den = p[D - 1];
             front = front && (den > 0.0f);
-            const float pu = p[0] / den;
+            // approximate reciprocal: the rectangle is widened by a pixel anyway
+            const float rden = ehp_rcp(den);
+            const float pu = p[0] * rden;
             umin = pu < umin ? pu : umin;
             umax = pu > umax ? pu : umax;
             if (D == 3) {
-                const float pv = p[1] / den;
+                const float pv = p[1] * rden;
                 vmin = pv < vmin ? pv : vmin;
                 vmax = pv > vmax ? pv : vmax;
             }
@@ -123,7 +164,9 @@ namespace ehp
                 shi = slo - 1;
             return;
         }
-        float lo = (L - p0) / st, hi = (H - p0) / st;
+        // approximate reciprocal: the range keeps one sample of slack on either side
+        const float rst = ehp_rcp(st);
+        float lo = (L - p0) * rst, hi = (H - p0) * rst;
         if (st < 0.0f) {
             const float t = lo;
             lo = hi;
@@ -137,58 +180,60 @@ namespace ehp
         shi = shi < h ? shi : h;
     }
 
-    // Samples [slo, shi] of a ray (driving axis a) that can touch the tile's voxels.
-    EHP_HD inline void tile_sample_range(const JRay& jr, const TileBox& t, int D, int& slo, int& shi)
+    // Samples [slo, shi] of a ray (driving axis A) that can touch the tile's voxels.
+    EHP_HD inline void tile_sample_range(const RayRec& r, const TileBox& t, int D, int& slo, int& shi)
     {
-        const int P0 = (int) (jr.pos0[t.a] - 0.5f);
-        if (jr.step[t.a] > 0.0f) {
-            slo = t.A0 - P0;
-            shi = t.A1 - 1 - P0;
+        if (!(r.meta & 4)) {
+            slo = t.A0 - r.P0;
+            shi = t.A1 - 1 - r.P0;
         } else {
-            slo = P0 - (t.A1 - 1);
-            shi = P0 - t.A0;
+            slo = r.P0 - (t.A1 - 1);
+            shi = r.P0 - t.A0;
         }
         slo = slo > 0 ? slo : 0;
-        shi = shi < jr.nsteps - 1 ? shi : jr.nsteps - 1;
-        clip_range(jr.pos0[t.b1], jr.step[t.b1], (float) t.B0 - 0.5f, (float) t.B1 + 0.5f, slo, shi);
+        shi = shi < r.nsteps - 1 ? shi : r.nsteps - 1;
+        clip_range(r.p1, r.s1, (float) t.B0 - 0.5f, (float) t.B1 + 0.5f, slo, shi);
         if (D == 3)
-            clip_range(jr.pos0[t.b2], jr.step[t.b2], (float) t.C0 - 0.5f, (float) t.C1 + 0.5f, slo, shi);
+            clip_range(r.p2, r.s2, (float) t.C0 - 0.5f, (float) t.C1 + 0.5f, slo, shi);
     }
 
-    // Back projection of one ray into the tile's LDS block: for every sample whose bilinear
-    // footprint touches the tile, add y*len*weight to its four (2D: two) slots through add(slot,
-    // value). Weights are formed as in elsa's doInterpolation (JosephsMethod.cpp:60-97): the
-    // product of the transverse weights in axis order, the driving weight being 1.
-    template <int D, class Add>
-    EHP_HD inline void tile_ray_adjoint(const PoseGeom& g, const Dims& dims, const TileBox& t, int u, int v, float yv,
-                                        Add add)
+    // Back projection of one ray (driving axis A) into a tile: for every sample whose bilinear
+    // footprint touches the tile, hand y*len*weight for each of its four (2D: two) slots to
+    // add(slot, value). Weights are formed as in elsa's doInterpolation
+    // (JosephsMethod.cpp:60-97): the product of the transverse weights in axis order, the
+    // driving weight being 1.
+    template <int D, int A, class Add>
+    EHP_HD inline void tile_ray_adjoint(const RayRec& r, const TileBox& t, float yv, Add add)
     {
-        float o[3], d[3];
-        make_ray<D>(g, u, v, o, d);
-        JRay jr;
-        joseph_setup<D>(o, d, dims.vol, jr);
-        if (jr.axis != t.a || jr.nsteps == 0)
+        if ((r.meta & 3) != A || r.nsteps == 0)
             return;
         int slo, shi;
-        tile_sample_range(jr, t, D, slo, shi);
-        const float yl = yv * jr.len;
+        tile_sample_range(r, t, D, slo, shi);
+        const float yl = yv * r.len;
+        const int sg = (r.meta & 4) ? -1 : 1;
         for (int s = slo; s <= shi; ++s) {
-            JSample js;
-            joseph_sample<D>(jr, s, js);
-            const int f1 = js.fl[t.b1];
+            const float fs = (float) s;
+            const float q1 = __builtin_fmaf(fs, r.s1, r.p1);
+            const float g1 = floorf(q1 - 0.5f);
+            const int f1 = (int) g1;
             if (f1 < t.B0 - 1 || f1 > t.B1 - 1)
                 continue;
-            const int pa = js.fl[t.a] - t.A0;
+            const float c1 = (q1 - g1) - 0.5f;
+            const float w1 = 1.0f - c1;
+            const int pa = r.P0 + sg * s - t.A0;
             if (D == 2) {
                 const int base = pa * t.PB + (f1 - (t.B0 - 1));
-                add(base, yl * js.w[t.b1]);
-                add(base + 1, yl * js.cw[t.b1]);
+                add(base, yl * w1);
+                add(base + 1, yl * c1);
             } else {
-                const int f2 = js.fl[t.b2];
+                const float q2 = __builtin_fmaf(fs, r.s2, r.p2);
+                const float g2 = floorf(q2 - 0.5f);
+                const int f2 = (int) g2;
                 if (f2 < t.C0 - 1 || f2 > t.C1 - 1)
                     continue;
+                const float c2 = (q2 - g2) - 0.5f;
+                const float w2 = 1.0f - c2;
                 const int base = (pa * t.PC + (f2 - (t.C0 - 1))) * t.PB + (f1 - (t.B0 - 1));
-                const float w1 = js.w[t.b1], c1 = js.cw[t.b1], w2 = js.w[t.b2], c2 = js.cw[t.b2];
                 add(base, yl * (w1 * w2));
                 add(base + 1, yl * (c1 * w2));
                 add(base + t.PB, yl * (w1 * c2));
@@ -198,9 +243,10 @@ namespace ehp
     }
 
     // Linear volume index of LDS slot i if the tile owns that voxel, else -1.
-    template <int D>
+    template <int D, int A>
     EHP_HD inline int64_t tile_slot_voxel(const TileBox& t, const Dims& dims, int i)
     {
+        constexpr int b1 = TileAxes<D, A>::b1, b2 = TileAxes<D, A>::b2;
         const int fb = i % t.PB;
         const int rest = i / t.PB;
         const int fc = rest % t.PC;
@@ -213,11 +259,101 @@ namespace ehp
         if (D == 3 && (xc < t.C0 || xc >= t.C1))
             return -1;
         int c[3] = {0, 0, 0};
-        c[t.a] = xa;
-        c[t.b1] = xb;
+        c[A] = xa;
+        c[b1] = xb;
         if (D == 3)
-            c[t.b2] = xc;
+            c[b2] = xc;
         return c[0] * dims.strides[0] + c[1] * dims.strides[1] + c[2] * dims.strides[2];
+    }
+
+    // Poses whose candidate rays one workgroup gathers between two barriers (at most
+    // kMaxPoseBatch).
+    constexpr int kMaxPoseBatch = 32;
+    constexpr int kPoseBatch = 8;
+
+    // Upper bound of sum |len * weight| over the contributions one LDS slot can receive from
+    // all poses. Per pose a slot receives one sample of each ray whose crossing point with the
+    // slot's plane lies within the slot's 2x2 (2D: 2) interpolation footprint; with
+    // neighbouring rays at least h apart there are at most (2/h + 2)^(dim-1) of them, each with
+    // weight <= 1. h is bounded below by half the smallest angle between neighbouring pixel
+    // rays (sampled at the detector corners and centre) times the smallest distance from the
+    // source to the volume. Returns a negative value when no useful bound exists (source inside
+    // or touching the volume); the caller then uses the one-thread-per-ray kernels.
+    inline double fixed_point_slot_bound(const Dims& dims, const PoseGeom* poses)
+    {
+        const int D = dims.dim;
+        double theta = 1.0e300, dmin = 1.0e300;
+        for (int p = 0; p < dims.nposes; ++p) {
+            const PoseGeom& g = poses[p];
+            auto dir = [&](double u, double v, double* n) {
+                const double h[3] = {u, D == 3 ? v : 1.0, 1.0};
+                double y[3] = {0, 0, 0}, n2 = 0;
+                for (int i = 0; i < D; ++i) {
+                    for (int j = 0; j < D; ++j)
+                        y[i] += (double) g.Pinv[i + j * (D + 1)] * h[j];
+                    n2 += y[i] * y[i];
+                }
+                const double nn = n2 > 0 ? 1.0 / sqrt(n2) : 0.0;
+                for (int i = 0; i < D; ++i)
+                    n[i] = y[i] * nn;
+            };
+            const double W = dims.det[0], H = dims.det[1];
+            const double us[3] = {0.5, W - 0.5, 0.5 * W};
+            const double vs[3] = {0.5, H - 0.5, 0.5 * H};
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < (D == 3 ? 3 : 1); ++b) {
+                    double n0[3], n1[3], n2[3];
+                    dir(us[a], vs[b], n0);
+                    dir(us[a] + 1.0, vs[b], n1);
+                    double d1 = 0, d2 = 0;
+                    for (int i = 0; i < D; ++i)
+                        d1 += (n1[i] - n0[i]) * (n1[i] - n0[i]);
+                    theta = sqrt(d1) < theta ? sqrt(d1) : theta;
+                    if (D == 3) {
+                        dir(us[a], vs[b] + 1.0, n2);
+                        for (int i = 0; i < D; ++i)
+                            d2 += (n2[i] - n0[i]) * (n2[i] - n0[i]);
+                        theta = sqrt(d2) < theta ? sqrt(d2) : theta;
+                    }
+                }
+            double dist2 = 0;
+            for (int i = 0; i < D; ++i) {
+                const double c = g.C[i];
+                const double lo = 0.0, hi = dims.vol[i];
+                const double e = c < lo ? lo - c : (c > hi ? c - hi : 0.0);
+                dist2 += e * e;
+            }
+            const double dd = sqrt(dist2);
+            dmin = dd < dmin ? dd : dmin;
+        }
+        const double h = 0.5 * theta * dmin;
+        if (!(h > 1.0e-3))
+            return -1.0;
+        const double per_pose = D == 3 ? (2.0 / h + 2.0) * (2.0 / h + 2.0) : (2.0 / h + 2.0);
+        const double len_max = D == 3 ? 1.7320509 : 1.4142137; // |step| with one component 1
+        return dims.nposes * per_pose * len_max;
+    }
+
+    // Fixed-point scale of a back projection with 64-bit integer sums: q = 2^(e - 62) with
+    // 2^e >= maxabs * bound, so no slot sum reaches 2^62 in magnitude; iq = 1/q exactly.
+    // maxabs = 0 gives q = 1; a non-finite maxabs gives q = NaN (the result turns NaN like a
+    // float accumulation would). bound <= 2^32 keeps both powers of two normal floats.
+    EHP_HD inline void fixed_point_scale(float maxabs, double bound, float& q, float& iq)
+    {
+        const double X = (double) maxabs * bound;
+        if (X == 0.0) {
+            q = 1.0f;
+            iq = 1.0f;
+        } else if (!(X <= 1.0e300)) {
+            q = nanf("");
+            iq = 0.0f;
+        } else {
+            int e;
+            frexp(X, &e); // X = f 2^e, f in [0.5, 1)
+            e = e < -60 ? -60 : e;
+            q = ldexpf(1.0f, e - 62);
+            iq = ldexpf(1.0f, 62 - e);
+        }
     }
 
     inline void tile_counts(const Dims& dims, const TileCfg& tc, int a, int& ntA, int& ntB, int& ntC)

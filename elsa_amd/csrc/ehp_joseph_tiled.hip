@@ -8,91 +8,32 @@
 // One launch per driving axis that occurs in the trajectory ("pass"); each voxel is owned by
 // exactly one tile per pass, so passes are plain read-modify-write.
 //
-// The per-ray state and the per-sample weights come from the same functions as the
-// one-thread-per-ray kernels (ehp_device.hpp, ehp_joseph_tile.hpp), so the weights are
-// bit-identical to the forward's; only the order of the additions differs.
+// Accumulation: LDS float atomics are slow on this part (a ds_add_f32 wave operation costs
+// about 15 times a ds_add_u64), so the tile is summed with 64-bit integer atomics in fixed
+// point, scale q = 2^k chosen per call from max|y| and a bound on what one slot can receive,
+// so that no sum can overflow. Integer sums do not depend on the order of the additions, so
+// the result is bitwise reproducible. Each contribution is rounded once, to a relative 2^-24
+// when it is formed in FP32 and to q/2 <= 2^-62 * max|y| * bound when it is quantised; the
+// bound is at most 2^32, so the sums are more accurate than a sequential FP32 accumulation.
+//
+// The per-ray state comes from a table built once per operator from the same functions as
+// the one-thread-per-ray kernels (make_ray, joseph_setup), and the per-sample weights use the
+// same arithmetic as joseph_sample, so they are bit-identical to the forward's.
 #include "ehp_kernels.hpp"
 
 namespace ehp
 {
     namespace
     {
-        constexpr int kViewBatch = 16; // poses whose candidate rays are flattened together
         constexpr int kThreads = 256;
     } // namespace
 
-    template <int D>
-    __global__ void __launch_bounds__(kThreads)
-        joseph_adj_tiled_kernel(const PoseGeom* __restrict__ geo, const Dims dims, const int* __restrict__ pose_axes,
-                                const int a, const TileCfg tc, const int ntA, const int ntB, const int ntC,
-                                const int tiles_per_xcd, const float* __restrict__ sino, float* __restrict__ vol)
-    {
-        extern __shared__ float acc[];
-        __shared__ int s_u0[kViewBatch], s_nu[kViewBatch], s_v0[kViewBatch], s_pose[kViewBatch];
-        __shared__ int s_cnt[kViewBatch], s_pref[kViewBatch + 1];
+    // ---------------------------------------------------------------------------------
+    // per-ray table
 
-        // consecutive workgroups land on different XCDs: give each XCD a contiguous block of tiles
-        const int tile = (int) (blockIdx.x % 8u) * tiles_per_xcd + (int) (blockIdx.x / 8u);
-        if (tile >= ntA * ntB * ntC)
-            return;
-        const int tid = threadIdx.x;
-        const TileBox t = make_tile<D>(dims, tc, a, tile % ntA, (tile / ntA) % ntB, tile / (ntA * ntB));
-        for (int i = tid; i < t.nslots; i += kThreads)
-            acc[i] = 0.0f;
-        __syncthreads();
-
-        for (int pbase = 0; pbase < dims.nposes; pbase += kViewBatch) {
-            if (tid < kViewBatch) {
-                const int p = pbase + tid;
-                int u0 = 0, u1 = -1, v0 = 0, v1 = -1;
-                int cnt = 0;
-                if (p < dims.nposes && (pose_axes[p] & (1 << a))) {
-                    if (tile_bbox<D>(geo[p], t, dims, u0, u1, v0, v1))
-                        cnt = (u1 - u0 + 1) * (v1 - v0 + 1);
-                }
-                s_u0[tid] = u0;
-                s_nu[tid] = u1 - u0 + 1;
-                s_v0[tid] = v0;
-                s_pose[tid] = p;
-                s_cnt[tid] = cnt;
-            }
-            __syncthreads();
-            if (tid == 0) {
-                s_pref[0] = 0;
-                for (int j = 0; j < kViewBatch; ++j)
-                    s_pref[j + 1] = s_pref[j] + s_cnt[j];
-            }
-            __syncthreads();
-            const int total = s_pref[kViewBatch];
-            for (int k = tid; k < total; k += kThreads) {
-                int j = 0;
-                while (k >= s_pref[j + 1])
-                    ++j;
-                const int loc = k - s_pref[j];
-                const int u = s_u0[j] + loc % s_nu[j];
-                const int v = s_v0[j] + loc / s_nu[j];
-                const int p = s_pose[j];
-                const float yv = sino[(int64_t) p * dims.pix_per_pose + (int64_t) v * dims.det[0] + u];
-                if (yv == 0.0f)
-                    continue;
-                tile_ray_adjoint<D>(geo[p], dims, t, u, v, yv, [&](int slot, float val) { atomicAdd(&acc[slot], val); });
-            }
-            __syncthreads();
-        }
-
-        // add the owned voxels of the tile to the volume
-        for (int i = tid; i < t.nslots; i += kThreads) {
-            const int64_t lin = tile_slot_voxel<D>(t, dims, i);
-            if (lin >= 0)
-                vol[lin] += acc[i];
-        }
-    }
-
-    // Which driving axes occur among the rays of each pose (bit a set when some ray that hits
-    // the volume drives along a).
     template <int D>
     __global__ void __launch_bounds__(256)
-        pose_axes_kernel(const PoseGeom* __restrict__ geo, const Dims dims, int* __restrict__ pose_axes)
+        joseph_ray_table_kernel(const PoseGeom* __restrict__ geo, const Dims dims, RayRec* __restrict__ table)
     {
         const int pose = blockIdx.y;
         const int64_t pix = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
@@ -104,33 +45,174 @@ namespace ehp
         make_ray<D>(geo[pose], u, v, o, d);
         JRay jr;
         joseph_setup<D>(o, d, dims.vol, jr);
-        if (jr.nsteps > 0)
-            atomicOr(pose_axes + pose, 1 << jr.axis);
+        table[(int64_t) pose * dims.pix_per_pose + pix] = make_ray_rec<D>(jr);
     }
 
-    hipError_t joseph_pose_axes(const PoseGeom* geo, const Dims& dims, int* pose_axes, hipStream_t s)
+    hipError_t joseph_ray_table(const PoseGeom* geo, const Dims& dims, RayRec* table, hipStream_t s)
+    {
+        const dim3 grid((unsigned) ((dims.pix_per_pose + 255) / 256), (unsigned) dims.nposes);
+        if (dims.dim == 2)
+            hipLaunchKernelGGL((joseph_ray_table_kernel<2>), grid, dim3(256), 0, s, geo, dims, table);
+        else
+            hipLaunchKernelGGL((joseph_ray_table_kernel<3>), grid, dim3(256), 0, s, geo, dims, table);
+        return hipGetLastError();
+    }
+
+    // ---------------------------------------------------------------------------------
+    // fixed-point scale of one back projection
+
+    __global__ void __launch_bounds__(256)
+        absmax_kernel(const float* __restrict__ x, int64_t n, unsigned int* __restrict__ out)
+    {
+        float m = 0.0f;
+        const int64_t stride = (int64_t) gridDim.x * blockDim.x;
+        for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+            const float a = fabsf(x[i]);
+            m = (a > m || a != a) ? a : m; // NaN wins
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const float o = __shfl_down(m, off, 64);
+            m = (o > m || o != o) ? o : m;
+        }
+        if ((threadIdx.x & 63) == 0)
+            atomicMax(out, __float_as_uint(m)); // |x| >= 0: the bit patterns order like the values (NaN last)
+    }
+
+    // scale[0] = q, scale[1] = 1/q for the slot bound of the operator
+    __global__ void scale_kernel(const unsigned int* __restrict__ maxbits, double bound, float* __restrict__ scale)
+    {
+        float q, iq;
+        fixed_point_scale(__uint_as_float(*maxbits), bound, q, iq);
+        scale[0] = q;
+        scale[1] = iq;
+    }
+
+    // ---------------------------------------------------------------------------------
+    // the tiled back projection
+
+    template <int D, int A>
+    __global__ void __launch_bounds__(kThreads)
+        joseph_adj_tiled_kernel(const PoseGeom* __restrict__ geo, const RayRec* __restrict__ table, const Dims dims,
+                                const int* __restrict__ pose_axes, const TileCfg tc, const int ntA, const int ntB,
+                                const int ntC, const int tiles_per_xcd, const int batch,
+                                const float* __restrict__ scale, const float* __restrict__ sino,
+                                float* __restrict__ vol)
+    {
+        extern __shared__ long long lds[];
+        __shared__ int s_u0[kMaxPoseBatch], s_nu[kMaxPoseBatch], s_v0[kMaxPoseBatch], s_pose[kMaxPoseBatch];
+        __shared__ int s_cnt[kMaxPoseBatch], s_pref[kMaxPoseBatch + 1];
+
+        // consecutive workgroups land on different XCDs: give each XCD a contiguous block of tiles
+        const int tile = (int) (blockIdx.x % 8u) * tiles_per_xcd + (int) (blockIdx.x / 8u);
+        if (tile >= ntA * ntB * ntC)
+            return;
+        const int tid = threadIdx.x;
+        const TileBox t = make_tile<D, A>(dims, tc, tile % ntA, (tile / ntA) % ntB, tile / (ntA * ntB));
+        long long* iacc = lds;
+        for (int i = tid; i < t.nslots; i += kThreads)
+            iacc[i] = 0;
+        const float q = scale[0];
+        const float iq = scale[1];
+        __syncthreads();
+
+        for (int pbase = 0; pbase < dims.nposes; pbase += batch) {
+            if (tid < batch) {
+                const int p = pbase + tid;
+                int u0 = 0, u1 = -1, v0 = 0, v1 = -1;
+                int cnt = 0;
+                if (p < dims.nposes && (pose_axes[p] & (1 << A))) {
+                    if (tile_bbox<D, A>(geo[p], t, dims, u0, u1, v0, v1))
+                        cnt = (u1 - u0 + 1) * (v1 - v0 + 1);
+                }
+                s_u0[tid] = u0;
+                s_nu[tid] = u1 - u0 + 1;
+                s_v0[tid] = v0;
+                s_pose[tid] = p;
+                s_cnt[tid] = cnt;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                s_pref[0] = 0;
+                for (int j = 0; j < batch; ++j)
+                    s_pref[j + 1] = s_pref[j] + s_cnt[j];
+            }
+            __syncthreads();
+            const int total = s_pref[batch];
+            for (int k = tid; k < total; k += kThreads) {
+                int j = 0;
+                while (k >= s_pref[j + 1])
+                    ++j;
+                const int loc = k - s_pref[j];
+                const int u = s_u0[j] + loc % s_nu[j];
+                const int v = s_v0[j] + loc / s_nu[j];
+                const int64_t ray = (int64_t) s_pose[j] * dims.pix_per_pose + (int64_t) v * dims.det[0] + u;
+                const float yv = sino[ray];
+                if (yv == 0.0f)
+                    continue;
+                const RayRec r = table[ray];
+                tile_ray_adjoint<D, A>(r, t, yv, [&](int slot, float val) {
+                    atomicAdd(reinterpret_cast<unsigned long long*>(iacc + slot),
+                              (unsigned long long) (long long) rintf(val * iq));
+                });
+            }
+            __syncthreads();
+        }
+
+        // add the owned voxels of the tile to the volume
+        for (int i = tid; i < t.nslots; i += kThreads) {
+            const int64_t lin = tile_slot_voxel<D, A>(t, dims, i);
+            if (lin >= 0)
+                vol[lin] += (float) ((double) iacc[i] * (double) q);
+        }
+    }
+
+    // Which driving axes occur among the rays of each pose (bit a set when some ray that hits
+    // the volume drives along a).
+    __global__ void __launch_bounds__(256)
+        pose_axes_kernel(const RayRec* __restrict__ table, const Dims dims, int* __restrict__ pose_axes)
+    {
+        const int pose = blockIdx.y;
+        const int64_t pix = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+        if (pix >= dims.pix_per_pose)
+            return;
+        const RayRec r = table[(int64_t) pose * dims.pix_per_pose + pix];
+        if (r.nsteps > 0)
+            atomicOr(pose_axes + pose, 1 << (r.meta & 3));
+    }
+
+    hipError_t joseph_pose_axes(const RayRec* table, const Dims& dims, int* pose_axes, hipStream_t s)
     {
         hipError_t e = launch_fill<int>(pose_axes, dims.nposes, 0, s);
         if (e != hipSuccess)
             return e;
         const dim3 grid((unsigned) ((dims.pix_per_pose + 255) / 256), (unsigned) dims.nposes);
-        if (dims.dim == 2)
-            hipLaunchKernelGGL((pose_axes_kernel<2>), grid, dim3(256), 0, s, geo, dims, pose_axes);
-        else
-            hipLaunchKernelGGL((pose_axes_kernel<3>), grid, dim3(256), 0, s, geo, dims, pose_axes);
+        hipLaunchKernelGGL(pose_axes_kernel, grid, dim3(256), 0, s, table, dims, pose_axes);
         return hipGetLastError();
     }
 
     size_t joseph_tile_lds_bytes(const Dims& dims, const TileCfg& tc)
     {
         const int PC = dims.dim == 3 ? tc.TC + 2 : 1;
-        return (size_t) tc.LA * PC * (tc.TB + 2) * sizeof(float);
+        return (size_t) tc.LA * PC * (tc.TB + 2) * sizeof(long long);
     }
 
-    hipError_t joseph_adjoint_tiled(const PoseGeom* geo, const Dims& dims, const int* pose_axes, int axes_present,
-                                    const TileCfg& tc, const float* sino, float* vol, hipStream_t s)
+    hipError_t joseph_adjoint_tiled(const PoseGeom* geo, const RayRec* table, const Dims& dims, const int* pose_axes,
+                                    int axes_present, const TileCfg& tc, int batch, double slot_bound,
+                                    unsigned int* d_maxbits, float* d_scale, const float* sino, float* vol,
+                                    hipStream_t s)
     {
+        if (batch < 1 || batch > kMaxPoseBatch)
+            return hipErrorInvalidValue;
         hipError_t e = launch_fill<float>(vol, dims.nvox, 0.0f, s);
+        if (e == hipSuccess)
+            e = launch_fill<unsigned int>(d_maxbits, 1, 0u, s);
+        if (e != hipSuccess)
+            return e;
+        int64_t blocks = (dims.nrays + 255) / 256;
+        blocks = blocks > 4096 ? 4096 : blocks;
+        hipLaunchKernelGGL(absmax_kernel, dim3((unsigned) blocks), dim3(256), 0, s, sino, dims.nrays, d_maxbits);
+        hipLaunchKernelGGL(scale_kernel, dim3(1), dim3(1), 0, s, d_maxbits, slot_bound, d_scale);
+        e = hipGetLastError();
         if (e != hipSuccess)
             return e;
         const size_t lds = joseph_tile_lds_bytes(dims, tc);
@@ -141,12 +223,23 @@ namespace ehp
             tile_counts(dims, tc, a, ntA, ntB, ntC);
             const int per_xcd = (ntA * ntB * ntC + 7) / 8;
             const dim3 grid((unsigned) (per_xcd * 8));
-            if (dims.dim == 2)
-                hipLaunchKernelGGL((joseph_adj_tiled_kernel<2>), grid, dim3(kThreads), lds, s, geo, dims, pose_axes, a, tc,
-                                   ntA, ntB, ntC, per_xcd, sino, vol);
-            else
-                hipLaunchKernelGGL((joseph_adj_tiled_kernel<3>), grid, dim3(kThreads), lds, s, geo, dims, pose_axes, a, tc,
-                                   ntA, ntB, ntC, per_xcd, sino, vol);
+#define EHP_ADJ_TILED(Dv, Av)                                                                                      \
+    hipLaunchKernelGGL((joseph_adj_tiled_kernel<Dv, Av>), grid, dim3(kThreads), lds, s, geo, table, dims, pose_axes, \
+                       tc, ntA, ntB, ntC, per_xcd, batch, d_scale, sino, vol)
+            if (dims.dim == 2) {
+                if (a == 0)
+                    EHP_ADJ_TILED(2, 0);
+                else
+                    EHP_ADJ_TILED(2, 1);
+            } else {
+                if (a == 0)
+                    EHP_ADJ_TILED(3, 0);
+                else if (a == 1)
+                    EHP_ADJ_TILED(3, 1);
+                else
+                    EHP_ADJ_TILED(3, 2);
+            }
+#undef EHP_ADJ_TILED
             e = hipGetLastError();
             if (e != hipSuccess)
                 return e;

@@ -37,6 +37,32 @@ namespace ehp
     // ---------------------------------------------------------------------------------
     // Joseph: one thread per ray (pixel of one pose). Grid: x = pixel block, y = pose.
 
+    // The samples of one ray with its driving axis A fixed at compile time.
+    template <int D, int A, class T>
+    __device__ inline T joseph_ray_forward(const JRay& jr, const Dims& dims, const T* __restrict__ vol)
+    {
+        T acc = T(0);
+        for (int s = 0; s < jr.nsteps; ++s) {
+            JSample js;
+            joseph_sample<D, A>(jr, s, js);
+            joseph_taps<D>(js, dims.vol, dims.strides, [&](int64_t lin, float wt) { acc += vol[lin] * (T) wt; });
+        }
+        return acc;
+    }
+
+    template <int D, int A, class T>
+    __device__ inline void joseph_ray_adjoint(const JRay& jr, const Dims& dims, T yl, T* __restrict__ vol)
+    {
+        for (int s = 0; s < jr.nsteps; ++s) {
+            JSample js;
+            joseph_sample<D, A>(jr, s, js);
+            joseph_taps<D>(js, dims.vol, dims.strides, [&](int64_t lin, float wt) {
+                if (wt != 0.0f)
+                    atomic_add(vol + lin, yl * (T) wt);
+            });
+        }
+    }
+
     template <int D, class T>
     __global__ void __launch_bounds__(256)
         joseph_fwd_kernel(const PoseGeom* __restrict__ geo, const Dims dims, const T* __restrict__ vol,
@@ -52,13 +78,13 @@ namespace ehp
         make_ray<D>(geo[pose], u, v, o, d);
         JRay jr;
         joseph_setup<D>(o, d, dims.vol, jr);
-        T acc = T(0);
-        for (int s = 0; s < jr.nsteps; ++s) {
-            JSample js;
-            joseph_sample<D>(jr, s, js);
-            joseph_taps<D>(js, dims.vol, dims.strides,
-                           [&](int64_t lin, float wt) { acc += vol[lin] * (T) wt; });
-        }
+        T acc;
+        if (jr.axis == 0)
+            acc = joseph_ray_forward<D, 0, T>(jr, dims, vol);
+        else if (D == 2 || jr.axis == 1)
+            acc = joseph_ray_forward<D, 1, T>(jr, dims, vol);
+        else
+            acc = joseph_ray_forward<D, D - 1, T>(jr, dims, vol);
         sino[(int64_t) pose * dims.pix_per_pose + pix] = acc * (T) jr.len;
     }
 
@@ -81,14 +107,12 @@ namespace ehp
         JRay jr;
         joseph_setup<D>(o, d, dims.vol, jr);
         const T yl = yv * (T) jr.len;
-        for (int s = 0; s < jr.nsteps; ++s) {
-            JSample js;
-            joseph_sample<D>(jr, s, js);
-            joseph_taps<D>(js, dims.vol, dims.strides, [&](int64_t lin, float wt) {
-                if (wt != 0.0f)
-                    atomic_add(vol + lin, yl * (T) wt);
-            });
-        }
+        if (jr.axis == 0)
+            joseph_ray_adjoint<D, 0, T>(jr, dims, yl, vol);
+        else if (D == 2 || jr.axis == 1)
+            joseph_ray_adjoint<D, 1, T>(jr, dims, yl, vol);
+        else
+            joseph_ray_adjoint<D, D - 1, T>(jr, dims, yl, vol);
     }
 
     // Per-ray traversal state (axis, sample count) and the total sample count.
@@ -530,5 +554,6 @@ namespace ehp
     EHP_INSTANTIATE(float)
     EHP_INSTANTIATE(double)
     template hipError_t launch_fill<int>(int*, int64_t, int, hipStream_t);
+    template hipError_t launch_fill<unsigned int>(unsigned int*, int64_t, unsigned int, hipStream_t);
 #undef EHP_INSTANTIATE
 } // namespace ehp

@@ -15,10 +15,13 @@ namespace ehp
     hipError_t joseph_adjoint(const PoseGeom* geo, const Dims& dims, const T* sino, T* vol, hipStream_t s);
     hipError_t joseph_info(const PoseGeom* geo, const Dims& dims, int32_t* axis, int64_t* nsteps,
                            unsigned long long* total, hipStream_t s);
-    hipError_t joseph_pose_axes(const PoseGeom* geo, const Dims& dims, int* pose_axes, hipStream_t s);
+    hipError_t joseph_ray_table(const PoseGeom* geo, const Dims& dims, RayRec* table, hipStream_t s);
+    hipError_t joseph_pose_axes(const RayRec* table, const Dims& dims, int* pose_axes, hipStream_t s);
     size_t joseph_tile_lds_bytes(const Dims& dims, const TileCfg& tc);
-    hipError_t joseph_adjoint_tiled(const PoseGeom* geo, const Dims& dims, const int* pose_axes, int axes_present,
-                                    const TileCfg& tc, const float* sino, float* vol, hipStream_t s);
+    hipError_t joseph_adjoint_tiled(const PoseGeom* geo, const RayRec* table, const Dims& dims, const int* pose_axes,
+                                    int axes_present, const TileCfg& tc, int batch, double slot_bound,
+                                    unsigned int* d_maxbits, float* d_scale, const float* sino, float* vol,
+                                    hipStream_t s);
 
     template <class T>
     hipError_t siddon_forward(const PoseGeom* geo, const Dims& dims, const T* vol, T* sino, hipStream_t s);

@@ -161,6 +161,12 @@ class JosephsMethodCUDA(_HipProjector):
         _lib.check(self._L.ehp_joseph_create(geo, _DT[self._dtype], flags, ctypes.byref(self._op)),
                    "JosephsMethodCUDA")
 
+    def tile(self, la: int = 0, tb: int = 0, tc: int = 0):
+        """Set (la > 0) and return the (LA, TB, TC) tile of the LDS-tiled back projection."""
+        o = [ctypes.c_int(), ctypes.c_int(), ctypes.c_int()]
+        _lib.check(self._L.ehp_joseph_tile(self._op, la, tb, tc, *(ctypes.byref(x) for x in o)), "tile")
+        return tuple(x.value for x in o)
+
     def ray_info(self):
         """Driving axis and sample count of every ray as the device computes them."""
         axis = np.empty(self._n_sino, np.int32)

@@ -110,6 +110,13 @@ int ehp_op_sizes(const ehp_op* op, int64_t* n_vol, int64_t* n_sino);
  * number of footprint taps. Computed on the device at create time. */
 int ehp_sample_count(const ehp_op* op, uint64_t* samples);
 
+/* Tile shape of the LDS-tiled Joseph back projection (FP32): la planes along the driving axis,
+ * tb x tc voxels across (tc ignored in 2D). No counterpart in elsa; it is the knob the tile
+ * autotuner turns. Fails with EHP_ERR_ARGUMENT if the tile does not fit the device's LDS or
+ * the operator does not use the tiled kernels. Queries only when out pointers are given
+ * and la <= 0. */
+int ehp_joseph_tile(ehp_op* op, int la, int tb, int tc, int* la_out, int* tb_out, int* tc_out);
+
 /* Per-ray Joseph traversal state as the kernels compute it (for parity checks): driving
  * axis and number of samples of every ray, host memory of n_sino entries each. */
 int ehp_joseph_ray_info(const ehp_op* op, int32_t* axis, int64_t* nsteps);

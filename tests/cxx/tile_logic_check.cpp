@@ -1,11 +1,14 @@
 // tile_logic_check.cpp -- runs the tile logic of the LDS-tiled Joseph kernels on the CPU.
 //
 // Built with g++ against elsa_amd/csrc/ehp_joseph_tile.hpp (the same functions the HIP
-// kernels call), it walks every tile of every pass the way a workgroup does, with bounds
-// checks on every LDS slot and volume index and a check that each voxel is written by exactly
-// one tile per pass. Used by tests/test_tile_logic.py to compare against the oracle.
+// kernels call), it builds the per-ray table, walks every tile of every pass the way a
+// workgroup does -- pose batches, 64-bit fixed-point sums -- with bounds checks on every LDS
+// slot and volume index, a check that no sum leaves the int64 range, and a check that each
+// voxel is written by exactly one tile per pass. Used by
+// tests/test_tile_logic.py to compare against the oracle.
 #include "ehp_joseph_tile.hpp"
 
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -32,42 +35,27 @@ namespace
         return d;
     }
 
-    template <int D>
-    int run(const Dims& dims, const std::vector<PoseGeom>& geo, TileCfg tc, const float* sino, float* vol,
-            int64_t* visits)
+    struct Stats {
+        double max_ratio = 0.0; // largest |slot sum| / 2^63 seen
+    };
+
+    template <int D, int A>
+    int run_pass(const Dims& dims, const std::vector<PoseGeom>& geo, const std::vector<RayRec>& table,
+                 const std::vector<int>& axes, TileCfg tc, float q, float iq, const float* sino, float* vol,
+                 Stats& st)
     {
-        // driving axes per pose, as pose_axes_kernel computes them
-        std::vector<int> axes(dims.nposes, 0);
-        int present = 0;
-        for (int p = 0; p < dims.nposes; ++p)
-            for (int v = 0; v < dims.det[1]; ++v)
-                for (int u = 0; u < dims.det[0]; ++u) {
-                    float o[3], d[3];
-                    make_ray<D>(geo[p], u, v, o, d);
-                    JRay jr;
-                    joseph_setup<D>(o, d, dims.vol, jr);
-                    if (jr.nsteps > 0)
-                        axes[p] |= 1 << jr.axis;
-                }
-        for (int p = 0; p < dims.nposes; ++p)
-            present |= axes[p];
-        for (int64_t i = 0; i < dims.nvox; ++i)
-            vol[i] = 0.0f;
-        std::vector<int> owner(dims.nvox);
-        for (int a = 0; a < D; ++a) {
-            if (!(present & (1 << a)))
-                continue;
-            std::fill(owner.begin(), owner.end(), 0);
-            int ntA, ntB, ntC;
-            tile_counts(dims, tc, a, ntA, ntB, ntC);
-            for (int tile = 0; tile < ntA * ntB * ntC; ++tile) {
-                const TileBox t = make_tile<D>(dims, tc, a, tile % ntA, (tile / ntA) % ntB, tile / (ntA * ntB));
-                std::vector<float> acc(t.nslots, 0.0f);
-                for (int p = 0; p < dims.nposes; ++p) {
-                    if (!(axes[p] & (1 << a)))
+        std::vector<int> owner(dims.nvox, 0);
+        int ntA, ntB, ntC;
+        tile_counts(dims, tc, A, ntA, ntB, ntC);
+        for (int tile = 0; tile < ntA * ntB * ntC; ++tile) {
+            const TileBox t = make_tile<D, A>(dims, tc, tile % ntA, (tile / ntA) % ntB, tile / (ntA * ntB));
+            std::vector<__int128> iacc(t.nslots, 0);
+            for (int pbase = 0; pbase < dims.nposes; pbase += kPoseBatch) {
+                for (int p = pbase; p < dims.nposes && p < pbase + kPoseBatch; ++p) {
+                    if (!(axes[p] & (1 << A)))
                         continue;
                     int u0, u1, v0, v1;
-                    if (!tile_bbox<D>(geo[p], t, dims, u0, u1, v0, v1))
+                    if (!tile_bbox<D, A>(geo[p], t, dims, u0, u1, v0, v1))
                         continue;
                     if (u0 < 0 || u1 >= dims.det[0] || v0 < 0 || v1 >= dims.det[1]) {
                         std::fprintf(stderr, "bbox out of the detector\n");
@@ -75,52 +63,117 @@ namespace
                     }
                     for (int v = v0; v <= v1; ++v)
                         for (int u = u0; u <= u1; ++u) {
-                            const float yv = sino[(int64_t) p * dims.pix_per_pose + (int64_t) v * dims.det[0] + u];
+                            const int64_t ray = (int64_t) p * dims.pix_per_pose + (int64_t) v * dims.det[0] + u;
+                            const float yv = sino[ray];
                             if (yv == 0.0f)
                                 continue;
                             bool bad = false;
-                            tile_ray_adjoint<D>(geo[p], dims, t, u, v, yv, [&](int slot, float val) {
+                            tile_ray_adjoint<D, A>(table[ray], t, yv, [&](int slot, float val) {
                                 if (slot < 0 || slot >= t.nslots) {
                                     bad = true;
                                     return;
                                 }
-                                acc[slot] += val;
+                                const float z = rintf(val * iq);
+                                if (!(fabsf(z) < 9.2e18f)) {
+                                    bad = true;
+                                    return;
+                                }
+                                iacc[slot] += (int64_t) z;
                             });
                             if (bad) {
-                                std::fprintf(stderr, "LDS slot out of range (pass %d tile %d pose %d u %d v %d)\n", a,
+                                std::fprintf(stderr, "LDS slot or contribution out of range (pass %d tile %d pose %d u %d v %d)\n", A,
                                              tile, p, u, v);
                                 return 3;
                             }
                         }
                 }
-                for (int i = 0; i < t.nslots; ++i) {
-                    const int64_t lin = tile_slot_voxel<D>(t, dims, i);
-                    if (lin < 0)
-                        continue;
-                    if (lin >= dims.nvox) {
-                        std::fprintf(stderr, "volume index out of range\n");
-                        return 4;
-                    }
-                    owner[lin] += 1;
-                    vol[lin] += acc[i];
+            }
+            for (int i = 0; i < t.nslots; ++i) {
+                const double r = (double) (iacc[i] < 0 ? -iacc[i] : iacc[i]) / 9223372036854775808.0;
+                st.max_ratio = r > st.max_ratio ? r : st.max_ratio;
+                if (r >= 1.0) {
+                    std::fprintf(stderr, "int64 slot sum overflow (pass %d tile %d slot %d)\n", A, tile, i);
+                    return 6;
                 }
             }
-            for (int64_t i = 0; i < dims.nvox; ++i) {
-                if (owner[i] != 1) {
-                    std::fprintf(stderr, "voxel %lld owned %d times in pass %d\n", (long long) i, owner[i], a);
-                    return 5;
+            for (int i = 0; i < t.nslots; ++i) {
+                const int64_t lin = tile_slot_voxel<D, A>(t, dims, i);
+                if (lin < 0)
+                    continue;
+                if (lin >= dims.nvox) {
+                    std::fprintf(stderr, "volume index out of range\n");
+                    return 4;
                 }
+                owner[lin] += 1;
+                vol[lin] += (float) ((double) (int64_t) iacc[i] * (double) q);
             }
-            if (visits)
-                visits[a] = 1;
         }
+        for (int64_t i = 0; i < dims.nvox; ++i) {
+            if (owner[i] != 1) {
+                std::fprintf(stderr, "voxel %lld owned %d times in pass %d\n", (long long) i, owner[i], A);
+                return 5;
+            }
+        }
+        return 0;
+    }
+
+    template <int D>
+    int run(const Dims& dims, const std::vector<PoseGeom>& geo, TileCfg tc, const float* sino, float* vol,
+            double* info)
+    {
+        std::vector<RayRec> table(dims.nrays);
+        std::vector<int> axes(dims.nposes, 0);
+        for (int p = 0; p < dims.nposes; ++p)
+            for (int v = 0; v < dims.det[1]; ++v)
+                for (int u = 0; u < dims.det[0]; ++u) {
+                    float o[3], d[3];
+                    make_ray<D>(geo[p], u, v, o, d);
+                    JRay jr;
+                    joseph_setup<D>(o, d, dims.vol, jr);
+                    const RayRec r = make_ray_rec<D>(jr);
+                    table[(int64_t) p * dims.pix_per_pose + (int64_t) v * dims.det[0] + u] = r;
+                    if (r.nsteps > 0)
+                        axes[p] |= 1 << (r.meta & 3);
+                }
+        int present = 0;
+        for (int p = 0; p < dims.nposes; ++p)
+            present |= axes[p];
+        const double bound = fixed_point_slot_bound(dims, geo.data());
+        info[0] = bound;
+        if (!(bound > 0.0))
+            return 10;
+        float m = 0.0f;
+        for (int64_t i = 0; i < dims.nrays; ++i)
+            m = fabsf(sino[i]) > m ? fabsf(sino[i]) : m;
+        float q, iq;
+        fixed_point_scale(m, bound, q, iq);
+        info[1] = q;
+        for (int64_t i = 0; i < dims.nvox; ++i)
+            vol[i] = 0.0f;
+        Stats st;
+        for (int a = 0; a < D; ++a) {
+            if (!(present & (1 << a)))
+                continue;
+            int rc = 0;
+            if (a == 0)
+                rc = run_pass<D, 0>(dims, geo, table, axes, tc, q, iq, sino, vol, st);
+            else if (a == 1)
+                rc = run_pass<D, 1>(dims, geo, table, axes, tc, q, iq, sino, vol, st);
+            else
+                rc = run_pass<D, (D == 3 ? 2 : 1)>(dims, geo, table, axes, tc, q, iq, sino, vol, st);
+            if (rc)
+                return rc;
+            info[3 + a] = 1;
+        }
+        info[2] = st.max_ratio;
         return 0;
     }
 } // namespace
 
+// info: [0] slot bound, [1] q, [2] largest |slot sum| / 2^63, [3..5] passes run
 extern "C" int tile_adjoint(int dim, const int64_t* vol_dims, const int64_t* det_dims, int nposes, const float* P,
                             const float* Pinv, const float* C, int LA, int TB, int TC, const float* sino, float* vol,
-                            int64_t* passes)
+                            double* info)
 {
     const Dims dims = make_dims(dim, vol_dims, det_dims, nposes);
     const int nm = dim * (dim + 1);
@@ -136,5 +189,5 @@ extern "C" int tile_adjoint(int dim, const int64_t* vol_dims, const int64_t* det
         geo[p] = g;
     }
     const TileCfg tc{LA, TB, TC};
-    return dim == 2 ? run<2>(dims, geo, tc, sino, vol, passes) : run<3>(dims, geo, tc, sino, vol, passes);
+    return dim == 2 ? run<2>(dims, geo, tc, sino, vol, info) : run<3>(dims, geo, tc, sino, vol, info);
 }
