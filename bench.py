@@ -48,10 +48,27 @@ def parse():
     p.add_argument("--cpu-poses", type=int, default=240,
                    help="poses of the bounded CPU sample (evenly spread over the trajectory)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    p.add_argument("--kernels", default="tiled", choices=["tiled", "simple"],
-                   help="Joseph kernels: LDS-tiled back projection (default) or one thread per ray")
+    p.add_argument("--kernels", default="fast", choices=["fast", "simple"],
+                   help="Joseph kernels: plane-major forward + LDS-tiled back projection (default) or one "
+                        "thread per ray")
     p.add_argument("--tile", default="", help="LA,TB,TC of the tiled back projection (default: built-in)")
+    p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                   help="only run a plain device copy of MIB MiB (3 launches), the reference kernel for "
+                        "scaling FETCH_SIZE / WRITE_SIZE counters, and exit")
     return p.parse_args()
+
+
+def copy_calibration(mib):
+    """A plain copy kernel of known size: counters read under it convert to bytes."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 3}), flush=True)
 
 
 def cpu_baseline(wl, x_np, n_poses):
@@ -79,6 +96,9 @@ def main():
         print(f"bench.py runs on exactly one GPU (got --gpus {a.gpus}); the projectors are single-GPU here",
               file=sys.stderr)
         sys.exit(2)
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration)
+        return
     import torch
 
     import elsa_amd as E
@@ -92,7 +112,7 @@ def main():
                                                       detectorSize=list(wl["det"]))
     op = E.JosephsMethodCUDA(vd, dd, kernels=a.kernels)
     tile = None
-    if a.kernels == "tiled":
+    if a.kernels == "fast":
         tile = op.tile(*[int(v) for v in a.tile.split(",")]) if a.tile else op.tile()
     S = op.sample_count()
     R = int(np.prod(wl["det"])) * wl["views"]
@@ -123,8 +143,9 @@ def main():
     t_adj = float(np.mean([e1.elapsed_time(e2) for _, e1, e2 in evs])) * 1e-3
     tap_bytes = 16 if wl["dim"] == 3 else 8
     alg_bytes = tap_bytes * S + 4 * R
-    adj_name = "joseph_adj_tiled_kernel" if a.kernels == "tiled" else "joseph_adj_kernel"
-    name, t_dom = (adj_name, t_adj) if t_adj >= t_fwd else ("joseph_fwd_kernel", t_fwd)
+    adj_name = "joseph_adj_tiled_kernel" if a.kernels == "fast" else "joseph_adj_kernel"
+    fwd_name = "joseph_fwd_planes_kernel" if a.kernels == "fast" else "joseph_fwd_kernel"
+    name, t_dom = (adj_name, t_adj) if t_adj >= t_fwd else (fwd_name, t_fwd)
     achieved = alg_bytes / t_dom / 1e9
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"{name}_{a.workload}_traffic.json")

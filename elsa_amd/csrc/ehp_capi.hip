@@ -7,6 +7,7 @@
 #include "elsa_hip.h"
 #include "ehp_kernels.hpp"
 
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -33,14 +34,17 @@ struct ehp_op {
     int n_lut = 0;
     float support = 0.0f;
     uint64_t samples = 0;
-    // Joseph, FP32: LDS-tiled back projection
+    // Joseph, FP32: plane-major forward and LDS-tiled back projection
+    bool planes = false;
+    ehp::PadLayout layouts[3]{};
+    float* d_view[3] = {nullptr, nullptr, nullptr}; // zero-padded plane-major volume views
     bool tiled = false;
     int* d_pose_axes = nullptr;
     int axes_present = 0;
     ehp::TileCfg tile{};
     ehp::RayRec* d_table = nullptr;   // per-ray traversal state
     unsigned int* d_maxbits = nullptr; // max|y| of the current back projection
-    float* d_scale = nullptr;          // fixed-point scale q, 1/q
+    int* d_fixexp = nullptr;           // fixed-point exponent of the current back projection
     double slot_bound = 0.0;
     int batch = ehp::kPoseBatch;
     std::vector<ehp::PoseGeom> host_geo;
@@ -230,15 +234,33 @@ namespace
         return tc;
     }
 
-    int setup_tiled(ehp_op* op)
+    // Fastest axis of the plane-major view for driving axis a: the transverse axis along
+    // which the rays of neighbouring pixels in a detector row spread most, summed over the
+    // poses that have rays of axis a (the first column of Pinv is that spread per pixel).
+    int fan_axis(const ehp_op* op, const std::vector<int>& pose_axes, int a)
     {
         const ehp::Dims& dm = op->dims;
-        // the bound keeps the 64-bit fixed-point sums in range with a resolution of at least
-        // 2^-30 max|y|
-        const double bound = ehp::fixed_point_slot_bound(dm, op->host_geo.data());
-        if (!(bound > 0.0) || bound > 4294967296.0)
-            return EHP_OK; // keep the one-thread-per-ray kernels
-        op->slot_bound = bound;
+        if (dm.dim == 2)
+            return 1 - a;
+        double score[3] = {0.0, 0.0, 0.0};
+        for (int p = 0; p < dm.nposes; ++p) {
+            if (!(pose_axes[p] & (1 << a)))
+                continue;
+            for (int i = 0; i < 3; ++i)
+                score[i] += std::fabs((double) op->host_geo[p].Pinv[i]);
+        }
+        int best = -1;
+        for (int i = 0; i < 3; ++i)
+            if (i != a && (best < 0 || score[i] > score[best]))
+                best = i;
+        return best;
+    }
+
+    // FP32 Joseph: per-ray table, driving axes per pose, plane-major views for the forward
+    // and, when the fixed-point bound exists, the LDS-tiled back projection.
+    int setup_fast(ehp_op* op)
+    {
+        const ehp::Dims& dm = op->dims;
         const int n = dm.nposes;
         const size_t tbytes = (size_t) dm.nrays * sizeof(ehp::RayRec);
         hipError_t e = hipMalloc((void**) &op->d_table, tbytes);
@@ -248,9 +270,9 @@ namespace
         if (e == hipSuccess)
             e = hipMalloc((void**) &op->d_maxbits, sizeof(unsigned int));
         if (e == hipSuccess)
-            e = hipMalloc((void**) &op->d_scale, 2 * sizeof(float));
+            e = hipMalloc((void**) &op->d_fixexp, 2 * sizeof(int));
         if (e != hipSuccess)
-            return fail(EHP_ERR_ALLOC, "hipMalloc of the tiled back projection state failed: %s", hipGetErrorString(e));
+            return fail(EHP_ERR_ALLOC, "hipMalloc of the Joseph state failed: %s", hipGetErrorString(e));
         e = ehp::joseph_ray_table(op->d_geo, dm, op->d_table, nullptr);
         if (e == hipSuccess)
             e = ehp::joseph_pose_axes(op->d_table, dm, op->d_pose_axes, nullptr);
@@ -262,6 +284,42 @@ namespace
         op->axes_present = 0;
         for (int v : h)
             op->axes_present |= v;
+
+        // forward: zero-padded plane-major views (32-bit byte offsets inside a view, so below
+        // 4 GiB each); the padding is zeroed once here and never written again
+        bool planes = true;
+        for (int a = 0; a < dm.dim; ++a) {
+            op->layouts[a] = ehp::pad_layout(dm, a, fan_axis(op, h, a));
+            if ((op->axes_present & (1 << a)) && op->layouts[a].size >= ((int64_t) 1 << 30))
+                planes = false;
+        }
+        for (int a = 0; planes && a < dm.dim; ++a) {
+            if (!(op->axes_present & (1 << a)))
+                continue;
+            const size_t vbytes = (size_t) op->layouts[a].size * sizeof(float);
+            e = hipMalloc((void**) &op->d_view[a], vbytes);
+            if (e != hipSuccess)
+                return fail(EHP_ERR_ALLOC, "hipMalloc of a %zu-byte volume view failed: %s", vbytes,
+                            hipGetErrorString(e));
+            e = ehp::launch_fill<float>(op->d_view[a], op->layouts[a].size, 0.0f, nullptr);
+            if (e != hipSuccess)
+                return hip_fail(e, "volume view");
+        }
+        op->planes = planes;
+        // the views' padding is cleared on the null stream; finish before any caller stream runs
+        e = hipDeviceSynchronize();
+        if (e != hipSuccess)
+            return hip_fail(e, "volume view");
+
+        // back projection: the bound keeps the 64-bit fixed-point sums in range; without it
+        // the one-thread-per-ray kernel runs
+        const double bound = ehp::fixed_point_slot_bound(dm, op->host_geo.data());
+        if (!(bound > 0.0) || bound > 1.0e18) {
+            (void) hipFree(op->d_table);
+            op->d_table = nullptr;
+            return EHP_OK;
+        }
+        op->slot_bound = bound;
         int max_lds = 65536;
         if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, op->device) != hipSuccess)
             max_lds = 65536;
@@ -280,7 +338,7 @@ int ehp_joseph_create(const ehp_geometry* g, int dtype, int flags, ehp_op** out)
         return rc;
     rc = count_samples(*out);
     if (rc == EHP_OK && dtype == EHP_F32 && !(flags & EHP_JOSEPH_SIMPLE_KERNELS))
-        rc = setup_tiled(*out);
+        rc = setup_fast(*out);
     if (rc != EHP_OK) {
         ehp_destroy(*out);
         *out = nullptr;
@@ -344,7 +402,7 @@ void ehp_destroy(ehp_op* op)
     if (cur != op->device)
         (void) hipSetDevice(op->device);
     for (void* p : {(void*) op->d_geo, op->d_lut, (void*) op->d_pose_axes, (void*) op->d_table, (void*) op->d_maxbits,
-                    (void*) op->d_scale})
+                    (void*) op->d_fixexp, (void*) op->d_view[0], (void*) op->d_view[1], (void*) op->d_view[2]})
         if (p)
             (void) hipFree(p);
     if (cur >= 0 && cur != op->device)
@@ -392,7 +450,9 @@ static int apply(ehp_op* op, const void* in, void* out, void* stream, bool adjoi
         float* y = (float*) out;
         if (op->kind == KIND_JOSEPH && adjoint && op->tiled)
             e = ehp::joseph_adjoint_tiled(op->d_geo, op->d_table, dm, op->d_pose_axes, op->axes_present, op->tile,
-                                          op->batch, op->slot_bound, op->d_maxbits, op->d_scale, x, y, s);
+                                          op->batch, op->slot_bound, op->d_maxbits, op->d_fixexp, x, y, s);
+        else if (op->kind == KIND_JOSEPH && !adjoint && op->planes)
+            e = ehp::joseph_forward_planes(op->d_geo, dm, op->layouts, op->d_view, op->axes_present, x, y, s);
         else if (op->kind == KIND_JOSEPH)
             e = adjoint ? ehp::joseph_adjoint<float>(op->d_geo, dm, x, y, s) : ehp::joseph_forward<float>(op->d_geo, dm, x, y, s);
         else if (op->kind == KIND_SIDDON)

@@ -1,0 +1,172 @@
+// ehp_joseph_planes.hip -- Joseph forward projection over plane-major volume views (FP32).
+//
+// The forward of elsa's Joseph method (elsa/projectors/JosephsMethod.cpp:140-198, forward
+// branch :187-194) with one lane per ray. A wavefront holds 64 neighbouring detector pixels
+// of one row, so at each step its lanes read the same plane of the volume at neighbouring
+// in-plane positions. Those reads coalesce only if the axis along which the rays fan out is
+// the fastest one in memory: rays driving along x fan out along z (circular trajectories
+// about y), which in the input's x-fastest layout puts every lane on a different cache line.
+// So each driving axis reads its own view of the volume, made at the start of the call (one
+// read and one write of the volume): that axis slowest, the fan-out axis fastest, and two
+// zero voxels around each plane so that taps outside the volume need no bounds test.
+//
+// Blocks of 256 lanes cover 64 x 4 pixels (2D: 256 x 1) of one pose. Tiles are ordered
+// detector-row band first, then pose, then column, and each XCD (consecutive workgroups go
+// to different XCDs, so the block index is remapped) takes a contiguous run: the poses of a
+// band then run together on one XCD. For a rotation about y a band of rows sees the same
+// slab of the volume from every pose, so that slab stays in the XCD's L2.
+#include "ehp_kernels.hpp"
+
+namespace ehp
+{
+    namespace
+    {
+        constexpr int kThreads = 256;
+        constexpr int kTile = 32; // transpose tile
+
+        // View with x fastest: one lane per voxel, rows copied as they are.
+        __global__ void __launch_bounds__(256)
+            view_copy_rows_kernel(const float* __restrict__ in, float* __restrict__ out, const Dims dims,
+                                  const PadLayout L)
+        {
+            const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+            if (i >= dims.nvox)
+                return;
+            const int x = (int) (i % dims.vol[0]);
+            const int64_t r = i / dims.vol[0];
+            const int y = (int) (r % dims.vol[1]);
+            const int z = (int) (r / dims.vol[1]);
+            out[pad_index(L, x, y, z)] = in[i];
+        }
+
+        // View with f != x fastest: a 32 x 32 LDS tile over the (x, f) plane, reads coalesced
+        // along x, writes along f. Grid: (ceil(nx / 32), ceil(n_f / 32), n_t), t the third axis.
+        __global__ void __launch_bounds__(256)
+            view_copy_transpose_kernel(const float* __restrict__ in, float* __restrict__ out, const Dims dims,
+                                       const PadLayout L)
+        {
+            __shared__ float tile[kTile][kTile + 1];
+            const int f = L.f;
+            const int t = 3 - f; // the axis that is neither x nor f
+            const int x0 = blockIdx.x * kTile, f0 = blockIdx.y * kTile, ct = blockIdx.z;
+            const int tx = threadIdx.x % kTile, ty = threadIdx.x / kTile; // 32 x 8
+            const int nx = dims.vol[0], nf = dims.vol[f];
+            const int64_t in_f = f == 1 ? dims.strides[1] : dims.strides[2];
+            const int64_t in_t = t == 1 ? dims.strides[1] : dims.strides[2];
+#pragma unroll
+            for (int j = 0; j < kTile; j += 8) {
+                const int cx = x0 + tx, cf = f0 + ty + j;
+                if (cx < nx && cf < nf)
+                    tile[ty + j][tx] = in[cx + cf * in_f + (int64_t) ct * in_t];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < kTile; j += 8) {
+                const int cx = x0 + ty + j, cf = f0 + tx;
+                if (cx < nx && cf < nf) {
+                    const int c1 = f == 1 ? cf : ct, c2 = f == 1 ? ct : cf;
+                    out[pad_index(L, cx, c1, c2)] = tile[tx][ty + j];
+                }
+            }
+        }
+
+        struct ViewSet {
+            PadView v[3];
+            int fb2; // bit a: the fast in-plane axis of view a is b2
+        };
+
+        template <int D, int A>
+        __device__ inline float fwd_ray(const RayRec& r, const Dims& dims, const ViewSet& vs)
+        {
+            if (D == 3 && (vs.fb2 >> A) & 1)
+                return plane_ray_forward<D, A, true>(r, dims, vs.v[A]);
+            return plane_ray_forward<D, A, false>(r, dims, vs.v[A]);
+        }
+
+        template <int D>
+        __global__ void __launch_bounds__(kThreads)
+            joseph_fwd_planes_kernel(const PoseGeom* __restrict__ geo, const Dims dims, const ViewSet views,
+                                     const int tiles_u, const int per_xcd, const int64_t total,
+                                     float* __restrict__ sino)
+        {
+            constexpr int TU = (D == 3) ? 64 : kThreads;
+            constexpr int TV = kThreads / TU;
+            const int64_t id = (int64_t) (blockIdx.x % 8u) * per_xcd + (blockIdx.x / 8u);
+            if (id >= total)
+                return;
+            // id = (band * nposes + pose) * tiles_u + tu
+            const int tu = (int) (id % tiles_u);
+            const int64_t bp = id / tiles_u;
+            const int pose = (int) (bp % dims.nposes);
+            const int band = (int) (bp / dims.nposes);
+            const int u = tu * TU + (int) threadIdx.x % TU;
+            const int v = band * TV + (int) threadIdx.x / TU;
+            if (u >= dims.det[0] || v >= dims.det[1])
+                return;
+            float o[3], d[3];
+            make_ray<D>(geo[pose], u, v, o, d);
+            JRay jr;
+            joseph_setup<D>(o, d, dims.vol, jr);
+            const RayRec r = make_ray_rec<D>(jr);
+            float val;
+            if (jr.axis == 0)
+                val = fwd_ray<D, 0>(r, dims, views);
+            else if (D == 2 || jr.axis == 1)
+                val = fwd_ray<D, 1>(r, dims, views);
+            else
+                val = fwd_ray<D, D - 1>(r, dims, views);
+            sino[(int64_t) pose * dims.pix_per_pose + (int64_t) v * dims.det[0] + u] = val;
+        }
+    } // namespace
+
+    hipError_t joseph_view_copy(const Dims& dims, const PadLayout& L, const float* vol, float* view, hipStream_t s)
+    {
+        if (L.f < 0 || L.f > 2 || L.f == L.A)
+            return hipErrorInvalidValue;
+        if (L.f == 0) {
+            const int64_t blocks = (dims.nvox + 255) / 256;
+            hipLaunchKernelGGL(view_copy_rows_kernel, dim3((unsigned) blocks), dim3(256), 0, s, vol, view, dims, L);
+        } else {
+            const int t = 3 - L.f;
+            const dim3 grid((unsigned) ((dims.vol[0] + kTile - 1) / kTile),
+                            (unsigned) ((dims.vol[L.f] + kTile - 1) / kTile), (unsigned) dims.vol[t]);
+            hipLaunchKernelGGL(view_copy_transpose_kernel, grid, dim3(256), 0, s, vol, view, dims, L);
+        }
+        return hipGetLastError();
+    }
+
+    hipError_t joseph_forward_planes(const PoseGeom* geo, const Dims& dims, const PadLayout* layouts,
+                                     float* const* views, int axes_present, const float* vol, float* sino,
+                                     hipStream_t s)
+    {
+        ViewSet vs{};
+        for (int a = 0; a < dims.dim; ++a) {
+            if (!(axes_present & (1 << a)))
+                continue;
+            const PadLayout& L = layouts[a];
+            hipError_t e = joseph_view_copy(dims, L, vol, views[a], s);
+            if (e != hipSuccess)
+                return e;
+            vs.v[a] = PadView{views[a], L.PS * 4u, L.RS * 4u, L.size};
+            const int b2 = dims.dim == 3 ? (a == 2 ? 1 : 2) : -1;
+            if (L.f == b2)
+                vs.fb2 |= 1 << a;
+        }
+        const int TU = dims.dim == 3 ? 64 : kThreads;
+        const int TV = kThreads / TU;
+        const int tiles_u = (dims.det[0] + TU - 1) / TU;
+        const int bands = (dims.det[1] + TV - 1) / TV;
+        const int64_t total = (int64_t) tiles_u * bands * dims.nposes;
+        const int64_t per_xcd = (total + 7) / 8;
+        if (per_xcd * 8 > (int64_t) 0x7fffffff)
+            return hipErrorInvalidValue;
+        const dim3 grid((unsigned) (per_xcd * 8));
+        if (dims.dim == 2)
+            hipLaunchKernelGGL((joseph_fwd_planes_kernel<2>), grid, dim3(kThreads), 0, s, geo, dims, vs, tiles_u,
+                               (int) per_xcd, total, sino);
+        else
+            hipLaunchKernelGGL((joseph_fwd_planes_kernel<3>), grid, dim3(kThreads), 0, s, geo, dims, vs, tiles_u,
+                               (int) per_xcd, total, sino);
+        return hipGetLastError();
+    }
+} // namespace ehp

@@ -8,6 +8,16 @@
 
 namespace ehp
 {
+    // a * b for operands below 2^24 (one full-rate 24-bit multiply on the GPU)
+    EHP_HD inline uint32_t mul24(uint32_t a, uint32_t b)
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        return __umul24(a, b);
+#else
+        return a * b;
+#endif
+    }
+
     // Tile of the LDS-privatised Joseph kernels: LA planes along the driving axis, TB x TC
     // voxels across (TC unused in 2D).
     struct TileCfg {
@@ -198,19 +208,20 @@ namespace ehp
     }
 
     // Back projection of one ray (driving axis A) into a tile: for every sample whose bilinear
-    // footprint touches the tile, hand y*len*weight for each of its four (2D: two) slots to
-    // add(slot, value). Weights are formed as in elsa's doInterpolation
-    // (JosephsMethod.cpp:60-97): the product of the transverse weights in axis order, the
-    // driving weight being 1.
+    // footprint touches the tile, hand ys * weight for each of its four (2D: two) slots to
+    // add(slot, value), ys being the ray's y * len in the ray's own fixed-point scale
+    // (ray_scale). Weights are elsa's bilinear weights (doInterpolation,
+    // JosephsMethod.cpp:60-97), the driving weight being 1; a 3D tap's weight is applied as
+    // (ys * w2) * w1.
     template <int D, int A, class Add>
-    EHP_HD inline void tile_ray_adjoint(const RayRec& r, const TileBox& t, float yv, Add add)
+    EHP_HD inline void tile_ray_adjoint(const RayRec& r, const TileBox& t, float ys, Add add)
     {
         if ((r.meta & 3) != A || r.nsteps == 0)
             return;
         int slo, shi;
         tile_sample_range(r, t, D, slo, shi);
-        const float yl = yv * r.len;
         const int sg = (r.meta & 4) ? -1 : 1;
+        const int PB = t.PB, PCB = t.PC * t.PB;
         for (int s = slo; s <= shi; ++s) {
             const float fs = (float) s;
             const float q1 = __builtin_fmaf(fs, r.s1, r.p1);
@@ -222,9 +233,9 @@ namespace ehp
             const float w1 = 1.0f - c1;
             const int pa = r.P0 + sg * s - t.A0;
             if (D == 2) {
-                const int base = pa * t.PB + (f1 - (t.B0 - 1));
-                add(base, yl * w1);
-                add(base + 1, yl * c1);
+                const int base = (int) mul24((uint32_t) pa, (uint32_t) PB) + (f1 - (t.B0 - 1));
+                add(base, ys * w1);
+                add(base + 1, ys * c1);
             } else {
                 const float q2 = __builtin_fmaf(fs, r.s2, r.p2);
                 const float g2 = floorf(q2 - 0.5f);
@@ -233,11 +244,14 @@ namespace ehp
                     continue;
                 const float c2 = (q2 - g2) - 0.5f;
                 const float w2 = 1.0f - c2;
-                const int base = (pa * t.PC + (f2 - (t.C0 - 1))) * t.PB + (f1 - (t.B0 - 1));
-                add(base, yl * (w1 * w2));
-                add(base + 1, yl * (c1 * w2));
-                add(base + t.PB, yl * (w1 * c2));
-                add(base + t.PB + 1, yl * (c1 * c2));
+                const int base = (int) (mul24((uint32_t) pa, (uint32_t) PCB) +
+                                        mul24((uint32_t) (f2 - (t.C0 - 1)), (uint32_t) PB)) +
+                                 (f1 - (t.B0 - 1));
+                const float yw = ys * w2, yc = ys * c2;
+                add(base, yw * w1);
+                add(base + 1, yw * c1);
+                add(base + PB, yc * w1);
+                add(base + PB + 1, yc * c1);
             }
         }
     }
@@ -334,26 +348,40 @@ namespace ehp
         return dims.nposes * per_pose * len_max;
     }
 
-    // Fixed-point scale of a back projection with 64-bit integer sums: q = 2^(e - 62) with
-    // 2^e >= maxabs * bound, so no slot sum reaches 2^62 in magnitude; iq = 1/q exactly.
-    // maxabs = 0 gives q = 1; a non-finite maxabs gives q = NaN (the result turns NaN like a
-    // float accumulation would). bound <= 2^32 keeps both powers of two normal floats.
-    EHP_HD inline void fixed_point_scale(float maxabs, double bound, float& q, float& iq)
+    // Fixed-point format of a back projection: slot sums are 64-bit integers in units of
+    // q = 2^(eg - 62), where 2^eg > max|y| * bound, so no sum reaches 2^62 in magnitude.
+    // Returns eg; 0 when max|y| is 0 (nothing to add), kNonFinite when it is not finite.
+    constexpr int kNonFinite = 1 << 20;
+
+    EHP_HD inline int fixed_point_exponent(float maxabs, double bound)
     {
         const double X = (double) maxabs * bound;
-        if (X == 0.0) {
-            q = 1.0f;
-            iq = 1.0f;
-        } else if (!(X <= 1.0e300)) {
-            q = nanf("");
-            iq = 0.0f;
-        } else {
-            int e;
-            frexp(X, &e); // X = f 2^e, f in [0.5, 1)
-            e = e < -60 ? -60 : e;
-            q = ldexpf(1.0f, e - 62);
-            iq = ldexpf(1.0f, 62 - e);
-        }
+        if (X == 0.0)
+            return 0;
+        if (!(X <= 1.0e300))
+            return kNonFinite;
+        int e;
+        frexp(X, &e); // X = f 2^e, f in [0.5, 1)
+        return e < -900 ? -900 : e;
+    }
+
+    // Per-ray scale of the contributions: with |yl| < 2^er (er raised to at least eg - 32),
+    // ys = yl * 2^(30 - er), so every contribution ys * w (w <= 1) rounds to an int32 of
+    // magnitude at most 2^30, and shifting it left by sh = er - eg + 32 (0..32) puts it in units
+    // of q. The rounding error of a contribution is at most 2^-31 |yl|.
+    EHP_HD inline void ray_scale(float yl, int eg, float& ys, int& sh)
+    {
+        int er;
+        frexpf(yl, &er);
+        er = er < eg - 32 ? eg - 32 : er;
+        ys = ldexpf(yl, 30 - er);
+        sh = er - eg + 32;
+    }
+
+    // The 64-bit fixed-point value of one contribution.
+    EHP_HD inline unsigned long long fixed_contrib(float t, int sh)
+    {
+        return (unsigned long long) (long long) (int) rintf(t) << sh;
     }
 
     inline void tile_counts(const Dims& dims, const TileCfg& tc, int a, int& ntA, int& ntB, int& ntC)
@@ -363,5 +391,158 @@ namespace ehp
         ntA = (dims.vol[a] + tc.LA - 1) / tc.LA;
         ntB = (dims.vol[b1] + tc.TB - 1) / tc.TB;
         ntC = dims.dim == 3 ? (dims.vol[b2] + tc.TC - 1) / tc.TC : 1;
+    }
+
+    // ---------------------------------------------------------------------------------
+    // plane-major forward projection
+
+    // Zero-padded plane-major view of the volume for the rays of driving axis A: A slowest,
+    // then the in-plane axis m, then the in-plane axis f fastest, with two zero voxels on
+    // either side of each in-plane axis (2D: of f only). Voxel (c_A, c_m, c_f) lives at
+    // c_A * PS + (c_m + mpad) * RS + (c_f + 2). A tap outside the volume then reads a zero
+    // instead of needing a bounds test, which gives the same sum as elsa's zero weight for
+    // finite volumes.
+    struct PadLayout {
+        int A, m, f;
+        int mpad;      // 2 in 3D, 0 in 2D
+        uint32_t RS;   // row stride (n_f + 4)
+        uint32_t PS;   // plane stride
+        int64_t size;  // floats in the view
+    };
+
+    inline PadLayout pad_layout(const Dims& dims, int A, int f)
+    {
+        PadLayout L{};
+        L.A = A;
+        L.f = f;
+        L.m = 3 - A - f; // in 2D the third axis has one voxel
+        L.mpad = dims.dim == 3 ? 2 : 0;
+        L.RS = (uint32_t) (dims.vol[f] + 4);
+        L.PS = L.RS * (uint32_t) (dims.vol[L.m] + 2 * L.mpad);
+        L.size = (int64_t) L.PS * dims.vol[A];
+        return L;
+    }
+
+    // Offset in the view of voxel (c0, c1, c2) of the input.
+    EHP_HD inline int64_t pad_index(const PadLayout& L, int c0, int c1, int c2)
+    {
+        const int c[3] = {c0, c1, c2};
+        const int ca = L.A == 0 ? c[0] : (L.A == 1 ? c[1] : c[2]);
+        const int cm = L.m == 0 ? c[0] : (L.m == 1 ? c[1] : c[2]);
+        const int cf = L.f == 0 ? c[0] : (L.f == 1 ? c[1] : c[2]);
+        return (int64_t) ca * L.PS + (int64_t) (cm + L.mpad) * L.RS + (cf + 2);
+    }
+
+    // What the kernels read of a view; strides in bytes (a view is below 4 GiB).
+    struct PadView {
+        const float* base;
+        uint32_t PS4, RS4;
+        int64_t size; // floats in the view (bounds checks of the CPU build)
+    };
+
+#if defined(EHP_VIEW_CHECK)
+    // CPU check builds: set when a read leaves its view
+    extern bool g_view_oob;
+#endif
+
+    // base + byte offset (a 32-bit offset from a uniform base: one VGPR per address)
+    EHP_HD inline float ld_view(const PadView& V, uint32_t off4)
+    {
+#if defined(EHP_VIEW_CHECK)
+        if ((off4 & 3u) || (int64_t) (off4 >> 2) >= V.size) {
+            g_view_oob = true;
+            return 0.0f;
+        }
+#endif
+        return *(const float*) ((const char*) V.base + off4);
+    }
+
+    // floor(x) clamped to [-2, n] and shifted by the padding: the in-plane index in a view
+    EHP_HD inline uint32_t pad_coord(float g, int n)
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        const float c = __builtin_amdgcn_fmed3f(g, -2.0f, (float) n);
+#else
+        const float c = g < -2.0f ? -2.0f : (g > (float) n ? (float) n : g);
+#endif
+        return (uint32_t) ((int) c + 2);
+    }
+
+    // Forward projection of one ray (driving axis A) through its padded view; FB2 says that
+    // the fast in-plane axis of the view is b2 (else b1). The sum over the ray's samples of
+    // the interpolated volume in the order and with the weights of elsa's doInterpolation
+    // (JosephsMethod.cpp:60-97): 2D taps (F0,C1)(C0,F1), 3D taps
+    // (F0,F1,F2)(F0,C1,C2)(C0,F1,C2)(C0,C1,F2), the driving axis' floor and ceil being its
+    // plane; each tap adds (value * len) * weight, elsa's association. C = F + 1 is read even
+    // where elsa's ceil gives C = F: the weight of that tap is 0 then. Samples whose plane lies
+    // outside the volume (a ray entering exactly through the far face) contribute 0 and are
+    // skipped. Positions are those of joseph_sample and tile_ray_adjoint.
+    template <int D, int A, bool FB2>
+    EHP_HD inline float plane_ray_forward(const RayRec& r, const Dims& dims, const PadView& V)
+    {
+        constexpr int b1 = TileAxes<D, A>::b1, b2 = TileAxes<D, A>::b2;
+        const int na = dims.vol[A], n1 = dims.vol[b1], n2 = (D == 3) ? dims.vol[b2] : 1;
+        const bool neg = (r.meta & 4) != 0;
+        // samples on planes [0, na)
+        int s0 = neg ? r.P0 - na + 1 : -r.P0;
+        int s1 = neg ? r.P0 + 1 : na - r.P0;
+        s0 = s0 > 0 ? s0 : 0;
+        s1 = s1 < r.nsteps ? s1 : r.nsteps;
+        const float len = r.len;
+        float acc = 0.0f;
+        if (s0 >= s1)
+            return acc;
+        // byte steps of the F + 1 taps along b1 and b2
+        const uint32_t d1 = (D == 3 && FB2) ? V.RS4 : 4u;
+        const uint32_t d2 = (D == 3 && FB2) ? 4u : V.RS4;
+        uint32_t poff = (uint32_t) (neg ? r.P0 - s0 : r.P0 + s0) * V.PS4;
+        const uint32_t dp = neg ? 0u - V.PS4 : V.PS4;
+        for (int s = s0; s < s1; ++s, poff += dp) {
+            const float fs = (float) s;
+            const float q1 = __builtin_fmaf(fs, r.s1, r.p1);
+            const float g1 = floorf(q1 - 0.5f);
+            const float c1 = (q1 - g1) - 0.5f;
+            const float w1 = 1.0f - c1;
+            const uint32_t i1 = pad_coord(g1, n1);
+            if (D == 2) {
+                const uint32_t o = poff + (i1 << 2);
+                const float vf = ld_view(V, o), vk = ld_view(V, o + 4u);
+                // A = 0: (pa, C1) then (pa, F1); A = 1: (F0, pa) then (C0, pa)
+                if (A == 0) {
+                    acc += vk * len * c1;
+                    acc += vf * len * w1;
+                } else {
+                    acc += vf * len * w1;
+                    acc += vk * len * c1;
+                }
+            } else {
+                const float q2 = __builtin_fmaf(fs, r.s2, r.p2);
+                const float g2 = floorf(q2 - 0.5f);
+                const float c2 = (q2 - g2) - 0.5f;
+                const float w2 = 1.0f - c2;
+                const uint32_t i2 = pad_coord(g2, n2);
+                const uint32_t o = FB2 ? poff + mul24(i1, V.RS4) + (i2 << 2) : poff + mul24(i2, V.RS4) + (i1 << 2);
+                const float vff = ld_view(V, o);
+                const float vkf = ld_view(V, o + d1);
+                const float vfk = ld_view(V, o + d2);
+                const float vkk = ld_view(V, o + d1 + d2);
+                const float wff = w1 * w2, wkk = c1 * c2, wfk = w1 * c2, wkf = c1 * w2;
+                acc += vff * len * wff;
+                if (A == 0) {
+                    acc += vkk * len * wkk;
+                    acc += vfk * len * wfk;
+                    acc += vkf * len * wkf;
+                } else if (A == 1) {
+                    acc += vfk * len * wfk;
+                    acc += vkk * len * wkk;
+                    acc += vkf * len * wkf;
+                } else {
+                    acc += vfk * len * wfk;
+                    acc += vkf * len * wkf;
+                    acc += vkk * len * wkk;
+                }
+            }
+        }
+        return acc;
     }
 } // namespace ehp

@@ -10,11 +10,13 @@
 //
 // Accumulation: LDS float atomics are slow on this part (a ds_add_f32 wave operation costs
 // about 15 times a ds_add_u64), so the tile is summed with 64-bit integer atomics in fixed
-// point, scale q = 2^k chosen per call from max|y| and a bound on what one slot can receive,
-// so that no sum can overflow. Integer sums do not depend on the order of the additions, so
-// the result is bitwise reproducible. Each contribution is rounded once, to a relative 2^-24
-// when it is formed in FP32 and to q/2 <= 2^-62 * max|y| * bound when it is quantised; the
-// bound is at most 2^32, so the sums are more accurate than a sequential FP32 accumulation.
+// point: units of q = 2^(eg - 62), eg chosen per call from max|y| and a bound on what one slot
+// can receive, so that no sum can overflow. Each ray rounds its contributions to int32 in its
+// own power-of-two scale (relative error 2^-31 of y * len) and shifts them into units of q,
+// which costs a few instructions instead of a float-to-int64 conversion. Integer sums do not
+// depend on the order of the additions, so the result is bitwise reproducible, and they are
+// more accurate than a sequential FP32 accumulation. A non-finite y makes the whole output
+// NaN (the fixed-point sums cannot carry it to just the voxels it touches).
 //
 // The per-ray state comes from a table built once per operator from the same functions as
 // the one-thread-per-ray kernels (make_ray, joseph_setup), and the per-sample weights use the
@@ -78,13 +80,10 @@ namespace ehp
             atomicMax(out, __float_as_uint(m)); // |x| >= 0: the bit patterns order like the values (NaN last)
     }
 
-    // scale[0] = q, scale[1] = 1/q for the slot bound of the operator
-    __global__ void scale_kernel(const unsigned int* __restrict__ maxbits, double bound, float* __restrict__ scale)
+    // the fixed-point exponent eg for the slot bound of the operator
+    __global__ void scale_kernel(const unsigned int* __restrict__ maxbits, double bound, int* __restrict__ fixexp)
     {
-        float q, iq;
-        fixed_point_scale(__uint_as_float(*maxbits), bound, q, iq);
-        scale[0] = q;
-        scale[1] = iq;
+        fixexp[0] = fixed_point_exponent(__uint_as_float(*maxbits), bound);
     }
 
     // ---------------------------------------------------------------------------------
@@ -95,12 +94,13 @@ namespace ehp
         joseph_adj_tiled_kernel(const PoseGeom* __restrict__ geo, const RayRec* __restrict__ table, const Dims dims,
                                 const int* __restrict__ pose_axes, const TileCfg tc, const int ntA, const int ntB,
                                 const int ntC, const int tiles_per_xcd, const int batch,
-                                const float* __restrict__ scale, const float* __restrict__ sino,
+                                const int* __restrict__ fixexp, const float* __restrict__ sino,
                                 float* __restrict__ vol)
     {
         extern __shared__ long long lds[];
         __shared__ int s_u0[kMaxPoseBatch], s_nu[kMaxPoseBatch], s_v0[kMaxPoseBatch], s_pose[kMaxPoseBatch];
         __shared__ int s_cnt[kMaxPoseBatch], s_pref[kMaxPoseBatch + 1];
+        __shared__ float s_rnu[kMaxPoseBatch];
 
         // consecutive workgroups land on different XCDs: give each XCD a contiguous block of tiles
         const int tile = (int) (blockIdx.x % 8u) * tiles_per_xcd + (int) (blockIdx.x / 8u);
@@ -111,11 +111,10 @@ namespace ehp
         long long* iacc = lds;
         for (int i = tid; i < t.nslots; i += kThreads)
             iacc[i] = 0;
-        const float q = scale[0];
-        const float iq = scale[1];
+        const int eg = fixexp[0];
         __syncthreads();
 
-        for (int pbase = 0; pbase < dims.nposes; pbase += batch) {
+        for (int pbase = 0; eg != kNonFinite && pbase < dims.nposes; pbase += batch) {
             if (tid < batch) {
                 const int p = pbase + tid;
                 int u0 = 0, u1 = -1, v0 = 0, v1 = -1;
@@ -126,6 +125,7 @@ namespace ehp
                 }
                 s_u0[tid] = u0;
                 s_nu[tid] = u1 - u0 + 1;
+                s_rnu[tid] = 1.0f / (float) (u1 - u0 + 1);
                 s_v0[tid] = v0;
                 s_pose[tid] = p;
                 s_cnt[tid] = cnt;
@@ -143,26 +143,32 @@ namespace ehp
                 while (k >= s_pref[j + 1])
                     ++j;
                 const int loc = k - s_pref[j];
-                const int u = s_u0[j] + loc % s_nu[j];
-                const int v = s_v0[j] + loc / s_nu[j];
+                // loc / nu through the reciprocal: exact, as loc + 0.5 is at least 0.5 away from
+                // a multiple of nu and loc < 2^22
+                const int row = (int) (((float) loc + 0.5f) * s_rnu[j]);
+                const int u = s_u0[j] + (loc - row * s_nu[j]);
+                const int v = s_v0[j] + row;
                 const int64_t ray = (int64_t) s_pose[j] * dims.pix_per_pose + (int64_t) v * dims.det[0] + u;
                 const float yv = sino[ray];
                 if (yv == 0.0f)
                     continue;
                 const RayRec r = table[ray];
-                tile_ray_adjoint<D, A>(r, t, yv, [&](int slot, float val) {
-                    atomicAdd(reinterpret_cast<unsigned long long*>(iacc + slot),
-                              (unsigned long long) (long long) rintf(val * iq));
+                float ys;
+                int sh;
+                ray_scale(yv * r.len, eg, ys, sh);
+                tile_ray_adjoint<D, A>(r, t, ys, [&](int slot, float val) {
+                    atomicAdd(reinterpret_cast<unsigned long long*>(iacc + slot), fixed_contrib(val, sh));
                 });
             }
             __syncthreads();
         }
 
         // add the owned voxels of the tile to the volume
+        const double q = eg == kNonFinite ? __longlong_as_double(0x7ff8000000000000ll) : ldexp(1.0, eg - 62);
         for (int i = tid; i < t.nslots; i += kThreads) {
             const int64_t lin = tile_slot_voxel<D, A>(t, dims, i);
             if (lin >= 0)
-                vol[lin] += (float) ((double) iacc[i] * (double) q);
+                vol[lin] += (float) ((double) iacc[i] * q);
         }
     }
 
@@ -198,7 +204,7 @@ namespace ehp
 
     hipError_t joseph_adjoint_tiled(const PoseGeom* geo, const RayRec* table, const Dims& dims, const int* pose_axes,
                                     int axes_present, const TileCfg& tc, int batch, double slot_bound,
-                                    unsigned int* d_maxbits, float* d_scale, const float* sino, float* vol,
+                                    unsigned int* d_maxbits, int* d_fixexp, const float* sino, float* vol,
                                     hipStream_t s)
     {
         if (batch < 1 || batch > kMaxPoseBatch)
@@ -211,7 +217,7 @@ namespace ehp
         int64_t blocks = (dims.nrays + 255) / 256;
         blocks = blocks > 4096 ? 4096 : blocks;
         hipLaunchKernelGGL(absmax_kernel, dim3((unsigned) blocks), dim3(256), 0, s, sino, dims.nrays, d_maxbits);
-        hipLaunchKernelGGL(scale_kernel, dim3(1), dim3(1), 0, s, d_maxbits, slot_bound, d_scale);
+        hipLaunchKernelGGL(scale_kernel, dim3(1), dim3(1), 0, s, d_maxbits, slot_bound, d_fixexp);
         e = hipGetLastError();
         if (e != hipSuccess)
             return e;
@@ -225,7 +231,7 @@ namespace ehp
             const dim3 grid((unsigned) (per_xcd * 8));
 #define EHP_ADJ_TILED(Dv, Av)                                                                                      \
     hipLaunchKernelGGL((joseph_adj_tiled_kernel<Dv, Av>), grid, dim3(kThreads), lds, s, geo, table, dims, pose_axes, \
-                       tc, ntA, ntB, ntC, per_xcd, batch, d_scale, sino, vol)
+                       tc, ntA, ntB, ntC, per_xcd, batch, d_fixexp, sino, vol)
             if (dims.dim == 2) {
                 if (a == 0)
                     EHP_ADJ_TILED(2, 0);

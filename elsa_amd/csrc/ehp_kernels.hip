@@ -41,11 +41,13 @@ namespace ehp
     template <int D, int A, class T>
     __device__ inline T joseph_ray_forward(const JRay& jr, const Dims& dims, const T* __restrict__ vol)
     {
+        // elsa's association: acc += (value * len) * weight per tap
+        const T len = (T) jr.len;
         T acc = T(0);
         for (int s = 0; s < jr.nsteps; ++s) {
             JSample js;
             joseph_sample<D, A>(jr, s, js);
-            joseph_taps<D>(js, dims.vol, dims.strides, [&](int64_t lin, float wt) { acc += vol[lin] * (T) wt; });
+            joseph_taps<D>(js, dims.vol, dims.strides, [&](int64_t lin, float wt) { acc += vol[lin] * len * (T) wt; });
         }
         return acc;
     }
@@ -85,7 +87,7 @@ namespace ehp
             acc = joseph_ray_forward<D, 1, T>(jr, dims, vol);
         else
             acc = joseph_ray_forward<D, D - 1, T>(jr, dims, vol);
-        sino[(int64_t) pose * dims.pix_per_pose + pix] = acc * (T) jr.len;
+        sino[(int64_t) pose * dims.pix_per_pose + pix] = acc;
     }
 
     template <int D, class T>

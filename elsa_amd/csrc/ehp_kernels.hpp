@@ -20,8 +20,12 @@ namespace ehp
     size_t joseph_tile_lds_bytes(const Dims& dims, const TileCfg& tc);
     hipError_t joseph_adjoint_tiled(const PoseGeom* geo, const RayRec* table, const Dims& dims, const int* pose_axes,
                                     int axes_present, const TileCfg& tc, int batch, double slot_bound,
-                                    unsigned int* d_maxbits, float* d_scale, const float* sino, float* vol,
+                                    unsigned int* d_maxbits, int* d_fixexp, const float* sino, float* vol,
                                     hipStream_t s);
+    hipError_t joseph_view_copy(const Dims& dims, const PadLayout& L, const float* vol, float* view, hipStream_t s);
+    hipError_t joseph_forward_planes(const PoseGeom* geo, const Dims& dims, const PadLayout* layouts,
+                                     float* const* views, int axes_present, const float* vol, float* sino,
+                                     hipStream_t s);
 
     template <class T>
     hipError_t siddon_forward(const PoseGeom* geo, const Dims& dims, const T* vol, T* sino, hipStream_t s);

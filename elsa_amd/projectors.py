@@ -147,12 +147,17 @@ class _HipProjector(LinearOperator):
 class JosephsMethodCUDA(_HipProjector):
     """Joseph's method with the discretization of elsa's CPU JosephsMethod
     (JosephsMethod.cpp:140-198) and its matched adjoint. ``fast`` is accepted for
-    interface compatibility; the adjoint is always the matched one."""
+    interface compatibility; the adjoint is always the matched one.
 
-    def __init__(self, domainDescriptor, rangeDescriptor, fast: bool = True, dtype=np.float32, kernels: str = "tiled"):
+    ``kernels="fast"`` (FP32) runs the forward over plane-major volume views and the back
+    projection over LDS-privatised tiles with 64-bit fixed-point sums; ``"simple"`` runs one
+    thread per ray for both (the back projection with float atomics). FP64 always uses the
+    simple kernels."""
+
+    def __init__(self, domainDescriptor, rangeDescriptor, fast: bool = True, dtype=np.float32, kernels: str = "fast"):
         self.fast = bool(fast)
-        if kernels not in ("tiled", "simple"):
-            raise ValueError("kernels must be 'tiled' or 'simple'")
+        if kernels not in ("fast", "simple"):
+            raise ValueError("kernels must be 'fast' or 'simple'")
         self.kernels = kernels
         super().__init__(domainDescriptor, rangeDescriptor, dtype)
 

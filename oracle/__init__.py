@@ -45,6 +45,8 @@ def lib():
     c_int, c_i64, c_f, c_d = ctypes.c_int, ctypes.c_int64, ctypes.c_float, ctypes.c_double
     L.orc_xray_f32.argtypes = [c_int, c_int, c_int, c_int, _i64p, _i64p, c_i64, _f32p, _f32p, _f32p, _f32p, c_int]
     L.orc_xray_f64.argtypes = [c_int, c_int, c_int, c_int, _i64p, _i64p, c_i64, _f64p, _f64p, _f64p, _f64p, c_int]
+    L.orc_xray_f32_acc64.argtypes = [c_int, c_int, c_int, c_int, _i64p, _i64p, c_i64, _f32p, _f32p, _f64p, _f64p,
+                                     c_int]
     L.orc_voxel_f32.argtypes = [c_int, c_int, _i64p, _i64p, c_i64, _f32p, _f32p, _f32p, _f32p, c_int, c_f,
                                 _f32p, _f32p, c_int]
     L.orc_voxel_f64.argtypes = [c_int, c_int, _i64p, _i64p, c_i64, _f64p, _f64p, _f64p, _f64p, c_int, c_d,
@@ -198,12 +200,16 @@ def _dims(g: Geom):
     return vol, det
 
 
-def xray(kind, g: Geom, data, adjoint=False, mode=MODE_ELSA, nthreads=0):
+def xray(kind, g: Geom, data, adjoint=False, mode=MODE_ELSA, nthreads=0, acc64=False):
     """Joseph (kind=JOSEPH; JosephsMethod.cpp:140-198) or Siddon (kind=SIDDON;
-    SiddonsMethod.cpp:76-124) forward (adjoint=False) or back projection."""
+    SiddonsMethod.cpp:76-124) forward (adjoint=False) or back projection. acc64 with an FP32
+    geometry keeps the FP32 geometry and weights but takes FP64 data and sums (the exact sum
+    of the FP32 terms)."""
     L = lib()
     dt = np.float64 if g.Pinv.dtype == np.float64 else np.float32
     f = L.orc_xray_f64 if dt == np.float64 else L.orc_xray_f32
+    if acc64 and dt == np.float32:
+        f, dt = L.orc_xray_f32_acc64, np.float64
     vol, det = _dims(g)
     data = np.ascontiguousarray(data, dtype=dt)
     out = np.empty(g.vol_shape if adjoint else g.sino_shape, dt)
@@ -213,8 +219,8 @@ def xray(kind, g: Geom, data, adjoint=False, mode=MODE_ELSA, nthreads=0):
     return out
 
 
-def joseph(g, data, adjoint=False, mode=MODE_ELSA, nthreads=0):
-    return xray(JOSEPH, g, data, adjoint, mode, nthreads)
+def joseph(g, data, adjoint=False, mode=MODE_ELSA, nthreads=0, acc64=False):
+    return xray(JOSEPH, g, data, adjoint, mode, nthreads, acc64)
 
 
 def siddon(g, data, adjoint=False, nthreads=0):

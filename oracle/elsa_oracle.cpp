@@ -974,6 +974,17 @@ int orc_xray_f64(int kind, int dim, int adjoint, int mode, const int64_t* vol, c
                                                    Pinv, C, in, out);
 }
 
+// FP32 geometry and weights with FP64 data and sums: the exact sum of the FP32 terms, the
+// reference for how much a summation order or format loses.
+int orc_xray_f32_acc64(int kind, int dim, int adjoint, int mode, const int64_t* vol, const int64_t* det,
+                       int64_t nviews, const float* Pinv, const float* C, const double* in, double* out,
+                       int nthreads)
+{
+    orc::set_threads(nthreads);
+    return orc::dispatch_projector<float, double>(kind, dim, adjoint, mode, vol, det, nviews, Pinv,
+                                                  C, in, out);
+}
+
 // Voxel projector (blob or B-spline LUT). P, ext: nviews blocks of dim x (dim+1).
 int orc_voxel_f32(int dim, int adjoint, const int64_t* vol, const int64_t* det, int64_t nviews,
                   const float* P, const float* ext, const float* sdd, const float* lut, int N,

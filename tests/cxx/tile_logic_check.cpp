@@ -4,14 +4,21 @@
 // kernels call), it builds the per-ray table, walks every tile of every pass the way a
 // workgroup does -- pose batches, 64-bit fixed-point sums -- with bounds checks on every LDS
 // slot and volume index, a check that no sum leaves the int64 range, and a check that each
-// voxel is written by exactly one tile per pass. Used by
-// tests/test_tile_logic.py to compare against the oracle.
+// voxel is written by exactly one tile per pass. It also runs the plane-major forward
+// (plane_layout, plane_view, plane_ray_forward) over every layout a driving axis can get.
+// Used by tests/test_tile_logic.py to compare against the oracle.
+#define EHP_VIEW_CHECK 1
 #include "ehp_joseph_tile.hpp"
 
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
+
+namespace ehp
+{
+    bool g_view_oob = false;
+}
 
 using namespace ehp;
 
@@ -41,8 +48,7 @@ namespace
 
     template <int D, int A>
     int run_pass(const Dims& dims, const std::vector<PoseGeom>& geo, const std::vector<RayRec>& table,
-                 const std::vector<int>& axes, TileCfg tc, float q, float iq, const float* sino, float* vol,
-                 Stats& st)
+                 const std::vector<int>& axes, TileCfg tc, int eg, const float* sino, float* vol, Stats& st)
     {
         std::vector<int> owner(dims.nvox, 0);
         int ntA, ntB, ntC;
@@ -68,17 +74,22 @@ namespace
                             if (yv == 0.0f)
                                 continue;
                             bool bad = false;
-                            tile_ray_adjoint<D, A>(table[ray], t, yv, [&](int slot, float val) {
+                            float ys;
+                            int sh;
+                            ray_scale(yv * table[ray].len, eg, ys, sh);
+                            if (sh < 0 || sh > 32)
+                                bad = true;
+                            tile_ray_adjoint<D, A>(table[ray], t, ys, [&](int slot, float val) {
                                 if (slot < 0 || slot >= t.nslots) {
                                     bad = true;
                                     return;
                                 }
-                                const float z = rintf(val * iq);
-                                if (!(fabsf(z) < 9.2e18f)) {
+                                // the contribution must round to an int32 of magnitude <= 2^30
+                                if (!(fabsf(rintf(val)) <= 1073741824.0f)) {
                                     bad = true;
                                     return;
                                 }
-                                iacc[slot] += (int64_t) z;
+                                iacc[slot] += (__int128) (long long) fixed_contrib(val, sh);
                             });
                             if (bad) {
                                 std::fprintf(stderr, "LDS slot or contribution out of range (pass %d tile %d pose %d u %d v %d)\n", A,
@@ -105,7 +116,7 @@ namespace
                     return 4;
                 }
                 owner[lin] += 1;
-                vol[lin] += (float) ((double) (int64_t) iacc[i] * (double) q);
+                vol[lin] += (float) ((double) (int64_t) iacc[i] * ldexp(1.0, eg - 62));
             }
         }
         for (int64_t i = 0; i < dims.nvox; ++i) {
@@ -145,9 +156,8 @@ namespace
         float m = 0.0f;
         for (int64_t i = 0; i < dims.nrays; ++i)
             m = fabsf(sino[i]) > m ? fabsf(sino[i]) : m;
-        float q, iq;
-        fixed_point_scale(m, bound, q, iq);
-        info[1] = q;
+        const int eg = fixed_point_exponent(m, bound);
+        info[1] = ldexp(1.0, eg - 62);
         for (int64_t i = 0; i < dims.nvox; ++i)
             vol[i] = 0.0f;
         Stats st;
@@ -156,11 +166,11 @@ namespace
                 continue;
             int rc = 0;
             if (a == 0)
-                rc = run_pass<D, 0>(dims, geo, table, axes, tc, q, iq, sino, vol, st);
+                rc = run_pass<D, 0>(dims, geo, table, axes, tc, eg, sino, vol, st);
             else if (a == 1)
-                rc = run_pass<D, 1>(dims, geo, table, axes, tc, q, iq, sino, vol, st);
+                rc = run_pass<D, 1>(dims, geo, table, axes, tc, eg, sino, vol, st);
             else
-                rc = run_pass<D, (D == 3 ? 2 : 1)>(dims, geo, table, axes, tc, q, iq, sino, vol, st);
+                rc = run_pass<D, (D == 3 ? 2 : 1)>(dims, geo, table, axes, tc, eg, sino, vol, st);
             if (rc)
                 return rc;
             info[3 + a] = 1;
@@ -190,4 +200,76 @@ extern "C" int tile_adjoint(int dim, const int64_t* vol_dims, const int64_t* det
     }
     const TileCfg tc{LA, TB, TC};
     return dim == 2 ? run<2>(dims, geo, tc, sino, vol, info) : run<3>(dims, geo, tc, sino, vol, info);
+}
+
+// Forward projection through zero-padded plane-major views: fan[a] is the fastest axis of the
+// view of driving axis a. The views are filled by the index map of the view copy kernels
+// (pad_index), every offset checked against the view and written at most once; the padding
+// stays zero.
+template <int D, int A>
+static float fwd_one(const RayRec& r, const Dims& dims, const PadView& V, bool fb2)
+{
+    return fb2 ? plane_ray_forward<D, A, true>(r, dims, V) : plane_ray_forward<D, A, false>(r, dims, V);
+}
+
+extern "C" int plane_forward(int dim, const int64_t* vol_dims, const int64_t* det_dims, int nposes, const float* P,
+                             const float* Pinv, const float* C, const int* fan, const float* vol, float* sino)
+{
+    const Dims dims = make_dims(dim, vol_dims, det_dims, nposes);
+    const int nm = dim * (dim + 1);
+    std::vector<PoseGeom> geo(nposes);
+    for (int p = 0; p < nposes; ++p) {
+        PoseGeom g{};
+        for (int i = 0; i < nm; ++i) {
+            g.P[i] = P[p * nm + i];
+            g.Pinv[i] = Pinv[p * nm + i];
+        }
+        for (int i = 0; i < dim; ++i)
+            g.C[i] = C[p * dim + i];
+        geo[p] = g;
+    }
+    g_view_oob = false;
+    std::vector<std::vector<float>> views(3);
+    PadView pv[3];
+    bool fb2[3] = {false, false, false};
+    for (int a = 0; a < dim; ++a) {
+        if (fan[a] == a || fan[a] < 0 || fan[a] > (dim == 3 ? 2 : 1))
+            return 20;
+        const PadLayout L = pad_layout(dims, a, fan[a]);
+        views[a].assign(L.size, 0.0f);
+        std::vector<char> seen(L.size, 0);
+        for (int z = 0; z < dims.vol[2]; ++z)
+            for (int y = 0; y < dims.vol[1]; ++y)
+                for (int x = 0; x < dims.vol[0]; ++x) {
+                    const int64_t o = pad_index(L, x, y, z);
+                    if (o < 0 || o >= L.size || seen[o])
+                        return 21; // out of range or written twice
+                    seen[o] = 1;
+                    views[a][o] = vol[x + dims.strides[1] * y + dims.strides[2] * z];
+                }
+        pv[a] = PadView{views[a].data(), L.PS * 4u, L.RS * 4u, L.size};
+        fb2[a] = dim == 3 && fan[a] == (a == 2 ? 1 : 2);
+    }
+    for (int p = 0; p < nposes; ++p)
+        for (int v = 0; v < dims.det[1]; ++v)
+            for (int u = 0; u < dims.det[0]; ++u) {
+                float o[3], d[3];
+                JRay jr;
+                float val;
+                if (dim == 2) {
+                    make_ray<2>(geo[p], u, v, o, d);
+                    joseph_setup<2>(o, d, dims.vol, jr);
+                    const RayRec r = make_ray_rec<2>(jr);
+                    val = jr.axis == 0 ? fwd_one<2, 0>(r, dims, pv[0], false) : fwd_one<2, 1>(r, dims, pv[1], false);
+                } else {
+                    make_ray<3>(geo[p], u, v, o, d);
+                    joseph_setup<3>(o, d, dims.vol, jr);
+                    const RayRec r = make_ray_rec<3>(jr);
+                    val = jr.axis == 0   ? fwd_one<3, 0>(r, dims, pv[0], fb2[0])
+                          : jr.axis == 1 ? fwd_one<3, 1>(r, dims, pv[1], fb2[1])
+                                         : fwd_one<3, 2>(r, dims, pv[2], fb2[2]);
+                }
+                sino[(int64_t) p * dims.pix_per_pose + (int64_t) v * dims.det[0] + u] = val;
+            }
+    return g_view_oob ? 22 : 0;
 }

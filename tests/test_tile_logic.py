@@ -26,6 +26,8 @@ def lib():
     f64 = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
     L.tile_adjoint.argtypes = [ctypes.c_int, i64, i64, ctypes.c_int, f32, f32, f32, ctypes.c_int, ctypes.c_int,
                                ctypes.c_int, f32, f32, f64]
+    L.plane_forward.argtypes = [ctypes.c_int, i64, i64, ctypes.c_int, f32, f32, f32,
+                                np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS"), f32, f32]
     return L
 
 
@@ -57,6 +59,71 @@ def test_tiled_adjoint_logic_matches_oracle(lib, case):
     ref = oracle.joseph(g, y, adjoint=True, mode=oracle.MODE_DIRECT)
     err = np.linalg.norm(out - ref) / np.linalg.norm(ref)
     assert err <= 1e-6, err
+    # the fixed-point sums are at least as accurate as the oracle's sequential FP32 sums, both
+    # measured against the exact sum of the same FP32 terms
+    exact = oracle.joseph(g, y, adjoint=True, mode=oracle.MODE_DIRECT, acc64=True)
+    e_fix = np.linalg.norm(out - exact) / np.linalg.norm(exact)
+    e_f32 = np.linalg.norm(ref - exact) / np.linalg.norm(exact)
+    assert e_fix <= e_f32, (e_fix, e_f32)
     assert info[3:].sum() >= 1
     # slot sums stay well inside int64 (the bound is a worst case)
     assert 0 < info[2] < 0.5, info
+
+
+FWD_CASES = [
+    # dim, vol, views, det, s2c, c2d, fan axes to try per driving axis
+    (3, (40, 24, 33), 13, (61, 29), 60.0, 40.0, [(2, 2, 1), (1, 0, 0), (2, 0, 1)]),
+    (3, (17, 9, 23), 7, (30, 20), 25.0, 25.0, [(2, 2, 1), (1, 0, 0)]),
+    (3, (64, 64, 64), 9, (96, 96), 6400.0, 320.0, [(2, 2, 1)]),
+    (2, (31, 17), 11, (80,), 40.0, 30.0, [(1, 0)]),
+    (2, (64, 64), 30, (90,), 6400.0, 320.0, [(1, 0)]),
+]
+
+
+@pytest.mark.parametrize("case", FWD_CASES, ids=lambda c: f"{c[0]}d-{'x'.join(map(str, c[1]))}")
+def test_plane_forward_logic_matches_oracle_bitwise(lib, case):
+    """Every zero-padded plane-major view a driving axis can get gives the oracle's forward, bit
+    for bit (same positions, taps and summation order as the one-thread-per-ray rule), with
+    every read inside its view."""
+    dim, vol, views, det, s2c, c2d, fans = case
+    rng = np.random.default_rng(2)
+    angles = np.sort(rng.uniform(0, 360, views)).astype(np.float32)
+    g = oracle.circle_geometry(dim, vol, views, 360, s2c, c2d, det=det, angles_deg=angles)
+    x = rng.standard_normal(g.vol_shape).astype(np.float32)
+    ref = oracle.joseph(g, x, mode=oracle.MODE_DIRECT)
+    for fan in fans:
+        out = np.full(g.sino_shape, np.nan, np.float32)
+        rc = lib.plane_forward(dim, np.array(list(vol) + [1] * (3 - dim), np.int64),
+                               np.array(list(det) + [1] * (2 - len(det)), np.int64), views,
+                               np.ascontiguousarray(g.P), np.ascontiguousarray(g.Pinv), np.ascontiguousarray(g.C),
+                               np.array(list(fan) + [0] * (3 - len(fan)), np.int32), x, out)
+        assert rc == 0, (fan, rc)
+        np.testing.assert_array_equal(out, ref, err_msg=f"fan axes {fan}")
+
+
+def test_tiled_adjoint_logic_high_dynamic_range(lib):
+    """One ray a million times larger than the rest. Each term is rounded in its own ray's
+    scale, so the documented bound holds voxel by voxel: |error| <= 2^-31 * sum over the rays
+    that reach the voxel of |y * len| (times 1.01), plus one FP32 rounding of the result; and
+    overall the sums are closer to the exact sums than sequential FP32 sums are."""
+    dim, vol, views, det, s2c, c2d, tile = CASES[1]
+    rng = np.random.default_rng(5)
+    angles = np.sort(rng.uniform(0, 360, views)).astype(np.float32)
+    g = oracle.circle_geometry(dim, vol, views, 360, s2c, c2d, det=det, angles_deg=angles)
+    y = rng.standard_normal(g.sino_shape).astype(np.float32)
+    y.flat[len(y.flat) // 2 + 17] = 3.0e6
+    out = np.zeros(g.vol_shape, np.float32)
+    info = np.zeros(6, np.float64)
+    rc = lib.tile_adjoint(dim, np.array(list(vol), np.int64), np.array(list(det), np.int64), views,
+                          np.ascontiguousarray(g.P), np.ascontiguousarray(g.Pinv), np.ascontiguousarray(g.C),
+                          tile[0], tile[1], tile[2], y, out, info)
+    assert rc == 0, rc
+    exact = oracle.joseph(g, y, adjoint=True, mode=oracle.MODE_DIRECT, acc64=True)
+    ref32 = oracle.joseph(g, y, adjoint=True, mode=oracle.MODE_DIRECT)
+    # a ray reaches a voxel with at most one sample (len <= sqrt 3), and at most info[0] rays
+    # (the slot bound) reach one voxel: the large ray plus that many ordinary ones
+    big, rest = np.sort(np.abs(y).ravel())[-1:-3:-1]
+    bound = 1.01 * 2.0 ** -31 * np.sqrt(3.0) * (float(big) + info[0] * float(rest))
+    err = np.abs(out - exact)
+    assert np.all(err <= bound + np.spacing(np.abs(exact).astype(np.float32))), float(err.max())
+    assert np.linalg.norm(out - exact) <= np.linalg.norm(ref32 - exact)
