@@ -94,23 +94,27 @@ namespace ehp
         return t;
     }
 
-    // Candidate pixels of one pose for a tile: the bounding rectangle of the projected corners
-    // of [A0, A1] x [B0 - 1, B1 + 1] x [C0 - 1, C1 + 1], widened by one pixel. A point X lies on
-    // the ray of pixel u when P X (homogeneous) maps to u + 0.5. Falls back to the whole
-    // detector when a corner is not in front of the source.
+    // Candidate pixels of one pose for a tile. A ray touches the tile's slots only with a
+    // sample point in [A0 + 0.5, A1 - 0.5] x [B0 - 0.5, B1 + 0.5) x [C0 - 0.5, C1 + 0.5)
+    // (sample points lie on plane centres; floor(q - 0.5) in [B0 - 1, B1 - 1] is
+    // q in [B0 - 0.5, B1 + 0.5)). A ray through that box projects inside the projection of the
+    // box, which lies in the bounding rectangle of its projected corners; a point X lies on
+    // the ray of pixel u when P X (homogeneous) maps to u + 0.5. The rectangle is widened by a
+    // quarter pixel for rounding. Falls back to the whole detector when a corner is not in
+    // front of the source.
     template <int D, int A>
     EHP_HD inline bool tile_bbox(const PoseGeom& g, const TileBox& t, const Dims& dims, int& u0, int& u1, int& v0,
                                  int& v1)
     {
         constexpr int b1 = TileAxes<D, A>::b1, b2 = TileAxes<D, A>::b2;
         float lo[3], hi[3];
-        lo[A] = (float) t.A0;
-        hi[A] = (float) t.A1;
-        lo[b1] = (float) (t.B0 - 1);
-        hi[b1] = (float) (t.B1 + 1);
+        lo[A] = (float) t.A0 + 0.5f;
+        hi[A] = (float) t.A1 - 0.5f;
+        lo[b1] = (float) t.B0 - 0.5f;
+        hi[b1] = (float) t.B1 + 0.5f;
         if (D == 3) {
-            lo[b2] = (float) (t.C0 - 1);
-            hi[b2] = (float) (t.C1 + 1);
+            lo[b2] = (float) t.C0 - 0.5f;
+            hi[b2] = (float) t.C1 + 0.5f;
         }
         float umin = 3.0e38f, umax = -3.0e38f, vmin = 3.0e38f, vmax = -3.0e38f;
         bool front = true;
@@ -127,7 +131,7 @@ namespace ehp
             }
             const float den = p[D - 1];
             front = front && (den > 0.0f);
-            // approximate reciprocal: the rectangle is widened by a pixel anyway
+            // approximate reciprocal: a few ulp, covered by the quarter-pixel margin
             const float rden = ehp_rcp(den);
             const float pu = p[0] * rden;
             umin = pu < umin ? pu : umin;
@@ -148,15 +152,16 @@ namespace ehp
         const float W = (float) dims.det[0] + 4.0f, H = (float) dims.det[1] + 4.0f;
         umin = umin < -4.0f ? -4.0f : (umin > W ? W : umin);
         umax = umax < -4.0f ? -4.0f : (umax > W ? W : umax);
-        u0 = (int) floorf(umin - 0.5f) - 1;
-        u1 = (int) ceilf(umax - 0.5f) + 1;
+        // pixels u with u + 0.5 in [umin - 0.25, umax + 0.25]
+        u0 = (int) ceilf(umin - 0.75f);
+        u1 = (int) floorf(umax - 0.25f);
         u0 = u0 < 0 ? 0 : u0;
         u1 = u1 > dims.det[0] - 1 ? dims.det[0] - 1 : u1;
         if (D == 3) {
             vmin = vmin < -4.0f ? -4.0f : (vmin > H ? H : vmin);
             vmax = vmax < -4.0f ? -4.0f : (vmax > H ? H : vmax);
-            v0 = (int) floorf(vmin - 0.5f) - 1;
-            v1 = (int) ceilf(vmax - 0.5f) + 1;
+            v0 = (int) ceilf(vmin - 0.75f);
+            v1 = (int) floorf(vmax - 0.25f);
             v0 = v0 < 0 ? 0 : v0;
             v1 = v1 > dims.det[1] - 1 ? dims.det[1] - 1 : v1;
         } else {
@@ -207,12 +212,49 @@ namespace ehp
             clip_range(r.p2, r.s2, (float) t.C0 - 0.5f, (float) t.C1 + 0.5f, slo, shi);
     }
 
-    // Back projection of one ray (driving axis A) into a tile: for every sample whose bilinear
-    // footprint touches the tile, hand ys * weight for each of its four (2D: two) slots to
-    // add(slot, value), ys being the ray's y * len in the ray's own fixed-point scale
-    // (ray_scale). Weights are elsa's bilinear weights (doInterpolation,
-    // JosephsMethod.cpp:60-97), the driving weight being 1; a 3D tap's weight is applied as
-    // (ys * w2) * w1.
+    // Back projection of sample s of one ray (driving axis A, transverse start p1, p2 and
+    // steps s1, s2) lying on tile plane pa: if its bilinear footprint touches the tile, hand
+    // ys * weight for each of its four (2D: two) slots to add(slot, value), ys being the
+    // ray's y * len in the ray's own fixed-point scale (ray_scale). Weights are elsa's
+    // bilinear weights (doInterpolation, JosephsMethod.cpp:60-97), the driving weight being 1;
+    // a 3D tap's weight is applied as (ys * w2) * w1.
+    template <int D, class Add>
+    EHP_HD inline void tile_sample_adjoint(float p1, float p2, float s1, float s2, int s, int pa, const TileBox& t,
+                                           float ys, Add add)
+    {
+        const float fs = (float) s;
+        const float q1 = __builtin_fmaf(fs, s1, p1);
+        const float g1 = floorf(q1 - 0.5f);
+        const int f1 = (int) g1;
+        if (f1 < t.B0 - 1 || f1 > t.B1 - 1)
+            return;
+        const float c1 = (q1 - g1) - 0.5f;
+        const float w1 = 1.0f - c1;
+        if (D == 2) {
+            const int base = (int) mul24((uint32_t) pa, (uint32_t) t.PB) + (f1 - (t.B0 - 1));
+            add(base, ys * w1);
+            add(base + 1, ys * c1);
+        } else {
+            const float q2 = __builtin_fmaf(fs, s2, p2);
+            const float g2 = floorf(q2 - 0.5f);
+            const int f2 = (int) g2;
+            if (f2 < t.C0 - 1 || f2 > t.C1 - 1)
+                return;
+            const float c2 = (q2 - g2) - 0.5f;
+            const float w2 = 1.0f - c2;
+            const int base = (int) (mul24((uint32_t) pa, (uint32_t) (t.PC * t.PB)) +
+                                    mul24((uint32_t) (f2 - (t.C0 - 1)), (uint32_t) t.PB)) +
+                             (f1 - (t.B0 - 1));
+            const float yw = ys * w2, yc = ys * c2;
+            add(base, yw * w1);
+            add(base + 1, yw * c1);
+            add(base + t.PB, yc * w1);
+            add(base + t.PB + 1, yc * c1);
+        }
+    }
+
+    // Back projection of one ray (driving axis A) into a tile: tile_sample_adjoint for each
+    // of its samples that can touch the tile.
     template <int D, int A, class Add>
     EHP_HD inline void tile_ray_adjoint(const RayRec& r, const TileBox& t, float ys, Add add)
     {
@@ -221,39 +263,8 @@ namespace ehp
         int slo, shi;
         tile_sample_range(r, t, D, slo, shi);
         const int sg = (r.meta & 4) ? -1 : 1;
-        const int PB = t.PB, PCB = t.PC * t.PB;
-        for (int s = slo; s <= shi; ++s) {
-            const float fs = (float) s;
-            const float q1 = __builtin_fmaf(fs, r.s1, r.p1);
-            const float g1 = floorf(q1 - 0.5f);
-            const int f1 = (int) g1;
-            if (f1 < t.B0 - 1 || f1 > t.B1 - 1)
-                continue;
-            const float c1 = (q1 - g1) - 0.5f;
-            const float w1 = 1.0f - c1;
-            const int pa = r.P0 + sg * s - t.A0;
-            if (D == 2) {
-                const int base = (int) mul24((uint32_t) pa, (uint32_t) PB) + (f1 - (t.B0 - 1));
-                add(base, ys * w1);
-                add(base + 1, ys * c1);
-            } else {
-                const float q2 = __builtin_fmaf(fs, r.s2, r.p2);
-                const float g2 = floorf(q2 - 0.5f);
-                const int f2 = (int) g2;
-                if (f2 < t.C0 - 1 || f2 > t.C1 - 1)
-                    continue;
-                const float c2 = (q2 - g2) - 0.5f;
-                const float w2 = 1.0f - c2;
-                const int base = (int) (mul24((uint32_t) pa, (uint32_t) PCB) +
-                                        mul24((uint32_t) (f2 - (t.C0 - 1)), (uint32_t) PB)) +
-                                 (f1 - (t.B0 - 1));
-                const float yw = ys * w2, yc = ys * c2;
-                add(base, yw * w1);
-                add(base + 1, yw * c1);
-                add(base + PB, yc * w1);
-                add(base + PB + 1, yc * c1);
-            }
-        }
+        for (int s = slo; s <= shi; ++s)
+            tile_sample_adjoint<D>(r.p1, r.p2, r.s1, r.s2, s, r.P0 + sg * s - t.A0, t, ys, add);
     }
 
     // Linear volume index of LDS slot i if the tile owns that voxel, else -1.

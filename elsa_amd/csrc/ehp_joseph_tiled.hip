@@ -138,7 +138,8 @@ namespace ehp
             }
             __syncthreads();
             const int total = s_pref[batch];
-            for (int k = tid; k < total; k += kThreads) {
+            // ray index of candidate k of this batch
+            auto cand_ray = [&](int k) -> int64_t {
                 int j = 0;
                 while (k >= s_pref[j + 1])
                     ++j;
@@ -148,11 +149,27 @@ namespace ehp
                 const int row = (int) (((float) loc + 0.5f) * s_rnu[j]);
                 const int u = s_u0[j] + (loc - row * s_nu[j]);
                 const int v = s_v0[j] + row;
-                const int64_t ray = (int64_t) s_pose[j] * dims.pix_per_pose + (int64_t) v * dims.det[0] + u;
-                const float yv = sino[ray];
+                return (int64_t) s_pose[j] * dims.pix_per_pose + (int64_t) v * dims.det[0] + u;
+            };
+            // the loads of the next candidate are issued before the current one is processed,
+            // so that their latency overlaps its samples
+            float yv_next = 0.0f;
+            RayRec r_next{};
+            if (tid < total) {
+                const int64_t ray = cand_ray(tid);
+                yv_next = sino[ray];
+                r_next = table[ray];
+            }
+            for (int k = tid; k < total; k += kThreads) {
+                const float yv = yv_next;
+                const RayRec r = r_next;
+                if (k + kThreads < total) {
+                    const int64_t ray = cand_ray(k + kThreads);
+                    yv_next = sino[ray];
+                    r_next = table[ray];
+                }
                 if (yv == 0.0f)
                     continue;
-                const RayRec r = table[ray];
                 float ys;
                 int sh;
                 ray_scale(yv * r.len, eg, ys, sh);
