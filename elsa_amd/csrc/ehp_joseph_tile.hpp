@@ -469,6 +469,24 @@ namespace ehp
     }
 
     // floor(x) clamped to [-2, n] and shifted by the padding: the in-plane index in a view
+    // Two neighbouring floats of a view (one 8-byte load; 4-byte alignment suffices).
+    struct __attribute__((aligned(4))) Pair {
+        float a, b;
+    };
+
+    EHP_HD inline Pair ld_view2(const PadView& V, uint32_t off4)
+    {
+#if defined(EHP_VIEW_CHECK)
+        if ((off4 & 3u) || (int64_t) (off4 >> 2) + 1 >= V.size) {
+            g_view_oob = true;
+            return Pair{0.0f, 0.0f};
+        }
+#endif
+        Pair p;
+        __builtin_memcpy(&p, (const char*) V.base + off4, sizeof(Pair));
+        return p;
+    }
+
     EHP_HD inline uint32_t pad_coord(float g, int n)
     {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -503,9 +521,6 @@ namespace ehp
         float acc = 0.0f;
         if (s0 >= s1)
             return acc;
-        // byte steps of the F + 1 taps along b1 and b2
-        const uint32_t d1 = (D == 3 && FB2) ? V.RS4 : 4u;
-        const uint32_t d2 = (D == 3 && FB2) ? 4u : V.RS4;
         uint32_t poff = (uint32_t) (neg ? r.P0 - s0 : r.P0 + s0) * V.PS4;
         const uint32_t dp = neg ? 0u - V.PS4 : V.PS4;
         for (int s = s0; s < s1; ++s, poff += dp) {
@@ -516,8 +531,8 @@ namespace ehp
             const float w1 = 1.0f - c1;
             const uint32_t i1 = pad_coord(g1, n1);
             if (D == 2) {
-                const uint32_t o = poff + (i1 << 2);
-                const float vf = ld_view(V, o), vk = ld_view(V, o + 4u);
+                const Pair fk = ld_view2(V, poff + (i1 << 2));
+                const float vf = fk.a, vk = fk.b;
                 // A = 0: (pa, C1) then (pa, F1); A = 1: (F0, pa) then (C0, pa)
                 if (A == 0) {
                     acc += vk * len * c1;
@@ -532,11 +547,15 @@ namespace ehp
                 const float c2 = (q2 - g2) - 0.5f;
                 const float w2 = 1.0f - c2;
                 const uint32_t i2 = pad_coord(g2, n2);
+                // the F and F + 1 taps along the fast axis are neighbours: two 8-byte loads, one per
+                // row of the slow axis
                 const uint32_t o = FB2 ? poff + mul24(i1, V.RS4) + (i2 << 2) : poff + mul24(i2, V.RS4) + (i1 << 2);
-                const float vff = ld_view(V, o);
-                const float vkf = ld_view(V, o + d1);
-                const float vfk = ld_view(V, o + d2);
-                const float vkk = ld_view(V, o + d1 + d2);
+                const Pair r0 = ld_view2(V, o), r1 = ld_view2(V, o + V.RS4);
+                // FB2: rows are b1 = F1, F1 + 1, pairs run along b2; else rows are b2, pairs along b1
+                const float vff = r0.a;
+                const float vkf = FB2 ? r1.a : r0.b;
+                const float vfk = FB2 ? r0.b : r1.a;
+                const float vkk = r1.b;
                 const float wff = w1 * w2, wkk = c1 * c2, wfk = w1 * c2, wkf = c1 * w2;
                 acc += vff * len * wff;
                 if (A == 0) {
