@@ -59,16 +59,24 @@ def parse():
 
 
 def copy_calibration(mib):
-    """A plain copy kernel of known size: counters read under it convert to bytes."""
+    """A plain copy kernel of known size: counters read under it convert to bytes. Also times
+    it, which gives the device-to-device copy rate (read + write bytes per second)."""
     import torch
 
     n = mib * (1 << 20) // 4
     a = torch.ones(n, dtype=torch.float32, device="cuda:0")
     b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
     for _ in range(3):
         b.copy_(a)
+    e1.record()
     torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 3}), flush=True)
+    t = e0.elapsed_time(e1) * 1e-3 / 3
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 4,
+                      "copy_GBps": round(2 * n * 4 / t / 1e9, 1)}), flush=True)
 
 
 def cpu_baseline(wl, x_np, n_poses):

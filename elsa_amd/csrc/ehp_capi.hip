@@ -228,7 +228,7 @@ namespace
     // device's LDS per workgroup.
     ehp::TileCfg default_tile(const ehp::Dims& dims, int max_lds)
     {
-        ehp::TileCfg tc = dims.dim == 3 ? ehp::TileCfg{8, 15, 15} : ehp::TileCfg{64, 125, 1};
+        ehp::TileCfg tc = dims.dim == 3 ? ehp::TileCfg{8, 15, 15} : ehp::TileCfg{8, 15, 1};
         while (tc.LA > 1 && (int) ehp::joseph_tile_lds_bytes(dims, tc) > max_lds)
             tc.LA /= 2;
         return tc;
