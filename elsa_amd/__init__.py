@@ -10,11 +10,13 @@ from .projectors import (BlobVoxelProjectorCUDA, BlobVoxelProjectorCUDAd, BlobVo
                          BSplineVoxelProjectorCUDA, BSplineVoxelProjectorCUDAd, BSplineVoxelProjectorCUDAf,
                          JosephsMethodCUDA, JosephsMethodCUDAd, JosephsMethodCUDAf, LinearOperator,
                          SiddonsMethodCUDA, SiddonsMethodCUDAd, SiddonsMethodCUDAf, blob_lut, bspline_lut)
+from .solvers import APGD, CGLS, ProximalIdentity, ProximalL1, powerIterations
 
 __all__ = [
     "CircleTrajectoryGenerator", "DataDescriptor", "DetectorDescriptor", "Geometry", "PlanarDetectorDescriptor",
     "VolumeDescriptor", "LinearOperator", "JosephsMethodCUDA", "JosephsMethodCUDAf", "JosephsMethodCUDAd",
     "SiddonsMethodCUDA", "SiddonsMethodCUDAf", "SiddonsMethodCUDAd", "BlobVoxelProjectorCUDA",
     "BlobVoxelProjectorCUDAf", "BlobVoxelProjectorCUDAd", "BSplineVoxelProjectorCUDA", "BSplineVoxelProjectorCUDAf",
-    "BSplineVoxelProjectorCUDAd", "blob_lut", "bspline_lut",
+    "BSplineVoxelProjectorCUDAd", "blob_lut", "bspline_lut", "CGLS", "APGD", "ProximalL1", "ProximalIdentity",
+    "powerIterations",
 ]

@@ -20,8 +20,9 @@ EXPORTS = (
     "ehp_last_error", "ehp_version", "ehp_geometry_create", "ehp_geometry_destroy",
     "ehp_joseph_create", "ehp_siddon_create", "ehp_voxel_create", "ehp_destroy", "ehp_forward",
     "ehp_adjoint", "ehp_op_sizes", "ehp_sample_count", "ehp_joseph_tile", "ehp_joseph_ray_info", "ehp_geometry_pose",
-    "ehp_circle_trajectory", "ehp_blob_lut", "ehp_bspline_lut",
+    "ehp_circle_trajectory", "ehp_blob_lut", "ehp_bspline_lut", "ehp_cgls", "ehp_apgd", "ehp_power_iterations",
 )
+EHP_SOLVE_GRAPH = 1
 
 _lib = None
 
@@ -30,14 +31,41 @@ class EhpError(RuntimeError):
     pass
 
 
+def _hip_runtimes():
+    """Paths of the HIP runtime libraries mapped into this process."""
+    paths = set()
+    try:
+        with open("/proc/self/maps") as f:
+            for line in f:
+                if "libamdhip64" in line:
+                    paths.add(os.path.realpath(line.split()[-1]))
+    except OSError:
+        pass
+    return paths
+
+
 def load():
-    """Load libelsahip.so and declare the C signatures (no GPU call happens here)."""
+    """Load libelsahip.so and declare the C signatures (no GPU call happens here).
+
+    torch (ROCm builds) ships its own HIP runtime. libelsahip.so must bind to that same
+    runtime, or torch's streams and ours would belong to different runtimes and each side
+    could even fail to see the GPU: torch is therefore imported first, so that the library's
+    libamdhip64 dependency resolves to the copy already loaded. Two different HIP runtimes
+    in the process afterwards are an error."""
     global _lib
     if _lib is not None:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: build it with `python -m elsa_amd.build`")
+    try:
+        import torch  # noqa: F401  (loads torch's HIP runtime first)
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
+    rts = _hip_runtimes()
+    if len(rts) > 1:
+        raise ImportError("two different HIP runtimes are loaded (" + ", ".join(sorted(rts)) +
+                          "); libelsahip.so must share torch's: import torch before loading it")
     vp, i64, c_int, c_f, c_d = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_float, ctypes.c_double
     fp = ctypes.POINTER(ctypes.c_float)
     i64p = ctypes.POINTER(ctypes.c_int64)
@@ -64,6 +92,10 @@ def load():
     L.ehp_circle_trajectory.argtypes = [c_int, i64, i64p, c_f, c_f, c_f, i64p, i64p, fp, fp, fp, fp, fp]
     L.ehp_blob_lut.argtypes = [c_int, c_d, c_d, i64, c_int, vp, ctypes.POINTER(c_d)]
     L.ehp_bspline_lut.argtypes = [c_int, i64, i64, c_int, vp, ctypes.POINTER(c_d)]
+    dp = ctypes.POINTER(c_d)
+    L.ehp_cgls.argtypes = [vp, vp, vp, c_int, c_int, c_d, c_d, c_int, dp, ip, vp]
+    L.ehp_apgd.argtypes = [vp, vp, vp, c_int, c_int, c_d, c_d, c_d, c_int, dp, ip, dp, vp]
+    L.ehp_power_iterations.argtypes = [vp, c_int, dp, vp]
     for name in EXPORTS:
         if name not in ("ehp_last_error", "ehp_version", "ehp_geometry_destroy", "ehp_destroy"):
             getattr(L, name).restype = c_int

@@ -228,7 +228,7 @@ namespace
     // device's LDS per workgroup.
     ehp::TileCfg default_tile(const ehp::Dims& dims, int max_lds)
     {
-        ehp::TileCfg tc = dims.dim == 3 ? ehp::TileCfg{8, 15, 15} : ehp::TileCfg{8, 15, 1};
+        ehp::TileCfg tc = dims.dim == 3 ? ehp::TileCfg{8, 15, 15} : ehp::TileCfg{4, 15, 1};
         while (tc.LA > 1 && (int) ehp::joseph_tile_lds_bytes(dims, tc) > max_lds)
             tc.LA /= 2;
         return tc;
@@ -542,3 +542,72 @@ int ehp_joseph_ray_info(const ehp_op* op, int32_t* axis, int64_t* nsteps)
 }
 
 } // extern "C"
+
+// ---- solvers -----------------------------------------------------------------------------
+
+int ehp_solver_fail(int code, const char* msg)
+{
+    return fail(code, "%s", msg);
+}
+
+namespace
+{
+    int solver_args(const ehp_op* op, const void* b, const void* x, int iterations)
+    {
+        if (!op)
+            return fail(EHP_ERR_ARGUMENT, "operator is NULL");
+        if (!b || !x)
+            return fail(EHP_ERR_ARGUMENT, "NULL data pointer");
+        if (b == x)
+            return fail(EHP_ERR_ARGUMENT, "b and x must not alias");
+        if (iterations < 0)
+            return fail(EHP_ERR_ARGUMENT, "negative iteration count %d", iterations);
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess)
+            return hip_fail(e, "hipGetDevice");
+        if (cur != op->device)
+            return fail(EHP_ERR_ARGUMENT, "operator lives on device %d, current device is %d", op->device, cur);
+        return EHP_OK;
+    }
+} // namespace
+
+int ehp_cgls(ehp_op* op, const void* d_b, void* d_x, int use_x0, int iterations, double eps, double tol, int flags,
+             double* residuals, int* iterations_done, void* stream)
+{
+    int rc = solver_args(op, d_b, d_x, iterations);
+    if (rc != EHP_OK)
+        return rc;
+    if (flags & ~EHP_SOLVE_GRAPH)
+        return fail(EHP_ERR_ARGUMENT, "unknown solver flags 0x%x", flags);
+    return ehp::solver_cgls(op, op->dtype, op->dims.nvox, op->dims.nrays, d_b, d_x, use_x0, iterations, eps, tol,
+                            (flags & EHP_SOLVE_GRAPH) != 0, residuals, iterations_done, (hipStream_t) stream);
+}
+
+int ehp_apgd(ehp_op* op, const void* d_b, void* d_x, int use_x0, int iterations, double mu, double l1_sigma,
+             double epsilon, int flags, double* grad_sq_norms, int* iterations_done, double* mu_used, void* stream)
+{
+    int rc = solver_args(op, d_b, d_x, iterations);
+    if (rc != EHP_OK)
+        return rc;
+    if (flags & ~EHP_SOLVE_GRAPH)
+        return fail(EHP_ERR_ARGUMENT, "unknown solver flags 0x%x", flags);
+    return ehp::solver_apgd(op, op->dtype, op->dims.nvox, op->dims.nrays, d_b, d_x, use_x0, iterations, mu, l1_sigma,
+                            epsilon, (flags & EHP_SOLVE_GRAPH) != 0, grad_sq_norms, iterations_done, mu_used,
+                            (hipStream_t) stream);
+}
+
+int ehp_power_iterations(ehp_op* op, int niters, double* L, void* stream)
+{
+    if (!op || !L)
+        return fail(EHP_ERR_ARGUMENT, "NULL argument");
+    if (niters < 1)
+        return fail(EHP_ERR_ARGUMENT, "power iterations need niters >= 1 (got %d)", niters);
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess)
+        return hip_fail(e, "hipGetDevice");
+    if (cur != op->device)
+        return fail(EHP_ERR_ARGUMENT, "operator lives on device %d, current device is %d", op->device, cur);
+    return ehp::solver_power_iterations(op, op->dtype, op->dims.nvox, op->dims.nrays, niters, L, (hipStream_t) stream);
+}

@@ -4,6 +4,8 @@
 #include "ehp_device.hpp"
 #include "ehp_joseph_tile.hpp"
 
+struct ehp_op;
+
 namespace ehp
 {
     template <class T>
@@ -41,4 +43,13 @@ namespace ehp
                              const T* sino, T* vol, hipStream_t s);
     hipError_t voxel_count(const PoseGeom* geo, const Dims& dims, float support, unsigned long long* total,
                            hipStream_t s);
+
+    // device-resident solvers (ehp_solvers.hip)
+    int solver_cgls(ehp_op* op, int dtype, int64_t nd, int64_t nr, const void* b, void* x, int use_x0, int iterations,
+                    double eps, double tol, bool graph, double* residuals, int* iters_done, hipStream_t s);
+    int solver_apgd(ehp_op* op, int dtype, int64_t nd, int64_t nr, const void* b, void* x, int use_x0, int iterations,
+                    double mu, double sigma, double epsilon, bool graph, double* grad_norms, int* iters_done,
+                    double* mu_used, hipStream_t s);
+    int solver_power_iterations(ehp_op* op, int dtype, int64_t nd, int64_t nr, int niters, double* L,
+                                hipStream_t s);
 } // namespace ehp

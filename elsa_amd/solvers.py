@@ -1,0 +1,123 @@
+"""elsa's CGLS and APGD (FISTA) solvers, run on the GPU by libelsahip.so.
+
+  CGLS(A, b, eps=0, tol=1e-4).solve(iterations, x0=None)
+      elsa/solvers/CGLS.h:45-46, CGLS.cpp:19-91
+  APGD(A, b, prox, mu=None, epsilon=eps(data_t)).solve(iterations, x0=None)
+      elsa/solvers/APGD.h:45-47, APGD.cpp:14-102 (FISTA), with prox ProximalL1(sigma)
+      (ProximalL1.cpp:32-43) or ProximalIdentity()
+  powerIterations(A, niters=5)
+      elsa/solvers/PowerIterations.cpp:10-27 applied to A^T A
+
+A is one of the projector classes of elsa_amd.projectors. The whole iteration runs on the
+device: step lengths, momentum and norms never leave it (FP64 reductions), and with
+graph=True one iteration is captured into a hipGraph and replayed. b is a GPU tensor or a
+numpy array; solve() returns the same kind. After solve(), `iterations_done` and
+`residuals` (CGLS: ||b - Ax|| at the start and per iteration; APGD: ||grad||^2 per
+iteration) hold what the run recorded.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+class ProximalL1:
+    """Soft thresholding sign(v) * max(|v| - t * sigma, 0) (elsa: ProximalL1(sigma))."""
+
+    def __init__(self, sigma=1.0):
+        self.sigma = float(sigma)
+
+
+class ProximalIdentity:
+    """prox(v) = v (elsa: ProximalIdentity)."""
+
+    sigma = -1.0
+
+
+def _prepare(A, b, x0):
+    torch = _torch()
+    tdt = torch.float32 if A._dtype == np.float32 else torch.float64
+    to_numpy = isinstance(b, np.ndarray)
+    if to_numpy:
+        bt = torch.from_numpy(np.ascontiguousarray(b, dtype=A._dtype)).cuda()
+    else:
+        if not (b.is_cuda and b.dtype == tdt):
+            raise ValueError("b must be a GPU tensor of the operator's dtype")
+        bt = b.contiguous()
+    if bt.numel() != A._n_sino:
+        raise ValueError(f"b has {bt.numel()} elements, expected {A._n_sino}")
+    shape = A.getDomainDescriptor().shape
+    if x0 is None:
+        xt = torch.empty(shape, dtype=tdt, device=bt.device)
+    elif isinstance(x0, np.ndarray):
+        xt = torch.from_numpy(np.array(x0, dtype=A._dtype).reshape(shape)).cuda()
+    else:
+        xt = x0.detach().to(dtype=tdt).reshape(shape).contiguous().clone()
+    if xt.numel() != A._n_vol:
+        raise ValueError(f"x0 has {xt.numel()} elements, expected {A._n_vol}")
+    stream = ctypes.c_void_p(torch.cuda.current_stream(bt.device).cuda_stream)
+    return bt, xt, to_numpy, stream
+
+
+class CGLS:
+    """Conjugate gradients on the normal equations of min ||Ax - b||^2 + eps^2 ||x||^2."""
+
+    def __init__(self, A, b, eps=0.0, tol=1e-4, graph=False):
+        self.A, self.b, self.eps, self.tol, self.graph = A, b, float(eps), float(tol), bool(graph)
+        self.iterations_done = 0
+        self.residuals = None
+
+    def solve(self, iterations, x0=None):
+        bt, xt, to_numpy, stream = _prepare(self.A, self.b, x0)
+        res = np.empty(iterations + 1, np.float64)
+        n = ctypes.c_int()
+        flags = _lib.EHP_SOLVE_GRAPH if self.graph else 0
+        _lib.check(self.A._L.ehp_cgls(self.A.handle, ctypes.c_void_p(bt.data_ptr()), ctypes.c_void_p(xt.data_ptr()),
+                                      int(x0 is not None), int(iterations), self.eps, self.tol, flags,
+                                      res.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(n), stream),
+                   "CGLS")
+        self.iterations_done, self.residuals = n.value, res
+        return xt.cpu().numpy() if to_numpy else xt
+
+
+class APGD:
+    """Accelerated proximal gradient (FISTA) on 0.5 ||Ax - b||^2 + g(x)."""
+
+    def __init__(self, A, b, prox=None, mu=None, epsilon=None, graph=False):
+        self.A, self.b, self.graph = A, b, bool(graph)
+        self.prox = prox if prox is not None else ProximalIdentity()
+        self.mu = float(mu) if mu is not None else 0.0
+        self.epsilon = float(np.finfo(A._dtype).eps) if epsilon is None else float(epsilon)
+        self.iterations_done = 0
+        self.residuals = None
+        self.mu_used = None
+
+    def solve(self, iterations, x0=None):
+        bt, xt, to_numpy, stream = _prepare(self.A, self.b, x0)
+        res = np.empty(iterations + 1, np.float64)
+        n, mu_used = ctypes.c_int(), ctypes.c_double()
+        flags = _lib.EHP_SOLVE_GRAPH if self.graph else 0
+        _lib.check(self.A._L.ehp_apgd(self.A.handle, ctypes.c_void_p(bt.data_ptr()), ctypes.c_void_p(xt.data_ptr()),
+                                      int(x0 is not None), int(iterations), self.mu, float(self.prox.sigma),
+                                      self.epsilon, flags, res.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                      ctypes.byref(n), ctypes.byref(mu_used), stream), "APGD")
+        self.iterations_done, self.residuals, self.mu_used = n.value, res, mu_used.value
+        return xt.cpu().numpy() if to_numpy else xt
+
+
+def powerIterations(A, niters=5):
+    """Largest eigenvalue estimate of A^T A (elsa: powerIterations(adjoint(A) * A, niters))."""
+    torch = _torch()
+    L = ctypes.c_double()
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    _lib.check(A._L.ehp_power_iterations(A.handle, int(niters), ctypes.byref(L), stream), "powerIterations")
+    return L.value

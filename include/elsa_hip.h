@@ -121,6 +121,38 @@ int ehp_joseph_tile(ehp_op* op, int la, int tb, int tc, int* la_out, int* tb_out
  * axis and number of samples of every ray, host memory of n_sino entries each. */
 int ehp_joseph_ray_info(const ehp_op* op, int32_t* axis, int64_t* nsteps);
 
+/* ---- solvers on the device ---------------------------------------------------------------
+ * The iterations of elsa's solvers with every scalar (step length, momentum, norms,
+ * convergence) kept in device memory: norms and dot products are FP64 reductions in a fixed
+ * order, vector updates run in the operator's data type. elsa's early return on convergence
+ * becomes a device flag after which x no longer changes. With EHP_SOLVE_GRAPH one iteration is
+ * captured into a hipGraph and replayed. Each call allocates its workspace, is ordered after
+ * `stream` and returns when the solve has finished. x is overwritten with the result
+ * (it is the start value when use_x0 != 0). */
+enum ehp_solve_flags {
+    EHP_SOLVE_GRAPH = 1
+};
+
+/* CGLS on min ||Ax - b||^2 + eps^2 ||x||^2 (elsa: CGLS(A, b, eps, tol).solve(iterations, x0),
+ * CGLS.cpp:19-91): stops before an iteration when ||A^T s - eps^2 x||^2 < tol. residuals
+ * (host, iterations + 1 entries or NULL) receives ||s|| = ||b - Ax|| at the start and after
+ * each iteration taken (-1 for iterations not taken); iterations_done the number taken. */
+int ehp_cgls(ehp_op* op, const void* d_b, void* d_x, int use_x0, int iterations, double eps, double tol, int flags,
+             double* residuals, int* iterations_done, void* stream);
+
+/* APGD / FISTA on 0.5 ||Ax - b||^2 + g(x) with g = l1_sigma ||x||_1 (l1_sigma >= 0, elsa's
+ * ProximalL1) or g = 0 (l1_sigma < 0, ProximalIdentity) (elsa: APGD(A, b, prox, mu, epsilon)
+ * .solve(iterations, x0), APGD.cpp:14-102). mu <= 0 takes elsa's 0.45 / L with L from 5
+ * power iterations on A^T A; mu_used receives the step. Stops after the iteration in which
+ * ||grad||^2 <= epsilon; grad_sq_norms (host, iterations + 1 entries or NULL) receives
+ * ||grad||^2 per iteration taken (-1 otherwise). */
+int ehp_apgd(ehp_op* op, const void* d_b, void* d_x, int use_x0, int iterations, double mu, double l1_sigma,
+             double epsilon, int flags, double* grad_sq_norms, int* iterations_done, double* mu_used, void* stream);
+
+/* Largest eigenvalue estimate of A^T A by power iterations from the all-ones vector
+ * (elsa: powerIterations(adjoint(A) * A, niters), PowerIterations.cpp:10-27). */
+int ehp_power_iterations(ehp_op* op, int niters, double* L, void* stream);
+
 /* ---- host-only helpers (no GPU) -------------------------------------------------------- */
 
 /* One pose of elsa's Geometry (elsa/core/Geometry.cpp:12-51 for dim 2, :53-109 for dim 3):

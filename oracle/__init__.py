@@ -322,3 +322,98 @@ def lut_eval(lut, support, distance):
     if lut.dtype == np.float64:
         return lib().orc_lut_eval_f64(lut, lut.shape[0], support, distance)
     return lib().orc_lut_eval_f32(lut, lut.shape[0], support, distance)
+
+
+# --------------------------------------------------------------------------------------
+# Solvers (elsa/solvers/CGLS.cpp, APGD.cpp, PowerIterations.cpp) over any pair of callables
+# A(x) -> sinogram, At(y) -> volume, in float64 unless the inputs say otherwise. Reference
+# implementations for the device solvers; written for small problems.
+
+
+def _dot(a, b):
+    return float(np.dot(a.ravel().astype(np.float64), b.ravel().astype(np.float64)))
+
+
+def cgls(A, At, b, shape, iterations, eps=0.0, tol=1e-4, x0=None, dtype=np.float64):
+    """CGLS.cpp:19-91. Returns (x, residual norms ||s|| at the start and after each iteration
+    taken, iterations taken)."""
+    damp = eps * eps
+    b = np.asarray(b, dtype)
+    if x0 is not None:
+        x = np.array(x0, dtype).reshape(shape)
+        s = b - A(x)                                   # :31-35
+        r = At(s) - damp * x                           # :37-38
+    else:
+        x = np.zeros(shape, dtype)
+        s = b.copy()
+        r = At(s)                                      # :42
+    c = r.copy()
+    kold = _dot(r, r)                                  # :48
+    res = [np.sqrt(_dot(s, s))]
+    taken = 0
+    for _ in range(iterations):
+        if kold < tol:                                 # :54-58
+            break
+        q = A(c)                                       # :60
+        delta = _dot(q, q)
+        alpha = kold / delta if damp == 0 else kold / (delta + damp * _dot(c, c))
+        x = x + dtype(alpha) * c                       # :71
+        s = s - dtype(alpha) * q                       # :72
+        r = At(s)
+        if damp != 0.0:
+            r = r - dtype(damp) * x
+        k = _dot(r, r)
+        beta = k / kold
+        c = r + dtype(beta) * c                        # :82
+        kold = k
+        taken += 1
+        res.append(np.sqrt(_dot(s, s)))
+    return x, np.array(res), taken
+
+
+def power_iterations(A, At, shape, niters=5, dtype=np.float64):
+    """PowerIterations.cpp:10-27 on A^T A: u = 1; niters times u = A^T A u / ||.||;
+    returns u . (A^T A u) / ||u||."""
+    u = np.ones(shape, dtype)
+    for _ in range(niters):
+        u = At(A(u))
+        u = u / np.sqrt(_dot(u, u))
+    return _dot(u, At(A(u))) / np.sqrt(_dot(u, u))
+
+
+def apgd(A, At, b, shape, iterations, mu=None, sigma=None, epsilon=None, x0=None, dtype=np.float64):
+    """APGD.cpp:33-102 (FISTA) with prox = soft thresholding of threshold mu * sigma
+    (ProximalL1.cpp:32-43), or the identity when sigma is None. mu None: 0.45 / L with L from
+    5 power iterations (APGD.cpp:19-27). Returns (x, ||grad||^2 per iteration taken,
+    iterations taken, mu)."""
+    if mu is None:
+        mu = float(dtype(0.45) / dtype(power_iterations(A, At, shape, 5, dtype)))
+    if epsilon is None:
+        epsilon = float(np.finfo(dtype).eps)
+    x = np.zeros(shape, dtype) if x0 is None else np.array(x0, dtype).reshape(shape)
+    xprev, y = x.copy(), x.copy()
+    tprev = dtype(1)
+    atb = At(np.asarray(b, dtype))
+    grad = At(A(y)) - atb                              # :55-60
+    gn = []
+    taken = 0
+    m = dtype(mu)
+    for _ in range(iterations):
+        z = y - m * grad                               # :69
+        if sigma is None:
+            x = z
+        else:
+            tmp = np.abs(z) - m * dtype(sigma)
+            tmp = dtype(0.5) * (tmp + np.abs(tmp))
+            x = np.sign(z) * tmp                       # :72 prox
+        t = (dtype(1) + np.sqrt(dtype(1) + dtype(4) * tprev * tprev)) / dtype(2)
+        y = x + ((tprev - dtype(1)) / t) * (x - xprev)
+        xprev = x
+        tprev = t
+        g2 = _dot(grad, grad)
+        gn.append(g2)
+        taken += 1
+        if g2 <= epsilon:                              # :83-87
+            break
+        grad = At(A(y)) - atb
+    return x, np.array(gn), taken, mu

@@ -157,3 +157,12 @@ def test_projector_without_gpu_fails_loudly():
     dd = E.CircleTrajectoryGenerator.createTrajectory(4, vd, 360, 100.0, 10.0)
     with pytest.raises(Exception):
         E.JosephsMethodCUDA(vd, dd)
+
+
+def test_library_shares_torchs_hip_runtime(lib):
+    """libelsahip.so binds to the HIP runtime torch already loaded (one libamdhip64 in the
+    process), so torch's streams and device memory are the library's too."""
+    from elsa_amd import _lib
+
+    rts = _lib._hip_runtimes()
+    assert len(rts) == 1, rts

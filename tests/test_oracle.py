@@ -327,3 +327,55 @@ def test_joseph_fp64_brute_force_tiny():
                     w = (t[bs[0]] if d1 else 1 - t[bs[0]]) * (t[bs[1]] if d2 else 1 - t[bs[1]])
                     acc += w * val(c)
         assert y[r] == pytest.approx(acc * ln[r], rel=1e-9, abs=1e-9), r
+
+
+# --------------------------------------------------------------------------------------
+# solver restatements, pinned by closed forms on small dense problems
+
+
+def _dense(seed, m=40, n=15):
+    rng = np.random.default_rng(seed)
+    M = rng.standard_normal((m, n))
+    b = rng.standard_normal(m)
+    return M, b, (lambda x: M @ x), (lambda y: M.T @ y)
+
+
+def test_cgls_reaches_least_squares_and_ridge_solutions():
+    M, b, A, At = _dense(1)
+    x, res, taken = oracle.cgls(A, At, b, (15,), 40, tol=1e-24)
+    ref = np.linalg.lstsq(M, b, rcond=None)[0]
+    assert np.linalg.norm(x - ref) <= 1e-9 * np.linalg.norm(ref)
+    assert np.all(np.diff(res) <= 1e-12)  # ||b - Ax|| never grows
+    eps = 0.7
+    xd, _, _ = oracle.cgls(A, At, b, (15,), 60, eps=eps, tol=1e-24)
+    ridge = np.linalg.solve(M.T @ M + eps * eps * np.eye(15), M.T @ b)
+    assert np.linalg.norm(xd - ridge) <= 1e-9 * np.linalg.norm(ridge)
+    # a start value: the same minimiser
+    x0 = np.random.default_rng(2).standard_normal(15)
+    x2, _, _ = oracle.cgls(A, At, b, (15,), 40, tol=1e-24, x0=x0)
+    assert np.linalg.norm(x2 - ref) <= 1e-9 * np.linalg.norm(ref)
+
+
+def test_cgls_stops_on_tolerance():
+    M, b, A, At = _dense(3)
+    kold = float(np.sum((M.T @ b) ** 2))
+    x, res, taken = oracle.cgls(A, At, b, (15,), 10, tol=2 * kold)
+    assert taken == 0 and not np.any(x) and len(res) == 1
+
+
+def test_power_iterations_and_apgd_closed_forms():
+    d = np.array([5.0, 3.0, 1.0, 0.5])
+    A = lambda x: np.sqrt(d) * x  # A^T A = diag(d)
+    L = oracle.power_iterations(A, A, (4,), niters=60)
+    assert abs(L - 5.0) <= 1e-9
+    # A = I: the L1 problem's minimiser is soft thresholding of b
+    b = np.array([3.0, -0.2, 0.5, -4.0, 0.0, 1.5])
+    ident = lambda x: x
+    x, gn, taken, mu = oracle.apgd(ident, ident, b, (6,), 200, mu=0.45, sigma=1.0, epsilon=0.0)
+    assert np.allclose(x, np.sign(b) * np.maximum(np.abs(b) - 1.0, 0.0), atol=1e-12)
+    # identity prox: least squares
+    M, bb, A2, At2 = _dense(4)
+    x2, gn2, taken2, mu2 = oracle.apgd(A2, At2, bb, (15,), 3000, epsilon=1e-26)
+    ref = np.linalg.lstsq(M, bb, rcond=None)[0]
+    assert np.linalg.norm(x2 - ref) <= 1e-6 * np.linalg.norm(ref)
+    assert abs(mu2 - 0.45 / oracle.power_iterations(A2, At2, (15,), 5)) <= 1e-15
