@@ -12,6 +12,12 @@ Prints ONE JSON line:
               MI355X's 8 TB/s HBM
   cpu_baseline  the CPU restatement (oracle/, OpenMP over rays) on a bounded sample of the same
               workload (a subset of the poses), on this host's cores
+  cg          20 CGLS iterations (elsa's CGLS.cpp semantics) from b = A x_true, run on the device
+              eagerly and with one iteration captured in a hipGraph: wall times and the error
+              reached
+
+--projector siddon | blob | bspline times those projectors on the same workload instead
+(voxel crossings or footprint taps per second), without the CG and CPU parts.
 """
 from __future__ import annotations
 
@@ -52,6 +58,10 @@ def parse():
                    help="Joseph kernels: plane-major forward + LDS-tiled back projection (default) or one "
                         "thread per ray")
     p.add_argument("--tile", default="", help="LA,TB,TC of the tiled back projection (default: built-in)")
+    p.add_argument("--projector", default="joseph", choices=["joseph", "siddon", "blob", "bspline"],
+                   help="projector to time (default: Joseph, the headline metric)")
+    p.add_argument("--cg-iters", type=int, default=20,
+                   help="CGLS iterations of the reconstruction part (0 skips it)")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="only run a plain device copy of MIB MiB (3 launches), the reference kernel for "
                         "scaling FETCH_SIZE / WRITE_SIZE counters, and exit")
@@ -98,6 +108,29 @@ def cpu_baseline(wl, x_np, n_poses):
                       f"{dt:.1f} s), oracle/elsa_oracle.cpp with OpenMP over rays"}
 
 
+def cg_run(E, op, x, iters):
+    """CGLS (elsa's CGLS.cpp semantics, tol 0) from b = A x_true, whole iterations on the
+    device: wall time of solve() (workspace, graph capture and all iterations) eager and with
+    one iteration captured in a hipGraph, and the error reached."""
+    import torch
+
+    b = op.apply(x)
+    res = {"iterations": iters}
+    for graph in (False, True):
+        solver = E.CGLS(op, b, tol=0.0, graph=graph)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        xr = solver.solve(iters)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        key = "graph" if graph else "eager"
+        res[f"wall_s_{key}"] = round(dt, 4)
+        res[f"ms_per_iter_{key}"] = round(dt / iters * 1e3, 3)
+    res["rel_residual"] = float(solver.residuals[solver.iterations_done] / solver.residuals[0])
+    res["rel_error_vs_phantom"] = float(torch.linalg.norm(xr - x) / torch.linalg.norm(x))
+    return res
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -118,9 +151,16 @@ def main():
     vd = E.VolumeDescriptor(list(wl["vol"]))
     dd = E.CircleTrajectoryGenerator.createTrajectory(wl["views"], vd, 360, wl["s2c"], wl["c2d"],
                                                       detectorSize=list(wl["det"]))
-    op = E.JosephsMethodCUDA(vd, dd, kernels=a.kernels)
+    if a.projector == "joseph":
+        op = E.JosephsMethodCUDA(vd, dd, kernels=a.kernels)
+    elif a.projector == "siddon":
+        op = E.SiddonsMethodCUDA(vd, dd)
+    elif a.projector == "blob":
+        op = E.BlobVoxelProjectorCUDA(vd, dd)
+    else:
+        op = E.BSplineVoxelProjectorCUDA(vd, dd)
     tile = None
-    if a.kernels == "fast":
+    if a.projector == "joseph" and a.kernels == "fast":
         tile = op.tile(*[int(v) for v in a.tile.split(",")]) if a.tile else op.tile()
     S = op.sample_count()
     R = int(np.prod(wl["det"])) * wl["views"]
@@ -149,10 +189,23 @@ def main():
 
     t_fwd = float(np.mean([e0.elapsed_time(e1) for e0, e1, _ in evs])) * 1e-3
     t_adj = float(np.mean([e1.elapsed_time(e2) for _, e1, e2 in evs])) * 1e-3
-    tap_bytes = 16 if wl["dim"] == 3 else 8
-    alg_bytes = tap_bytes * S + 4 * R
-    adj_name = "joseph_adj_tiled_kernel" if a.kernels == "fast" else "joseph_adj_kernel"
-    fwd_name = "joseph_fwd_planes_kernel" if a.kernels == "fast" else "joseph_fwd_kernel"
+    nvox = int(np.prod(wl["vol"]))
+    if a.projector == "joseph":
+        # 4 (2D: 2) FP32 taps per sample, one FP32 value per ray
+        alg_bytes = (16 if wl["dim"] == 3 else 8) * S + 4 * R
+        adj_name = "joseph_adj_tiled_kernel" if a.kernels == "fast" else "joseph_adj_kernel"
+        fwd_name = "joseph_fwd_planes_kernel" if a.kernels == "fast" else "joseph_fwd_kernel"
+        unit, metric = "Gsamples/s", METRIC
+    elif a.projector == "siddon":
+        # one FP32 voxel per crossing, one FP32 value per ray
+        alg_bytes = 4 * S + 4 * R
+        adj_name = fwd_name = "siddon_kernel"
+        unit, metric = "Gcrossings/s", "Siddon fwd+adj voxel crossings/s & %HBM roofline"
+    else:
+        # one FP32 detector value per footprint tap, one FP32 voxel per voxel and pose
+        alg_bytes = 4 * S + 4 * nvox * wl["views"]
+        adj_name, fwd_name = "voxel_adj_kernel", "voxel_fwd_kernel"
+        unit, metric = "Gtaps/s", f"{a.projector} voxel projector fwd+adj footprint taps/s & %HBM roofline"
     name, t_dom = (adj_name, t_adj) if t_adj >= t_fwd else (fwd_name, t_fwd)
     achieved = alg_bytes / t_dom / 1e9
     traffic = None
@@ -161,9 +214,9 @@ def main():
         traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
 
     out = {
-        "metric": METRIC,
+        "metric": metric,
         "value": round(2 * S * a.steps / wall / 1e9, 3),
-        "unit": "Gsamples/s",
+        "unit": unit,
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
@@ -173,14 +226,17 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (modified Shepp-Logan phantom)",
-        "config": {"workload": wl["text"], "kernels": a.kernels, "tile": tile, "volume": list(wl["vol"]), "poses": wl["views"],
+        "config": {"workload": wl["text"], "projector": a.projector, "kernels": a.kernels, "tile": tile,
+                   "volume": list(wl["vol"]), "poses": wl["views"],
                    "detector": list(wl["det"]), "samples_per_pass": S, "rays": R,
                    "fwd_ms": round(t_fwd * 1e3, 3), "adj_ms": round(t_adj * 1e3, 3)},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": alg_bytes},
     }
-    if not a.no_cpu:
+    if a.cg_iters > 0 and a.projector == "joseph":
+        out["cg"] = cg_run(E, op, x, a.cg_iters)
+    if not a.no_cpu and a.projector == "joseph":
         out["cpu_baseline"] = cpu_baseline(wl, x_np, a.cpu_poses)
     print(json.dumps(out), flush=True)
 

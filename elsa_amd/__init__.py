@@ -11,6 +11,7 @@ from .projectors import (BlobVoxelProjectorCUDA, BlobVoxelProjectorCUDAd, BlobVo
                          JosephsMethodCUDA, JosephsMethodCUDAd, JosephsMethodCUDAf, LinearOperator,
                          SiddonsMethodCUDA, SiddonsMethodCUDAd, SiddonsMethodCUDAf, blob_lut, bspline_lut)
 from .solvers import APGD, CGLS, ProximalIdentity, ProximalL1, powerIterations
+from . import autograd
 
 __all__ = [
     "CircleTrajectoryGenerator", "DataDescriptor", "DetectorDescriptor", "Geometry", "PlanarDetectorDescriptor",

@@ -135,6 +135,8 @@ class _HipProjector(LinearOperator):
             ot = out
             if not (ot.is_cuda and ot.dtype == tdt and ot.is_contiguous() and ot.numel() == n_out):
                 raise ValueError("output tensor must be a contiguous GPU tensor of the range size and dtype")
+            if ot.data_ptr() == xt.data_ptr():
+                raise ValueError("input and output must not alias")
         stream = ctypes.c_void_p(torch.cuda.current_stream(xt.device).cuda_stream)
         f = self._L.ehp_adjoint if adjoint else self._L.ehp_forward
         _lib.check(f(self._op, ctypes.c_void_p(xt.data_ptr()), ctypes.c_void_p(ot.data_ptr()), stream),
