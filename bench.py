@@ -154,13 +154,13 @@ def main():
     if a.projector == "joseph":
         op = E.JosephsMethodCUDA(vd, dd, kernels=a.kernels)
     elif a.projector == "siddon":
-        op = E.SiddonsMethodCUDA(vd, dd)
+        op = E.SiddonsMethodCUDA(vd, dd, kernels=a.kernels)
     elif a.projector == "blob":
         op = E.BlobVoxelProjectorCUDA(vd, dd)
     else:
         op = E.BSplineVoxelProjectorCUDA(vd, dd)
     tile = None
-    if a.projector == "joseph" and a.kernels == "fast":
+    if a.projector in ("joseph", "siddon") and a.kernels == "fast":
         tile = op.tile(*[int(v) for v in a.tile.split(",")]) if a.tile else op.tile()
     S = op.sample_count()
     R = int(np.prod(wl["det"])) * wl["views"]
@@ -199,7 +199,8 @@ def main():
     elif a.projector == "siddon":
         # one FP32 voxel per crossing, one FP32 value per ray
         alg_bytes = 4 * S + 4 * R
-        adj_name = fwd_name = "siddon_kernel"
+        fwd_name = "siddon_kernel"
+        adj_name = "siddon_adj_tiled_kernel" if a.kernels == "fast" else "siddon_kernel"
         unit, metric = "Gcrossings/s", "Siddon fwd+adj voxel crossings/s & %HBM roofline"
     else:
         # one FP32 detector value per footprint tap, one FP32 voxel per voxel and pose

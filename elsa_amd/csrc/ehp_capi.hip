@@ -42,7 +42,8 @@ struct ehp_op {
     int* d_pose_axes = nullptr;
     int axes_present = 0;
     ehp::TileCfg tile{};
-    ehp::RayRec* d_table = nullptr;   // per-ray traversal state
+    ehp::RayRec* d_table = nullptr;   // per-ray traversal state (Joseph)
+    ehp::SidRec* d_sidtable = nullptr; // per-ray traversal state (Siddon)
     unsigned int* d_maxbits = nullptr; // max|y| of the current back projection
     int* d_fixexp = nullptr;           // fixed-point exponent of the current back projection
     double slot_bound = 0.0;
@@ -226,10 +227,19 @@ namespace
 {
     // Tile shape of the LDS-privatised Joseph back projection, shrunk until it fits the
     // device's LDS per workgroup.
-    ehp::TileCfg default_tile(const ehp::Dims& dims, int max_lds)
+    size_t tile_lds_bytes(int kind, const ehp::Dims& dims, const ehp::TileCfg& tc)
     {
-        ehp::TileCfg tc = dims.dim == 3 ? ehp::TileCfg{8, 15, 15} : ehp::TileCfg{4, 15, 1};
-        while (tc.LA > 1 && (int) ehp::joseph_tile_lds_bytes(dims, tc) > max_lds)
+        return kind == KIND_SIDDON ? ehp::siddon_tile_lds_bytes(dims, tc) : ehp::joseph_tile_lds_bytes(dims, tc);
+    }
+
+    ehp::TileCfg default_tile(int kind, const ehp::Dims& dims, int max_lds)
+    {
+        ehp::TileCfg tc;
+        if (kind == KIND_SIDDON)
+            tc = dims.dim == 3 ? ehp::TileCfg{8, 16, 16} : ehp::TileCfg{4, 16, 1};
+        else
+            tc = dims.dim == 3 ? ehp::TileCfg{8, 15, 15} : ehp::TileCfg{4, 15, 1};
+        while (tc.LA > 1 && (int) tile_lds_bytes(kind, dims, tc) > max_lds)
             tc.LA /= 2;
         return tc;
     }
@@ -323,7 +333,48 @@ namespace
         int max_lds = 65536;
         if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, op->device) != hipSuccess)
             max_lds = 65536;
-        op->tile = default_tile(dm, max_lds);
+        op->tile = default_tile(KIND_JOSEPH, dm, max_lds);
+        op->tiled = true;
+        return EHP_OK;
+    }
+
+    // FP32 Siddon: per-ray table, driving axes per pose and, when the fixed-point bound
+    // exists, the LDS-tiled back projection.
+    int setup_siddon_tiled(ehp_op* op)
+    {
+        const ehp::Dims& dm = op->dims;
+        const double bound = ehp::fixed_point_slot_bound(dm, op->host_geo.data());
+        if (!(bound > 0.0) || bound > 1.0e18)
+            return EHP_OK; // keep the one-thread-per-ray kernel
+        for (int i = 0; i < dm.dim; ++i)
+            if (dm.vol[i] > 32767)
+                return EHP_OK; // the table stores voxel indices as 16-bit values
+        const int n = dm.nposes;
+        const size_t tbytes = (size_t) dm.nrays * sizeof(ehp::SidRec);
+        hipError_t e = hipMalloc((void**) &op->d_sidtable, tbytes);
+        if (e != hipSuccess)
+            return fail(EHP_ERR_ALLOC, "hipMalloc of the %zu-byte ray table failed: %s", tbytes, hipGetErrorString(e));
+        e = hipMalloc((void**) &op->d_pose_axes, n * sizeof(int));
+        if (e == hipSuccess)
+            e = hipMalloc((void**) &op->d_maxbits, sizeof(unsigned int));
+        if (e == hipSuccess)
+            e = hipMalloc((void**) &op->d_fixexp, 2 * sizeof(int));
+        if (e != hipSuccess)
+            return fail(EHP_ERR_ALLOC, "hipMalloc of the Siddon state failed: %s", hipGetErrorString(e));
+        e = ehp::siddon_ray_table(op->d_geo, dm, op->d_sidtable, op->d_pose_axes, nullptr);
+        std::vector<int> h(n);
+        if (e == hipSuccess)
+            e = hipMemcpy(h.data(), op->d_pose_axes, n * sizeof(int), hipMemcpyDeviceToHost);
+        if (e != hipSuccess)
+            return hip_fail(e, "Siddon ray table");
+        op->axes_present = 0;
+        for (int v : h)
+            op->axes_present |= v;
+        op->slot_bound = bound;
+        int max_lds = 65536;
+        if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, op->device) != hipSuccess)
+            max_lds = 65536;
+        op->tile = default_tile(KIND_SIDDON, dm, max_lds);
         op->tiled = true;
         return EHP_OK;
     }
@@ -331,13 +382,13 @@ namespace
 
 int ehp_joseph_create(const ehp_geometry* g, int dtype, int flags, ehp_op** out)
 {
-    if (flags & ~EHP_JOSEPH_SIMPLE_KERNELS)
+    if (flags & ~EHP_SIMPLE_KERNELS)
         return fail(EHP_ERR_ARGUMENT, "unknown Joseph flags 0x%x", flags);
     int rc = new_op(g, KIND_JOSEPH, dtype, out);
     if (rc != EHP_OK)
         return rc;
     rc = count_samples(*out);
-    if (rc == EHP_OK && dtype == EHP_F32 && !(flags & EHP_JOSEPH_SIMPLE_KERNELS))
+    if (rc == EHP_OK && dtype == EHP_F32 && !(flags & EHP_SIMPLE_KERNELS))
         rc = setup_fast(*out);
     if (rc != EHP_OK) {
         ehp_destroy(*out);
@@ -346,12 +397,16 @@ int ehp_joseph_create(const ehp_geometry* g, int dtype, int flags, ehp_op** out)
     return rc;
 }
 
-int ehp_siddon_create(const ehp_geometry* g, int dtype, ehp_op** out)
+int ehp_siddon_create(const ehp_geometry* g, int dtype, int flags, ehp_op** out)
 {
+    if (flags & ~EHP_SIMPLE_KERNELS)
+        return fail(EHP_ERR_ARGUMENT, "unknown Siddon flags 0x%x", flags);
     int rc = new_op(g, KIND_SIDDON, dtype, out);
     if (rc != EHP_OK)
         return rc;
     rc = count_samples(*out);
+    if (rc == EHP_OK && dtype == EHP_F32 && !(flags & EHP_SIMPLE_KERNELS))
+        rc = setup_siddon_tiled(*out);
     if (rc != EHP_OK) {
         ehp_destroy(*out);
         *out = nullptr;
@@ -401,7 +456,8 @@ void ehp_destroy(ehp_op* op)
     hipGetDevice(&cur);
     if (cur != op->device)
         (void) hipSetDevice(op->device);
-    for (void* p : {(void*) op->d_geo, op->d_lut, (void*) op->d_pose_axes, (void*) op->d_table, (void*) op->d_maxbits,
+    for (void* p : {(void*) op->d_geo, op->d_lut, (void*) op->d_pose_axes, (void*) op->d_table,
+                    (void*) op->d_sidtable, (void*) op->d_maxbits,
                     (void*) op->d_fixexp, (void*) op->d_view[0], (void*) op->d_view[1], (void*) op->d_view[2]})
         if (p)
             (void) hipFree(p);
@@ -455,6 +511,9 @@ static int apply(ehp_op* op, const void* in, void* out, void* stream, bool adjoi
             e = ehp::joseph_forward_planes(op->d_geo, dm, op->layouts, op->d_view, op->axes_present, x, y, s);
         else if (op->kind == KIND_JOSEPH)
             e = adjoint ? ehp::joseph_adjoint<float>(op->d_geo, dm, x, y, s) : ehp::joseph_forward<float>(op->d_geo, dm, x, y, s);
+        else if (op->kind == KIND_SIDDON && adjoint && op->tiled)
+            e = ehp::siddon_adjoint_tiled(op->d_geo, op->d_sidtable, dm, op->d_pose_axes, op->axes_present, op->tile,
+                                          op->batch, op->slot_bound, op->d_maxbits, op->d_fixexp, x, y, s);
         else if (op->kind == KIND_SIDDON)
             e = adjoint ? ehp::siddon_adjoint<float>(op->d_geo, dm, x, y, s) : ehp::siddon_forward<float>(op->d_geo, dm, x, y, s);
         else
@@ -486,12 +545,12 @@ int ehp_adjoint(ehp_op* op, const void* d_sino, void* d_vol, void* stream)
     return apply(op, d_sino, d_vol, stream, true);
 }
 
-int ehp_joseph_tile(ehp_op* op, int la, int tb, int tc, int* la_out, int* tb_out, int* tc_out)
+int ehp_adjoint_tile(ehp_op* op, int la, int tb, int tc, int* la_out, int* tb_out, int* tc_out)
 {
     if (!op)
         return fail(EHP_ERR_ARGUMENT, "operator is NULL");
-    if (op->kind != KIND_JOSEPH || !op->tiled)
-        return fail(EHP_ERR_ARGUMENT, "operator does not use the tiled Joseph kernels");
+    if (!op->tiled)
+        return fail(EHP_ERR_ARGUMENT, "operator does not use a tiled back projection");
     if (la > 0) {
         if (tb <= 0 || (op->dims.dim == 3 && tc <= 0))
             return fail(EHP_ERR_ARGUMENT, "tile extents must be positive");
@@ -499,7 +558,7 @@ int ehp_joseph_tile(ehp_op* op, int la, int tb, int tc, int* la_out, int* tb_out
         int max_lds = 65536;
         if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, op->device) != hipSuccess)
             max_lds = 65536;
-        const size_t need = ehp::joseph_tile_lds_bytes(op->dims, t);
+        const size_t need = tile_lds_bytes(op->kind, op->dims, t);
         if ((int64_t) need > max_lds)
             return fail(EHP_ERR_ARGUMENT, "tile needs %zu bytes of LDS, the device allows %d", need, max_lds);
         op->tile = t;

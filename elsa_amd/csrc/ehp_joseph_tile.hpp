@@ -94,28 +94,14 @@ namespace ehp
         return t;
     }
 
-    // Candidate pixels of one pose for a tile. A ray touches the tile's slots only with a
-    // sample point in [A0 + 0.5, A1 - 0.5] x [B0 - 0.5, B1 + 0.5) x [C0 - 0.5, C1 + 0.5)
-    // (sample points lie on plane centres; floor(q - 0.5) in [B0 - 1, B1 - 1] is
-    // q in [B0 - 0.5, B1 + 0.5)). A ray through that box projects inside the projection of the
-    // box, which lies in the bounding rectangle of its projected corners; a point X lies on
-    // the ray of pixel u when P X (homogeneous) maps to u + 0.5. The rectangle is widened by a
-    // quarter pixel for rounding. Falls back to the whole detector when a corner is not in
-    // front of the source.
-    template <int D, int A>
-    EHP_HD inline bool tile_bbox(const PoseGeom& g, const TileBox& t, const Dims& dims, int& u0, int& u1, int& v0,
-                                 int& v1)
+    // Candidate pixels of one pose for the box [lo, hi]: the bounding rectangle of the box's
+    // projected corners (a point X lies on the ray of pixel u when P X, homogeneous, maps to
+    // u + 0.5), widened by a quarter pixel for rounding. Falls back to the whole detector when
+    // a corner is not in front of the source.
+    template <int D>
+    EHP_HD inline bool box_bbox(const PoseGeom& g, const float* lo, const float* hi, const Dims& dims, int& u0,
+                                int& u1, int& v0, int& v1)
     {
-        constexpr int b1 = TileAxes<D, A>::b1, b2 = TileAxes<D, A>::b2;
-        float lo[3], hi[3];
-        lo[A] = (float) t.A0 + 0.5f;
-        hi[A] = (float) t.A1 - 0.5f;
-        lo[b1] = (float) t.B0 - 0.5f;
-        hi[b1] = (float) t.B1 + 0.5f;
-        if (D == 3) {
-            lo[b2] = (float) t.C0 - 0.5f;
-            hi[b2] = (float) t.C1 + 0.5f;
-        }
         float umin = 3.0e38f, umax = -3.0e38f, vmin = 3.0e38f, vmax = -3.0e38f;
         bool front = true;
         for (int c = 0; c < (1 << D); ++c) {
@@ -168,6 +154,27 @@ namespace ehp
             v0 = v1 = 0;
         }
         return u0 <= u1 && v0 <= v1;
+    }
+
+    // Candidate pixels of one pose for a Joseph tile. A ray touches the tile's slots only with
+    // a sample point in [A0 + 0.5, A1 - 0.5] x [B0 - 0.5, B1 + 0.5) x [C0 - 0.5, C1 + 0.5)
+    // (sample points lie on plane centres; floor(q - 0.5) in [B0 - 1, B1 - 1] is
+    // q in [B0 - 0.5, B1 + 0.5)), and a ray through that box projects inside box_bbox of it.
+    template <int D, int A>
+    EHP_HD inline bool tile_bbox(const PoseGeom& g, const TileBox& t, const Dims& dims, int& u0, int& u1, int& v0,
+                                 int& v1)
+    {
+        constexpr int b1 = TileAxes<D, A>::b1, b2 = TileAxes<D, A>::b2;
+        float lo[3] = {0.0f, 0.0f, 0.0f}, hi[3] = {0.0f, 0.0f, 0.0f};
+        lo[A] = (float) t.A0 + 0.5f;
+        hi[A] = (float) t.A1 - 0.5f;
+        lo[b1] = (float) t.B0 - 0.5f;
+        hi[b1] = (float) t.B1 + 0.5f;
+        if (D == 3) {
+            lo[b2] = (float) t.C0 - 0.5f;
+            hi[b2] = (float) t.C1 + 0.5f;
+        }
+        return box_bbox<D>(g, lo, hi, dims, u0, u1, v0, v1);
     }
 
     // Narrow [slo, shi] to the samples whose position along one transverse axis may lie in

@@ -86,6 +86,19 @@ namespace ehp
         fixexp[0] = fixed_point_exponent(__uint_as_float(*maxbits), bound);
     }
 
+    hipError_t fixed_point_exponent_launch(const float* y, int64_t n, double slot_bound, unsigned int* d_maxbits,
+                                           int* d_fixexp, hipStream_t s)
+    {
+        hipError_t e = launch_fill<unsigned int>(d_maxbits, 1, 0u, s);
+        if (e != hipSuccess)
+            return e;
+        int64_t blocks = (n + 255) / 256;
+        blocks = blocks > 4096 ? 4096 : (blocks < 1 ? 1 : blocks);
+        hipLaunchKernelGGL(absmax_kernel, dim3((unsigned) blocks), dim3(256), 0, s, y, n, d_maxbits);
+        hipLaunchKernelGGL(scale_kernel, dim3(1), dim3(1), 0, s, d_maxbits, slot_bound, d_fixexp);
+        return hipGetLastError();
+    }
+
     // ---------------------------------------------------------------------------------
     // the tiled back projection
 
@@ -228,14 +241,7 @@ namespace ehp
             return hipErrorInvalidValue;
         hipError_t e = launch_fill<float>(vol, dims.nvox, 0.0f, s);
         if (e == hipSuccess)
-            e = launch_fill<unsigned int>(d_maxbits, 1, 0u, s);
-        if (e != hipSuccess)
-            return e;
-        int64_t blocks = (dims.nrays + 255) / 256;
-        blocks = blocks > 4096 ? 4096 : blocks;
-        hipLaunchKernelGGL(absmax_kernel, dim3((unsigned) blocks), dim3(256), 0, s, sino, dims.nrays, d_maxbits);
-        hipLaunchKernelGGL(scale_kernel, dim3(1), dim3(1), 0, s, d_maxbits, slot_bound, d_fixexp);
-        e = hipGetLastError();
+            e = fixed_point_exponent_launch(sino, dims.nrays, slot_bound, d_maxbits, d_fixexp, s);
         if (e != hipSuccess)
             return e;
         const size_t lds = joseph_tile_lds_bytes(dims, tc);

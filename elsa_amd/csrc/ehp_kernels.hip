@@ -150,7 +150,8 @@ namespace ehp
     }
 
     // ---------------------------------------------------------------------------------
-    // Siddon: one thread per ray; the traversal of TraverseAABB (ties step several axes).
+    // Siddon: one thread per ray; the traversal of TraverseAABB with the direct rule for the
+    // crossing parameters (siddon_traverse, ehp_siddon.hpp).
 
     template <int D, bool ADJ, bool COUNT, class T>
     __device__ inline unsigned long long siddon_ray(const float* o, const float* d, const Dims& dims,
@@ -158,85 +159,18 @@ namespace ehp
     {
         T acc = T(0);
         unsigned long long crossings = 0;
-        float tmin, tmax;
-        bool inbox = intersect<D>(o, d, dims.vol, tmin, tmax);
         const T yv = (ADJ && !COUNT) ? in[ray] : T(0);
-        if (inbox) {
-            float entry[3], cur[3], tdelta[3], Tn[3];
-            int stepdir[3];
-#pragma unroll
-            for (int i = 0; i < D; ++i) {
-                const float hi = (float) dims.vol[i];
-                entry[i] = o[i] + tmin * d[i];
-                entry[i] = entry[i] < 0.0f ? 0.0f : entry[i];
-                entry[i] = entry[i] > hi ? hi : entry[i];
-                stepdir[i] = (int) sgnf(d[i]);
-            }
-#pragma unroll
-            for (int i = 0; i < D; ++i) {
-                float lower = floorf(entry[i]);
-                if (lower == entry[i] && stepdir[i] < 0)
-                    lower = lower - 1.0f;
-                cur[i] = lower;
-                if (cur[i] >= (float) dims.vol[i] || cur[i] < 0.0f)
-                    inbox = false;
-            }
-            const float EPS = 1.1920928955078125e-07f;
-            const float MAXF = 3.4028234663852886e+38f;
-            int64_t idx = 0;
-#pragma unroll
-            for (int i = 0; i < D; ++i) {
-                tdelta[i] = fabsf(d[i]) > EPS ? (float) stepdir[i] / d[i] : MAXF;
-                Tn[i] = fabsf(d[i]) > EPS ? ((d[i] > 0.0f ? cur[i] + 1.0f : cur[i]) - entry[i]) / d[i] : MAXF;
-                idx += dims.strides[i] * (int64_t) cur[i];
-            }
-            float tExit = 0.0f;
-            // a ray crosses at most vol[0] + vol[1] + vol[2] voxels; the cap only matters for
-            // degenerate (non-finite) rays, which would otherwise never leave the loop
-            int guard = dims.vol[0] + dims.vol[1] + dims.vol[2] + 2;
-            while (inbox && guard-- > 0) {
-                const float tEntry = tExit;
-                float tm = Tn[0];
-#pragma unroll
-                for (int i = 1; i < D; ++i)
-                    tm = fminf(tm, Tn[i]);
-                tExit = tm;
-                bool mask[3];
-#pragma unroll
-                for (int i = 0; i < D; ++i) {
-                    bool mi = true;
-#pragma unroll
-                    for (int j = 0; j < D; ++j)
-                        if (j != i)
-                            mi = mi && (Tn[i] <= Tn[j]);
-                    mask[i] = mi;
-                }
-                int64_t nidx = idx;
-                bool nin = true;
-#pragma unroll
-                for (int i = 0; i < D; ++i) {
-                    if (mask[i]) {
-                        cur[i] += (float) stepdir[i];
-                        Tn[i] += tdelta[i];
-                        nidx += (int64_t) stepdir[i] * dims.strides[i];
-                    }
-                }
-#pragma unroll
-                for (int i = 0; i < D; ++i)
-                    nin = nin && (cur[i] < (float) dims.vol[i]) && (cur[i] >= 0.0f);
-                const float w = tExit - tEntry;
-                if (COUNT) {
-                    ++crossings;
-                } else if (ADJ) {
-                    if (w != 0.0f)
-                        atomic_add(out + idx, yv * (T) w);
-                } else {
-                    acc += in[idx] * (T) w;
-                }
-                idx = nidx;
-                inbox = nin;
-            }
-        }
+        const SidRec r = siddon_setup<D>(o, d, dims.vol);
+        siddon_traverse<D>(r, dims.vol, [&](int c0, int c1, int c2, float w) {
+            const int64_t idx = c0 + c1 * dims.strides[1] + (int64_t) c2 * dims.strides[2];
+            if (COUNT)
+                ++crossings;
+            else if (ADJ) {
+                if (w != 0.0f)
+                    atomic_add(out + idx, yv * (T) w);
+            } else
+                acc += in[idx] * (T) w;
+        });
         if (!ADJ && !COUNT)
             out[ray] = acc;
         return crossings;

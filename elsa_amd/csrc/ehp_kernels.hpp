@@ -3,6 +3,7 @@
 
 #include "ehp_device.hpp"
 #include "ehp_joseph_tile.hpp"
+#include "ehp_siddon.hpp"
 
 struct ehp_op;
 
@@ -21,6 +22,14 @@ namespace ehp
     hipError_t joseph_pose_axes(const RayRec* table, const Dims& dims, int* pose_axes, hipStream_t s);
     size_t joseph_tile_lds_bytes(const Dims& dims, const TileCfg& tc);
     hipError_t joseph_adjoint_tiled(const PoseGeom* geo, const RayRec* table, const Dims& dims, const int* pose_axes,
+                                    int axes_present, const TileCfg& tc, int batch, double slot_bound,
+                                    unsigned int* d_maxbits, int* d_fixexp, const float* sino, float* vol,
+                                    hipStream_t s);
+    hipError_t fixed_point_exponent_launch(const float* y, int64_t n, double slot_bound, unsigned int* d_maxbits,
+                                           int* d_fixexp, hipStream_t s);
+    hipError_t siddon_ray_table(const PoseGeom* geo, const Dims& dims, SidRec* table, int* pose_axes, hipStream_t s);
+    size_t siddon_tile_lds_bytes(const Dims& dims, const TileCfg& tc);
+    hipError_t siddon_adjoint_tiled(const PoseGeom* geo, const SidRec* table, const Dims& dims, const int* pose_axes,
                                     int axes_present, const TileCfg& tc, int batch, double slot_bound,
                                     unsigned int* d_maxbits, int* d_fixexp, const float* sino, float* vol,
                                     hipStream_t s);

@@ -1,0 +1,243 @@
+// ehp_siddon_tiled.hip -- Siddon back projection with LDS-privatised volume tiles (FP32).
+//
+// The adjoint of Siddon's method (elsa/projectors/SiddonsMethod.cpp:76-124, adjoint
+// :119-120) without global atomics, on the scheme of the tiled Joseph back projection: the
+// volume is cut into tiles long along one driving axis, one workgroup owns a tile for all
+// poses and sums in LDS every crossing of every ray (driving along that axis) that passes
+// through it, then adds the tile to the volume once; one launch per driving axis. A ray
+// enters a tile through siddon_box, which restarts its traversal exactly at the tile's
+// boundary, so every crossing -- voxel and length -- is the one the forward visits.
+//
+// Sums are 64-bit fixed point as in the Joseph kernel: one exponent per call from max|y|
+// and the slot bound, each ray's contributions rounded to int32 in the ray's own scale.
+#include "ehp_kernels.hpp"
+
+namespace ehp
+{
+    namespace
+    {
+        constexpr int kThreads = 256;
+    } // namespace
+
+    template <int D>
+    __global__ void __launch_bounds__(256)
+        siddon_table_kernel(const PoseGeom* __restrict__ geo, const Dims dims, SidRec* __restrict__ table)
+    {
+        const int pose = blockIdx.y;
+        const int64_t pix = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+        if (pix >= dims.pix_per_pose)
+            return;
+        const int u = (int) (pix % dims.det[0]);
+        const int v = (int) (pix / dims.det[0]);
+        float o[3], d[3];
+        make_ray<D>(geo[pose], u, v, o, d);
+        table[(int64_t) pose * dims.pix_per_pose + pix] = siddon_setup<D>(o, d, dims.vol);
+    }
+
+    __global__ void __launch_bounds__(256)
+        siddon_pose_axes_kernel(const SidRec* __restrict__ table, const Dims dims, int* __restrict__ pose_axes)
+    {
+        const int pose = blockIdx.y;
+        const int64_t pix = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+        if (pix >= dims.pix_per_pose)
+            return;
+        const SidRec r = table[(int64_t) pose * dims.pix_per_pose + pix];
+        if (sid_hit(r))
+            atomicOr(pose_axes + pose, 1 << sid_axis(r));
+    }
+
+    hipError_t siddon_ray_table(const PoseGeom* geo, const Dims& dims, SidRec* table, int* pose_axes, hipStream_t s)
+    {
+        const dim3 grid((unsigned) ((dims.pix_per_pose + 255) / 256), (unsigned) dims.nposes);
+        if (dims.dim == 2)
+            hipLaunchKernelGGL((siddon_table_kernel<2>), grid, dim3(256), 0, s, geo, dims, table);
+        else
+            hipLaunchKernelGGL((siddon_table_kernel<3>), grid, dim3(256), 0, s, geo, dims, table);
+        hipError_t e = launch_fill<int>(pose_axes, dims.nposes, 0, s);
+        if (e != hipSuccess)
+            return e;
+        hipLaunchKernelGGL(siddon_pose_axes_kernel, grid, dim3(256), 0, s, table, dims, pose_axes);
+        return hipGetLastError();
+    }
+
+    size_t siddon_tile_lds_bytes(const Dims& dims, const TileCfg& tc)
+    {
+        return (size_t) tc.LA * tc.TB * (dims.dim == 3 ? tc.TC : 1) * sizeof(long long);
+    }
+
+    template <int D, int A>
+    __global__ void __launch_bounds__(kThreads)
+        siddon_adj_tiled_kernel(const PoseGeom* __restrict__ geo, const SidRec* __restrict__ table, const Dims dims,
+                                const int* __restrict__ pose_axes, const TileCfg tc, const int ntA, const int ntB,
+                                const int ntC, const int tiles_per_xcd, const int batch,
+                                const int* __restrict__ fixexp, const float* __restrict__ sino,
+                                float* __restrict__ vol)
+    {
+        constexpr int b1 = TileAxes<D, A>::b1, b2 = TileAxes<D, A>::b2;
+        extern __shared__ long long lds[];
+        __shared__ int s_u0[kMaxPoseBatch], s_nu[kMaxPoseBatch], s_v0[kMaxPoseBatch], s_pose[kMaxPoseBatch];
+        __shared__ int s_cnt[kMaxPoseBatch], s_pref[kMaxPoseBatch + 1];
+        __shared__ float s_rnu[kMaxPoseBatch];
+
+        // consecutive workgroups land on different XCDs: give each XCD a contiguous block of tiles
+        const int tile = (int) (blockIdx.x % 8u) * tiles_per_xcd + (int) (blockIdx.x / 8u);
+        if (tile >= ntA * ntB * ntC)
+            return;
+        const int tid = threadIdx.x;
+        const TileBox t = make_tile<D, A>(dims, tc, tile % ntA, (tile / ntA) % ntB, tile / (ntA * ntB));
+        const int nslots = tc.LA * tc.TB * (D == 3 ? tc.TC : 1);
+        long long* iacc = lds;
+        for (int i = tid; i < nslots; i += kThreads)
+            iacc[i] = 0;
+        // the tile's voxel box
+        int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        lo[A] = t.A0;
+        hi[A] = t.A1 - 1;
+        lo[b1] = t.B0;
+        hi[b1] = t.B1 - 1;
+        if (D == 3) {
+            lo[b2] = t.C0;
+            hi[b2] = t.C1 - 1;
+        }
+        float flo[3] = {0.0f, 0.0f, 0.0f}, fhi[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            flo[i] = (float) lo[i];
+            fhi[i] = (float) (hi[i] + 1);
+        }
+        const int eg = fixexp[0];
+        __syncthreads();
+
+        for (int pbase = 0; eg != kNonFinite && pbase < dims.nposes; pbase += batch) {
+            if (tid < batch) {
+                const int p = pbase + tid;
+                int u0 = 0, u1 = -1, v0 = 0, v1 = -1;
+                int cnt = 0;
+                if (p < dims.nposes && (pose_axes[p] & (1 << A))) {
+                    if (box_bbox<D>(geo[p], flo, fhi, dims, u0, u1, v0, v1))
+                        cnt = (u1 - u0 + 1) * (v1 - v0 + 1);
+                }
+                s_u0[tid] = u0;
+                s_nu[tid] = u1 - u0 + 1;
+                s_rnu[tid] = 1.0f / (float) (u1 - u0 + 1);
+                s_v0[tid] = v0;
+                s_pose[tid] = p;
+                s_cnt[tid] = cnt;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                s_pref[0] = 0;
+                for (int j = 0; j < batch; ++j)
+                    s_pref[j + 1] = s_pref[j] + s_cnt[j];
+            }
+            __syncthreads();
+            const int total = s_pref[batch];
+            auto cand_ray = [&](int k) -> int64_t {
+                int j = 0;
+                while (k >= s_pref[j + 1])
+                    ++j;
+                const int loc = k - s_pref[j];
+                // loc / nu through the reciprocal: exact, as loc + 0.5 is at least 0.5 away from
+                // a multiple of nu and loc < 2^22
+                const int row = (int) (((float) loc + 0.5f) * s_rnu[j]);
+                const int u = s_u0[j] + (loc - row * s_nu[j]);
+                const int v = s_v0[j] + row;
+                return (int64_t) s_pose[j] * dims.pix_per_pose + (int64_t) v * dims.det[0] + u;
+            };
+            float yv_next = 0.0f;
+            SidRec r_next{};
+            if (tid < total) {
+                const int64_t ray = cand_ray(tid);
+                yv_next = sino[ray];
+                r_next = table[ray];
+            }
+            for (int k = tid; k < total; k += kThreads) {
+                const float yv = yv_next;
+                const SidRec r = r_next;
+                if (k + kThreads < total) {
+                    const int64_t ray = cand_ray(k + kThreads);
+                    yv_next = sino[ray];
+                    r_next = table[ray];
+                }
+                if (yv == 0.0f || sid_axis(r) != A)
+                    continue;
+                // lengths are at most sqrt(3) < 2: scale 2 y, contribute (ys) (w / 2)
+                float ys;
+                int sh;
+                ray_scale(yv * 2.0f, eg, ys, sh);
+                siddon_box<D>(r, lo, hi, [&](int c0, int c1, int c2, float w) {
+                    if (w != 0.0f) {
+                        const int slot = sid_slot<D, A>(t, tc, c0, c1, c2);
+                        atomicAdd(reinterpret_cast<unsigned long long*>(iacc + slot),
+                                  fixed_contrib(ys * (w * 0.5f), sh));
+                    }
+                });
+            }
+            __syncthreads();
+        }
+
+        // add the tile to the volume
+        const double q = eg == kNonFinite ? __longlong_as_double(0x7ff8000000000000ll) : ldexp(1.0, eg - 62);
+        const int TBe = tc.TB, TCe = D == 3 ? tc.TC : 1;
+        for (int i = tid; i < nslots; i += kThreads) {
+            const int xb = i % TBe;
+            const int rest = i / TBe;
+            const int xc = rest % TCe;
+            const int xa = rest / TCe;
+            const int ca = t.A0 + xa, cb = t.B0 + xb, cc = t.C0 + xc;
+            if (ca >= t.A1 || cb >= t.B1 || (D == 3 && cc >= t.C1))
+                continue;
+            int c[3] = {0, 0, 0};
+            c[A] = ca;
+            c[b1] = cb;
+            if (D == 3)
+                c[b2] = cc;
+            const int64_t lin = c[0] + c[1] * dims.strides[1] + (int64_t) c[2] * dims.strides[2];
+            vol[lin] += (float) ((double) iacc[i] * q);
+        }
+    }
+
+    hipError_t siddon_adjoint_tiled(const PoseGeom* geo, const SidRec* table, const Dims& dims, const int* pose_axes,
+                                    int axes_present, const TileCfg& tc, int batch, double slot_bound,
+                                    unsigned int* d_maxbits, int* d_fixexp, const float* sino, float* vol,
+                                    hipStream_t s)
+    {
+        if (batch < 1 || batch > kMaxPoseBatch)
+            return hipErrorInvalidValue;
+        hipError_t e = launch_fill<float>(vol, dims.nvox, 0.0f, s);
+        if (e == hipSuccess)
+            e = fixed_point_exponent_launch(sino, dims.nrays, slot_bound, d_maxbits, d_fixexp, s);
+        if (e != hipSuccess)
+            return e;
+        const size_t lds = siddon_tile_lds_bytes(dims, tc);
+        for (int a = 0; a < dims.dim; ++a) {
+            if (!(axes_present & (1 << a)))
+                continue;
+            int ntA, ntB, ntC;
+            tile_counts(dims, tc, a, ntA, ntB, ntC);
+            const int per_xcd = (ntA * ntB * ntC + 7) / 8;
+            const dim3 grid((unsigned) (per_xcd * 8));
+#define EHP_SID_TILED(Dv, Av)                                                                                      \
+    hipLaunchKernelGGL((siddon_adj_tiled_kernel<Dv, Av>), grid, dim3(kThreads), lds, s, geo, table, dims, pose_axes, \
+                       tc, ntA, ntB, ntC, per_xcd, batch, d_fixexp, sino, vol)
+            if (dims.dim == 2) {
+                if (a == 0)
+                    EHP_SID_TILED(2, 0);
+                else
+                    EHP_SID_TILED(2, 1);
+            } else {
+                if (a == 0)
+                    EHP_SID_TILED(3, 0);
+                else if (a == 1)
+                    EHP_SID_TILED(3, 1);
+                else
+                    EHP_SID_TILED(3, 2);
+            }
+#undef EHP_SID_TILED
+            e = hipGetLastError();
+            if (e != hipSuccess)
+                return e;
+        }
+        return hipSuccess;
+    }
+} // namespace ehp

@@ -106,6 +106,13 @@ class _HipProjector(LinearOperator):
     def handle(self) -> ctypes.c_void_p:
         return self._op
 
+    def tile(self, la: int = 0, tb: int = 0, tc: int = 0):
+        """Set (la > 0) and return the (LA, TB, TC) tile of the LDS-tiled back projection
+        (Joseph and Siddon, FP32)."""
+        o = [ctypes.c_int(), ctypes.c_int(), ctypes.c_int()]
+        _lib.check(self._L.ehp_adjoint_tile(self._op, la, tb, tc, *(ctypes.byref(x) for x in o)), "tile")
+        return tuple(x.value for x in o)
+
     def sample_count(self) -> int:
         """Exact number of samples (Joseph), voxel crossings (Siddon) or footprint taps
         (voxel projectors) of one pass."""
@@ -164,15 +171,9 @@ class JosephsMethodCUDA(_HipProjector):
         super().__init__(domainDescriptor, rangeDescriptor, dtype)
 
     def _create(self, geo):
-        flags = _lib.EHP_JOSEPH_SIMPLE_KERNELS if self.kernels == "simple" else 0
+        flags = _lib.EHP_SIMPLE_KERNELS if self.kernels == "simple" else 0
         _lib.check(self._L.ehp_joseph_create(geo, _DT[self._dtype], flags, ctypes.byref(self._op)),
                    "JosephsMethodCUDA")
-
-    def tile(self, la: int = 0, tb: int = 0, tc: int = 0):
-        """Set (la > 0) and return the (LA, TB, TC) tile of the LDS-tiled back projection."""
-        o = [ctypes.c_int(), ctypes.c_int(), ctypes.c_int()]
-        _lib.check(self._L.ehp_joseph_tile(self._op, la, tb, tc, *(ctypes.byref(x) for x in o)), "tile")
-        return tuple(x.value for x in o)
 
     def ray_info(self):
         """Driving axis and sample count of every ray as the device computes them."""
@@ -184,13 +185,22 @@ class JosephsMethodCUDA(_HipProjector):
 
 
 class SiddonsMethodCUDA(_HipProjector):
-    """Siddon's method (traversal of TraverseAABB.cpp:6-184)."""
+    """Siddon's method (traversal of TraverseAABB.cpp:6-184, crossing parameters T0 + k tDelta).
 
-    def __init__(self, domainDescriptor, rangeDescriptor, dtype=np.float32):
+    ``kernels="fast"`` (FP32) runs the back projection over LDS-privatised tiles with 64-bit
+    fixed-point sums; ``"simple"`` runs one thread per ray with float atomics. FP64 always uses
+    the simple kernels."""
+
+    def __init__(self, domainDescriptor, rangeDescriptor, dtype=np.float32, kernels: str = "fast"):
+        if kernels not in ("fast", "simple"):
+            raise ValueError("kernels must be 'fast' or 'simple'")
+        self.kernels = kernels
         super().__init__(domainDescriptor, rangeDescriptor, dtype)
 
     def _create(self, geo):
-        _lib.check(self._L.ehp_siddon_create(geo, _DT[self._dtype], ctypes.byref(self._op)), "SiddonsMethodCUDA")
+        flags = _lib.EHP_SIMPLE_KERNELS if self.kernels == "simple" else 0
+        _lib.check(self._L.ehp_siddon_create(geo, _DT[self._dtype], flags, ctypes.byref(self._op)),
+                   "SiddonsMethodCUDA")
 
 
 def blob_lut(radius=2.0, alpha=10.83, order=2, n=100, dtype=np.float32):

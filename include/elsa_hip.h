@@ -50,8 +50,9 @@ enum ehp_status {
 
 /* flags of ehp_joseph_create */
 enum ehp_joseph_flags {
-    EHP_JOSEPH_SIMPLE_KERNELS = 1 /* one thread per ray in both directions (atomic back
-                                     projection); the reference point for the tiled kernels */
+    EHP_SIMPLE_KERNELS = 1 /* Joseph and Siddon: one thread per ray in both directions
+                              (atomic back projection); the reference point for the tiled
+                              kernels */
 };
 
 enum ehp_dtype {
@@ -79,12 +80,13 @@ void ehp_geometry_destroy(ehp_geometry* g);
 /* Joseph's method (elsa: JosephsMethodCUDA(volume, detector, fast), JosephsMethodCUDA.h:78-79,
  * with the discretization of the CPU JosephsMethod, JosephsMethod.cpp:140-198). The adjoint
  * is always the matched one (elsa's fast=false); elsa's unmatched fast=true back projection
- * has no counterpart. flags: 0 or EHP_JOSEPH_SIMPLE_KERNELS. */
+ * has no counterpart. flags: 0 or EHP_SIMPLE_KERNELS. */
 int ehp_joseph_create(const ehp_geometry* g, int dtype, int flags, ehp_op** out);
 
 /* Siddon's method (elsa: SiddonsMethodCUDA(volume, detector), SiddonsMethodCUDA.h:71-72,
- * traversal of TraverseAABB.cpp:6-184). */
-int ehp_siddon_create(const ehp_geometry* g, int dtype, ehp_op** out);
+ * traversal of TraverseAABB.cpp:6-184, crossing parameters T0 + k * tDelta with one rounding
+ * each). FP32 back projections run LDS-tiled; flags: 0 or EHP_SIMPLE_KERNELS. */
+int ehp_siddon_create(const ehp_geometry* g, int dtype, int flags, ehp_op** out);
 
 /* Voxel-driven projector with a radially symmetric basis function sampled in a table of
  * n_lut values over [0, support] (elsa: BlobVoxelProjectorCUDA / BSplineVoxelProjectorCUDA,
@@ -110,12 +112,12 @@ int ehp_op_sizes(const ehp_op* op, int64_t* n_vol, int64_t* n_sino);
  * number of footprint taps. Computed on the device at create time. */
 int ehp_sample_count(const ehp_op* op, uint64_t* samples);
 
-/* Tile shape of the LDS-tiled Joseph back projection (FP32): la planes along the driving axis,
- * tb x tc voxels across (tc ignored in 2D). No counterpart in elsa; it is the knob the tile
- * autotuner turns. Fails with EHP_ERR_ARGUMENT if the tile does not fit the device's LDS or
- * the operator does not use the tiled kernels. Queries only when out pointers are given
- * and la <= 0. */
-int ehp_joseph_tile(ehp_op* op, int la, int tb, int tc, int* la_out, int* tb_out, int* tc_out);
+/* Tile shape of the LDS-tiled back projection (Joseph and Siddon, FP32): la planes along the
+ * driving axis, tb x tc voxels across (tc ignored in 2D). No counterpart in elsa; it is the
+ * knob the tile autotuner turns. Fails with EHP_ERR_ARGUMENT if the tile does not fit the
+ * device's LDS or the operator does not use a tiled back projection. Queries only when out
+ * pointers are given and la <= 0. */
+int ehp_adjoint_tile(ehp_op* op, int la, int tb, int tc, int* la_out, int* tb_out, int* tc_out);
 
 /* Per-ray Joseph traversal state as the kernels compute it (for parity checks): driving
  * axis and number of samples of every ray, host memory of n_sino entries each. */

@@ -223,8 +223,8 @@ def joseph(g, data, adjoint=False, mode=MODE_ELSA, nthreads=0, acc64=False):
     return xray(JOSEPH, g, data, adjoint, mode, nthreads, acc64)
 
 
-def siddon(g, data, adjoint=False, nthreads=0):
-    return xray(SIDDON, g, data, adjoint, MODE_ELSA, nthreads)
+def siddon(g, data, adjoint=False, nthreads=0, mode=MODE_ELSA, acc64=False):
+    return xray(SIDDON, g, data, adjoint, mode, nthreads, acc64)
 
 
 def voxel(g: Geom, data, lut, support, adjoint=False, nthreads=0):

@@ -9,6 +9,7 @@
 //   *_f32 : R = float,  T = float   (elsa's own precision: real_t is float,
 //                                    elsa/core/elsaDefines.h:20)
 //   *_f64 : R = double, T = double  (an FP64 oracle with double geometry; elsa has none)
+//   orc_xray_f32_acc64 : R = float, T = double (FP32 geometry and weights, FP64 sums)
 // Built with -ffp-contract=off so that no product/sum pair is fused behind our back;
 // where a fused multiply-add is part of the algorithm it is written as std::fma.
 //
@@ -19,8 +20,9 @@
 //   joseph_*          KATs of elsa/projectors/tests/test_JosephsMethod.cpp, linearity,
 //                     dot-product adjointness, FP64 brute force on tiny grids
 //   siddon_*          KATs of elsa/projectors/tests/test_SiddonsMethod.cpp and
-//                     elsa/projectors_cuda/tests/test_SiddonsMethodCUDA.cpp, exact
-//                     chord length sum, adjointness
+//                     elsa/projectors_cuda/tests/test_SiddonsMethodCUDA.cpp in both position
+//                     rules, exact chord length sum, adjointness, the rules against the
+//                     FP64 traversal
 //   voxel_*           centre-voxel blob(0) KAT and blob(d/s) back-projection weights of
 //                     elsa/projectors_cuda/tests/test_VoxelProjectorCUDA.cpp, adjointness
 //   LUTs / Bessel     golden vectors of elsa/projectors/tests/test_Luts.cpp, Simpson
@@ -319,9 +321,12 @@ namespace orc
     // (elsa/projectors/SiddonsMethod.cpp:76-124). Ties in T step several axes at once
     // (calcMask, TraverseAABB.cpp:59-69). The length of each segment is tExit - tEntry with
     // tExit starting at 0 (TraverseAABB.h: _tExit{0.0}).
+    // mode MODE_ELSA: T[i] += tDelta[i] per step, as TraverseAABB.cpp:59-83 does;
+    // mode MODE_DIRECT: T[i] = fma(k[i], tDelta[i], T0[i]) after k[i] steps along axis i (one
+    // rounding per value, and the state at any point of the ray follows from the step counts).
     template <class R, class T, int D, bool ADJ>
-    void siddon_ray(const R* o, const R* d, const i64* dims, const i64* strides, const T* in,
-                    T* out, i64 ir)
+    void siddon_ray(const R* o, const R* d, const i64* dims, const i64* strides, int mode,
+                    const T* in, T* out, i64 ir)
     {
         T acc = T(0);
         R tmin, tmax;
@@ -347,10 +352,13 @@ namespace orc
             const R EPS = std::numeric_limits<R>::epsilon();
             const R MAX = std::numeric_limits<R>::max();
             i64 idx = 0;
+            R T0[3];
+            i64 k[3] = {0, 0, 0};
             for (int i = 0; i < D; ++i) {
                 tdelta[i] = std::abs(d[i]) > EPS ? R(stepdir[i]) / d[i] : MAX;
                 T_[i] = std::abs(d[i]) > EPS ? ((d[i] > R(0) ? cur[i] + R(1) : cur[i]) - entry[i]) / d[i]
                                              : MAX;
+                T0[i] = T_[i];
                 idx += strides[i] * static_cast<i64>(cur[i]);
             }
             R tExit = R(0);
@@ -375,7 +383,12 @@ namespace orc
                 for (int i = 0; i < D; ++i) {
                     if (mask[i]) {
                         cur[i] += R(stepdir[i]);
-                        T_[i] += tdelta[i];
+                        if (mode == 1) {
+                            k[i] += 1;
+                            T_[i] = tdelta[i] == MAX ? MAX : std::fma(R(k[i]), tdelta[i], T0[i]);
+                        } else {
+                            T_[i] += tdelta[i];
+                        }
                         nidx += stepdir[i] * strides[i];
                     }
                 }
@@ -397,7 +410,7 @@ namespace orc
     }
 
     template <class R, class T, int D, bool ADJ>
-    void siddon_all(const Shape& sh, const R* Pinv, const R* C, const T* in, T* out)
+    void siddon_all(const Shape& sh, const R* Pinv, const R* C, int mode, const T* in, T* out)
     {
         if (ADJ)
             std::fill(out, out + sh.nvox, T(0));
@@ -405,7 +418,7 @@ namespace orc
         for (i64 ir = 0; ir < sh.nrays; ++ir) {
             R o[3], d[3];
             ray_of_index<R, D>(sh, Pinv, C, ir, o, d);
-            siddon_ray<R, T, D, ADJ>(o, d, sh.vol, sh.strides, in, out, ir);
+            siddon_ray<R, T, D, ADJ>(o, d, sh.vol, sh.strides, mode, in, out, ir);
         }
     }
 
@@ -893,11 +906,11 @@ namespace orc
                         : joseph_all<R, T, 3, false>(sh, Pinv, C, mode, in, out);
         } else {
             if (dim == 2)
-                adjoint ? siddon_all<R, T, 2, true>(sh, Pinv, C, in, out)
-                        : siddon_all<R, T, 2, false>(sh, Pinv, C, in, out);
+                adjoint ? siddon_all<R, T, 2, true>(sh, Pinv, C, mode, in, out)
+                        : siddon_all<R, T, 2, false>(sh, Pinv, C, mode, in, out);
             else
-                adjoint ? siddon_all<R, T, 3, true>(sh, Pinv, C, in, out)
-                        : siddon_all<R, T, 3, false>(sh, Pinv, C, in, out);
+                adjoint ? siddon_all<R, T, 3, true>(sh, Pinv, C, mode, in, out)
+                        : siddon_all<R, T, 3, false>(sh, Pinv, C, mode, in, out);
         }
         return 0;
     }
@@ -953,8 +966,9 @@ namespace orc
 
 extern "C" {
 
-// Projectors. kind: 0 = Joseph, 1 = Siddon. mode (Joseph only): 0 = elsa's repeated
-// addition, 1 = pos0 + s*step. Pinv: nviews blocks of (dim+1) x dim, C: nviews x dim.
+// Projectors. kind: 0 = Joseph, 1 = Siddon. mode: 0 = elsa's repeated addition (Joseph
+// positions, Siddon crossing parameters), 1 = one rounding per value (Joseph pos0 + s*step,
+// Siddon T0 + k*tDelta). Pinv: nviews blocks of (dim+1) x dim, C: nviews x dim.
 // Volume x fastest; sinogram [u][v][view] with u fastest. Returns 0 on success.
 int orc_xray_f32(int kind, int dim, int adjoint, int mode, const int64_t* vol, const int64_t* det,
                  int64_t nviews, const float* Pinv, const float* C, const float* in, float* out,

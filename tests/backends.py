@@ -33,7 +33,7 @@ class OracleBackend:
         return oracle.joseph(g, np.asarray(x, self.dtype), adjoint, self.mode)
 
     def siddon(self, g, x, adjoint=False):
-        return oracle.siddon(g, np.asarray(x, self.dtype), adjoint)
+        return oracle.siddon(g, np.asarray(x, self.dtype), adjoint, mode=self.mode)
 
     def voxel(self, g, x, kind="blob", adjoint=False, **kw):
         if kind == "blob":

@@ -214,9 +214,27 @@ def test_joseph_tie_kats_oracle(kat):
 
 
 @pytest.mark.parametrize("kat", kats.SIDDON_KATS, ids=lambda k: k.__name__)
+@pytest.mark.parametrize("mode", [oracle.MODE_ELSA, oracle.MODE_DIRECT], ids=["elsa", "direct"])
 @pytest.mark.parametrize("dt", [np.float32, np.float64], ids=["f32", "f64"])
-def test_siddon_kats_oracle(kat, dt):
-    kat(OracleBackend(dt))
+def test_siddon_kats_oracle(kat, mode, dt):
+    kat(OracleBackend(dt, mode))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_siddon_direct_rule_close_to_repeated_addition(dim):
+    """The two rules for the crossing parameters differ only by rounding: FP32 results agree to
+    1e-5, and both are within FP32 rounding of the FP64 traversal."""
+    n = 32 if dim == 3 else 64
+    det = (45, 40) if dim == 3 else (90,)
+    g = _rand_geom(dim, n, 9, det, np.float32, seed=11)
+    g64 = _rand_geom(dim, n, 9, det, np.float64, seed=11)
+    x = np.random.default_rng(12).random(g.vol_shape).astype(np.float32)
+    a = oracle.siddon(g, x, mode=oracle.MODE_ELSA)
+    b = oracle.siddon(g, x, mode=oracle.MODE_DIRECT)
+    ref = oracle.siddon(g64, x.astype(np.float64), mode=oracle.MODE_DIRECT)
+    rel = lambda u, v: np.linalg.norm(u - v) / np.linalg.norm(v)
+    assert rel(a, b) <= 1e-5
+    assert rel(b, ref) <= 1e-4 and rel(a, ref) <= 1e-4
 
 
 @pytest.mark.parametrize("kat", kats.VOXEL_KATS, ids=lambda k: k.__name__)
@@ -246,8 +264,8 @@ def test_adjointness_fp64(dim, kind):
         A = lambda v: oracle.joseph(g, v)
         At = lambda v: oracle.joseph(g, v, adjoint=True)
     elif kind == "siddon":
-        A = lambda v: oracle.siddon(g, v)
-        At = lambda v: oracle.siddon(g, v, adjoint=True)
+        A = lambda v: oracle.siddon(g, v, mode=oracle.MODE_DIRECT)
+        At = lambda v: oracle.siddon(g, v, adjoint=True, mode=oracle.MODE_DIRECT)
     else:
         lut = oracle.blob_lut(2.0, 10.83, 2, 100, np.float64)
         A = lambda v: oracle.voxel(g, v, lut, 2.0)
@@ -285,7 +303,7 @@ def test_siddon_lengths_sum_to_chord():
     """Siddon: the intersection lengths of a ray sum to its chord through the box."""
     n = 9
     g = _rand_geom(3, n, 6, (13, 11), np.float64, seed=8)
-    ones = oracle.siddon(g, np.ones(g.vol_shape))
+    ones = oracle.siddon(g, np.ones(g.vol_shape), mode=oracle.MODE_DIRECT)
     axis, nsteps, pos0, step, ln = oracle.joseph_rays(g)
     for r in np.random.default_rng(9).choice(ones.size, 40, replace=False):
         view, rem = divmod(int(r), 13 * 11)
