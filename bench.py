@@ -57,7 +57,8 @@ def parse():
     p.add_argument("--kernels", default="fast", choices=["fast", "simple"],
                    help="Joseph kernels: plane-major forward + LDS-tiled back projection (default) or one "
                         "thread per ray")
-    p.add_argument("--tile", default="", help="LA,TB,TC of the tiled back projection (default: built-in)")
+    p.add_argument("--tile", default="", help="LA,TB,TC of the tiled Joseph / Siddon back projection, or W,H "
+                                              "pixels of the tiled voxel forward (default: built-in)")
     p.add_argument("--projector", default="joseph", choices=["joseph", "siddon", "blob", "bspline"],
                    help="projector to time (default: Joseph, the headline metric)")
     p.add_argument("--cg-iters", type=int, default=20,
@@ -156,11 +157,11 @@ def main():
     elif a.projector == "siddon":
         op = E.SiddonsMethodCUDA(vd, dd, kernels=a.kernels)
     elif a.projector == "blob":
-        op = E.BlobVoxelProjectorCUDA(vd, dd)
+        op = E.BlobVoxelProjectorCUDA(vd, dd, kernels=a.kernels)
     else:
-        op = E.BSplineVoxelProjectorCUDA(vd, dd)
+        op = E.BSplineVoxelProjectorCUDA(vd, dd, kernels=a.kernels)
     tile = None
-    if a.projector in ("joseph", "siddon") and a.kernels == "fast":
+    if a.kernels == "fast":
         tile = op.tile(*[int(v) for v in a.tile.split(",")]) if a.tile else op.tile()
     S = op.sample_count()
     R = int(np.prod(wl["det"])) * wl["views"]
@@ -205,7 +206,8 @@ def main():
     else:
         # one FP32 detector value per footprint tap, one FP32 voxel per voxel and pose
         alg_bytes = 4 * S + 4 * nvox * wl["views"]
-        adj_name, fwd_name = "voxel_adj_kernel", "voxel_fwd_kernel"
+        adj_name = "voxel_adj_kernel"
+        fwd_name = "voxel_fwd_tiled_kernel" if a.kernels == "fast" else "voxel_fwd_kernel"
         unit, metric = "Gtaps/s", f"{a.projector} voxel projector fwd+adj footprint taps/s & %HBM roofline"
     name, t_dom = (adj_name, t_adj) if t_adj >= t_fwd else (fwd_name, t_fwd)
     achieved = alg_bytes / t_dom / 1e9

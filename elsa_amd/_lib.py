@@ -19,7 +19,7 @@ EHP_SIMPLE_KERNELS = 1
 EXPORTS = (
     "ehp_last_error", "ehp_version", "ehp_geometry_create", "ehp_geometry_destroy",
     "ehp_joseph_create", "ehp_siddon_create", "ehp_voxel_create", "ehp_destroy", "ehp_forward",
-    "ehp_adjoint", "ehp_op_sizes", "ehp_sample_count", "ehp_adjoint_tile", "ehp_joseph_ray_info", "ehp_geometry_pose",
+    "ehp_adjoint", "ehp_op_sizes", "ehp_sample_count", "ehp_tile", "ehp_joseph_ray_info", "ehp_geometry_pose",
     "ehp_circle_trajectory", "ehp_blob_lut", "ehp_bspline_lut", "ehp_cgls", "ehp_apgd", "ehp_power_iterations",
 )
 EHP_SOLVE_GRAPH = 1
@@ -78,7 +78,7 @@ def load():
     L.ehp_geometry_destroy.restype = None
     L.ehp_joseph_create.argtypes = [vp, c_int, c_int, ctypes.POINTER(vp)]
     L.ehp_siddon_create.argtypes = [vp, c_int, c_int, ctypes.POINTER(vp)]
-    L.ehp_voxel_create.argtypes = [vp, c_int, vp, c_int, c_d, ctypes.POINTER(vp)]
+    L.ehp_voxel_create.argtypes = [vp, c_int, vp, c_int, c_d, c_int, ctypes.POINTER(vp)]
     L.ehp_destroy.argtypes = [vp]
     L.ehp_destroy.restype = None
     L.ehp_forward.argtypes = [vp, vp, vp, vp]
@@ -87,7 +87,7 @@ def load():
     L.ehp_sample_count.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
     L.ehp_joseph_ray_info.argtypes = [vp, vp, vp]
     ip = ctypes.POINTER(ctypes.c_int)
-    L.ehp_adjoint_tile.argtypes = [vp, c_int, c_int, c_int, ip, ip, ip]
+    L.ehp_tile.argtypes = [vp, c_int, c_int, c_int, ip, ip, ip]
     L.ehp_geometry_pose.argtypes = [c_int, c_f, c_f, fp, fp, fp, fp, c_f, c_f, c_f, fp, fp, fp, fp, fp, fp, fp]
     L.ehp_circle_trajectory.argtypes = [c_int, i64, i64p, c_f, c_f, c_f, i64p, i64p, fp, fp, fp, fp, fp]
     L.ehp_blob_lut.argtypes = [c_int, c_d, c_d, i64, c_int, vp, ctypes.POINTER(c_d)]

@@ -44,6 +44,8 @@ struct ehp_op {
     ehp::TileCfg tile{};
     ehp::RayRec* d_table = nullptr;   // per-ray traversal state (Joseph)
     ehp::SidRec* d_sidtable = nullptr; // per-ray traversal state (Siddon)
+    float* d_pose_margin = nullptr;    // voxel forward: frustum margin per pose (pixels)
+    float lutscale = 1.0f;             // voxel forward: power of two >= max |table|
     unsigned int* d_maxbits = nullptr; // max|y| of the current back projection
     int* d_fixexp = nullptr;           // fixed-point exponent of the current back projection
     double slot_bound = 0.0;
@@ -229,6 +231,8 @@ namespace
     // device's LDS per workgroup.
     size_t tile_lds_bytes(int kind, const ehp::Dims& dims, const ehp::TileCfg& tc)
     {
+        if (kind == KIND_VOXEL)
+            return ehp::voxel_tile_lds_bytes(tc);
         return kind == KIND_SIDDON ? ehp::siddon_tile_lds_bytes(dims, tc) : ehp::joseph_tile_lds_bytes(dims, tc);
     }
 
@@ -378,6 +382,59 @@ namespace
         op->tiled = true;
         return EHP_OK;
     }
+    // FP32 voxel projectors: the detector-tiled forward, when every source lies outside the
+    // volume (the footprint scaling is then bounded).
+    int setup_voxel_tiled(ehp_op* op, const float* host_lut)
+    {
+        const ehp::Dims& dm = op->dims;
+        const int n = dm.nposes;
+        if (op->n_lut > ehp::voxel_max_lut())
+            return EHP_OK;
+        std::vector<int> axis(n);
+        std::vector<float> margin(n);
+        double smax_all = 0.0;
+        for (int p = 0; p < n; ++p) {
+            const double smax = ehp::voxel_max_scaling(op->host_geo[p], dm);
+            if (!(smax > 0.0) || smax > 1.0e6)
+                return EHP_OK; // keep the one-thread-per-voxel kernel
+            smax_all = smax > smax_all ? smax : smax_all;
+            margin[p] = (float) (op->support * smax + 1.0);
+            axis[p] = dm.dim == 3 ? ehp::pose_axis<3>(op->host_geo[p], dm) : ehp::pose_axis<2>(op->host_geo[p], dm);
+        }
+        float lutmax = 0.0f;
+        for (int i = 0; i < op->n_lut; ++i)
+            lutmax = std::fabs(host_lut[i]) > lutmax ? std::fabs(host_lut[i]) : lutmax;
+        if (!(lutmax < 1.0e30f))
+            return EHP_OK;
+        float ls = 1.0f;
+        while (ls < lutmax)
+            ls *= 2.0f;
+        op->lutscale = ls;
+        // what one pixel can receive: at most (2 r s + 2)^(dim-1) voxels per plane, a weight of at
+        // most the table's maximum each, over the longest axis of planes
+        int maxdim = 0;
+        for (int i = 0; i < dm.dim; ++i)
+            maxdim = dm.vol[i] > maxdim ? dm.vol[i] : maxdim;
+        const double per_plane = 2.0 * op->support * smax_all + 2.0;
+        op->slot_bound = (double) maxdim * (dm.dim == 3 ? per_plane * per_plane : per_plane) * ls;
+        hipError_t e = hipMalloc((void**) &op->d_pose_axes, n * sizeof(int));
+        if (e == hipSuccess)
+            e = hipMalloc((void**) &op->d_pose_margin, n * sizeof(float));
+        if (e == hipSuccess)
+            e = hipMalloc((void**) &op->d_maxbits, sizeof(unsigned int));
+        if (e == hipSuccess)
+            e = hipMalloc((void**) &op->d_fixexp, 2 * sizeof(int));
+        if (e != hipSuccess)
+            return fail(EHP_ERR_ALLOC, "hipMalloc of the voxel projector state failed: %s", hipGetErrorString(e));
+        e = hipMemcpy(op->d_pose_axes, axis.data(), n * sizeof(int), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(op->d_pose_margin, margin.data(), n * sizeof(float), hipMemcpyHostToDevice);
+        if (e != hipSuccess)
+            return hip_fail(e, "voxel projector state");
+        op->tile = dm.dim == 3 ? ehp::TileCfg{32, 32, 1} : ehp::TileCfg{256, 1, 1};
+        op->tiled = true;
+        return EHP_OK;
+    }
 } // namespace
 
 int ehp_joseph_create(const ehp_geometry* g, int dtype, int flags, ehp_op** out)
@@ -414,9 +471,11 @@ int ehp_siddon_create(const ehp_geometry* g, int dtype, int flags, ehp_op** out)
     return rc;
 }
 
-int ehp_voxel_create(const ehp_geometry* g, int dtype, const void* lut, int n_lut, double support,
+int ehp_voxel_create(const ehp_geometry* g, int dtype, const void* lut, int n_lut, double support, int flags,
                      ehp_op** out)
 {
+    if (flags & ~EHP_SIMPLE_KERNELS)
+        return fail(EHP_ERR_ARGUMENT, "unknown voxel projector flags 0x%x", flags);
     if (!lut || n_lut < 2)
         return fail(EHP_ERR_ARGUMENT, "the basis-function table needs at least 2 entries");
     if (!(support > 0.0))
@@ -441,6 +500,8 @@ int ehp_voxel_create(const ehp_geometry* g, int dtype, const void* lut, int n_lu
         return hip_fail(e, "basis-function table upload");
     }
     rc = count_samples(op);
+    if (rc == EHP_OK && dtype == EHP_F32 && !(flags & EHP_SIMPLE_KERNELS))
+        rc = setup_voxel_tiled(op, (const float*) lut);
     if (rc != EHP_OK) {
         ehp_destroy(op);
         *out = nullptr;
@@ -457,7 +518,7 @@ void ehp_destroy(ehp_op* op)
     if (cur != op->device)
         (void) hipSetDevice(op->device);
     for (void* p : {(void*) op->d_geo, op->d_lut, (void*) op->d_pose_axes, (void*) op->d_table,
-                    (void*) op->d_sidtable, (void*) op->d_maxbits,
+                    (void*) op->d_sidtable, (void*) op->d_pose_margin, (void*) op->d_maxbits,
                     (void*) op->d_fixexp, (void*) op->d_view[0], (void*) op->d_view[1], (void*) op->d_view[2]})
         if (p)
             (void) hipFree(p);
@@ -516,6 +577,10 @@ static int apply(ehp_op* op, const void* in, void* out, void* stream, bool adjoi
                                           op->batch, op->slot_bound, op->d_maxbits, op->d_fixexp, x, y, s);
         else if (op->kind == KIND_SIDDON)
             e = adjoint ? ehp::siddon_adjoint<float>(op->d_geo, dm, x, y, s) : ehp::siddon_forward<float>(op->d_geo, dm, x, y, s);
+        else if (!adjoint && op->tiled)
+            e = ehp::voxel_forward_tiled(op->d_geo, dm, (const float*) op->d_lut, op->n_lut, op->support,
+                                         op->d_pose_axes, op->d_pose_margin, op->tile, op->lutscale, op->slot_bound,
+                                         op->d_maxbits, op->d_fixexp, x, y, s);
         else
             e = adjoint ? ehp::voxel_adjoint<float>(op->d_geo, dm, (const float*) op->d_lut, op->n_lut, op->support, x, y, s)
                         : ehp::voxel_forward<float>(op->d_geo, dm, (const float*) op->d_lut, op->n_lut, op->support, x, y, s);
@@ -545,16 +610,19 @@ int ehp_adjoint(ehp_op* op, const void* d_sino, void* d_vol, void* stream)
     return apply(op, d_sino, d_vol, stream, true);
 }
 
-int ehp_adjoint_tile(ehp_op* op, int la, int tb, int tc, int* la_out, int* tb_out, int* tc_out)
+int ehp_tile(ehp_op* op, int la, int tb, int tc, int* la_out, int* tb_out, int* tc_out)
 {
     if (!op)
         return fail(EHP_ERR_ARGUMENT, "operator is NULL");
     if (!op->tiled)
-        return fail(EHP_ERR_ARGUMENT, "operator does not use a tiled back projection");
+        return fail(EHP_ERR_ARGUMENT, "operator does not use a tiled kernel");
     if (la > 0) {
-        if (tb <= 0 || (op->dims.dim == 3 && tc <= 0))
+        const bool vox = op->kind == KIND_VOXEL;
+        if (tb <= 0 || (op->dims.dim == 3 && !vox && tc <= 0))
             return fail(EHP_ERR_ARGUMENT, "tile extents must be positive");
         ehp::TileCfg t{la, tb, op->dims.dim == 3 ? tc : 1};
+        if (vox)
+            t = ehp::TileCfg{la, op->dims.dim == 3 ? tb : 1, 1};
         int max_lds = 65536;
         if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, op->device) != hipSuccess)
             max_lds = 65536;

@@ -205,89 +205,6 @@ namespace ehp
     // ---------------------------------------------------------------------------------
     // Voxel-driven blob / B-spline projector.
 
-    template <class T>
-    __device__ inline T lut_eval(const T* __restrict__ lut, int n, float support, float distance)
-    {
-        const float index = (distance / support) * (float) (n - 1);
-        if (index < 0.0f || index > (float) (n - 1))
-            return T(0);
-        const int a = (int) floorf(index);
-        const int b = (int) ceilf(index);
-        const T fa = lut[a];
-        const T fb = lut[b];
-        const T t = (T) (index - (float) a);
-        return t * fb + (T(1) - t) * fa;
-    }
-
-    // Footprint of voxel (x, y, z) in one pose (VoxelComputation.h:40-107, full rectangle).
-    struct Footprint {
-        float center[2];
-        float scaling;
-        int lo[2];
-        int hi[2];
-    };
-
-    template <int D>
-    __device__ inline void footprint(const PoseGeom& g, const Dims& dims, int x, int y, int z, float radius,
-                                     Footprint& fp)
-    {
-        float hv[4];
-        hv[0] = (float) x + 0.5f;
-        hv[1] = (float) y + 0.5f;
-        hv[2] = (float) z + 0.5f;
-        hv[D] = 1.0f;
-        float pc[3], ce[3];
-#pragma unroll
-        for (int i = 0; i < D; ++i) {
-            float acc = g.P[i] * hv[0];
-            float acc2 = g.ext[i] * hv[0];
-#pragma unroll
-            for (int j = 1; j <= D; ++j) {
-                acc = acc + g.P[i + j * D] * hv[j];
-                acc2 = acc2 + g.ext[i + j * D] * hv[j];
-            }
-            pc[i] = acc;
-            ce[i] = acc2;
-        }
-#pragma unroll
-        for (int i = 0; i < D - 1; ++i)
-            fp.center[i] = pc[i] / pc[D - 1] - 0.5f;
-        float n2 = ce[0] * ce[0];
-#pragma unroll
-        for (int i = 1; i < D; ++i)
-            n2 = n2 + ce[i] * ce[i];
-        fp.scaling = g.sdd / sqrtf(n2);
-        const float rod = fp.scaling * radius;
-#pragma unroll
-        for (int i = 0; i < D - 1; ++i) {
-            const int l = (int) ceilf(fp.center[i] - rod);
-            const int h = (int) floorf(fp.center[i] + rod);
-            fp.lo[i] = l < 0 ? 0 : l;
-            fp.hi[i] = h > dims.det[i] - 1 ? dims.det[i] - 1 : h;
-        }
-        if (D == 2) {
-            fp.lo[1] = 0;
-            fp.hi[1] = 0;
-        }
-    }
-
-    template <int D, class T, class Fn>
-    __device__ inline void visit(const Footprint& fp, const Dims& dims, Fn fn)
-    {
-        for (int j = fp.lo[1]; j <= fp.hi[1]; ++j)
-            for (int i = fp.lo[0]; i <= fp.hi[0]; ++i) {
-                const float d0 = (fp.center[0] - (float) i) / fp.scaling;
-                float dist;
-                if (D == 2) {
-                    dist = sqrtf(d0 * d0);
-                } else {
-                    const float d1 = (fp.center[1] - (float) j) / fp.scaling;
-                    dist = sqrtf(d0 * d0 + d1 * d1);
-                }
-                fn((int64_t) j * dims.det[0] + i, dist);
-            }
-    }
-
     // Adjoint: one thread per voxel gathers over all poses (no atomics).
     template <int D, class T>
     __global__ void __launch_bounds__(256)

@@ -4,6 +4,7 @@
 #include "ehp_device.hpp"
 #include "ehp_joseph_tile.hpp"
 #include "ehp_siddon.hpp"
+#include "ehp_voxel.hpp"
 
 struct ehp_op;
 
@@ -33,6 +34,12 @@ namespace ehp
                                     int axes_present, const TileCfg& tc, int batch, double slot_bound,
                                     unsigned int* d_maxbits, int* d_fixexp, const float* sino, float* vol,
                                     hipStream_t s);
+    size_t voxel_tile_lds_bytes(const TileCfg& tc);
+    int voxel_max_lut();
+    hipError_t voxel_forward_tiled(const PoseGeom* geo, const Dims& dims, const float* lut, int n_lut, float support,
+                                   const int* pose_axis, const float* pose_margin, const TileCfg& tc, float lutscale,
+                                   double slot_bound, unsigned int* d_maxbits, int* d_fixexp, const float* vol,
+                                   float* sino, hipStream_t s);
     hipError_t joseph_view_copy(const Dims& dims, const PadLayout& L, const float* vol, float* view, hipStream_t s);
     hipError_t joseph_forward_planes(const PoseGeom* geo, const Dims& dims, const PadLayout* layouts,
                                      float* const* views, int axes_present, const float* vol, float* sino,

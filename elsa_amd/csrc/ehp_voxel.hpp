@@ -1,0 +1,213 @@
+// ehp_voxel.hpp -- footprint arithmetic of the voxel-driven projectors, shared by the GPU
+// kernels and the CPU checks.
+//
+// elsa/projectors/VoxelComputation.h:40-107 (projectAndScaleVoxelOnDetector,
+// visitDetector): the voxel centre projected to the detector (minus 0.5), the scaling
+// sdd / |ext x|, and the detector pixels within radius * scaling of the centre -- the full
+// rectangle [ceil(c - r s), floor(c + r s)] (elsa's CPU loop visits it correctly only for
+// square footprints, VoxelComputation.h:90-105) -- each weighted with the projected basis
+// function at the distance |c - pixel| / scaling, read from a table by Luts.hpp:60-78.
+#pragma once
+
+#include "ehp_joseph_tile.hpp"
+
+namespace ehp
+{
+    template <class T>
+    EHP_HD inline T lut_eval(const T* __restrict__ lut, int n, float support, float distance)
+    {
+        const float index = (distance / support) * (float) (n - 1);
+        if (index < 0.0f || index > (float) (n - 1))
+            return T(0);
+        const int a = (int) floorf(index);
+        const int b = (int) ceilf(index);
+        const T fa = lut[a];
+        const T fb = lut[b];
+        const T t = (T) (index - (float) a);
+        return t * fb + (T(1) - t) * fa;
+    }
+
+    // Footprint of voxel (x, y, z) in one pose (VoxelComputation.h:40-107, full rectangle).
+    struct Footprint {
+        float center[2];
+        float scaling;
+        int lo[2];
+        int hi[2];
+    };
+
+    template <int D>
+    EHP_HD inline void footprint(const PoseGeom& g, const Dims& dims, int x, int y, int z, float radius,
+                                     Footprint& fp)
+    {
+        float hv[4];
+        hv[0] = (float) x + 0.5f;
+        hv[1] = (float) y + 0.5f;
+        hv[2] = (float) z + 0.5f;
+        hv[D] = 1.0f;
+        float pc[3], ce[3];
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            float acc = g.P[i] * hv[0];
+            float acc2 = g.ext[i] * hv[0];
+#pragma unroll
+            for (int j = 1; j <= D; ++j) {
+                acc = acc + g.P[i + j * D] * hv[j];
+                acc2 = acc2 + g.ext[i + j * D] * hv[j];
+            }
+            pc[i] = acc;
+            ce[i] = acc2;
+        }
+#pragma unroll
+        for (int i = 0; i < D - 1; ++i)
+            fp.center[i] = pc[i] / pc[D - 1] - 0.5f;
+        float n2 = ce[0] * ce[0];
+#pragma unroll
+        for (int i = 1; i < D; ++i)
+            n2 = n2 + ce[i] * ce[i];
+        fp.scaling = g.sdd / sqrtf(n2);
+        const float rod = fp.scaling * radius;
+#pragma unroll
+        for (int i = 0; i < D - 1; ++i) {
+            const int l = (int) ceilf(fp.center[i] - rod);
+            const int h = (int) floorf(fp.center[i] + rod);
+            fp.lo[i] = l < 0 ? 0 : l;
+            fp.hi[i] = h > dims.det[i] - 1 ? dims.det[i] - 1 : h;
+        }
+        if (D == 2) {
+            fp.lo[1] = 0;
+            fp.hi[1] = 0;
+        }
+    }
+
+    template <int D, class T, class Fn>
+    EHP_HD inline void visit(const Footprint& fp, const Dims& dims, Fn fn)
+    {
+        for (int j = fp.lo[1]; j <= fp.hi[1]; ++j)
+            for (int i = fp.lo[0]; i <= fp.hi[0]; ++i) {
+                const float d0 = (fp.center[0] - (float) i) / fp.scaling;
+                float dist;
+                if (D == 2) {
+                    dist = sqrtf(d0 * d0);
+                } else {
+                    const float d1 = (fp.center[1] - (float) j) / fp.scaling;
+                    dist = sqrtf(d0 * d0 + d1 * d1);
+                }
+                fn((int64_t) j * dims.det[0] + i, dist);
+            }
+    }
+
+
+    // Largest footprint scaling of a pose over the volume: sdd over the smallest distance from
+    // the source to the volume box (scaling is sdd / |ext x| and |ext x| is the distance from
+    // the source to the voxel centre x).
+    inline double voxel_max_scaling(const PoseGeom& g, const Dims& dims)
+    {
+        double dist2 = 0.0;
+        for (int i = 0; i < dims.dim; ++i) {
+            const double c = g.C[i];
+            const double e = c < 0.0 ? -c : (c > dims.vol[i] ? c - dims.vol[i] : 0.0);
+            dist2 += e * e;
+        }
+        return dist2 > 0.0 ? (double) g.sdd / sqrt(dist2) : -1.0;
+    }
+
+    // Driving axis of a pose for the tiled forward: the largest component of the central ray.
+    template <int D>
+    EHP_HD inline int pose_axis(const PoseGeom& g, const Dims& dims)
+    {
+        float o[3], d[3];
+        make_ray<D>(g, dims.det[0] / 2, D == 3 ? dims.det[1] / 2 : 0, o, d);
+        int a = 0;
+        float m = fabsf(d[0]);
+        for (int i = 1; i < D; ++i)
+            if (fabsf(d[i]) > m) {
+                m = fabsf(d[i]);
+                a = i;
+            }
+        return a;
+    }
+
+    // Voxels of plane `pa` (along axis A) whose footprint can reach the detector rectangle
+    // [u0, u1] x [v0, v1]: the footprint centre c = (P x)_hom - 0.5 of such a voxel lies within
+    // margin m of the rectangle, i.e. P x lies in [u0 + 0.5 - m, u1 + 0.5 + m] x ..., the
+    // frustum through the source spanned by the rays of those four corners. Its cross-section
+    // with the plane of voxel centres pa + 0.5 is the quadrilateral of the corner rays' hits;
+    // the voxel ranges cover its bounding rectangle with one voxel to spare. Returns false
+    // when no voxel qualifies; falls back to the whole plane when a corner ray does not reach
+    // the plane in front of the source.
+    template <int D>
+    EHP_HD inline bool frustum_plane_range(const PoseGeom& g, const Dims& dims, int A, int pa, const float* cdir,
+                                           int ncorners, int& blo, int& bhi, int& clo, int& chi)
+    {
+        const int b1 = (D == 2) ? 1 - A : (A == 0 ? 1 : 0);
+        const int b2 = (D == 2) ? -1 : (A == 2 ? 1 : 2);
+        const float X = (float) pa + 0.5f;
+        const float CA = pick<D>(g.C, A);
+        float bmin = 3.0e38f, bmax = -3.0e38f, cmin = 3.0e38f, cmax = -3.0e38f;
+        bool ok = true;
+        for (int k = 0; k < ncorners; ++k) {
+            const float* dk = cdir + 3 * k;
+            const float da = pick<D>(dk, A);
+            const float t = (X - CA) / da;
+            ok = ok && (da != 0.0f) && (t > 0.0f) && (t < 3.0e38f);
+            const float pb = pick<D>(g.C, b1) + t * pick<D>(dk, b1);
+            bmin = pb < bmin ? pb : bmin;
+            bmax = pb > bmax ? pb : bmax;
+            if (D == 3) {
+                const float pc = pick<D>(g.C, b2) + t * pick<D>(dk, b2);
+                cmin = pc < cmin ? pc : cmin;
+                cmax = pc > cmax ? pc : cmax;
+            }
+        }
+        const int nb = dims.vol[b1], nc = D == 3 ? dims.vol[b2] : 1;
+        if (!ok) {
+            blo = 0;
+            bhi = nb - 1;
+            clo = 0;
+            chi = nc - 1;
+            return true;
+        }
+        // voxel centres b + 0.5 in [bmin, bmax], one voxel to spare
+        bmin = bmin < -2.0f ? -2.0f : (bmin > (float) nb + 2.0f ? (float) nb + 2.0f : bmin);
+        bmax = bmax < -2.0f ? -2.0f : (bmax > (float) nb + 2.0f ? (float) nb + 2.0f : bmax);
+        blo = (int) floorf(bmin - 0.5f) - 1;
+        bhi = (int) ceilf(bmax - 0.5f) + 1;
+        blo = blo < 0 ? 0 : blo;
+        bhi = bhi > nb - 1 ? nb - 1 : bhi;
+        if (D == 3) {
+            cmin = cmin < -2.0f ? -2.0f : (cmin > (float) nc + 2.0f ? (float) nc + 2.0f : cmin);
+            cmax = cmax < -2.0f ? -2.0f : (cmax > (float) nc + 2.0f ? (float) nc + 2.0f : cmax);
+            clo = (int) floorf(cmin - 0.5f) - 1;
+            chi = (int) ceilf(cmax - 0.5f) + 1;
+            clo = clo < 0 ? 0 : clo;
+            chi = chi > nc - 1 ? nc - 1 : chi;
+        } else {
+            clo = chi = 0;
+        }
+        return blo <= bhi && clo <= chi;
+    }
+
+    // Directions of the corner rays of the detector rectangle [u0, u1] x [v0, v1] widened by
+    // margin m (unnormalised: Pinv applied to the homogeneous detector point).
+    template <int D>
+    EHP_HD inline int frustum_corners(const PoseGeom& g, int u0, int u1, int v0, int v1, float m, float* cdir)
+    {
+        const float us[2] = {(float) u0 + 0.5f - m, (float) u1 + 0.5f + m};
+        const float vs[2] = {(float) v0 + 0.5f - m, (float) v1 + 0.5f + m};
+        int n = 0;
+        for (int jv = 0; jv < (D == 3 ? 2 : 1); ++jv)
+            for (int ju = 0; ju < 2; ++ju) {
+                const float h[3] = {us[ju], D == 3 ? vs[jv] : 1.0f, 1.0f};
+                for (int i = 0; i < 3; ++i)
+                    cdir[3 * n + i] = 0.0f;
+                for (int i = 0; i < D; ++i) {
+                    float acc = 0.0f;
+                    for (int j = 0; j < D; ++j)
+                        acc += g.Pinv[i + j * (D + 1)] * h[j];
+                    cdir[3 * n + i] = acc;
+                }
+                ++n;
+            }
+        return n;
+    }
+} // namespace ehp

@@ -1,0 +1,159 @@
+// ehp_voxel_tiled.hip -- voxel-driven forward projection with LDS-privatised detector tiles
+// (FP32).
+//
+// The forward of elsa's voxel projectors (VoxelComputation.h:110-152: every voxel adds
+// x * basis(distance) to the detector pixels of its footprint) without global atomics. One
+// workgroup owns one detector tile of one pose and accumulates it in LDS: it walks the planes
+// of the volume along the pose's driving axis, takes on each plane the voxels whose footprint
+// can reach the tile (frustum_plane_range), computes each one's footprint exactly as the
+// one-thread-per-voxel kernels do and adds the taps that fall in the tile. Every pixel then
+// belongs to one tile and is written once, so the output needs no clearing. Tiles are ordered
+// tile first, pose second: the poses of a tile, which read nearly the same column of the
+// volume, run together on one XCD.
+//
+// Sums are 64-bit fixed point as in the tiled back projections: one exponent per call from
+// max|x| and a bound on what one pixel can receive, each voxel's contributions rounded to
+// int32 in the voxel's own scale.
+#include "ehp_kernels.hpp"
+
+namespace ehp
+{
+    namespace
+    {
+        constexpr int kThreads = 256;
+        constexpr int kMaxLut = 1024;
+    } // namespace
+
+    template <int D>
+    __global__ void __launch_bounds__(kThreads)
+        voxel_fwd_tiled_kernel(const PoseGeom* __restrict__ geo, const Dims dims, const float* __restrict__ lut,
+                               const int n_lut, const float support, const int* __restrict__ pose_axis,
+                               const float* __restrict__ pose_margin, const int TU, const int TV, const int ntu,
+                               const int per_xcd, const int64_t total, const float lutscale,
+                               const int* __restrict__ fixexp, const float* __restrict__ vol,
+                               float* __restrict__ sino)
+    {
+        extern __shared__ long long lds[];
+        __shared__ float s_lut[kMaxLut];
+        const int64_t id = (int64_t) (blockIdx.x % 8u) * per_xcd + (blockIdx.x / 8u);
+        if (id >= total)
+            return;
+        const int tid = threadIdx.x;
+        // id = tile * nposes + pose
+        const int pose = (int) (id % dims.nposes);
+        const int tile = (int) (id / dims.nposes);
+        const int tu = tile % ntu, tv = tile / ntu;
+        const int u0 = tu * TU, v0 = tv * TV;
+        const int u1 = (u0 + TU < dims.det[0] ? u0 + TU : dims.det[0]) - 1;
+        const int v1 = (v0 + TV < dims.det[1] ? v0 + TV : dims.det[1]) - 1;
+        long long* acc = lds;
+        for (int i = tid; i < TU * TV; i += kThreads)
+            acc[i] = 0;
+        for (int i = tid; i < n_lut; i += kThreads)
+            s_lut[i] = lut[i];
+        const int eg = fixexp[0];
+        const float inv_scale = 1.0f / lutscale; // a power of two: exact
+        const PoseGeom& g = geo[pose];
+        const int A = pose_axis[pose];
+        float cdir[12];
+        const int ncorner = frustum_corners<D>(g, u0, u1, v0, v1, pose_margin[pose], cdir);
+        __syncthreads();
+
+        const int b1 = (D == 2) ? 1 - A : (A == 0 ? 1 : 0);
+        const int b2 = (D == 2) ? 2 : (A == 2 ? 1 : 2);
+        for (int pa = 0; eg != kNonFinite && pa < dims.vol[A]; ++pa) {
+            int blo, bhi, clo, chi;
+            if (!frustum_plane_range<D>(g, dims, A, pa, cdir, ncorner, blo, bhi, clo, chi))
+                continue;
+            const int nb = bhi - blo + 1;
+            const int cnt = nb * (chi - clo + 1);
+            const float rnb = 1.0f / (float) nb;
+            for (int k = tid; k < cnt; k += kThreads) {
+                // k / nb through the reciprocal: exact, as k + 0.5 is at least 0.5 away from a
+                // multiple of nb and k < 2^22
+                const int row = (int) (((float) k + 0.5f) * rnb);
+                int c[3] = {0, 0, 0};
+                c[A] = pa;
+                c[b1] = blo + (k - row * nb);
+                if (D == 3)
+                    c[b2] = clo + row;
+                const int64_t lin = c[0] + c[1] * dims.strides[1] + (int64_t) c[2] * dims.strides[2];
+                const float xv = vol[lin];
+                if (xv == 0.0f)
+                    continue;
+                Footprint fp;
+                footprint<D>(g, dims, c[0], c[1], c[2], support, fp);
+                const int lo0 = fp.lo[0] > u0 ? fp.lo[0] : u0, hi0 = fp.hi[0] < u1 ? fp.hi[0] : u1;
+                const int lo1 = fp.lo[1] > v0 ? fp.lo[1] : v0, hi1 = fp.hi[1] < v1 ? fp.hi[1] : v1;
+                if (lo0 > hi0 || lo1 > hi1)
+                    continue;
+                float ys;
+                int sh;
+                ray_scale(xv * lutscale, eg, ys, sh);
+                for (int j = lo1; j <= hi1; ++j)
+                    for (int i = lo0; i <= hi0; ++i) {
+                        // the distance as visit() forms it
+                        const float d0 = (fp.center[0] - (float) i) / fp.scaling;
+                        float dist;
+                        if (D == 2) {
+                            dist = sqrtf(d0 * d0);
+                        } else {
+                            const float d1 = (fp.center[1] - (float) j) / fp.scaling;
+                            dist = sqrtf(d0 * d0 + d1 * d1);
+                        }
+                        const float w = lut_eval<float>(s_lut, n_lut, support, dist);
+                        if (w != 0.0f)
+                            atomicAdd(reinterpret_cast<unsigned long long*>(acc + (j - v0) * TU + (i - u0)),
+                                      fixed_contrib(ys * (w * inv_scale), sh));
+                    }
+            }
+        }
+        __syncthreads();
+        const double q = eg == kNonFinite ? __longlong_as_double(0x7ff8000000000000ll) : ldexp(1.0, eg - 62);
+        float* sp = sino + (int64_t) pose * dims.pix_per_pose;
+        for (int i = tid; i < TU * TV; i += kThreads) {
+            const int u = u0 + i % TU, v = v0 + i / TU;
+            if (u <= u1 && v <= v1)
+                sp[(int64_t) v * dims.det[0] + u] = (float) ((double) acc[i] * q);
+        }
+    }
+
+    // dynamic LDS (the tile's sums) plus the table
+    size_t voxel_tile_lds_bytes(const TileCfg& tc)
+    {
+        return (size_t) tc.LA * tc.TB * sizeof(long long) + kMaxLut * sizeof(float);
+    }
+
+    int voxel_max_lut()
+    {
+        return kMaxLut;
+    }
+
+    // tc.LA x tc.TB: tile width x height in pixels (height 1 in 2D)
+    hipError_t voxel_forward_tiled(const PoseGeom* geo, const Dims& dims, const float* lut, int n_lut, float support,
+                                   const int* pose_axis, const float* pose_margin, const TileCfg& tc, float lutscale,
+                                   double slot_bound, unsigned int* d_maxbits, int* d_fixexp, const float* vol,
+                                   float* sino, hipStream_t s)
+    {
+        if (n_lut < 2 || n_lut > kMaxLut)
+            return hipErrorInvalidValue;
+        hipError_t e = fixed_point_exponent_launch(vol, dims.nvox, slot_bound, d_maxbits, d_fixexp, s);
+        if (e != hipSuccess)
+            return e;
+        const int TU = tc.LA, TV = dims.dim == 3 ? tc.TB : 1;
+        const int ntu = (dims.det[0] + TU - 1) / TU, ntv = (dims.det[1] + TV - 1) / TV;
+        const int64_t total = (int64_t) ntu * ntv * dims.nposes;
+        const int64_t per_xcd = (total + 7) / 8;
+        if (per_xcd * 8 > (int64_t) 0x7fffffff)
+            return hipErrorInvalidValue;
+        const dim3 grid((unsigned) (per_xcd * 8));
+        const size_t lds = (size_t) TU * TV * sizeof(long long);
+        if (dims.dim == 2)
+            hipLaunchKernelGGL((voxel_fwd_tiled_kernel<2>), grid, dim3(kThreads), lds, s, geo, dims, lut, n_lut, support,
+                               pose_axis, pose_margin, TU, TV, ntu, (int) per_xcd, total, lutscale, d_fixexp, vol, sino);
+        else
+            hipLaunchKernelGGL((voxel_fwd_tiled_kernel<3>), grid, dim3(kThreads), lds, s, geo, dims, lut, n_lut, support,
+                               pose_axis, pose_margin, TU, TV, ntu, (int) per_xcd, total, lutscale, d_fixexp, vol, sino);
+        return hipGetLastError();
+    }
+} // namespace ehp

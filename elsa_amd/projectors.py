@@ -107,10 +107,10 @@ class _HipProjector(LinearOperator):
         return self._op
 
     def tile(self, la: int = 0, tb: int = 0, tc: int = 0):
-        """Set (la > 0) and return the (LA, TB, TC) tile of the LDS-tiled back projection
-        (Joseph and Siddon, FP32)."""
+        """Set (la > 0) and return the tile of the LDS-tiled kernel (FP32): (LA, TB, TC) of the
+        Joseph / Siddon back projection, (width, height, -) in pixels of the voxel forward."""
         o = [ctypes.c_int(), ctypes.c_int(), ctypes.c_int()]
-        _lib.check(self._L.ehp_adjoint_tile(self._op, la, tb, tc, *(ctypes.byref(x) for x in o)), "tile")
+        _lib.check(self._L.ehp_tile(self._op, la, tb, tc, *(ctypes.byref(x) for x in o)), "tile")
         return tuple(x.value for x in o)
 
     def sample_count(self) -> int:
@@ -222,27 +222,41 @@ def bspline_lut(dim, order=2, n=100, dtype=np.float32):
 
 
 class _VoxelProjector(_HipProjector):
+    """``kernels="fast"`` (FP32) runs the forward projection over LDS-privatised detector tiles
+    with 64-bit fixed-point sums; ``"simple"`` runs one thread per voxel and pose with float
+    atomics. The back projection is a gather in both. FP64 always uses the simple kernels."""
+
+    def _set_kernels(self, kernels):
+        if kernels not in ("fast", "simple"):
+            raise ValueError("kernels must be 'fast' or 'simple'")
+        self.kernels = kernels
+
     def _create(self, geo):
+        flags = _lib.EHP_SIMPLE_KERNELS if self.kernels == "simple" else 0
         _lib.check(self._L.ehp_voxel_create(geo, _DT[self._dtype], self.lut.ctypes.data_as(ctypes.c_void_p),
-                                            int(self.lut.shape[0]), float(self.support), ctypes.byref(self._op)),
+                                            int(self.lut.shape[0]), float(self.support), flags,
+                                            ctypes.byref(self._op)),
                    type(self).__name__)
 
 
 class BlobVoxelProjectorCUDA(_VoxelProjector):
     """Voxel-driven projector with Kaiser-Bessel blobs (elsa BlobVoxelProjectorCUDA)."""
 
-    def __init__(self, domainDescriptor, rangeDescriptor, radius=2.0, alpha=10.83, order=2, dtype=np.float32):
+    def __init__(self, domainDescriptor, rangeDescriptor, radius=2.0, alpha=10.83, order=2, dtype=np.float32,
+                 kernels: str = "fast"):
         self.radius, self.alpha, self.order = radius, alpha, int(order)
         self.lut, self.support = blob_lut(radius, alpha, order, 100, dtype)
+        self._set_kernels(kernels)
         super().__init__(domainDescriptor, rangeDescriptor, dtype)
 
 
 class BSplineVoxelProjectorCUDA(_VoxelProjector):
     """Voxel-driven projector with B-splines (elsa BSplineVoxelProjectorCUDA)."""
 
-    def __init__(self, domainDescriptor, rangeDescriptor, order=2, dtype=np.float32):
+    def __init__(self, domainDescriptor, rangeDescriptor, order=2, dtype=np.float32, kernels: str = "fast"):
         self.order = int(order)
         self.lut, self.support = bspline_lut(domainDescriptor.getNumberOfDimensions(), order, 100, dtype)
+        self._set_kernels(kernels)
         super().__init__(domainDescriptor, rangeDescriptor, dtype)
 
 

@@ -50,9 +50,9 @@ enum ehp_status {
 
 /* flags of ehp_joseph_create */
 enum ehp_joseph_flags {
-    EHP_SIMPLE_KERNELS = 1 /* Joseph and Siddon: one thread per ray in both directions
-                              (atomic back projection); the reference point for the tiled
-                              kernels */
+    EHP_SIMPLE_KERNELS = 1 /* one thread per ray (Joseph, Siddon) or per voxel and pose (voxel
+                              projectors) in both directions, atomic scatters; the reference
+                              point for the tiled kernels */
 };
 
 enum ehp_dtype {
@@ -92,8 +92,9 @@ int ehp_siddon_create(const ehp_geometry* g, int dtype, int flags, ehp_op** out)
  * n_lut values over [0, support] (elsa: BlobVoxelProjectorCUDA / BSplineVoxelProjectorCUDA,
  * VoxelProjectorCUDA.h:39-43, :82-84; the table is ProjectedBlob::get_lut() or
  * ProjectedBSpline::get_lut(), Blobs.h:249-317, BSplines.h:132-208). lut is host memory of
- * the op's dtype. */
-int ehp_voxel_create(const ehp_geometry* g, int dtype, const void* lut, int n_lut, double support,
+ * the op's dtype. FP32 forward projections run on LDS-tiled detector tiles; flags: 0 or
+ * EHP_SIMPLE_KERNELS (one thread per voxel and pose, atomic splat). */
+int ehp_voxel_create(const ehp_geometry* g, int dtype, const void* lut, int n_lut, double support, int flags,
                      ehp_op** out);
 
 void ehp_destroy(ehp_op* op);
@@ -112,12 +113,13 @@ int ehp_op_sizes(const ehp_op* op, int64_t* n_vol, int64_t* n_sino);
  * number of footprint taps. Computed on the device at create time. */
 int ehp_sample_count(const ehp_op* op, uint64_t* samples);
 
-/* Tile shape of the LDS-tiled back projection (Joseph and Siddon, FP32): la planes along the
- * driving axis, tb x tc voxels across (tc ignored in 2D). No counterpart in elsa; it is the
- * knob the tile autotuner turns. Fails with EHP_ERR_ARGUMENT if the tile does not fit the
- * device's LDS or the operator does not use a tiled back projection. Queries only when out
- * pointers are given and la <= 0. */
-int ehp_adjoint_tile(ehp_op* op, int la, int tb, int tc, int* la_out, int* tb_out, int* tc_out);
+/* Tile shape of the operator's LDS-tiled kernel (FP32). Joseph and Siddon back projections:
+ * la planes along the driving axis, tb x tc voxels across (tc ignored in 2D). Voxel forward
+ * projection: la x tb detector pixels (tb ignored in 2D, tc ignored). No counterpart in elsa;
+ * it is the knob the tile autotuner turns. Fails with EHP_ERR_ARGUMENT if the tile does not
+ * fit the device's LDS or the operator has no tiled kernel. Queries only when out pointers
+ * are given and la <= 0. */
+int ehp_tile(ehp_op* op, int la, int tb, int tc, int* la_out, int* tb_out, int* tc_out);
 
 /* Per-ray Joseph traversal state as the kernels compute it (for parity checks): driving
  * axis and number of samples of every ray, host memory of n_sino entries each. */

@@ -72,7 +72,11 @@ def test_geometry_create_rejects_bad_arguments(lib):
     op = ctypes.c_void_p()
     assert lib.ehp_joseph_create(g, 7, 0, ctypes.byref(op)) == _lib.EHP_ERR_ARGUMENT
     assert lib.ehp_joseph_create(g, 0, 3, ctypes.byref(op)) == _lib.EHP_ERR_ARGUMENT
-    assert lib.ehp_voxel_create(g, 0, None, 100, 2.0, ctypes.byref(op)) == _lib.EHP_ERR_ARGUMENT
+    assert lib.ehp_voxel_create(g, 0, None, 100, 2.0, 0, ctypes.byref(op)) == _lib.EHP_ERR_ARGUMENT
+    assert lib.ehp_siddon_create(g, 0, 2, ctypes.byref(op)) == _lib.EHP_ERR_ARGUMENT
+    lut = np.ones(100, np.float32)
+    assert lib.ehp_voxel_create(g, 0, lut.ctypes.data_as(ctypes.c_void_p), 100, 2.0, 4,
+                                ctypes.byref(op)) == _lib.EHP_ERR_ARGUMENT
     lib.ehp_geometry_destroy(g)
     assert lib.ehp_forward(None, None, None, None) == _lib.EHP_ERR_ARGUMENT
 
