@@ -58,7 +58,8 @@ def parse():
                    help="Joseph kernels: plane-major forward + LDS-tiled back projection (default) or one "
                         "thread per ray")
     p.add_argument("--tile", default="", help="LA,TB,TC of the tiled Joseph / Siddon back projection, or W,H "
-                                              "pixels of the tiled voxel forward (default: built-in)")
+                                              "pixels of the tiled voxel forward (default: built-in), or "
+                                              "'auto' to time a set of candidates first (untimed)")
     p.add_argument("--projector", default="joseph", choices=["joseph", "siddon", "blob", "bspline"],
                    help="projector to time (default: Joseph, the headline metric)")
     p.add_argument("--cg-iters", type=int, default=20,
@@ -161,12 +162,19 @@ def main():
     else:
         op = E.BSplineVoxelProjectorCUDA(vd, dd, kernels=a.kernels)
     tile = None
-    if a.kernels == "fast":
-        tile = op.tile(*[int(v) for v in a.tile.split(",")]) if a.tile else op.tile()
     S = op.sample_count()
     R = int(np.prod(wl["det"])) * wl["views"]
     x_np = modified_shepp_logan(wl["vol"][::-1])
     x = torch.from_numpy(x_np).to(dev)
+    if a.kernels == "fast":
+        if a.tile == "auto":
+            # tune on the workload's own data: the phantom for the voxel forward, its sinogram
+            # for the back projections
+            voxel = a.projector in ("blob", "bspline")
+            tile, tile_times = E.autotune(op, x=x if voxel else op.apply(x))
+            print(json.dumps({"autotune": {str(k): round(v, 3) for k, v in tile_times.items()}}), file=sys.stderr)
+        else:
+            tile = op.tile(*[int(v) for v in a.tile.split(",")]) if a.tile else op.tile()
     sino = torch.empty(dd.shape, dtype=torch.float32, device=dev)
     back = torch.empty(vd.shape, dtype=torch.float32, device=dev)
 

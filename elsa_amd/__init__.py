@@ -12,6 +12,7 @@ from .projectors import (BlobVoxelProjectorCUDA, BlobVoxelProjectorCUDAd, BlobVo
                          SiddonsMethodCUDA, SiddonsMethodCUDAd, SiddonsMethodCUDAf, blob_lut, bspline_lut)
 from .solvers import APGD, CGLS, ProximalIdentity, ProximalL1, powerIterations
 from . import autograd
+from .autotune import autotune
 
 __all__ = [
     "CircleTrajectoryGenerator", "DataDescriptor", "DetectorDescriptor", "Geometry", "PlanarDetectorDescriptor",
@@ -19,5 +20,5 @@ __all__ = [
     "SiddonsMethodCUDA", "SiddonsMethodCUDAf", "SiddonsMethodCUDAd", "BlobVoxelProjectorCUDA",
     "BlobVoxelProjectorCUDAf", "BlobVoxelProjectorCUDAd", "BSplineVoxelProjectorCUDA", "BSplineVoxelProjectorCUDAf",
     "BSplineVoxelProjectorCUDAd", "blob_lut", "bspline_lut", "CGLS", "APGD", "ProximalL1", "ProximalIdentity",
-    "powerIterations",
+    "powerIterations", "autotune",
 ]
