@@ -9,16 +9,24 @@ from .descriptors import (CircleTrajectoryGenerator, DataDescriptor, DetectorDes
 from .projectors import (BlobVoxelProjectorCUDA, BlobVoxelProjectorCUDAd, BlobVoxelProjectorCUDAf,
                          BSplineVoxelProjectorCUDA, BSplineVoxelProjectorCUDAd, BSplineVoxelProjectorCUDAf,
                          JosephsMethodCUDA, JosephsMethodCUDAd, JosephsMethodCUDAf, LinearOperator,
-                         SiddonsMethodCUDA, SiddonsMethodCUDAd, SiddonsMethodCUDAf, blob_lut, bspline_lut)
+                         PhaseContrastBlobVoxelProjectorCUDA, PhaseContrastBlobVoxelProjectorCUDAd,
+                         PhaseContrastBlobVoxelProjectorCUDAf, PhaseContrastBSplineVoxelProjectorCUDA,
+                         PhaseContrastBSplineVoxelProjectorCUDAd, PhaseContrastBSplineVoxelProjectorCUDAf,
+                         SiddonsMethodCUDA, SiddonsMethodCUDAd, SiddonsMethodCUDAf, blob_derivative_lut, blob_lut,
+                         bspline_derivative_lut, bspline_lut)
 from .solvers import APGD, CGLS, ProximalIdentity, ProximalL1, powerIterations
 from . import autograd
 from .autotune import autotune
+from .batch import ProjectorBatch
 
 __all__ = [
     "CircleTrajectoryGenerator", "DataDescriptor", "DetectorDescriptor", "Geometry", "PlanarDetectorDescriptor",
     "VolumeDescriptor", "LinearOperator", "JosephsMethodCUDA", "JosephsMethodCUDAf", "JosephsMethodCUDAd",
     "SiddonsMethodCUDA", "SiddonsMethodCUDAf", "SiddonsMethodCUDAd", "BlobVoxelProjectorCUDA",
     "BlobVoxelProjectorCUDAf", "BlobVoxelProjectorCUDAd", "BSplineVoxelProjectorCUDA", "BSplineVoxelProjectorCUDAf",
-    "BSplineVoxelProjectorCUDAd", "blob_lut", "bspline_lut", "CGLS", "APGD", "ProximalL1", "ProximalIdentity",
-    "powerIterations", "autotune",
+    "BSplineVoxelProjectorCUDAd", "PhaseContrastBlobVoxelProjectorCUDA", "PhaseContrastBlobVoxelProjectorCUDAf",
+    "PhaseContrastBlobVoxelProjectorCUDAd", "PhaseContrastBSplineVoxelProjectorCUDA",
+    "PhaseContrastBSplineVoxelProjectorCUDAf", "PhaseContrastBSplineVoxelProjectorCUDAd", "blob_lut", "bspline_lut",
+    "blob_derivative_lut", "bspline_derivative_lut", "CGLS", "APGD", "ProximalL1", "ProximalIdentity",
+    "powerIterations", "autotune", "ProjectorBatch",
 ]

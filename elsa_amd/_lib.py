@@ -21,8 +21,12 @@ EXPORTS = (
     "ehp_joseph_create", "ehp_siddon_create", "ehp_voxel_create", "ehp_destroy", "ehp_forward",
     "ehp_adjoint", "ehp_op_sizes", "ehp_sample_count", "ehp_tile", "ehp_joseph_ray_info", "ehp_geometry_pose",
     "ehp_circle_trajectory", "ehp_blob_lut", "ehp_bspline_lut", "ehp_cgls", "ehp_apgd", "ehp_power_iterations",
+    "ehp_batch_create", "ehp_batch_destroy", "ehp_batch_layout", "ehp_batch_forward", "ehp_batch_adjoint",
+    "ehp_batch_cgls", "ehp_batch_apgd", "ehp_batch_power_iterations", "ehp_blob_derivative_lut",
+    "ehp_bspline_derivative_lut",
 )
 EHP_SOLVE_GRAPH = 1
+EHP_DIFFERENTIAL = 2
 
 _lib = None
 
@@ -92,12 +96,23 @@ def load():
     L.ehp_circle_trajectory.argtypes = [c_int, i64, i64p, c_f, c_f, c_f, i64p, i64p, fp, fp, fp, fp, fp]
     L.ehp_blob_lut.argtypes = [c_int, c_d, c_d, i64, c_int, vp, ctypes.POINTER(c_d)]
     L.ehp_bspline_lut.argtypes = [c_int, i64, i64, c_int, vp, ctypes.POINTER(c_d)]
+    L.ehp_blob_derivative_lut.argtypes = [c_int, c_d, c_d, i64, c_int, c_int, vp, ctypes.POINTER(c_d)]
+    L.ehp_bspline_derivative_lut.argtypes = [c_int, i64, i64, c_int, c_int, vp, ctypes.POINTER(c_d)]
     dp = ctypes.POINTER(c_d)
     L.ehp_cgls.argtypes = [vp, vp, vp, c_int, c_int, c_d, c_d, c_int, dp, ip, vp]
     L.ehp_apgd.argtypes = [vp, vp, vp, c_int, c_int, c_d, c_d, c_d, c_int, dp, ip, dp, vp]
     L.ehp_power_iterations.argtypes = [vp, c_int, dp, vp]
+    L.ehp_batch_create.argtypes = [ctypes.POINTER(vp), c_int, ctypes.POINTER(vp)]
+    L.ehp_batch_destroy.argtypes = [vp]
+    L.ehp_batch_destroy.restype = None
+    L.ehp_batch_layout.argtypes = [vp, i64p, i64p, ip]
+    L.ehp_batch_forward.argtypes = [vp, vp, vp, vp]
+    L.ehp_batch_adjoint.argtypes = [vp, vp, vp, vp]
+    L.ehp_batch_cgls.argtypes = [vp, vp, vp, c_int, c_int, c_d, c_d, c_int, dp, ip, vp]
+    L.ehp_batch_apgd.argtypes = [vp, vp, vp, c_int, c_int, dp, c_d, c_d, c_int, dp, ip, dp, vp]
+    L.ehp_batch_power_iterations.argtypes = [vp, c_int, dp, vp]
     for name in EXPORTS:
-        if name not in ("ehp_last_error", "ehp_version", "ehp_geometry_destroy", "ehp_destroy"):
+        if name not in ("ehp_last_error", "ehp_version", "ehp_geometry_destroy", "ehp_destroy", "ehp_batch_destroy"):
             getattr(L, name).restype = c_int
     _lib = L
     return L

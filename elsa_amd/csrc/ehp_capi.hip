@@ -6,6 +6,7 @@
 // (elsa/projectors_cuda/JosephsMethodCUDA.cpp:17-69, SiddonsMethodCUDA.cpp:17-30).
 #include "elsa_hip.h"
 #include "ehp_kernels.hpp"
+#include "ehp_solvers.hpp"
 
 #include <cmath>
 #include <cstdarg>
@@ -14,44 +15,7 @@
 #include <string>
 #include <vector>
 
-struct ehp_geometry {
-    int dim;
-    int64_t nposes;
-    int vol[3];
-    int det[2];
-    std::vector<ehp::PoseGeom> host;
-};
-
-enum op_kind { KIND_JOSEPH = 0, KIND_SIDDON = 1, KIND_VOXEL = 2 };
-
-struct ehp_op {
-    int kind;
-    int dtype;
-    int device;
-    ehp::Dims dims;
-    ehp::PoseGeom* d_geo = nullptr;
-    void* d_lut = nullptr;
-    int n_lut = 0;
-    float support = 0.0f;
-    uint64_t samples = 0;
-    // Joseph, FP32: plane-major forward and LDS-tiled back projection
-    bool planes = false;
-    ehp::PadLayout layouts[3]{};
-    float* d_view[3] = {nullptr, nullptr, nullptr}; // zero-padded plane-major volume views
-    bool tiled = false;
-    int* d_pose_axes = nullptr;
-    int axes_present = 0;
-    ehp::TileCfg tile{};
-    ehp::RayRec* d_table = nullptr;   // per-ray traversal state (Joseph)
-    ehp::SidRec* d_sidtable = nullptr; // per-ray traversal state (Siddon)
-    float* d_pose_margin = nullptr;    // voxel forward: frustum margin per pose (pixels)
-    float lutscale = 1.0f;             // voxel forward: power of two >= max |table|
-    unsigned int* d_maxbits = nullptr; // max|y| of the current back projection
-    int* d_fixexp = nullptr;           // fixed-point exponent of the current back projection
-    double slot_bound = 0.0;
-    int batch = ehp::kPoseBatch;
-    std::vector<ehp::PoseGeom> host_geo;
-};
+#include "ehp_op.hpp"
 
 namespace
 {
@@ -404,6 +368,10 @@ namespace
         float lutmax = 0.0f;
         for (int i = 0; i < op->n_lut; ++i)
             lutmax = std::fabs(host_lut[i]) > lutmax ? std::fabs(host_lut[i]) : lutmax;
+        // the 3D phase-contrast weight is the table times d_u, |d_u| <= |d| <= support where the
+        // table is not zero (with a little room for the rounding of |d|)
+        if (op->differential && dm.dim == 3)
+            lutmax = (float) ((double) lutmax * (op->support > 1.0f ? op->support : 1.0f) * 1.001);
         if (!(lutmax < 1.0e30f))
             return EHP_OK;
         float ls = 1.0f;
@@ -474,7 +442,7 @@ int ehp_siddon_create(const ehp_geometry* g, int dtype, int flags, ehp_op** out)
 int ehp_voxel_create(const ehp_geometry* g, int dtype, const void* lut, int n_lut, double support, int flags,
                      ehp_op** out)
 {
-    if (flags & ~EHP_SIMPLE_KERNELS)
+    if (flags & ~(EHP_SIMPLE_KERNELS | EHP_DIFFERENTIAL))
         return fail(EHP_ERR_ARGUMENT, "unknown voxel projector flags 0x%x", flags);
     if (!lut || n_lut < 2)
         return fail(EHP_ERR_ARGUMENT, "the basis-function table needs at least 2 entries");
@@ -486,6 +454,7 @@ int ehp_voxel_create(const ehp_geometry* g, int dtype, const void* lut, int n_lu
     ehp_op* op = *out;
     op->n_lut = n_lut;
     op->support = (float) support;
+    op->differential = (flags & EHP_DIFFERENTIAL) ? 1 : 0;
     const size_t bytes = (size_t) n_lut * dtype_size(dtype);
     hipError_t e = hipMalloc(&op->d_lut, bytes);
     if (e != hipSuccess) {
@@ -579,11 +548,13 @@ static int apply(ehp_op* op, const void* in, void* out, void* stream, bool adjoi
             e = adjoint ? ehp::siddon_adjoint<float>(op->d_geo, dm, x, y, s) : ehp::siddon_forward<float>(op->d_geo, dm, x, y, s);
         else if (!adjoint && op->tiled)
             e = ehp::voxel_forward_tiled(op->d_geo, dm, (const float*) op->d_lut, op->n_lut, op->support,
-                                         op->d_pose_axes, op->d_pose_margin, op->tile, op->lutscale, op->slot_bound,
-                                         op->d_maxbits, op->d_fixexp, x, y, s);
+                                         op->differential, op->d_pose_axes, op->d_pose_margin, op->tile, op->lutscale,
+                                         op->slot_bound, op->d_maxbits, op->d_fixexp, x, y, s);
         else
-            e = adjoint ? ehp::voxel_adjoint<float>(op->d_geo, dm, (const float*) op->d_lut, op->n_lut, op->support, x, y, s)
-                        : ehp::voxel_forward<float>(op->d_geo, dm, (const float*) op->d_lut, op->n_lut, op->support, x, y, s);
+            e = adjoint ? ehp::voxel_adjoint<float>(op->d_geo, dm, (const float*) op->d_lut, op->n_lut, op->support,
+                                                    op->differential, x, y, s)
+                        : ehp::voxel_forward<float>(op->d_geo, dm, (const float*) op->d_lut, op->n_lut, op->support,
+                                                    op->differential, x, y, s);
     } else {
         const double* x = (const double*) in;
         double* y = (double*) out;
@@ -592,8 +563,10 @@ static int apply(ehp_op* op, const void* in, void* out, void* stream, bool adjoi
         else if (op->kind == KIND_SIDDON)
             e = adjoint ? ehp::siddon_adjoint<double>(op->d_geo, dm, x, y, s) : ehp::siddon_forward<double>(op->d_geo, dm, x, y, s);
         else
-            e = adjoint ? ehp::voxel_adjoint<double>(op->d_geo, dm, (const double*) op->d_lut, op->n_lut, op->support, x, y, s)
-                        : ehp::voxel_forward<double>(op->d_geo, dm, (const double*) op->d_lut, op->n_lut, op->support, x, y, s);
+            e = adjoint ? ehp::voxel_adjoint<double>(op->d_geo, dm, (const double*) op->d_lut, op->n_lut, op->support,
+                                                     op->differential, x, y, s)
+                        : ehp::voxel_forward<double>(op->d_geo, dm, (const double*) op->d_lut, op->n_lut, op->support,
+                                                     op->differential, x, y, s);
     }
     if (e != hipSuccess)
         return hip_fail(e, adjoint ? "adjoint launch" : "forward launch");
@@ -679,6 +652,26 @@ int ehp_solver_fail(int code, const char* msg)
 
 namespace
 {
+    // one operator as a batch of one for the solvers
+    struct SingleOps {
+        int64_t doff[2], roff[2];
+        ehp::SolveOps ops;
+        explicit SingleOps(ehp_op* op)
+        {
+            doff[0] = roff[0] = 0;
+            doff[1] = op->dims.nvox;
+            roff[1] = op->dims.nrays;
+            ops.nseg = 1;
+            ops.doff = doff;
+            ops.roff = roff;
+            ops.fwd = [op](const void* x, void* y, hipStream_t s) { return ehp_forward(op, x, y, s); };
+            ops.adj = [op](const void* y, void* x, hipStream_t s) { return ehp_adjoint(op, y, x, s); };
+        }
+    };
+} // namespace
+
+namespace
+{
     int solver_args(const ehp_op* op, const void* b, const void* x, int iterations)
     {
         if (!op)
@@ -707,8 +700,9 @@ int ehp_cgls(ehp_op* op, const void* d_b, void* d_x, int use_x0, int iterations,
         return rc;
     if (flags & ~EHP_SOLVE_GRAPH)
         return fail(EHP_ERR_ARGUMENT, "unknown solver flags 0x%x", flags);
-    return ehp::solver_cgls(op, op->dtype, op->dims.nvox, op->dims.nrays, d_b, d_x, use_x0, iterations, eps, tol,
-                            (flags & EHP_SOLVE_GRAPH) != 0, residuals, iterations_done, (hipStream_t) stream);
+    SingleOps A(op);
+    return ehp::solve_cgls(A.ops, op->dtype, d_b, d_x, use_x0, iterations, eps, tol, (flags & EHP_SOLVE_GRAPH) != 0,
+                           residuals, iterations_done, (hipStream_t) stream);
 }
 
 int ehp_apgd(ehp_op* op, const void* d_b, void* d_x, int use_x0, int iterations, double mu, double l1_sigma,
@@ -719,9 +713,10 @@ int ehp_apgd(ehp_op* op, const void* d_b, void* d_x, int use_x0, int iterations,
         return rc;
     if (flags & ~EHP_SOLVE_GRAPH)
         return fail(EHP_ERR_ARGUMENT, "unknown solver flags 0x%x", flags);
-    return ehp::solver_apgd(op, op->dtype, op->dims.nvox, op->dims.nrays, d_b, d_x, use_x0, iterations, mu, l1_sigma,
-                            epsilon, (flags & EHP_SOLVE_GRAPH) != 0, grad_sq_norms, iterations_done, mu_used,
-                            (hipStream_t) stream);
+    SingleOps A(op);
+    return ehp::solve_apgd(A.ops, op->dtype, d_b, d_x, use_x0, iterations, &mu, l1_sigma, epsilon,
+                           (flags & EHP_SOLVE_GRAPH) != 0, grad_sq_norms, iterations_done, mu_used,
+                           (hipStream_t) stream);
 }
 
 int ehp_power_iterations(ehp_op* op, int niters, double* L, void* stream)
@@ -736,5 +731,6 @@ int ehp_power_iterations(ehp_op* op, int niters, double* L, void* stream)
         return hip_fail(e, "hipGetDevice");
     if (cur != op->device)
         return fail(EHP_ERR_ARGUMENT, "operator lives on device %d, current device is %d", op->device, cur);
-    return ehp::solver_power_iterations(op, op->dtype, op->dims.nvox, op->dims.nrays, niters, L, (hipStream_t) stream);
+    SingleOps A(op);
+    return ehp::solve_power_iterations(A.ops, op->dtype, niters, L, (hipStream_t) stream);
 }

@@ -6,6 +6,8 @@
 //                BaseCircleTrajectoryGenerator.cpp:12-74, TrajectoryGenerator.cpp:9-47
 //   blob table   elsa/projectors/Blobs.h:67-98, :249-317 with Bessel.cpp:7-80
 //   B-spline     elsa/projectors/BSplines.h:22-38, :72-81, :132-208
+//   derivative tables (phase contrast)
+//                elsa/projectors/Blobs.h:129-207, :249-317; BSplines.h:40-45, :83-92, :132-208
 //   tables       elsa/projectors/Luts.hpp:14-29
 //
 // FP32 throughout (elsa's real_t), products and sums in elsa's order, no contraction
@@ -193,6 +195,23 @@ namespace
         }
     }
 
+    // Line integral of the blob's derivative (Blobs.h:129-153), or that divided by s (the
+    // normalized gradient, :183-207), orders 1 and 2. As in elsa the factors -2.0 s / a and
+    // the Bessel values are double and sinh / cosh stay in T.
+    template <class T>
+    T blob_derivative_line_integral(T s, T a, T alpha, int64_t m, bool normalized)
+    {
+        const T q = s / a;
+        const T w = static_cast<T>(1.0 - static_cast<T>(std::pow(q, 2)));
+        if (!(w > 1.0e-10))
+            return T(0);
+        const T arg = alpha * std::sqrt(w);
+        const double lead = normalized ? -2.0 : -2.0 * s;
+        if (m == 1)
+            return static_cast<T>((lead / a) * std::sinh(arg) / i1(alpha));
+        return static_cast<T>((lead / alpha / a) * (std::cosh(arg) * arg - std::sinh(arg)) / i2(alpha));
+    }
+
     int64_t fact(int64_t n) { return n <= 1 ? 1 : n * fact(n - 1); }
 
     int64_t choose(int64_t n, int64_t k)
@@ -222,6 +241,22 @@ namespace
             sum += static_cast<T>(term);
         }
         return sum;
+    }
+
+    // Derivative of the projected B-spline (BSplines.h:40-45 with :83-92 at dim - 1): the
+    // difference of two double B-splines of order m - 1 at x +- 0.5, in T, times b(0) (double)
+    // per dimension beyond the second; normalized divides by x, x = 0 taken as 1e-10
+    // (BSplines.h:182-188).
+    template <class T>
+    T bspline_derivative(T x, int64_t m, int64_t dim, bool normalized)
+    {
+        if (normalized && x == T(0))
+            x = static_cast<T>(1e-10);
+        T v = static_cast<T>(bspline<double>(static_cast<double>(x) + 0.5, m - 1) -
+                             bspline<double>(static_cast<double>(x) - 0.5, m - 1));
+        for (int64_t i = 1; i < dim - 1; ++i)
+            v = static_cast<T>(v * bspline<double>(0.0, m));
+        return normalized ? v / x : v;
     }
 
     template <class T, class F>
@@ -362,6 +397,53 @@ int ehp_bspline_lut(int dtype, int64_t dim, int64_t order, int n, void* out, dou
                 v *= bspline<double>(0.0, order);
             return v;
         });
+        if (support)
+            *support = r;
+    } else {
+        return EHP_ERR_ARGUMENT;
+    }
+    return EHP_OK;
+}
+
+int ehp_blob_derivative_lut(int dtype, double radius, double alpha, int64_t order, int n, int normalized, void* out,
+                            double* support)
+{
+    if (!out || n < 2 || order < 0 || order > 2 || !(radius > 0) || !(alpha > 0))
+        return EHP_ERR_ARGUMENT;
+    const bool nz = normalized != 0;
+    if (dtype == EHP_F32) {
+        const float a = (float) radius, al = (float) alpha;
+        table<float>(a, n, (float*) out, [&](float s) {
+            return order > 0 ? blob_derivative_line_integral<float>(s, a, al, order, nz) : 0.0f;
+        });
+    } else if (dtype == EHP_F64) {
+        table<double>(radius, n, (double*) out, [&](double s) {
+            return order > 0 ? blob_derivative_line_integral<double>(s, radius, alpha, order, nz) : 0.0;
+        });
+    } else {
+        return EHP_ERR_ARGUMENT;
+    }
+    if (support)
+        *support = dtype == EHP_F32 ? (double) (float) radius : radius;
+    return EHP_OK;
+}
+
+int ehp_bspline_derivative_lut(int dtype, int64_t dim, int64_t order, int n, int normalized, void* out,
+                               double* support)
+{
+    if (!out || n < 2 || order < 0 || dim < 2)
+        return EHP_ERR_ARGUMENT;
+    const bool nz = normalized != 0;
+    if (dtype == EHP_F32) {
+        const float r = (float) ((order + 1) * 0.5);
+        table<float>(r, n, (float*) out,
+                     [&](float x) { return order > 0 ? bspline_derivative<float>(x, order, dim, nz) : 0.0f; });
+        if (support)
+            *support = r;
+    } else if (dtype == EHP_F64) {
+        const double r = (order + 1) * 0.5;
+        table<double>(r, n, (double*) out,
+                      [&](double x) { return order > 0 ? bspline_derivative<double>(x, order, dim, nz) : 0.0; });
         if (support)
             *support = r;
     } else {

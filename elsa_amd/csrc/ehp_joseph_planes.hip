@@ -25,11 +25,10 @@ namespace ehp
         constexpr int kTile = 32; // transpose tile
 
         // View with x fastest: one lane per voxel, rows copied as they are.
-        __global__ void __launch_bounds__(256)
-            view_copy_rows_kernel(const float* __restrict__ in, float* __restrict__ out, const Dims dims,
-                                  const PadLayout L)
+        __device__ __forceinline__ void view_copy_rows(const float* __restrict__ in, float* __restrict__ out,
+                                                       const Dims& dims, const PadLayout& L, int64_t block)
         {
-            const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+            const int64_t i = block * 256 + threadIdx.x;
             if (i >= dims.nvox)
                 return;
             const int x = (int) (i % dims.vol[0]);
@@ -40,15 +39,16 @@ namespace ehp
         }
 
         // View with f != x fastest: a 32 x 32 LDS tile over the (x, f) plane, reads coalesced
-        // along x, writes along f. Grid: (ceil(nx / 32), ceil(n_f / 32), n_t), t the third axis.
-        __global__ void __launch_bounds__(256)
-            view_copy_transpose_kernel(const float* __restrict__ in, float* __restrict__ out, const Dims dims,
-                                       const PadLayout L)
+        // along x, writes along f. Block (bx, by, bz) of the grid (ceil(nx / 32), ceil(n_f / 32),
+        // n_t), t the third axis.
+        __device__ __forceinline__ void view_copy_transpose(const float* __restrict__ in, float* __restrict__ out,
+                                                            const Dims& dims, const PadLayout& L, int bx, int by,
+                                                            int bz)
         {
             __shared__ float tile[kTile][kTile + 1];
             const int f = L.f;
             const int t = 3 - f; // the axis that is neither x nor f
-            const int x0 = blockIdx.x * kTile, f0 = blockIdx.y * kTile, ct = blockIdx.z;
+            const int x0 = bx * kTile, f0 = by * kTile, ct = bz;
             const int tx = threadIdx.x % kTile, ty = threadIdx.x / kTile; // 32 x 8
             const int nx = dims.vol[0], nf = dims.vol[f];
             const int64_t in_f = f == 1 ? dims.strides[1] : dims.strides[2];
@@ -70,11 +70,6 @@ namespace ehp
             }
         }
 
-        struct ViewSet {
-            PadView v[3];
-            int fb2; // bit a: the fast in-plane axis of view a is b2
-        };
-
         template <int D, int A>
         __device__ inline float fwd_ray(const RayRec& r, const Dims& dims, const ViewSet& vs)
         {
@@ -83,18 +78,41 @@ namespace ehp
             return plane_ray_forward<D, A, false>(r, dims, vs.v[A]);
         }
 
+        __global__ void __launch_bounds__(256)
+            view_copy_rows_kernel(const float* __restrict__ in, float* __restrict__ out, const Dims dims,
+                                  const PadLayout L)
+        {
+            view_copy_rows(in, out, dims, L, blockIdx.x);
+        }
+
+        __global__ void __launch_bounds__(256)
+            view_copy_transpose_kernel(const float* __restrict__ in, float* __restrict__ out, const Dims dims,
+                                       const PadLayout L)
+        {
+            view_copy_transpose(in, out, dims, L, blockIdx.x, blockIdx.y, blockIdx.z);
+        }
+
+        // Blocks of the view copy of one volume into its view (rows: a flat range; transpose:
+        // the grid (gx, gy, n_t) flattened with x fastest).
+        inline int64_t view_copy_blocks(const Dims& dims, const PadLayout& L, int& gx, int& gy)
+        {
+            if (L.f == 0) {
+                gx = gy = 0;
+                return (dims.nvox + 255) / 256;
+            }
+            gx = (dims.vol[0] + kTile - 1) / kTile;
+            gy = (dims.vol[L.f] + kTile - 1) / kTile;
+            return (int64_t) gx * gy * dims.vol[3 - L.f];
+        }
+
+        // Block id of one pose-band tile of the forward: id = (band * nposes + pose) * tiles_u + tu
         template <int D>
-        __global__ void __launch_bounds__(kThreads)
-            joseph_fwd_planes_kernel(const PoseGeom* __restrict__ geo, const Dims dims, const ViewSet views,
-                                     const int tiles_u, const int per_xcd, const int64_t total,
-                                     float* __restrict__ sino)
+        __device__ __forceinline__ void fwd_planes_block(const PoseGeom* __restrict__ geo, const Dims& dims,
+                                                         const ViewSet& views, const int tiles_u, const int64_t id,
+                                                         float* __restrict__ sino)
         {
             constexpr int TU = (D == 3) ? 64 : kThreads;
             constexpr int TV = kThreads / TU;
-            const int64_t id = (int64_t) (blockIdx.x % 8u) * per_xcd + (blockIdx.x / 8u);
-            if (id >= total)
-                return;
-            // id = (band * nposes + pose) * tiles_u + tu
             const int tu = (int) (id % tiles_u);
             const int64_t bp = id / tiles_u;
             const int pose = (int) (bp % dims.nposes);
@@ -116,6 +134,50 @@ namespace ehp
             else
                 val = fwd_ray<D, D - 1>(r, dims, views);
             sino[(int64_t) pose * dims.pix_per_pose + (int64_t) v * dims.det[0] + u] = val;
+        }
+
+        template <int D>
+        __global__ void __launch_bounds__(kThreads)
+            joseph_fwd_planes_kernel(const PoseGeom* __restrict__ geo, const Dims dims, const ViewSet views,
+                                     const int tiles_u, const int per_xcd, const int64_t total,
+                                     float* __restrict__ sino)
+        {
+            const int64_t id = (int64_t) (blockIdx.x % 8u) * per_xcd + (blockIdx.x / 8u);
+            if (id >= total)
+                return;
+            fwd_planes_block<D>(geo, dims, views, tiles_u, id, sino);
+        }
+
+        // ---- batches: one launch over the jobs of many operators; block b runs block
+        // b - job.block0 of job block_job[b], exactly as the single-operator launch would
+
+        __global__ void __launch_bounds__(256)
+            view_copy_batch_kernel(const CopyJob* __restrict__ jobs, const int* __restrict__ block_job,
+                                   const float* __restrict__ vols)
+        {
+            const CopyJob& J = jobs[block_job[blockIdx.x]];
+            const int64_t b = (int64_t) blockIdx.x - J.block0;
+            const float* in = vols + J.vol_off;
+            if (J.L.f == 0) {
+                view_copy_rows(in, J.view, J.dims, J.L, b);
+            } else {
+                const int64_t gxy = (int64_t) J.gx * J.gy;
+                const int bz = (int) (b / gxy);
+                const int r = (int) (b - bz * gxy);
+                view_copy_transpose(in, J.view, J.dims, J.L, r % J.gx, r / J.gx, bz);
+            }
+        }
+
+        template <int D>
+        __global__ void __launch_bounds__(kThreads)
+            joseph_fwd_planes_batch_kernel(const FwdJob* __restrict__ jobs, const int* __restrict__ block_job,
+                                           const int per_xcd, const int64_t total, float* __restrict__ sinos)
+        {
+            const int64_t id = (int64_t) (blockIdx.x % 8u) * per_xcd + (blockIdx.x / 8u);
+            if (id >= total)
+                return;
+            const FwdJob& J = jobs[block_job[id]];
+            fwd_planes_block<D>(J.geo, J.dims, J.views, J.tiles_u, id - J.block0, sinos + J.sino_off);
         }
     } // namespace
 
@@ -167,6 +229,73 @@ namespace ehp
         else
             hipLaunchKernelGGL((joseph_fwd_planes_kernel<3>), grid, dim3(kThreads), 0, s, geo, dims, vs, tiles_u,
                                (int) per_xcd, total, sino);
+        return hipGetLastError();
+    }
+
+    // Jobs of a batched forward: the view copies of every operator (one job per view) and the
+    // forward blocks (one job per operator); block_job maps each block to its job.
+    void joseph_forward_batch_jobs(const JosephBatchItem* items, int n, std::vector<CopyJob>& cjobs,
+                                   std::vector<int>& cblocks, std::vector<FwdJob>& fjobs, std::vector<int>& fblocks)
+    {
+        cjobs.clear();
+        cblocks.clear();
+        fjobs.clear();
+        fblocks.clear();
+        for (int j = 0; j < n; ++j) {
+            const JosephBatchItem& it = items[j];
+            FwdJob fj{};
+            fj.geo = it.geo;
+            fj.dims = it.dims;
+            fj.sino_off = it.sino_off;
+            for (int a = 0; a < it.dims.dim; ++a) {
+                if (!(it.axes_present & (1 << a)))
+                    continue;
+                const PadLayout& L = it.layouts[a];
+                CopyJob cj{};
+                cj.view = it.views[a];
+                cj.dims = it.dims;
+                cj.L = L;
+                cj.vol_off = it.vol_off;
+                cj.block0 = (int64_t) cblocks.size();
+                const int64_t nb = view_copy_blocks(it.dims, L, cj.gx, cj.gy);
+                cblocks.insert(cblocks.end(), (size_t) nb, (int) cjobs.size());
+                cjobs.push_back(cj);
+                fj.views.v[a] = PadView{it.views[a], L.PS * 4u, L.RS * 4u, L.size};
+                const int b2 = it.dims.dim == 3 ? (a == 2 ? 1 : 2) : -1;
+                if (L.f == b2)
+                    fj.views.fb2 |= 1 << a;
+            }
+            const int TU = it.dims.dim == 3 ? 64 : kThreads;
+            const int TV = kThreads / TU;
+            fj.tiles_u = (it.dims.det[0] + TU - 1) / TU;
+            const int bands = (it.dims.det[1] + TV - 1) / TV;
+            const int64_t nb = (int64_t) fj.tiles_u * bands * it.dims.nposes;
+            fj.block0 = (int64_t) fblocks.size();
+            fblocks.insert(fblocks.end(), (size_t) nb, (int) fjobs.size());
+            fjobs.push_back(fj);
+        }
+    }
+
+    hipError_t joseph_forward_batch(int dim, const CopyJob* cjobs, const int* cblocks, int64_t ncblocks,
+                                    const FwdJob* fjobs, const int* fblocks, int64_t nfblocks, const float* vols,
+                                    float* sinos, hipStream_t s)
+    {
+        if (ncblocks > 0x7fffffff || nfblocks > 0x7fffffff)
+            return hipErrorInvalidValue;
+        if (ncblocks > 0)
+            hipLaunchKernelGGL(view_copy_batch_kernel, dim3((unsigned) ncblocks), dim3(256), 0, s, cjobs, cblocks, vols);
+        const int64_t per_xcd = (nfblocks + 7) / 8;
+        if (per_xcd * 8 > (int64_t) 0x7fffffff)
+            return hipErrorInvalidValue;
+        if (nfblocks > 0) {
+            const dim3 grid((unsigned) (per_xcd * 8));
+            if (dim == 2)
+                hipLaunchKernelGGL((joseph_fwd_planes_batch_kernel<2>), grid, dim3(kThreads), 0, s, fjobs, fblocks,
+                                   (int) per_xcd, nfblocks, sinos);
+            else
+                hipLaunchKernelGGL((joseph_fwd_planes_batch_kernel<3>), grid, dim3(kThreads), 0, s, fjobs, fblocks,
+                                   (int) per_xcd, nfblocks, sinos);
+        }
         return hipGetLastError();
     }
 } // namespace ehp

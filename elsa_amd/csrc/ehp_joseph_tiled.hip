@@ -102,23 +102,20 @@ namespace ehp
     // ---------------------------------------------------------------------------------
     // the tiled back projection
 
+    // One tile of the back projection along driving axis A (tile < ntA * ntB * ntC).
     template <int D, int A>
-    __global__ void __launch_bounds__(kThreads)
-        joseph_adj_tiled_kernel(const PoseGeom* __restrict__ geo, const RayRec* __restrict__ table, const Dims dims,
-                                const int* __restrict__ pose_axes, const TileCfg tc, const int ntA, const int ntB,
-                                const int ntC, const int tiles_per_xcd, const int batch,
-                                const int* __restrict__ fixexp, const float* __restrict__ sino,
-                                float* __restrict__ vol)
+    __device__ __forceinline__ void adj_tiled_block(const PoseGeom* __restrict__ geo,
+                                                    const RayRec* __restrict__ table, const Dims& dims,
+                                                    const int* __restrict__ pose_axes, const TileCfg& tc,
+                                                    const int ntA, const int ntB, const int tile, const int batch,
+                                                    const int* __restrict__ fixexp, const float* __restrict__ sino,
+                                                    float* __restrict__ vol)
     {
         extern __shared__ long long lds[];
         __shared__ int s_u0[kMaxPoseBatch], s_nu[kMaxPoseBatch], s_v0[kMaxPoseBatch], s_pose[kMaxPoseBatch];
         __shared__ int s_cnt[kMaxPoseBatch], s_pref[kMaxPoseBatch + 1];
         __shared__ float s_rnu[kMaxPoseBatch];
 
-        // consecutive workgroups land on different XCDs: give each XCD a contiguous block of tiles
-        const int tile = (int) (blockIdx.x % 8u) * tiles_per_xcd + (int) (blockIdx.x / 8u);
-        if (tile >= ntA * ntB * ntC)
-            return;
         const int tid = threadIdx.x;
         const TileBox t = make_tile<D, A>(dims, tc, tile % ntA, (tile / ntA) % ntB, tile / (ntA * ntB));
         long long* iacc = lds;
@@ -202,6 +199,68 @@ namespace ehp
         }
     }
 
+    template <int D, int A>
+    __global__ void __launch_bounds__(kThreads)
+        joseph_adj_tiled_kernel(const PoseGeom* __restrict__ geo, const RayRec* __restrict__ table, const Dims dims,
+                                const int* __restrict__ pose_axes, const TileCfg tc, const int ntA, const int ntB,
+                                const int ntC, const int tiles_per_xcd, const int batch,
+                                const int* __restrict__ fixexp, const float* __restrict__ sino,
+                                float* __restrict__ vol)
+    {
+        // consecutive workgroups land on different XCDs: give each XCD a contiguous block of tiles
+        const int tile = (int) (blockIdx.x % 8u) * tiles_per_xcd + (int) (blockIdx.x / 8u);
+        if (tile >= ntA * ntB * ntC)
+            return;
+        adj_tiled_block<D, A>(geo, table, dims, pose_axes, tc, ntA, ntB, tile, batch, fixexp, sino, vol);
+    }
+
+    // the tiles of axis A of many operators in one launch (block b: tile b - job.block0 of job
+    // block_job[b])
+    template <int D, int A>
+    __global__ void __launch_bounds__(kThreads)
+        joseph_adj_tiled_batch_kernel(const AdjJob* __restrict__ jobs, const int* __restrict__ block_job,
+                                      const int per_xcd, const int64_t total, const int* __restrict__ fixexp,
+                                      const float* __restrict__ sinos, float* __restrict__ vols)
+    {
+        const int64_t id = (int64_t) (blockIdx.x % 8u) * per_xcd + (blockIdx.x / 8u);
+        if (id >= total)
+            return;
+        const int j = block_job[id];
+        const AdjJob& J = jobs[j];
+        adj_tiled_block<D, A>(J.geo, J.table, J.dims, J.pose_axes, J.tc, J.ntA, J.ntB, (int) (id - J.block0),
+                              J.batch, fixexp + J.item, sinos + J.sino_off, vols + J.vol_off);
+    }
+
+    // max|y| per operator of a batch (block b scans its share of segment seg_of_block[b])
+    __global__ void __launch_bounds__(256)
+        absmax_batch_kernel(const float* __restrict__ y, const SegJob* __restrict__ segs,
+                            const int* __restrict__ block_seg, unsigned int* __restrict__ out)
+    {
+        const int j = block_seg[blockIdx.x];
+        const SegJob& S = segs[j];
+        const int64_t b = (int64_t) blockIdx.x - S.block0;
+        float m = 0.0f;
+        const int64_t stride = (int64_t) S.nblocks * 256;
+        for (int64_t i = b * 256 + threadIdx.x; i < S.n; i += stride) {
+            const float a = fabsf(y[S.off + i]);
+            m = (a > m || a != a) ? a : m;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const float o = __shfl_down(m, off, 64);
+            m = (o > m || o != o) ? o : m;
+        }
+        if ((threadIdx.x & 63) == 0)
+            atomicMax(out + j, __float_as_uint(m));
+    }
+
+    __global__ void scale_batch_kernel(const unsigned int* __restrict__ maxbits, const double* __restrict__ bounds,
+                                       int n, int* __restrict__ fixexp)
+    {
+        const int j = blockIdx.x * blockDim.x + threadIdx.x;
+        if (j < n)
+            fixexp[j] = fixed_point_exponent(__uint_as_float(maxbits[j]), bounds[j]);
+    }
+
     // Which driving axes occur among the rays of each pose (bit a set when some ray that hits
     // the volume drives along a).
     __global__ void __launch_bounds__(256)
@@ -274,5 +333,102 @@ namespace ehp
                 return e;
         }
         return hipSuccess;
+    }
+
+    // Segments of a batch's sinograms (absmax) and the tiles of every operator per driving
+    // axis (one job per operator that has rays of that axis).
+    void joseph_adjoint_batch_jobs(const JosephBatchItem* items, int n, std::vector<SegJob>& segs,
+                                   std::vector<int>& sblocks, std::vector<AdjJob> ajobs[3],
+                                   std::vector<int> ablocks[3], size_t lds[3])
+    {
+        segs.clear();
+        sblocks.clear();
+        for (int a = 0; a < 3; ++a) {
+            ajobs[a].clear();
+            ablocks[a].clear();
+            lds[a] = 0;
+        }
+        for (int j = 0; j < n; ++j) {
+            const JosephBatchItem& it = items[j];
+            SegJob sg{};
+            sg.off = it.sino_off;
+            sg.n = it.dims.nrays;
+            int64_t nb = (sg.n + 255) / 256;
+            nb = nb > 256 ? 256 : (nb < 1 ? 1 : nb);
+            sg.nblocks = (int) nb;
+            sg.block0 = (int64_t) sblocks.size();
+            sblocks.insert(sblocks.end(), (size_t) nb, j);
+            segs.push_back(sg);
+            for (int a = 0; a < it.dims.dim; ++a) {
+                if (!(it.axes_present & (1 << a)))
+                    continue;
+                AdjJob aj{};
+                aj.geo = it.geo;
+                aj.table = it.table;
+                aj.pose_axes = it.pose_axes;
+                aj.dims = it.dims;
+                aj.tc = it.tile;
+                aj.batch = it.batch;
+                aj.item = j;
+                aj.sino_off = it.sino_off;
+                aj.vol_off = it.vol_off;
+                int ntC;
+                tile_counts(it.dims, it.tile, a, aj.ntA, aj.ntB, ntC);
+                aj.block0 = (int64_t) ablocks[a].size();
+                ablocks[a].insert(ablocks[a].end(), (size_t) aj.ntA * aj.ntB * ntC, (int) ajobs[a].size());
+                ajobs[a].push_back(aj);
+                const size_t need = joseph_tile_lds_bytes(it.dims, it.tile);
+                lds[a] = need > lds[a] ? need : lds[a];
+            }
+        }
+    }
+
+    hipError_t joseph_adjoint_batch(int dim, int nitems, const SegJob* segs, const int* sblocks, int64_t nsblocks,
+                                    const double* bounds, const AdjJob* const ajobs[3], const int* const ablocks[3],
+                                    const int64_t nablocks[3], const size_t lds[3], int64_t nvol_total,
+                                    unsigned int* d_maxbits, int* d_fixexp, const float* sinos, float* vols,
+                                    hipStream_t s)
+    {
+        if (nsblocks > 0x7fffffff)
+            return hipErrorInvalidValue;
+        hipError_t e = launch_fill<float>(vols, nvol_total, 0.0f, s);
+        if (e == hipSuccess)
+            e = launch_fill<unsigned int>(d_maxbits, nitems, 0u, s);
+        if (e != hipSuccess)
+            return e;
+        hipLaunchKernelGGL(absmax_batch_kernel, dim3((unsigned) nsblocks), dim3(256), 0, s, sinos, segs, sblocks,
+                           d_maxbits);
+        hipLaunchKernelGGL(scale_batch_kernel, dim3((unsigned) ((nitems + 63) / 64)), dim3(64), 0, s, d_maxbits,
+                           bounds, nitems, d_fixexp);
+        for (int a = 0; a < dim; ++a) {
+            const int64_t total = nablocks[a];
+            if (total == 0)
+                continue;
+            const int64_t per_xcd = (total + 7) / 8;
+            if (per_xcd * 8 > (int64_t) 0x7fffffff)
+                return hipErrorInvalidValue;
+            const dim3 grid((unsigned) (per_xcd * 8));
+#define EHP_ADJ_BATCH(Dv, Av)                                                                                      \
+    hipLaunchKernelGGL((joseph_adj_tiled_batch_kernel<Dv, Av>), grid, dim3(kThreads), lds[a], s, ajobs[a], ablocks[a], \
+                       (int) per_xcd, total, d_fixexp, sinos, vols)
+            if (dim == 2) {
+                if (a == 0)
+                    EHP_ADJ_BATCH(2, 0);
+                else
+                    EHP_ADJ_BATCH(2, 1);
+            } else {
+                if (a == 0)
+                    EHP_ADJ_BATCH(3, 0);
+                else if (a == 1)
+                    EHP_ADJ_BATCH(3, 1);
+                else
+                    EHP_ADJ_BATCH(3, 2);
+            }
+#undef EHP_ADJ_BATCH
+            e = hipGetLastError();
+            if (e != hipSuccess)
+                return e;
+        }
+        return hipGetLastError();
     }
 } // namespace ehp

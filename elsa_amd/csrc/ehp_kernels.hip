@@ -209,7 +209,7 @@ namespace ehp
     template <int D, class T>
     __global__ void __launch_bounds__(256)
         voxel_adj_kernel(const PoseGeom* __restrict__ geo, const Dims dims, const T* __restrict__ lut, int n_lut,
-                         float support, const T* __restrict__ sino, T* __restrict__ vol)
+                         float support, const int diff, const T* __restrict__ sino, T* __restrict__ vol)
     {
         const int64_t vox = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
         if (vox >= dims.nvox)
@@ -222,7 +222,9 @@ namespace ehp
             Footprint fp;
             footprint<D>(geo[p], dims, x, y, z, support, fp);
             const T* sp = sino + (int64_t) p * dims.pix_per_pose;
-            visit<D, T>(fp, dims, [&](int64_t pix, float dist) { acc += lut_eval<T>(lut, n_lut, support, dist) * sp[pix]; });
+            visit<D, T>(fp, dims, [&](int64_t pix, float dist, float d0) {
+                acc += tap_weight<D, T>(lut, n_lut, support, dist, d0, diff != 0) * sp[pix];
+            });
         }
         vol[vox] = acc;
     }
@@ -231,7 +233,7 @@ namespace ehp
     template <int D, class T>
     __global__ void __launch_bounds__(256)
         voxel_fwd_kernel(const PoseGeom* __restrict__ geo, const Dims dims, const T* __restrict__ lut, int n_lut,
-                         float support, const T* __restrict__ vol, T* __restrict__ sino,
+                         float support, const int diff, const T* __restrict__ vol, T* __restrict__ sino,
                          unsigned long long* count)
     {
         const int p = blockIdx.y;
@@ -251,8 +253,8 @@ namespace ehp
                 const T xv = vol[vox];
                 if (xv != T(0)) {
                     T* sp = sino + (int64_t) p * dims.pix_per_pose;
-                    visit<D, T>(fp, dims, [&](int64_t pix, float dist) {
-                        const T w = lut_eval<T>(lut, n_lut, support, dist);
+                    visit<D, T>(fp, dims, [&](int64_t pix, float dist, float d0) {
+                        const T w = tap_weight<D, T>(lut, n_lut, support, dist, d0, diff != 0);
                         if (w != T(0))
                             atomic_add(sp + pix, xv * w);
                     });
@@ -353,31 +355,31 @@ namespace ehp
 
     template <class T>
     hipError_t voxel_forward(const PoseGeom* geo, const Dims& dims, const T* lut, int n_lut, float support,
-                             const T* vol, T* sino, hipStream_t s)
+                             int diff, const T* vol, T* sino, hipStream_t s)
     {
         hipError_t e = launch_fill<T>(sino, dims.nrays, T(0), s);
         if (e != hipSuccess)
             return e;
         if (dims.dim == 2)
             hipLaunchKernelGGL((voxel_fwd_kernel<2, T>), vox_pose_grid(dims), dim3(256), 0, s, geo, dims, lut, n_lut,
-                               support, vol, sino, nullptr);
+                               support, diff, vol, sino, nullptr);
         else
             hipLaunchKernelGGL((voxel_fwd_kernel<3, T>), vox_pose_grid(dims), dim3(256), 0, s, geo, dims, lut, n_lut,
-                               support, vol, sino, nullptr);
+                               support, diff, vol, sino, nullptr);
         return hipGetLastError();
     }
 
     template <class T>
     hipError_t voxel_adjoint(const PoseGeom* geo, const Dims& dims, const T* lut, int n_lut, float support,
-                             const T* sino, T* vol, hipStream_t s)
+                             int diff, const T* sino, T* vol, hipStream_t s)
     {
         const dim3 grid((unsigned) ((dims.nvox + 255) / 256));
         if (dims.dim == 2)
-            hipLaunchKernelGGL((voxel_adj_kernel<2, T>), grid, dim3(256), 0, s, geo, dims, lut, n_lut, support, sino,
-                               vol);
+            hipLaunchKernelGGL((voxel_adj_kernel<2, T>), grid, dim3(256), 0, s, geo, dims, lut, n_lut, support, diff,
+                               sino, vol);
         else
-            hipLaunchKernelGGL((voxel_adj_kernel<3, T>), grid, dim3(256), 0, s, geo, dims, lut, n_lut, support, sino,
-                               vol);
+            hipLaunchKernelGGL((voxel_adj_kernel<3, T>), grid, dim3(256), 0, s, geo, dims, lut, n_lut, support, diff,
+                               sino, vol);
         return hipGetLastError();
     }
 
@@ -386,10 +388,10 @@ namespace ehp
     {
         if (dims.dim == 2)
             hipLaunchKernelGGL((voxel_fwd_kernel<2, float>), vox_pose_grid(dims), dim3(256), 0, s, geo, dims,
-                               (const float*) nullptr, 0, support, (const float*) nullptr, (float*) nullptr, total);
+                               (const float*) nullptr, 0, support, 0, (const float*) nullptr, (float*) nullptr, total);
         else
             hipLaunchKernelGGL((voxel_fwd_kernel<3, float>), vox_pose_grid(dims), dim3(256), 0, s, geo, dims,
-                               (const float*) nullptr, 0, support, (const float*) nullptr, (float*) nullptr, total);
+                               (const float*) nullptr, 0, support, 0, (const float*) nullptr, (float*) nullptr, total);
         return hipGetLastError();
     }
 
@@ -399,9 +401,9 @@ namespace ehp
     template hipError_t joseph_adjoint<T>(const PoseGeom*, const Dims&, const T*, T*, hipStream_t);              \
     template hipError_t siddon_forward<T>(const PoseGeom*, const Dims&, const T*, T*, hipStream_t);              \
     template hipError_t siddon_adjoint<T>(const PoseGeom*, const Dims&, const T*, T*, hipStream_t);              \
-    template hipError_t voxel_forward<T>(const PoseGeom*, const Dims&, const T*, int, float, const T*, T*,        \
+    template hipError_t voxel_forward<T>(const PoseGeom*, const Dims&, const T*, int, float, int, const T*, T*,   \
                                          hipStream_t);                                                           \
-    template hipError_t voxel_adjoint<T>(const PoseGeom*, const Dims&, const T*, int, float, const T*, T*,        \
+    template hipError_t voxel_adjoint<T>(const PoseGeom*, const Dims&, const T*, int, float, int, const T*, T*,   \
                                          hipStream_t);
 
     EHP_INSTANTIATE(float)

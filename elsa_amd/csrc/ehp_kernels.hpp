@@ -6,10 +6,88 @@
 #include "ehp_siddon.hpp"
 #include "ehp_voxel.hpp"
 
+#include <vector>
+
 struct ehp_op;
 
 namespace ehp
 {
+    // the views of one plane-major forward
+    struct ViewSet {
+        PadView v[3];
+        int fb2; // bit a: the fast in-plane axis of view a is b2
+    };
+
+    // ---- batches of Joseph operators (FP32 fast kernels): one launch serves many operators.
+    // A job is one operator's share of a launch; block b of the launch runs block
+    // b - job.block0 of job block_job[b], exactly as that operator's own launch would. Volumes
+    // and sinograms of the batch are packed back to back (offsets in elements).
+
+    // what the batch needs of one operator
+    struct JosephBatchItem {
+        const PoseGeom* geo;
+        const RayRec* table;
+        const int* pose_axes;
+        float* views[3];
+        PadLayout layouts[3];
+        Dims dims;
+        TileCfg tile;
+        int axes_present;
+        int batch;
+        int64_t vol_off, sino_off;
+    };
+
+    struct CopyJob { // the view copy of one volume into one view
+        float* view;
+        Dims dims;
+        PadLayout L;
+        int64_t vol_off;
+        int64_t block0;
+        int gx, gy;
+    };
+
+    struct FwdJob { // the forward of one operator
+        const PoseGeom* geo;
+        ViewSet views;
+        Dims dims;
+        int tiles_u;
+        int64_t sino_off;
+        int64_t block0;
+    };
+
+    struct AdjJob { // the tiles of one operator along one driving axis
+        const PoseGeom* geo;
+        const RayRec* table;
+        const int* pose_axes;
+        Dims dims;
+        TileCfg tc;
+        int ntA, ntB;
+        int batch;
+        int item; // index of the operator in the batch (its fixed-point exponent)
+        int64_t sino_off, vol_off;
+        int64_t block0;
+    };
+
+    struct SegJob { // one operator's segment of a packed vector
+        int64_t off, n;
+        int64_t block0;
+        int nblocks;
+    };
+
+    void joseph_forward_batch_jobs(const JosephBatchItem* items, int n, std::vector<CopyJob>& cjobs,
+                                   std::vector<int>& cblocks, std::vector<FwdJob>& fjobs, std::vector<int>& fblocks);
+    hipError_t joseph_forward_batch(int dim, const CopyJob* cjobs, const int* cblocks, int64_t ncblocks,
+                                    const FwdJob* fjobs, const int* fblocks, int64_t nfblocks, const float* vols,
+                                    float* sinos, hipStream_t s);
+    void joseph_adjoint_batch_jobs(const JosephBatchItem* items, int n, std::vector<SegJob>& segs,
+                                   std::vector<int>& sblocks, std::vector<AdjJob> ajobs[3],
+                                   std::vector<int> ablocks[3], size_t lds[3]);
+    hipError_t joseph_adjoint_batch(int dim, int nitems, const SegJob* segs, const int* sblocks, int64_t nsblocks,
+                                    const double* bounds, const AdjJob* const ajobs[3], const int* const ablocks[3],
+                                    const int64_t nablocks[3], const size_t lds[3], int64_t nvol_total,
+                                    unsigned int* d_maxbits, int* d_fixexp, const float* sinos, float* vols,
+                                    hipStream_t s);
+
     template <class T>
     hipError_t launch_fill(T* p, int64_t n, T value, hipStream_t s);
 
@@ -37,9 +115,9 @@ namespace ehp
     size_t voxel_tile_lds_bytes(const TileCfg& tc);
     int voxel_max_lut();
     hipError_t voxel_forward_tiled(const PoseGeom* geo, const Dims& dims, const float* lut, int n_lut, float support,
-                                   const int* pose_axis, const float* pose_margin, const TileCfg& tc, float lutscale,
-                                   double slot_bound, unsigned int* d_maxbits, int* d_fixexp, const float* vol,
-                                   float* sino, hipStream_t s);
+                                   int diff, const int* pose_axis, const float* pose_margin, const TileCfg& tc,
+                                   float lutscale, double slot_bound, unsigned int* d_maxbits, int* d_fixexp,
+                                   const float* vol, float* sino, hipStream_t s);
     hipError_t joseph_view_copy(const Dims& dims, const PadLayout& L, const float* vol, float* view, hipStream_t s);
     hipError_t joseph_forward_planes(const PoseGeom* geo, const Dims& dims, const PadLayout* layouts,
                                      float* const* views, int axes_present, const float* vol, float* sino,
@@ -53,19 +131,11 @@ namespace ehp
 
     template <class T>
     hipError_t voxel_forward(const PoseGeom* geo, const Dims& dims, const T* lut, int n_lut, float support,
-                             const T* vol, T* sino, hipStream_t s);
+                             int diff, const T* vol, T* sino, hipStream_t s);
     template <class T>
     hipError_t voxel_adjoint(const PoseGeom* geo, const Dims& dims, const T* lut, int n_lut, float support,
-                             const T* sino, T* vol, hipStream_t s);
+                             int diff, const T* sino, T* vol, hipStream_t s);
     hipError_t voxel_count(const PoseGeom* geo, const Dims& dims, float support, unsigned long long* total,
                            hipStream_t s);
 
-    // device-resident solvers (ehp_solvers.hip)
-    int solver_cgls(ehp_op* op, int dtype, int64_t nd, int64_t nr, const void* b, void* x, int use_x0, int iterations,
-                    double eps, double tol, bool graph, double* residuals, int* iters_done, hipStream_t s);
-    int solver_apgd(ehp_op* op, int dtype, int64_t nd, int64_t nr, const void* b, void* x, int use_x0, int iterations,
-                    double mu, double sigma, double epsilon, bool graph, double* grad_norms, int* iters_done,
-                    double* mu_used, hipStream_t s);
-    int solver_power_iterations(ehp_op* op, int dtype, int64_t nd, int64_t nr, int niters, double* L,
-                                hipStream_t s);
 } // namespace ehp

@@ -1,0 +1,47 @@
+// ehp_op.hpp -- the library's handles (opaque in include/elsa_hip.h): a geometry, an operator
+// and its device state.
+#pragma once
+
+#include "ehp_kernels.hpp"
+
+#include <vector>
+
+struct ehp_geometry {
+    int dim;
+    int64_t nposes;
+    int vol[3];
+    int det[2];
+    std::vector<ehp::PoseGeom> host;
+};
+
+enum op_kind { KIND_JOSEPH = 0, KIND_SIDDON = 1, KIND_VOXEL = 2 };
+
+struct ehp_op {
+    int kind;
+    int dtype;
+    int device;
+    ehp::Dims dims;
+    ehp::PoseGeom* d_geo = nullptr;
+    void* d_lut = nullptr;
+    int n_lut = 0;
+    float support = 0.0f;
+    int differential = 0; // voxel projectors: phase-contrast weight
+    uint64_t samples = 0;
+    // Joseph, FP32: plane-major forward and LDS-tiled back projection
+    bool planes = false;
+    ehp::PadLayout layouts[3]{};
+    float* d_view[3] = {nullptr, nullptr, nullptr}; // zero-padded plane-major volume views
+    bool tiled = false;
+    int* d_pose_axes = nullptr;
+    int axes_present = 0;
+    ehp::TileCfg tile{};
+    ehp::RayRec* d_table = nullptr;   // per-ray traversal state (Joseph)
+    ehp::SidRec* d_sidtable = nullptr; // per-ray traversal state (Siddon)
+    float* d_pose_margin = nullptr;    // voxel forward: frustum margin per pose (pixels)
+    float lutscale = 1.0f;             // voxel forward: power of two >= max |table|
+    unsigned int* d_maxbits = nullptr; // max|y| of the current back projection
+    int* d_fixexp = nullptr;           // fixed-point exponent of the current back projection
+    double slot_bound = 0.0;
+    int batch = ehp::kPoseBatch;
+    std::vector<ehp::PoseGeom> host_geo;
+};

@@ -1,33 +1,40 @@
 // ehp_solvers.hip -- elsa's CGLS, APGD (FISTA) and power iterations, resident on the device.
 //
-// The iterations call the projector through the C-ABI (ehp_forward / ehp_adjoint) and keep
-// every scalar of the method -- step lengths, momentum, norms, the convergence state -- in
-// device memory, so an iteration needs no host round trip and can be captured once into a
-// hipGraph and replayed. Norms and dot products are FP64 two-stage reductions in a fixed
-// order (bitwise reproducible); the vector updates run in the operator's data type.
+// The iterations call the projector through the C-ABI (an operator's ehp_forward /
+// ehp_adjoint, or a batch's) and keep every scalar of the method -- step lengths, momentum,
+// norms, the convergence state -- in device memory, so an iteration needs no host round trip
+// and can be captured once into a hipGraph and replayed. Norms and dot products are FP64
+// two-stage reductions in a fixed order (bitwise reproducible); the vector updates run in the
+// operator's data type.
+//
+// A solve runs on nseg independent problems at once (a batch; a single operator is a batch of
+// one): vectors are the problems' own vectors packed back to back, every scalar exists once
+// per problem, and each problem follows its own iteration of the method exactly as if it were
+// solved alone -- its own step lengths, its own convergence test. Element-wise kernels give
+// each problem ceil(len / 256) blocks; reductions give it up to 1024 partial sums.
 //
 // elsa returns early when a method has converged (CGLS.cpp:54-58, APGD.cpp:83-87). Here the
-// check sets a device flag; once it is set the updates of x are skipped, so the result is the
-// iterate at which elsa would have returned, while the remaining replays do no harm.
+// check sets a per-problem device flag; once it is set the updates of that problem's x are
+// skipped, so its result is the iterate at which elsa would have returned, while the remaining
+// replays do no harm.
 #include "elsa_hip.h"
-#include "ehp_kernels.hpp"
+#include "ehp_solvers.hpp"
 
 #include <cmath>
 #include <cstdio>
 #include <string>
 #include <vector>
 
-struct ehp_op; // defined in ehp_capi.hip
 int ehp_solver_fail(int code, const char* msg);
 
 namespace ehp
 {
     namespace
     {
-        constexpr int kRedBlocks = 1024;
+        constexpr int kRedBlocks = 1024; // partial sums per problem at most
         constexpr int kThreads = 256;
 
-        // scalar slots
+        // scalar slots (per problem)
         enum {
             S_KOLD = 0, // CGLS: ||r||^2 of the previous iteration
             S_K,        // CGLS: ||r||^2
@@ -41,17 +48,45 @@ namespace ehp
             S_T,        // APGD: t of the previous iteration (in the data type)
             S_COEF,     // APGD: (t_prev - 1) / t
             S_TMP,      // scratch (power iterations)
+            S_MU,       // APGD step length (in the data type)
             S_COUNT
         };
 
+        // Blocks of one vector space (domain or range) of the packed problems, on the device.
+        struct SegSpace {
+            const int64_t* off; // nseg + 1 element offsets
+            const int* eseg;    // element-wise blocks: problem of each block
+            const int64_t* eb0; // nseg + 1: first element-wise block of each problem
+            const int* rseg;    // reduction blocks: problem of each block
+            const int64_t* rb0; // nseg + 1: first reduction block of each problem
+            int64_t neb, nrb;   // block counts
+        };
+
+        // Element i (packed index) of this thread in an element-wise launch over S; false past the
+        // end of its problem.
+        __device__ inline bool seg_elem(const SegSpace& S, int& seg, int64_t& i)
+        {
+            seg = S.eseg[blockIdx.x];
+            i = S.off[seg] + ((int64_t) blockIdx.x - S.eb0[seg]) * kThreads + threadIdx.x;
+            return i < S.off[seg + 1];
+        }
+
+        __device__ inline double& scal(double* sc, int seg, int slot)
+        {
+            return sc[seg * S_COUNT + slot];
+        }
+
         template <class T>
         __global__ void __launch_bounds__(kThreads)
-            dot_partial_kernel(const T* __restrict__ a, const T* __restrict__ b, int64_t n,
+            dot_partial_kernel(const T* __restrict__ a, const T* __restrict__ b, const SegSpace S,
                                double* __restrict__ partial)
         {
+            const int seg = S.rseg[blockIdx.x];
+            const int64_t lb = (int64_t) blockIdx.x - S.rb0[seg];
+            const int64_t nb = S.rb0[seg + 1] - S.rb0[seg];
+            const int64_t end = S.off[seg + 1];
             double acc = 0.0;
-            const int64_t stride = (int64_t) gridDim.x * kThreads;
-            for (int64_t i = (int64_t) blockIdx.x * kThreads + threadIdx.x; i < n; i += stride)
+            for (int64_t i = S.off[seg] + lb * kThreads + threadIdx.x; i < end; i += nb * kThreads)
                 acc += (double) a[i] * (double) b[i];
             __shared__ double red[kThreads / 64];
             for (int off = 32; off > 0; off >>= 1)
@@ -67,15 +102,16 @@ namespace ehp
             }
         }
 
-        // out = sum of partial[0..np) in a fixed order; hist, when given, gets a copy at
-        // index iters (only while not converged)
+        // per problem (one block each): slot = sum of its partial sums in a fixed order; hist, when
+        // given, gets a copy at index iters of the problem's row (only while not converged)
         __global__ void __launch_bounds__(kThreads)
-            dot_final_kernel(const double* __restrict__ partial, int np, double* __restrict__ sc, int slot,
-                             double* __restrict__ hist)
+            dot_final_kernel(const double* __restrict__ partial, const SegSpace S, double* __restrict__ sc, int slot,
+                             double* __restrict__ hist, int nh)
         {
+            const int seg = blockIdx.x;
             __shared__ double red[kThreads];
             double acc = 0.0;
-            for (int i = threadIdx.x; i < np; i += kThreads)
+            for (int64_t i = S.rb0[seg] + threadIdx.x; i < S.rb0[seg + 1]; i += kThreads)
                 acc += partial[i];
             red[threadIdx.x] = acc;
             __syncthreads();
@@ -85,9 +121,9 @@ namespace ehp
                 __syncthreads();
             }
             if (threadIdx.x == 0) {
-                sc[slot] = red[0];
-                if (hist && sc[S_DONE] == 0.0)
-                    hist[(int) sc[S_ITERS]] = red[0];
+                scal(sc, seg, slot) = red[0];
+                if (hist && scal(sc, seg, S_DONE) == 0.0)
+                    hist[(int64_t) seg * nh + (int) scal(sc, seg, S_ITERS)] = red[0];
             }
         }
 
@@ -117,75 +153,90 @@ namespace ehp
                 y[i] = a * x[i] + y[i];
         }
 
-        // CGLS: x += alpha c over nd elements and s -= alpha q over nr elements
+        // CGLS: y += sign * alpha c per problem (x += alpha c over the domain, s -= alpha q over the
+        // range)
         template <class T>
         __global__ void __launch_bounds__(kThreads)
-            cgls_update_kernel(T* __restrict__ x, const T* __restrict__ c, int64_t nd, T* __restrict__ s,
-                               const T* __restrict__ q, int64_t nr, const double* __restrict__ sc)
+            cgls_update_kernel(T* __restrict__ y, const T* __restrict__ c, const SegSpace S, double* __restrict__ sc,
+                               int negate)
         {
-            if (sc[S_DONE] != 0.0)
+            int seg;
+            int64_t i;
+            if (!seg_elem(S, seg, i) || scal(sc, seg, S_DONE) != 0.0)
                 return;
-            const T alpha = (T) sc[S_ALPHA];
-            const int64_t i = (int64_t) blockIdx.x * kThreads + threadIdx.x;
-            if (i < nd)
-                x[i] += alpha * c[i];
-            if (i < nr)
-                s[i] -= alpha * q[i];
+            const T alpha = (T) scal(sc, seg, S_ALPHA);
+            if (negate)
+                y[i] -= alpha * c[i];
+            else
+                y[i] += alpha * c[i];
         }
 
         // c = r + beta c
         template <class T>
         __global__ void __launch_bounds__(kThreads)
-            cgls_direction_kernel(const T* __restrict__ r, T* __restrict__ c, int64_t n, const double* __restrict__ sc)
+            cgls_direction_kernel(const T* __restrict__ r, T* __restrict__ c, const SegSpace S, double* __restrict__ sc)
         {
-            const T beta = (T) sc[S_BETA];
-            const int64_t i = (int64_t) blockIdx.x * kThreads + threadIdx.x;
-            if (i < n)
-                c[i] = r[i] + beta * c[i];
+            int seg;
+            int64_t i;
+            if (!seg_elem(S, seg, i))
+                return;
+            const T beta = (T) scal(sc, seg, S_BETA);
+            c[i] = r[i] + beta * c[i];
         }
 
         // start of a CGLS iteration: stop when kold < tol (CGLS.cpp:54-58), else count it
-        __global__ void cgls_check_kernel(double* sc, double tol)
+        __global__ void cgls_check_kernel(double* sc, int nseg, double tol)
         {
-            if (sc[S_DONE] == 0.0 && sc[S_KOLD] < tol)
-                sc[S_DONE] = 1.0;
-            if (sc[S_DONE] == 0.0)
-                sc[S_ITERS] += 1.0;
+            const int seg = blockIdx.x * blockDim.x + threadIdx.x;
+            if (seg >= nseg)
+                return;
+            if (scal(sc, seg, S_DONE) == 0.0 && scal(sc, seg, S_KOLD) < tol)
+                scal(sc, seg, S_DONE) = 1.0;
+            if (scal(sc, seg, S_DONE) == 0.0)
+                scal(sc, seg, S_ITERS) += 1.0;
         }
 
-        __global__ void cgls_alpha_kernel(double* sc, double damp)
+        __global__ void cgls_alpha_kernel(double* sc, int nseg, double damp)
         {
-            const double d = damp == 0.0 ? sc[S_DELTA] : sc[S_DELTA] + damp * sc[S_CN];
-            sc[S_ALPHA] = sc[S_KOLD] / d;
+            const int seg = blockIdx.x * blockDim.x + threadIdx.x;
+            if (seg >= nseg)
+                return;
+            const double d = damp == 0.0 ? scal(sc, seg, S_DELTA) : scal(sc, seg, S_DELTA) + damp * scal(sc, seg, S_CN);
+            scal(sc, seg, S_ALPHA) = scal(sc, seg, S_KOLD) / d;
         }
 
-        __global__ void cgls_beta_kernel(double* sc)
+        __global__ void cgls_beta_kernel(double* sc, int nseg)
         {
-            if (sc[S_DONE] != 0.0) {
-                sc[S_BETA] = 0.0;
+            const int seg = blockIdx.x * blockDim.x + threadIdx.x;
+            if (seg >= nseg)
+                return;
+            if (scal(sc, seg, S_DONE) != 0.0) {
+                scal(sc, seg, S_BETA) = 0.0;
                 return;
             }
-            sc[S_BETA] = sc[S_K] / sc[S_KOLD];
-            sc[S_KOLD] = sc[S_K];
+            scal(sc, seg, S_BETA) = scal(sc, seg, S_K) / scal(sc, seg, S_KOLD);
+            scal(sc, seg, S_KOLD) = scal(sc, seg, S_K);
         }
 
-        __global__ void set_scalar_kernel(double* sc, int slot, double v)
+        __global__ void set_scalar_kernel(double* sc, int nseg, int slot, double v)
         {
-            sc[slot] = v;
+            const int seg = blockIdx.x * blockDim.x + threadIdx.x;
+            if (seg < nseg)
+                scal(sc, seg, slot) = v;
         }
 
         // APGD: x = prox(y - mu grad), prox the identity (sigma < 0) or L1 soft thresholding with
         // threshold mu * sigma (ProximalL1.cpp:32-43)
         template <class T>
         __global__ void __launch_bounds__(kThreads)
-            apgd_prox_kernel(const T* __restrict__ y, const T* __restrict__ grad, T mu, T sigma, T* __restrict__ x,
-                             int64_t n, const double* __restrict__ sc)
+            apgd_prox_kernel(const T* __restrict__ y, const T* __restrict__ grad, T sigma, T* __restrict__ x,
+                             const SegSpace S, double* __restrict__ sc)
         {
-            if (sc[S_DONE] != 0.0)
+            int seg;
+            int64_t i;
+            if (!seg_elem(S, seg, i) || scal(sc, seg, S_DONE) != 0.0)
                 return;
-            const int64_t i = (int64_t) blockIdx.x * kThreads + threadIdx.x;
-            if (i >= n)
-                return;
+            const T mu = (T) scal(sc, seg, S_MU);
             const T z = y[i] - mu * grad[i];
             if (sigma < T(0)) {
                 x[i] = z;
@@ -199,28 +250,28 @@ namespace ehp
 
         // APGD momentum scalars in the data type, as elsa computes them (APGD.cpp:74-78)
         template <class T>
-        __global__ void apgd_t_kernel(double* sc)
+        __global__ void apgd_t_kernel(double* sc, int nseg)
         {
-            if (sc[S_DONE] != 0.0)
+            const int seg = blockIdx.x * blockDim.x + threadIdx.x;
+            if (seg >= nseg || scal(sc, seg, S_DONE) != 0.0)
                 return;
-            const T tprev = (T) sc[S_T];
+            const T tprev = (T) scal(sc, seg, S_T);
             const T t = (T(1) + sqrt(T(1) + T(4) * tprev * tprev)) / T(2);
-            sc[S_COEF] = (double) ((tprev - T(1)) / t);
-            sc[S_T] = (double) t;
+            scal(sc, seg, S_COEF) = (double) ((tprev - T(1)) / t);
+            scal(sc, seg, S_T) = (double) t;
         }
 
         // y = x + coef (x - xprev); xprev = x
         template <class T>
         __global__ void __launch_bounds__(kThreads)
-            apgd_momentum_kernel(const T* __restrict__ x, T* __restrict__ xprev, T* __restrict__ y, int64_t n,
-                                 const double* __restrict__ sc)
+            apgd_momentum_kernel(const T* __restrict__ x, T* __restrict__ xprev, T* __restrict__ y, const SegSpace S,
+                                 double* __restrict__ sc)
         {
-            if (sc[S_DONE] != 0.0)
+            int seg;
+            int64_t i;
+            if (!seg_elem(S, seg, i) || scal(sc, seg, S_DONE) != 0.0)
                 return;
-            const int64_t i = (int64_t) blockIdx.x * kThreads + threadIdx.x;
-            if (i >= n)
-                return;
-            const T coef = (T) sc[S_COEF];
+            const T coef = (T) scal(sc, seg, S_COEF);
             const T xi = x[i];
             y[i] = xi + coef * (xi - xprev[i]);
             xprev[i] = xi;
@@ -228,23 +279,25 @@ namespace ehp
 
         // APGD convergence: ||grad||^2 <= epsilon ends the method after this iteration
         // (APGD.cpp:83-87)
-        __global__ void apgd_check_kernel(double* sc, double eps)
+        __global__ void apgd_check_kernel(double* sc, int nseg, double eps)
         {
-            if (sc[S_DONE] != 0.0)
+            const int seg = blockIdx.x * blockDim.x + threadIdx.x;
+            if (seg >= nseg || scal(sc, seg, S_DONE) != 0.0)
                 return;
-            sc[S_ITERS] += 1.0;
-            if (sc[S_SN] <= eps)
-                sc[S_DONE] = 1.0;
+            scal(sc, seg, S_ITERS) += 1.0;
+            if (scal(sc, seg, S_SN) <= eps)
+                scal(sc, seg, S_DONE) = 1.0;
         }
 
-        // u = u / sqrt(sc[slot])
+        // u = u / sqrt(||u||^2) per problem
         template <class T>
-        __global__ void __launch_bounds__(kThreads) normalize_kernel(T* __restrict__ u, int64_t n, const double* __restrict__ sc,
-                                                                   int slot)
+        __global__ void __launch_bounds__(kThreads)
+            normalize_kernel(T* __restrict__ u, const SegSpace S, double* __restrict__ sc, int slot)
         {
-            const int64_t i = (int64_t) blockIdx.x * kThreads + threadIdx.x;
-            if (i < n)
-                u[i] = (T) ((double) u[i] / sqrt(sc[slot]));
+            int seg;
+            int64_t i;
+            if (seg_elem(S, seg, i))
+                u[i] = (T) ((double) u[i] / sqrt(scal(sc, seg, slot)));
         }
 
         inline unsigned blocks(int64_t n)
@@ -252,14 +305,18 @@ namespace ehp
             return (unsigned) ((n + kThreads - 1) / kThreads);
         }
 
-        template <class T>
-        hipError_t dot(const T* a, const T* b, int64_t n, double* partial, double* sc, int slot, double* hist,
-                       hipStream_t s)
+        inline unsigned seg_blocks(int nseg)
         {
-            int64_t nb = (n + kThreads - 1) / kThreads;
-            nb = nb > kRedBlocks ? kRedBlocks : (nb < 1 ? 1 : nb);
-            hipLaunchKernelGGL((dot_partial_kernel<T>), dim3((unsigned) nb), dim3(kThreads), 0, s, a, b, n, partial);
-            hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kThreads), 0, s, partial, (int) nb, sc, slot, hist);
+            return (unsigned) ((nseg + 63) / 64);
+        }
+
+        template <class T>
+        hipError_t dot(const T* a, const T* b, const SegSpace& S, int nseg, double* partial, double* sc, int slot,
+                       double* hist, int nh, hipStream_t s)
+        {
+            hipLaunchKernelGGL((dot_partial_kernel<T>), dim3((unsigned) S.nrb), dim3(kThreads), 0, s, a, b, S, partial);
+            hipLaunchKernelGGL(dot_final_kernel, dim3((unsigned) nseg), dim3(kThreads), 0, s, partial, S, sc, slot,
+                               hist, nh);
             return hipGetLastError();
         }
 
@@ -280,6 +337,62 @@ namespace ehp
                 return e;
             }
         };
+
+        // The block tables of one vector space with segment offsets off[0..nseg], uploaded with s
+        // (synchronously: the host tables are local).
+        hipError_t make_space(const int64_t* off, int nseg, Scratch& m, SegSpace& S, hipStream_t s)
+        {
+            std::vector<int64_t> eb0(nseg + 1), rb0(nseg + 1);
+            std::vector<int> eseg, rseg;
+            for (int j = 0; j < nseg; ++j) {
+                const int64_t len = off[j + 1] - off[j];
+                const int64_t ne = (len + kThreads - 1) / kThreads;
+                int64_t nr = (len + 8 * kThreads - 1) / (8 * kThreads);
+                nr = nr < 1 ? 1 : (nr > kRedBlocks ? kRedBlocks : nr);
+                eb0[j] = (int64_t) eseg.size();
+                rb0[j] = (int64_t) rseg.size();
+                eseg.insert(eseg.end(), (size_t) ne, j);
+                rseg.insert(rseg.end(), (size_t) nr, j);
+            }
+            eb0[nseg] = (int64_t) eseg.size();
+            rb0[nseg] = (int64_t) rseg.size();
+            if (eseg.size() > 0x7fffffffu)
+                return hipErrorInvalidValue;
+            int64_t *d_off, *d_eb0, *d_rb0;
+            int *d_eseg, *d_rseg;
+            const size_t n64 = (size_t) (nseg + 1) * sizeof(int64_t);
+            hipError_t e = m.alloc(&d_off, n64);
+            if (e == hipSuccess)
+                e = m.alloc(&d_eb0, n64);
+            if (e == hipSuccess)
+                e = m.alloc(&d_rb0, n64);
+            if (e == hipSuccess)
+                e = m.alloc(&d_eseg, (eseg.size() + 1) * sizeof(int));
+            if (e == hipSuccess)
+                e = m.alloc(&d_rseg, (rseg.size() + 1) * sizeof(int));
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(d_off, off, n64, hipMemcpyHostToDevice, s);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(d_eb0, eb0.data(), n64, hipMemcpyHostToDevice, s);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(d_rb0, rb0.data(), n64, hipMemcpyHostToDevice, s);
+            if (e == hipSuccess && !eseg.empty())
+                e = hipMemcpyAsync(d_eseg, eseg.data(), eseg.size() * sizeof(int), hipMemcpyHostToDevice, s);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(d_rseg, rseg.data(), rseg.size() * sizeof(int), hipMemcpyHostToDevice, s);
+            if (e == hipSuccess)
+                e = hipStreamSynchronize(s);
+            if (e != hipSuccess)
+                return e;
+            S.off = d_off;
+            S.eb0 = d_eb0;
+            S.rb0 = d_rb0;
+            S.eseg = d_eseg;
+            S.rseg = d_rseg;
+            S.neb = (int64_t) eseg.size();
+            S.nrb = (int64_t) rseg.size();
+            return hipSuccess;
+        }
 
         // A stream of the solver's own (default priority) ordered after the caller's stream.
         struct SolveStream {
@@ -345,190 +458,243 @@ namespace ehp
             return e;
         }
 
-        template <class T>
-        int cgls(ehp_op* op, int64_t nd, int64_t nr, const T* b, T* x, int use_x0, int iterations, double eps,
-                 double tol, bool graph, double* residuals, int* iters_done, hipStream_t caller)
-        {
+        // common set-up of a solve: its stream, workspace and block tables
+        struct Solve {
+            Scratch m;       // declared first: freed after the stream has drained
             SolveStream ss;
-            hipError_t e = ss.init(caller);
+            SegSpace D{}, R{};
+            double *partial = nullptr, *sc = nullptr;
+            int nseg = 0;
+            int64_t nd = 0, nr = 0;
+
+            int init(const SolveOps& A, hipStream_t caller, const char* what)
+            {
+                nseg = A.nseg;
+                nd = A.doff[nseg];
+                nr = A.roff[nseg];
+                hipError_t e = ss.init(caller);
+                if (e == hipSuccess)
+                    e = make_space(A.doff, nseg, m, D, ss.s);
+                if (e == hipSuccess)
+                    e = make_space(A.roff, nseg, m, R, ss.s);
+                if (e != hipSuccess)
+                    return ehp_solver_fail(EHP_ERR_HIP, hipGetErrorString(e));
+                const int64_t np = D.nrb > R.nrb ? D.nrb : R.nrb;
+                if (m.alloc(&partial, (size_t) np * sizeof(double)) != hipSuccess ||
+                    m.alloc(&sc, (size_t) nseg * S_COUNT * sizeof(double)) != hipSuccess)
+                    return ehp_solver_fail(EHP_ERR_ALLOC, what);
+                if (launch_fill<double>(sc, (int64_t) nseg * S_COUNT, 0.0, ss.s) != hipSuccess)
+                    return ehp_solver_fail(EHP_ERR_HIP, "fill");
+                return EHP_OK;
+            }
+        };
+
+        // device scalars and history rows back to the host
+        int fetch(hipStream_t s, const double* sc, int nseg, const double* hist, int nh, std::vector<double>& hsc,
+                  std::vector<double>& hh)
+        {
+            hsc.resize((size_t) nseg * S_COUNT);
+            hh.resize((size_t) nseg * nh);
+            hipError_t e = hipMemcpyAsync(hsc.data(), sc, hsc.size() * sizeof(double), hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess && hist)
+                e = hipMemcpyAsync(hh.data(), hist, hh.size() * sizeof(double), hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess)
+                e = hipStreamSynchronize(s);
             if (e != hipSuccess)
                 return ehp_solver_fail(EHP_ERR_HIP, hipGetErrorString(e));
-            hipStream_t s = ss.s;
-            Scratch m;
-            T *r, *c, *sv, *q;
-            double *partial, *sc, *hist;
+            return EHP_OK;
+        }
+
+        template <class T>
+        int cgls(const SolveOps& A, const T* b, T* x, int use_x0, int iterations, double eps, double tol, bool graph,
+                 double* residuals, int* iters_done, hipStream_t caller)
+        {
+            Solve sv;
+            int rc = sv.init(A, caller, "hipMalloc of the CGLS workspace failed");
+            if (rc != EHP_OK)
+                return rc;
+            hipStream_t s = sv.ss.s;
+            const int nseg = sv.nseg;
+            const int64_t nd = sv.nd, nr = sv.nr;
+            const SegSpace &D = sv.D, &R = sv.R;
+            double *partial = sv.partial, *sc = sv.sc, *hist;
+            T *r, *c, *sres, *q;
             const int nh = iterations + 1;
+            Scratch& m = sv.m;
             if (m.alloc(&r, nd * sizeof(T)) != hipSuccess || m.alloc(&c, nd * sizeof(T)) != hipSuccess ||
-                m.alloc(&sv, nr * sizeof(T)) != hipSuccess || m.alloc(&q, nr * sizeof(T)) != hipSuccess ||
-                m.alloc(&partial, kRedBlocks * sizeof(double)) != hipSuccess ||
-                m.alloc(&sc, S_COUNT * sizeof(double)) != hipSuccess || m.alloc(&hist, nh * sizeof(double)) != hipSuccess)
+                m.alloc(&sres, nr * sizeof(T)) != hipSuccess || m.alloc(&q, nr * sizeof(T)) != hipSuccess ||
+                m.alloc(&hist, (size_t) nseg * nh * sizeof(double)) != hipSuccess)
                 return ehp_solver_fail(EHP_ERR_ALLOC, "hipMalloc of the CGLS workspace failed");
             const double damp = eps * eps;
-            int rc = EHP_OK;
             auto ok = [&](hipError_t err) {
                 if (err != hipSuccess && rc == EHP_OK)
                     rc = ehp_solver_fail(EHP_ERR_HIP, hipGetErrorString(err));
                 return rc == EHP_OK;
             };
-            ok(launch_fill<double>(sc, S_COUNT, 0.0, s));
-            ok(launch_fill<double>(hist, nh, -1.0, s));
+            ok(launch_fill<double>(hist, (int64_t) nseg * nh, -1.0, s));
             // s = b - A x (or b), r = A^T s - damp x (CGLS.cpp:30-45)
             if (use_x0) {
                 if (rc == EHP_OK)
-                    rc = ehp_forward(op, x, sv, s);
-                hipLaunchKernelGGL((rsub_kernel<T>), dim3(blocks(nr)), dim3(kThreads), 0, s, b, sv, nr);
+                    rc = A.fwd(x, sres, s);
+                hipLaunchKernelGGL((rsub_kernel<T>), dim3(blocks(nr)), dim3(kThreads), 0, s, b, sres, nr);
             } else {
                 ok(launch_fill<T>(x, nd, T(0), s));
-                hipLaunchKernelGGL((copy_kernel<T>), dim3(blocks(nr)), dim3(kThreads), 0, s, b, sv, nr);
+                hipLaunchKernelGGL((copy_kernel<T>), dim3(blocks(nr)), dim3(kThreads), 0, s, b, sres, nr);
             }
             if (rc == EHP_OK)
-                rc = ehp_adjoint(op, sv, r, s);
+                rc = A.adj(sres, r, s);
             if (use_x0 && damp != 0.0)
                 hipLaunchKernelGGL((axpy_kernel<T>), dim3(blocks(nd)), dim3(kThreads), 0, s, (T) -damp, x, r, nd);
             hipLaunchKernelGGL((copy_kernel<T>), dim3(blocks(nd)), dim3(kThreads), 0, s, r, c, nd);
-            ok(dot<T>(r, r, nd, partial, sc, S_KOLD, nullptr, s));
-            ok(dot<T>(sv, sv, nr, partial, sc, S_SN, hist, s)); // hist[0] = ||s||^2 at the start
+            ok(dot<T>(r, r, D, nseg, partial, sc, S_KOLD, nullptr, nh, s));
+            ok(dot<T>(sres, sres, R, nseg, partial, sc, S_SN, hist, nh, s)); // hist[0] = ||s||^2 at the start
             if (rc != EHP_OK)
                 return rc;
 
-            const int64_t nmax = nd > nr ? nd : nr;
             auto body = [&]() -> int {
-                hipLaunchKernelGGL(cgls_check_kernel, dim3(1), dim3(1), 0, s, sc, tol);
-                int r2 = ehp_forward(op, c, q, s); // q = A c
+                hipLaunchKernelGGL(cgls_check_kernel, dim3(seg_blocks(nseg)), dim3(64), 0, s, sc, nseg, tol);
+                int r2 = A.fwd(c, q, s); // q = A c
                 if (r2 != EHP_OK)
                     return r2;
-                if (dot<T>(q, q, nr, partial, sc, S_DELTA, nullptr, s) != hipSuccess)
+                if (dot<T>(q, q, R, nseg, partial, sc, S_DELTA, nullptr, nh, s) != hipSuccess)
                     return EHP_ERR_HIP;
-                if (damp != 0.0 && dot<T>(c, c, nd, partial, sc, S_CN, nullptr, s) != hipSuccess)
+                if (damp != 0.0 && dot<T>(c, c, D, nseg, partial, sc, S_CN, nullptr, nh, s) != hipSuccess)
                     return EHP_ERR_HIP;
-                hipLaunchKernelGGL(cgls_alpha_kernel, dim3(1), dim3(1), 0, s, sc, damp);
-                hipLaunchKernelGGL((cgls_update_kernel<T>), dim3(blocks(nmax)), dim3(kThreads), 0, s, x, c, nd, sv, q,
-                                   nr, sc);
-                r2 = ehp_adjoint(op, sv, r, s); // r = A^T s
+                hipLaunchKernelGGL(cgls_alpha_kernel, dim3(seg_blocks(nseg)), dim3(64), 0, s, sc, nseg, damp);
+                hipLaunchKernelGGL((cgls_update_kernel<T>), dim3((unsigned) D.neb), dim3(kThreads), 0, s, x, c, D, sc, 0);
+                hipLaunchKernelGGL((cgls_update_kernel<T>), dim3((unsigned) R.neb), dim3(kThreads), 0, s, sres, q, R, sc,
+                                   1);
+                r2 = A.adj(sres, r, s); // r = A^T s
                 if (r2 != EHP_OK)
                     return r2;
                 if (damp != 0.0)
                     hipLaunchKernelGGL((axpy_kernel<T>), dim3(blocks(nd)), dim3(kThreads), 0, s, (T) -damp, x, r, nd);
-                if (dot<T>(r, r, nd, partial, sc, S_K, nullptr, s) != hipSuccess)
+                if (dot<T>(r, r, D, nseg, partial, sc, S_K, nullptr, nh, s) != hipSuccess)
                     return EHP_ERR_HIP;
-                hipLaunchKernelGGL(cgls_beta_kernel, dim3(1), dim3(1), 0, s, sc);
-                hipLaunchKernelGGL((cgls_direction_kernel<T>), dim3(blocks(nd)), dim3(kThreads), 0, s, r, c, nd, sc);
+                hipLaunchKernelGGL(cgls_beta_kernel, dim3(seg_blocks(nseg)), dim3(64), 0, s, sc, nseg);
+                hipLaunchKernelGGL((cgls_direction_kernel<T>), dim3((unsigned) D.neb), dim3(kThreads), 0, s, r, c, D, sc);
                 // ||s||^2 of this iteration, recorded at index iters while not converged
-                if (dot<T>(sv, sv, nr, partial, sc, S_SN, hist, s) != hipSuccess)
+                if (dot<T>(sres, sres, R, nseg, partial, sc, S_SN, hist, nh, s) != hipSuccess)
                     return EHP_ERR_HIP;
                 return hipGetLastError() == hipSuccess ? EHP_OK : EHP_ERR_HIP;
             };
-            e = iterate(s, iterations, graph, body);
+            hipError_t e = iterate(s, iterations, graph, body);
             if (e != hipSuccess)
                 return rc != EHP_OK ? rc : ehp_solver_fail(EHP_ERR_HIP, hipGetErrorString(e));
-            // the last check: elsa returns before iteration `iterations` if kold < tol
-            double hsc[S_COUNT];
-            std::vector<double> hh(nh);
-            e = hipMemcpyAsync(hsc, sc, sizeof(hsc), hipMemcpyDeviceToHost, s);
-            if (e == hipSuccess)
-                e = hipMemcpyAsync(hh.data(), hist, nh * sizeof(double), hipMemcpyDeviceToHost, s);
-            if (e == hipSuccess)
-                e = hipStreamSynchronize(s);
-            if (e != hipSuccess)
-                return ehp_solver_fail(EHP_ERR_HIP, hipGetErrorString(e));
-            if (iters_done)
-                *iters_done = (int) hsc[S_ITERS];
-            if (residuals)
-                for (int i = 0; i < nh; ++i)
-                    residuals[i] = hh[i] >= 0.0 ? std::sqrt(hh[i]) : -1.0;
+            std::vector<double> hsc, hh;
+            rc = fetch(s, sc, nseg, hist, nh, hsc, hh);
+            if (rc != EHP_OK)
+                return rc;
+            for (int j = 0; j < nseg; ++j) {
+                if (iters_done)
+                    iters_done[j] = (int) hsc[(size_t) j * S_COUNT + S_ITERS];
+                if (residuals)
+                    for (int i = 0; i < nh; ++i) {
+                        const double h = hh[(size_t) j * nh + i];
+                        residuals[(size_t) j * nh + i] = h >= 0.0 ? std::sqrt(h) : -1.0;
+                    }
+            }
             return EHP_OK;
         }
 
+        // L[j] of every problem: 5 (niters) power iterations on A_j^T A_j from the ones vector
         template <class T>
-        int power_iterations(ehp_op* op, int64_t nd, int64_t nr, int niters, double* L, hipStream_t s, Scratch& m,
-                             double* partial, double* sc)
+        int power_iterations(const SolveOps& A, Solve& sv, int niters, double* L)
         {
-            T *u, *t;
-            if (m.alloc(&u, nd * sizeof(T)) != hipSuccess || m.alloc(&t, nr * sizeof(T)) != hipSuccess)
+            hipStream_t s = sv.ss.s;
+            const int nseg = sv.nseg;
+            T *u, *t, *v;
+            if (sv.m.alloc(&u, sv.nd * sizeof(T)) != hipSuccess || sv.m.alloc(&t, sv.nr * sizeof(T)) != hipSuccess ||
+                sv.m.alloc(&v, sv.nd * sizeof(T)) != hipSuccess)
                 return ehp_solver_fail(EHP_ERR_ALLOC, "hipMalloc of the power-iteration workspace failed");
-            if (launch_fill<T>(u, nd, T(1), s) != hipSuccess)
+            if (launch_fill<T>(u, sv.nd, T(1), s) != hipSuccess)
                 return ehp_solver_fail(EHP_ERR_HIP, "fill");
             // u = A^T A u; u /= ||u|| (PowerIterations.cpp:18-23)
             for (int i = 0; i < niters; ++i) {
-                int rc = ehp_forward(op, u, t, s);
+                int rc = A.fwd(u, t, s);
                 if (rc == EHP_OK)
-                    rc = ehp_adjoint(op, t, u, s);
+                    rc = A.adj(t, u, s);
                 if (rc != EHP_OK)
                     return rc;
-                if (dot<T>(u, u, nd, partial, sc, S_TMP, nullptr, s) != hipSuccess)
+                if (dot<T>(u, u, sv.D, nseg, sv.partial, sv.sc, S_TMP, nullptr, 0, s) != hipSuccess)
                     return ehp_solver_fail(EHP_ERR_HIP, "dot");
-                hipLaunchKernelGGL((normalize_kernel<T>), dim3(blocks(nd)), dim3(kThreads), 0, s, u, nd, sc, S_TMP);
+                hipLaunchKernelGGL((normalize_kernel<T>), dim3((unsigned) sv.D.neb), dim3(kThreads), 0, s, u, sv.D, sv.sc,
+                                   (int) S_TMP);
             }
             // u . (A^T A u) / ||u||  (PowerIterations.cpp:25)
-            T* v;
-            if (m.alloc(&v, nd * sizeof(T)) != hipSuccess)
-                return ehp_solver_fail(EHP_ERR_ALLOC, "hipMalloc of the power-iteration workspace failed");
-            int rc = ehp_forward(op, u, t, s);
+            int rc = A.fwd(u, t, s);
             if (rc == EHP_OK)
-                rc = ehp_adjoint(op, t, v, s);
+                rc = A.adj(t, v, s);
             if (rc != EHP_OK)
                 return rc;
             // (S_KOLD is free here and takes ||u||^2)
-            if (dot<T>(u, v, nd, partial, sc, S_TMP, nullptr, s) != hipSuccess ||
-                dot<T>(u, u, nd, partial, sc, S_KOLD, nullptr, s) != hipSuccess)
+            if (dot<T>(u, v, sv.D, nseg, sv.partial, sv.sc, S_TMP, nullptr, 0, s) != hipSuccess ||
+                dot<T>(u, u, sv.D, nseg, sv.partial, sv.sc, S_KOLD, nullptr, 0, s) != hipSuccess)
                 return ehp_solver_fail(EHP_ERR_HIP, "dot");
-            double h[S_COUNT];
-            hipError_t e = hipMemcpyAsync(h, sc, sizeof(h), hipMemcpyDeviceToHost, s);
-            if (e == hipSuccess)
-                e = hipStreamSynchronize(s);
-            if (e != hipSuccess)
-                return ehp_solver_fail(EHP_ERR_HIP, hipGetErrorString(e));
-            *L = h[S_TMP] / std::sqrt(h[S_KOLD]);
+            std::vector<double> hsc, hh;
+            rc = fetch(s, sv.sc, nseg, nullptr, 0, hsc, hh);
+            if (rc != EHP_OK)
+                return rc;
+            for (int j = 0; j < nseg; ++j)
+                L[j] = hsc[(size_t) j * S_COUNT + S_TMP] / std::sqrt(hsc[(size_t) j * S_COUNT + S_KOLD]);
             return EHP_OK;
         }
 
         template <class T>
-        int apgd(ehp_op* op, int64_t nd, int64_t nr, const T* b, T* x, int use_x0, int iterations, double mu_in,
-                 double sigma, double epsilon, bool graph, double* grad_norms, int* iters_done, double* mu_used,
-                 hipStream_t caller)
+        int apgd(const SolveOps& A, const T* b, T* x, int use_x0, int iterations, const double* mu_in, double sigma,
+                 double epsilon, bool graph, double* grad_norms, int* iters_done, double* mu_used, hipStream_t caller)
         {
-            SolveStream ss;
-            hipError_t e = ss.init(caller);
-            if (e != hipSuccess)
-                return ehp_solver_fail(EHP_ERR_HIP, hipGetErrorString(e));
-            hipStream_t s = ss.s;
-            Scratch m;
-            double *partial, *sc, *hist;
+            Solve sv;
+            int rc = sv.init(A, caller, "hipMalloc of the APGD workspace failed");
+            if (rc != EHP_OK)
+                return rc;
+            hipStream_t s = sv.ss.s;
+            const int nseg = sv.nseg;
+            const int64_t nd = sv.nd, nr = sv.nr;
+            const SegSpace& D = sv.D;
+            double *partial = sv.partial, *sc = sv.sc, *hist;
             const int nh = iterations + 1;
-            if (m.alloc(&partial, kRedBlocks * sizeof(double)) != hipSuccess ||
-                m.alloc(&sc, S_COUNT * sizeof(double)) != hipSuccess || m.alloc(&hist, nh * sizeof(double)) != hipSuccess)
+            Scratch& m = sv.m;
+            if (m.alloc(&hist, (size_t) nseg * nh * sizeof(double)) != hipSuccess)
                 return ehp_solver_fail(EHP_ERR_ALLOC, "hipMalloc of the APGD workspace failed");
-            if (launch_fill<double>(sc, S_COUNT, 0.0, s) != hipSuccess ||
-                launch_fill<double>(hist, nh, -1.0, s) != hipSuccess)
+            if (launch_fill<double>(hist, (int64_t) nseg * nh, -1.0, s) != hipSuccess)
                 return ehp_solver_fail(EHP_ERR_HIP, "fill");
-            // mu = 0.45 / L with L from power iterations on A^T A (APGD.cpp:19-27)
-            double mu = mu_in;
-            if (!(mu > 0.0)) {
-                double L = 0.0;
-                const int rc = power_iterations<T>(op, nd, nr, 5, &L, s, m, partial, sc);
+            // mu = 0.45 / L with L from power iterations on A^T A (APGD.cpp:19-27), per problem
+            std::vector<double> mu(nseg), L(nseg, 0.0);
+            bool need_L = false;
+            for (int j = 0; j < nseg; ++j) {
+                mu[j] = mu_in ? mu_in[j] : 0.0;
+                need_L = need_L || !(mu[j] > 0.0);
+            }
+            if (need_L) {
+                rc = power_iterations<T>(A, sv, 5, L.data());
                 if (rc != EHP_OK)
                     return rc;
-                mu = (double) (T(0.45) / (T) L);
             }
-            if (mu_used)
-                *mu_used = mu;
+            for (int j = 0; j < nseg; ++j) {
+                if (!(mu[j] > 0.0))
+                    mu[j] = (double) (T(0.45) / (T) L[j]);
+                if (mu_used)
+                    mu_used[j] = mu[j];
+                hipLaunchKernelGGL(set_scalar_kernel, dim3(1), dim3(1), 0, s, sc + (size_t) j * S_COUNT, 1, (int) S_MU,
+                                   mu[j]);
+            }
             T *xprev, *y, *grad, *atb, *ay;
             if (m.alloc(&xprev, nd * sizeof(T)) != hipSuccess || m.alloc(&y, nd * sizeof(T)) != hipSuccess ||
                 m.alloc(&grad, nd * sizeof(T)) != hipSuccess || m.alloc(&atb, nd * sizeof(T)) != hipSuccess ||
                 m.alloc(&ay, nr * sizeof(T)) != hipSuccess)
                 return ehp_solver_fail(EHP_ERR_ALLOC, "hipMalloc of the APGD workspace failed");
-            int rc = EHP_OK;
             if (!use_x0 && launch_fill<T>(x, nd, T(0), s) != hipSuccess)
                 return ehp_solver_fail(EHP_ERR_HIP, "fill");
             hipLaunchKernelGGL((copy_kernel<T>), dim3(blocks(nd)), dim3(kThreads), 0, s, x, xprev, nd);
             hipLaunchKernelGGL((copy_kernel<T>), dim3(blocks(nd)), dim3(kThreads), 0, s, x, y, nd);
-            hipLaunchKernelGGL(set_scalar_kernel, dim3(1), dim3(1), 0, s, sc, (int) S_T, 1.0);
-            rc = ehp_adjoint(op, b, atb, s); // A^T b
+            hipLaunchKernelGGL(set_scalar_kernel, dim3(seg_blocks(nseg)), dim3(64), 0, s, sc, nseg, (int) S_T, 1.0);
+            rc = A.adj(b, atb, s); // A^T b
             // grad = A^T (A y) - A^T b (APGD.cpp:55-60)
             auto gradient = [&]() -> int {
-                int r2 = ehp_forward(op, y, ay, s);
+                int r2 = A.fwd(y, ay, s);
                 if (r2 == EHP_OK)
-                    r2 = ehp_adjoint(op, ay, grad, s);
+                    r2 = A.adj(ay, grad, s);
                 if (r2 == EHP_OK)
                     hipLaunchKernelGGL((axpy_kernel<T>), dim3(blocks(nd)), dim3(kThreads), 0, s, T(-1), atb, grad, nd);
                 return r2;
@@ -537,76 +703,68 @@ namespace ehp
                 rc = gradient();
             if (rc != EHP_OK)
                 return rc;
-            const T tmu = (T) mu, tsig = sigma >= 0.0 ? (T) sigma : T(-1);
+            const T tsig = sigma >= 0.0 ? (T) sigma : T(-1);
             auto body = [&]() -> int {
-                hipLaunchKernelGGL((apgd_prox_kernel<T>), dim3(blocks(nd)), dim3(kThreads), 0, s, y, grad, tmu, tsig, x,
-                                   nd, sc);
-                hipLaunchKernelGGL((apgd_t_kernel<T>), dim3(1), dim3(1), 0, s, sc);
-                hipLaunchKernelGGL((apgd_momentum_kernel<T>), dim3(blocks(nd)), dim3(kThreads), 0, s, x, xprev, y, nd, sc);
-                if (dot<T>(grad, grad, nd, partial, sc, S_SN, hist, s) != hipSuccess)
+                hipLaunchKernelGGL((apgd_prox_kernel<T>), dim3((unsigned) D.neb), dim3(kThreads), 0, s, y, grad, tsig,
+                                   x, D, sc);
+                hipLaunchKernelGGL((apgd_t_kernel<T>), dim3(seg_blocks(nseg)), dim3(64), 0, s, sc, nseg);
+                hipLaunchKernelGGL((apgd_momentum_kernel<T>), dim3((unsigned) D.neb), dim3(kThreads), 0, s, x, xprev, y,
+                                   D, sc);
+                if (dot<T>(grad, grad, D, nseg, partial, sc, S_SN, hist, nh, s) != hipSuccess)
                     return EHP_ERR_HIP;
-                hipLaunchKernelGGL(apgd_check_kernel, dim3(1), dim3(1), 0, s, sc, epsilon);
+                hipLaunchKernelGGL(apgd_check_kernel, dim3(seg_blocks(nseg)), dim3(64), 0, s, sc, nseg, epsilon);
                 const int r2 = gradient();
                 if (r2 != EHP_OK)
                     return r2;
                 return hipGetLastError() == hipSuccess ? EHP_OK : EHP_ERR_HIP;
             };
-            e = iterate(s, iterations, graph, body);
+            hipError_t e = iterate(s, iterations, graph, body);
             if (e != hipSuccess)
                 return ehp_solver_fail(EHP_ERR_HIP, hipGetErrorString(e));
-            double hsc[S_COUNT];
-            std::vector<double> hh(nh);
-            e = hipMemcpyAsync(hsc, sc, sizeof(hsc), hipMemcpyDeviceToHost, s);
-            if (e == hipSuccess)
-                e = hipMemcpyAsync(hh.data(), hist, nh * sizeof(double), hipMemcpyDeviceToHost, s);
-            if (e == hipSuccess)
-                e = hipStreamSynchronize(s);
-            if (e != hipSuccess)
-                return ehp_solver_fail(EHP_ERR_HIP, hipGetErrorString(e));
-            if (iters_done)
-                *iters_done = (int) hsc[S_ITERS];
-            if (grad_norms)
-                for (int i = 0; i < nh; ++i)
-                    grad_norms[i] = hh[i];
+            std::vector<double> hsc, hh;
+            rc = fetch(s, sc, nseg, hist, nh, hsc, hh);
+            if (rc != EHP_OK)
+                return rc;
+            for (int j = 0; j < nseg; ++j) {
+                if (iters_done)
+                    iters_done[j] = (int) hsc[(size_t) j * S_COUNT + S_ITERS];
+                if (grad_norms)
+                    for (int i = 0; i < nh; ++i)
+                        grad_norms[(size_t) j * nh + i] = hh[(size_t) j * nh + i];
+            }
             return EHP_OK;
         }
     } // namespace
 
-    // dispatch helpers for the C-ABI (ehp_capi.hip owns the operator)
-    int solver_cgls(ehp_op* op, int dtype, int64_t nd, int64_t nr, const void* b, void* x, int use_x0, int iterations,
-                    double eps, double tol, bool graph, double* residuals, int* iters_done, hipStream_t s)
+    int solve_cgls(const SolveOps& A, int dtype, const void* b, void* x, int use_x0, int iterations, double eps,
+                   double tol, bool graph, double* residuals, int* iters_done, hipStream_t s)
     {
         if (dtype == EHP_F64)
-            return cgls<double>(op, nd, nr, (const double*) b, (double*) x, use_x0, iterations, eps, tol, graph,
-                                residuals, iters_done, s);
-        return cgls<float>(op, nd, nr, (const float*) b, (float*) x, use_x0, iterations, eps, tol, graph, residuals,
-                           iters_done, s);
+            return cgls<double>(A, (const double*) b, (double*) x, use_x0, iterations, eps, tol, graph, residuals,
+                                iters_done, s);
+        return cgls<float>(A, (const float*) b, (float*) x, use_x0, iterations, eps, tol, graph, residuals, iters_done,
+                           s);
     }
 
-    int solver_apgd(ehp_op* op, int dtype, int64_t nd, int64_t nr, const void* b, void* x, int use_x0, int iterations,
-                    double mu, double sigma, double epsilon, bool graph, double* grad_norms, int* iters_done,
-                    double* mu_used, hipStream_t s)
+    int solve_apgd(const SolveOps& A, int dtype, const void* b, void* x, int use_x0, int iterations, const double* mu,
+                   double sigma, double epsilon, bool graph, double* grad_norms, int* iters_done, double* mu_used,
+                   hipStream_t s)
     {
         if (dtype == EHP_F64)
-            return apgd<double>(op, nd, nr, (const double*) b, (double*) x, use_x0, iterations, mu, sigma, epsilon,
-                                graph, grad_norms, iters_done, mu_used, s);
-        return apgd<float>(op, nd, nr, (const float*) b, (float*) x, use_x0, iterations, mu, sigma, epsilon, graph,
-                           grad_norms, iters_done, mu_used, s);
+            return apgd<double>(A, (const double*) b, (double*) x, use_x0, iterations, mu, sigma, epsilon, graph,
+                                grad_norms, iters_done, mu_used, s);
+        return apgd<float>(A, (const float*) b, (float*) x, use_x0, iterations, mu, sigma, epsilon, graph, grad_norms,
+                           iters_done, mu_used, s);
     }
 
-    int solver_power_iterations(ehp_op* op, int dtype, int64_t nd, int64_t nr, int niters, double* L, hipStream_t caller)
+    int solve_power_iterations(const SolveOps& A, int dtype, int niters, double* L, hipStream_t caller)
     {
-        SolveStream ss;
-        hipError_t e = ss.init(caller);
-        if (e != hipSuccess)
-            return ehp_solver_fail(EHP_ERR_HIP, hipGetErrorString(e));
-        Scratch m;
-        double *partial, *sc;
-        if (m.alloc(&partial, kRedBlocks * sizeof(double)) != hipSuccess ||
-            m.alloc(&sc, S_COUNT * sizeof(double)) != hipSuccess)
-            return ehp_solver_fail(EHP_ERR_ALLOC, "hipMalloc of the power-iteration workspace failed");
+        Solve sv;
+        const int rc = sv.init(A, caller, "hipMalloc of the power-iteration workspace failed");
+        if (rc != EHP_OK)
+            return rc;
         if (dtype == EHP_F64)
-            return power_iterations<double>(op, nd, nr, niters, L, ss.s, m, partial, sc);
-        return power_iterations<float>(op, nd, nr, niters, L, ss.s, m, partial, sc);
+            return power_iterations<double>(A, sv, niters, L);
+        return power_iterations<float>(A, sv, niters, L);
     }
 } // namespace ehp

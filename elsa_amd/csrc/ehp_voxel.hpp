@@ -79,6 +79,7 @@ namespace ehp
         }
     }
 
+    // fn(pixel, |d|, d_u) for every pixel of the footprint, d = (centre - pixel) / scaling
     template <int D, class T, class Fn>
     EHP_HD inline void visit(const Footprint& fp, const Dims& dims, Fn fn)
     {
@@ -92,8 +93,22 @@ namespace ehp
                     const float d1 = (fp.center[1] - (float) j) / fp.scaling;
                     dist = sqrtf(d0 * d0 + d1 * d1);
                 }
-                fn((int64_t) j * dims.det[0] + i, dist);
+                fn((int64_t) j * dims.det[0] + i, dist, d0);
             }
+    }
+
+    // Weight of one footprint tap: the table at |d| (VoxelComputation.h:20-24), or for the
+    // phase-contrast projectors (diff) times sgn(d_u) in 2D and d_u in 3D (VoxelComputation.h:
+    // 26-38, ProjectVoxelsCUDA.cu:73-76, :160-165).
+    template <int D, class T>
+    EHP_HD inline T tap_weight(const T* __restrict__ lut, int n, float support, float dist, float d0, bool diff)
+    {
+        const T w = lut_eval<T>(lut, n, support, dist);
+        if (!diff)
+            return w;
+        if (D == 2)
+            return d0 > 0.0f ? w : (d0 < 0.0f ? -w : T(0));
+        return w * (T) d0;
     }
 
 

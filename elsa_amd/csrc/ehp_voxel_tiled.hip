@@ -27,7 +27,7 @@ namespace ehp
     template <int D>
     __global__ void __launch_bounds__(kThreads)
         voxel_fwd_tiled_kernel(const PoseGeom* __restrict__ geo, const Dims dims, const float* __restrict__ lut,
-                               const int n_lut, const float support, const int* __restrict__ pose_axis,
+                               const int n_lut, const float support, const int diff, const int* __restrict__ pose_axis,
                                const float* __restrict__ pose_margin, const int TU, const int TV, const int ntu,
                                const int per_xcd, const int64_t total, const float lutscale,
                                const int* __restrict__ fixexp, const float* __restrict__ vol,
@@ -101,7 +101,7 @@ namespace ehp
                             const float d1 = (fp.center[1] - (float) j) / fp.scaling;
                             dist = sqrtf(d0 * d0 + d1 * d1);
                         }
-                        const float w = lut_eval<float>(s_lut, n_lut, support, dist);
+                        const float w = tap_weight<D, float>(s_lut, n_lut, support, dist, d0, diff != 0);
                         if (w != 0.0f)
                             atomicAdd(reinterpret_cast<unsigned long long*>(acc + (j - v0) * TU + (i - u0)),
                                       fixed_contrib(ys * (w * inv_scale), sh));
@@ -131,9 +131,9 @@ namespace ehp
 
     // tc.LA x tc.TB: tile width x height in pixels (height 1 in 2D)
     hipError_t voxel_forward_tiled(const PoseGeom* geo, const Dims& dims, const float* lut, int n_lut, float support,
-                                   const int* pose_axis, const float* pose_margin, const TileCfg& tc, float lutscale,
-                                   double slot_bound, unsigned int* d_maxbits, int* d_fixexp, const float* vol,
-                                   float* sino, hipStream_t s)
+                                   int diff, const int* pose_axis, const float* pose_margin, const TileCfg& tc,
+                                   float lutscale, double slot_bound, unsigned int* d_maxbits, int* d_fixexp,
+                                   const float* vol, float* sino, hipStream_t s)
     {
         if (n_lut < 2 || n_lut > kMaxLut)
             return hipErrorInvalidValue;
@@ -150,10 +150,10 @@ namespace ehp
         const size_t lds = (size_t) TU * TV * sizeof(long long);
         if (dims.dim == 2)
             hipLaunchKernelGGL((voxel_fwd_tiled_kernel<2>), grid, dim3(kThreads), lds, s, geo, dims, lut, n_lut, support,
-                               pose_axis, pose_margin, TU, TV, ntu, (int) per_xcd, total, lutscale, d_fixexp, vol, sino);
+                               diff, pose_axis, pose_margin, TU, TV, ntu, (int) per_xcd, total, lutscale, d_fixexp, vol, sino);
         else
             hipLaunchKernelGGL((voxel_fwd_tiled_kernel<3>), grid, dim3(kThreads), lds, s, geo, dims, lut, n_lut, support,
-                               pose_axis, pose_margin, TU, TV, ntu, (int) per_xcd, total, lutscale, d_fixexp, vol, sino);
+                               diff, pose_axis, pose_margin, TU, TV, ntu, (int) per_xcd, total, lutscale, d_fixexp, vol, sino);
         return hipGetLastError();
     }
 } // namespace ehp

@@ -8,6 +8,11 @@
       elsa/projectors_cuda/VoxelProjectorCUDA.h:39-43
   BSplineVoxelProjectorCUDA(domainDescriptor, rangeDescriptor, order=2)
       elsa/projectors_cuda/VoxelProjectorCUDA.h:82-84
+  PhaseContrastBlobVoxelProjectorCUDA(domainDescriptor, rangeDescriptor, radius=2, alpha=10.83,
+                                      order=2)
+      elsa/projectors_cuda/VoxelProjectorCUDA.h:109-150
+  PhaseContrastBSplineVoxelProjectorCUDA(domainDescriptor, rangeDescriptor, order=2)
+      elsa/projectors_cuda/VoxelProjectorCUDA.h:152-191
 
 apply(x[, out]) / applyAdjoint(y[, out]) follow elsa::LinearOperator
 (elsa/core/LinearOperator.h:75-111): the result is overwritten. Data are torch tensors on the
@@ -221,6 +226,28 @@ def bspline_lut(dim, order=2, n=100, dtype=np.float32):
     return out, sup.value
 
 
+def blob_derivative_lut(radius=2.0, alpha=10.83, order=2, n=100, dtype=np.float32, normalized=False):
+    """ProjectedBlob::get_derivative_lut() or, normalized, get_normalized_gradient_lut()
+    (Blobs.h:249-317) and the support."""
+    out = np.empty(n, dtype)
+    sup = ctypes.c_double()
+    _lib.check(_lib.load().ehp_blob_derivative_lut(_DT[np.dtype(dtype)], radius, alpha, order, n, int(normalized),
+                                                   out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(sup)),
+               "blob_derivative_lut")
+    return out, sup.value
+
+
+def bspline_derivative_lut(dim, order=2, n=100, dtype=np.float32, normalized=False):
+    """ProjectedBSpline::get_derivative_lut() or, normalized, get_normalized_gradient_lut()
+    (BSplines.h:132-208) and the support."""
+    out = np.empty(n, dtype)
+    sup = ctypes.c_double()
+    _lib.check(_lib.load().ehp_bspline_derivative_lut(_DT[np.dtype(dtype)], dim, order, n, int(normalized),
+                                                      out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(sup)),
+               "bspline_derivative_lut")
+    return out, sup.value
+
+
 class _VoxelProjector(_HipProjector):
     """``kernels="fast"`` (FP32) runs the forward projection over LDS-privatised detector tiles
     with 64-bit fixed-point sums; ``"simple"`` runs one thread per voxel and pose with float
@@ -231,8 +258,12 @@ class _VoxelProjector(_HipProjector):
             raise ValueError("kernels must be 'fast' or 'simple'")
         self.kernels = kernels
 
+    _differential = False
+
     def _create(self, geo):
         flags = _lib.EHP_SIMPLE_KERNELS if self.kernels == "simple" else 0
+        if self._differential:
+            flags |= _lib.EHP_DIFFERENTIAL
         _lib.check(self._L.ehp_voxel_create(geo, _DT[self._dtype], self.lut.ctypes.data_as(ctypes.c_void_p),
                                             int(self.lut.shape[0]), float(self.support), flags,
                                             ctypes.byref(self._op)),
@@ -260,6 +291,36 @@ class BSplineVoxelProjectorCUDA(_VoxelProjector):
         super().__init__(domainDescriptor, rangeDescriptor, dtype)
 
 
+class PhaseContrastBlobVoxelProjectorCUDA(_VoxelProjector):
+    """Differential phase-contrast projector with Kaiser-Bessel blobs (elsa
+    PhaseContrastBlobVoxelProjectorCUDA, VoxelProjectorCUDA.cpp:75-114): the blob's derivative
+    table weighted by sgn(d_u) in 2D, its normalized-gradient table weighted by d_u in 3D."""
+
+    _differential = True
+
+    def __init__(self, domainDescriptor, rangeDescriptor, radius=2.0, alpha=10.83, order=2, dtype=np.float32,
+                 kernels: str = "fast"):
+        self.radius, self.alpha, self.order = radius, alpha, int(order)
+        dim = domainDescriptor.getNumberOfDimensions()
+        self.lut, self.support = blob_derivative_lut(radius, alpha, order, 100, dtype, normalized=dim == 3)
+        self._set_kernels(kernels)
+        super().__init__(domainDescriptor, rangeDescriptor, dtype)
+
+
+class PhaseContrastBSplineVoxelProjectorCUDA(_VoxelProjector):
+    """Differential phase-contrast projector with B-splines (elsa
+    PhaseContrastBSplineVoxelProjectorCUDA, VoxelProjectorCUDA.cpp:196-235)."""
+
+    _differential = True
+
+    def __init__(self, domainDescriptor, rangeDescriptor, order=2, dtype=np.float32, kernels: str = "fast"):
+        self.order = int(order)
+        dim = domainDescriptor.getNumberOfDimensions()
+        self.lut, self.support = bspline_derivative_lut(dim, order, 100, dtype, normalized=dim == 3)
+        self._set_kernels(kernels)
+        super().__init__(domainDescriptor, rangeDescriptor, dtype)
+
+
 # pyelsa exposes float and double variants with f / d suffixes and un-suffixed float aliases
 def _variant(cls, dtype, name):
     def init(self, *args, **kwargs):
@@ -277,3 +338,11 @@ BlobVoxelProjectorCUDAf = _variant(BlobVoxelProjectorCUDA, np.float32, "BlobVoxe
 BlobVoxelProjectorCUDAd = _variant(BlobVoxelProjectorCUDA, np.float64, "BlobVoxelProjectorCUDAd")
 BSplineVoxelProjectorCUDAf = _variant(BSplineVoxelProjectorCUDA, np.float32, "BSplineVoxelProjectorCUDAf")
 BSplineVoxelProjectorCUDAd = _variant(BSplineVoxelProjectorCUDA, np.float64, "BSplineVoxelProjectorCUDAd")
+PhaseContrastBlobVoxelProjectorCUDAf = _variant(PhaseContrastBlobVoxelProjectorCUDA, np.float32,
+                                                "PhaseContrastBlobVoxelProjectorCUDAf")
+PhaseContrastBlobVoxelProjectorCUDAd = _variant(PhaseContrastBlobVoxelProjectorCUDA, np.float64,
+                                                "PhaseContrastBlobVoxelProjectorCUDAd")
+PhaseContrastBSplineVoxelProjectorCUDAf = _variant(PhaseContrastBSplineVoxelProjectorCUDA, np.float32,
+                                                   "PhaseContrastBSplineVoxelProjectorCUDAf")
+PhaseContrastBSplineVoxelProjectorCUDAd = _variant(PhaseContrastBSplineVoxelProjectorCUDA, np.float64,
+                                                   "PhaseContrastBSplineVoxelProjectorCUDAd")

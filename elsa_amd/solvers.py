@@ -8,8 +8,11 @@
   powerIterations(A, niters=5)
       elsa/solvers/PowerIterations.cpp:10-27 applied to A^T A
 
-A is one of the projector classes of elsa_amd.projectors. The whole iteration runs on the
-device: step lengths, momentum and norms never leave it (FP64 reductions), and with
+A is one of the projector classes of elsa_amd.projectors, or an elsa_amd.batch.ProjectorBatch:
+then every problem of the batch is solved independently at once (per-problem step lengths and
+convergence), b and x are packed vectors, mu may be a sequence of per-problem steps, and
+`iterations_done`, `residuals`, `mu_used` (and powerIterations' result) get one row or entry
+per problem. The whole iteration runs on the device: step lengths, momentum and norms never leave it (FP64 reductions), and with
 graph=True one iteration is captured into a hipGraph and replayed. b is a GPU tensor or a
 numpy array; solve() returns the same kind. After solve(), `iterations_done` and
 `residuals` (CGLS: ||b - Ax|| at the start and per iteration; APGD: ||grad||^2 per
@@ -41,6 +44,12 @@ class ProximalIdentity:
     """prox(v) = v (elsa: ProximalIdentity)."""
 
     sigma = -1.0
+
+
+def _is_batch(A):
+    from .batch import ProjectorBatch
+
+    return isinstance(A, ProjectorBatch)
 
 
 def _prepare(A, b, x0):
@@ -78,14 +87,20 @@ class CGLS:
 
     def solve(self, iterations, x0=None):
         bt, xt, to_numpy, stream = _prepare(self.A, self.b, x0)
-        res = np.empty(iterations + 1, np.float64)
-        n = ctypes.c_int()
+        batch = _is_batch(self.A)
+        nseg = len(self.A) if batch else 1
+        res = np.empty((nseg, iterations + 1), np.float64)
+        n = np.zeros(nseg, np.int32)
         flags = _lib.EHP_SOLVE_GRAPH if self.graph else 0
-        _lib.check(self.A._L.ehp_cgls(self.A.handle, ctypes.c_void_p(bt.data_ptr()), ctypes.c_void_p(xt.data_ptr()),
-                                      int(x0 is not None), int(iterations), self.eps, self.tol, flags,
-                                      res.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(n), stream),
-                   "CGLS")
-        self.iterations_done, self.residuals = n.value, res
+        fn = self.A._L.ehp_batch_cgls if batch else self.A._L.ehp_cgls
+        _lib.check(fn(self.A.handle, ctypes.c_void_p(bt.data_ptr()), ctypes.c_void_p(xt.data_ptr()),
+                      int(x0 is not None), int(iterations), self.eps, self.tol, flags,
+                      res.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                      n.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), stream), "CGLS")
+        if batch:
+            self.iterations_done, self.residuals = n, res
+        else:
+            self.iterations_done, self.residuals = int(n[0]), res[0]
         return xt.cpu().numpy() if to_numpy else xt
 
 
@@ -95,7 +110,11 @@ class APGD:
     def __init__(self, A, b, prox=None, mu=None, epsilon=None, graph=False):
         self.A, self.b, self.graph = A, b, bool(graph)
         self.prox = prox if prox is not None else ProximalIdentity()
-        self.mu = float(mu) if mu is not None else 0.0
+        if _is_batch(A):
+            mu = np.zeros(len(A)) if mu is None else np.broadcast_to(np.asarray(mu, np.float64), (len(A),))
+            self.mu = np.ascontiguousarray(mu, np.float64)
+        else:
+            self.mu = float(mu) if mu is not None else 0.0
         self.epsilon = float(np.finfo(A._dtype).eps) if epsilon is None else float(epsilon)
         self.iterations_done = 0
         self.residuals = None
@@ -103,21 +122,35 @@ class APGD:
 
     def solve(self, iterations, x0=None):
         bt, xt, to_numpy, stream = _prepare(self.A, self.b, x0)
-        res = np.empty(iterations + 1, np.float64)
-        n, mu_used = ctypes.c_int(), ctypes.c_double()
+        batch = _is_batch(self.A)
+        nseg = len(self.A) if batch else 1
+        res = np.empty((nseg, iterations + 1), np.float64)
+        n = np.zeros(nseg, np.int32)
+        mu_used = np.zeros(nseg, np.float64)
         flags = _lib.EHP_SOLVE_GRAPH if self.graph else 0
-        _lib.check(self.A._L.ehp_apgd(self.A.handle, ctypes.c_void_p(bt.data_ptr()), ctypes.c_void_p(xt.data_ptr()),
-                                      int(x0 is not None), int(iterations), self.mu, float(self.prox.sigma),
-                                      self.epsilon, flags, res.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-                                      ctypes.byref(n), ctypes.byref(mu_used), stream), "APGD")
-        self.iterations_done, self.residuals, self.mu_used = n.value, res, mu_used.value
+        dp = ctypes.POINTER(ctypes.c_double)
+        args = (ctypes.c_void_p(bt.data_ptr()), ctypes.c_void_p(xt.data_ptr()), int(x0 is not None), int(iterations))
+        tail = (float(self.prox.sigma), self.epsilon, flags, res.ctypes.data_as(dp),
+                n.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), mu_used.ctypes.data_as(dp), stream)
+        if batch:
+            _lib.check(self.A._L.ehp_batch_apgd(self.A.handle, *args, self.mu.ctypes.data_as(dp), *tail), "APGD")
+            self.iterations_done, self.residuals, self.mu_used = n, res, mu_used
+        else:
+            _lib.check(self.A._L.ehp_apgd(self.A.handle, *args, self.mu, *tail), "APGD")
+            self.iterations_done, self.residuals, self.mu_used = int(n[0]), res[0], float(mu_used[0])
         return xt.cpu().numpy() if to_numpy else xt
 
 
 def powerIterations(A, niters=5):
-    """Largest eigenvalue estimate of A^T A (elsa: powerIterations(adjoint(A) * A, niters))."""
+    """Largest eigenvalue estimate of A^T A (elsa: powerIterations(adjoint(A) * A, niters));
+    for a batch, an array with one estimate per problem."""
     torch = _torch()
-    L = ctypes.c_double()
     stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    if _is_batch(A):
+        L = np.zeros(len(A), np.float64)
+        _lib.check(A._L.ehp_batch_power_iterations(A.handle, int(niters), L.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                   stream), "powerIterations")
+        return L
+    L = ctypes.c_double()
     _lib.check(A._L.ehp_power_iterations(A.handle, int(niters), ctypes.byref(L), stream), "powerIterations")
     return L.value

@@ -50,9 +50,15 @@ enum ehp_status {
 
 /* flags of ehp_joseph_create */
 enum ehp_joseph_flags {
-    EHP_SIMPLE_KERNELS = 1 /* one thread per ray (Joseph, Siddon) or per voxel and pose (voxel
-                              projectors) in both directions, atomic scatters; the reference
-                              point for the tiled kernels */
+    EHP_SIMPLE_KERNELS = 1, /* one thread per ray (Joseph, Siddon) or per voxel and pose (voxel
+                               projectors) in both directions, atomic scatters; the reference
+                               point for the tiled kernels */
+    EHP_DIFFERENTIAL = 2    /* ehp_voxel_create only: the phase-contrast (differential) weight
+                               of elsa's PhaseContrast*VoxelProjectorCUDA, lut(|d|) * sgn(d_u)
+                               in 2D and lut(|d|) * d_u in 3D (ProjectVoxelsCUDA.cu:73-76,
+                               :160-165; VoxelComputation.h:26-38), with d_u the detector-u
+                               component of the scaled distance; pass the derivative table in
+                               2D and the normalized-gradient table in 3D */
 };
 
 enum ehp_dtype {
@@ -62,6 +68,7 @@ enum ehp_dtype {
 
 typedef struct ehp_geometry ehp_geometry; /* device-resident per-pose matrices      */
 typedef struct ehp_op ehp_op;             /* one projector (Joseph / Siddon / voxel) */
+typedef struct ehp_batch ehp_batch;       /* many independent projectors as one         */
 
 /* Message of the last failing call on this thread ("" if none). */
 const char* ehp_last_error(void);
@@ -92,8 +99,10 @@ int ehp_siddon_create(const ehp_geometry* g, int dtype, int flags, ehp_op** out)
  * n_lut values over [0, support] (elsa: BlobVoxelProjectorCUDA / BSplineVoxelProjectorCUDA,
  * VoxelProjectorCUDA.h:39-43, :82-84; the table is ProjectedBlob::get_lut() or
  * ProjectedBSpline::get_lut(), Blobs.h:249-317, BSplines.h:132-208). lut is host memory of
- * the op's dtype. FP32 forward projections run on LDS-tiled detector tiles; flags: 0 or
- * EHP_SIMPLE_KERNELS (one thread per voxel and pose, atomic splat). */
+ * the op's dtype. FP32 forward projections run on LDS-tiled detector tiles; flags: 0,
+ * EHP_SIMPLE_KERNELS (one thread per voxel and pose, atomic splat) and EHP_DIFFERENTIAL (the
+ * phase-contrast projectors PhaseContrastBlobVoxelProjectorCUDA /
+ * PhaseContrastBSplineVoxelProjectorCUDA, VoxelProjectorCUDA.h:109-191). */
 int ehp_voxel_create(const ehp_geometry* g, int dtype, const void* lut, int n_lut, double support, int flags,
                      ehp_op** out);
 
@@ -157,6 +166,42 @@ int ehp_apgd(ehp_op* op, const void* d_b, void* d_x, int use_x0, int iterations,
  * (elsa: powerIterations(adjoint(A) * A, niters), PowerIterations.cpp:10-27). */
 int ehp_power_iterations(ehp_op* op, int niters, double* L, void* stream);
 
+/* ---- batches of independent problems -----------------------------------------------------
+ * A batch is the block-diagonal operator diag(A_0, ..., A_{n-1}) of n operators of one data
+ * type on one device, each with its own volume, trajectory and detector: its volume is the
+ * operators' volumes packed back to back (problem j at element vol_offsets[j]), its sinogram
+ * likewise. No counterpart in elsa, whose BlockLinearOperator (BlockLinearOperator.h:31-38)
+ * stacks operators that share a domain (ROW) or a range (COL); applied to problem j's block, a
+ * batch gives exactly what ehp_forward / ehp_adjoint of A_j give (bit for bit), and its
+ * solvers run elsa's iteration on every problem independently, each with its own scalars and
+ * its own convergence test. When all operators are FP32 Joseph operators of one dimension with
+ * the default (fast) kernels, every projector pass is one launch over all problems (plus the
+ * view copies, the zero fill and the fixed-point scale of the back projection); otherwise the
+ * batch calls the operators one after another. The operators must outlive the batch; tile
+ * shapes are taken at creation; a batch and its operators share device state, so calls on
+ * either must not overlap. */
+int ehp_batch_create(ehp_op* const* ops, int n, ehp_batch** out);
+void ehp_batch_destroy(ehp_batch* b);
+
+/* Element offsets of each problem in the packed volume and sinogram (n + 1 entries each, the
+ * last one the total; host memory; either may be NULL). *fused is 1 when passes are single
+ * launches (NULL allowed). */
+int ehp_batch_layout(const ehp_batch* b, int64_t* vol_offsets, int64_t* sino_offsets, int* fused);
+
+int ehp_batch_forward(ehp_batch* b, const void* d_vols, void* d_sinos, void* stream);
+int ehp_batch_adjoint(ehp_batch* b, const void* d_sinos, void* d_vols, void* stream);
+
+/* ehp_cgls / ehp_apgd / ehp_power_iterations on every problem of the batch at once. Per-problem
+ * outputs are host arrays of n rows: residuals and grad_sq_norms n x (iterations + 1),
+ * iterations_done, mu_used and L n entries. mu: n step lengths (entries <= 0, or mu NULL:
+ * 0.45 / L_j from power iterations on that problem). */
+int ehp_batch_cgls(ehp_batch* b, const void* d_b, void* d_x, int use_x0, int iterations, double eps, double tol,
+                   int flags, double* residuals, int* iterations_done, void* stream);
+int ehp_batch_apgd(ehp_batch* b, const void* d_b, void* d_x, int use_x0, int iterations, const double* mu,
+                   double l1_sigma, double epsilon, int flags, double* grad_sq_norms, int* iterations_done,
+                   double* mu_used, void* stream);
+int ehp_batch_power_iterations(ehp_batch* b, int niters, double* L, void* stream);
+
 /* ---- host-only helpers (no GPU) -------------------------------------------------------- */
 
 /* One pose of elsa's Geometry (elsa/core/Geometry.cpp:12-51 for dim 2, :53-109 for dim 3):
@@ -182,6 +227,15 @@ int ehp_circle_trajectory(int dim, int64_t nposes, const int64_t* vol_dims, floa
 int ehp_blob_lut(int dtype, double radius, double alpha, int64_t order, int n, void* out,
                  double* support);
 int ehp_bspline_lut(int dtype, int64_t dim, int64_t order, int n, void* out, double* support);
+
+/* Derivative tables of the phase-contrast projectors: ProjectedBlob::get_derivative_lut()
+ * (normalized = 0) or get_normalized_gradient_lut() (normalized = 1) (Blobs.h:249-317 with
+ * blob_derivative_projected :129-153 and blob_normalized_derivative_projected :183-207), and
+ * ProjectedBSpline's (BSplines.h:132-208 with :40-45, :83-92). Order 0 gives zeros. */
+int ehp_blob_derivative_lut(int dtype, double radius, double alpha, int64_t order, int n, int normalized, void* out,
+                            double* support);
+int ehp_bspline_derivative_lut(int dtype, int64_t dim, int64_t order, int n, int normalized, void* out,
+                               double* support);
 
 #ifdef __cplusplus
 }

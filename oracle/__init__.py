@@ -48,9 +48,9 @@ def lib():
     L.orc_xray_f32_acc64.argtypes = [c_int, c_int, c_int, c_int, _i64p, _i64p, c_i64, _f32p, _f32p, _f64p, _f64p,
                                      c_int]
     L.orc_voxel_f32.argtypes = [c_int, c_int, _i64p, _i64p, c_i64, _f32p, _f32p, _f32p, _f32p, c_int, c_f,
-                                _f32p, _f32p, c_int]
+                                c_int, _f32p, _f32p, c_int]
     L.orc_voxel_f64.argtypes = [c_int, c_int, _i64p, _i64p, c_i64, _f64p, _f64p, _f64p, _f64p, c_int, c_d,
-                                _f64p, _f64p, c_int]
+                                c_int, _f64p, _f64p, c_int]
     vp = ctypes.c_void_p
     L.orc_joseph_rays_f32.argtypes = [c_int, _i64p, _i64p, c_i64, _f32p, _f32p, vp, vp, vp, vp, vp]
     L.orc_joseph_rays_f64.argtypes = [c_int, _i64p, _i64p, c_i64, _f64p, _f64p, vp, vp, vp, vp, vp]
@@ -75,6 +75,10 @@ def lib():
     L.orc_blob_lut_f64.argtypes = [c_d, c_d, c_i64, c_int, _f64p]
     L.orc_bspline_lut_f32.argtypes = [c_i64, c_i64, c_int, _f32p]
     L.orc_bspline_lut_f64.argtypes = [c_i64, c_i64, c_int, _f64p]
+    L.orc_blob_deriv_lut_f32.argtypes = [c_f, c_f, c_i64, c_int, c_int, _f32p]
+    L.orc_blob_deriv_lut_f64.argtypes = [c_d, c_d, c_i64, c_int, c_int, _f64p]
+    L.orc_bspline_deriv_lut_f32.argtypes = [c_i64, c_i64, c_int, c_int, _f32p]
+    L.orc_bspline_deriv_lut_f64.argtypes = [c_i64, c_i64, c_int, c_int, _f64p]
     L.orc_lut_eval_f64.argtypes = [_f64p, c_int, c_d, c_d]
     L.orc_lut_eval_f64.restype = c_d
     L.orc_lut_eval_f32.argtypes = [_f32p, c_int, c_f, c_f]
@@ -227,8 +231,10 @@ def siddon(g, data, adjoint=False, nthreads=0, mode=MODE_ELSA, acc64=False):
     return xray(SIDDON, g, data, adjoint, mode, nthreads, acc64)
 
 
-def voxel(g: Geom, data, lut, support, adjoint=False, nthreads=0):
-    """Blob / B-spline voxel projector (VoxelComputation.h:40-185, full-rectangle footprint)."""
+def voxel(g: Geom, data, lut, support, adjoint=False, nthreads=0, differential=False):
+    """Blob / B-spline voxel projector (VoxelComputation.h:40-185, full-rectangle footprint);
+    differential=True gives the phase-contrast weight (VoxelComputation.h:26-38, used by
+    PhaseContrastProjector.cpp with the derivative LUT in 2D, the normalized-gradient LUT in 3D)."""
     L = lib()
     dt = np.float64 if g.P.dtype == np.float64 else np.float32
     f = L.orc_voxel_f64 if dt == np.float64 else L.orc_voxel_f32
@@ -237,7 +243,8 @@ def voxel(g: Geom, data, lut, support, adjoint=False, nthreads=0):
     lut = np.ascontiguousarray(lut, dtype=dt)
     out = np.empty(g.vol_shape if adjoint else g.sino_shape, dt)
     rc = f(g.dim, int(adjoint), vol, det, g.nviews, np.ascontiguousarray(g.P), np.ascontiguousarray(g.ext),
-           np.ascontiguousarray(g.sdd), lut, lut.shape[0], support, data.reshape(-1), out.reshape(-1), nthreads)
+           np.ascontiguousarray(g.sdd), lut, lut.shape[0], support, int(differential), data.reshape(-1),
+           out.reshape(-1), nthreads)
     assert rc == 0
     return out
 
@@ -314,6 +321,22 @@ def bspline_lut(dim, order=2, N=100, dtype=np.float32):
         lib().orc_bspline_lut_f32(dim, order, N, out)
     else:
         lib().orc_bspline_lut_f64(dim, order, N, out)
+    return out
+
+
+def blob_derivative_lut(radius=2.0, alpha=10.83, order=2, N=100, dtype=np.float32, normalized=False):
+    """ProjectedBlob::get_derivative_lut() (normalized: get_normalized_gradient_lut())."""
+    out = np.empty(N, dtype)
+    f = lib().orc_blob_deriv_lut_f32 if np.dtype(dtype) == np.float32 else lib().orc_blob_deriv_lut_f64
+    f(radius, alpha, order, N, int(normalized), out)
+    return out
+
+
+def bspline_derivative_lut(dim, order=2, N=100, dtype=np.float32, normalized=False):
+    """ProjectedBSpline::get_derivative_lut() (normalized: get_normalized_gradient_lut())."""
+    out = np.empty(N, dtype)
+    f = lib().orc_bspline_deriv_lut_f32 if np.dtype(dtype) == np.float32 else lib().orc_bspline_deriv_lut_f64
+    f(dim, order, N, int(normalized), out)
     return out
 
 

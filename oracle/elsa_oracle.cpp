@@ -483,23 +483,37 @@ namespace orc
         if (D == 2) {
             for (i64 i = lo[0]; i <= hi[0]; ++i) {
                 const R dd = (center[0] - R(i)) / scaling;
-                apply(base + i, std::sqrt(dd * dd));
+                apply(base + i, std::sqrt(dd * dd), dd);
             }
         } else {
             for (i64 j = lo[1]; j <= hi[1]; ++j)
                 for (i64 i = lo[0]; i <= hi[0]; ++i) {
                     const R d0 = (center[0] - R(i)) / scaling;
                     const R d1 = (center[1] - R(j)) / scaling;
-                    apply(base + j * sh.det[0] + i, std::sqrt(d0 * d0 + d1 * d1));
+                    apply(base + j * sh.det[0] + i, std::sqrt(d0 * d0 + d1 * d1), d0);
                 }
         }
+    }
+
+    // Weight of one footprint tap: classic lut(|d|) (VoxelComputation.h:20-24), or the
+    // differential (phase-contrast) weight lut(|d|) * sgn(d0) in 2D and lut(|d|) * d0 in 3D with
+    // d0 the first detector component of d (VoxelComputation.h:26-38).
+    template <class R, class T, int D>
+    inline T tap_weight(const T* lut, int N, R support, R dist, R d0, int diff)
+    {
+        const T w = lut_eval<R, T>(lut, N, support, dist);
+        if (!diff)
+            return w;
+        if (D == 2)
+            return w * static_cast<T>((R(0) < d0) - (d0 < R(0)));
+        return w * static_cast<T>(d0);
     }
 
     // Forward: views in parallel, voxels in order (VoxelComputation.h:109-146).
     // Adjoint: voxels in parallel, views in order (VoxelComputation.h:148-185).
     template <class R, class T, int D, bool ADJ>
     void voxel_all(const Shape& sh, const R* P, const R* ext, const R* sdd, const T* lut, int N,
-                   R support, const T* in, T* out)
+                   R support, int diff, const T* in, T* out)
     {
         const int pstride = D * (D + 1);
         if (!ADJ) {
@@ -509,8 +523,8 @@ namespace orc
                 for (i64 vox = 0; vox < sh.nvox; ++vox) {
                     const T xv = in[vox];
                     visit_footprint<R, D>(sh, P + g * pstride, ext + g * pstride, sdd[g], support,
-                                          vox, g, [&](i64 pix, R dist) {
-                                              out[pix] += xv * lut_eval<R, T>(lut, N, support, dist);
+                                          vox, g, [&](i64 pix, R dist, R d0) {
+                                              out[pix] += xv * tap_weight<R, T, D>(lut, N, support, dist, d0, diff);
                                           });
                 }
             }
@@ -520,8 +534,8 @@ namespace orc
                 T acc = T(0);
                 for (i64 g = 0; g < sh.nviews; ++g) {
                     visit_footprint<R, D>(sh, P + g * pstride, ext + g * pstride, sdd[g], support,
-                                          vox, g, [&](i64 pix, R dist) {
-                                              acc += lut_eval<R, T>(lut, N, support, dist) * in[pix];
+                                          vox, g, [&](i64 pix, R dist, R d0) {
+                                              acc += tap_weight<R, T, D>(lut, N, support, dist, d0, diff) * in[pix];
                                           });
                 }
                 out[vox] = acc;
@@ -691,6 +705,45 @@ namespace orc
         for (i64 i = 1; i < dim - 1; ++i)
             res *= bspline1d<T>(T(0), m);
         return res;
+    }
+
+    // elsa/projectors/Blobs.h:129-153 (blob_derivative_projected) and :183-207
+    // (blob_normalized_derivative_projected): the line integral of the blob's derivative, and
+    // the same divided by s, orders 1 and 2. The products with -2.0 and the Bessel values are
+    // double, as in elsa; sinh / cosh stay in T. Other orders give NaN (elsa throws).
+    template <class T>
+    T blob_derivative_projected(T s, T a, T alpha, i64 m, bool normalized)
+    {
+        const T sda = s / a;
+        const T sdas = static_cast<T>(std::pow(sda, 2));
+        const T w = static_cast<T>(1.0 - sdas);
+        if (w > 1.0e-10) {
+            const T arg = alpha * std::sqrt(w);
+            const double num = normalized ? -2.0 : -2.0 * s;
+            if (m == 1)
+                return static_cast<T>((num / a) * std::sinh(arg) / bessi1(alpha));
+            if (m == 2)
+                return static_cast<T>((num / alpha / a) * (std::cosh(arg) * arg - std::sinh(arg)) / bessi2(alpha));
+            return std::numeric_limits<T>::quiet_NaN();
+        }
+        return T(0);
+    }
+
+    // elsa/projectors/BSplines.h:40-45 (bsplineDerivative1d_evaluate: x +- 0.5 promotes to
+    // double, so both B-splines of order m - 1 are double) and :83-92
+    // (nd_bspline_derivative_centered: times bspline(0, m) in double per extra dimension), with
+    // dim - 1 as ProjectedBSpline passes it (:168-180); normalized divides by x (x = 0 taken as
+    // 1e-10, :182-188).
+    template <class T>
+    T bspline_derivative_projected(T x, i64 m, i64 dim, bool normalized)
+    {
+        if (normalized && x == T(0))
+            x = static_cast<T>(1e-10);
+        T res = static_cast<T>(bspline1d<double>(static_cast<double>(x) + 0.5, m - 1) -
+                               bspline1d<double>(static_cast<double>(x) - 0.5, m - 1));
+        for (i64 i = 1; i < dim - 1; ++i)
+            res = static_cast<T>(res * bspline1d<double>(0.0, m));
+        return normalized ? res / x : res;
     }
 
     // elsa/projectors/Luts.hpp:14-29: N samples at step*i, step = support / (N - 1).
@@ -918,17 +971,17 @@ namespace orc
     template <class R, class T>
     int dispatch_voxel(int dim, int adjoint, const i64* vol, const i64* det, i64 nviews,
                        const R* P, const R* ext, const R* sdd, const T* lut, int N, R support,
-                       const T* in, T* out)
+                       int diff, const T* in, T* out)
     {
         if (dim != 2 && dim != 3)
             return -1;
         const Shape sh = make_shape(dim, vol, det, nviews);
         if (dim == 2)
-            adjoint ? voxel_all<R, T, 2, true>(sh, P, ext, sdd, lut, N, support, in, out)
-                    : voxel_all<R, T, 2, false>(sh, P, ext, sdd, lut, N, support, in, out);
+            adjoint ? voxel_all<R, T, 2, true>(sh, P, ext, sdd, lut, N, support, diff, in, out)
+                    : voxel_all<R, T, 2, false>(sh, P, ext, sdd, lut, N, support, diff, in, out);
         else
-            adjoint ? voxel_all<R, T, 3, true>(sh, P, ext, sdd, lut, N, support, in, out)
-                    : voxel_all<R, T, 3, false>(sh, P, ext, sdd, lut, N, support, in, out);
+            adjoint ? voxel_all<R, T, 3, true>(sh, P, ext, sdd, lut, N, support, diff, in, out)
+                    : voxel_all<R, T, 3, false>(sh, P, ext, sdd, lut, N, support, diff, in, out);
         return 0;
     }
 
@@ -999,23 +1052,24 @@ int orc_xray_f32_acc64(int kind, int dim, int adjoint, int mode, const int64_t* 
                                                   C, in, out);
 }
 
-// Voxel projector (blob or B-spline LUT). P, ext: nviews blocks of dim x (dim+1).
+// Voxel projector (blob or B-spline LUT). P, ext: nviews blocks of dim x (dim+1). diff selects
+// the differential (phase-contrast) weight.
 int orc_voxel_f32(int dim, int adjoint, const int64_t* vol, const int64_t* det, int64_t nviews,
                   const float* P, const float* ext, const float* sdd, const float* lut, int N,
-                  float support, const float* in, float* out, int nthreads)
+                  float support, int diff, const float* in, float* out, int nthreads)
 {
     orc::set_threads(nthreads);
     return orc::dispatch_voxel<float, float>(dim, adjoint, vol, det, nviews, P, ext, sdd, lut, N,
-                                             support, in, out);
+                                             support, diff, in, out);
 }
 
 int orc_voxel_f64(int dim, int adjoint, const int64_t* vol, const int64_t* det, int64_t nviews,
                   const double* P, const double* ext, const double* sdd, const double* lut, int N,
-                  double support, const double* in, double* out, int nthreads)
+                  double support, int diff, const double* in, double* out, int nthreads)
 {
     orc::set_threads(nthreads);
     return orc::dispatch_voxel<double, double>(dim, adjoint, vol, det, nviews, P, ext, sdd, lut,
-                                               N, support, in, out);
+                                               N, support, diff, in, out);
 }
 
 // Per-ray Joseph traversal state (driving axis, number of samples, snapped start,
@@ -1123,6 +1177,38 @@ void orc_bspline_lut_f64(int64_t dim, int64_t order, int N, double* out)
     const double support = (order + 1) * 0.5;
     orc::make_lut<double>(support, N, out,
                           [&](double s) { return orc::bspline_projected<double>(s, order, dim); });
+}
+
+// Derivative LUTs of ProjectedBlob / ProjectedBSpline (Blobs.h:260-267, BSplines.h:150-157):
+// normalized = 0 get_derivative_lut(), 1 get_normalized_gradient_lut(); order 0 gives zeros.
+void orc_blob_deriv_lut_f32(float radius, float alpha, int64_t order, int N, int normalized, float* out)
+{
+    orc::make_lut<float>(radius, N, out, [&](float s) {
+        return order > 0 ? orc::blob_derivative_projected<float>(s, radius, alpha, order, normalized != 0) : 0.0f;
+    });
+}
+
+void orc_blob_deriv_lut_f64(double radius, double alpha, int64_t order, int N, int normalized, double* out)
+{
+    orc::make_lut<double>(radius, N, out, [&](double s) {
+        return order > 0 ? orc::blob_derivative_projected<double>(s, radius, alpha, order, normalized != 0) : 0.0;
+    });
+}
+
+void orc_bspline_deriv_lut_f32(int64_t dim, int64_t order, int N, int normalized, float* out)
+{
+    const float support = static_cast<float>((order + 1) * 0.5);
+    orc::make_lut<float>(support, N, out, [&](float s) {
+        return order > 0 ? orc::bspline_derivative_projected<float>(s, order, dim, normalized != 0) : 0.0f;
+    });
+}
+
+void orc_bspline_deriv_lut_f64(int64_t dim, int64_t order, int N, int normalized, double* out)
+{
+    const double support = (order + 1) * 0.5;
+    orc::make_lut<double>(support, N, out, [&](double s) {
+        return order > 0 ? orc::bspline_derivative_projected<double>(s, order, dim, normalized != 0) : 0.0;
+    });
 }
 
 // Lut::operator() of elsa/projectors/Luts.hpp:60-78 with a double distance.

@@ -36,14 +36,21 @@ class OracleBackend:
         return oracle.siddon(g, np.asarray(x, self.dtype), adjoint, mode=self.mode)
 
     def voxel(self, g, x, kind="blob", adjoint=False, **kw):
-        if kind == "blob":
-            lut = oracle.blob_lut(kw.get("radius", 2.0), kw.get("alpha", 10.83), kw.get("order", 2), 100, self.dtype)
+        """kind: blob, bspline, or pc_blob / pc_bspline for the phase-contrast projectors (the
+        derivative table in 2D, the normalized-gradient table in 3D, differential weight)."""
+        pc = kind.startswith("pc_")
+        base = kind[3:] if pc else kind
+        norm = pc and g.dim == 3
+        if base == "blob":
+            args = (kw.get("radius", 2.0), kw.get("alpha", 10.83), kw.get("order", 2), 100, self.dtype)
+            lut = oracle.blob_derivative_lut(*args, normalized=norm) if pc else oracle.blob_lut(*args)
             sup = kw.get("radius", 2.0)
         else:
             order = kw.get("order", 2)
-            lut = oracle.bspline_lut(g.dim, order, 100, self.dtype)
+            lut = (oracle.bspline_derivative_lut(g.dim, order, 100, self.dtype, normalized=norm) if pc
+                   else oracle.bspline_lut(g.dim, order, 100, self.dtype))
             sup = (order + 1) * 0.5
-        return oracle.voxel(g, np.asarray(x, self.dtype), lut, sup, adjoint)
+        return oracle.voxel(g, np.asarray(x, self.dtype), lut, sup, adjoint, differential=pc)
 
 
 class GpuBackend:
@@ -83,8 +90,11 @@ class GpuBackend:
         x = np.asarray(x, self.dtype)
         return op.applyAdjoint(x) if adjoint else op.apply(x)
 
+    VOXEL = {"blob": "BlobVoxelProjectorCUDA", "bspline": "BSplineVoxelProjectorCUDA",
+             "pc_blob": "PhaseContrastBlobVoxelProjectorCUDA", "pc_bspline": "PhaseContrastBSplineVoxelProjectorCUDA"}
+
     def voxel(self, g, x, kind="blob", adjoint=False, **kw):
-        op = self._op("BlobVoxelProjectorCUDA" if kind == "blob" else "BSplineVoxelProjectorCUDA", g, **kw)
+        op = self._op(self.VOXEL[kind], g, **kw)
         x = np.asarray(x, self.dtype)
         return op.applyAdjoint(x) if adjoint else op.apply(x)
 

@@ -5,6 +5,7 @@ Sources (paths relative to the elsa tree):
   Siddon  elsa/projectors/tests/test_SiddonsMethod.cpp and
           elsa/projectors_cuda/tests/test_SiddonsMethodCUDA.cpp
   voxel   elsa/projectors_cuda/tests/test_VoxelProjectorCUDA.cpp
+  phase contrast  elsa/projectors_cuda/tests/test_PhaseContrastVoxelProjectorCUDA.cpp
 Volumes are indexed vol[z, y, x] here (elsa's volume(x, y, z) with x fastest).
 Tolerances are those of the reference tests: doctest Approx (relative 100*FLT_EPSILON on
 scale 1), isApprox (||a-b|| <= prec*min(||a||,||b||), prec 1e-5 or the test's epsilon 0.01)
@@ -407,3 +408,97 @@ def voxel_backward_2d(B):
 
 
 VOXEL_KATS = [voxel_centre_voxel_3d, voxel_centre_voxel_2d, voxel_backward_2d]
+
+
+# --------------------------------------------------------------------------------------
+# Phase-contrast voxel projectors, test_PhaseContrastVoxelProjectorCUDA.cpp
+
+
+def _derivative(B, kind):
+    """blob.derivative(s) / bspline.derivative(s) of elsa (Blobs.h:280-287, BSplines.h:168-175):
+    the 2D derivative table at |s| times sgn(s), in the back end's dtype."""
+    import oracle
+
+    if kind == "pc_blob":
+        lut, sup = oracle.blob_derivative_lut(2.0, 10.83, 2, 100, B.dtype), 2.0
+    else:
+        lut, sup = oracle.bspline_derivative_lut(2, 2, 100, B.dtype), 1.5
+    return lambda s: float(oracle.lut_eval(lut, sup, abs(s))) * float(np.sign(s))
+
+
+def _pc_two_pixels(B, kind):
+    """:40-88 (blob) and :2174-2222 (B-spline): the centre voxel of 5^2 on a two-pixel detector
+    gives derivative(0.4761878) and its negative (epsilon 0.01), poses 0, 4, ..., 356 deg."""
+    dv = _derivative(B, kind)
+    w = dv(0.4761878)
+    x = np.zeros((5, 5)); x[2, 2] = 1
+    for i in range(0, 360, 4):
+        g = B.setup((5, 5), (2,), [pose(100, 5, deg(i))])
+        y = B.voxel(g, x, kind).ravel()
+        assert approx(w, y[0], 0.01) and approx(-w, y[1], 0.01), (i, y, w)
+
+
+def _pc_three_pixels(B, kind):
+    """:90-146 (blob) and :797-853 (B-spline): three pixels, poses 0, 5, ..., 355 deg: the
+    centre pixel gets derivative(0), the outer ones +-derivative(0.95233786) (epsilon 0.01)."""
+    dv = _derivative(B, kind)
+    w0, w1 = dv(0.0), dv(0.95233786)
+    x = np.zeros((5, 5)); x[2, 2] = 1
+    for i in range(0, 360, 5):
+        g = B.setup((5, 5), (3,), [pose(100, 5, deg(i))])
+        y = B.voxel(g, x, kind).ravel()
+        assert approx(w0, y[1]), (i, y)
+        assert approx(w1, y[0], 0.01) and approx(-w1, y[2], 0.01), (i, y, w1)
+
+
+def _pc_single_pixel(B, kind):
+    """:148-246 (blob) and :855-953 (B-spline), pose 0: the column on the ray sums to
+    5 derivative(0); the columns beside it to the sum of derivative(u / scaling) at their
+    projected, scaled positions (projectAndScaleVoxelOnDetector, DetectorDescriptor.cpp:
+    130-151, evaluated here in FP64)."""
+    import oracle
+
+    dv = _derivative(B, kind)
+    g = B.setup((5, 5), (1,), [pose(100, 5, 0.0)])
+    og = oracle.geom_from_poses(2, (5, 5), (1,), [oracle.geometry_pose(2, 100.0, 5.0, (5, 5), (1,), 0.0,
+                                                                       dtype=np.float64)], np.float64)
+    P = og.P[0].reshape(3, 2).T
+    E = og.ext[0].reshape(3, 2).T
+    for col in (2, 3, 1):
+        x = np.zeros((5, 5)); x[:, col] = 1
+        y = B.voxel(g, x, kind).ravel()
+        total = 0.0
+        for s in range(5):
+            hv = np.array([col + 0.5, s + 0.5, 1.0])
+            pc = P @ hv
+            u = pc[0] / pc[1] - 0.5
+            total += dv(u / (og.sdd[0] / np.linalg.norm(E @ hv)))
+        assert approx(y[0], total), (col, y, total)
+
+
+def pc_blob_two_pixels_2d(B):
+    _pc_two_pixels(B, "pc_blob")
+
+
+def pc_bspline_two_pixels_2d(B):
+    _pc_two_pixels(B, "pc_bspline")
+
+
+def pc_blob_three_pixels_2d(B):
+    _pc_three_pixels(B, "pc_blob")
+
+
+def pc_bspline_three_pixels_2d(B):
+    _pc_three_pixels(B, "pc_bspline")
+
+
+def pc_blob_single_pixel_2d(B):
+    _pc_single_pixel(B, "pc_blob")
+
+
+def pc_bspline_single_pixel_2d(B):
+    _pc_single_pixel(B, "pc_bspline")
+
+
+PHASE_CONTRAST_KATS = [pc_blob_two_pixels_2d, pc_bspline_two_pixels_2d, pc_blob_three_pixels_2d,
+                       pc_bspline_three_pixels_2d, pc_blob_single_pixel_2d, pc_bspline_single_pixel_2d]

@@ -60,6 +60,12 @@ def test_voxel_kats_gpu(kat, dt):
     kat(GpuBackend(dt))
 
 
+@pytest.mark.parametrize("kat", kats.PHASE_CONTRAST_KATS, ids=lambda k: k.__name__)
+@pytest.mark.parametrize("dt", [np.float32, np.float64], ids=["f32", "f64"])
+def test_phase_contrast_kats_gpu(kat, dt):
+    kat(GpuBackend(dt))
+
+
 # --------------------------------------------------------------------------------------
 # parity on seeded inputs
 
@@ -188,6 +194,37 @@ def test_voxel_parity(name, kind, kernels, ops):
     if kernels == "fast":
         # fixed-point sums: bitwise reproducible, and no further from the exact sum of the FP32
         # terms than the oracle's own FP32 sum (up to one rounding of each output)
+        np.testing.assert_array_equal(op.apply(x), fwd)
+
+
+@pytest.mark.parametrize("name", ["fan2d_256", "cone3d_wide", "fan2d_wide"])
+@pytest.mark.parametrize("kind", ["blob", "bspline"])
+@pytest.mark.parametrize("kernels", ["fast", "simple"])
+def test_phase_contrast_parity(name, kind, kernels, ops):
+    """Phase-contrast projectors (derivative table in 2D, normalized gradient in 3D, weight
+    times sgn(d_u) or d_u): the same bars as the classic voxel projectors."""
+    g = make_case(name)
+    x = phantom(g.vol_shape, 7)
+    y = phantom(g.sino_shape, 8)
+    kw = {} if kernels == "fast" else {"kernels": "simple"}
+    norm = g.dim == 3
+    if kind == "blob":
+        op = ops("PhaseContrastBlobVoxelProjectorCUDA", name, **kw)
+        lut, sup = oracle.blob_derivative_lut(2.0, 10.83, 2, 100, np.float32, normalized=norm), 2.0
+    else:
+        op = ops("PhaseContrastBSplineVoxelProjectorCUDA", name, **kw)
+        lut, sup = oracle.bspline_derivative_lut(g.dim, 2, 100, np.float32, normalized=norm), 1.5
+    np.testing.assert_array_equal(op.lut, lut)
+    np.testing.assert_array_equal(op.applyAdjoint(y), oracle.voxel(g, y, lut, sup, adjoint=True, differential=True))
+    fwd = op.apply(x)
+    ref = oracle.voxel(g, x, lut, sup, differential=True)
+    # the differential weights change sign across the footprint, so the sums cancel: rounding
+    # is measured against the sum of the terms' magnitudes (|weight| <= |table| in 2D and
+    # <= |table| * support in 3D)
+    mag = oracle.voxel(g, np.abs(x), np.abs(lut), sup) * (sup if g.dim == 3 else 1.0)
+    err = np.linalg.norm(np.asarray(fwd, np.float64) - ref) / np.linalg.norm(mag)
+    assert err <= 1e-6, err
+    if kernels == "fast":
         np.testing.assert_array_equal(op.apply(x), fwd)
 
 

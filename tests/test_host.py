@@ -74,6 +74,7 @@ def test_geometry_create_rejects_bad_arguments(lib):
     assert lib.ehp_joseph_create(g, 0, 3, ctypes.byref(op)) == _lib.EHP_ERR_ARGUMENT
     assert lib.ehp_voxel_create(g, 0, None, 100, 2.0, 0, ctypes.byref(op)) == _lib.EHP_ERR_ARGUMENT
     assert lib.ehp_siddon_create(g, 0, 2, ctypes.byref(op)) == _lib.EHP_ERR_ARGUMENT
+    assert lib.ehp_joseph_create(g, 0, 2, ctypes.byref(op)) == _lib.EHP_ERR_ARGUMENT  # differential: voxel only
     lut = np.ones(100, np.float32)
     assert lib.ehp_voxel_create(g, 0, lut.ctypes.data_as(ctypes.c_void_p), 100, 2.0, 4,
                                 ctypes.byref(op)) == _lib.EHP_ERR_ARGUMENT
@@ -170,3 +171,31 @@ def test_library_shares_torchs_hip_runtime(lib):
 
     rts = _lib._hip_runtimes()
     assert len(rts) == 1, rts
+
+
+def test_derivative_tables_equal_oracle_and_golden():
+    """The phase-contrast tables of the product (ehp_blob_derivative_lut,
+    ehp_bspline_derivative_lut) equal the oracle's bit for bit, and through it the reference's
+    golden vectors (test_BlobDerivative.cpp, test_BSplines.cpp)."""
+    import elsa_amd as E
+
+    for dt in (np.float32, np.float64):
+        for norm in (False, True):
+            for order in (0, 1, 2):
+                for a, alpha in ((2.0, 10.83), (1.0, 2.5)):
+                    t, sup = E.blob_derivative_lut(a, alpha, order, 100, dt, normalized=norm)
+                    np.testing.assert_array_equal(t, oracle.blob_derivative_lut(a, alpha, order, 100, dt,
+                                                                                normalized=norm))
+                    assert sup == a
+            for dim in (2, 3):
+                for order in (0, 1, 2, 3):
+                    t, sup = E.bspline_derivative_lut(dim, order, 100, dt, normalized=norm)
+                    np.testing.assert_array_equal(t, oracle.bspline_derivative_lut(dim, order, 100, dt,
+                                                                                   normalized=norm))
+                    assert sup == (order + 1) * 0.5
+    gold = json.load(open(os.path.join(ROOT, "tests", "golden", "derivatives.json")))
+    t, _ = E.bspline_derivative_lut(2, 3, 101, np.float64)
+    np.testing.assert_allclose(t, gold["bspline_derivative_lut"]["expected"], rtol=1e-6, atol=1e-9)
+    tab = [x for x in gold["blob_derivative"]["tables"] if x["alpha"] == 10.83 and x["m"] == 2][0]
+    t, _ = E.blob_derivative_lut(1.0, 10.83, 2, 101, np.float64)
+    np.testing.assert_allclose(t, tab["expected"], rtol=1e-5, atol=1e-7)

@@ -397,3 +397,139 @@ def test_power_iterations_and_apgd_closed_forms():
     ref = np.linalg.lstsq(M, bb, rcond=None)[0]
     assert np.linalg.norm(x2 - ref) <= 1e-6 * np.linalg.norm(ref)
     assert abs(mu2 - 0.45 / oracle.power_iterations(A2, At2, (15,), 5)) <= 1e-15
+
+
+# --------------------------------------------------------------------------------------
+# phase contrast: derivative tables and the differential weight
+
+
+def _golden(name):
+    import json
+    import os
+
+    return json.load(open(os.path.join(os.path.dirname(__file__), "golden", name)))
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64], ids=["f32", "f64"])
+def test_blob_derivative_lut_golden(dt):
+    """ProjectedBlob derivative table against test_BlobDerivative.cpp's vectors (a = 1 at
+    s = i/100 and a = 2 at s = i/50, where s/a is the same), and the normalized gradient
+    against the same numbers divided by s (test_BlobGradientHelper.cpp), doctest Approx."""
+    gold = _golden("derivatives.json")["blob_derivative"]
+    for t in gold["tables"]:
+        exp = t["expected"]
+        for a in (1.0, 2.0):
+            d = oracle.blob_derivative_lut(a, t["alpha"], t["m"], 101, dt)
+            g = oracle.blob_derivative_lut(a, t["alpha"], t["m"], 101, dt, normalized=True)
+            for i in range(101):
+                assert kats.approx(d[i], exp[i]), (a, t["alpha"], t["m"], i, d[i], exp[i])
+                if i >= 1:
+                    s = i * a / 100.0
+                    assert kats.approx(g[i], exp[i] / s), (a, t["alpha"], t["m"], i)
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64], ids=["f32", "f64"])
+def test_bspline_derivative_lut_golden(dt):
+    """ProjectedBSpline(2, 3) derivative table against test_BSplines.cpp:144-164."""
+    gold = _golden("derivatives.json")["bspline_derivative_lut"]
+    lut = oracle.bspline_derivative_lut(2, 3, 101, dt)
+    for i, e in enumerate(gold["expected"]):
+        assert kats.approx(lut[i], e), (i, lut[i], e)
+
+
+def test_derivative_luts_are_derivatives():
+    """The derivative tables are the derivatives of the projected basis functions (central
+    differences of the projected tables), the 3D normalized gradient times s is the 3D
+    derivative, and the 3D B-spline derivative is the 2D one times b(0): a dropped factor or a
+    wrong sign fails one of these."""
+    n, h = 2001, None
+    for a, alpha, m in ((2.0, 10.83, 2), (1.0, 2.5, 1), (1.5, 7.5, 2)):
+        f = oracle.blob_lut(a, alpha, m, n, np.float64)
+        d = oracle.blob_derivative_lut(a, alpha, m, n, np.float64)
+        h = a / (n - 1)
+        cd = (f[2:] - f[:-2]) / (2 * h)
+        assert np.max(np.abs(cd[5:-40] - d[1:-1][5:-40])) <= 1e-4 * np.max(np.abs(d))
+        g = oracle.blob_derivative_lut(a, alpha, m, n, np.float64, normalized=True)
+        s = np.arange(n) * h
+        np.testing.assert_allclose(g[1:] * s[1:], d[1:], rtol=1e-12, atol=1e-14)
+    for dim in (2, 3):
+        for order in (1, 2, 3):
+            sup = (order + 1) * 0.5
+            f = oracle.bspline_lut(dim, order, n, np.float64)
+            d = oracle.bspline_derivative_lut(dim, order, n, np.float64)
+            h = sup / (n - 1)
+            cd = (f[2:] - f[:-2]) / (2 * h)
+            # away from the knots (multiples of 1/2), where lower orders have kinks
+            x = np.arange(1, n - 1) * h
+            ok = np.abs(x * 2 - np.round(x * 2)) > 4 * h
+            assert np.max(np.abs(cd - d[1:-1])[ok]) <= 1e-5, (dim, order)
+            g = oracle.bspline_derivative_lut(dim, order, n, np.float64, normalized=True)
+            np.testing.assert_allclose(g[1:] * (np.arange(1, n) * h), d[1:], rtol=1e-12, atol=1e-14)
+        b0 = oracle.bspline_lut(2, 2, 3, np.float64)[0]  # b(0) of order 2 (2D table is b itself)
+        np.testing.assert_allclose(oracle.bspline_derivative_lut(3, 2, 101, np.float64),
+                                   oracle.bspline_derivative_lut(2, 2, 101, np.float64) * b0, rtol=1e-12, atol=1e-15)
+
+
+@pytest.mark.parametrize("kat", kats.PHASE_CONTRAST_KATS, ids=lambda k: k.__name__)
+@pytest.mark.parametrize("dt", [np.float32, np.float64], ids=["f32", "f64"])
+def test_phase_contrast_kats_oracle(kat, dt):
+    kat(OracleBackend(dt))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_phase_contrast_adjoint_and_antisymmetry(dim):
+    """The differential projector's adjoint is matched, it is linear in the table, and it
+    differs from the classic projector with the same table."""
+    n = 9 if dim == 3 else 15
+    det = (11, 9) if dim == 3 else (17,)
+    g = _rand_geom(dim, n, 4, det, np.float64, seed=8)
+    lut = oracle.blob_derivative_lut(2.0, 10.83, 2, 100, np.float64, normalized=dim == 3)
+    rng = np.random.default_rng(9)
+    x = rng.standard_normal(g.vol_shape)
+    y = rng.standard_normal(g.sino_shape)
+    A = oracle.voxel(g, x, lut, 2.0, differential=True)
+    At = oracle.voxel(g, y, lut, 2.0, adjoint=True, differential=True)
+    assert abs(np.vdot(A, y) - np.vdot(x, At)) <= 1e-12 * np.linalg.norm(A) * np.linalg.norm(y)
+    np.testing.assert_allclose(oracle.voxel(g, x, -lut, 2.0, differential=True), -A, rtol=0, atol=0)
+    assert np.linalg.norm(oracle.voxel(g, x, lut, 2.0) - A) > 1e-3 * np.linalg.norm(A)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("kind", ["blob", "bspline"])
+def test_phase_contrast_is_u_derivative_of_classic(dim, kind):
+    """For one voxel, the phase-contrast projection is the derivative of the classic
+    projection with respect to the detector u coordinate in units of the scaled distance:
+    shifting the detector by +-delta pixels (principal point offset) moves d_u by
+    +-delta / scaling, so (classic(+delta) - classic(-delta)) * scaling / (2 delta) must equal
+    the differential projection (2D: g'(|d|) sgn(d_u), 3D: g'(|d|) / |d| * d_u). Fine tables
+    (2001 entries) keep the interpolation error small."""
+    n = 7
+    vol = (n,) * dim
+    det = (13, 11) if dim == 3 else (13,)
+    N = 2001
+    if kind == "blob":
+        sup = 2.0
+        classic = oracle.blob_lut(2.0, 10.83, 2, N, np.float64)
+        diff = oracle.blob_derivative_lut(2.0, 10.83, 2, N, np.float64, normalized=dim == 3)
+    else:
+        sup = 1.5
+        classic = oracle.bspline_lut(dim, 2, N, np.float64)
+        diff = oracle.bspline_derivative_lut(dim, 2, N, np.float64, normalized=dim == 3)
+    delta = 1e-3
+    for gamma in (0.3, 1.9):
+        for vox in ((3, 3, 3)[:dim], (2, 4, 5)[:dim]):
+            x = np.zeros(vol[::-1]); x[vox[::-1]] = 1.0
+
+            def geom(off):
+                ppo = np.array([off] + [0.0] * (dim - 2))
+                p = oracle.geometry_pose(dim, 60.0, 6.0, vol, det, angle_rad=gamma, ppo=ppo, dtype=np.float64)
+                return oracle.geom_from_poses(dim, vol, det, [p], np.float64)
+
+            g0 = geom(0.0)
+            hv = np.array([v + 0.5 for v in vox] + [1.0])
+            E = g0.ext[0].reshape(dim + 1, dim).T
+            scaling = g0.sdd[0] / np.linalg.norm(E @ hv)
+            fd = (oracle.voxel(geom(delta), x, classic, sup) - oracle.voxel(geom(-delta), x, classic, sup))
+            fd *= scaling / (2 * delta)
+            pc = oracle.voxel(g0, x, diff, sup, differential=True)
+            assert np.max(np.abs(fd - pc)) <= 2e-3 * np.max(np.abs(pc)), (kind, dim, gamma, vox)
