@@ -72,6 +72,8 @@ def parse():
     p.add_argument("--tile", default="", help="LA,TB,TC of the tiled Joseph / Siddon back projection, or W,H "
                                               "pixels of the tiled voxel forward (default: built-in), or "
                                               "'auto' to time a set of candidates first (untimed)")
+    p.add_argument("--pose-batch", type=int, default=0,
+                   help="poses per step of the tiled back projection (default: built-in)")
     p.add_argument("--projector", default="joseph", choices=["joseph", "siddon", "blob", "bspline"],
                    help="projector to time (default: Joseph, the headline metric)")
     p.add_argument("--cg-iters", type=int, default=20,
@@ -181,13 +183,18 @@ def batch_main(a, E, wl):
     import torch
 
     ops, xs, bs = batch_problems(E, wl)
-    tiles = {}
-    for op, b in zip(ops, bs):
-        if a.tile == "auto":
-            tiles[op.getDomainDescriptor().shape[0]] = E.autotune(op, x=b)[0]
-        elif a.tile:
-            op.tile(*[int(v) for v in a.tile.split(",")])
-    B = E.ProjectorBatch(ops)
+    tuned = None
+    if a.tile == "auto":
+        B, tile, pose_batch, times = E.autotune_batch(ops)
+        tuned = {"tile": tile, "pose_batch": pose_batch}
+        print(json.dumps({"autotune_batch": {str(k): round(v, 3) for k, v in times.items()}}), file=sys.stderr)
+    else:
+        for op in ops:
+            if a.tile:
+                op.tile(*[int(v) for v in a.tile.split(",")])
+            if a.pose_batch:
+                op.pose_batch(a.pose_batch)
+        B = E.ProjectorBatch(ops)
     S = sum(op.sample_count() for op in ops)
     R = int(B.sino_offsets[-1])
     x = B.pack(xs)
@@ -272,7 +279,8 @@ def batch_main(a, E, wl):
         "config": {"workload": wl["text"], "problems": len(ops), "fused": B.fused, "samples_per_pass": S,
                    "rays": R, "voxels": int(B.vol_offsets[-1]), "fwd_ms": round(t_fwd * 1e3, 3),
                    "adj_ms": round(t_adj * 1e3, 3), "ms_per_step_operator_by_operator":
-                       round(wall_plain / a.steps * 1e3, 3), "autotuned_tiles": {str(k): v for k, v in tiles.items()}},
+                       round(wall_plain / a.steps * 1e3, 3), "tile": list(ops[0].tile()),
+                   "pose_batch": ops[0].pose_batch(), "autotuned": tuned},
         "solve": solve,
     }
     print(json.dumps(out), flush=True)
@@ -323,6 +331,8 @@ def main():
             print(json.dumps({"autotune": {str(k): round(v, 3) for k, v in tile_times.items()}}), file=sys.stderr)
         else:
             tile = op.tile(*[int(v) for v in a.tile.split(",")]) if a.tile else op.tile()
+            if a.pose_batch and a.projector in ("joseph", "siddon"):
+                op.pose_batch(a.pose_batch)
     sino = torch.empty(dd.shape, dtype=torch.float32, device=dev)
     back = torch.empty(vd.shape, dtype=torch.float32, device=dev)
 
@@ -386,6 +396,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (modified Shepp-Logan phantom)",
         "config": {"workload": wl["text"], "projector": a.projector, "kernels": a.kernels, "tile": tile,
+                   "pose_batch": op.pose_batch() if a.kernels == "fast" and a.projector in ("joseph", "siddon")
+                   else None,
                    "volume": list(wl["vol"]), "poses": wl["views"],
                    "detector": list(wl["det"]), "samples_per_pass": S, "rays": R,
                    "fwd_ms": round(t_fwd * 1e3, 3), "adj_ms": round(t_adj * 1e3, 3)},

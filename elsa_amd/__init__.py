@@ -14,9 +14,9 @@ from .projectors import (BlobVoxelProjectorCUDA, BlobVoxelProjectorCUDAd, BlobVo
                          PhaseContrastBSplineVoxelProjectorCUDAd, PhaseContrastBSplineVoxelProjectorCUDAf,
                          SiddonsMethodCUDA, SiddonsMethodCUDAd, SiddonsMethodCUDAf, blob_derivative_lut, blob_lut,
                          bspline_derivative_lut, bspline_lut)
-from .solvers import APGD, CGLS, ProximalIdentity, ProximalL1, powerIterations
-from . import autograd
-from .autotune import autotune
+from .solvers import APGD, CGLS, CGLSRefined, ProximalIdentity, ProximalL1, powerIterations
+from . import autograd, io
+from .autotune import autotune, autotune_batch
 from .batch import ProjectorBatch
 
 __all__ = [
@@ -27,6 +27,6 @@ __all__ = [
     "BSplineVoxelProjectorCUDAd", "PhaseContrastBlobVoxelProjectorCUDA", "PhaseContrastBlobVoxelProjectorCUDAf",
     "PhaseContrastBlobVoxelProjectorCUDAd", "PhaseContrastBSplineVoxelProjectorCUDA",
     "PhaseContrastBSplineVoxelProjectorCUDAf", "PhaseContrastBSplineVoxelProjectorCUDAd", "blob_lut", "bspline_lut",
-    "blob_derivative_lut", "bspline_derivative_lut", "CGLS", "APGD", "ProximalL1", "ProximalIdentity",
-    "powerIterations", "autotune", "ProjectorBatch",
+    "blob_derivative_lut", "bspline_derivative_lut", "CGLS", "CGLSRefined", "APGD", "ProximalL1", "ProximalIdentity",
+    "powerIterations", "autotune", "autotune_batch", "ProjectorBatch",
 ]

@@ -23,10 +23,11 @@ EXPORTS = (
     "ehp_circle_trajectory", "ehp_blob_lut", "ehp_bspline_lut", "ehp_cgls", "ehp_apgd", "ehp_power_iterations",
     "ehp_batch_create", "ehp_batch_destroy", "ehp_batch_layout", "ehp_batch_forward", "ehp_batch_adjoint",
     "ehp_batch_cgls", "ehp_batch_apgd", "ehp_batch_power_iterations", "ehp_blob_derivative_lut",
-    "ehp_bspline_derivative_lut",
+    "ehp_bspline_derivative_lut", "ehp_pose_batch", "ehp_cgls_refined",
 )
 EHP_SOLVE_GRAPH = 1
 EHP_DIFFERENTIAL = 2
+EHP_BF16 = 2
 
 _lib = None
 
@@ -92,6 +93,7 @@ def load():
     L.ehp_joseph_ray_info.argtypes = [vp, vp, vp]
     ip = ctypes.POINTER(ctypes.c_int)
     L.ehp_tile.argtypes = [vp, c_int, c_int, c_int, ip, ip, ip]
+    L.ehp_pose_batch.argtypes = [vp, c_int, ip]
     L.ehp_geometry_pose.argtypes = [c_int, c_f, c_f, fp, fp, fp, fp, c_f, c_f, c_f, fp, fp, fp, fp, fp, fp, fp]
     L.ehp_circle_trajectory.argtypes = [c_int, i64, i64p, c_f, c_f, c_f, i64p, i64p, fp, fp, fp, fp, fp]
     L.ehp_blob_lut.argtypes = [c_int, c_d, c_d, i64, c_int, vp, ctypes.POINTER(c_d)]
@@ -102,6 +104,7 @@ def load():
     L.ehp_cgls.argtypes = [vp, vp, vp, c_int, c_int, c_d, c_d, c_int, dp, ip, vp]
     L.ehp_apgd.argtypes = [vp, vp, vp, c_int, c_int, c_d, c_d, c_d, c_int, dp, ip, dp, vp]
     L.ehp_power_iterations.argtypes = [vp, c_int, dp, vp]
+    L.ehp_cgls_refined.argtypes = [vp, vp, vp, vp, c_int, c_int, c_int, c_int, dp, vp]
     L.ehp_batch_create.argtypes = [ctypes.POINTER(vp), c_int, ctypes.POINTER(vp)]
     L.ehp_batch_destroy.argtypes = [vp]
     L.ehp_batch_destroy.restype = None

@@ -92,7 +92,7 @@ namespace
             it.table = op->d_table;
             it.pose_axes = op->d_pose_axes;
             for (int a = 0; a < 3; ++a) {
-                it.views[a] = op->d_view[a];
+                it.views[a] = (float*) op->d_view[a];
                 it.layouts[a] = op->layouts[a];
             }
             it.dims = op->dims;
@@ -170,7 +170,7 @@ namespace
     int batch_apply(ehp_batch* b, const void* in, void* out, hipStream_t s, bool adjoint)
     {
         if (!b->fused) {
-            const size_t es = b->dtype == EHP_F64 ? sizeof(double) : sizeof(float);
+            const size_t es = b->dtype == EHP_F64 ? sizeof(double) : (b->dtype == EHP_BF16 ? 2 : sizeof(float));
             for (size_t j = 0; j < b->ops.size(); ++j) {
                 const int64_t ai = adjoint ? b->sino_off[j] : b->vol_off[j];
                 const int64_t ao = adjoint ? b->vol_off[j] : b->sino_off[j];

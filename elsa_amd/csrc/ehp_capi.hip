@@ -37,9 +37,12 @@ namespace
         return fail(EHP_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
     }
 
-    bool valid_dtype(int dtype) { return dtype == EHP_F32 || dtype == EHP_F64; }
+    bool valid_dtype(int dtype) { return dtype == EHP_F32 || dtype == EHP_F64 || dtype == EHP_BF16; }
 
-    size_t dtype_size(int dtype) { return dtype == EHP_F64 ? sizeof(double) : sizeof(float); }
+    size_t dtype_size(int dtype)
+    {
+        return dtype == EHP_F64 ? sizeof(double) : (dtype == EHP_BF16 ? sizeof(ehp::bf16) : sizeof(float));
+    }
 
     ehp::Dims make_dims(const ehp_geometry* g)
     {
@@ -271,17 +274,28 @@ namespace
             if ((op->axes_present & (1 << a)) && op->layouts[a].size >= ((int64_t) 1 << 30))
                 planes = false;
         }
+        const bool b16 = op->dtype == EHP_BF16;
         for (int a = 0; planes && a < dm.dim; ++a) {
             if (!(op->axes_present & (1 << a)))
                 continue;
-            const size_t vbytes = (size_t) op->layouts[a].size * sizeof(float);
+            const size_t vbytes = (size_t) op->layouts[a].size * (b16 ? sizeof(ehp::bf16) : sizeof(float));
             e = hipMalloc((void**) &op->d_view[a], vbytes);
             if (e != hipSuccess)
                 return fail(EHP_ERR_ALLOC, "hipMalloc of a %zu-byte volume view failed: %s", vbytes,
                             hipGetErrorString(e));
-            e = ehp::launch_fill<float>(op->d_view[a], op->layouts[a].size, 0.0f, nullptr);
+            if (b16)
+                e = ehp::launch_fill<ehp::bf16>((ehp::bf16*) op->d_view[a], op->layouts[a].size, ehp::bf16{0}, nullptr);
+            else
+                e = ehp::launch_fill<float>((float*) op->d_view[a], op->layouts[a].size, 0.0f, nullptr);
             if (e != hipSuccess)
                 return hip_fail(e, "volume view");
+        }
+        if (b16 && planes) {
+            const size_t abytes = (size_t) dm.nvox * sizeof(float);
+            e = hipMalloc((void**) &op->d_acc32, abytes);
+            if (e != hipSuccess)
+                return fail(EHP_ERR_ALLOC, "hipMalloc of the %zu-byte FP32 back projection failed: %s", abytes,
+                            hipGetErrorString(e));
         }
         op->planes = planes;
         // the views' padding is cleared on the null stream; finish before any caller stream runs
@@ -409,12 +423,16 @@ int ehp_joseph_create(const ehp_geometry* g, int dtype, int flags, ehp_op** out)
 {
     if (flags & ~EHP_SIMPLE_KERNELS)
         return fail(EHP_ERR_ARGUMENT, "unknown Joseph flags 0x%x", flags);
+    if (dtype == EHP_BF16 && (flags & EHP_SIMPLE_KERNELS))
+        return fail(EHP_ERR_UNSUPPORTED, "bfloat16 Joseph operators run the default kernels only");
     int rc = new_op(g, KIND_JOSEPH, dtype, out);
     if (rc != EHP_OK)
         return rc;
     rc = count_samples(*out);
-    if (rc == EHP_OK && dtype == EHP_F32 && !(flags & EHP_SIMPLE_KERNELS))
+    if (rc == EHP_OK && (dtype == EHP_F32 || dtype == EHP_BF16) && !(flags & EHP_SIMPLE_KERNELS))
         rc = setup_fast(*out);
+    if (rc == EHP_OK && dtype == EHP_BF16 && !((*out)->planes && (*out)->tiled))
+        rc = fail(EHP_ERR_UNSUPPORTED, "this geometry has no plane-major / tiled Joseph kernels, which bfloat16 needs");
     if (rc != EHP_OK) {
         ehp_destroy(*out);
         *out = nullptr;
@@ -426,6 +444,8 @@ int ehp_siddon_create(const ehp_geometry* g, int dtype, int flags, ehp_op** out)
 {
     if (flags & ~EHP_SIMPLE_KERNELS)
         return fail(EHP_ERR_ARGUMENT, "unknown Siddon flags 0x%x", flags);
+    if (dtype == EHP_BF16)
+        return fail(EHP_ERR_UNSUPPORTED, "bfloat16 storage is implemented for Joseph only");
     int rc = new_op(g, KIND_SIDDON, dtype, out);
     if (rc != EHP_OK)
         return rc;
@@ -444,6 +464,8 @@ int ehp_voxel_create(const ehp_geometry* g, int dtype, const void* lut, int n_lu
 {
     if (flags & ~(EHP_SIMPLE_KERNELS | EHP_DIFFERENTIAL))
         return fail(EHP_ERR_ARGUMENT, "unknown voxel projector flags 0x%x", flags);
+    if (dtype == EHP_BF16)
+        return fail(EHP_ERR_UNSUPPORTED, "bfloat16 storage is implemented for Joseph only");
     if (!lut || n_lut < 2)
         return fail(EHP_ERR_ARGUMENT, "the basis-function table needs at least 2 entries");
     if (!(support > 0.0))
@@ -488,7 +510,7 @@ void ehp_destroy(ehp_op* op)
         (void) hipSetDevice(op->device);
     for (void* p : {(void*) op->d_geo, op->d_lut, (void*) op->d_pose_axes, (void*) op->d_table,
                     (void*) op->d_sidtable, (void*) op->d_pose_margin, (void*) op->d_maxbits,
-                    (void*) op->d_fixexp, (void*) op->d_view[0], (void*) op->d_view[1], (void*) op->d_view[2]})
+                    (void*) op->d_fixexp, op->d_view[0], op->d_view[1], op->d_view[2], (void*) op->d_acc32})
         if (p)
             (void) hipFree(p);
     if (cur >= 0 && cur != op->device)
@@ -531,14 +553,27 @@ static int apply(ehp_op* op, const void* in, void* out, void* stream, bool adjoi
         return fail(EHP_ERR_ARGUMENT, "operator lives on device %d, current device is %d", op->device, cur);
     hipStream_t s = (hipStream_t) stream;
     const ehp::Dims& dm = op->dims;
-    if (op->dtype == EHP_F32) {
+    if (op->dtype == EHP_BF16) {
+        // bfloat16 storage, FP32 arithmetic: the forward reads bfloat16 views and rounds each ray
+        // sum once; the back projection sums in fixed point into FP32 and rounds each voxel once
+        if (!adjoint) {
+            e = ehp::joseph_forward_planes<ehp::bf16>(op->d_geo, dm, op->layouts, op->d_view, op->axes_present,
+                                                      (const ehp::bf16*) in, (ehp::bf16*) out, s);
+        } else {
+            e = ehp::joseph_adjoint_tiled<ehp::bf16>(op->d_geo, op->d_table, dm, op->d_pose_axes, op->axes_present,
+                                                     op->tile, op->batch, op->slot_bound, op->d_maxbits, op->d_fixexp,
+                                                     (const ehp::bf16*) in, op->d_acc32, s);
+            if (e == hipSuccess)
+                e = ehp::launch_to_bf16(op->d_acc32, (ehp::bf16*) out, dm.nvox, s);
+        }
+    } else if (op->dtype == EHP_F32) {
         const float* x = (const float*) in;
         float* y = (float*) out;
         if (op->kind == KIND_JOSEPH && adjoint && op->tiled)
             e = ehp::joseph_adjoint_tiled(op->d_geo, op->d_table, dm, op->d_pose_axes, op->axes_present, op->tile,
                                           op->batch, op->slot_bound, op->d_maxbits, op->d_fixexp, x, y, s);
         else if (op->kind == KIND_JOSEPH && !adjoint && op->planes)
-            e = ehp::joseph_forward_planes(op->d_geo, dm, op->layouts, op->d_view, op->axes_present, x, y, s);
+            e = ehp::joseph_forward_planes<float>(op->d_geo, dm, op->layouts, op->d_view, op->axes_present, x, y, s);
         else if (op->kind == KIND_JOSEPH)
             e = adjoint ? ehp::joseph_adjoint<float>(op->d_geo, dm, x, y, s) : ehp::joseph_forward<float>(op->d_geo, dm, x, y, s);
         else if (op->kind == KIND_SIDDON && adjoint && op->tiled)
@@ -610,6 +645,22 @@ int ehp_tile(ehp_op* op, int la, int tb, int tc, int* la_out, int* tb_out, int* 
         *tb_out = op->tile.TB;
     if (tc_out)
         *tc_out = op->tile.TC;
+    return EHP_OK;
+}
+
+int ehp_pose_batch(ehp_op* op, int batch, int* batch_out)
+{
+    if (!op)
+        return fail(EHP_ERR_ARGUMENT, "operator is NULL");
+    if (!op->tiled || op->kind == KIND_VOXEL)
+        return fail(EHP_ERR_ARGUMENT, "operator has no tiled back projection");
+    if (batch > 0) {
+        if (batch > ehp::kMaxPoseBatch)
+            return fail(EHP_ERR_ARGUMENT, "pose batch %d exceeds %d", batch, ehp::kMaxPoseBatch);
+        op->batch = batch;
+    }
+    if (batch_out)
+        *batch_out = op->batch;
     return EHP_OK;
 }
 
@@ -717,6 +768,29 @@ int ehp_apgd(ehp_op* op, const void* d_b, void* d_x, int use_x0, int iterations,
     return ehp::solve_apgd(A.ops, op->dtype, d_b, d_x, use_x0, iterations, &mu, l1_sigma, epsilon,
                            (flags & EHP_SOLVE_GRAPH) != 0, grad_sq_norms, iterations_done, mu_used,
                            (hipStream_t) stream);
+}
+
+int ehp_cgls_refined(ehp_op* op_hi, ehp_op* op_lo, const void* d_b, void* d_x, int use_x0, int outer, int inner,
+                     int flags, double* residuals, void* stream)
+{
+    int rc = solver_args(op_hi, d_b, d_x, outer);
+    if (rc != EHP_OK)
+        return rc;
+    if (!op_lo)
+        return fail(EHP_ERR_ARGUMENT, "operator is NULL");
+    if (op_hi->dtype != EHP_F32 || op_lo->dtype != EHP_BF16)
+        return fail(EHP_ERR_ARGUMENT, "refinement needs an FP32 operator and a bfloat16 operator");
+    if (op_hi->kind != op_lo->kind || op_hi->dims.nvox != op_lo->dims.nvox || op_hi->dims.nrays != op_lo->dims.nrays ||
+        op_hi->host_geo.size() != op_lo->host_geo.size() ||
+        std::memcmp(op_hi->host_geo.data(), op_lo->host_geo.data(), op_hi->host_geo.size() * sizeof(ehp::PoseGeom)))
+        return fail(EHP_ERR_ARGUMENT, "the two operators must have the same kind and geometry");
+    if (inner < 1)
+        return fail(EHP_ERR_ARGUMENT, "inner iterations must be at least 1 (got %d)", inner);
+    if (flags & ~EHP_SOLVE_GRAPH)
+        return fail(EHP_ERR_ARGUMENT, "unknown solver flags 0x%x", flags);
+    SingleOps hi(op_hi), lo(op_lo);
+    return ehp::solve_cgls_refined(hi.ops, lo.ops, (const float*) d_b, (float*) d_x, use_x0, outer, inner,
+                                   (flags & EHP_SOLVE_GRAPH) != 0, residuals, (hipStream_t) stream);
 }
 
 int ehp_power_iterations(ehp_op* op, int niters, double* L, void* stream)

@@ -33,6 +33,61 @@ namespace ehp
 
 namespace ehp
 {
+    // bfloat16 storage: the upper half of an IEEE single, kept as raw bits. Arithmetic on it is
+    // FP32: up() widens exactly, down() rounds to nearest even (NaN stays a quiet NaN).
+    struct bf16 {
+        uint16_t bits;
+    };
+
+    EHP_HD inline float bf16_to_float(bf16 v)
+    {
+        const uint32_t u = (uint32_t) v.bits << 16;
+        float f;
+        __builtin_memcpy(&f, &u, 4);
+        return f;
+    }
+
+    EHP_HD inline bf16 float_to_bf16(float f)
+    {
+        uint32_t u;
+        __builtin_memcpy(&u, &f, 4);
+        if ((u & 0x7fffffffu) > 0x7f800000u)
+            return bf16{(uint16_t) ((u >> 16) | 0x40u)};
+        u += 0x7fffu + ((u >> 16) & 1u);
+        return bf16{(uint16_t) (u >> 16)};
+    }
+
+    // arithmetic type of a storage type
+    template <class T>
+    struct Acc {
+        using type = T;
+    };
+    template <>
+    struct Acc<bf16> {
+        using type = float;
+    };
+
+    template <class T>
+    EHP_HD inline typename Acc<T>::type up(T v)
+    {
+        return v;
+    }
+    EHP_HD inline float up(bf16 v)
+    {
+        return bf16_to_float(v);
+    }
+
+    template <class T>
+    EHP_HD inline T down(typename Acc<T>::type v)
+    {
+        return v;
+    }
+    template <>
+    EHP_HD inline bf16 down<bf16>(float v)
+    {
+        return float_to_bf16(v);
+    }
+
     // Per-pose geometry as kernels read it. One 64-float record per pose so a wavefront can
     // read it with scalar loads. Column-major blocks as in elsa (Eigen).
     struct PoseGeom {

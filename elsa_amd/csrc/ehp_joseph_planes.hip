@@ -25,7 +25,8 @@ namespace ehp
         constexpr int kTile = 32; // transpose tile
 
         // View with x fastest: one lane per voxel, rows copied as they are.
-        __device__ __forceinline__ void view_copy_rows(const float* __restrict__ in, float* __restrict__ out,
+        template <class E>
+        __device__ __forceinline__ void view_copy_rows(const E* __restrict__ in, E* __restrict__ out,
                                                        const Dims& dims, const PadLayout& L, int64_t block)
         {
             const int64_t i = block * 256 + threadIdx.x;
@@ -41,11 +42,12 @@ namespace ehp
         // View with f != x fastest: a 32 x 32 LDS tile over the (x, f) plane, reads coalesced
         // along x, writes along f. Block (bx, by, bz) of the grid (ceil(nx / 32), ceil(n_f / 32),
         // n_t), t the third axis.
-        __device__ __forceinline__ void view_copy_transpose(const float* __restrict__ in, float* __restrict__ out,
+        template <class E>
+        __device__ __forceinline__ void view_copy_transpose(const E* __restrict__ in, E* __restrict__ out,
                                                             const Dims& dims, const PadLayout& L, int bx, int by,
                                                             int bz)
         {
-            __shared__ float tile[kTile][kTile + 1];
+            __shared__ E tile[kTile][kTile + 1];
             const int f = L.f;
             const int t = 3 - f; // the axis that is neither x nor f
             const int x0 = bx * kTile, f0 = by * kTile, ct = bz;
@@ -70,26 +72,27 @@ namespace ehp
             }
         }
 
-        template <int D, int A>
+        template <int D, int A, bool V16>
         __device__ inline float fwd_ray(const RayRec& r, const Dims& dims, const ViewSet& vs)
         {
             if (D == 3 && (vs.fb2 >> A) & 1)
-                return plane_ray_forward<D, A, true>(r, dims, vs.v[A]);
-            return plane_ray_forward<D, A, false>(r, dims, vs.v[A]);
+                return plane_ray_forward<D, A, true, V16>(r, dims, vs.v[A]);
+            return plane_ray_forward<D, A, false, V16>(r, dims, vs.v[A]);
         }
 
+        template <class E>
         __global__ void __launch_bounds__(256)
-            view_copy_rows_kernel(const float* __restrict__ in, float* __restrict__ out, const Dims dims,
-                                  const PadLayout L)
+            view_copy_rows_kernel(const E* __restrict__ in, E* __restrict__ out, const Dims dims, const PadLayout L)
         {
-            view_copy_rows(in, out, dims, L, blockIdx.x);
+            view_copy_rows<E>(in, out, dims, L, blockIdx.x);
         }
 
+        template <class E>
         __global__ void __launch_bounds__(256)
-            view_copy_transpose_kernel(const float* __restrict__ in, float* __restrict__ out, const Dims dims,
+            view_copy_transpose_kernel(const E* __restrict__ in, E* __restrict__ out, const Dims dims,
                                        const PadLayout L)
         {
-            view_copy_transpose(in, out, dims, L, blockIdx.x, blockIdx.y, blockIdx.z);
+            view_copy_transpose<E>(in, out, dims, L, blockIdx.x, blockIdx.y, blockIdx.z);
         }
 
         // Blocks of the view copy of one volume into its view (rows: a flat range; transpose:
@@ -106,10 +109,10 @@ namespace ehp
         }
 
         // Block id of one pose-band tile of the forward: id = (band * nposes + pose) * tiles_u + tu
-        template <int D>
+        template <int D, class OT = float>
         __device__ __forceinline__ void fwd_planes_block(const PoseGeom* __restrict__ geo, const Dims& dims,
                                                          const ViewSet& views, const int tiles_u, const int64_t id,
-                                                         float* __restrict__ sino)
+                                                         OT* __restrict__ sino)
         {
             constexpr int TU = (D == 3) ? 64 : kThreads;
             constexpr int TV = kThreads / TU;
@@ -126,26 +129,27 @@ namespace ehp
             JRay jr;
             joseph_setup<D>(o, d, dims.vol, jr);
             const RayRec r = make_ray_rec<D>(jr);
+            constexpr bool V16 = sizeof(OT) == 2; // bfloat16 operators read bfloat16 views
             float val;
             if (jr.axis == 0)
-                val = fwd_ray<D, 0>(r, dims, views);
+                val = fwd_ray<D, 0, V16>(r, dims, views);
             else if (D == 2 || jr.axis == 1)
-                val = fwd_ray<D, 1>(r, dims, views);
+                val = fwd_ray<D, 1, V16>(r, dims, views);
             else
-                val = fwd_ray<D, D - 1>(r, dims, views);
-            sino[(int64_t) pose * dims.pix_per_pose + (int64_t) v * dims.det[0] + u] = val;
+                val = fwd_ray<D, D - 1, V16>(r, dims, views);
+            sino[(int64_t) pose * dims.pix_per_pose + (int64_t) v * dims.det[0] + u] = down<OT>(val);
         }
 
-        template <int D>
+        template <int D, class OT>
         __global__ void __launch_bounds__(kThreads)
             joseph_fwd_planes_kernel(const PoseGeom* __restrict__ geo, const Dims dims, const ViewSet views,
                                      const int tiles_u, const int per_xcd, const int64_t total,
-                                     float* __restrict__ sino)
+                                     OT* __restrict__ sino)
         {
             const int64_t id = (int64_t) (blockIdx.x % 8u) * per_xcd + (blockIdx.x / 8u);
             if (id >= total)
                 return;
-            fwd_planes_block<D>(geo, dims, views, tiles_u, id, sino);
+            fwd_planes_block<D, OT>(geo, dims, views, tiles_u, id, sino);
         }
 
         // ---- batches: one launch over the jobs of many operators; block b runs block
@@ -159,12 +163,12 @@ namespace ehp
             const int64_t b = (int64_t) blockIdx.x - J.block0;
             const float* in = vols + J.vol_off;
             if (J.L.f == 0) {
-                view_copy_rows(in, J.view, J.dims, J.L, b);
+                view_copy_rows<float>(in, J.view, J.dims, J.L, b);
             } else {
                 const int64_t gxy = (int64_t) J.gx * J.gy;
                 const int bz = (int) (b / gxy);
                 const int r = (int) (b - bz * gxy);
-                view_copy_transpose(in, J.view, J.dims, J.L, r % J.gx, r / J.gx, bz);
+                view_copy_transpose<float>(in, J.view, J.dims, J.L, r % J.gx, r / J.gx, bz);
             }
         }
 
@@ -177,39 +181,44 @@ namespace ehp
             if (id >= total)
                 return;
             const FwdJob& J = jobs[block_job[id]];
-            fwd_planes_block<D>(J.geo, J.dims, J.views, J.tiles_u, id - J.block0, sinos + J.sino_off);
+            fwd_planes_block<D, float>(J.geo, J.dims, J.views, J.tiles_u, id - J.block0, sinos + J.sino_off);
         }
     } // namespace
 
-    hipError_t joseph_view_copy(const Dims& dims, const PadLayout& L, const float* vol, float* view, hipStream_t s)
+    template <class E>
+    hipError_t joseph_view_copy(const Dims& dims, const PadLayout& L, const E* vol, E* view, hipStream_t s)
     {
         if (L.f < 0 || L.f > 2 || L.f == L.A)
             return hipErrorInvalidValue;
         if (L.f == 0) {
             const int64_t blocks = (dims.nvox + 255) / 256;
-            hipLaunchKernelGGL(view_copy_rows_kernel, dim3((unsigned) blocks), dim3(256), 0, s, vol, view, dims, L);
+            hipLaunchKernelGGL((view_copy_rows_kernel<E>), dim3((unsigned) blocks), dim3(256), 0, s, vol, view, dims,
+                               L);
         } else {
             const int t = 3 - L.f;
             const dim3 grid((unsigned) ((dims.vol[0] + kTile - 1) / kTile),
                             (unsigned) ((dims.vol[L.f] + kTile - 1) / kTile), (unsigned) dims.vol[t]);
-            hipLaunchKernelGGL(view_copy_transpose_kernel, grid, dim3(256), 0, s, vol, view, dims, L);
+            hipLaunchKernelGGL((view_copy_transpose_kernel<E>), grid, dim3(256), 0, s, vol, view, dims, L);
         }
         return hipGetLastError();
     }
 
+    // T = float (FP32 operators) or bf16 (bfloat16 storage: bfloat16 volume, views and
+    // sinogram, FP32 arithmetic)
+    template <class T>
     hipError_t joseph_forward_planes(const PoseGeom* geo, const Dims& dims, const PadLayout* layouts,
-                                     float* const* views, int axes_present, const float* vol, float* sino,
-                                     hipStream_t s)
+                                     void* const* views, int axes_present, const T* vol, T* sino, hipStream_t s)
     {
+        constexpr uint32_t es = sizeof(T);
         ViewSet vs{};
         for (int a = 0; a < dims.dim; ++a) {
             if (!(axes_present & (1 << a)))
                 continue;
             const PadLayout& L = layouts[a];
-            hipError_t e = joseph_view_copy(dims, L, vol, views[a], s);
+            hipError_t e = joseph_view_copy<T>(dims, L, vol, (T*) views[a], s);
             if (e != hipSuccess)
                 return e;
-            vs.v[a] = PadView{views[a], L.PS * 4u, L.RS * 4u, L.size};
+            vs.v[a] = PadView{views[a], L.PS * es, L.RS * es, L.size};
             const int b2 = dims.dim == 3 ? (a == 2 ? 1 : 2) : -1;
             if (L.f == b2)
                 vs.fb2 |= 1 << a;
@@ -224,13 +233,18 @@ namespace ehp
             return hipErrorInvalidValue;
         const dim3 grid((unsigned) (per_xcd * 8));
         if (dims.dim == 2)
-            hipLaunchKernelGGL((joseph_fwd_planes_kernel<2>), grid, dim3(kThreads), 0, s, geo, dims, vs, tiles_u,
+            hipLaunchKernelGGL((joseph_fwd_planes_kernel<2, T>), grid, dim3(kThreads), 0, s, geo, dims, vs, tiles_u,
                                (int) per_xcd, total, sino);
         else
-            hipLaunchKernelGGL((joseph_fwd_planes_kernel<3>), grid, dim3(kThreads), 0, s, geo, dims, vs, tiles_u,
+            hipLaunchKernelGGL((joseph_fwd_planes_kernel<3, T>), grid, dim3(kThreads), 0, s, geo, dims, vs, tiles_u,
                                (int) per_xcd, total, sino);
         return hipGetLastError();
     }
+
+    template hipError_t joseph_forward_planes<float>(const PoseGeom*, const Dims&, const PadLayout*, void* const*, int,
+                                                     const float*, float*, hipStream_t);
+    template hipError_t joseph_forward_planes<bf16>(const PoseGeom*, const Dims&, const PadLayout*, void* const*, int,
+                                                    const bf16*, bf16*, hipStream_t);
 
     // Jobs of a batched forward: the view copies of every operator (one job per view) and the
     // forward blocks (one job per operator); block_job maps each block to its job.

@@ -300,6 +300,21 @@ namespace ehp
 
     // Poses whose candidate rays one workgroup gathers between two barriers (at most
     // kMaxPoseBatch).
+    // Pose j of candidate k of a pose batch: the largest j with pref[j] <= k (pref is the
+    // prefix sum of the batch's per-pose candidate counts, pref[0] = 0, k < pref[batch]).
+    EHP_HD inline int batch_pose_slot(const int* pref, int batch, int k)
+    {
+        int lo = 0, hi = batch - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (pref[mid] <= k)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        return lo;
+    }
+
     constexpr int kMaxPoseBatch = 32;
     constexpr int kPoseBatch = 8;
 
@@ -451,11 +466,12 @@ namespace ehp
         return (int64_t) ca * L.PS + (int64_t) (cm + L.mpad) * L.RS + (cf + 2);
     }
 
-    // What the kernels read of a view; strides in bytes (a view is below 4 GiB).
+    // What the kernels read of a view; strides in bytes (a view is below 4 GiB). Views hold
+    // FP32 or bfloat16 elements.
     struct PadView {
-        const float* base;
+        const void* base;
         uint32_t PS4, RS4;
-        int64_t size; // floats in the view (bounds checks of the CPU build)
+        int64_t size; // elements in the view (bounds checks of the CPU build)
     };
 
 #if defined(EHP_VIEW_CHECK)
@@ -494,6 +510,26 @@ namespace ehp
         return p;
     }
 
+    // Two neighbouring bfloat16 elements of a view (one 4-byte load), widened to FP32.
+    EHP_HD inline Pair ld_view2_bf16(const PadView& V, uint32_t off2)
+    {
+#if defined(EHP_VIEW_CHECK)
+        if ((off2 & 1u) || (int64_t) (off2 >> 1) + 1 >= V.size) {
+            g_view_oob = true;
+            return Pair{0.0f, 0.0f};
+        }
+#endif
+        uint16_t h[2];
+        __builtin_memcpy(h, (const char*) V.base + off2, sizeof(h));
+        return Pair{bf16_to_float(bf16{h[0]}), bf16_to_float(bf16{h[1]})};
+    }
+
+    template <bool V16>
+    EHP_HD inline Pair ld_pair(const PadView& V, uint32_t off)
+    {
+        return V16 ? ld_view2_bf16(V, off) : ld_view2(V, off);
+    }
+
     EHP_HD inline uint32_t pad_coord(float g, int n)
     {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -513,9 +549,10 @@ namespace ehp
     // where elsa's ceil gives C = F: the weight of that tap is 0 then. Samples whose plane lies
     // outside the volume (a ray entering exactly through the far face) contribute 0 and are
     // skipped. Positions are those of joseph_sample and tile_ray_adjoint.
-    template <int D, int A, bool FB2>
+    template <int D, int A, bool FB2, bool V16 = false>
     EHP_HD inline float plane_ray_forward(const RayRec& r, const Dims& dims, const PadView& V)
     {
+        constexpr uint32_t ES = V16 ? 1u : 2u; // log2 of the element size
         constexpr int b1 = TileAxes<D, A>::b1, b2 = TileAxes<D, A>::b2;
         const int na = dims.vol[A], n1 = dims.vol[b1], n2 = (D == 3) ? dims.vol[b2] : 1;
         const bool neg = (r.meta & 4) != 0;
@@ -538,7 +575,7 @@ namespace ehp
             const float w1 = 1.0f - c1;
             const uint32_t i1 = pad_coord(g1, n1);
             if (D == 2) {
-                const Pair fk = ld_view2(V, poff + (i1 << 2));
+                const Pair fk = ld_pair<V16>(V, poff + (i1 << ES));
                 const float vf = fk.a, vk = fk.b;
                 // A = 0: (pa, C1) then (pa, F1); A = 1: (F0, pa) then (C0, pa)
                 if (A == 0) {
@@ -556,8 +593,8 @@ namespace ehp
                 const uint32_t i2 = pad_coord(g2, n2);
                 // the F and F + 1 taps along the fast axis are neighbours: two 8-byte loads, one per
                 // row of the slow axis
-                const uint32_t o = FB2 ? poff + mul24(i1, V.RS4) + (i2 << 2) : poff + mul24(i2, V.RS4) + (i1 << 2);
-                const Pair r0 = ld_view2(V, o), r1 = ld_view2(V, o + V.RS4);
+                const uint32_t o = FB2 ? poff + mul24(i1, V.RS4) + (i2 << ES) : poff + mul24(i2, V.RS4) + (i1 << ES);
+                const Pair r0 = ld_pair<V16>(V, o), r1 = ld_pair<V16>(V, o + V.RS4);
                 // FB2: rows are b1 = F1, F1 + 1, pairs run along b2; else rows are b2, pairs along b1
                 const float vff = r0.a;
                 const float vkf = FB2 ? r1.a : r0.b;

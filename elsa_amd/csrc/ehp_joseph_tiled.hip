@@ -63,13 +63,14 @@ namespace ehp
     // ---------------------------------------------------------------------------------
     // fixed-point scale of one back projection
 
+    template <class T>
     __global__ void __launch_bounds__(256)
-        absmax_kernel(const float* __restrict__ x, int64_t n, unsigned int* __restrict__ out)
+        absmax_kernel(const T* __restrict__ x, int64_t n, unsigned int* __restrict__ out)
     {
         float m = 0.0f;
         const int64_t stride = (int64_t) gridDim.x * blockDim.x;
         for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-            const float a = fabsf(x[i]);
+            const float a = fabsf(up(x[i]));
             m = (a > m || a != a) ? a : m; // NaN wins
         }
         for (int off = 32; off > 0; off >>= 1) {
@@ -86,7 +87,8 @@ namespace ehp
         fixexp[0] = fixed_point_exponent(__uint_as_float(*maxbits), bound);
     }
 
-    hipError_t fixed_point_exponent_launch(const float* y, int64_t n, double slot_bound, unsigned int* d_maxbits,
+    template <class T>
+    hipError_t fixed_point_exponent_launch(const T* y, int64_t n, double slot_bound, unsigned int* d_maxbits,
                                            int* d_fixexp, hipStream_t s)
     {
         hipError_t e = launch_fill<unsigned int>(d_maxbits, 1, 0u, s);
@@ -94,21 +96,26 @@ namespace ehp
             return e;
         int64_t blocks = (n + 255) / 256;
         blocks = blocks > 4096 ? 4096 : (blocks < 1 ? 1 : blocks);
-        hipLaunchKernelGGL(absmax_kernel, dim3((unsigned) blocks), dim3(256), 0, s, y, n, d_maxbits);
+        hipLaunchKernelGGL((absmax_kernel<T>), dim3((unsigned) blocks), dim3(256), 0, s, y, n, d_maxbits);
         hipLaunchKernelGGL(scale_kernel, dim3(1), dim3(1), 0, s, d_maxbits, slot_bound, d_fixexp);
         return hipGetLastError();
     }
+
+    template hipError_t fixed_point_exponent_launch<float>(const float*, int64_t, double, unsigned int*, int*,
+                                                           hipStream_t);
+    template hipError_t fixed_point_exponent_launch<bf16>(const bf16*, int64_t, double, unsigned int*, int*,
+                                                          hipStream_t);
 
     // ---------------------------------------------------------------------------------
     // the tiled back projection
 
     // One tile of the back projection along driving axis A (tile < ntA * ntB * ntC).
-    template <int D, int A>
+    template <int D, int A, class ST = float>
     __device__ __forceinline__ void adj_tiled_block(const PoseGeom* __restrict__ geo,
                                                     const RayRec* __restrict__ table, const Dims& dims,
                                                     const int* __restrict__ pose_axes, const TileCfg& tc,
                                                     const int ntA, const int ntB, const int tile, const int batch,
-                                                    const int* __restrict__ fixexp, const float* __restrict__ sino,
+                                                    const int* __restrict__ fixexp, const ST* __restrict__ sino,
                                                     float* __restrict__ vol)
     {
         extern __shared__ long long lds[];
@@ -150,9 +157,7 @@ namespace ehp
             const int total = s_pref[batch];
             // ray index of candidate k of this batch
             auto cand_ray = [&](int k) -> int64_t {
-                int j = 0;
-                while (k >= s_pref[j + 1])
-                    ++j;
+                const int j = batch_pose_slot(s_pref, batch, k);
                 const int loc = k - s_pref[j];
                 // loc / nu through the reciprocal: exact, as loc + 0.5 is at least 0.5 away from
                 // a multiple of nu and loc < 2^22
@@ -167,7 +172,7 @@ namespace ehp
             RayRec r_next{};
             if (tid < total) {
                 const int64_t ray = cand_ray(tid);
-                yv_next = sino[ray];
+                yv_next = up(sino[ray]);
                 r_next = table[ray];
             }
             for (int k = tid; k < total; k += kThreads) {
@@ -175,7 +180,7 @@ namespace ehp
                 const RayRec r = r_next;
                 if (k + kThreads < total) {
                     const int64_t ray = cand_ray(k + kThreads);
-                    yv_next = sino[ray];
+                    yv_next = up(sino[ray]);
                     r_next = table[ray];
                 }
                 if (yv == 0.0f)
@@ -199,19 +204,19 @@ namespace ehp
         }
     }
 
-    template <int D, int A>
+    template <int D, int A, class ST>
     __global__ void __launch_bounds__(kThreads)
         joseph_adj_tiled_kernel(const PoseGeom* __restrict__ geo, const RayRec* __restrict__ table, const Dims dims,
                                 const int* __restrict__ pose_axes, const TileCfg tc, const int ntA, const int ntB,
                                 const int ntC, const int tiles_per_xcd, const int batch,
-                                const int* __restrict__ fixexp, const float* __restrict__ sino,
+                                const int* __restrict__ fixexp, const ST* __restrict__ sino,
                                 float* __restrict__ vol)
     {
         // consecutive workgroups land on different XCDs: give each XCD a contiguous block of tiles
         const int tile = (int) (blockIdx.x % 8u) * tiles_per_xcd + (int) (blockIdx.x / 8u);
         if (tile >= ntA * ntB * ntC)
             return;
-        adj_tiled_block<D, A>(geo, table, dims, pose_axes, tc, ntA, ntB, tile, batch, fixexp, sino, vol);
+        adj_tiled_block<D, A, ST>(geo, table, dims, pose_axes, tc, ntA, ntB, tile, batch, fixexp, sino, vol);
     }
 
     // the tiles of axis A of many operators in one launch (block b: tile b - job.block0 of job
@@ -227,8 +232,8 @@ namespace ehp
             return;
         const int j = block_job[id];
         const AdjJob& J = jobs[j];
-        adj_tiled_block<D, A>(J.geo, J.table, J.dims, J.pose_axes, J.tc, J.ntA, J.ntB, (int) (id - J.block0),
-                              J.batch, fixexp + J.item, sinos + J.sino_off, vols + J.vol_off);
+        adj_tiled_block<D, A, float>(J.geo, J.table, J.dims, J.pose_axes, J.tc, J.ntA, J.ntB, (int) (id - J.block0),
+                                     J.batch, fixexp + J.item, sinos + J.sino_off, vols + J.vol_off);
     }
 
     // max|y| per operator of a batch (block b scans its share of segment seg_of_block[b])
@@ -291,9 +296,12 @@ namespace ehp
         return (size_t) tc.LA * PC * (tc.TB + 2) * sizeof(long long);
     }
 
+    // ST: the sinogram's storage type (float, or bf16 for bfloat16 operators); the volume is
+    // FP32 either way.
+    template <class ST>
     hipError_t joseph_adjoint_tiled(const PoseGeom* geo, const RayRec* table, const Dims& dims, const int* pose_axes,
                                     int axes_present, const TileCfg& tc, int batch, double slot_bound,
-                                    unsigned int* d_maxbits, int* d_fixexp, const float* sino, float* vol,
+                                    unsigned int* d_maxbits, int* d_fixexp, const ST* sino, float* vol,
                                     hipStream_t s)
     {
         if (batch < 1 || batch > kMaxPoseBatch)
@@ -312,8 +320,8 @@ namespace ehp
             const int per_xcd = (ntA * ntB * ntC + 7) / 8;
             const dim3 grid((unsigned) (per_xcd * 8));
 #define EHP_ADJ_TILED(Dv, Av)                                                                                      \
-    hipLaunchKernelGGL((joseph_adj_tiled_kernel<Dv, Av>), grid, dim3(kThreads), lds, s, geo, table, dims, pose_axes, \
-                       tc, ntA, ntB, ntC, per_xcd, batch, d_fixexp, sino, vol)
+    hipLaunchKernelGGL((joseph_adj_tiled_kernel<Dv, Av, ST>), grid, dim3(kThreads), lds, s, geo, table, dims,         \
+                       pose_axes, tc, ntA, ntB, ntC, per_xcd, batch, d_fixexp, sino, vol)
             if (dims.dim == 2) {
                 if (a == 0)
                     EHP_ADJ_TILED(2, 0);
@@ -334,6 +342,13 @@ namespace ehp
         }
         return hipSuccess;
     }
+
+    template hipError_t joseph_adjoint_tiled<float>(const PoseGeom*, const RayRec*, const Dims&, const int*, int,
+                                                    const TileCfg&, int, double, unsigned int*, int*, const float*,
+                                                    float*, hipStream_t);
+    template hipError_t joseph_adjoint_tiled<bf16>(const PoseGeom*, const RayRec*, const Dims&, const int*, int,
+                                                   const TileCfg&, int, double, unsigned int*, int*, const bf16*,
+                                                   float*, hipStream_t);
 
     // Segments of a batch's sinograms (absmax) and the tiles of every operator per driving
     // axis (one job per operator that has rays of that axis).

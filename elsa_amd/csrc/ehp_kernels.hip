@@ -34,6 +34,24 @@ namespace ehp
         return hipGetLastError();
     }
 
+    __global__ void __launch_bounds__(256) to_bf16_kernel(const float* __restrict__ src, bf16* __restrict__ dst, int64_t n)
+    {
+        const int64_t stride = (int64_t) gridDim.x * blockDim.x;
+        for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+            dst[i] = float_to_bf16(src[i]);
+    }
+
+    hipError_t launch_to_bf16(const float* src, bf16* dst, int64_t n, hipStream_t s)
+    {
+        if (n <= 0)
+            return hipSuccess;
+        int64_t blocks = (n + 255) / 256;
+        if (blocks > 8192)
+            blocks = 8192;
+        hipLaunchKernelGGL(to_bf16_kernel, dim3((unsigned) blocks), dim3(256), 0, s, src, dst, n);
+        return hipGetLastError();
+    }
+
     // ---------------------------------------------------------------------------------
     // Joseph: one thread per ray (pixel of one pose). Grid: x = pixel block, y = pose.
 
@@ -409,6 +427,7 @@ namespace ehp
     EHP_INSTANTIATE(float)
     EHP_INSTANTIATE(double)
     template hipError_t launch_fill<int>(int*, int64_t, int, hipStream_t);
+    template hipError_t launch_fill<bf16>(bf16*, int64_t, bf16, hipStream_t);
     template hipError_t launch_fill<unsigned int>(unsigned int*, int64_t, unsigned int, hipStream_t);
 #undef EHP_INSTANTIATE
 } // namespace ehp

@@ -100,12 +100,16 @@ namespace ehp
     hipError_t joseph_ray_table(const PoseGeom* geo, const Dims& dims, RayRec* table, hipStream_t s);
     hipError_t joseph_pose_axes(const RayRec* table, const Dims& dims, int* pose_axes, hipStream_t s);
     size_t joseph_tile_lds_bytes(const Dims& dims, const TileCfg& tc);
+    template <class ST>
     hipError_t joseph_adjoint_tiled(const PoseGeom* geo, const RayRec* table, const Dims& dims, const int* pose_axes,
                                     int axes_present, const TileCfg& tc, int batch, double slot_bound,
-                                    unsigned int* d_maxbits, int* d_fixexp, const float* sino, float* vol,
+                                    unsigned int* d_maxbits, int* d_fixexp, const ST* sino, float* vol,
                                     hipStream_t s);
-    hipError_t fixed_point_exponent_launch(const float* y, int64_t n, double slot_bound, unsigned int* d_maxbits,
+    template <class T>
+    hipError_t fixed_point_exponent_launch(const T* y, int64_t n, double slot_bound, unsigned int* d_maxbits,
                                            int* d_fixexp, hipStream_t s);
+    // dst[i] = round-to-nearest-even bfloat16 of src[i]
+    hipError_t launch_to_bf16(const float* src, bf16* dst, int64_t n, hipStream_t s);
     hipError_t siddon_ray_table(const PoseGeom* geo, const Dims& dims, SidRec* table, int* pose_axes, hipStream_t s);
     size_t siddon_tile_lds_bytes(const Dims& dims, const TileCfg& tc);
     hipError_t siddon_adjoint_tiled(const PoseGeom* geo, const SidRec* table, const Dims& dims, const int* pose_axes,
@@ -118,10 +122,9 @@ namespace ehp
                                    int diff, const int* pose_axis, const float* pose_margin, const TileCfg& tc,
                                    float lutscale, double slot_bound, unsigned int* d_maxbits, int* d_fixexp,
                                    const float* vol, float* sino, hipStream_t s);
-    hipError_t joseph_view_copy(const Dims& dims, const PadLayout& L, const float* vol, float* view, hipStream_t s);
+    template <class T>
     hipError_t joseph_forward_planes(const PoseGeom* geo, const Dims& dims, const PadLayout* layouts,
-                                     float* const* views, int axes_present, const float* vol, float* sino,
-                                     hipStream_t s);
+                                     void* const* views, int axes_present, const T* vol, T* sino, hipStream_t s);
 
     template <class T>
     hipError_t siddon_forward(const PoseGeom* geo, const Dims& dims, const T* vol, T* sino, hipStream_t s);

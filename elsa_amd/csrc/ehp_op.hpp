@@ -30,7 +30,8 @@ struct ehp_op {
     // Joseph, FP32: plane-major forward and LDS-tiled back projection
     bool planes = false;
     ehp::PadLayout layouts[3]{};
-    float* d_view[3] = {nullptr, nullptr, nullptr}; // zero-padded plane-major volume views
+    void* d_view[3] = {nullptr, nullptr, nullptr}; // zero-padded plane-major volume views (op's storage type)
+    float* d_acc32 = nullptr;                        // bfloat16 operators: FP32 back projection before rounding
     bool tiled = false;
     int* d_pose_axes = nullptr;
     int axes_present = 0;

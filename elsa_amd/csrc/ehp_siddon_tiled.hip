@@ -133,9 +133,7 @@ namespace ehp
             __syncthreads();
             const int total = s_pref[batch];
             auto cand_ray = [&](int k) -> int64_t {
-                int j = 0;
-                while (k >= s_pref[j + 1])
-                    ++j;
+                const int j = batch_pose_slot(s_pref, batch, k);
                 const int loc = k - s_pref[j];
                 // loc / nu through the reciprocal: exact, as loc + 0.5 is at least 0.5 away from
                 // a multiple of nu and loc < 2^22

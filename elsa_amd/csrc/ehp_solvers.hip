@@ -87,7 +87,7 @@ namespace ehp
             const int64_t end = S.off[seg + 1];
             double acc = 0.0;
             for (int64_t i = S.off[seg] + lb * kThreads + threadIdx.x; i < end; i += nb * kThreads)
-                acc += (double) a[i] * (double) b[i];
+                acc += (double) up(a[i]) * (double) up(b[i]);
             __shared__ double red[kThreads / 64];
             for (int off = 32; off > 0; off >>= 1)
                 acc += __shfl_down(acc, off, 64);
@@ -141,16 +141,17 @@ namespace ehp
         {
             const int64_t i = (int64_t) blockIdx.x * kThreads + threadIdx.x;
             if (i < n)
-                s[i] = b[i] - s[i];
+                s[i] = down<T>(up(b[i]) - up(s[i]));
         }
 
-        // y = a * x + y with a host scalar
+        // y = a * x + y with a host scalar (in the arithmetic type)
         template <class T>
-        __global__ void __launch_bounds__(kThreads) axpy_kernel(T a, const T* __restrict__ x, T* __restrict__ y, int64_t n)
+        __global__ void __launch_bounds__(kThreads)
+            axpy_kernel(typename Acc<T>::type a, const T* __restrict__ x, T* __restrict__ y, int64_t n)
         {
             const int64_t i = (int64_t) blockIdx.x * kThreads + threadIdx.x;
             if (i < n)
-                y[i] = a * x[i] + y[i];
+                y[i] = down<T>(a * up(x[i]) + up(y[i]));
         }
 
         // CGLS: y += sign * alpha c per problem (x += alpha c over the domain, s -= alpha q over the
@@ -164,11 +165,12 @@ namespace ehp
             int64_t i;
             if (!seg_elem(S, seg, i) || scal(sc, seg, S_DONE) != 0.0)
                 return;
-            const T alpha = (T) scal(sc, seg, S_ALPHA);
+            using A = typename Acc<T>::type;
+            const A alpha = (A) scal(sc, seg, S_ALPHA);
             if (negate)
-                y[i] -= alpha * c[i];
+                y[i] = down<T>(up(y[i]) - alpha * up(c[i]));
             else
-                y[i] += alpha * c[i];
+                y[i] = down<T>(up(y[i]) + alpha * up(c[i]));
         }
 
         // c = r + beta c
@@ -180,8 +182,9 @@ namespace ehp
             int64_t i;
             if (!seg_elem(S, seg, i))
                 return;
-            const T beta = (T) scal(sc, seg, S_BETA);
-            c[i] = r[i] + beta * c[i];
+            using A = typename Acc<T>::type;
+            const A beta = (A) scal(sc, seg, S_BETA);
+            c[i] = down<T>(up(r[i]) + beta * up(c[i]));
         }
 
         // start of a CGLS iteration: stop when kold < tol (CGLS.cpp:54-58), else count it
@@ -297,7 +300,7 @@ namespace ehp
             int seg;
             int64_t i;
             if (seg_elem(S, seg, i))
-                u[i] = (T) ((double) u[i] / sqrt(scal(sc, seg, slot)));
+                u[i] = down<T>((typename Acc<T>::type) ((double) up(u[i]) / sqrt(scal(sc, seg, slot))));
         }
 
         inline unsigned blocks(int64_t n)
@@ -538,13 +541,14 @@ namespace ehp
                     rc = A.fwd(x, sres, s);
                 hipLaunchKernelGGL((rsub_kernel<T>), dim3(blocks(nr)), dim3(kThreads), 0, s, b, sres, nr);
             } else {
-                ok(launch_fill<T>(x, nd, T(0), s));
+                ok(launch_fill<T>(x, nd, down<T>(0), s));
                 hipLaunchKernelGGL((copy_kernel<T>), dim3(blocks(nr)), dim3(kThreads), 0, s, b, sres, nr);
             }
             if (rc == EHP_OK)
                 rc = A.adj(sres, r, s);
             if (use_x0 && damp != 0.0)
-                hipLaunchKernelGGL((axpy_kernel<T>), dim3(blocks(nd)), dim3(kThreads), 0, s, (T) -damp, x, r, nd);
+                hipLaunchKernelGGL((axpy_kernel<T>), dim3(blocks(nd)), dim3(kThreads), 0, s, (typename Acc<T>::type) -damp, x,
+                                   r, nd);
             hipLaunchKernelGGL((copy_kernel<T>), dim3(blocks(nd)), dim3(kThreads), 0, s, r, c, nd);
             ok(dot<T>(r, r, D, nseg, partial, sc, S_KOLD, nullptr, nh, s));
             ok(dot<T>(sres, sres, R, nseg, partial, sc, S_SN, hist, nh, s)); // hist[0] = ||s||^2 at the start
@@ -568,7 +572,8 @@ namespace ehp
                 if (r2 != EHP_OK)
                     return r2;
                 if (damp != 0.0)
-                    hipLaunchKernelGGL((axpy_kernel<T>), dim3(blocks(nd)), dim3(kThreads), 0, s, (T) -damp, x, r, nd);
+                    hipLaunchKernelGGL((axpy_kernel<T>), dim3(blocks(nd)), dim3(kThreads), 0, s, (typename Acc<T>::type) -damp, x,
+                                   r, nd);
                 if (dot<T>(r, r, D, nseg, partial, sc, S_K, nullptr, nh, s) != hipSuccess)
                     return EHP_ERR_HIP;
                 hipLaunchKernelGGL(cgls_beta_kernel, dim3(seg_blocks(nseg)), dim3(64), 0, s, sc, nseg);
@@ -607,7 +612,7 @@ namespace ehp
             if (sv.m.alloc(&u, sv.nd * sizeof(T)) != hipSuccess || sv.m.alloc(&t, sv.nr * sizeof(T)) != hipSuccess ||
                 sv.m.alloc(&v, sv.nd * sizeof(T)) != hipSuccess)
                 return ehp_solver_fail(EHP_ERR_ALLOC, "hipMalloc of the power-iteration workspace failed");
-            if (launch_fill<T>(u, sv.nd, T(1), s) != hipSuccess)
+            if (launch_fill<T>(u, sv.nd, down<T>(1), s) != hipSuccess)
                 return ehp_solver_fail(EHP_ERR_HIP, "fill");
             // u = A^T A u; u /= ||u|| (PowerIterations.cpp:18-23)
             for (int i = 0; i < niters; ++i) {
@@ -684,7 +689,7 @@ namespace ehp
                 m.alloc(&grad, nd * sizeof(T)) != hipSuccess || m.alloc(&atb, nd * sizeof(T)) != hipSuccess ||
                 m.alloc(&ay, nr * sizeof(T)) != hipSuccess)
                 return ehp_solver_fail(EHP_ERR_ALLOC, "hipMalloc of the APGD workspace failed");
-            if (!use_x0 && launch_fill<T>(x, nd, T(0), s) != hipSuccess)
+            if (!use_x0 && launch_fill<T>(x, nd, down<T>(0), s) != hipSuccess)
                 return ehp_solver_fail(EHP_ERR_HIP, "fill");
             hipLaunchKernelGGL((copy_kernel<T>), dim3(blocks(nd)), dim3(kThreads), 0, s, x, xprev, nd);
             hipLaunchKernelGGL((copy_kernel<T>), dim3(blocks(nd)), dim3(kThreads), 0, s, x, y, nd);
@@ -696,7 +701,8 @@ namespace ehp
                 if (r2 == EHP_OK)
                     r2 = A.adj(ay, grad, s);
                 if (r2 == EHP_OK)
-                    hipLaunchKernelGGL((axpy_kernel<T>), dim3(blocks(nd)), dim3(kThreads), 0, s, T(-1), atb, grad, nd);
+                    hipLaunchKernelGGL((axpy_kernel<T>), dim3(blocks(nd)), dim3(kThreads), 0, s, (typename Acc<T>::type) -1, atb,
+                                       grad, nd);
                 return r2;
             };
             if (rc == EHP_OK)
@@ -734,11 +740,98 @@ namespace ehp
             }
             return EHP_OK;
         }
+        // r = b - q formed in FP64 and rounded to bfloat16, and partial sums of its square (the
+        // residual norm of the refinement, FP64, before rounding)
+        __global__ void __launch_bounds__(kThreads)
+            refine_residual_kernel(const float* __restrict__ b, const float* __restrict__ q, bf16* __restrict__ r,
+                                   const SegSpace S, double* __restrict__ partial)
+        {
+            const int seg = S.rseg[blockIdx.x];
+            const int64_t lb = (int64_t) blockIdx.x - S.rb0[seg];
+            const int64_t nb = S.rb0[seg + 1] - S.rb0[seg];
+            const int64_t end = S.off[seg + 1];
+            double acc = 0.0;
+            for (int64_t i = S.off[seg] + lb * kThreads + threadIdx.x; i < end; i += nb * kThreads) {
+                const double ri = (double) b[i] - (double) q[i];
+                r[i] = float_to_bf16((float) ri);
+                acc += ri * ri;
+            }
+            __shared__ double red[kThreads / 64];
+            for (int off = 32; off > 0; off >>= 1)
+                acc += __shfl_down(acc, off, 64);
+            if ((threadIdx.x & 63) == 0)
+                red[threadIdx.x >> 6] = acc;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                double sum = 0.0;
+                for (int w = 0; w < kThreads / 64; ++w)
+                    sum += red[w];
+                partial[blockIdx.x] = sum;
+            }
+        }
+
+        // x += d (FP32 += bfloat16)
+        __global__ void __launch_bounds__(kThreads)
+            refine_update_kernel(float* __restrict__ x, const bf16* __restrict__ d, int64_t n)
+        {
+            const int64_t i = (int64_t) blockIdx.x * kThreads + threadIdx.x;
+            if (i < n)
+                x[i] += bf16_to_float(d[i]);
+        }
     } // namespace
+
+    int solve_cgls_refined(const SolveOps& Ahi, const SolveOps& Alo, const float* b, float* x, int use_x0, int outer,
+                           int inner, bool graph, double* residuals, hipStream_t caller)
+    {
+        Solve sv;
+        int rc = sv.init(Ahi, caller, "hipMalloc of the refinement workspace failed");
+        if (rc != EHP_OK)
+            return rc;
+        hipStream_t s = sv.ss.s;
+        float* q;
+        bf16 *r, *d;
+        if (sv.m.alloc(&q, sv.nr * sizeof(float)) != hipSuccess || sv.m.alloc(&r, sv.nr * sizeof(bf16)) != hipSuccess ||
+            sv.m.alloc(&d, sv.nd * sizeof(bf16)) != hipSuccess)
+            return ehp_solver_fail(EHP_ERR_ALLOC, "hipMalloc of the refinement workspace failed");
+        if (!use_x0 && launch_fill<float>(x, sv.nd, 0.0f, s) != hipSuccess)
+            return ehp_solver_fail(EHP_ERR_HIP, "fill");
+        for (int k = 0; k <= outer; ++k) {
+            // r = b - A_hi x in FP64, ||r||^2 in FP64
+            rc = Ahi.fwd(x, q, s);
+            if (rc != EHP_OK)
+                return rc;
+            hipLaunchKernelGGL(refine_residual_kernel, dim3((unsigned) sv.R.nrb), dim3(kThreads), 0, s, b, q, r, sv.R,
+                               sv.partial);
+            hipLaunchKernelGGL(dot_final_kernel, dim3((unsigned) sv.nseg), dim3(kThreads), 0, s, sv.partial, sv.R,
+                               sv.sc, (int) S_SN, (double*) nullptr, 0);
+            double rr = 0.0;
+            hipError_t e = hipMemcpyAsync(&rr, sv.sc + S_SN, sizeof(double), hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess)
+                e = hipStreamSynchronize(s);
+            if (e != hipSuccess)
+                return ehp_solver_fail(EHP_ERR_HIP, hipGetErrorString(e));
+            if (residuals)
+                residuals[k] = std::sqrt(rr);
+            if (k == outer)
+                break;
+            // d ~ argmin ||A_lo d - r|| by CGLS on bfloat16 storage, then x += d
+            rc = solve_cgls(Alo, EHP_BF16, r, d, 0, inner, 0.0, 0.0, graph, nullptr, nullptr, s);
+            if (rc != EHP_OK)
+                return rc;
+            hipLaunchKernelGGL(refine_update_kernel, dim3(blocks(sv.nd)), dim3(kThreads), 0, s, x, d, sv.nd);
+            e = hipGetLastError();
+            if (e != hipSuccess)
+                return ehp_solver_fail(EHP_ERR_HIP, hipGetErrorString(e));
+        }
+        return EHP_OK;
+    }
 
     int solve_cgls(const SolveOps& A, int dtype, const void* b, void* x, int use_x0, int iterations, double eps,
                    double tol, bool graph, double* residuals, int* iters_done, hipStream_t s)
     {
+        if (dtype == EHP_BF16)
+            return cgls<bf16>(A, (const bf16*) b, (bf16*) x, use_x0, iterations, eps, tol, graph, residuals, iters_done,
+                              s);
         if (dtype == EHP_F64)
             return cgls<double>(A, (const double*) b, (double*) x, use_x0, iterations, eps, tol, graph, residuals,
                                 iters_done, s);
@@ -750,6 +843,8 @@ namespace ehp
                    double sigma, double epsilon, bool graph, double* grad_norms, int* iters_done, double* mu_used,
                    hipStream_t s)
     {
+        if (dtype == EHP_BF16)
+            return ehp_solver_fail(EHP_ERR_UNSUPPORTED, "APGD on bfloat16 storage is not implemented");
         if (dtype == EHP_F64)
             return apgd<double>(A, (const double*) b, (double*) x, use_x0, iterations, mu, sigma, epsilon, graph,
                                 grad_norms, iters_done, mu_used, s);
@@ -765,6 +860,8 @@ namespace ehp
             return rc;
         if (dtype == EHP_F64)
             return power_iterations<double>(A, sv, niters, L);
+        if (dtype == EHP_BF16)
+            return power_iterations<bf16>(A, sv, niters, L);
         return power_iterations<float>(A, sv, niters, L);
     }
 } // namespace ehp

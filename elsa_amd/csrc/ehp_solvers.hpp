@@ -24,4 +24,7 @@ namespace ehp
                    double sigma, double epsilon, bool graph, double* grad_norms, int* iters_done, double* mu_used,
                    hipStream_t s);
     int solve_power_iterations(const SolveOps& A, int dtype, int niters, double* L, hipStream_t s);
+    // CGLS with iterative refinement: FP32 operator Ahi, bfloat16 operator Alo, one problem
+    int solve_cgls_refined(const SolveOps& Ahi, const SolveOps& Alo, const float* b, float* x, int use_x0, int outer,
+                           int inner, bool graph, double* residuals, hipStream_t s);
 } // namespace ehp

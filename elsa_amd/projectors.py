@@ -29,7 +29,24 @@ import numpy as np
 from . import _lib
 from .descriptors import DetectorDescriptor, VolumeDescriptor
 
-_DT = {np.dtype(np.float32): _lib.EHP_F32, np.dtype(np.float64): _lib.EHP_F64}
+# bfloat16 storage (FP32 arithmetic; Joseph only): dtype="bfloat16" or torch.bfloat16. Data are
+# torch.bfloat16 GPU tensors, or numpy arrays rounded to bfloat16 on the way in and returned as
+# float32.
+BF16 = "bfloat16"
+_DT = {np.dtype(np.float32): _lib.EHP_F32, np.dtype(np.float64): _lib.EHP_F64, BF16: _lib.EHP_BF16}
+
+
+def _dtype_key(dtype):
+    if isinstance(dtype, str) and dtype.lower() in ("bfloat16", "bf16"):
+        return BF16
+    if str(dtype) == "torch.bfloat16":
+        return BF16
+    return np.dtype(dtype)
+
+
+def _torch_dtype(key):
+    torch = _torch()
+    return {BF16: torch.bfloat16, np.dtype(np.float32): torch.float32, np.dtype(np.float64): torch.float64}[key]
 
 
 def _torch():
@@ -75,7 +92,7 @@ class _HipProjector(LinearOperator):
             raise ValueError(f"{type(self).__name__}: domain and range dimension need to match")
         if rangeDescriptor.getNumberOfGeometryPoses() == 0:
             raise ValueError(f"{type(self).__name__}: geometry list was empty")
-        self._dtype = np.dtype(dtype)
+        self._dtype = _dtype_key(dtype)
         if self._dtype not in _DT:
             raise ValueError(f"unsupported dtype {dtype}")
         self._op = ctypes.c_void_p()
@@ -118,6 +135,13 @@ class _HipProjector(LinearOperator):
         _lib.check(self._L.ehp_tile(self._op, la, tb, tc, *(ctypes.byref(x) for x in o)), "tile")
         return tuple(x.value for x in o)
 
+    def pose_batch(self, n: int = 0) -> int:
+        """Set (n > 0) and return the poses per step of the tiled back projection (FP32 Joseph
+        and Siddon)."""
+        o = ctypes.c_int()
+        _lib.check(self._L.ehp_pose_batch(self._op, int(n), ctypes.byref(o)), "pose_batch")
+        return o.value
+
     def sample_count(self) -> int:
         """Exact number of samples (Joseph), voxel crossings (Siddon) or footprint taps
         (voxel projectors) of one pass."""
@@ -130,9 +154,12 @@ class _HipProjector(LinearOperator):
         n_in, n_out = (self._n_sino, self._n_vol) if adjoint else (self._n_vol, self._n_sino)
         shape_out = self._domain.shape if adjoint else self._range.shape
         to_numpy = isinstance(x, np.ndarray)
-        tdt = torch.float32 if self._dtype == np.float32 else torch.float64
+        tdt = _torch_dtype(self._dtype)
         if to_numpy:
-            xt = torch.from_numpy(np.ascontiguousarray(x, dtype=self._dtype)).cuda()
+            if self._dtype == BF16:
+                xt = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32)).cuda().to(torch.bfloat16)
+            else:
+                xt = torch.from_numpy(np.ascontiguousarray(x, dtype=self._dtype)).cuda()
         else:
             if not x.is_cuda:
                 raise ValueError("input tensor must live on the GPU")
@@ -154,7 +181,7 @@ class _HipProjector(LinearOperator):
         _lib.check(f(self._op, ctypes.c_void_p(xt.data_ptr()), ctypes.c_void_p(ot.data_ptr()), stream),
                    "applyAdjoint" if adjoint else "apply")
         if to_numpy:
-            return ot.cpu().numpy()
+            return ot.float().cpu().numpy() if self._dtype == BF16 else ot.cpu().numpy()
         return ot
 
 

@@ -53,11 +53,16 @@ def _is_batch(A):
 
 
 def _prepare(A, b, x0):
+    from .projectors import BF16, _torch_dtype
+
     torch = _torch()
-    tdt = torch.float32 if A._dtype == np.float32 else torch.float64
+    tdt = _torch_dtype(A._dtype)
     to_numpy = isinstance(b, np.ndarray)
     if to_numpy:
-        bt = torch.from_numpy(np.ascontiguousarray(b, dtype=A._dtype)).cuda()
+        if A._dtype == BF16:
+            bt = torch.from_numpy(np.ascontiguousarray(b, dtype=np.float32)).cuda().to(tdt)
+        else:
+            bt = torch.from_numpy(np.ascontiguousarray(b, dtype=A._dtype)).cuda()
     else:
         if not (b.is_cuda and b.dtype == tdt):
             raise ValueError("b must be a GPU tensor of the operator's dtype")
@@ -68,7 +73,8 @@ def _prepare(A, b, x0):
     if x0 is None:
         xt = torch.empty(shape, dtype=tdt, device=bt.device)
     elif isinstance(x0, np.ndarray):
-        xt = torch.from_numpy(np.array(x0, dtype=A._dtype).reshape(shape)).cuda()
+        xt = torch.from_numpy(np.array(x0, dtype=np.float32 if A._dtype == BF16 else A._dtype)
+                              .reshape(shape)).cuda().to(tdt)
     else:
         xt = x0.detach().to(dtype=tdt).reshape(shape).contiguous().clone()
     if xt.numel() != A._n_vol:
@@ -101,7 +107,7 @@ class CGLS:
             self.iterations_done, self.residuals = n, res
         else:
             self.iterations_done, self.residuals = int(n[0]), res[0]
-        return xt.cpu().numpy() if to_numpy else xt
+        return (xt.float() if xt.dtype == _torch().bfloat16 else xt).cpu().numpy() if to_numpy else xt
 
 
 class APGD:
@@ -115,7 +121,8 @@ class APGD:
             self.mu = np.ascontiguousarray(mu, np.float64)
         else:
             self.mu = float(mu) if mu is not None else 0.0
-        self.epsilon = float(np.finfo(A._dtype).eps) if epsilon is None else float(epsilon)
+        self.epsilon = float(np.finfo(A._dtype if A._dtype != "bfloat16" else np.float32).eps) if epsilon is None \
+            else float(epsilon)
         self.iterations_done = 0
         self.residuals = None
         self.mu_used = None
@@ -138,6 +145,29 @@ class APGD:
         else:
             _lib.check(self.A._L.ehp_apgd(self.A.handle, *args, self.mu, *tail), "APGD")
             self.iterations_done, self.residuals, self.mu_used = int(n[0]), res[0], float(mu_used[0])
+        return (xt.float() if xt.dtype == _torch().bfloat16 else xt).cpu().numpy() if to_numpy else xt
+
+
+class CGLSRefined:
+    """CGLS with iterative refinement in mixed precision (no counterpart in elsa): x (FP32)
+    takes `outer` steps x += d, each d from `inner` CGLS iterations on A_lo, a bfloat16
+    operator, applied to r = b - A_hi x, the residual of the FP32 operator A_hi of the same
+    geometry formed in FP64. b and x are FP32. After solve(), `residuals` holds ||b - A_hi x||
+    before the first step and after each."""
+
+    def __init__(self, A_hi, A_lo, b, inner=10, graph=False):
+        self.A_hi, self.A_lo, self.b, self.inner, self.graph = A_hi, A_lo, b, int(inner), bool(graph)
+        self.residuals = None
+
+    def solve(self, outer, x0=None):
+        bt, xt, to_numpy, stream = _prepare(self.A_hi, self.b, x0)
+        res = np.empty(outer + 1, np.float64)
+        flags = _lib.EHP_SOLVE_GRAPH if self.graph else 0
+        _lib.check(self.A_hi._L.ehp_cgls_refined(self.A_hi.handle, self.A_lo.handle, ctypes.c_void_p(bt.data_ptr()),
+                                                 ctypes.c_void_p(xt.data_ptr()), int(x0 is not None), int(outer),
+                                                 self.inner, flags, res.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                 stream), "CGLSRefined")
+        self.residuals = res
         return xt.cpu().numpy() if to_numpy else xt
 
 

@@ -63,7 +63,10 @@ enum ehp_joseph_flags {
 
 enum ehp_dtype {
     EHP_F32 = 0,  /* data_t = float  */
-    EHP_F64 = 1   /* data_t = double */
+    EHP_F64 = 1,  /* data_t = double */
+    EHP_BF16 = 2  /* bfloat16 storage (raw bits, the upper half of an IEEE single) of volume and
+                     sinogram, FP32 arithmetic: Joseph with the default (fast) kernels only. No
+                     counterpart in elsa; the storage form of mixed-precision reconstruction */
 };
 
 typedef struct ehp_geometry ehp_geometry; /* device-resident per-pose matrices      */
@@ -130,6 +133,12 @@ int ehp_sample_count(const ehp_op* op, uint64_t* samples);
  * are given and la <= 0. */
 int ehp_tile(ehp_op* op, int la, int tb, int tc, int* la_out, int* tb_out, int* tc_out);
 
+/* Poses per step of the tiled back projections (Joseph, Siddon; 1-32, default 8): each step
+ * gathers the candidate rays of that many poses for one tile, so larger steps keep more lanes
+ * busy where few rays meet a tile (2D). No counterpart in elsa; a knob of the tile autotuner.
+ * Queries only when batch <= 0. */
+int ehp_pose_batch(ehp_op* op, int batch, int* batch_out);
+
 /* Per-ray Joseph traversal state as the kernels compute it (for parity checks): driving
  * axis and number of samples of every ray, host memory of n_sino entries each. */
 int ehp_joseph_ray_info(const ehp_op* op, int32_t* axis, int64_t* nsteps);
@@ -161,6 +170,15 @@ int ehp_cgls(ehp_op* op, const void* d_b, void* d_x, int use_x0, int iterations,
  * ||grad||^2 per iteration taken (-1 otherwise). */
 int ehp_apgd(ehp_op* op, const void* d_b, void* d_x, int use_x0, int iterations, double mu, double l1_sigma,
              double epsilon, int flags, double* grad_sq_norms, int* iterations_done, double* mu_used, void* stream);
+
+/* CGLS with iterative refinement in mixed precision (no counterpart in elsa): x (FP32) takes
+ * `outer` steps x += d. Each d approximately minimises ||A d - r|| by `inner` CGLS iterations
+ * (the ehp_cgls iteration, eps = 0) on op_lo, a bfloat16 operator; r = b - A x is the
+ * residual of op_hi, the FP32 operator of the same geometry, formed in FP64 and rounded to
+ * bfloat16 for the inner solve. d_b and d_x are FP32. residuals (host, outer + 1 entries or
+ * NULL) receives ||b - A x|| (FP64) before the first step and after each. */
+int ehp_cgls_refined(ehp_op* op_hi, ehp_op* op_lo, const void* d_b, void* d_x, int use_x0, int outer, int inner,
+                     int flags, double* residuals, void* stream);
 
 /* Largest eigenvalue estimate of A^T A by power iterations from the all-ones vector
  * (elsa: powerIterations(adjoint(A) * A, niters), PowerIterations.cpp:10-27). */

@@ -207,13 +207,18 @@ extern "C" int tile_adjoint(int dim, const int64_t* vol_dims, const int64_t* det
 // (pad_index), every offset checked against the view and written at most once; the padding
 // stays zero.
 template <int D, int A>
-static float fwd_one(const RayRec& r, const Dims& dims, const PadView& V, bool fb2)
+static float fwd_one(const RayRec& r, const Dims& dims, const PadView& V, bool fb2, bool b16)
 {
+    if (b16)
+        return fb2 ? plane_ray_forward<D, A, true, true>(r, dims, V) : plane_ray_forward<D, A, false, true>(r, dims, V);
     return fb2 ? plane_ray_forward<D, A, true>(r, dims, V) : plane_ray_forward<D, A, false>(r, dims, V);
 }
 
+// b16: bfloat16 views (the volume rounded to bfloat16), read by the bfloat16 variant of the
+// kernel's ray loop
 extern "C" int plane_forward(int dim, const int64_t* vol_dims, const int64_t* det_dims, int nposes, const float* P,
-                             const float* Pinv, const float* C, const int* fan, const float* vol, float* sino)
+                             const float* Pinv, const float* C, const int* fan, const float* vol, float* sino,
+                             int b16)
 {
     const Dims dims = make_dims(dim, vol_dims, det_dims, nposes);
     const int nm = dim * (dim + 1);
@@ -230,6 +235,7 @@ extern "C" int plane_forward(int dim, const int64_t* vol_dims, const int64_t* de
     }
     g_view_oob = false;
     std::vector<std::vector<float>> views(3);
+    std::vector<std::vector<bf16>> views16(3);
     PadView pv[3];
     bool fb2[3] = {false, false, false};
     for (int a = 0; a < dim; ++a) {
@@ -247,7 +253,14 @@ extern "C" int plane_forward(int dim, const int64_t* vol_dims, const int64_t* de
                     seen[o] = 1;
                     views[a][o] = vol[x + dims.strides[1] * y + dims.strides[2] * z];
                 }
-        pv[a] = PadView{views[a].data(), L.PS * 4u, L.RS * 4u, L.size};
+        if (b16) {
+            views16[a].resize(L.size);
+            for (int64_t i = 0; i < L.size; ++i)
+                views16[a][i] = float_to_bf16(views[a][i]);
+            pv[a] = PadView{views16[a].data(), L.PS * 2u, L.RS * 2u, L.size};
+        } else {
+            pv[a] = PadView{views[a].data(), L.PS * 4u, L.RS * 4u, L.size};
+        }
         fb2[a] = dim == 3 && fan[a] == (a == 2 ? 1 : 2);
     }
     for (int p = 0; p < nposes; ++p)
@@ -260,14 +273,15 @@ extern "C" int plane_forward(int dim, const int64_t* vol_dims, const int64_t* de
                     make_ray<2>(geo[p], u, v, o, d);
                     joseph_setup<2>(o, d, dims.vol, jr);
                     const RayRec r = make_ray_rec<2>(jr);
-                    val = jr.axis == 0 ? fwd_one<2, 0>(r, dims, pv[0], false) : fwd_one<2, 1>(r, dims, pv[1], false);
+                    val = jr.axis == 0 ? fwd_one<2, 0>(r, dims, pv[0], false, b16)
+                                       : fwd_one<2, 1>(r, dims, pv[1], false, b16);
                 } else {
                     make_ray<3>(geo[p], u, v, o, d);
                     joseph_setup<3>(o, d, dims.vol, jr);
                     const RayRec r = make_ray_rec<3>(jr);
-                    val = jr.axis == 0   ? fwd_one<3, 0>(r, dims, pv[0], fb2[0])
-                          : jr.axis == 1 ? fwd_one<3, 1>(r, dims, pv[1], fb2[1])
-                                         : fwd_one<3, 2>(r, dims, pv[2], fb2[2]);
+                    val = jr.axis == 0   ? fwd_one<3, 0>(r, dims, pv[0], fb2[0], b16)
+                          : jr.axis == 1 ? fwd_one<3, 1>(r, dims, pv[1], fb2[1], b16)
+                                         : fwd_one<3, 2>(r, dims, pv[2], fb2[2], b16);
                 }
                 sino[(int64_t) p * dims.pix_per_pose + (int64_t) v * dims.det[0] + u] = val;
             }

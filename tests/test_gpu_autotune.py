@@ -36,8 +36,35 @@ def test_tiles_give_bitwise_identical_results_and_autotune_keeps_them(cls, dim):
         op.tile(*t)
         np.testing.assert_array_equal(op.apply(x), ref_f)
         np.testing.assert_array_equal(op.applyAdjoint(y), ref_b)
+    if "Voxel" not in cls:
+        for b in (1, 5, 32):
+            assert op.pose_batch(b) == b
+            np.testing.assert_array_equal(op.applyAdjoint(y), ref_b)
+        with pytest.raises(ValueError):
+            op.pose_batch(33)
+        op.pose_batch(8)
     best, times = E.autotune(op, reps=1)
     assert best in times and len(times) >= 2
     assert tuple(op.tile()) == best
     np.testing.assert_array_equal(op.apply(x), ref_f)
     np.testing.assert_array_equal(op.applyAdjoint(y), ref_b)
+
+
+def test_autotune_batch_keeps_results():
+    """One tile and pose batch for all operators of a batch, tuned on the fused launch."""
+    import torch
+
+    import elsa_amd as E
+
+    geoms = [make(2)[0], oracle.circle_geometry(2, (40, 40), 17, 360, 4000.0, 200.0)]
+    ops = [E.JosephsMethodCUDA(*geom_to_descriptors(g)) for g in geoms]
+    rng = np.random.default_rng(2)
+    ys = [rng.random(g.sino_shape, dtype=np.float32) for g in geoms]
+    refs = [op.applyAdjoint(y) for op, y in zip(ops, ys)]
+    B, tile, pb, times = E.autotune_batch(ops, reps=1)
+    assert B.fused and tile in times and f"pose_batch={pb}" in times
+    assert all(tuple(op.tile()) == tile and op.pose_batch() == pb for op in ops)
+    got = B.unpack(B.applyAdjoint(B.pack(ys, adjoint_space=True)))
+    torch.cuda.synchronize()
+    for g_, r in zip(got, refs):
+        np.testing.assert_array_equal(g_.cpu().numpy(), r)

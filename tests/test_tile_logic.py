@@ -27,7 +27,7 @@ def lib():
     L.tile_adjoint.argtypes = [ctypes.c_int, i64, i64, ctypes.c_int, f32, f32, f32, ctypes.c_int, ctypes.c_int,
                                ctypes.c_int, f32, f32, f64]
     L.plane_forward.argtypes = [ctypes.c_int, i64, i64, ctypes.c_int, f32, f32, f32,
-                                np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS"), f32, f32]
+                                np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS"), f32, f32, ctypes.c_int]
     return L
 
 
@@ -96,7 +96,34 @@ def test_plane_forward_logic_matches_oracle_bitwise(lib, case):
         rc = lib.plane_forward(dim, np.array(list(vol) + [1] * (3 - dim), np.int64),
                                np.array(list(det) + [1] * (2 - len(det)), np.int64), views,
                                np.ascontiguousarray(g.P), np.ascontiguousarray(g.Pinv), np.ascontiguousarray(g.C),
-                               np.array(list(fan) + [0] * (3 - len(fan)), np.int32), x, out)
+                               np.array(list(fan) + [0] * (3 - len(fan)), np.int32), x, out, 0)
+        assert rc == 0, (fan, rc)
+        np.testing.assert_array_equal(out, ref, err_msg=f"fan axes {fan}")
+
+
+def _to_bf16(x):
+    """Round float32 to bfloat16 (nearest even) and back to float32."""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+    return u.astype(np.uint32).view(np.float32)
+
+
+@pytest.mark.parametrize("case", FWD_CASES, ids=lambda c: f"{c[0]}d-{'x'.join(map(str, c[1]))}")
+def test_plane_forward_bf16_views_match_oracle_bitwise(lib, case):
+    """bfloat16 views (half-width offsets, two-element loads) give the oracle's forward of the
+    bfloat16-rounded volume bit for bit, with every read inside its view."""
+    dim, vol, views, det, s2c, c2d, fans = case
+    rng = np.random.default_rng(5)
+    angles = np.sort(rng.uniform(0, 360, views)).astype(np.float32)
+    g = oracle.circle_geometry(dim, vol, views, 360, s2c, c2d, det=det, angles_deg=angles)
+    x = _to_bf16(rng.standard_normal(g.vol_shape).astype(np.float32))
+    ref = oracle.joseph(g, x, mode=oracle.MODE_DIRECT)
+    for fan in fans:
+        out = np.full(g.sino_shape, np.nan, np.float32)
+        rc = lib.plane_forward(dim, np.array(list(vol) + [1] * (3 - dim), np.int64),
+                               np.array(list(det) + [1] * (2 - len(det)), np.int64), views,
+                               np.ascontiguousarray(g.P), np.ascontiguousarray(g.Pinv), np.ascontiguousarray(g.C),
+                               np.array(list(fan) + [0] * (3 - len(fan)), np.int32), x, out, 1)
         assert rc == 0, (fan, rc)
         np.testing.assert_array_equal(out, ref, err_msg=f"fan axes {fan}")
 
