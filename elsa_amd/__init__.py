@@ -14,7 +14,7 @@ from .projectors import (BlobVoxelProjectorCUDA, BlobVoxelProjectorCUDAd, BlobVo
                          PhaseContrastBSplineVoxelProjectorCUDAd, PhaseContrastBSplineVoxelProjectorCUDAf,
                          SiddonsMethodCUDA, SiddonsMethodCUDAd, SiddonsMethodCUDAf, blob_derivative_lut, blob_lut,
                          bspline_derivative_lut, bspline_lut)
-from .solvers import APGD, CGLS, CGLSRefined, ProximalIdentity, ProximalL1, powerIterations
+from .solvers import ADMML2, APGD, CGLS, CGLSRefined, ProximalIdentity, ProximalL1, powerIterations
 from . import autograd, io
 from .autotune import autotune, autotune_batch
 from .batch import ProjectorBatch
@@ -27,6 +27,6 @@ __all__ = [
     "BSplineVoxelProjectorCUDAd", "PhaseContrastBlobVoxelProjectorCUDA", "PhaseContrastBlobVoxelProjectorCUDAf",
     "PhaseContrastBlobVoxelProjectorCUDAd", "PhaseContrastBSplineVoxelProjectorCUDA",
     "PhaseContrastBSplineVoxelProjectorCUDAf", "PhaseContrastBSplineVoxelProjectorCUDAd", "blob_lut", "bspline_lut",
-    "blob_derivative_lut", "bspline_derivative_lut", "CGLS", "CGLSRefined", "APGD", "ProximalL1", "ProximalIdentity",
+    "blob_derivative_lut", "bspline_derivative_lut", "CGLS", "CGLSRefined", "APGD", "ADMML2", "ProximalL1", "ProximalIdentity",
     "powerIterations", "autotune", "autotune_batch", "ProjectorBatch",
 ]

@@ -23,7 +23,7 @@ EXPORTS = (
     "ehp_circle_trajectory", "ehp_blob_lut", "ehp_bspline_lut", "ehp_cgls", "ehp_apgd", "ehp_power_iterations",
     "ehp_batch_create", "ehp_batch_destroy", "ehp_batch_layout", "ehp_batch_forward", "ehp_batch_adjoint",
     "ehp_batch_cgls", "ehp_batch_apgd", "ehp_batch_power_iterations", "ehp_blob_derivative_lut",
-    "ehp_bspline_derivative_lut", "ehp_pose_batch", "ehp_cgls_refined",
+    "ehp_bspline_derivative_lut", "ehp_pose_batch", "ehp_cgls_refined", "ehp_admml2", "ehp_batch_admml2",
 )
 EHP_SOLVE_GRAPH = 1
 EHP_DIFFERENTIAL = 2
@@ -105,6 +105,8 @@ def load():
     L.ehp_apgd.argtypes = [vp, vp, vp, c_int, c_int, c_d, c_d, c_d, c_int, dp, ip, dp, vp]
     L.ehp_power_iterations.argtypes = [vp, c_int, dp, vp]
     L.ehp_cgls_refined.argtypes = [vp, vp, vp, vp, c_int, c_int, c_int, c_int, dp, vp]
+    L.ehp_admml2.argtypes = [vp, vp, vp, c_int, c_int, c_d, c_d, c_int, c_int, dp, dp, vp]
+    L.ehp_batch_admml2.argtypes = [vp, vp, vp, c_int, c_int, c_d, c_d, c_int, c_int, dp, dp, vp]
     L.ehp_batch_create.argtypes = [ctypes.POINTER(vp), c_int, ctypes.POINTER(vp)]
     L.ehp_batch_destroy.argtypes = [vp]
     L.ehp_batch_destroy.restype = None

@@ -354,6 +354,24 @@ int ehp_batch_apgd(ehp_batch* b, const void* d_b, void* d_x, int use_x0, int ite
                            (hipStream_t) stream);
 }
 
+int ehp_batch_admml2(ehp_batch* b, const void* d_b, void* d_x, int use_x0, int iterations, double tau,
+                     double l1_sigma, int ninner, int flags, double* residuals, double* tau_used, void* stream)
+{
+    int rc = solver_args(b, d_b, d_x, iterations);
+    if (rc != EHP_OK)
+        return rc;
+    if (flags & ~EHP_SOLVE_GRAPH)
+        return bfail(EHP_ERR_ARGUMENT, "unknown solver flags 0x%x", flags);
+    if (ninner < 0)
+        return bfail(EHP_ERR_ARGUMENT, "negative inner iteration count %d", ninner);
+    const double t = tau > 0.0 ? tau : 0.9;
+    if (tau_used)
+        *tau_used = t;
+    const ehp::SolveOps A = solve_ops(b);
+    return ehp::solve_admml2(A, b->dtype, d_b, d_x, use_x0, iterations, t, l1_sigma, ninner,
+                             (flags & EHP_SOLVE_GRAPH) != 0, residuals, (hipStream_t) stream);
+}
+
 int ehp_batch_power_iterations(ehp_batch* b, int niters, double* L, void* stream)
 {
     if (!b || !L)

@@ -770,6 +770,25 @@ int ehp_apgd(ehp_op* op, const void* d_b, void* d_x, int use_x0, int iterations,
                            (hipStream_t) stream);
 }
 
+int ehp_admml2(ehp_op* op, const void* d_b, void* d_x, int use_x0, int iterations, double tau, double l1_sigma,
+               int ninner, int flags, double* residuals, double* tau_used, void* stream)
+{
+    int rc = solver_args(op, d_b, d_x, iterations);
+    if (rc != EHP_OK)
+        return rc;
+    if (flags & ~EHP_SOLVE_GRAPH)
+        return fail(EHP_ERR_ARGUMENT, "unknown solver flags 0x%x", flags);
+    if (ninner < 0)
+        return fail(EHP_ERR_ARGUMENT, "negative inner iteration count %d", ninner);
+    // elsa's default tau = 0.9 / powerIterations(A^T A), A the identity here (ADMML2.cpp:30-43)
+    const double t = tau > 0.0 ? tau : 0.9;
+    if (tau_used)
+        *tau_used = t;
+    SingleOps A(op);
+    return ehp::solve_admml2(A.ops, op->dtype, d_b, d_x, use_x0, iterations, t, l1_sigma, ninner,
+                             (flags & EHP_SOLVE_GRAPH) != 0, residuals, (hipStream_t) stream);
+}
+
 int ehp_cgls_refined(ehp_op* op_hi, ehp_op* op_lo, const void* d_b, void* d_x, int use_x0, int outer, int inner,
                      int flags, double* residuals, void* stream)
 {

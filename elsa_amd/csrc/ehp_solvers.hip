@@ -740,6 +740,72 @@ namespace ehp
             }
             return EHP_OK;
         }
+        // ---- ADMML2 with the identity as regularisation operator (ADMML2.cpp:46-97): the
+        // x-update is CGNE (CGNE.cpp) on K = [op; lambda I], K^T w = op^T b + lambda (z - u)
+
+        // rhs = atb + lambda (z - u), as BlockLinearOperator's adjoint sums the blocks
+        template <class T>
+        __global__ void __launch_bounds__(kThreads)
+            admm_rhs_kernel(const T* __restrict__ atb, const T* __restrict__ z, const T* __restrict__ u, T lambda,
+                            T* __restrict__ rhs, int64_t n)
+        {
+            const int64_t i = (int64_t) blockIdx.x * kThreads + threadIdx.x;
+            if (i < n) {
+                const T zu = z[i] - u[i];
+                rhs[i] = atb[i] + lambda * zu;
+            }
+        }
+
+        // K^T K v = op^T op v + lambda (lambda v): ktk holds op^T op v on entry
+        template <class T>
+        __global__ void __launch_bounds__(kThreads)
+            admm_normal_kernel(T* __restrict__ ktk, const T* __restrict__ v, T lambda, int64_t n)
+        {
+            const int64_t i = (int64_t) blockIdx.x * kThreads + threadIdx.x;
+            if (i < n) {
+                const T lv = lambda * v[i];
+                ktk[i] = ktk[i] + lambda * lv;
+            }
+        }
+
+        // CGNE start: r = rhs - K^T K x (elsa: r = A x; r *= -1; r += b), c = r
+        template <class T>
+        __global__ void __launch_bounds__(kThreads)
+            cgne_start_kernel(const T* __restrict__ rhs, const T* __restrict__ ktkx, T* __restrict__ r,
+                              T* __restrict__ c, int64_t n)
+        {
+            const int64_t i = (int64_t) blockIdx.x * kThreads + threadIdx.x;
+            if (i < n) {
+                const T ri = -ktkx[i] + rhs[i];
+                r[i] = ri;
+                c[i] = ri;
+            }
+        }
+
+        // z = prox(x + u) with threshold tau * sigma (ProximalL1.cpp:32-43; sigma < 0: identity),
+        // then u += x; u -= z
+        template <class T>
+        __global__ void __launch_bounds__(kThreads)
+            admm_zu_kernel(const T* __restrict__ x, T* __restrict__ z, T* __restrict__ u, T tau, T sigma, int64_t n)
+        {
+            const int64_t i = (int64_t) blockIdx.x * kThreads + threadIdx.x;
+            if (i >= n)
+                return;
+            const T v = x[i] + u[i];
+            T zi;
+            if (sigma < T(0)) {
+                zi = v;
+            } else {
+                T tmp = (v < T(0) ? -v : v) - tau * sigma;
+                tmp = T(0.5) * (tmp + (tmp < T(0) ? -tmp : tmp));
+                const T sg = v > T(0) ? T(1) : (v < T(0) ? T(-1) : T(0));
+                zi = sg * tmp;
+            }
+            z[i] = zi;
+            const T ux = u[i] + x[i];
+            u[i] = ux - zi;
+        }
+
         // r = b - q formed in FP64 and rounded to bfloat16, and partial sums of its square (the
         // residual norm of the refinement, FP64, before rounding)
         __global__ void __launch_bounds__(kThreads)
@@ -824,6 +890,123 @@ namespace ehp
                 return ehp_solver_fail(EHP_ERR_HIP, hipGetErrorString(e));
         }
         return EHP_OK;
+    }
+
+    namespace
+    {
+        template <class T>
+        int admml2(const SolveOps& A, const T* b, T* x, int use_x0, int iterations, double tau, double sigma,
+                   int ninner, bool graph, double* residuals, hipStream_t caller)
+        {
+            Solve sv;
+            int rc = sv.init(A, caller, "hipMalloc of the ADMML2 workspace failed");
+            if (rc != EHP_OK)
+                return rc;
+            hipStream_t s = sv.ss.s;
+            const int nseg = sv.nseg;
+            const int64_t nd = sv.nd, nr = sv.nr;
+            const SegSpace &D = sv.D, &R = sv.R;
+            Scratch& m = sv.m;
+            T *z, *u, *atb, *rhs, *r, *c, *ac, *q;
+            const int nh = iterations + 1;
+            if (m.alloc(&z, nd * sizeof(T)) != hipSuccess || m.alloc(&u, nd * sizeof(T)) != hipSuccess ||
+                m.alloc(&atb, nd * sizeof(T)) != hipSuccess || m.alloc(&rhs, nd * sizeof(T)) != hipSuccess ||
+                m.alloc(&r, nd * sizeof(T)) != hipSuccess || m.alloc(&c, nd * sizeof(T)) != hipSuccess ||
+                m.alloc(&ac, nd * sizeof(T)) != hipSuccess || m.alloc(&q, nr * sizeof(T)) != hipSuccess)
+                return ehp_solver_fail(EHP_ERR_ALLOC, "hipMalloc of the ADMML2 workspace failed");
+            // elsa: lambda = 1 / sqrt(tau) ("sqrttau", ADMML2.cpp:70), in the data type
+            const T ttau = (T) tau, lambda = T(1) / std::sqrt(ttau), tsig = sigma >= 0.0 ? (T) sigma : T(-1);
+            bool okf = true;
+            if (!use_x0)
+                okf = launch_fill<T>(x, nd, T(0), s) == hipSuccess;
+            okf = okf && launch_fill<T>(z, nd, T(0), s) == hipSuccess && launch_fill<T>(u, nd, T(0), s) == hipSuccess;
+            if (!okf)
+                return ehp_solver_fail(EHP_ERR_HIP, "fill");
+            rc = A.adj(b, atb, s); // op^T b, the same every iteration
+            if (rc != EHP_OK)
+                return rc;
+            // K^T K v = op^T op v + lambda^2 v into out
+            auto normal = [&](const T* v, T* out) -> int {
+                int r2 = A.fwd(v, q, s);
+                if (r2 == EHP_OK)
+                    r2 = A.adj(q, out, s);
+                if (r2 == EHP_OK)
+                    hipLaunchKernelGGL((admm_normal_kernel<T>), dim3(blocks(nd)), dim3(kThreads), 0, s, out, v, lambda,
+                                       nd);
+                return r2;
+            };
+            auto body = [&]() -> int {
+                // x = CGNE(K, [b; z - u], ninner, x) (RegularizedInversion.cpp, CGNE.cpp)
+                hipLaunchKernelGGL((admm_rhs_kernel<T>), dim3(blocks(nd)), dim3(kThreads), 0, s, atb, z, u, lambda, rhs,
+                                   nd);
+                int r2 = normal(x, ac);
+                if (r2 != EHP_OK)
+                    return r2;
+                hipLaunchKernelGGL((cgne_start_kernel<T>), dim3(blocks(nd)), dim3(kThreads), 0, s, rhs, ac, r, c, nd);
+                if (dot<T>(r, r, D, nseg, sv.partial, sv.sc, S_KOLD, nullptr, nh, s) != hipSuccess)
+                    return EHP_ERR_HIP;
+                for (int it = 0; it < ninner; ++it) {
+                    r2 = normal(c, ac);
+                    if (r2 != EHP_OK)
+                        return r2;
+                    if (dot<T>(c, ac, D, nseg, sv.partial, sv.sc, S_DELTA, nullptr, nh, s) != hipSuccess)
+                        return EHP_ERR_HIP;
+                    hipLaunchKernelGGL(cgls_alpha_kernel, dim3(seg_blocks(nseg)), dim3(64), 0, s, sv.sc, nseg, 0.0);
+                    hipLaunchKernelGGL((cgls_update_kernel<T>), dim3((unsigned) D.neb), dim3(kThreads), 0, s, x, c, D,
+                                       sv.sc, 0);
+                    hipLaunchKernelGGL((cgls_update_kernel<T>), dim3((unsigned) D.neb), dim3(kThreads), 0, s, r, ac, D,
+                                       sv.sc, 1);
+                    if (dot<T>(r, r, D, nseg, sv.partial, sv.sc, S_K, nullptr, nh, s) != hipSuccess)
+                        return EHP_ERR_HIP;
+                    hipLaunchKernelGGL(cgls_beta_kernel, dim3(seg_blocks(nseg)), dim3(64), 0, s, sv.sc, nseg);
+                    hipLaunchKernelGGL((cgls_direction_kernel<T>), dim3((unsigned) D.neb), dim3(kThreads), 0, s, r, c, D,
+                                       sv.sc);
+                }
+                // z = prox(x + u, tau); u += x; u -= z (ADMML2.cpp:77-85)
+                hipLaunchKernelGGL((admm_zu_kernel<T>), dim3(blocks(nd)), dim3(kThreads), 0, s, x, z, u, ttau, tsig, nd);
+                return hipGetLastError() == hipSuccess ? EHP_OK : EHP_ERR_HIP;
+            };
+            if (!residuals) {
+                hipError_t e = iterate(s, iterations, graph, body);
+                if (e == hipSuccess)
+                    e = hipStreamSynchronize(s);
+                return e == hipSuccess ? EHP_OK : ehp_solver_fail(EHP_ERR_HIP, hipGetErrorString(e));
+            }
+            // with a residual history the iterations run eagerly: ||b - op x|| (FP64) at the
+            // start and after each iteration
+            std::vector<double> hsc, hh;
+            for (int k = 0; k <= iterations; ++k) {
+                if (k > 0) {
+                    rc = body();
+                    if (rc != EHP_OK)
+                        return rc == EHP_ERR_HIP ? ehp_solver_fail(EHP_ERR_HIP, "ADMML2 iteration") : rc;
+                }
+                rc = A.fwd(x, q, s);
+                if (rc != EHP_OK)
+                    return rc;
+                hipLaunchKernelGGL((rsub_kernel<T>), dim3(blocks(nr)), dim3(kThreads), 0, s, b, q, nr);
+                if (dot<T>(q, q, R, nseg, sv.partial, sv.sc, S_SN, nullptr, nh, s) != hipSuccess)
+                    return ehp_solver_fail(EHP_ERR_HIP, "dot");
+                rc = fetch(s, sv.sc, nseg, nullptr, 0, hsc, hh);
+                if (rc != EHP_OK)
+                    return rc;
+                for (int j = 0; j < nseg; ++j)
+                    residuals[(size_t) j * nh + k] = std::sqrt(hsc[(size_t) j * S_COUNT + S_SN]);
+            }
+            return EHP_OK;
+        }
+    } // namespace
+
+    int solve_admml2(const SolveOps& A, int dtype, const void* b, void* x, int use_x0, int iterations, double tau,
+                     double sigma, int ninner, bool graph, double* residuals, hipStream_t s)
+    {
+        if (dtype == EHP_F64)
+            return admml2<double>(A, (const double*) b, (double*) x, use_x0, iterations, tau, sigma, ninner, graph,
+                                  residuals, s);
+        if (dtype == EHP_F32)
+            return admml2<float>(A, (const float*) b, (float*) x, use_x0, iterations, tau, sigma, ninner, graph,
+                                 residuals, s);
+        return ehp_solver_fail(EHP_ERR_UNSUPPORTED, "ADMML2 needs an FP32 or FP64 operator");
     }
 
     int solve_cgls(const SolveOps& A, int dtype, const void* b, void* x, int use_x0, int iterations, double eps,

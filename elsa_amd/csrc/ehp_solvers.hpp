@@ -24,6 +24,9 @@ namespace ehp
                    double sigma, double epsilon, bool graph, double* grad_norms, int* iters_done, double* mu_used,
                    hipStream_t s);
     int solve_power_iterations(const SolveOps& A, int dtype, int niters, double* L, hipStream_t s);
+    // ADMML2 (regularisation operator: the identity); tau > 0
+    int solve_admml2(const SolveOps& A, int dtype, const void* b, void* x, int use_x0, int iterations, double tau,
+                     double sigma, int ninner, bool graph, double* residuals, hipStream_t s);
     // CGLS with iterative refinement: FP32 operator Ahi, bfloat16 operator Alo, one problem
     int solve_cgls_refined(const SolveOps& Ahi, const SolveOps& Alo, const float* b, float* x, int use_x0, int outer,
                            int inner, bool graph, double* residuals, hipStream_t s);

@@ -148,6 +148,39 @@ class APGD:
         return (xt.float() if xt.dtype == _torch().bfloat16 else xt).cpu().numpy() if to_numpy else xt
 
 
+class ADMML2:
+    """ADMM with an L2 data term, 0.5 ||op x - b||^2 + g(x), for elsa's ADMML2 with the identity as
+    regularisation operator (ADMML2.h:62-64, ADMML2.cpp:46-97): ninneriters CGNE iterations per
+    x-update, prox ProximalL1(sigma) or ProximalIdentity(), tau None for elsa's default (0.9
+    with the identity). After solve(), `residuals` holds ||b - op x|| at the start and after each
+    iteration when record=True (the iterations then run without graph capture)."""
+
+    def __init__(self, op, b, prox=None, tau=None, ninneriters=5, graph=False, record=False):
+        self.A, self.b, self.graph, self.record = op, b, bool(graph), bool(record)
+        self.prox = prox if prox is not None else ProximalIdentity()
+        self.tau = float(tau) if tau is not None else 0.0
+        self.ninner = int(ninneriters)
+        self.residuals = None
+        self.tau_used = None
+
+    def solve(self, iterations, x0=None):
+        bt, xt, to_numpy, stream = _prepare(self.A, self.b, x0)
+        batch = _is_batch(self.A)
+        nseg = len(self.A) if batch else 1
+        res = np.empty((nseg, iterations + 1), np.float64) if self.record else None
+        tau = ctypes.c_double()
+        flags = _lib.EHP_SOLVE_GRAPH if self.graph else 0
+        fn = self.A._L.ehp_batch_admml2 if batch else self.A._L.ehp_admml2
+        dp = ctypes.POINTER(ctypes.c_double)
+        _lib.check(fn(self.A.handle, ctypes.c_void_p(bt.data_ptr()), ctypes.c_void_p(xt.data_ptr()),
+                      int(x0 is not None), int(iterations), self.tau, float(self.prox.sigma), self.ninner, flags,
+                      res.ctypes.data_as(dp) if res is not None else None, ctypes.byref(tau), stream), "ADMML2")
+        self.tau_used = tau.value
+        if res is not None:
+            self.residuals = res if batch else res[0]
+        return xt.cpu().numpy() if to_numpy else xt
+
+
 class CGLSRefined:
     """CGLS with iterative refinement in mixed precision (no counterpart in elsa): x (FP32)
     takes `outer` steps x += d, each d from `inner` CGLS iterations on A_lo, a bfloat16

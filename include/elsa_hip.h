@@ -171,6 +171,17 @@ int ehp_cgls(ehp_op* op, const void* d_b, void* d_x, int use_x0, int iterations,
 int ehp_apgd(ehp_op* op, const void* d_b, void* d_x, int use_x0, int iterations, double mu, double l1_sigma,
              double epsilon, int flags, double* grad_sq_norms, int* iterations_done, double* mu_used, void* stream);
 
+/* ADMM with an L2 data term (elsa: ADMML2(op, b, A, proxg, tau, ninneriters).solve(iterations,
+ * x0), ADMML2.cpp:20-97) for the regularisation operator A = identity: per iteration the x-update
+ * runs ninner CGNE iterations (CGNE.cpp, via RegularizedInversion.cpp) on [op; lambda I] with
+ * right-hand side [b; z - u] and lambda = 1 / sqrt(tau), starting from the current x; then
+ * z = prox(x + u) with threshold tau * l1_sigma (ProximalL1; l1_sigma < 0: identity) and
+ * u += x - z. tau <= 0 takes elsa's default 0.9 / powerIterations(A^T A) = 0.9; tau_used
+ * receives it. residuals (host, iterations + 1 or NULL) receives ||b - op x|| (FP64) at the
+ * start and after each iteration; asking for it runs the iterations without graph capture. */
+int ehp_admml2(ehp_op* op, const void* d_b, void* d_x, int use_x0, int iterations, double tau, double l1_sigma,
+               int ninner, int flags, double* residuals, double* tau_used, void* stream);
+
 /* CGLS with iterative refinement in mixed precision (no counterpart in elsa): x (FP32) takes
  * `outer` steps x += d. Each d approximately minimises ||A d - r|| by `inner` CGLS iterations
  * (the ehp_cgls iteration, eps = 0) on op_lo, a bfloat16 operator; r = b - A x is the
@@ -209,7 +220,7 @@ int ehp_batch_layout(const ehp_batch* b, int64_t* vol_offsets, int64_t* sino_off
 int ehp_batch_forward(ehp_batch* b, const void* d_vols, void* d_sinos, void* stream);
 int ehp_batch_adjoint(ehp_batch* b, const void* d_sinos, void* d_vols, void* stream);
 
-/* ehp_cgls / ehp_apgd / ehp_power_iterations on every problem of the batch at once. Per-problem
+/* ehp_cgls / ehp_apgd / ehp_power_iterations / ehp_admml2 on every problem of the batch at once. Per-problem
  * outputs are host arrays of n rows: residuals and grad_sq_norms n x (iterations + 1),
  * iterations_done, mu_used and L n entries. mu: n step lengths (entries <= 0, or mu NULL:
  * 0.45 / L_j from power iterations on that problem). */
@@ -219,6 +230,8 @@ int ehp_batch_apgd(ehp_batch* b, const void* d_b, void* d_x, int use_x0, int ite
                    double l1_sigma, double epsilon, int flags, double* grad_sq_norms, int* iterations_done,
                    double* mu_used, void* stream);
 int ehp_batch_power_iterations(ehp_batch* b, int niters, double* L, void* stream);
+int ehp_batch_admml2(ehp_batch* b, const void* d_b, void* d_x, int use_x0, int iterations, double tau,
+                     double l1_sigma, int ninner, int flags, double* residuals, double* tau_used, void* stream);
 
 /* ---- host-only helpers (no GPU) -------------------------------------------------------- */
 

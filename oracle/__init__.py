@@ -404,6 +404,46 @@ def power_iterations(A, At, shape, niters=5, dtype=np.float64):
     return _dot(u, At(A(u))) / np.sqrt(_dot(u, u))
 
 
+def admml2(A, At, b, shape, iterations, tau=0.9, sigma=None, ninner=5, x0=None, dtype=np.float64):
+    """ADMML2.cpp:46-97 with the identity as regularisation operator: x = CGNE(K = [A; lam I],
+    w = [b; z - u], ninner, x) (RegularizedInversion.cpp:11-61, CGNE.cpp:22-69) with lam =
+    1/sqrt(tau); z = prox(x + u) (soft thresholding of threshold tau * sigma, or the identity
+    when sigma is None); u += x; u -= z. Returns (x, ||b - A x|| at the start and after each
+    iteration)."""
+    t = dtype(tau)
+    lam = dtype(1) / np.sqrt(t)
+    b = np.asarray(b, dtype)
+    x = np.zeros(shape, dtype) if x0 is None else np.array(x0, dtype).reshape(shape)
+    z = np.zeros(shape, dtype)
+    u = np.zeros(shape, dtype)
+    atb = At(b)
+    ktk = lambda v: At(A(v)) + lam * (lam * v)
+    res = [np.sqrt(_dot(b - A(x), b - A(x)))]
+    for _ in range(iterations):
+        rhs = atb + lam * (z - u)
+        r = rhs - ktk(x)
+        c = r.copy()
+        kold = _dot(r, r)
+        for _ in range(ninner):
+            ac = ktk(c)
+            alpha = kold / _dot(c, ac)
+            x = x + dtype(alpha) * c
+            r = r - dtype(alpha) * ac
+            k = _dot(r, r)
+            c = r + dtype(k / kold) * c
+            kold = k
+        v = x + u
+        if sigma is None:
+            z = v
+        else:
+            tmp = np.abs(v) - t * dtype(sigma)
+            tmp = dtype(0.5) * (tmp + np.abs(tmp))
+            z = np.sign(v) * tmp
+        u = (u + x) - z
+        res.append(np.sqrt(_dot(b - A(x), b - A(x))))
+    return x, np.array(res)
+
+
 def apgd(A, At, b, shape, iterations, mu=None, sigma=None, epsilon=None, x0=None, dtype=np.float64):
     """APGD.cpp:33-102 (FISTA) with prox = soft thresholding of threshold mu * sigma
     (ProximalL1.cpp:32-43), or the identity when sigma is None. mu None: 0.45 / L with L from

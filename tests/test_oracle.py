@@ -399,6 +399,34 @@ def test_power_iterations_and_apgd_closed_forms():
     assert abs(mu2 - 0.45 / oracle.power_iterations(A2, At2, (15,), 5)) <= 1e-15
 
 
+def test_admml2_closed_forms():
+    """ADMML2 with the identity regulariser. elsa's x-update (RegularizedInversion.cpp) solves
+    min 0.5 ||Ax - b||^2 + 0.5 ||lam x - (z - u)||^2 with lam = 1/sqrt(tau), while z and u
+    track x itself (ADMML2.cpp:77-85). Its fixed points are therefore:
+      * no prox (u -> 0, z = x): (A^T A + lam (lam - 1) I) x = A^T b, a ridge solution that is
+        least squares exactly at tau = 1;
+      * L1 prox, tau = 1: the optimality conditions of 0.5 ||Ax - b||^2 + sigma ||x||_1
+        (A^T (b - A x) = sigma sign(x) where x != 0, |.| <= sigma where x = 0).
+    Zero inner CGNE steps keep x at its start."""
+    M, b, A, At = _dense(5)
+    for tau in (0.9, 1.0, 0.5):
+        lam = 1.0 / np.sqrt(tau)
+        ref = np.linalg.solve(M.T @ M + lam * (lam - 1.0) * np.eye(15), M.T @ b)
+        x, res = oracle.admml2(A, At, b, (15,), 400, tau=tau, ninner=5)
+        assert np.linalg.norm(x - ref) <= 1e-9 * np.linalg.norm(ref), tau
+    assert np.allclose(np.linalg.lstsq(M, b, rcond=None)[0], oracle.admml2(A, At, b, (15,), 400, tau=1.0)[0])
+    sigma = 2.0
+    x1, _ = oracle.admml2(A, At, b, (15,), 3000, tau=1.0, sigma=sigma, ninner=10)
+    g = M.T @ (b - M @ x1)
+    nz = np.abs(x1) > 1e-8
+    assert nz.any() and (~nz).any()
+    np.testing.assert_allclose(g[nz], sigma * np.sign(x1[nz]), atol=1e-6)
+    assert np.all(np.abs(g[~nz]) <= sigma + 1e-6)
+    x0 = np.ones(15)
+    x2, _ = oracle.admml2(A, At, b, (15,), 3, ninner=0, x0=x0)
+    np.testing.assert_array_equal(x2, x0)
+
+
 # --------------------------------------------------------------------------------------
 # phase contrast: derivative tables and the differential weight
 
