@@ -49,6 +49,13 @@ WORKLOADS = {
     "fan2d_256": dict(dim=2, vol=(256, 256), views=180, det=(384,), s2c=25600.0, c2d=1280.0,
                       text="2D fan-beam Joseph fwd+adj: 256^2 f32 Shepp-Logan, 180 poses over 360 deg, "
                            "384 detector, s2c 25600, c2d 1280"),
+    "mixed512": dict(dim=3, vol=(512, 512, 512), views=720, det=(768, 768), s2c=51200.0, c2d=2560.0, sub=64,
+                     text="mixed precision: Joseph fwd+adj at 512^3 f32 / bf16, 720 poses, 768x768 detector, s2c 51200, "
+                          "c2d 2560; CGLS FP32 vs bf16 with iterative refinement on a 64^3 subproblem"),
+    "autograd128": dict(dim=3, vol=(128, 128, 128), views=180, det=(192, 192), s2c=12800.0, c2d=640.0, batch=4,
+                        text="differentiable Joseph: 4 x 128^3 f32 volumes of 20 random ellipsoids, 180 poses, "
+                             "192x192 detector; one step = loss 0.5||Ax-b||^2 + 0.01 TV(x), backward (hand-written "
+                             "adjoint), gradient step"),
     "batch2d_64": dict(dim=2, n=64, sizes=(128, 192, 256, 384, 512), views=(90, 360), seed=2, cg=50, fista=50,
                        lam=1e-3,
                        text="64 independent 2D fan-beam Joseph problems: N in {128,192,256,384,512}, 90-360 poses "
@@ -184,7 +191,7 @@ def batch_main(a, E, wl):
 
     ops, xs, bs = batch_problems(E, wl)
     tuned = None
-    if a.tile == "auto":
+    if a.tile in ("", "auto"):  # the batched workload's tiles are chosen by the autotuner
         B, tile, pose_batch, times = E.autotune_batch(ops)
         tuned = {"tile": tile, "pose_batch": pose_batch}
         print(json.dumps({"autotune_batch": {str(k): round(v, 3) for k, v in times.items()}}), file=sys.stderr)
@@ -286,6 +293,160 @@ def batch_main(a, E, wl):
     print(json.dumps(out), flush=True)
 
 
+def _line(a, wl, value, ms, dtype, data, config, **extra):
+    out = {"metric": METRIC, "value": value, "unit": "Gsamples/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+           "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+           "data": data, "config": {"workload": wl["text"], **config}}
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+def mixed_main(a, E, wl):
+    """Joseph at 512^3 in FP32 and bfloat16 storage, then CGLS in FP32 against bfloat16 CGLS with
+    iterative refinement on a 64^3 subproblem, both measured against an FP64 GPU CGLS of 400
+    iterations (the reference iterate) at equal iteration budgets."""
+    import torch
+
+    from elsa_amd.phantoms import modified_shepp_logan
+
+    def timed(f, reps):
+        f()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            f()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    n = wl["vol"][0]
+    vd = E.VolumeDescriptor(list(wl["vol"]))
+    dd = E.CircleTrajectoryGenerator.createTrajectory(wl["views"], vd, 360, wl["s2c"], wl["c2d"],
+                                                      detectorSize=list(wl["det"]))
+    passes = {}
+    S = None
+    for name, dt in (("f32", np.float32), ("bf16", "bfloat16")):
+        op = E.JosephsMethodCUDA(vd, dd, dtype=dt)
+        S = op.sample_count()
+        tdt = torch.float32 if name == "f32" else torch.bfloat16
+        x = torch.from_numpy(modified_shepp_logan((n, n, n))).cuda().to(tdt)
+        sino = torch.empty(dd.shape, dtype=tdt, device="cuda")
+        back = torch.empty(vd.shape, dtype=tdt, device="cuda")
+        tf = timed(lambda: op.apply(x, sino), a.steps)
+        ta = timed(lambda: op.applyAdjoint(sino, back), a.steps)
+        passes[name] = {"fwd_ms": round(tf, 2), "adj_ms": round(ta, 2), "Gsamples_per_s": round(2 * S / (tf + ta) / 1e6, 1)}
+        del op, x, sino, back
+        torch.cuda.empty_cache()
+    m = wl["sub"]
+    svd = E.VolumeDescriptor([m, m, m])
+    sdd = E.CircleTrajectoryGenerator.createTrajectory(wl["views"] // 6, svd, 360, 100.0 * m, 5.0 * m,
+                                                       detectorSize=[m * 3 // 2, m * 3 // 2])
+    A64 = E.JosephsMethodCUDA(svd, sdd, dtype=np.float64)
+    A32 = E.JosephsMethodCUDA(svd, sdd)
+    A16 = E.JosephsMethodCUDA(svd, sdd, dtype="bfloat16")
+    xt = torch.from_numpy(modified_shepp_logan((m, m, m))).cuda()
+    b = A32.apply(xt)
+    x_ref = E.CGLS(A64, b.double(), tol=0.0, graph=True).solve(400)
+    err = lambda x: round(float(torch.linalg.norm(x.double() - x_ref) / torch.linalg.norm(x_ref)), 5)
+    cg = {}
+    for it in (40, 80, 160):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        x = E.CGLS(A32, b, tol=0.0, graph=True).solve(it)
+        torch.cuda.synchronize()
+        cg[f"fp32_cgls_{it}"] = {"rel_error": err(x), "wall_s": round(time.perf_counter() - t, 4)}
+    for inner, outer in ((20, 2), (20, 4), (40, 4)):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        r = E.CGLSRefined(A32, A16, b, inner=inner, graph=True)
+        x = r.solve(outer)
+        torch.cuda.synchronize()
+        cg[f"bf16_refined_{outer}x{inner}"] = {"rel_error": err(x), "wall_s": round(time.perf_counter() - t, 4)}
+    f32 = passes["f32"]
+    _line(a, wl, f32["Gsamples_per_s"], round(f32["fwd_ms"] + f32["adj_ms"], 2), "f32 and bf16 (storage)",
+          "synthetic (modified Shepp-Logan phantom)",
+          {"volume": list(wl["vol"]), "poses": wl["views"], "detector": list(wl["det"]), "samples_per_pass": S,
+           "passes": passes, "subproblem": {"volume": [m] * 3, "poses": wl["views"] // 6,
+                                            "reference": "FP64 CGLS, 400 iterations, on the GPU"}},
+          solve=cg)
+
+
+def autograd_main(a, E, wl):
+    """Training steps through the autograd wrapper (backward = the hand-written adjoint), plus
+    the dot-product test and FP64 central finite differences (h = 1e-3) of the same loss on 64
+    random directions."""
+    import torch
+
+    from elsa_amd.autograd import project
+    from elsa_amd.phantoms import random_ellipsoids
+
+    n, B = wl["vol"][0], wl["batch"]
+    vd = E.VolumeDescriptor(list(wl["vol"]))
+    dd = E.CircleTrajectoryGenerator.createTrajectory(wl["views"], vd, 360, wl["s2c"], wl["c2d"],
+                                                      detectorSize=list(wl["det"]))
+    op = E.JosephsMethodCUDA(vd, dd)
+    S = op.sample_count()
+    truth = torch.stack([torch.from_numpy(random_ellipsoids((n, n, n), 20, seed=1 + i)) for i in range(B)]).cuda()
+    with torch.no_grad():
+        b = project(op, truth)
+
+    def tv(v):
+        dz = v[:, 1:, :-1, :-1] - v[:, :-1, :-1, :-1]
+        dy = v[:, :-1, 1:, :-1] - v[:, :-1, :-1, :-1]
+        dx = v[:, :-1, :-1, 1:] - v[:, :-1, :-1, :-1]
+        return torch.sqrt(dx * dx + dy * dy + dz * dz + 1e-8).sum()
+
+    def loss_fn(v, A=op, bb=b):
+        r = project(A, v) - bb
+        return 0.5 * (r * r).sum() + 0.01 * tv(v)
+
+    x = torch.zeros_like(truth, requires_grad=True)
+    lr = 1e-5
+
+    def step():
+        loss = loss_fn(x)
+        loss.backward()
+        with torch.no_grad():
+            x.sub_(lr * x.grad)
+            x.grad = None
+        return loss
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        last = step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    # checks (untimed): dot-product test in FP32 and FP64 finite differences on one volume
+    g = torch.Generator(device="cuda").manual_seed(5)
+    u = torch.rand(vd.shape, device="cuda", generator=g)
+    w = torch.rand(dd.shape, device="cuda", generator=g)
+    Au, Atw = op.apply(u), op.applyAdjoint(w)
+    dot_rel = abs(float((Au.double() * w.double()).sum() - (u.double() * Atw.double()).sum())) / \
+        float(torch.linalg.norm(Au.double()) * torch.linalg.norm(w.double()))
+    op64 = E.JosephsMethodCUDA(vd, dd, dtype=np.float64)
+    b64 = b[0].double()
+    x64 = (truth[0].double() * 0.5).requires_grad_(True)
+    l64 = loss_fn(x64[None], op64, b64[None])
+    grad, = torch.autograd.grad(l64, x64)
+    h, worst = 1e-3, 0.0
+    with torch.no_grad():
+        for _ in range(64):
+            d = torch.randn(x64.shape, dtype=torch.float64, device="cuda", generator=g)
+            fd = (loss_fn((x64 + h * d)[None], op64, b64[None]) - loss_fn((x64 - h * d)[None], op64, b64[None])) / (2 * h)
+            an = (grad * d).sum()
+            worst = max(worst, abs(float(fd - an)) / max(abs(float(an)), 1e-300))
+    passes = 2 * B * a.steps
+    _line(a, wl, round(passes * S / wall / 1e9, 3), round(wall / a.steps * 1e3, 3), "f32",
+          "synthetic (random ellipsoids)",
+          {"batch": B, "volume": list(wl["vol"]), "poses": wl["views"], "detector": list(wl["det"]),
+           "samples_per_pass": S, "final_loss": float(last)},
+          checks={"dot_product_rel": dot_rel, "fd_fp64_h": h, "fd_directions": 64, "fd_max_rel_error": worst})
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -305,6 +466,12 @@ def main():
     torch.cuda.set_device(dev)
     if a.workload == "batch2d_64":
         batch_main(a, E, wl)
+        return
+    if a.workload == "mixed512":
+        mixed_main(a, E, wl)
+        return
+    if a.workload == "autograd128":
+        autograd_main(a, E, wl)
         return
     vd = E.VolumeDescriptor(list(wl["vol"]))
     dd = E.CircleTrajectoryGenerator.createTrajectory(wl["views"], vd, 360, wl["s2c"], wl["c2d"],

@@ -138,7 +138,7 @@ namespace
             b->nablocks[a] = (int64_t) ab[a].size();
         }
         if (e == hipSuccess) {
-            e = hipMalloc((void**) &b->d_maxbits, n * sizeof(unsigned int));
+            e = hipMalloc((void**) &b->d_maxbits, (sb.size() + 1) * sizeof(unsigned int)); // one per max-|y| block
             if (e == hipSuccess)
                 b->allocs.push_back(b->d_maxbits);
         }

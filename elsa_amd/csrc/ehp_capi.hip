@@ -215,6 +215,14 @@ namespace
         return tc;
     }
 
+    // Poses per step of the tiled back projections: in 2D few rays meet a tile per pose, so
+    // 32 poses keep the workgroup's lanes busy (0.105 -> 0.078 ms at 256^2 / 180 poses); in 3D
+    // 8 suffice.
+    int default_pose_batch(const ehp::Dims& dims)
+    {
+        return dims.dim == 2 ? ehp::kMaxPoseBatch : ehp::kPoseBatch;
+    }
+
     // Fastest axis of the plane-major view for driving axis a: the transverse axis along
     // which the rays of neighbouring pixels in a detector row spread most, summed over the
     // poses that have rays of axis a (the first column of Pinv is that spread per pixel).
@@ -249,7 +257,7 @@ namespace
             return fail(EHP_ERR_ALLOC, "hipMalloc of the %zu-byte ray table failed: %s", tbytes, hipGetErrorString(e));
         e = hipMalloc((void**) &op->d_pose_axes, n * sizeof(int));
         if (e == hipSuccess)
-            e = hipMalloc((void**) &op->d_maxbits, sizeof(unsigned int));
+            e = hipMalloc((void**) &op->d_maxbits, ehp::kAbsmaxBlocks * sizeof(unsigned int));
         if (e == hipSuccess)
             e = hipMalloc((void**) &op->d_fixexp, 2 * sizeof(int));
         if (e != hipSuccess)
@@ -316,6 +324,7 @@ namespace
         if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, op->device) != hipSuccess)
             max_lds = 65536;
         op->tile = default_tile(KIND_JOSEPH, dm, max_lds);
+        op->batch = default_pose_batch(dm);
         op->tiled = true;
         return EHP_OK;
     }
@@ -338,7 +347,7 @@ namespace
             return fail(EHP_ERR_ALLOC, "hipMalloc of the %zu-byte ray table failed: %s", tbytes, hipGetErrorString(e));
         e = hipMalloc((void**) &op->d_pose_axes, n * sizeof(int));
         if (e == hipSuccess)
-            e = hipMalloc((void**) &op->d_maxbits, sizeof(unsigned int));
+            e = hipMalloc((void**) &op->d_maxbits, ehp::kAbsmaxBlocks * sizeof(unsigned int));
         if (e == hipSuccess)
             e = hipMalloc((void**) &op->d_fixexp, 2 * sizeof(int));
         if (e != hipSuccess)
@@ -357,6 +366,7 @@ namespace
         if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, op->device) != hipSuccess)
             max_lds = 65536;
         op->tile = default_tile(KIND_SIDDON, dm, max_lds);
+        op->batch = default_pose_batch(dm);
         op->tiled = true;
         return EHP_OK;
     }
@@ -403,7 +413,7 @@ namespace
         if (e == hipSuccess)
             e = hipMalloc((void**) &op->d_pose_margin, n * sizeof(float));
         if (e == hipSuccess)
-            e = hipMalloc((void**) &op->d_maxbits, sizeof(unsigned int));
+            e = hipMalloc((void**) &op->d_maxbits, ehp::kAbsmaxBlocks * sizeof(unsigned int));
         if (e == hipSuccess)
             e = hipMalloc((void**) &op->d_fixexp, 2 * sizeof(int));
         if (e != hipSuccess)

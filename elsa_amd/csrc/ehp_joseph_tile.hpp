@@ -316,6 +316,8 @@ namespace ehp
     }
 
     constexpr int kMaxPoseBatch = 32;
+    // partial maxima of the fixed-point scale (one per block of the max-|y| launch)
+    constexpr int kAbsmaxBlocks = 2048;
     constexpr int kPoseBatch = 8;
 
     // Upper bound of sum |len * weight| over the contributions one LDS slot can receive from

@@ -63,41 +63,61 @@ namespace ehp
     // ---------------------------------------------------------------------------------
     // fixed-point scale of one back projection
 
-    template <class T>
-    __global__ void __launch_bounds__(256)
-        absmax_kernel(const T* __restrict__ x, int64_t n, unsigned int* __restrict__ out)
+    // Largest |y| in two stages without atomics: each block writes the maximum of its share
+    // as the bit pattern of |y| (for non-negative floats, and NaN above infinity, the integer
+    // order of the bits is the order of the values), one block reduces those.
+    __device__ inline unsigned int block_max_bits(unsigned int m)
     {
-        float m = 0.0f;
-        const int64_t stride = (int64_t) gridDim.x * blockDim.x;
-        for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-            const float a = fabsf(up(x[i]));
-            m = (a > m || a != a) ? a : m; // NaN wins
-        }
         for (int off = 32; off > 0; off >>= 1) {
-            const float o = __shfl_down(m, off, 64);
-            m = (o > m || o != o) ? o : m;
+            const unsigned int o = __shfl_down(m, off, 64);
+            m = o > m ? o : m;
         }
+        __shared__ unsigned int red[4];
         if ((threadIdx.x & 63) == 0)
-            atomicMax(out, __float_as_uint(m)); // |x| >= 0: the bit patterns order like the values (NaN last)
+            red[threadIdx.x >> 6] = m;
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int w = 1; w < (int) (blockDim.x >> 6); ++w)
+                m = red[w] > m ? red[w] : m;
+        return m; // valid in thread 0
     }
 
-    // the fixed-point exponent eg for the slot bound of the operator
-    __global__ void scale_kernel(const unsigned int* __restrict__ maxbits, double bound, int* __restrict__ fixexp)
+    template <class T>
+    __global__ void __launch_bounds__(256)
+        absmax_kernel(const T* __restrict__ x, int64_t n, unsigned int* __restrict__ partial)
     {
-        fixexp[0] = fixed_point_exponent(__uint_as_float(*maxbits), bound);
+        unsigned int m = 0u;
+        const int64_t stride = (int64_t) gridDim.x * blockDim.x;
+        for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+            const unsigned int u = __float_as_uint(fabsf(up(x[i])));
+            m = u > m ? u : m;
+        }
+        m = block_max_bits(m);
+        if (threadIdx.x == 0)
+            partial[blockIdx.x] = m;
+    }
+
+    // the fixed-point exponent eg for the slot bound of the operator, from the partial maxima
+    __global__ void __launch_bounds__(256)
+        scale_kernel(const unsigned int* __restrict__ partial, int nblocks, double bound, int* __restrict__ fixexp)
+    {
+        unsigned int m = 0u;
+        for (int i = threadIdx.x; i < nblocks; i += blockDim.x)
+            m = partial[i] > m ? partial[i] : m;
+        m = block_max_bits(m);
+        if (threadIdx.x == 0)
+            fixexp[0] = fixed_point_exponent(__uint_as_float(m), bound);
     }
 
     template <class T>
     hipError_t fixed_point_exponent_launch(const T* y, int64_t n, double slot_bound, unsigned int* d_maxbits,
                                            int* d_fixexp, hipStream_t s)
     {
-        hipError_t e = launch_fill<unsigned int>(d_maxbits, 1, 0u, s);
-        if (e != hipSuccess)
-            return e;
-        int64_t blocks = (n + 255) / 256;
-        blocks = blocks > 4096 ? 4096 : (blocks < 1 ? 1 : blocks);
+        // 16 elements per thread at least
+        int64_t blocks = (n + 256 * 16 - 1) / (256 * 16);
+        blocks = blocks > kAbsmaxBlocks ? kAbsmaxBlocks : (blocks < 1 ? 1 : blocks);
         hipLaunchKernelGGL((absmax_kernel<T>), dim3((unsigned) blocks), dim3(256), 0, s, y, n, d_maxbits);
-        hipLaunchKernelGGL(scale_kernel, dim3(1), dim3(1), 0, s, d_maxbits, slot_bound, d_fixexp);
+        hipLaunchKernelGGL(scale_kernel, dim3(1), dim3(256), 0, s, d_maxbits, (int) blocks, slot_bound, d_fixexp);
         return hipGetLastError();
     }
 
@@ -236,34 +256,39 @@ namespace ehp
                                      J.batch, fixexp + J.item, sinos + J.sino_off, vols + J.vol_off);
     }
 
-    // max|y| per operator of a batch (block b scans its share of segment seg_of_block[b])
+    // max|y| per operator of a batch: block b writes the maximum of its share of segment
+    // block_seg[b] to partial[b]; then one block per segment reduces its partials
     __global__ void __launch_bounds__(256)
         absmax_batch_kernel(const float* __restrict__ y, const SegJob* __restrict__ segs,
-                            const int* __restrict__ block_seg, unsigned int* __restrict__ out)
+                            const int* __restrict__ block_seg, unsigned int* __restrict__ partial)
     {
-        const int j = block_seg[blockIdx.x];
-        const SegJob& S = segs[j];
+        const SegJob& S = segs[block_seg[blockIdx.x]];
         const int64_t b = (int64_t) blockIdx.x - S.block0;
-        float m = 0.0f;
+        unsigned int m = 0u;
         const int64_t stride = (int64_t) S.nblocks * 256;
         for (int64_t i = b * 256 + threadIdx.x; i < S.n; i += stride) {
-            const float a = fabsf(y[S.off + i]);
-            m = (a > m || a != a) ? a : m;
+            const unsigned int u = __float_as_uint(fabsf(y[S.off + i]));
+            m = u > m ? u : m;
         }
-        for (int off = 32; off > 0; off >>= 1) {
-            const float o = __shfl_down(m, off, 64);
-            m = (o > m || o != o) ? o : m;
-        }
-        if ((threadIdx.x & 63) == 0)
-            atomicMax(out + j, __float_as_uint(m));
+        m = block_max_bits(m);
+        if (threadIdx.x == 0)
+            partial[blockIdx.x] = m;
     }
 
-    __global__ void scale_batch_kernel(const unsigned int* __restrict__ maxbits, const double* __restrict__ bounds,
-                                       int n, int* __restrict__ fixexp)
+    __global__ void __launch_bounds__(256)
+        scale_batch_kernel(const unsigned int* __restrict__ partial, const SegJob* __restrict__ segs,
+                           const double* __restrict__ bounds, int* __restrict__ fixexp)
     {
-        const int j = blockIdx.x * blockDim.x + threadIdx.x;
-        if (j < n)
-            fixexp[j] = fixed_point_exponent(__uint_as_float(maxbits[j]), bounds[j]);
+        const int j = blockIdx.x;
+        const SegJob& S = segs[j];
+        unsigned int m = 0u;
+        for (int i = threadIdx.x; i < S.nblocks; i += blockDim.x) {
+            const unsigned int u = partial[S.block0 + i];
+            m = u > m ? u : m;
+        }
+        m = block_max_bits(m);
+        if (threadIdx.x == 0)
+            fixexp[j] = fixed_point_exponent(__uint_as_float(m), bounds[j]);
     }
 
     // Which driving axes occur among the rays of each pose (bit a set when some ray that hits
@@ -368,7 +393,7 @@ namespace ehp
             SegJob sg{};
             sg.off = it.sino_off;
             sg.n = it.dims.nrays;
-            int64_t nb = (sg.n + 255) / 256;
+            int64_t nb = (sg.n + 256 * 16 - 1) / (256 * 16);
             nb = nb > 256 ? 256 : (nb < 1 ? 1 : nb);
             sg.nblocks = (int) nb;
             sg.block0 = (int64_t) sblocks.size();
@@ -407,14 +432,12 @@ namespace ehp
         if (nsblocks > 0x7fffffff)
             return hipErrorInvalidValue;
         hipError_t e = launch_fill<float>(vols, nvol_total, 0.0f, s);
-        if (e == hipSuccess)
-            e = launch_fill<unsigned int>(d_maxbits, nitems, 0u, s);
         if (e != hipSuccess)
             return e;
         hipLaunchKernelGGL(absmax_batch_kernel, dim3((unsigned) nsblocks), dim3(256), 0, s, sinos, segs, sblocks,
                            d_maxbits);
-        hipLaunchKernelGGL(scale_batch_kernel, dim3((unsigned) ((nitems + 63) / 64)), dim3(64), 0, s, d_maxbits,
-                           bounds, nitems, d_fixexp);
+        hipLaunchKernelGGL(scale_batch_kernel, dim3((unsigned) nitems), dim3(256), 0, s, d_maxbits, segs, bounds,
+                           d_fixexp);
         for (int a = 0; a < dim; ++a) {
             const int64_t total = nablocks[a];
             if (total == 0)
