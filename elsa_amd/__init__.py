@@ -4,8 +4,8 @@ The projectors (Joseph, Siddon, blob / B-spline voxel) are HIP kernels for gfx95
 libelsahip.so, reached through the C-ABI of include/elsa_hip.h. This package is the Python
 face of that library, with pyelsa's class names.
 """
-from .descriptors import (CircleTrajectoryGenerator, DataDescriptor, DetectorDescriptor, Geometry,
-                          PlanarDetectorDescriptor, VolumeDescriptor)
+from .descriptors import (CircleTrajectoryGenerator, CurvedCircleTrajectoryGenerator, CurvedDetectorDescriptor,
+                          DataDescriptor, DetectorDescriptor, Geometry, PlanarDetectorDescriptor, VolumeDescriptor)
 from .projectors import (BlobVoxelProjectorCUDA, BlobVoxelProjectorCUDAd, BlobVoxelProjectorCUDAf,
                          BSplineVoxelProjectorCUDA, BSplineVoxelProjectorCUDAd, BSplineVoxelProjectorCUDAf,
                          JosephsMethodCUDA, JosephsMethodCUDAd, JosephsMethodCUDAf, LinearOperator,
@@ -18,9 +18,10 @@ from .solvers import ADMML2, APGD, CGLS, CGLSRefined, ProximalIdentity, Proximal
 from . import autograd, io
 from .autotune import autotune, autotune_batch
 from .batch import ProjectorBatch
+from .subsets import SubsetSampler
 
 __all__ = [
-    "CircleTrajectoryGenerator", "DataDescriptor", "DetectorDescriptor", "Geometry", "PlanarDetectorDescriptor",
+    "CircleTrajectoryGenerator", "CurvedCircleTrajectoryGenerator", "CurvedDetectorDescriptor", "DataDescriptor", "DetectorDescriptor", "Geometry", "PlanarDetectorDescriptor",
     "VolumeDescriptor", "LinearOperator", "JosephsMethodCUDA", "JosephsMethodCUDAf", "JosephsMethodCUDAd",
     "SiddonsMethodCUDA", "SiddonsMethodCUDAf", "SiddonsMethodCUDAd", "BlobVoxelProjectorCUDA",
     "BlobVoxelProjectorCUDAf", "BlobVoxelProjectorCUDAd", "BSplineVoxelProjectorCUDA", "BSplineVoxelProjectorCUDAf",
@@ -28,5 +29,5 @@ __all__ = [
     "PhaseContrastBlobVoxelProjectorCUDAd", "PhaseContrastBSplineVoxelProjectorCUDA",
     "PhaseContrastBSplineVoxelProjectorCUDAf", "PhaseContrastBSplineVoxelProjectorCUDAd", "blob_lut", "bspline_lut",
     "blob_derivative_lut", "bspline_derivative_lut", "CGLS", "CGLSRefined", "APGD", "ADMML2", "ProximalL1", "ProximalIdentity",
-    "powerIterations", "autotune", "autotune_batch", "ProjectorBatch",
+    "powerIterations", "autotune", "autotune_batch", "ProjectorBatch", "SubsetSampler",
 ]

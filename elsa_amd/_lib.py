@@ -24,6 +24,7 @@ EXPORTS = (
     "ehp_batch_create", "ehp_batch_destroy", "ehp_batch_layout", "ehp_batch_forward", "ehp_batch_adjoint",
     "ehp_batch_cgls", "ehp_batch_apgd", "ehp_batch_power_iterations", "ehp_blob_derivative_lut",
     "ehp_bspline_derivative_lut", "ehp_pose_batch", "ehp_cgls_refined", "ehp_admml2", "ehp_batch_admml2",
+    "ehp_geometry_set_pixel_coords", "ehp_curved_detector_coords",
 )
 EHP_SOLVE_GRAPH = 1
 EHP_DIFFERENTIAL = 2
@@ -80,6 +81,8 @@ def load():
     L.ehp_version.argtypes = []
     L.ehp_geometry_create.argtypes = [c_int, i64, i64p, i64p, fp, fp, fp, fp, fp, ctypes.POINTER(vp)]
     L.ehp_geometry_destroy.argtypes = [vp]
+    L.ehp_geometry_set_pixel_coords.argtypes = [vp, fp]
+    L.ehp_curved_detector_coords.argtypes = [c_int, i64p, c_f, c_f, fp]
     L.ehp_geometry_destroy.restype = None
     L.ehp_joseph_create.argtypes = [vp, c_int, c_int, ctypes.POINTER(vp)]
     L.ehp_siddon_create.argtypes = [vp, c_int, c_int, ctypes.POINTER(vp)]

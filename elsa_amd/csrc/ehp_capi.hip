@@ -93,6 +93,19 @@ namespace
             return hip_fail(e, "geometry upload");
         }
         op->host_geo = g->host;
+        if (!g->pix.empty()) {
+            const size_t pbytes = g->pix.size() * sizeof(float);
+            e = hipMalloc((void**) &op->d_pix, pbytes);
+            if (e == hipSuccess)
+                e = hipMemcpy(op->d_pix, g->pix.data(), pbytes, hipMemcpyHostToDevice);
+            if (e != hipSuccess) {
+                (void) hipFree(op->d_geo);
+                (void) hipFree(op->d_pix);
+                delete op;
+                return fail(EHP_ERR_ALLOC, "detector coordinates: %s", hipGetErrorString(e));
+            }
+            op->dims.pix = op->d_pix;
+        }
         *out = op;
         return EHP_OK;
     }
@@ -191,6 +204,22 @@ int ehp_geometry_create(int dim, int64_t nposes, const int64_t* vol_dims, const 
 }
 
 void ehp_geometry_destroy(ehp_geometry* g) { delete g; }
+
+int ehp_geometry_set_pixel_coords(ehp_geometry* g, const float* coords)
+{
+    if (!g)
+        return fail(EHP_ERR_ARGUMENT, "geometry is NULL");
+    if (!coords) {
+        g->pix.clear();
+        return EHP_OK;
+    }
+    const size_t n = (size_t) g->det[0] * (g->dim == 3 ? g->det[1] : 1) * (size_t) (g->dim - 1);
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(coords[i]))
+            return fail(EHP_ERR_ARGUMENT, "detector coordinate %zu is not finite", i);
+    g->pix.assign(coords, coords + n);
+    return EHP_OK;
+}
 
 namespace
 {
@@ -311,10 +340,11 @@ namespace
         if (e != hipSuccess)
             return hip_fail(e, "volume view");
 
-        // back projection: the bound keeps the 64-bit fixed-point sums in range; without it
+        // back projection: the bound keeps the 64-bit fixed-point sums in range; without it,
+        // and for curved detectors (the tiles find their rays through planar pixel centres),
         // the one-thread-per-ray kernel runs
         const double bound = ehp::fixed_point_slot_bound(dm, op->host_geo.data());
-        if (!(bound > 0.0) || bound > 1.0e18) {
+        if (!(bound > 0.0) || bound > 1.0e18 || op->d_pix) {
             (void) hipFree(op->d_table);
             op->d_table = nullptr;
             return EHP_OK;
@@ -335,8 +365,8 @@ namespace
     {
         const ehp::Dims& dm = op->dims;
         const double bound = ehp::fixed_point_slot_bound(dm, op->host_geo.data());
-        if (!(bound > 0.0) || bound > 1.0e18)
-            return EHP_OK; // keep the one-thread-per-ray kernel
+        if (!(bound > 0.0) || bound > 1.0e18 || op->d_pix)
+            return EHP_OK; // keep the one-thread-per-ray kernel (also for curved detectors)
         for (int i = 0; i < dm.dim; ++i)
             if (dm.vol[i] > 32767)
                 return EHP_OK; // the table stores voxel indices as 16-bit values
@@ -476,6 +506,8 @@ int ehp_voxel_create(const ehp_geometry* g, int dtype, const void* lut, int n_lu
         return fail(EHP_ERR_ARGUMENT, "unknown voxel projector flags 0x%x", flags);
     if (dtype == EHP_BF16)
         return fail(EHP_ERR_UNSUPPORTED, "bfloat16 storage is implemented for Joseph only");
+    if (g && !g->pix.empty())
+        return fail(EHP_ERR_UNSUPPORTED, "the voxel projectors take planar detectors only (as elsa's)");
     if (!lut || n_lut < 2)
         return fail(EHP_ERR_ARGUMENT, "the basis-function table needs at least 2 entries");
     if (!(support > 0.0))
@@ -520,7 +552,8 @@ void ehp_destroy(ehp_op* op)
         (void) hipSetDevice(op->device);
     for (void* p : {(void*) op->d_geo, op->d_lut, (void*) op->d_pose_axes, (void*) op->d_table,
                     (void*) op->d_sidtable, (void*) op->d_pose_margin, (void*) op->d_maxbits,
-                    (void*) op->d_fixexp, op->d_view[0], op->d_view[1], op->d_view[2], (void*) op->d_acc32})
+                    (void*) op->d_fixexp, op->d_view[0], op->d_view[1], op->d_view[2], (void*) op->d_acc32,
+                    (void*) op->d_pix})
         if (p)
             (void) hipFree(p);
     if (cur >= 0 && cur != op->device)

@@ -109,6 +109,9 @@ namespace ehp
         int64_t nvox;
         int64_t pix_per_pose;
         int64_t nrays;
+        // planar detector coordinates of the pixel centres (dim - 1 per pixel, u fastest) of a
+        // curved detector; nullptr for a planar detector (centre of pixel u at u + 0.5)
+        const float* pix;
     };
 
     EHP_HD inline float minNum(float x, float y)
@@ -127,12 +130,12 @@ namespace ehp
     // Ray through the centre of pixel (u, v): origin C, direction normalize(Pinv h).
     // elsa/core/Descriptors/DetectorDescriptor.cpp:90-128.
     template <int D>
-    EHP_HD inline void make_ray(const PoseGeom& g, int u, int v, float* o, float* d)
+    EHP_HD inline void make_ray_at(const PoseGeom& g, float hu, float hv, float* o, float* d)
     {
         float h[3];
-        h[0] = (float) u + 0.5f;
+        h[0] = hu;
         if (D == 3) {
-            h[1] = (float) v + 0.5f;
+            h[1] = hv;
             h[2] = 1.0f;
         } else {
             h[1] = 1.0f;
@@ -155,6 +158,25 @@ namespace ehp
         for (int i = 0; i < D; ++i) {
             d[i] = n2 > 0.0f ? y[i] / n : y[i];
             o[i] = g.C[i];
+        }
+    }
+
+    template <int D>
+    EHP_HD inline void make_ray(const PoseGeom& g, int u, int v, float* o, float* d)
+    {
+        make_ray_at<D>(g, (float) u + 0.5f, (float) v + 0.5f, o, d);
+    }
+
+    // The ray of pixel (u, v) of a detector that may be curved: through its planar coordinate
+    // (elsa/core/Descriptors/CurvedDetectorDescriptor.cpp:97-119) when dims.pix is set.
+    template <int D>
+    EHP_HD inline void make_ray_px(const PoseGeom& g, const Dims& dims, int u, int v, float* o, float* d)
+    {
+        if (dims.pix) {
+            const float* c = dims.pix + ((int64_t) v * dims.det[0] + u) * (D - 1);
+            make_ray_at<D>(g, c[0], D == 3 ? c[D == 3 ? 1 : 0] : 0.0f, o, d);
+        } else {
+            make_ray<D>(g, u, v, o, d);
         }
     }
 

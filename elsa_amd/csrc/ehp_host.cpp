@@ -405,6 +405,35 @@ int ehp_bspline_lut(int dtype, int64_t dim, int64_t order, int n, void* out, dou
     return EHP_OK;
 }
 
+int ehp_curved_detector_coords(int dim, const int64_t* det_dims, float angle, float s2d, float* out)
+{
+    // CurvedDetectorDescriptor::setup / mapCurvedCoordToPlanarCoord
+    // (elsa/core/Descriptors/CurvedDetectorDescriptor.cpp:36-95), in FP32 as elsa's real_t
+    if ((dim != 2 && dim != 3) || !det_dims || !out || !(angle > 0.0f) || !(s2d > 0.0f))
+        return EHP_ERR_ARGUMENT;
+    const int64_t len = det_dims[0], height = dim == 3 ? det_dims[1] : 1;
+    if (len <= 0 || height <= 0)
+        return EHP_ERR_ARGUMENT;
+    const float radius = (float) len / angle;
+    const float delta = angle / (float) len;
+    const float cx = (float) len / 2.0f, cy = (float) height / 2.0f;
+    const float ccx = cx, ccy = s2d - radius; // centre of the circle
+    int64_t k = 0;
+    for (int64_t h = 0; h < height; ++h)
+        for (int64_t l = 0; l < len; ++l) {
+            const float pu = (float) l + 0.5f, pv = (float) h + 0.5f;
+            const float ray_angle = (pu - cx) * delta;
+            const float dx = std::sin(ray_angle) * radius;
+            const float depth = std::cos(ray_angle) * radius;
+            const float x = ccx + dx, z = ccy + depth;
+            const float factor = z == 0.0f ? 1.0f : s2d / z;
+            out[k++] = (x - cx) * factor + cx;
+            if (dim == 3)
+                out[k++] = (pv - cy) * factor + cy;
+        }
+    return EHP_OK;
+}
+
 int ehp_blob_derivative_lut(int dtype, double radius, double alpha, int64_t order, int n, int normalized, void* out,
                             double* support)
 {

@@ -125,7 +125,7 @@ namespace ehp
             if (u >= dims.det[0] || v >= dims.det[1])
                 return;
             float o[3], d[3];
-            make_ray<D>(geo[pose], u, v, o, d);
+            make_ray_px<D>(geo[pose], dims, u, v, o, d);
             JRay jr;
             joseph_setup<D>(o, d, dims.vol, jr);
             const RayRec r = make_ray_rec<D>(jr);

@@ -44,7 +44,7 @@ namespace ehp
         const int u = (int) (pix % dims.det[0]);
         const int v = (int) (pix / dims.det[0]);
         float o[3], d[3];
-        make_ray<D>(geo[pose], u, v, o, d);
+        make_ray_px<D>(geo[pose], dims, u, v, o, d);
         JRay jr;
         joseph_setup<D>(o, d, dims.vol, jr);
         table[(int64_t) pose * dims.pix_per_pose + pix] = make_ray_rec<D>(jr);

@@ -95,7 +95,7 @@ namespace ehp
         const int u = (int) (pix % dims.det[0]);
         const int v = (int) (pix / dims.det[0]);
         float o[3], d[3];
-        make_ray<D>(geo[pose], u, v, o, d);
+        make_ray_px<D>(geo[pose], dims, u, v, o, d);
         JRay jr;
         joseph_setup<D>(o, d, dims.vol, jr);
         T acc;
@@ -123,7 +123,7 @@ namespace ehp
         const int u = (int) (pix % dims.det[0]);
         const int v = (int) (pix / dims.det[0]);
         float o[3], d[3];
-        make_ray<D>(geo[pose], u, v, o, d);
+        make_ray_px<D>(geo[pose], dims, u, v, o, d);
         JRay jr;
         joseph_setup<D>(o, d, dims.vol, jr);
         const T yl = yv * (T) jr.len;
@@ -148,7 +148,7 @@ namespace ehp
             const int u = (int) (pix % dims.det[0]);
             const int v = (int) (pix / dims.det[0]);
             float o[3], d[3];
-            make_ray<D>(geo[pose], u, v, o, d);
+            make_ray_px<D>(geo[pose], dims, u, v, o, d);
             JRay jr;
             joseph_setup<D>(o, d, dims.vol, jr);
             n = jr.nsteps;
@@ -209,7 +209,7 @@ namespace ehp
             const int u = (int) (pix % dims.det[0]);
             const int v = (int) (pix / dims.det[0]);
             float o[3], d[3];
-            make_ray<D>(geo[pose], u, v, o, d);
+            make_ray_px<D>(geo[pose], dims, u, v, o, d);
             c = siddon_ray<D, ADJ, COUNT, T>(o, d, dims, in, out, (int64_t) pose * dims.pix_per_pose + pix);
         }
         if (COUNT) {

@@ -12,6 +12,7 @@ struct ehp_geometry {
     int vol[3];
     int det[2];
     std::vector<ehp::PoseGeom> host;
+    std::vector<float> pix; // curved detectors: planar coordinate of each pixel centre (empty: planar)
 };
 
 enum op_kind { KIND_JOSEPH = 0, KIND_SIDDON = 1, KIND_VOXEL = 2 };
@@ -26,6 +27,7 @@ struct ehp_op {
     int n_lut = 0;
     float support = 0.0f;
     int differential = 0; // voxel projectors: phase-contrast weight
+    float* d_pix = nullptr; // curved detectors: planar pixel coordinates (dims.pix points here)
     uint64_t samples = 0;
     // Joseph, FP32: plane-major forward and LDS-tiled back projection
     bool planes = false;

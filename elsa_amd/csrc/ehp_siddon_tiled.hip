@@ -30,7 +30,7 @@ namespace ehp
         const int u = (int) (pix % dims.det[0]);
         const int v = (int) (pix / dims.det[0]);
         float o[3], d[3];
-        make_ray<D>(geo[pose], u, v, o, d);
+        make_ray_px<D>(geo[pose], dims, u, v, o, d);
         table[(int64_t) pose * dims.pix_per_pose + pix] = siddon_setup<D>(o, d, dims.vol);
     }
 

@@ -159,6 +159,52 @@ class PlanarDetectorDescriptor(DetectorDescriptor):
     pass
 
 
+class CurvedDetectorDescriptor(DetectorDescriptor):
+    """A detector curved around the source (elsa CurvedDetectorDescriptor,
+    CurvedDetectorDescriptor.h/.cpp): its numberOfCoefficientsPerDimension[0] pixels span
+    `angle` radians on a circle of radius n / angle at source-detector distance s2d. Each pixel
+    centre maps to a planar detector coordinate (getPlanarCoords), through which the
+    Joseph and Siddon projectors trace its ray."""
+
+    def __init__(self, numberOfCoefficientsPerDimension, geometryList, angle: float, s2d: float,
+                 spacingPerDimension=None):
+        super().__init__(numberOfCoefficientsPerDimension, geometryList, spacingPerDimension)
+        self._angle = float(angle)
+        self._s2d = float(s2d)
+        dim = self.getNumberOfDimensions()
+        det = np.asarray(self._n[: dim - 1], np.int64)
+        npix = int(np.prod(det))
+        self._coords = np.zeros(npix * (dim - 1), np.float32)
+        _lib.check(_lib.load().ehp_curved_detector_coords(dim, _i64p(det), self._angle, self._s2d, _fp(self._coords)),
+                   "CurvedDetectorDescriptor")
+
+    def getPlanarCoords(self) -> np.ndarray:
+        """(pixels, dim - 1): the planar coordinate of every pixel centre, u fastest."""
+        return self._coords.reshape(-1, self.getNumberOfDimensions() - 1).copy()
+
+    def getRadius(self) -> float:
+        return float(np.float32(self._n[0]) / np.float32(self._angle))
+
+    def pixel_coords(self):
+        return self._coords
+
+
+class CurvedCircleTrajectoryGenerator:
+    """elsa CurvedCircleTrajectoryGenerator (CurvedCircleTrajectoryGenerator.cpp:8-29): the
+    circular trajectory of CircleTrajectoryGenerator with a curved detector spanning `angle`
+    radians at source-detector distance sourceToCenter + centerToDetector."""
+
+    @staticmethod
+    def createTrajectory(numberOfPoses: int, volumeDescriptor: DataDescriptor, arcDegrees: float,
+                         sourceToCenter: float, centerToDetector: float, angle: float,
+                         detectorSize=None) -> "CurvedDetectorDescriptor":
+        planar = CircleTrajectoryGenerator.createTrajectory(numberOfPoses, volumeDescriptor, arcDegrees, sourceToCenter,
+                                                            centerToDetector, detectorSize=detectorSize)
+        return CurvedDetectorDescriptor(list(planar.getNumberOfCoefficientsPerDimension()), planar.getGeometry(), angle,
+                                        float(np.float32(sourceToCenter) + np.float32(centerToDetector)),
+                                        list(planar.getSpacingPerDimension()))
+
+
 class CircleTrajectoryGenerator:
     """Circular trajectory (elsa/generators/CircleTrajectoryGenerator.cpp:9-24): poses at
     LinSpaced(numberOfPoses, 0, arcDegrees) -- both ends included -- about the last volume

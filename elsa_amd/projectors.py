@@ -108,6 +108,9 @@ class _HipProjector(LinearOperator):
                                                C.ctypes.data_as(fp), ext.ctypes.data_as(fp), sdd.ctypes.data_as(fp),
                                                ctypes.byref(geo)), type(self).__name__)
         try:
+            coords = getattr(rangeDescriptor, "pixel_coords", None)
+            if coords is not None:  # a curved detector
+                _lib.check(self._L.ehp_geometry_set_pixel_coords(geo, coords().ctypes.data_as(fp)), type(self).__name__)
             self._create(geo)
         finally:
             self._L.ehp_geometry_destroy(geo)

@@ -87,6 +87,15 @@ int ehp_geometry_create(int dim, int64_t nposes, const int64_t* vol_dims, const 
                         const float* sdd, ehp_geometry** out);
 void ehp_geometry_destroy(ehp_geometry* g);
 
+/* A curved detector (elsa: CurvedDetectorDescriptor, CurvedDetectorDescriptor.cpp:97-119):
+ * the planar detector coordinate of each pixel centre, det_dims[0] x det_dims[1] pixels (u
+ * fastest), dim - 1 floats each (host memory; NULL returns to a planar detector). Operators
+ * created from the geometry afterwards trace the ray of each pixel through that coordinate
+ * instead of u + 0.5 (v + 0.5). Joseph and Siddon take it (their back projections then run one
+ * thread per ray); the voxel projectors, like elsa's, take planar detectors only. elsa's GPU
+ * projectors ignore curved detectors; its CPU projectors honour them, as these do. */
+int ehp_geometry_set_pixel_coords(ehp_geometry* g, const float* coords);
+
 /* Joseph's method (elsa: JosephsMethodCUDA(volume, detector, fast), JosephsMethodCUDA.h:78-79,
  * with the discretization of the CPU JosephsMethod, JosephsMethod.cpp:140-198). The adjoint
  * is always the matched one (elsa's fast=false); elsa's unmatched fast=true back projection
@@ -252,6 +261,12 @@ int ehp_geometry_pose(int dim, float s2c, float c2d, const float* vol_spacing,
 int ehp_circle_trajectory(int dim, int64_t nposes, const int64_t* vol_dims, float arc_degrees,
                           float s2c, float c2d, const int64_t* det_dims, int64_t* det_out,
                           float* P, float* Pinv, float* C, float* ext, float* sdd);
+
+/* Planar coordinates of the pixel centres of a curved detector of det_dims pixels spanning
+ * `angle` radians at source-detector distance s2d (CurvedDetectorDescriptor::setup,
+ * CurvedDetectorDescriptor.cpp:36-95; getPlanarCoords()). Writes det_dims[0] x det_dims[1] x
+ * (dim - 1) floats. */
+int ehp_curved_detector_coords(int dim, const int64_t* det_dims, float angle, float s2d, float* out);
 
 /* LUT of elsa's ProjectedBlob<data_t, n> (Blobs.h:249-317) and ProjectedBSpline<data_t, n>
  * (BSplines.h:132-208), in the given dtype; *support receives the table's support. */

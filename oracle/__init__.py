@@ -47,6 +47,10 @@ def lib():
     L.orc_xray_f64.argtypes = [c_int, c_int, c_int, c_int, _i64p, _i64p, c_i64, _f64p, _f64p, _f64p, _f64p, c_int]
     L.orc_xray_f32_acc64.argtypes = [c_int, c_int, c_int, c_int, _i64p, _i64p, c_i64, _f32p, _f32p, _f64p, _f64p,
                                      c_int]
+    L.orc_xray_px_f32.argtypes = [c_int, c_int, c_int, c_int, _i64p, _i64p, c_i64, _f32p, _f32p, _f32p, _f32p,
+                                  _f32p, c_int]
+    L.orc_xray_px_f64.argtypes = [c_int, c_int, c_int, c_int, _i64p, _i64p, c_i64, _f64p, _f64p, _f64p, _f64p,
+                                  _f64p, c_int]
     L.orc_voxel_f32.argtypes = [c_int, c_int, _i64p, _i64p, c_i64, _f32p, _f32p, _f32p, _f32p, c_int, c_f,
                                 c_int, _f32p, _f32p, c_int]
     L.orc_voxel_f64.argtypes = [c_int, c_int, _i64p, _i64p, c_i64, _f64p, _f64p, _f64p, _f64p, c_int, c_d,
@@ -204,13 +208,26 @@ def _dims(g: Geom):
     return vol, det
 
 
-def xray(kind, g: Geom, data, adjoint=False, mode=MODE_ELSA, nthreads=0, acc64=False):
+def xray(kind, g: Geom, data, adjoint=False, mode=MODE_ELSA, nthreads=0, acc64=False, pix=None):
     """Joseph (kind=JOSEPH; JosephsMethod.cpp:140-198) or Siddon (kind=SIDDON;
     SiddonsMethod.cpp:76-124) forward (adjoint=False) or back projection. acc64 with an FP32
     geometry keeps the FP32 geometry and weights but takes FP64 data and sums (the exact sum
-    of the FP32 terms)."""
+    of the FP32 terms). pix: a curved detector's planar coordinates of the pixel centres
+    (CurvedDetectorDescriptor::getPlanarCoords, dim - 1 per pixel, u fastest)."""
     L = lib()
     dt = np.float64 if g.Pinv.dtype == np.float64 else np.float32
+    if pix is not None:
+        if acc64:
+            raise ValueError("curved detectors: acc64 is not provided")
+        f = L.orc_xray_px_f64 if dt == np.float64 else L.orc_xray_px_f32
+        vol, det = _dims(g)
+        data = np.ascontiguousarray(data, dtype=dt)
+        out = np.empty(g.vol_shape if adjoint else g.sino_shape, dt)
+        rc = f(kind, g.dim, int(adjoint), mode, vol, det, g.nviews, np.ascontiguousarray(g.Pinv),
+               np.ascontiguousarray(g.C), np.ascontiguousarray(pix, dtype=dt).reshape(-1), data.reshape(-1),
+               out.reshape(-1), nthreads)
+        assert rc == 0
+        return out
     f = L.orc_xray_f64 if dt == np.float64 else L.orc_xray_f32
     if acc64 and dt == np.float32:
         f, dt = L.orc_xray_f32_acc64, np.float64
@@ -223,12 +240,39 @@ def xray(kind, g: Geom, data, adjoint=False, mode=MODE_ELSA, nthreads=0, acc64=F
     return out
 
 
-def joseph(g, data, adjoint=False, mode=MODE_ELSA, nthreads=0, acc64=False):
-    return xray(JOSEPH, g, data, adjoint, mode, nthreads, acc64)
+def joseph(g, data, adjoint=False, mode=MODE_ELSA, nthreads=0, acc64=False, pix=None):
+    return xray(JOSEPH, g, data, adjoint, mode, nthreads, acc64, pix)
 
 
-def siddon(g, data, adjoint=False, nthreads=0, mode=MODE_ELSA, acc64=False):
-    return xray(SIDDON, g, data, adjoint, mode, nthreads, acc64)
+def siddon(g, data, adjoint=False, nthreads=0, mode=MODE_ELSA, acc64=False, pix=None):
+    return xray(SIDDON, g, data, adjoint, mode, nthreads, acc64, pix)
+
+
+def curved_planar_coords(det, angle, s2d, dtype=np.float32):
+    """CurvedDetectorDescriptor::setup / mapCurvedCoordToPlanarCoord
+    (CurvedDetectorDescriptor.cpp:36-95): the planar coordinate of every pixel centre of a
+    curved detector of det = (len[, height]) pixels spanning `angle` radians at source-detector
+    distance s2d, evaluated in dtype (elsa: real_t = float). Shape (pixels, dim - 1)."""
+    dt = np.dtype(dtype).type
+    det = tuple(int(d) for d in det)
+    length, height = det[0], (det[1] if len(det) > 1 else 1)
+    radius = dt(length) / dt(angle)
+    delta = dt(angle) / dt(length)
+    cx, cy = dt(length) / dt(2), dt(height) / dt(2)
+    ccy = dt(s2d) - radius
+    out = []
+    for h in range(height):
+        for l in range(length):
+            pu, pv = dt(l) + dt(0.5), dt(h) + dt(0.5)
+            ra = (pu - cx) * delta
+            x = cx + np.sin(ra, dtype=dt) * radius
+            z = ccy + np.cos(ra, dtype=dt) * radius
+            factor = dt(1) if z == 0 else dt(s2d) / z
+            row = [(x - cx) * factor + cx]
+            if len(det) > 1:
+                row.append((pv - cy) * factor + cy)
+            out.append(row)
+    return np.array(out, dtype=dt)
 
 
 def voxel(g: Geom, data, lut, support, adjoint=False, nthreads=0, differential=False):

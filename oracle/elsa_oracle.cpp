@@ -91,13 +91,16 @@ namespace orc
     // the camera centre C. The direction is Pinv * [u+.5, (v+.5,) 1], head(D), normalised
     // (division by the square root of the sequential sum of squares, as Eigen's
     // normalized() on a dynamic vector does).
+    // make_ray_at takes the detector coordinate itself (DetectorDescriptor::
+    // computeRayFromDetectorCoord), e.g. a curved detector's planar coordinate of a pixel
+    // centre (CurvedDetectorDescriptor.cpp:97-119).
     template <class R, int D>
-    void make_ray(const R* Pinv, const R* C, i64 u, i64 v, R* o, R* d)
+    void make_ray_at(const R* Pinv, const R* C, R hu, R hv, R* o, R* d)
     {
         R h[3];
-        h[0] = R(u) + R(0.5);
+        h[0] = hu;
         if (D == 3) {
-            h[1] = R(v) + R(0.5);
+            h[1] = hv;
             h[2] = R(1);
         } else {
             h[1] = R(1);
@@ -117,6 +120,12 @@ namespace orc
             d[i] = n2 > R(0) ? y[i] / n : y[i];
             o[i] = C[i];
         }
+    }
+
+    template <class R, int D>
+    void make_ray(const R* Pinv, const R* C, i64 u, i64 v, R* o, R* d)
+    {
+        make_ray_at<R, D>(Pinv, C, R(u) + R(0.5), R(v) + R(0.5), o, d);
     }
 
     // State of elsa's DrivingDirectionTraversal after construction
@@ -262,6 +271,7 @@ namespace orc
         i64 nvox;
         i64 pix_per_view;
         i64 nrays;
+        const void* pix; // curved detectors: planar pixel coordinates (type R, dim - 1 per pixel)
     };
 
     inline Shape make_shape(int dim, const i64* vol, const i64* det, i64 nviews)
@@ -289,6 +299,11 @@ namespace orc
         const i64 rem = ir - view * sh.pix_per_view;
         const i64 u = rem % sh.det[0];
         const i64 v = rem / sh.det[0];
+        if (sh.pix) {
+            const R* c = static_cast<const R*>(sh.pix) + rem * (D - 1);
+            make_ray_at<R, D>(Pinv + view * (D + 1) * D, C + view * D, c[0], D == 3 ? c[D - 2] : R(0), o, d);
+            return;
+        }
         make_ray<R, D>(Pinv + view * (D + 1) * D, C + view * D, u, v, o, d);
     }
 
@@ -945,11 +960,12 @@ namespace orc
 
     template <class R, class T>
     int dispatch_projector(int kind, int dim, int adjoint, int mode, const i64* vol, const i64* det,
-                           i64 nviews, const R* Pinv, const R* C, const T* in, T* out)
+                           i64 nviews, const R* Pinv, const R* C, const T* in, T* out, const R* pix = nullptr)
     {
         if (dim != 2 && dim != 3)
             return -1;
-        const Shape sh = make_shape(dim, vol, det, nviews);
+        Shape sh = make_shape(dim, vol, det, nviews);
+        sh.pix = pix;
         if (kind == 0) {
             if (dim == 2)
                 adjoint ? joseph_all<R, T, 2, true>(sh, Pinv, C, mode, in, out)
@@ -1039,6 +1055,25 @@ int orc_xray_f64(int kind, int dim, int adjoint, int mode, const int64_t* vol, c
     orc::set_threads(nthreads);
     return orc::dispatch_projector<double, double>(kind, dim, adjoint, mode, vol, det, nviews,
                                                    Pinv, C, in, out);
+}
+
+// The same for a curved detector: pix holds the planar coordinate of every pixel centre
+// (det[0] x det[1] pixels, dim - 1 values each, in the geometry's type).
+int orc_xray_px_f32(int kind, int dim, int adjoint, int mode, const int64_t* vol, const int64_t* det,
+                    int64_t nviews, const float* Pinv, const float* C, const float* pix, const float* in,
+                    float* out, int nthreads)
+{
+    orc::set_threads(nthreads);
+    return orc::dispatch_projector<float, float>(kind, dim, adjoint, mode, vol, det, nviews, Pinv, C, in, out, pix);
+}
+
+int orc_xray_px_f64(int kind, int dim, int adjoint, int mode, const int64_t* vol, const int64_t* det,
+                    int64_t nviews, const double* Pinv, const double* C, const double* pix, const double* in,
+                    double* out, int nthreads)
+{
+    orc::set_threads(nthreads);
+    return orc::dispatch_projector<double, double>(kind, dim, adjoint, mode, vol, det, nviews, Pinv, C, in, out,
+                                                   pix);
 }
 
 // FP32 geometry and weights with FP64 data and sums: the exact sum of the FP32 terms, the
