@@ -72,12 +72,57 @@ namespace ehp
             }
         }
 
-        template <int D, int A, bool V16>
-        __device__ inline float fwd_ray(const RayRec& r, const Dims& dims, const ViewSet& vs)
+        template <int D, int A, bool NEG, bool V16>
+        __device__ inline float fwd_span(const RayRec& r, const Dims& dims, const ViewSet& vs, int t0, int niter,
+                                         int s_first, int full0, int full1)
         {
             if (D == 3 && (vs.fb2 >> A) & 1)
-                return plane_ray_forward<D, A, true, V16>(r, dims, vs.v[A]);
-            return plane_ray_forward<D, A, false, V16>(r, dims, vs.v[A]);
+                return plane_span_forward<D, A, NEG, true, V16>(r, dims, vs.v[A], t0, niter, s_first, full0, full1);
+            return plane_span_forward<D, A, NEG, false, V16>(r, dims, vs.v[A], t0, niter, s_first, full0, full1);
+        }
+
+        // Reductions over the 64 lanes of a wavefront (every lane active), returned in a scalar register.
+        __device__ __forceinline__ int wave_min(int x)
+        {
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1)
+                x = min(x, __shfl_xor(x, o, 64));
+            return __builtin_amdgcn_readfirstlane(x);
+        }
+
+        __device__ __forceinline__ int wave_max(int x)
+        {
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1)
+                x = max(x, __shfl_xor(x, o, 64));
+            return __builtin_amdgcn_readfirstlane(x);
+        }
+
+        __device__ __forceinline__ int wave_or(int x)
+        {
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1)
+                x |= __shfl_xor(x, o, 64);
+            return __builtin_amdgcn_readfirstlane(x);
+        }
+
+        // The rays of group K = 2 * axis + direction of a wavefront share one plane loop over the
+        // group's span. Called with every lane active; `keys` holds the groups present.
+        template <int D, int K, bool V16>
+        __device__ __forceinline__ void fwd_group(int keys, int key, const RayRec& r, const Dims& dims,
+                                                  const ViewSet& vs, int tfirst, int tlast, int s0, float& val)
+        {
+            if (!((keys >> K) & 1))
+                return;
+            const bool mine = key == K;
+            const int t0 = wave_min(mine ? tfirst : 0x7fffffff);
+            const int t1 = wave_max(mine ? tlast : -0x7fffffff);
+            const int ta = wave_max(mine ? tfirst : -0x7fffffff); // every ray of the group has a sample
+            const int tb = wave_min(mine ? tlast : 0x7fffffff);   // in [ta, tb]
+            const int full0 = ta - t0, full1 = tb - t0 + 1 > full0 ? tb - t0 + 1 : full0;
+            if (mine)
+                val = fwd_span<D, K / 2, (K & 1) != 0, V16>(r, dims, vs, t0, t1 - t0 + 1, s0 + t0 - tfirst, full0,
+                                                            full1);
         }
 
         template <class E>
@@ -108,7 +153,13 @@ namespace ehp
             return (int64_t) gx * gy * dims.vol[3 - L.f];
         }
 
-        // Block id of one pose-band tile of the forward: id = (band * nposes + pose) * tiles_u + tu
+        // Block id of one pose-band tile of the forward: id = (band * nposes + pose) * tiles_u + tu.
+        // Each wavefront holds 64 neighbouring pixels of one detector row. Its rays fall into
+        // groups by driving axis and direction (in practice one or two), and each group runs one
+        // plane loop (plane_span_forward), so that at every step all its lanes read one plane. A
+        // per-lane loop would put rays that enter through a side face of the volume, each at its
+        // own plane, on a cache line of their own at every step. Every lane of the block takes
+        // part in the wavefront reductions; lanes past the detector edge join no group.
         template <int D, class OT = float>
         __device__ __forceinline__ void fwd_planes_block(const PoseGeom* __restrict__ geo, const Dims& dims,
                                                          const ViewSet& views, const int tiles_u, const int64_t id,
@@ -122,22 +173,31 @@ namespace ehp
             const int band = (int) (bp / dims.nposes);
             const int u = tu * TU + (int) threadIdx.x % TU;
             const int v = band * TV + (int) threadIdx.x / TU;
-            if (u >= dims.det[0] || v >= dims.det[1])
-                return;
-            float o[3], d[3];
-            make_ray_px<D>(geo[pose], dims, u, v, o, d);
-            JRay jr;
-            joseph_setup<D>(o, d, dims.vol, jr);
-            const RayRec r = make_ray_rec<D>(jr);
+            const bool inside = u < dims.det[0] && v < dims.det[1];
+            RayRec r{};
+            int key = -1, tfirst = 0, tlast = 0, s0 = 0;
+            if (inside) {
+                float o[3], d[3];
+                make_ray_px<D>(geo[pose], dims, u, v, o, d);
+                JRay jr;
+                joseph_setup<D>(o, d, dims.vol, jr);
+                r = make_ray_rec<D>(jr);
+                if (plane_ray_span(r, dims.vol[jr.axis], tfirst, tlast, s0))
+                    key = 2 * jr.axis + ((r.meta & 4) ? 1 : 0);
+            }
             constexpr bool V16 = sizeof(OT) == 2; // bfloat16 operators read bfloat16 views
-            float val;
-            if (jr.axis == 0)
-                val = fwd_ray<D, 0, V16>(r, dims, views);
-            else if (D == 2 || jr.axis == 1)
-                val = fwd_ray<D, 1, V16>(r, dims, views);
-            else
-                val = fwd_ray<D, D - 1, V16>(r, dims, views);
-            sino[(int64_t) pose * dims.pix_per_pose + (int64_t) v * dims.det[0] + u] = down<OT>(val);
+            const int keys = wave_or(key >= 0 ? 1 << key : 0);
+            float val = 0.0f;
+            fwd_group<D, 0, V16>(keys, key, r, dims, views, tfirst, tlast, s0, val);
+            fwd_group<D, 1, V16>(keys, key, r, dims, views, tfirst, tlast, s0, val);
+            fwd_group<D, 2, V16>(keys, key, r, dims, views, tfirst, tlast, s0, val);
+            fwd_group<D, 3, V16>(keys, key, r, dims, views, tfirst, tlast, s0, val);
+            if (D == 3) {
+                fwd_group<D, 2 * D - 2, V16>(keys, key, r, dims, views, tfirst, tlast, s0, val);
+                fwd_group<D, 2 * D - 1, V16>(keys, key, r, dims, views, tfirst, tlast, s0, val);
+            }
+            if (inside)
+                sino[(int64_t) pose * dims.pix_per_pose + (int64_t) v * dims.det[0] + u] = down<OT>(val);
         }
 
         template <int D, class OT>

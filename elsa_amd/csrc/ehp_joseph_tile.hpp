@@ -481,144 +481,213 @@ namespace ehp
     extern bool g_view_oob;
 #endif
 
-    // base + byte offset (a 32-bit offset from a uniform base: one VGPR per address)
-    EHP_HD inline float ld_view(const PadView& V, uint32_t off4)
-    {
-#if defined(EHP_VIEW_CHECK)
-        if ((off4 & 3u) || (int64_t) (off4 >> 2) >= V.size) {
-            g_view_oob = true;
-            return 0.0f;
-        }
-#endif
-        return *(const float*) ((const char*) V.base + off4);
-    }
-
-    // floor(x) clamped to [-2, n] and shifted by the padding: the in-plane index in a view
     // Two neighbouring floats of a view (one 8-byte load; 4-byte alignment suffices).
     struct __attribute__((aligned(4))) Pair {
         float a, b;
     };
 
-    EHP_HD inline Pair ld_view2(const PadView& V, uint32_t off4)
+    // The two elements at byte offset off from pb, a pointer into view V: pb is the same for a
+    // whole wavefront (a plane or row start, held in scalar registers), off a 32-bit offset.
+    EHP_HD inline Pair ld_view2(const PadView& V, const char* pb, uint32_t off)
     {
 #if defined(EHP_VIEW_CHECK)
-        if ((off4 & 3u) || (int64_t) (off4 >> 2) + 1 >= V.size) {
+        const int64_t o = (int64_t) (pb - (const char*) V.base) + off;
+        if ((o & 3) || o < 0 || (o >> 2) + 1 >= V.size) {
             g_view_oob = true;
             return Pair{0.0f, 0.0f};
         }
 #endif
         Pair p;
-        __builtin_memcpy(&p, (const char*) V.base + off4, sizeof(Pair));
+        __builtin_memcpy(&p, pb + off, sizeof(Pair));
         return p;
     }
 
     // Two neighbouring bfloat16 elements of a view (one 4-byte load), widened to FP32.
-    EHP_HD inline Pair ld_view2_bf16(const PadView& V, uint32_t off2)
+    EHP_HD inline Pair ld_view2_bf16(const PadView& V, const char* pb, uint32_t off)
     {
 #if defined(EHP_VIEW_CHECK)
-        if ((off2 & 1u) || (int64_t) (off2 >> 1) + 1 >= V.size) {
+        const int64_t o = (int64_t) (pb - (const char*) V.base) + off;
+        if ((o & 1) || o < 0 || (o >> 1) + 1 >= V.size) {
             g_view_oob = true;
             return Pair{0.0f, 0.0f};
         }
 #endif
         uint16_t h[2];
-        __builtin_memcpy(h, (const char*) V.base + off2, sizeof(h));
+        __builtin_memcpy(h, pb + off, sizeof(h));
         return Pair{bf16_to_float(bf16{h[0]}), bf16_to_float(bf16{h[1]})};
     }
 
     template <bool V16>
-    EHP_HD inline Pair ld_pair(const PadView& V, uint32_t off)
+    EHP_HD inline Pair ld_pair(const PadView& V, const char* pb, uint32_t off)
     {
-        return V16 ? ld_view2_bf16(V, off) : ld_view2(V, off);
+        return V16 ? ld_view2_bf16(V, pb, off) : ld_view2(V, pb, off);
     }
 
-    EHP_HD inline uint32_t pad_coord(float g, int n)
+    // floor value g clamped to [-2, n]: taps outside the volume land in the zero padding
+    EHP_HD inline int pad_clamp(float g, int n)
     {
 #if defined(__HIP_DEVICE_COMPILE__)
         const float c = __builtin_amdgcn_fmed3f(g, -2.0f, (float) n);
 #else
         const float c = g < -2.0f ? -2.0f : (g > (float) n ? (float) n : g);
 #endif
-        return (uint32_t) ((int) c + 2);
+        return (int) c;
     }
 
-    // Forward projection of one ray (driving axis A) through its padded view; FB2 says that
-    // the fast in-plane axis of the view is b2 (else b1). The sum over the ray's samples of
-    // the interpolated volume in the order and with the weights of elsa's doInterpolation
-    // (JosephsMethod.cpp:60-97): 2D taps (F0,C1)(C0,F1), 3D taps
-    // (F0,F1,F2)(F0,C1,C2)(C0,F1,C2)(C0,C1,F2), the driving axis' floor and ceil being its
-    // plane; each tap adds (value * len) * weight, elsa's association. C = F + 1 is read even
-    // where elsa's ceil gives C = F: the weight of that tap is 0 then. Samples whose plane lies
-    // outside the volume (a ray entering exactly through the far face) contribute 0 and are
-    // skipped. Positions are those of joseph_sample and tile_ray_adjoint.
-    template <int D, int A, bool FB2, bool V16 = false>
-    EHP_HD inline float plane_ray_forward(const RayRec& r, const Dims& dims, const PadView& V)
+    // a * b for |a|, |b| < 2^23 (one 24-bit multiply on the GPU)
+    EHP_HD inline int smul24(int a, int b)
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        return __mul24(a, b);
+#else
+        return a * b;
+#endif
+    }
+
+    // Samples [s0, s1) of a ray whose planes lie inside the volume of na planes along its
+    // driving axis (a ray entering exactly through the far face has samples outside, which
+    // contribute 0). Empty when s0 >= s1.
+    EHP_HD inline void plane_sample_range(const RayRec& r, int na, int& s0, int& s1)
+    {
+        const bool neg = (r.meta & 4) != 0;
+        s0 = neg ? r.P0 - na + 1 : -r.P0;
+        s1 = neg ? r.P0 + 1 : na - r.P0;
+        s0 = s0 > 0 ? s0 : 0;
+        s1 = s1 < r.nsteps ? s1 : r.nsteps;
+    }
+
+    // Sample s (fs = s) of a ray (driving axis A) on the plane at byte offset poff of its padded
+    // view, added to acc; FB2 says that the fast in-plane axis of the view is b2 (else b1). The
+    // interpolation has the order and weights of elsa's doInterpolation (JosephsMethod.cpp:60-97):
+    // 2D taps (F0,C1)(C0,F1), 3D taps (F0,F1,F2)(F0,C1,C2)(C0,F1,C2)(C0,C1,F2), the driving axis'
+    // floor and ceil being its plane; each tap adds (value * len) * weight, elsa's association.
+    // C = F + 1 is read even where elsa's ceil gives C = F: the weight of that tap is 0 then.
+    // Positions are those of joseph_sample and tile_ray_adjoint.
+    template <int D, int A, bool FB2, bool V16>
+    EHP_HD inline float plane_sample_forward(const RayRec& r, const Dims& dims, const PadView& V, uint32_t poff,
+                                             float fs, float acc)
     {
         constexpr uint32_t ES = V16 ? 1u : 2u; // log2 of the element size
         constexpr int b1 = TileAxes<D, A>::b1, b2 = TileAxes<D, A>::b2;
-        const int na = dims.vol[A], n1 = dims.vol[b1], n2 = (D == 3) ? dims.vol[b2] : 1;
-        const bool neg = (r.meta & 4) != 0;
-        // samples on planes [0, na)
-        int s0 = neg ? r.P0 - na + 1 : -r.P0;
-        int s1 = neg ? r.P0 + 1 : na - r.P0;
-        s0 = s0 > 0 ? s0 : 0;
-        s1 = s1 < r.nsteps ? s1 : r.nsteps;
+        const int n1 = dims.vol[b1], n2 = (D == 3) ? dims.vol[b2] : 1;
         const float len = r.len;
+        const char* pb = (const char*) V.base + poff; // the plane (the same for the whole wavefront)
+        const float q1 = __builtin_fmaf(fs, r.s1, r.p1);
+        const float g1 = floorf(q1 - 0.5f);
+        const float c1 = (q1 - g1) - 0.5f;
+        const float w1 = 1.0f - c1;
+        const int k1 = pad_clamp(g1, n1);
+        if (D == 2) {
+            // offset of element k1 + 2 of the plane's only row
+            const Pair fk = ld_pair<V16>(V, pb, ((uint32_t) k1 << ES) + (2u << ES));
+            const float vf = fk.a, vk = fk.b;
+            // A = 0: (pa, C1) then (pa, F1); A = 1: (F0, pa) then (C0, pa)
+            if (A == 0) {
+                acc += vk * len * c1;
+                acc += vf * len * w1;
+            } else {
+                acc += vf * len * w1;
+                acc += vk * len * c1;
+            }
+        } else {
+            const float q2 = __builtin_fmaf(fs, r.s2, r.p2);
+            const float g2 = floorf(q2 - 0.5f);
+            const float c2 = (q2 - g2) - 0.5f;
+            const float w2 = 1.0f - c2;
+            const int k2 = pad_clamp(g2, n2);
+            // the F and F + 1 taps along the fast axis are neighbours: two 8-byte loads, one per
+            // row of the slow axis; (slow + 2) * RS + (fast + 2) elements in, the 2s being the padding
+            const int ks = FB2 ? k1 : k2, kf = FB2 ? k2 : k1;
+            const uint32_t o =
+                (uint32_t) (smul24(ks, (int) V.RS4) + (int) (2u * V.RS4 + (2u << ES))) + ((uint32_t) kf << ES);
+            const Pair r0 = ld_pair<V16>(V, pb, o), r1 = ld_pair<V16>(V, pb + V.RS4, o);
+            // FB2: rows are b1 = F1, F1 + 1, pairs run along b2; else rows are b2, pairs along b1
+            const float vff = r0.a;
+            const float vkf = FB2 ? r1.a : r0.b;
+            const float vfk = FB2 ? r0.b : r1.a;
+            const float vkk = r1.b;
+            const float wff = w1 * w2, wkk = c1 * c2, wfk = w1 * c2, wkf = c1 * w2;
+            acc += vff * len * wff;
+            if (A == 0) {
+                acc += vkk * len * wkk;
+                acc += vfk * len * wfk;
+                acc += vkf * len * wkf;
+            } else if (A == 1) {
+                acc += vfk * len * wfk;
+                acc += vkk * len * wkk;
+                acc += vkf * len * wkf;
+            } else {
+                acc += vfk * len * wfk;
+                acc += vkf * len * wkf;
+                acc += vkk * len * wkk;
+            }
+        }
+        return acc;
+    }
+
+    // Forward projection of one ray (driving axis A) through its padded view: the sum of
+    // plane_sample_forward over its samples in the volume, in traversal order.
+    template <int D, int A, bool FB2, bool V16 = false>
+    EHP_HD inline float plane_ray_forward(const RayRec& r, const Dims& dims, const PadView& V)
+    {
+        int s0, s1;
+        plane_sample_range(r, dims.vol[A], s0, s1);
         float acc = 0.0f;
         if (s0 >= s1)
             return acc;
+        const bool neg = (r.meta & 4) != 0;
         uint32_t poff = (uint32_t) (neg ? r.P0 - s0 : r.P0 + s0) * V.PS4;
         const uint32_t dp = neg ? 0u - V.PS4 : V.PS4;
-        for (int s = s0; s < s1; ++s, poff += dp) {
-            const float fs = (float) s;
-            const float q1 = __builtin_fmaf(fs, r.s1, r.p1);
-            const float g1 = floorf(q1 - 0.5f);
-            const float c1 = (q1 - g1) - 0.5f;
-            const float w1 = 1.0f - c1;
-            const uint32_t i1 = pad_coord(g1, n1);
-            if (D == 2) {
-                const Pair fk = ld_pair<V16>(V, poff + (i1 << ES));
-                const float vf = fk.a, vk = fk.b;
-                // A = 0: (pa, C1) then (pa, F1); A = 1: (F0, pa) then (C0, pa)
-                if (A == 0) {
-                    acc += vk * len * c1;
-                    acc += vf * len * w1;
-                } else {
-                    acc += vf * len * w1;
-                    acc += vk * len * c1;
-                }
-            } else {
-                const float q2 = __builtin_fmaf(fs, r.s2, r.p2);
-                const float g2 = floorf(q2 - 0.5f);
-                const float c2 = (q2 - g2) - 0.5f;
-                const float w2 = 1.0f - c2;
-                const uint32_t i2 = pad_coord(g2, n2);
-                // the F and F + 1 taps along the fast axis are neighbours: two 8-byte loads, one per
-                // row of the slow axis
-                const uint32_t o = FB2 ? poff + mul24(i1, V.RS4) + (i2 << ES) : poff + mul24(i2, V.RS4) + (i1 << ES);
-                const Pair r0 = ld_pair<V16>(V, o), r1 = ld_pair<V16>(V, o + V.RS4);
-                // FB2: rows are b1 = F1, F1 + 1, pairs run along b2; else rows are b2, pairs along b1
-                const float vff = r0.a;
-                const float vkf = FB2 ? r1.a : r0.b;
-                const float vfk = FB2 ? r0.b : r1.a;
-                const float vkk = r1.b;
-                const float wff = w1 * w2, wkk = c1 * c2, wfk = w1 * c2, wkf = c1 * w2;
-                acc += vff * len * wff;
-                if (A == 0) {
-                    acc += vkk * len * wkk;
-                    acc += vfk * len * wfk;
-                    acc += vkf * len * wkf;
-                } else if (A == 1) {
-                    acc += vfk * len * wfk;
-                    acc += vkk * len * wkk;
-                    acc += vkf * len * wkf;
-                } else {
-                    acc += vfk * len * wfk;
-                    acc += vkf * len * wkf;
-                    acc += vkk * len * wkk;
-                }
-            }
-        }
+        for (int s = s0; s < s1; ++s, poff += dp)
+            acc = plane_sample_forward<D, A, FB2, V16>(r, dims, V, poff, (float) s, acc);
+        return acc;
+    }
+
+    // A ray's samples in traversal coordinates, t = plane when it steps towards higher planes and
+    // t = -plane when it steps towards lower ones (NEG), so that t rises by one per sample:
+    // samples s0 .. s1 - 1 lie at t = tfirst .. tlast. False for a ray with no sample in the volume.
+    EHP_HD inline bool plane_ray_span(const RayRec& r, int na, int& tfirst, int& tlast, int& s0)
+    {
+        int s1;
+        plane_sample_range(r, na, s0, s1);
+        if (s0 >= s1)
+            return false;
+        const bool neg = (r.meta & 4) != 0;
+        tfirst = neg ? s0 - r.P0 : r.P0 + s0;
+        tlast = tfirst + (s1 - s0 - 1);
+        return true;
+    }
+
+    // plane_ray_forward with the plane loop shared by a group of rays of one driving axis and
+    // direction (a wavefront, on the GPU): iteration it visits the plane at traversal coordinate
+    // t0 + it, the same plane for every ray of the group, so their reads fall on neighbouring
+    // addresses; the ray adds its sample s = s_first + it when s0 <= s < s1. t0 and niter span
+    // the group's [min tfirst, max tlast] (plane_ray_span), s_first = s0 + t0 - tfirst. In
+    // iterations [full0, full1) every ray of the group has a sample (full0 = max tfirst - t0,
+    // full1 = max(full0, min tlast - t0 + 1)), so they need no test. Same samples, positions
+    // and order as plane_ray_forward, so the same sum bit for bit.
+    template <int D, int A, bool NEG, bool FB2, bool V16 = false>
+    EHP_HD inline float plane_span_forward(const RayRec& r, const Dims& dims, const PadView& V, int t0, int niter,
+                                           int s_first, int full0, int full1)
+    {
+        int s0, s1;
+        plane_sample_range(r, dims.vol[A], s0, s1);
+        const uint32_t n = (uint32_t) (s1 - s0);
+        uint32_t j = (uint32_t) (s_first - s0); // the ray's sample index minus s0 at this iteration
+        float fs = (float) s_first;            // its sample index (exact: |s| < 2^24)
+        uint32_t poff = (uint32_t) (NEG ? -t0 : t0) * V.PS4;
+        const uint32_t dp = NEG ? 0u - V.PS4 : V.PS4;
+        float acc = 0.0f;
+        int it = 0;
+        for (; it < full0; ++it, ++j, fs += 1.0f, poff += dp)
+            if (j < n)
+                acc = plane_sample_forward<D, A, FB2, V16>(r, dims, V, poff, fs, acc);
+        for (; it < full1; ++it, fs += 1.0f, poff += dp)
+            acc = plane_sample_forward<D, A, FB2, V16>(r, dims, V, poff, fs, acc);
+        j += (uint32_t) (full1 - full0);
+        for (; it < niter; ++it, ++j, fs += 1.0f, poff += dp)
+            if (j < n)
+                acc = plane_sample_forward<D, A, FB2, V16>(r, dims, V, poff, fs, acc);
         return acc;
     }
 } // namespace ehp

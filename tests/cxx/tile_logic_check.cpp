@@ -214,12 +214,100 @@ static float fwd_one(const RayRec& r, const Dims& dims, const PadView& V, bool f
     return fb2 ? plane_ray_forward<D, A, true>(r, dims, V) : plane_ray_forward<D, A, false>(r, dims, V);
 }
 
-// b16: bfloat16 views (the volume rounded to bfloat16), read by the bfloat16 variant of the
-// kernel's ray loop
+struct Span {
+    int t0, niter, s_first, full0, full1;
+};
+
+template <int D, int A, bool NEG>
+static float span_one(const RayRec& r, const Dims& dims, const PadView& V, bool fb2, bool b16, const Span& q)
+{
+    if (b16)
+        return fb2 ? plane_span_forward<D, A, NEG, true, true>(r, dims, V, q.t0, q.niter, q.s_first, q.full0, q.full1)
+                   : plane_span_forward<D, A, NEG, false, true>(r, dims, V, q.t0, q.niter, q.s_first, q.full0,
+                                                                 q.full1);
+    return fb2 ? plane_span_forward<D, A, NEG, true>(r, dims, V, q.t0, q.niter, q.s_first, q.full0, q.full1)
+               : plane_span_forward<D, A, NEG, false>(r, dims, V, q.t0, q.niter, q.s_first, q.full0, q.full1);
+}
+
+template <int D>
+static float span_key(int key, const RayRec& r, const Dims& dims, const PadView* pv, const bool* fb2, bool b16,
+                      const Span& q)
+{
+    switch (key) {
+    case 0:
+        return span_one<D, 0, false>(r, dims, pv[0], fb2[0], b16, q);
+    case 1:
+        return span_one<D, 0, true>(r, dims, pv[0], fb2[0], b16, q);
+    case 2:
+        return span_one<D, 1, false>(r, dims, pv[1], fb2[1], b16, q);
+    case 3:
+        return span_one<D, 1, true>(r, dims, pv[1], fb2[1], b16, q);
+    case 4:
+        return span_one<D, D - 1, false>(r, dims, pv[D - 1], fb2[D - 1], b16, q);
+    default:
+        return span_one<D, D - 1, true>(r, dims, pv[D - 1], fb2[D - 1], b16, q);
+    }
+}
+
+// The GPU forward's grouping: the 64 pixels u0 .. u0 + 63 of row v form a wavefront; its rays
+// fall into groups by driving axis and direction, each with one plane loop over the group's
+// span (plane_ray_span, plane_span_forward). Pixels past the detector edge join no group.
+static int64_t g_mixed_waves = 0; // wavefronts of the last plane_forward call with more than one group
+
+template <int D>
+static void wave_forward(const Dims& dims, const PoseGeom& g, int v, int u0, const PadView* pv, const bool* fb2,
+                         bool b16, float* out)
+{
+    RayRec rr[64];
+    int key[64], tf[64], tl[64], s0[64];
+    for (int l = 0; l < 64; ++l) {
+        key[l] = -1;
+        const int u = u0 + l;
+        if (u >= dims.det[0])
+            continue;
+        float o[3], d[3];
+        JRay jr;
+        make_ray<D>(g, u, v, o, d);
+        joseph_setup<D>(o, d, dims.vol, jr);
+        rr[l] = make_ray_rec<D>(jr);
+        if (plane_ray_span(rr[l], dims.vol[jr.axis], tf[l], tl[l], s0[l]))
+            key[l] = 2 * jr.axis + ((rr[l].meta & 4) ? 1 : 0);
+        out[l] = 0.0f;
+    }
+    int groups = 0;
+    for (int k = 0; k < 2 * D; ++k) {
+        int t0 = 0x7fffffff, t1 = -0x7fffffff, ta = -0x7fffffff, tb = 0x7fffffff;
+        for (int l = 0; l < 64; ++l)
+            if (key[l] == k) {
+                t0 = tf[l] < t0 ? tf[l] : t0;
+                t1 = tl[l] > t1 ? tl[l] : t1;
+                ta = tf[l] > ta ? tf[l] : ta;
+                tb = tl[l] < tb ? tl[l] : tb;
+            }
+        if (t0 > t1)
+            continue;
+        ++groups;
+        const int full0 = ta - t0, full1 = tb - t0 + 1 > full0 ? tb - t0 + 1 : full0;
+        for (int l = 0; l < 64; ++l)
+            if (key[l] == k)
+                out[l] = span_key<D>(k, rr[l], dims, pv, fb2, b16, Span{t0, t1 - t0 + 1, s0[l] + t0 - tf[l], full0, full1});
+    }
+    g_mixed_waves += groups > 1;
+}
+
+extern "C" int64_t plane_forward_mixed_waves()
+{
+    return g_mixed_waves;
+}
+
+// flags: bit 0 bfloat16 views (the volume rounded to bfloat16), read by the bfloat16 variant of
+// the kernel's ray loop; bit 1 the GPU's wavefront grouping (wave_forward) instead of one loop
+// per ray
 extern "C" int plane_forward(int dim, const int64_t* vol_dims, const int64_t* det_dims, int nposes, const float* P,
                              const float* Pinv, const float* C, const int* fan, const float* vol, float* sino,
-                             int b16)
+                             int flags)
 {
+    const bool b16 = flags & 1, waves = (flags & 2) != 0;
     const Dims dims = make_dims(dim, vol_dims, det_dims, nposes);
     const int nm = dim * (dim + 1);
     std::vector<PoseGeom> geo(nposes);
@@ -262,6 +350,21 @@ extern "C" int plane_forward(int dim, const int64_t* vol_dims, const int64_t* de
             pv[a] = PadView{views[a].data(), L.PS * 4u, L.RS * 4u, L.size};
         }
         fb2[a] = dim == 3 && fan[a] == (a == 2 ? 1 : 2);
+    }
+    if (waves) {
+        g_mixed_waves = 0;
+        float lane[64];
+        for (int p = 0; p < nposes; ++p)
+            for (int v = 0; v < dims.det[1]; ++v)
+                for (int u0 = 0; u0 < dims.det[0]; u0 += 64) {
+                    if (dim == 2)
+                        wave_forward<2>(dims, geo[p], v, u0, pv, fb2, b16, lane);
+                    else
+                        wave_forward<3>(dims, geo[p], v, u0, pv, fb2, b16, lane);
+                    for (int l = 0; l < 64 && u0 + l < dims.det[0]; ++l)
+                        sino[(int64_t) p * dims.pix_per_pose + (int64_t) v * dims.det[0] + u0 + l] = lane[l];
+                }
+        return g_view_oob ? 22 : 0;
     }
     for (int p = 0; p < nposes; ++p)
         for (int v = 0; v < dims.det[1]; ++v)

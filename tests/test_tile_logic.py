@@ -28,6 +28,7 @@ def lib():
                                ctypes.c_int, f32, f32, f64]
     L.plane_forward.argtypes = [ctypes.c_int, i64, i64, ctypes.c_int, f32, f32, f32,
                                 np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS"), f32, f32, ctypes.c_int]
+    L.plane_forward_mixed_waves.restype = ctypes.c_int64
     return L
 
 
@@ -80,11 +81,13 @@ FWD_CASES = [
 ]
 
 
+@pytest.mark.parametrize("waves", [False, True], ids=["per_ray", "wavefront"])
 @pytest.mark.parametrize("case", FWD_CASES, ids=lambda c: f"{c[0]}d-{'x'.join(map(str, c[1]))}")
-def test_plane_forward_logic_matches_oracle_bitwise(lib, case):
+def test_plane_forward_logic_matches_oracle_bitwise(lib, case, waves):
     """Every zero-padded plane-major view a driving axis can get gives the oracle's forward, bit
     for bit (same positions, taps and summation order as the one-thread-per-ray rule), with
-    every read inside its view."""
+    every read inside its view -- both with one plane loop per ray and with the GPU's grouping,
+    where the rays of a wavefront of one driving axis and direction share one plane loop."""
     dim, vol, views, det, s2c, c2d, fans = case
     rng = np.random.default_rng(2)
     angles = np.sort(rng.uniform(0, 360, views)).astype(np.float32)
@@ -96,9 +99,28 @@ def test_plane_forward_logic_matches_oracle_bitwise(lib, case):
         rc = lib.plane_forward(dim, np.array(list(vol) + [1] * (3 - dim), np.int64),
                                np.array(list(det) + [1] * (2 - len(det)), np.int64), views,
                                np.ascontiguousarray(g.P), np.ascontiguousarray(g.Pinv), np.ascontiguousarray(g.C),
-                               np.array(list(fan) + [0] * (3 - len(fan)), np.int32), x, out, 0)
+                               np.array(list(fan) + [0] * (3 - len(fan)), np.int32), x, out, 2 if waves else 0)
         assert rc == 0, (fan, rc)
         np.testing.assert_array_equal(out, ref, err_msg=f"fan axes {fan}")
+
+
+def test_plane_forward_wavefront_groups_cover_mixed_waves(lib):
+    """A strongly divergent cone (source 60, detector 40 from the centre) puts rays of several
+    driving axes and directions into one wavefront; each group's shared plane loop still gives
+    the oracle bit for bit."""
+    dim, vol, views, det = 3, (40, 24, 33), 13, (61, 29)
+    rng = np.random.default_rng(7)
+    angles = np.sort(rng.uniform(0, 360, views)).astype(np.float32)
+    g = oracle.circle_geometry(dim, vol, views, 360, 20.0, 40.0, det=det, angles_deg=angles)
+    x = rng.standard_normal(g.vol_shape).astype(np.float32)
+    ref = oracle.joseph(g, x, mode=oracle.MODE_DIRECT)
+    out = np.full(g.sino_shape, np.nan, np.float32)
+    rc = lib.plane_forward(dim, np.array(vol, np.int64), np.array(det, np.int64), views,
+                           np.ascontiguousarray(g.P), np.ascontiguousarray(g.Pinv), np.ascontiguousarray(g.C),
+                           np.array([2, 2, 1], np.int32), x, out, 2)
+    assert rc == 0
+    assert lib.plane_forward_mixed_waves() > 0
+    np.testing.assert_array_equal(out, ref)
 
 
 def _to_bf16(x):
@@ -108,8 +130,9 @@ def _to_bf16(x):
     return u.astype(np.uint32).view(np.float32)
 
 
+@pytest.mark.parametrize("waves", [False, True], ids=["per_ray", "wavefront"])
 @pytest.mark.parametrize("case", FWD_CASES, ids=lambda c: f"{c[0]}d-{'x'.join(map(str, c[1]))}")
-def test_plane_forward_bf16_views_match_oracle_bitwise(lib, case):
+def test_plane_forward_bf16_views_match_oracle_bitwise(lib, case, waves):
     """bfloat16 views (half-width offsets, two-element loads) give the oracle's forward of the
     bfloat16-rounded volume bit for bit, with every read inside its view."""
     dim, vol, views, det, s2c, c2d, fans = case
@@ -123,7 +146,7 @@ def test_plane_forward_bf16_views_match_oracle_bitwise(lib, case):
         rc = lib.plane_forward(dim, np.array(list(vol) + [1] * (3 - dim), np.int64),
                                np.array(list(det) + [1] * (2 - len(det)), np.int64), views,
                                np.ascontiguousarray(g.P), np.ascontiguousarray(g.Pinv), np.ascontiguousarray(g.C),
-                               np.array(list(fan) + [0] * (3 - len(fan)), np.int32), x, out, 1)
+                               np.array(list(fan) + [0] * (3 - len(fan)), np.int32), x, out, 3 if waves else 1)
         assert rc == 0, (fan, rc)
         np.testing.assert_array_equal(out, ref, err_msg=f"fan axes {fan}")
 
