@@ -18,6 +18,16 @@ namespace ehp
 #endif
     }
 
+    // a * b for |a|, |b| < 2^23 (one 24-bit multiply on the GPU)
+    EHP_HD inline int smul24(int a, int b)
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        return __mul24(a, b);
+#else
+        return a * b;
+#endif
+    }
+
     // Tile of the LDS-privatised Joseph kernels: LA planes along the driving axis, TB x TC
     // voxels across (TC unused in 2D).
     struct TileCfg {
@@ -67,6 +77,7 @@ namespace ehp
         int A0, A1, B0, B1, C0, C1;
         int PB, PC;
         int nslots;
+        int B0m1, C0m1; // B0 - 1 and C0 - 1 (the first slot's voxel along b1 and b2)
     };
 
     // The transverse axes b1 < b2 of driving axis A (compile-time, so that per-axis arrays are
@@ -91,6 +102,8 @@ namespace ehp
         t.PB = tc.TB + 2;
         t.PC = (D == 3) ? tc.TC + 2 : 1;
         t.nslots = tc.LA * t.PC * t.PB;
+        t.B0m1 = t.B0 - 1;
+        t.C0m1 = t.C0 - 1;
         return t;
     }
 
@@ -177,54 +190,13 @@ namespace ehp
         return box_bbox<D>(g, lo, hi, dims, u0, u1, v0, v1);
     }
 
-    // Narrow [slo, shi] to the samples whose position along one transverse axis may lie in
-    // [L, H), with one sample of slack on either side (the exact test follows per sample).
-    EHP_HD inline void clip_range(float p0, float st, float L, float H, int& slo, int& shi)
-    {
-        if (fabsf(st) < 1.0e-12f) {
-            if (p0 < L - 1.0f || p0 >= H + 1.0f)
-                shi = slo - 1;
-            return;
-        }
-        // approximate reciprocal: the range keeps one sample of slack on either side
-        const float rst = ehp_rcp(st);
-        float lo = (L - p0) * rst, hi = (H - p0) * rst;
-        if (st < 0.0f) {
-            const float t = lo;
-            lo = hi;
-            hi = t;
-        }
-        lo = lo < -2.0f ? -2.0f : (lo > 2.0e9f ? 2.0e9f : lo);
-        hi = hi < -2.0f ? -2.0f : (hi > 2.0e9f ? 2.0e9f : hi);
-        const int l = (int) ceilf(lo) - 1;
-        const int h = (int) floorf(hi) + 1;
-        slo = slo > l ? slo : l;
-        shi = shi < h ? shi : h;
-    }
-
-    // Samples [slo, shi] of a ray (driving axis A) that can touch the tile's voxels.
-    EHP_HD inline void tile_sample_range(const RayRec& r, const TileBox& t, int D, int& slo, int& shi)
-    {
-        if (!(r.meta & 4)) {
-            slo = t.A0 - r.P0;
-            shi = t.A1 - 1 - r.P0;
-        } else {
-            slo = r.P0 - (t.A1 - 1);
-            shi = r.P0 - t.A0;
-        }
-        slo = slo > 0 ? slo : 0;
-        shi = shi < r.nsteps - 1 ? shi : r.nsteps - 1;
-        clip_range(r.p1, r.s1, (float) t.B0 - 0.5f, (float) t.B1 + 0.5f, slo, shi);
-        if (D == 3)
-            clip_range(r.p2, r.s2, (float) t.C0 - 0.5f, (float) t.C1 + 0.5f, slo, shi);
-    }
-
     // Back projection of sample s of one ray (driving axis A, transverse start p1, p2 and
     // steps s1, s2) lying on tile plane pa: if its bilinear footprint touches the tile, hand
     // ys * weight for each of its four (2D: two) slots to add(slot, value), ys being the
     // ray's y * len in the ray's own fixed-point scale (ray_scale). Weights are elsa's
     // bilinear weights (doInterpolation, JosephsMethod.cpp:60-97), the driving weight being 1;
-    // a 3D tap's weight is applied as (ys * w2) * w1.
+    // a 3D tap's weight is applied as (ys * w2) * w1. The footprint's first slot is
+    // (pa * PC + i2) * PB + i1 with i1 = f1 - (B0 - 1), i2 = f2 - (C0 - 1).
     template <int D, class Add>
     EHP_HD inline void tile_sample_adjoint(float p1, float p2, float s1, float s2, int s, int pa, const TileBox& t,
                                            float ys, Add add)
@@ -232,26 +204,25 @@ namespace ehp
         const float fs = (float) s;
         const float q1 = __builtin_fmaf(fs, s1, p1);
         const float g1 = floorf(q1 - 0.5f);
-        const int f1 = (int) g1;
-        if (f1 < t.B0 - 1 || f1 > t.B1 - 1)
+        const uint32_t i1 = (uint32_t) ((int) g1 - t.B0m1);
+        if (i1 > (uint32_t) (t.B1 - t.B0)) // f1 outside [B0 - 1, B1 - 1]
             return;
         const float c1 = (q1 - g1) - 0.5f;
         const float w1 = 1.0f - c1;
         if (D == 2) {
-            const int base = (int) mul24((uint32_t) pa, (uint32_t) t.PB) + (f1 - (t.B0 - 1));
+            const int base = (int) (mul24((uint32_t) pa, (uint32_t) t.PB) + i1);
             add(base, ys * w1);
             add(base + 1, ys * c1);
         } else {
             const float q2 = __builtin_fmaf(fs, s2, p2);
             const float g2 = floorf(q2 - 0.5f);
-            const int f2 = (int) g2;
-            if (f2 < t.C0 - 1 || f2 > t.C1 - 1)
+            const uint32_t i2 = (uint32_t) ((int) g2 - t.C0m1);
+            if (i2 > (uint32_t) (t.C1 - t.C0)) // f2 outside [C0 - 1, C1 - 1]
                 return;
             const float c2 = (q2 - g2) - 0.5f;
             const float w2 = 1.0f - c2;
-            const int base = (int) (mul24((uint32_t) pa, (uint32_t) (t.PC * t.PB)) +
-                                    mul24((uint32_t) (f2 - (t.C0 - 1)), (uint32_t) t.PB)) +
-                             (f1 - (t.B0 - 1));
+            const uint32_t in_plane = mul24(i2, (uint32_t) t.PB) + i1;
+            const int base = (int) (in_plane + mul24((uint32_t) pa, (uint32_t) (t.PC * t.PB)));
             const float yw = ys * w2, yc = ys * c2;
             add(base, yw * w1);
             add(base + 1, yw * c1);
@@ -260,18 +231,23 @@ namespace ehp
         }
     }
 
-    // Back projection of one ray (driving axis A) into a tile: tile_sample_adjoint for each
-    // of its samples that can touch the tile.
+    // Back projection of one ray (driving axis A) into a tile: for each plane of the tile on
+    // which the ray has a sample (s in [0, nsteps)), tile_sample_adjoint, which keeps the
+    // samples whose footprint touches the tile's slots. Every ray visits the tile's planes in
+    // the same order, so that the lanes of a wavefront address the same LDS plane at each
+    // step; the slots hold integer sums, which the order of the additions does not change.
     template <int D, int A, class Add>
     EHP_HD inline void tile_ray_adjoint(const RayRec& r, const TileBox& t, float ys, Add add)
     {
         if ((r.meta & 3) != A || r.nsteps == 0)
             return;
-        int slo, shi;
-        tile_sample_range(r, t, D, slo, shi);
-        const int sg = (r.meta & 4) ? -1 : 1;
-        for (int s = slo; s <= shi; ++s)
-            tile_sample_adjoint<D>(r.p1, r.p2, r.s1, r.s2, s, r.P0 + sg * s - t.A0, t, ys, add);
+        const bool neg = (r.meta & 4) != 0;
+        const int sg = neg ? -1 : 1;
+        int s = neg ? r.P0 - t.A0 : t.A0 - r.P0; // the ray's sample on the tile's first plane
+        const int la = t.A1 - t.A0;
+        for (int pa = 0; pa < la; ++pa, s += sg)
+            if ((uint32_t) s < (uint32_t) r.nsteps)
+                tile_sample_adjoint<D>(r.p1, r.p2, r.s1, r.s2, s, pa, t, ys, add);
     }
 
     // Linear volume index of LDS slot i if the tile owns that voxel, else -1.
@@ -532,16 +508,6 @@ namespace ehp
         const float c = g < -2.0f ? -2.0f : (g > (float) n ? (float) n : g);
 #endif
         return (int) c;
-    }
-
-    // a * b for |a|, |b| < 2^23 (one 24-bit multiply on the GPU)
-    EHP_HD inline int smul24(int a, int b)
-    {
-#if defined(__HIP_DEVICE_COMPILE__)
-        return __mul24(a, b);
-#else
-        return a * b;
-#endif
     }
 
     // Samples [s0, s1) of a ray whose planes lie inside the volume of na planes along its
