@@ -249,21 +249,21 @@ static float span_key(int key, const RayRec& r, const Dims& dims, const PadView*
     }
 }
 
-// The GPU forward's grouping: the 64 pixels u0 .. u0 + 63 of row v form a wavefront; its rays
-// fall into groups by driving axis and direction, each with one plane loop over the group's
-// span (plane_ray_span, plane_span_forward). Pixels past the detector edge join no group.
+// The GPU forward's grouping: the pixels (u0 + l % wu, v0 + l / wu), l < 64, form a wavefront;
+// its rays fall into groups by driving axis and direction, each with one plane loop over the
+// group's span (plane_ray_span, plane_span_forward). Pixels past the detector edge join no group.
 static int64_t g_mixed_waves = 0; // wavefronts of the last plane_forward call with more than one group
 
 template <int D>
-static void wave_forward(const Dims& dims, const PoseGeom& g, int v, int u0, const PadView* pv, const bool* fb2,
-                         bool b16, float* out)
+static void wave_forward(const Dims& dims, const PoseGeom& g, int u0, int v0, int wu, const PadView* pv,
+                         const bool* fb2, bool b16, float* out)
 {
     RayRec rr[64];
     int key[64], tf[64], tl[64], s0[64];
     for (int l = 0; l < 64; ++l) {
         key[l] = -1;
-        const int u = u0 + l;
-        if (u >= dims.det[0])
+        const int u = u0 + l % wu, v = v0 + l / wu;
+        if (u >= dims.det[0] || v >= dims.det[1])
             continue;
         float o[3], d[3];
         JRay jr;
@@ -302,7 +302,7 @@ extern "C" int64_t plane_forward_mixed_waves()
 
 // flags: bit 0 bfloat16 views (the volume rounded to bfloat16), read by the bfloat16 variant of
 // the kernel's ray loop; bit 1 the GPU's wavefront grouping (wave_forward) instead of one loop
-// per ray
+// per ray, with wavefronts of wu = flags >> 8 pixels along u (3D; 0: 64) and 64 / wu along v
 extern "C" int plane_forward(int dim, const int64_t* vol_dims, const int64_t* det_dims, int nposes, const float* P,
                              const float* Pinv, const float* C, const int* fan, const float* vol, float* sino,
                              int flags)
@@ -353,16 +353,23 @@ extern "C" int plane_forward(int dim, const int64_t* vol_dims, const int64_t* de
     }
     if (waves) {
         g_mixed_waves = 0;
+        const int wu = dim == 2 || (flags >> 8) == 0 ? 64 : flags >> 8;
+        if (wu < 1 || 64 % wu != 0)
+            return 23;
+        const int wv = 64 / wu;
         float lane[64];
         for (int p = 0; p < nposes; ++p)
-            for (int v = 0; v < dims.det[1]; ++v)
-                for (int u0 = 0; u0 < dims.det[0]; u0 += 64) {
+            for (int v0 = 0; v0 < dims.det[1]; v0 += wv)
+                for (int u0 = 0; u0 < dims.det[0]; u0 += wu) {
                     if (dim == 2)
-                        wave_forward<2>(dims, geo[p], v, u0, pv, fb2, b16, lane);
+                        wave_forward<2>(dims, geo[p], u0, v0, wu, pv, fb2, b16, lane);
                     else
-                        wave_forward<3>(dims, geo[p], v, u0, pv, fb2, b16, lane);
-                    for (int l = 0; l < 64 && u0 + l < dims.det[0]; ++l)
-                        sino[(int64_t) p * dims.pix_per_pose + (int64_t) v * dims.det[0] + u0 + l] = lane[l];
+                        wave_forward<3>(dims, geo[p], u0, v0, wu, pv, fb2, b16, lane);
+                    for (int l = 0; l < 64; ++l) {
+                        const int u = u0 + l % wu, v = v0 + l / wu;
+                        if (u < dims.det[0] && v < dims.det[1])
+                            sino[(int64_t) p * dims.pix_per_pose + (int64_t) v * dims.det[0] + u] = lane[l];
+                    }
                 }
         return g_view_oob ? 22 : 0;
     }

@@ -81,7 +81,13 @@ FWD_CASES = [
 ]
 
 
-@pytest.mark.parametrize("waves", [False, True], ids=["per_ray", "wavefront"])
+# how the forward's rays are grouped: one plane loop per ray, or wavefronts of wu x (64 / wu)
+# pixels whose rays of one driving axis and direction share one plane loop (the GPU's grouping)
+WAVES = [0, 2 | (64 << 8), 2 | (16 << 8), 2 | (8 << 8)]
+WAVE_IDS = ["per_ray", "wave64x1", "wave16x4", "wave8x8"]
+
+
+@pytest.mark.parametrize("waves", WAVES, ids=WAVE_IDS)
 @pytest.mark.parametrize("case", FWD_CASES, ids=lambda c: f"{c[0]}d-{'x'.join(map(str, c[1]))}")
 def test_plane_forward_logic_matches_oracle_bitwise(lib, case, waves):
     """Every zero-padded plane-major view a driving axis can get gives the oracle's forward, bit
@@ -99,7 +105,7 @@ def test_plane_forward_logic_matches_oracle_bitwise(lib, case, waves):
         rc = lib.plane_forward(dim, np.array(list(vol) + [1] * (3 - dim), np.int64),
                                np.array(list(det) + [1] * (2 - len(det)), np.int64), views,
                                np.ascontiguousarray(g.P), np.ascontiguousarray(g.Pinv), np.ascontiguousarray(g.C),
-                               np.array(list(fan) + [0] * (3 - len(fan)), np.int32), x, out, 2 if waves else 0)
+                               np.array(list(fan) + [0] * (3 - len(fan)), np.int32), x, out, waves)
         assert rc == 0, (fan, rc)
         np.testing.assert_array_equal(out, ref, err_msg=f"fan axes {fan}")
 
@@ -117,7 +123,7 @@ def test_plane_forward_wavefront_groups_cover_mixed_waves(lib):
     out = np.full(g.sino_shape, np.nan, np.float32)
     rc = lib.plane_forward(dim, np.array(vol, np.int64), np.array(det, np.int64), views,
                            np.ascontiguousarray(g.P), np.ascontiguousarray(g.Pinv), np.ascontiguousarray(g.C),
-                           np.array([2, 2, 1], np.int32), x, out, 2)
+                           np.array([2, 2, 1], np.int32), x, out, 2 | (16 << 8))
     assert rc == 0
     assert lib.plane_forward_mixed_waves() > 0
     np.testing.assert_array_equal(out, ref)
@@ -130,7 +136,7 @@ def _to_bf16(x):
     return u.astype(np.uint32).view(np.float32)
 
 
-@pytest.mark.parametrize("waves", [False, True], ids=["per_ray", "wavefront"])
+@pytest.mark.parametrize("waves", WAVES, ids=WAVE_IDS)
 @pytest.mark.parametrize("case", FWD_CASES, ids=lambda c: f"{c[0]}d-{'x'.join(map(str, c[1]))}")
 def test_plane_forward_bf16_views_match_oracle_bitwise(lib, case, waves):
     """bfloat16 views (half-width offsets, two-element loads) give the oracle's forward of the
@@ -146,7 +152,7 @@ def test_plane_forward_bf16_views_match_oracle_bitwise(lib, case, waves):
         rc = lib.plane_forward(dim, np.array(list(vol) + [1] * (3 - dim), np.int64),
                                np.array(list(det) + [1] * (2 - len(det)), np.int64), views,
                                np.ascontiguousarray(g.P), np.ascontiguousarray(g.Pinv), np.ascontiguousarray(g.C),
-                               np.array(list(fan) + [0] * (3 - len(fan)), np.int32), x, out, 3 if waves else 1)
+                               np.array(list(fan) + [0] * (3 - len(fan)), np.int32), x, out, waves | 1)
         assert rc == 0, (fan, rc)
         np.testing.assert_array_equal(out, ref, err_msg=f"fan axes {fan}")
 
