@@ -77,7 +77,7 @@ namespace ehp
         int A0, A1, B0, B1, C0, C1;
         int PB, PC;
         int nslots;
-        int B0m1, C0m1; // B0 - 1 and C0 - 1 (the first slot's voxel along b1 and b2)
+        float B0m1f, C0m1f; // B0 - 1 and C0 - 1 (the first slot's voxel along b1 and b2)
     };
 
     // The transverse axes b1 < b2 of driving axis A (compile-time, so that per-axis arrays are
@@ -102,8 +102,8 @@ namespace ehp
         t.PB = tc.TB + 2;
         t.PC = (D == 3) ? tc.TC + 2 : 1;
         t.nslots = tc.LA * t.PC * t.PB;
-        t.B0m1 = t.B0 - 1;
-        t.C0m1 = t.C0 - 1;
+        t.B0m1f = (float) (t.B0 - 1);
+        t.C0m1f = (float) (t.C0 - 1);
         return t;
     }
 
@@ -192,11 +192,12 @@ namespace ehp
 
     // Back projection of sample s of one ray (driving axis A, transverse start p1, p2 and
     // steps s1, s2) lying on tile plane pa: if its bilinear footprint touches the tile, hand
-    // ys * weight for each of its four (2D: two) slots to add(slot, value), ys being the
-    // ray's y * len in the ray's own fixed-point scale (ray_scale). Weights are elsa's
-    // bilinear weights (doInterpolation, JosephsMethod.cpp:60-97), the driving weight being 1;
-    // a 3D tap's weight is applied as (ys * w2) * w1. The footprint's first slot is
-    // (pa * PC + i2) * PB + i1 with i1 = f1 - (B0 - 1), i2 = f2 - (C0 - 1).
+    // ys * weight for each of its four (2D: two) slots to add(offset, value), offset being the
+    // slot's byte offset in the tile's array of 8-byte slots and ys the ray's y * len in the
+    // ray's own fixed-point scale (ray_scale). Weights are elsa's bilinear weights
+    // (doInterpolation, JosephsMethod.cpp:60-97), the driving weight being 1; a 3D tap's
+    // weight is applied as (ys * w2) * w1. The footprint's first slot is (pa * PC + i2) * PB + i1
+    // with i1 = f1 - (B0 - 1), i2 = f2 - (C0 - 1).
     template <int D, class Add>
     EHP_HD inline void tile_sample_adjoint(float p1, float p2, float s1, float s2, int s, int pa, const TileBox& t,
                                            float ys, Add add)
@@ -204,30 +205,31 @@ namespace ehp
         const float fs = (float) s;
         const float q1 = __builtin_fmaf(fs, s1, p1);
         const float g1 = floorf(q1 - 0.5f);
-        const uint32_t i1 = (uint32_t) ((int) g1 - t.B0m1);
+        // f1 - (B0 - 1), exact in float (both integers); negative or beyond the tile: rejected
+        const uint32_t i1 = (uint32_t) (int) (g1 - t.B0m1f);
         if (i1 > (uint32_t) (t.B1 - t.B0)) // f1 outside [B0 - 1, B1 - 1]
             return;
         const float c1 = (q1 - g1) - 0.5f;
         const float w1 = 1.0f - c1;
+        const uint32_t pb8 = (uint32_t) t.PB * 8u;
         if (D == 2) {
-            const int base = (int) (mul24((uint32_t) pa, (uint32_t) t.PB) + i1);
-            add(base, ys * w1);
-            add(base + 1, ys * c1);
+            const uint32_t off = (i1 << 3) + mul24((uint32_t) pa, pb8);
+            add(off, ys * w1);
+            add(off + 8u, ys * c1);
         } else {
             const float q2 = __builtin_fmaf(fs, s2, p2);
             const float g2 = floorf(q2 - 0.5f);
-            const uint32_t i2 = (uint32_t) ((int) g2 - t.C0m1);
+            const uint32_t i2 = (uint32_t) (int) (g2 - t.C0m1f);
             if (i2 > (uint32_t) (t.C1 - t.C0)) // f2 outside [C0 - 1, C1 - 1]
                 return;
             const float c2 = (q2 - g2) - 0.5f;
             const float w2 = 1.0f - c2;
-            const uint32_t in_plane = mul24(i2, (uint32_t) t.PB) + i1;
-            const int base = (int) (in_plane + mul24((uint32_t) pa, (uint32_t) (t.PC * t.PB)));
+            const uint32_t off = mul24(i2, pb8) + mul24((uint32_t) pa, (uint32_t) t.PC * pb8) + (i1 << 3);
             const float yw = ys * w2, yc = ys * c2;
-            add(base, yw * w1);
-            add(base + 1, yw * c1);
-            add(base + t.PB, yc * w1);
-            add(base + t.PB + 1, yc * c1);
+            add(off, yw * w1);
+            add(off + 8u, yw * c1);
+            add(off + pb8, yc * w1);
+            add(off + pb8 + 8u, yc * c1);
         }
     }
 

@@ -149,10 +149,7 @@ namespace ehp
         __shared__ int s_cnt[kMaxPoseBatch], s_pref[kMaxPoseBatch + 1];
 
         const int tid = threadIdx.x;
-        TileBox t = make_tile<D, A>(dims, tc, tile % ntA, (tile / ntA) % ntB, tile / (ntA * ntB));
-        // held as opaque scalars, so that f - (B0 - 1) stays one subtraction per sample
-        t.B0m1 = __builtin_amdgcn_readfirstlane(t.B0m1);
-        t.C0m1 = __builtin_amdgcn_readfirstlane(t.C0m1);
+        const TileBox t = make_tile<D, A>(dims, tc, tile % ntA, (tile / ntA) % ntB, tile / (ntA * ntB));
         long long* iacc = lds;
         for (int i = tid; i < t.nslots; i += kThreads)
             iacc[i] = 0;
@@ -216,8 +213,9 @@ namespace ehp
                 float ys;
                 int sh;
                 ray_scale(yv * r.len, eg, ys, sh);
-                tile_ray_adjoint<D, A>(r, t, ys, [&](int slot, float val) {
-                    atomicAdd(reinterpret_cast<unsigned long long*>(iacc + slot), fixed_contrib(val, sh));
+                tile_ray_adjoint<D, A>(r, t, ys, [&](uint32_t off, float val) {
+                    atomicAdd(reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(iacc) + off),
+                              fixed_contrib(val, sh));
                 });
             }
             __syncthreads();

@@ -79,8 +79,9 @@ namespace
                             ray_scale(yv * table[ray].len, eg, ys, sh);
                             if (sh < 0 || sh > 32)
                                 bad = true;
-                            tile_ray_adjoint<D, A>(table[ray], t, ys, [&](int slot, float val) {
-                                if (slot < 0 || slot >= t.nslots) {
+                            tile_ray_adjoint<D, A>(table[ray], t, ys, [&](uint32_t off, float val) {
+                                const int64_t slot = off / 8;
+                                if (off % 8 != 0 || slot >= t.nslots) {
                                     bad = true;
                                     return;
                                 }
