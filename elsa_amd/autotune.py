@@ -13,7 +13,7 @@ import numpy as np
 from .projectors import BlobVoxelProjectorCUDA, BSplineVoxelProjectorCUDA, JosephsMethodCUDA, SiddonsMethodCUDA
 
 CANDIDATES = {
-    ("joseph", 3): [(8, 15, 15), (4, 15, 15), (12, 15, 15), (16, 15, 15), (8, 23, 23), (8, 31, 15), (8, 11, 11)],
+    ("joseph", 3): [(8, 17, 17), (8, 15, 15), (8, 13, 13), (6, 19, 19), (4, 23, 23), (8, 19, 19), (12, 15, 15)],
     ("joseph", 2): [(4, 15, 1), (8, 15, 1), (2, 15, 1), (8, 31, 1), (4, 7, 1), (16, 31, 1), (16, 15, 1)],
     ("siddon", 3): [(8, 16, 16), (4, 16, 16), (16, 16, 16), (8, 32, 16), (8, 8, 8), (12, 16, 16)],
     ("siddon", 2): [(4, 16, 1), (8, 16, 1), (2, 16, 1), (8, 32, 1), (4, 8, 1), (16, 16, 1)],

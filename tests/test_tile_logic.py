@@ -38,6 +38,7 @@ CASES = [
     (3, (32, 32, 32), 24, (48, 48), 3200.0, 160.0, (32, 15, 15)),
     (3, (17, 9, 23), 7, (30, 20), 25.0, 25.0, (8, 5, 3)),
     (3, (64, 64, 64), 9, (96, 96), 6400.0, 320.0, (16, 7, 9)),
+    (3, (64, 64, 64), 9, (96, 96), 6400.0, 320.0, (8, 17, 17)),
     (2, (31, 17), 11, (80,), 40.0, 30.0, (8, 5, 1)),
     (2, (64, 64), 30, (90,), 6400.0, 320.0, (64, 125, 1)),
 ]
