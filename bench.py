@@ -52,8 +52,9 @@ WORKLOADS = {
                       text="2D fan-beam Joseph fwd+adj: 256^2 f32 Shepp-Logan, 180 poses over 360 deg, "
                            "384 detector, s2c 25600, c2d 1280"),
     "mixed512": dict(dim=3, vol=(512, 512, 512), views=720, det=(768, 768), s2c=51200.0, c2d=2560.0, sub=64,
-                     text="mixed precision: Joseph fwd+adj at 512^3 f32 / bf16, 720 poses, 768x768 detector, s2c 51200, "
-                          "c2d 2560; CGLS FP32 vs bf16 with iterative refinement on a 64^3 subproblem"),
+                     text="mixed precision: Joseph and blob voxel (r 2, alpha 10.83, m 2) fwd+adj at 512^3 f32 / bf16, "
+                          "720 poses, 768x768 detector, s2c 51200, c2d 2560; CGLS FP32 vs bf16 with iterative "
+                          "refinement on a 64^3 subproblem"),
     "autograd128": dict(dim=3, vol=(128, 128, 128), views=180, det=(192, 192), s2c=12800.0, c2d=640.0, batch=4,
                         text="differentiable Joseph: 4 x 128^3 f32 volumes of 20 random ellipsoids, 180 poses, "
                              "192x192 detector; one step = loss 0.5||Ax-b||^2 + 0.01 TV(x), backward (hand-written "
@@ -304,9 +305,10 @@ def _line(a, wl, value, ms, dtype, data, config, **extra):
 
 
 def mixed_main(a, E, wl):
-    """Joseph at 512^3 in FP32 and bfloat16 storage, then CGLS in FP32 against bfloat16 CGLS with
-    iterative refinement on a 64^3 subproblem, both measured against an FP64 GPU CGLS of 400
-    iterations (the reference iterate) at equal iteration budgets."""
+    """Joseph and the blob voxel projector at 512^3 in FP32 and bfloat16 storage, then CGLS in
+    FP32 against bfloat16 CGLS with iterative refinement on a 64^3 subproblem, both measured
+    against an FP64 GPU CGLS of 400 iterations (the reference iterate) at equal iteration
+    budgets."""
     import torch
 
     from elsa_amd.phantoms import modified_shepp_logan
@@ -338,6 +340,23 @@ def mixed_main(a, E, wl):
         tf = timed(lambda: op.apply(x, sino), a.steps)
         ta = timed(lambda: op.applyAdjoint(sino, back), a.steps)
         passes[name] = {"fwd_ms": round(tf, 2), "adj_ms": round(ta, 2), "Gsamples_per_s": round(2 * S / (tf + ta) / 1e6, 1)}
+        print(f"joseph {name}: {passes[name]}", file=sys.stderr, flush=True)
+        del op, x, sino, back
+        torch.cuda.empty_cache()
+    # the blob voxel projector (Kaiser-Bessel, radius 2, alpha 10.83, order 2) on the same problem
+    blob = {}
+    for name, dt in (("f32", np.float32), ("bf16", "bfloat16")):
+        op = E.BlobVoxelProjectorCUDA(vd, dd, radius=2.0, alpha=10.83, order=2, dtype=dt)
+        taps = op.sample_count()
+        tdt = torch.float32 if name == "f32" else torch.bfloat16
+        x = torch.from_numpy(modified_shepp_logan((n, n, n))).cuda().to(tdt)
+        sino = torch.empty(dd.shape, dtype=tdt, device="cuda")
+        back = torch.empty(vd.shape, dtype=tdt, device="cuda")
+        tf = timed(lambda: op.apply(x, sino), 1)
+        ta = timed(lambda: op.applyAdjoint(sino, back), 1)
+        blob[name] = {"fwd_ms": round(tf, 1), "adj_ms": round(ta, 1), "taps_per_pass": taps,
+                      "Gtaps_per_s": round(2 * taps / (tf + ta) / 1e6, 1)}
+        print(f"blob {name}: {blob[name]}", file=sys.stderr, flush=True)
         del op, x, sino, back
         torch.cuda.empty_cache()
     m = wl["sub"]
@@ -369,7 +388,7 @@ def mixed_main(a, E, wl):
     _line(a, wl, f32["Gsamples_per_s"], round(f32["fwd_ms"] + f32["adj_ms"], 2), "f32 and bf16 (storage)",
           "synthetic (modified Shepp-Logan phantom)",
           {"volume": list(wl["vol"]), "poses": wl["views"], "detector": list(wl["det"]), "samples_per_pass": S,
-           "passes": passes, "subproblem": {"volume": [m] * 3, "poses": wl["views"] // 6,
+           "passes": passes, "blob_passes": blob, "subproblem": {"volume": [m] * 3, "poses": wl["views"] // 6,
                                             "reference": "FP64 CGLS, 400 iterations, on the GPU"}},
           solve=cg)
 

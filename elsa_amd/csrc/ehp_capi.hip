@@ -317,13 +317,15 @@ namespace
         for (int a = 0; planes && a < dm.dim; ++a) {
             if (!(op->axes_present & (1 << a)))
                 continue;
-            const size_t vbytes = (size_t) op->layouts[a].size * (b16 ? sizeof(ehp::bf16) : sizeof(float));
+            const size_t vbytes =
+                (size_t) op->layouts[a].size * (b16 ? sizeof(ehp::bf16) : sizeof(float)) + (b16 ? ehp::kViewSlack : 0);
             e = hipMalloc((void**) &op->d_view[a], vbytes);
             if (e != hipSuccess)
                 return fail(EHP_ERR_ALLOC, "hipMalloc of a %zu-byte volume view failed: %s", vbytes,
                             hipGetErrorString(e));
             if (b16)
-                e = ehp::launch_fill<ehp::bf16>((ehp::bf16*) op->d_view[a], op->layouts[a].size, ehp::bf16{0}, nullptr);
+                e = ehp::launch_fill<ehp::bf16>((ehp::bf16*) op->d_view[a], op->layouts[a].size + ehp::kViewSlack / 2,
+                                                ehp::bf16{0}, nullptr);
             else
                 e = ehp::launch_fill<float>((float*) op->d_view[a], op->layouts[a].size, 0.0f, nullptr);
             if (e != hipSuccess)
@@ -506,8 +508,8 @@ int ehp_voxel_create(const ehp_geometry* g, int dtype, const void* lut, int n_lu
 {
     if (flags & ~(EHP_SIMPLE_KERNELS | EHP_DIFFERENTIAL))
         return fail(EHP_ERR_ARGUMENT, "unknown voxel projector flags 0x%x", flags);
-    if (dtype == EHP_BF16)
-        return fail(EHP_ERR_UNSUPPORTED, "bfloat16 storage is implemented for Joseph only");
+    if (dtype == EHP_BF16 && (flags & EHP_SIMPLE_KERNELS))
+        return fail(EHP_ERR_UNSUPPORTED, "bfloat16 voxel projectors run the default kernels only");
     if (g && !g->pix.empty())
         return fail(EHP_ERR_UNSUPPORTED, "the voxel projectors take planar detectors only (as elsa's)");
     if (!lut || n_lut < 2)
@@ -521,7 +523,8 @@ int ehp_voxel_create(const ehp_geometry* g, int dtype, const void* lut, int n_lu
     op->n_lut = n_lut;
     op->support = (float) support;
     op->differential = (flags & EHP_DIFFERENTIAL) ? 1 : 0;
-    const size_t bytes = (size_t) n_lut * dtype_size(dtype);
+    // bfloat16 operators compute in FP32 and take an FP32 table
+    const size_t bytes = (size_t) n_lut * (dtype == EHP_BF16 ? sizeof(float) : dtype_size(dtype));
     hipError_t e = hipMalloc(&op->d_lut, bytes);
     if (e != hipSuccess) {
         ehp_destroy(op);
@@ -535,8 +538,12 @@ int ehp_voxel_create(const ehp_geometry* g, int dtype, const void* lut, int n_lu
         return hip_fail(e, "basis-function table upload");
     }
     rc = count_samples(op);
-    if (rc == EHP_OK && dtype == EHP_F32 && !(flags & EHP_SIMPLE_KERNELS))
+    if (rc == EHP_OK && (dtype == EHP_F32 || dtype == EHP_BF16) && !(flags & EHP_SIMPLE_KERNELS))
         rc = setup_voxel_tiled(op, (const float*) lut);
+    if (rc == EHP_OK && dtype == EHP_BF16 && !op->tiled)
+        rc = fail(EHP_ERR_UNSUPPORTED, "bfloat16 storage needs the tiled voxel forward, which this geometry or "
+                                       "table does not allow (source inside the volume, or a table over %d entries)",
+                  ehp::voxel_max_lut());
     if (rc != EHP_OK) {
         ehp_destroy(op);
         *out = nullptr;
@@ -598,7 +605,18 @@ static int apply(ehp_op* op, const void* in, void* out, void* stream, bool adjoi
         return fail(EHP_ERR_ARGUMENT, "operator lives on device %d, current device is %d", op->device, cur);
     hipStream_t s = (hipStream_t) stream;
     const ehp::Dims& dm = op->dims;
-    if (op->dtype == EHP_BF16) {
+    if (op->dtype == EHP_BF16 && op->kind == KIND_VOXEL) {
+        // bfloat16 storage, FP32 arithmetic: the tiled forward rounds each pixel's fixed-point
+        // sum once, the gather each voxel's FP32 sum
+        if (!adjoint)
+            e = ehp::voxel_forward_tiled<ehp::bf16>(op->d_geo, dm, (const float*) op->d_lut, op->n_lut, op->support,
+                                                    op->differential, op->d_pose_axes, op->d_pose_margin, op->tile,
+                                                    op->lutscale, op->slot_bound, op->d_maxbits, op->d_fixexp,
+                                                    (const ehp::bf16*) in, (ehp::bf16*) out, s);
+        else
+            e = ehp::voxel_adjoint<float, ehp::bf16>(op->d_geo, dm, (const float*) op->d_lut, op->n_lut, op->support,
+                                                     op->differential, (const ehp::bf16*) in, (ehp::bf16*) out, s);
+    } else if (op->dtype == EHP_BF16) {
         // bfloat16 storage, FP32 arithmetic: the forward reads bfloat16 views and rounds each ray
         // sum once; the back projection sums in fixed point into FP32 and rounds each voxel once
         if (!adjoint) {
@@ -627,7 +645,7 @@ static int apply(ehp_op* op, const void* in, void* out, void* stream, bool adjoi
         else if (op->kind == KIND_SIDDON)
             e = adjoint ? ehp::siddon_adjoint<float>(op->d_geo, dm, x, y, s) : ehp::siddon_forward<float>(op->d_geo, dm, x, y, s);
         else if (!adjoint && op->tiled)
-            e = ehp::voxel_forward_tiled(op->d_geo, dm, (const float*) op->d_lut, op->n_lut, op->support,
+            e = ehp::voxel_forward_tiled<float>(op->d_geo, dm, (const float*) op->d_lut, op->n_lut, op->support,
                                          op->differential, op->d_pose_axes, op->d_pose_margin, op->tile, op->lutscale,
                                          op->slot_bound, op->d_maxbits, op->d_fixexp, x, y, s);
         else

@@ -418,9 +418,9 @@ namespace ehp
     struct PadLayout {
         int A, m, f;
         int mpad;      // 2 in 3D, 0 in 2D
-        uint32_t RS;   // row stride (n_f + 4)
+        uint32_t RS;   // row stride (n_f + 4, rounded up to even)
         uint32_t PS;   // plane stride
-        int64_t size;  // floats in the view
+        int64_t size;  // elements in the view
     };
 
     inline PadLayout pad_layout(const Dims& dims, int A, int f)
@@ -430,7 +430,9 @@ namespace ehp
         L.f = f;
         L.m = 3 - A - f; // in 2D the third axis has one voxel
         L.mpad = dims.dim == 3 ? 2 : 0;
-        L.RS = (uint32_t) (dims.vol[f] + 4);
+        // rows of an even number of elements, so that with bfloat16 elements every row and plane
+        // starts on a 4-byte boundary
+        L.RS = (uint32_t) ((dims.vol[f] + 5) & ~1);
         L.PS = L.RS * (uint32_t) (dims.vol[L.m] + 2 * L.mpad);
         L.size = (int64_t) L.PS * dims.vol[A];
         return L;
@@ -445,6 +447,9 @@ namespace ehp
         const int cf = L.f == 0 ? c[0] : (L.f == 1 ? c[1] : c[2]);
         return (int64_t) ca * L.PS + (int64_t) (cm + L.mpad) * L.RS + (cf + 2);
     }
+
+    // Bytes allocated past the last element of a bfloat16 view (ld_view2_bf16 reads up to 4).
+    constexpr int kViewSlack = 8;
 
     // What the kernels read of a view; strides in bytes (a view is below 4 GiB). Views hold
     // FP32 or bfloat16 elements.
@@ -480,19 +485,30 @@ namespace ehp
         return p;
     }
 
-    // Two neighbouring bfloat16 elements of a view (one 4-byte load), widened to FP32.
+    // Two neighbouring bfloat16 elements of a view, widened to FP32. pb is 4-byte aligned (rows
+    // have an even number of elements), the pair half the time not: the GPU reads the aligned
+    // 8 bytes around it and takes the pair out with one byte-align, instead of a misaligned
+    // 4-byte load. Those 8 bytes can reach 4 bytes past the view's last element, so bfloat16
+    // views are allocated with kViewSlack bytes to spare.
     EHP_HD inline Pair ld_view2_bf16(const PadView& V, const char* pb, uint32_t off)
     {
 #if defined(EHP_VIEW_CHECK)
         const int64_t o = (int64_t) (pb - (const char*) V.base) + off;
-        if ((o & 1) || o < 0 || (o >> 1) + 1 >= V.size) {
+        if ((o & 1) || o < 0 || (o >> 1) + 1 >= V.size || ((pb - (const char*) V.base) & 3)) {
             g_view_oob = true;
             return Pair{0.0f, 0.0f};
         }
 #endif
+#if defined(__HIP_DEVICE_COMPILE__)
+        uint32_t w[2];
+        __builtin_memcpy(w, __builtin_assume_aligned(pb + (off & ~3u), 4), sizeof(w));
+        const uint32_t pair = __builtin_amdgcn_alignbyte(w[1], w[0], off);
+        return Pair{__uint_as_float(pair << 16), __uint_as_float(pair & 0xffff0000u)};
+#else
         uint16_t h[2];
         __builtin_memcpy(h, pb + off, sizeof(h));
         return Pair{bf16_to_float(bf16{h[0]}), bf16_to_float(bf16{h[1]})};
+#endif
     }
 
     template <bool V16>

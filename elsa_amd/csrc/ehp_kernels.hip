@@ -223,11 +223,12 @@ namespace ehp
     // ---------------------------------------------------------------------------------
     // Voxel-driven blob / B-spline projector.
 
-    // Adjoint: one thread per voxel gathers over all poses (no atomics).
-    template <int D, class T>
+    // Adjoint: one thread per voxel gathers over all poses (no atomics). ST is the storage type
+    // (bfloat16 storage: widened exactly, summed in FP32, each voxel rounded once).
+    template <int D, class T, class ST>
     __global__ void __launch_bounds__(256)
         voxel_adj_kernel(const PoseGeom* __restrict__ geo, const Dims dims, const T* __restrict__ lut, int n_lut,
-                         float support, const int diff, const T* __restrict__ sino, T* __restrict__ vol)
+                         float support, const int diff, const ST* __restrict__ sino, ST* __restrict__ vol)
     {
         const int64_t vox = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
         if (vox >= dims.nvox)
@@ -239,12 +240,12 @@ namespace ehp
         for (int p = 0; p < dims.nposes; ++p) {
             Footprint fp;
             footprint<D>(geo[p], dims, x, y, z, support, fp);
-            const T* sp = sino + (int64_t) p * dims.pix_per_pose;
+            const ST* sp = sino + (int64_t) p * dims.pix_per_pose;
             visit<D, T>(fp, dims, [&](int64_t pix, float dist, float d0) {
-                acc += tap_weight<D, T>(lut, n_lut, support, dist, d0, diff != 0) * sp[pix];
+                acc += tap_weight<D, T>(lut, n_lut, support, dist, d0, diff != 0) * up(sp[pix]);
             });
         }
-        vol[vox] = acc;
+        vol[vox] = down<ST>(acc);
     }
 
     // Forward: one thread per (voxel, pose) scatters into the detector.
@@ -387,17 +388,17 @@ namespace ehp
         return hipGetLastError();
     }
 
-    template <class T>
+    template <class T, class ST>
     hipError_t voxel_adjoint(const PoseGeom* geo, const Dims& dims, const T* lut, int n_lut, float support,
-                             int diff, const T* sino, T* vol, hipStream_t s)
+                             int diff, const ST* sino, ST* vol, hipStream_t s)
     {
         const dim3 grid((unsigned) ((dims.nvox + 255) / 256));
         if (dims.dim == 2)
-            hipLaunchKernelGGL((voxel_adj_kernel<2, T>), grid, dim3(256), 0, s, geo, dims, lut, n_lut, support, diff,
-                               sino, vol);
+            hipLaunchKernelGGL((voxel_adj_kernel<2, T, ST>), grid, dim3(256), 0, s, geo, dims, lut, n_lut, support,
+                               diff, sino, vol);
         else
-            hipLaunchKernelGGL((voxel_adj_kernel<3, T>), grid, dim3(256), 0, s, geo, dims, lut, n_lut, support, diff,
-                               sino, vol);
+            hipLaunchKernelGGL((voxel_adj_kernel<3, T, ST>), grid, dim3(256), 0, s, geo, dims, lut, n_lut, support,
+                               diff, sino, vol);
         return hipGetLastError();
     }
 
@@ -421,13 +422,15 @@ namespace ehp
     template hipError_t siddon_adjoint<T>(const PoseGeom*, const Dims&, const T*, T*, hipStream_t);              \
     template hipError_t voxel_forward<T>(const PoseGeom*, const Dims&, const T*, int, float, int, const T*, T*,   \
                                          hipStream_t);                                                           \
-    template hipError_t voxel_adjoint<T>(const PoseGeom*, const Dims&, const T*, int, float, int, const T*, T*,   \
-                                         hipStream_t);
+    template hipError_t voxel_adjoint<T, T>(const PoseGeom*, const Dims&, const T*, int, float, int, const T*,    \
+                                            T*, hipStream_t);
 
     EHP_INSTANTIATE(float)
     EHP_INSTANTIATE(double)
     template hipError_t launch_fill<int>(int*, int64_t, int, hipStream_t);
     template hipError_t launch_fill<bf16>(bf16*, int64_t, bf16, hipStream_t);
+    template hipError_t voxel_adjoint<float, bf16>(const PoseGeom*, const Dims&, const float*, int, float, int,
+                                                   const bf16*, bf16*, hipStream_t);
     template hipError_t launch_fill<unsigned int>(unsigned int*, int64_t, unsigned int, hipStream_t);
 #undef EHP_INSTANTIATE
 } // namespace ehp

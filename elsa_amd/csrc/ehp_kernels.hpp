@@ -118,10 +118,11 @@ namespace ehp
                                     hipStream_t s);
     size_t voxel_tile_lds_bytes(const TileCfg& tc);
     int voxel_max_lut();
+    template <class ST>
     hipError_t voxel_forward_tiled(const PoseGeom* geo, const Dims& dims, const float* lut, int n_lut, float support,
                                    int diff, const int* pose_axis, const float* pose_margin, const TileCfg& tc,
                                    float lutscale, double slot_bound, unsigned int* d_maxbits, int* d_fixexp,
-                                   const float* vol, float* sino, hipStream_t s);
+                                   const ST* vol, ST* sino, hipStream_t s);
     template <class T>
     hipError_t joseph_forward_planes(const PoseGeom* geo, const Dims& dims, const PadLayout* layouts,
                                      void* const* views, int axes_present, const T* vol, T* sino, hipStream_t s);
@@ -135,9 +136,10 @@ namespace ehp
     template <class T>
     hipError_t voxel_forward(const PoseGeom* geo, const Dims& dims, const T* lut, int n_lut, float support,
                              int diff, const T* vol, T* sino, hipStream_t s);
-    template <class T>
+    // ST: the storage type of sinogram and volume (T, or bfloat16 with T = float)
+    template <class T, class ST = T>
     hipError_t voxel_adjoint(const PoseGeom* geo, const Dims& dims, const T* lut, int n_lut, float support,
-                             int diff, const T* sino, T* vol, hipStream_t s);
+                             int diff, const ST* sino, ST* vol, hipStream_t s);
     hipError_t voxel_count(const PoseGeom* geo, const Dims& dims, float support, unsigned long long* total,
                            hipStream_t s);
 

@@ -1,5 +1,5 @@
 // ehp_voxel_tiled.hip -- voxel-driven forward projection with LDS-privatised detector tiles
-// (FP32).
+// (FP32 arithmetic; volume and sinogram stored in FP32 or bfloat16).
 //
 // The forward of elsa's voxel projectors (VoxelComputation.h:110-152: every voxel adds
 // x * basis(distance) to the detector pixels of its footprint) without global atomics. One
@@ -13,7 +13,9 @@
 //
 // Sums are 64-bit fixed point as in the tiled back projections: one exponent per call from
 // max|x| and a bound on what one pixel can receive, each voxel's contributions rounded to
-// int32 in the voxel's own scale.
+// int32 in the voxel's own scale. With bfloat16 storage the voxels are widened exactly and each
+// pixel's sum is rounded to bfloat16 once, so the result is the FP32 operator's on the same
+// (bfloat16) volume, rounded once.
 #include "ehp_kernels.hpp"
 
 namespace ehp
@@ -24,14 +26,13 @@ namespace ehp
         constexpr int kMaxLut = 1024;
     } // namespace
 
-    template <int D>
+    template <int D, class ST>
     __global__ void __launch_bounds__(kThreads)
         voxel_fwd_tiled_kernel(const PoseGeom* __restrict__ geo, const Dims dims, const float* __restrict__ lut,
                                const int n_lut, const float support, const int diff, const int* __restrict__ pose_axis,
                                const float* __restrict__ pose_margin, const int TU, const int TV, const int ntu,
                                const int per_xcd, const int64_t total, const float lutscale,
-                               const int* __restrict__ fixexp, const float* __restrict__ vol,
-                               float* __restrict__ sino)
+                               const int* __restrict__ fixexp, const ST* __restrict__ vol, ST* __restrict__ sino)
     {
         extern __shared__ long long lds[];
         __shared__ float s_lut[kMaxLut];
@@ -78,7 +79,7 @@ namespace ehp
                 if (D == 3)
                     c[b2] = clo + row;
                 const int64_t lin = c[0] + c[1] * dims.strides[1] + (int64_t) c[2] * dims.strides[2];
-                const float xv = vol[lin];
+                const float xv = up(vol[lin]);
                 if (xv == 0.0f)
                     continue;
                 Footprint fp;
@@ -110,11 +111,11 @@ namespace ehp
         }
         __syncthreads();
         const double q = eg == kNonFinite ? __longlong_as_double(0x7ff8000000000000ll) : ldexp(1.0, eg - 62);
-        float* sp = sino + (int64_t) pose * dims.pix_per_pose;
+        ST* sp = sino + (int64_t) pose * dims.pix_per_pose;
         for (int i = tid; i < TU * TV; i += kThreads) {
             const int u = u0 + i % TU, v = v0 + i / TU;
             if (u <= u1 && v <= v1)
-                sp[(int64_t) v * dims.det[0] + u] = (float) ((double) acc[i] * q);
+                sp[(int64_t) v * dims.det[0] + u] = down<ST>((float) ((double) acc[i] * q));
         }
     }
 
@@ -130,10 +131,11 @@ namespace ehp
     }
 
     // tc.LA x tc.TB: tile width x height in pixels (height 1 in 2D)
+    template <class ST>
     hipError_t voxel_forward_tiled(const PoseGeom* geo, const Dims& dims, const float* lut, int n_lut, float support,
                                    int diff, const int* pose_axis, const float* pose_margin, const TileCfg& tc,
                                    float lutscale, double slot_bound, unsigned int* d_maxbits, int* d_fixexp,
-                                   const float* vol, float* sino, hipStream_t s)
+                                   const ST* vol, ST* sino, hipStream_t s)
     {
         if (n_lut < 2 || n_lut > kMaxLut)
             return hipErrorInvalidValue;
@@ -149,11 +151,20 @@ namespace ehp
         const dim3 grid((unsigned) (per_xcd * 8));
         const size_t lds = (size_t) TU * TV * sizeof(long long);
         if (dims.dim == 2)
-            hipLaunchKernelGGL((voxel_fwd_tiled_kernel<2>), grid, dim3(kThreads), lds, s, geo, dims, lut, n_lut, support,
-                               diff, pose_axis, pose_margin, TU, TV, ntu, (int) per_xcd, total, lutscale, d_fixexp, vol, sino);
+            hipLaunchKernelGGL((voxel_fwd_tiled_kernel<2, ST>), grid, dim3(kThreads), lds, s, geo, dims, lut, n_lut,
+                               support, diff, pose_axis, pose_margin, TU, TV, ntu, (int) per_xcd, total, lutscale,
+                               d_fixexp, vol, sino);
         else
-            hipLaunchKernelGGL((voxel_fwd_tiled_kernel<3>), grid, dim3(kThreads), lds, s, geo, dims, lut, n_lut, support,
-                               diff, pose_axis, pose_margin, TU, TV, ntu, (int) per_xcd, total, lutscale, d_fixexp, vol, sino);
+            hipLaunchKernelGGL((voxel_fwd_tiled_kernel<3, ST>), grid, dim3(kThreads), lds, s, geo, dims, lut, n_lut,
+                               support, diff, pose_axis, pose_margin, TU, TV, ntu, (int) per_xcd, total, lutscale,
+                               d_fixexp, vol, sino);
         return hipGetLastError();
     }
+
+    template hipError_t voxel_forward_tiled<float>(const PoseGeom*, const Dims&, const float*, int, float, int,
+                                                   const int*, const float*, const TileCfg&, float, double,
+                                                   unsigned int*, int*, const float*, float*, hipStream_t);
+    template hipError_t voxel_forward_tiled<bf16>(const PoseGeom*, const Dims&, const float*, int, float, int,
+                                                  const int*, const float*, const TileCfg&, float, double,
+                                                  unsigned int*, int*, const bf16*, bf16*, hipStream_t);
 } // namespace ehp

@@ -29,7 +29,7 @@ import numpy as np
 from . import _lib
 from .descriptors import DetectorDescriptor, VolumeDescriptor
 
-# bfloat16 storage (FP32 arithmetic; Joseph only): dtype="bfloat16" or torch.bfloat16. Data are
+# bfloat16 storage (FP32 arithmetic; Joseph and the voxel projectors): dtype="bfloat16" or torch.bfloat16. Data are
 # torch.bfloat16 GPU tensors, or numpy arrays rounded to bfloat16 on the way in and returned as
 # float32.
 BF16 = "bfloat16"
@@ -278,10 +278,17 @@ def bspline_derivative_lut(dim, order=2, n=100, dtype=np.float32, normalized=Fal
     return out, sup.value
 
 
+def _table_dtype(dtype):
+    """The basis-function table's dtype: the operator's, FP32 for bfloat16 storage."""
+    key = _dtype_key(dtype)
+    return np.float32 if key == BF16 else key
+
+
 class _VoxelProjector(_HipProjector):
-    """``kernels="fast"`` (FP32) runs the forward projection over LDS-privatised detector tiles
-    with 64-bit fixed-point sums; ``"simple"`` runs one thread per voxel and pose with float
-    atomics. The back projection is a gather in both. FP64 always uses the simple kernels."""
+    """``kernels="fast"`` (FP32 and bfloat16 storage) runs the forward projection over
+    LDS-privatised detector tiles with 64-bit fixed-point sums; ``"simple"`` runs one thread per
+    voxel and pose with float atomics. The back projection is a gather in both. FP64 always uses
+    the simple kernels; bfloat16 operators the fast ones only."""
 
     def _set_kernels(self, kernels):
         if kernels not in ("fast", "simple"):
@@ -306,7 +313,7 @@ class BlobVoxelProjectorCUDA(_VoxelProjector):
     def __init__(self, domainDescriptor, rangeDescriptor, radius=2.0, alpha=10.83, order=2, dtype=np.float32,
                  kernels: str = "fast"):
         self.radius, self.alpha, self.order = radius, alpha, int(order)
-        self.lut, self.support = blob_lut(radius, alpha, order, 100, dtype)
+        self.lut, self.support = blob_lut(radius, alpha, order, 100, _table_dtype(dtype))
         self._set_kernels(kernels)
         super().__init__(domainDescriptor, rangeDescriptor, dtype)
 
@@ -316,7 +323,7 @@ class BSplineVoxelProjectorCUDA(_VoxelProjector):
 
     def __init__(self, domainDescriptor, rangeDescriptor, order=2, dtype=np.float32, kernels: str = "fast"):
         self.order = int(order)
-        self.lut, self.support = bspline_lut(domainDescriptor.getNumberOfDimensions(), order, 100, dtype)
+        self.lut, self.support = bspline_lut(domainDescriptor.getNumberOfDimensions(), order, 100, _table_dtype(dtype))
         self._set_kernels(kernels)
         super().__init__(domainDescriptor, rangeDescriptor, dtype)
 
@@ -332,7 +339,8 @@ class PhaseContrastBlobVoxelProjectorCUDA(_VoxelProjector):
                  kernels: str = "fast"):
         self.radius, self.alpha, self.order = radius, alpha, int(order)
         dim = domainDescriptor.getNumberOfDimensions()
-        self.lut, self.support = blob_derivative_lut(radius, alpha, order, 100, dtype, normalized=dim == 3)
+        self.lut, self.support = blob_derivative_lut(radius, alpha, order, 100, _table_dtype(dtype),
+                                                     normalized=dim == 3)
         self._set_kernels(kernels)
         super().__init__(domainDescriptor, rangeDescriptor, dtype)
 
@@ -346,7 +354,7 @@ class PhaseContrastBSplineVoxelProjectorCUDA(_VoxelProjector):
     def __init__(self, domainDescriptor, rangeDescriptor, order=2, dtype=np.float32, kernels: str = "fast"):
         self.order = int(order)
         dim = domainDescriptor.getNumberOfDimensions()
-        self.lut, self.support = bspline_derivative_lut(dim, order, 100, dtype, normalized=dim == 3)
+        self.lut, self.support = bspline_derivative_lut(dim, order, 100, _table_dtype(dtype), normalized=dim == 3)
         self._set_kernels(kernels)
         super().__init__(domainDescriptor, rangeDescriptor, dtype)
 

@@ -65,8 +65,9 @@ enum ehp_dtype {
     EHP_F32 = 0,  /* data_t = float  */
     EHP_F64 = 1,  /* data_t = double */
     EHP_BF16 = 2  /* bfloat16 storage (raw bits, the upper half of an IEEE single) of volume and
-                     sinogram, FP32 arithmetic: Joseph with the default (fast) kernels only. No
-                     counterpart in elsa; the storage form of mixed-precision reconstruction */
+                     sinogram, FP32 arithmetic: Joseph and the voxel projectors with the default
+                     (fast) kernels only. No counterpart in elsa; the storage form of
+                     mixed-precision reconstruction */
 };
 
 typedef struct ehp_geometry ehp_geometry; /* device-resident per-pose matrices      */
@@ -111,7 +112,8 @@ int ehp_siddon_create(const ehp_geometry* g, int dtype, int flags, ehp_op** out)
  * n_lut values over [0, support] (elsa: BlobVoxelProjectorCUDA / BSplineVoxelProjectorCUDA,
  * VoxelProjectorCUDA.h:39-43, :82-84; the table is ProjectedBlob::get_lut() or
  * ProjectedBSpline::get_lut(), Blobs.h:249-317, BSplines.h:132-208). lut is host memory of
- * the op's dtype. FP32 forward projections run on LDS-tiled detector tiles; flags: 0,
+ * the op's dtype (FP32 for EHP_BF16 operators, which compute in FP32). FP32 and bfloat16
+ * forward projections run on LDS-tiled detector tiles; flags: 0,
  * EHP_SIMPLE_KERNELS (one thread per voxel and pose, atomic splat) and EHP_DIFFERENTIAL (the
  * phase-contrast projectors PhaseContrastBlobVoxelProjectorCUDA /
  * PhaseContrastBSplineVoxelProjectorCUDA, VoxelProjectorCUDA.h:109-191). */

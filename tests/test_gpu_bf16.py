@@ -1,4 +1,5 @@
-"""bfloat16 storage (FP32 arithmetic) for the Joseph projector and iterative refinement.
+"""bfloat16 storage (FP32 arithmetic) for the Joseph and voxel projectors, and iterative
+refinement.
 
 A bfloat16 operator reads bfloat16 volume views and sums each ray in FP32, so its forward
 equals the FP32 operator's on the same (bfloat16-valued) volume rounded once to bfloat16; its
@@ -46,7 +47,38 @@ def test_bf16_projections_equal_fp32_rounded_once(name):
     np.testing.assert_array_equal(A16.apply(xn), f16.float().cpu().numpy())
 
 
-def test_bf16_operator_is_joseph_fast_only():
+VOXEL_KINDS = ["BlobVoxelProjectorCUDA", "BSplineVoxelProjectorCUDA", "PhaseContrastBlobVoxelProjectorCUDA",
+               "PhaseContrastBSplineVoxelProjectorCUDA"]
+
+
+@pytest.mark.parametrize("kind", VOXEL_KINDS)
+@pytest.mark.parametrize("name", list(CASES))
+def test_bf16_voxel_projections_equal_fp32_rounded_once(name, kind):
+    """The voxel projectors on bfloat16 storage: the tiled forward's fixed-point pixel sums and
+    the gather's FP32 voxel sums are those of the FP32 operator on the same (bfloat16-valued)
+    data, each rounded to bfloat16 once."""
+    import torch
+
+    import elsa_amd as E
+
+    c = CASES[name]
+    g = oracle.circle_geometry(c["dim"], c["vol"], c["views"], 360, c["s2c"], c["c2d"], det=c["det"])
+    vd, dd = geom_to_descriptors(g)
+    A32 = getattr(E, kind)(vd, dd)
+    A16 = getattr(E, kind)(vd, dd, dtype="bfloat16")
+    assert A16.lut.dtype == np.float32 and np.array_equal(A16.lut, A32.lut)
+    gen = torch.Generator(device="cuda").manual_seed(4)
+    x = (torch.rand(g.vol_shape, device="cuda", generator=gen) * 3 - 1).to(torch.bfloat16)
+    y = torch.rand(g.sino_shape, device="cuda", generator=gen).to(torch.bfloat16)
+    f16 = A16.apply(x)
+    b16 = A16.applyAdjoint(y)
+    assert f16.dtype == torch.bfloat16 and b16.dtype == torch.bfloat16
+    assert torch.equal(f16, A32.apply(x.float()).to(torch.bfloat16))
+    assert torch.equal(b16, A32.applyAdjoint(y.float()).to(torch.bfloat16))
+    assert float(f16.float().abs().max()) > 0 and float(b16.float().abs().max()) > 0
+
+
+def test_bf16_operator_is_fast_kernels_only():
     import elsa_amd as E
 
     c = CASES["fan2d"]
@@ -54,6 +86,8 @@ def test_bf16_operator_is_joseph_fast_only():
     vd, dd = geom_to_descriptors(g)
     with pytest.raises(NotImplementedError):
         E.JosephsMethodCUDA(vd, dd, dtype="bfloat16", kernels="simple")
+    with pytest.raises(NotImplementedError):
+        E.BlobVoxelProjectorCUDA(vd, dd, dtype="bfloat16", kernels="simple")
     with pytest.raises(NotImplementedError):
         E.SiddonsMethodCUDA(vd, dd, dtype="bfloat16")
 
