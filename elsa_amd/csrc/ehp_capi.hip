@@ -276,6 +276,42 @@ namespace
         return best;
     }
 
+    // Zero-padded plane-major views of the volume for the plane-walking forwards, one per
+    // driving axis that occurs (pose_axes: the axes per pose): 32-bit byte offsets inside a view,
+    // so below 4 GiB each; the padding is zeroed once here and never written again. Sets
+    // op->planes when every view fits.
+    int setup_views(ehp_op* op, const std::vector<int>& h)
+    {
+        const ehp::Dims& dm = op->dims;
+        hipError_t e = hipSuccess;
+        bool planes = true;
+        for (int a = 0; a < dm.dim; ++a) {
+            op->layouts[a] = ehp::pad_layout(dm, a, fan_axis(op, h, a));
+            if ((op->axes_present & (1 << a)) && op->layouts[a].size >= ((int64_t) 1 << 30))
+                planes = false;
+        }
+        const bool b16 = op->dtype == EHP_BF16;
+        for (int a = 0; planes && a < dm.dim; ++a) {
+            if (!(op->axes_present & (1 << a)))
+                continue;
+            const size_t vbytes =
+                (size_t) op->layouts[a].size * (b16 ? sizeof(ehp::bf16) : sizeof(float)) + (b16 ? ehp::kViewSlack : 0);
+            e = hipMalloc((void**) &op->d_view[a], vbytes);
+            if (e != hipSuccess)
+                return fail(EHP_ERR_ALLOC, "hipMalloc of a %zu-byte volume view failed: %s", vbytes,
+                            hipGetErrorString(e));
+            if (b16)
+                e = ehp::launch_fill<ehp::bf16>((ehp::bf16*) op->d_view[a], op->layouts[a].size + ehp::kViewSlack / 2,
+                                                ehp::bf16{0}, nullptr);
+            else
+                e = ehp::launch_fill<float>((float*) op->d_view[a], op->layouts[a].size, 0.0f, nullptr);
+            if (e != hipSuccess)
+                return hip_fail(e, "volume view");
+        }
+        op->planes = planes;
+        return EHP_OK;
+    }
+
     // FP32 Joseph: per-ray table, driving axes per pose, plane-major views for the forward
     // and, when the fixed-point bound exists, the LDS-tiled back projection.
     int setup_fast(ehp_op* op)
@@ -305,32 +341,11 @@ namespace
         for (int v : h)
             op->axes_present |= v;
 
-        // forward: zero-padded plane-major views (32-bit byte offsets inside a view, so below
-        // 4 GiB each); the padding is zeroed once here and never written again
-        bool planes = true;
-        for (int a = 0; a < dm.dim; ++a) {
-            op->layouts[a] = ehp::pad_layout(dm, a, fan_axis(op, h, a));
-            if ((op->axes_present & (1 << a)) && op->layouts[a].size >= ((int64_t) 1 << 30))
-                planes = false;
-        }
+        int rc = setup_views(op, h);
+        if (rc != EHP_OK)
+            return rc;
         const bool b16 = op->dtype == EHP_BF16;
-        for (int a = 0; planes && a < dm.dim; ++a) {
-            if (!(op->axes_present & (1 << a)))
-                continue;
-            const size_t vbytes =
-                (size_t) op->layouts[a].size * (b16 ? sizeof(ehp::bf16) : sizeof(float)) + (b16 ? ehp::kViewSlack : 0);
-            e = hipMalloc((void**) &op->d_view[a], vbytes);
-            if (e != hipSuccess)
-                return fail(EHP_ERR_ALLOC, "hipMalloc of a %zu-byte volume view failed: %s", vbytes,
-                            hipGetErrorString(e));
-            if (b16)
-                e = ehp::launch_fill<ehp::bf16>((ehp::bf16*) op->d_view[a], op->layouts[a].size + ehp::kViewSlack / 2,
-                                                ehp::bf16{0}, nullptr);
-            else
-                e = ehp::launch_fill<float>((float*) op->d_view[a], op->layouts[a].size, 0.0f, nullptr);
-            if (e != hipSuccess)
-                return hip_fail(e, "volume view");
-        }
+        const bool planes = op->planes;
         if (b16 && planes) {
             const size_t abytes = (size_t) dm.nvox * sizeof(float);
             e = hipMalloc((void**) &op->d_acc32, abytes);
@@ -363,8 +378,8 @@ namespace
         return EHP_OK;
     }
 
-    // FP32 Siddon: per-ray table, driving axes per pose and, when the fixed-point bound
-    // exists, the LDS-tiled back projection.
+    // FP32 Siddon: when the fixed-point bound exists, the per-ray table, the driving axes per
+    // pose, plane-major views for the forward and the LDS-tiled back projection.
     int setup_siddon_tiled(ehp_op* op)
     {
         const ehp::Dims& dm = op->dims;
@@ -395,6 +410,13 @@ namespace
         op->axes_present = 0;
         for (int v : h)
             op->axes_present |= v;
+        // forward: the plane-major views of the Joseph forward, walked plane by plane
+        const int rc = setup_views(op, h);
+        if (rc != EHP_OK)
+            return rc;
+        e = hipDeviceSynchronize(); // the views' padding is cleared on the null stream
+        if (e != hipSuccess)
+            return hip_fail(e, "volume view");
         op->slot_bound = bound;
         int max_lds = 65536;
         if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, op->device) != hipSuccess)
@@ -639,6 +661,8 @@ static int apply(ehp_op* op, const void* in, void* out, void* stream, bool adjoi
             e = ehp::joseph_forward_planes<float>(op->d_geo, dm, op->layouts, op->d_view, op->axes_present, x, y, s);
         else if (op->kind == KIND_JOSEPH)
             e = adjoint ? ehp::joseph_adjoint<float>(op->d_geo, dm, x, y, s) : ehp::joseph_forward<float>(op->d_geo, dm, x, y, s);
+        else if (op->kind == KIND_SIDDON && !adjoint && op->planes)
+            e = ehp::siddon_forward_planes(op->d_sidtable, dm, op->layouts, op->d_view, op->axes_present, x, y, s);
         else if (op->kind == KIND_SIDDON && adjoint && op->tiled)
             e = ehp::siddon_adjoint_tiled(op->d_geo, op->d_sidtable, dm, op->d_pose_axes, op->axes_present, op->tile,
                                           op->batch, op->slot_bound, op->d_maxbits, op->d_fixexp, x, y, s);

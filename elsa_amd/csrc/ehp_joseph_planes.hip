@@ -16,23 +16,14 @@
 // to different XCDs, so the block index is remapped) takes a contiguous run: the poses of a
 // band then run together on one XCD. For a rotation about y a band of rows sees the same
 // slab of the volume from every pose, so that slab stays in the XCD's L2.
-#include "ehp_kernels.hpp"
+#include "ehp_waves.hpp"
 
 namespace ehp
 {
     namespace
     {
-        constexpr int kThreads = 256;
+        constexpr int kThreads = kFwdThreads;
         constexpr int kTile = 32; // transpose tile
-        // 3D forward: a wavefront covers WU x WV pixels (u x v), a block TU x TV (four
-        // wavefronts side by side along u). 8 x 8: neighbouring pixels of a square have nearly the
-        // same entry and exit planes (96 % of a wavefront's lane-steps carry a sample at 256^3 /
-        // 384^2, against 77 % for a 64 x 1 row), at about twice the cache lines per step of a row.
-        constexpr int kWU = 8;
-        constexpr int kWV = 64 / kWU;
-        constexpr int kTV = kWV > 4 ? kWV : 4;
-        constexpr int kTU = kThreads / kTV;
-        static_assert(kWU * kWV == 64 && kTU % kWU == 0 && kTV % kWV == 0, "forward wavefront shape");
 
         // View with x fastest: one lane per voxel, rows copied as they are.
         template <class E>
@@ -89,31 +80,6 @@ namespace ehp
             if (D == 3 && (vs.fb2 >> A) & 1)
                 return plane_span_forward<D, A, NEG, true, V16>(r, dims, vs.v[A], t0, niter, s_first, full0, full1);
             return plane_span_forward<D, A, NEG, false, V16>(r, dims, vs.v[A], t0, niter, s_first, full0, full1);
-        }
-
-        // Reductions over the 64 lanes of a wavefront (every lane active), returned in a scalar register.
-        __device__ __forceinline__ int wave_min(int x)
-        {
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1)
-                x = min(x, __shfl_xor(x, o, 64));
-            return __builtin_amdgcn_readfirstlane(x);
-        }
-
-        __device__ __forceinline__ int wave_max(int x)
-        {
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1)
-                x = max(x, __shfl_xor(x, o, 64));
-            return __builtin_amdgcn_readfirstlane(x);
-        }
-
-        __device__ __forceinline__ int wave_or(int x)
-        {
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1)
-                x |= __shfl_xor(x, o, 64);
-            return __builtin_amdgcn_readfirstlane(x);
         }
 
         // The rays of group K = 2 * axis + direction of a wavefront share one plane loop over the
@@ -175,16 +141,8 @@ namespace ehp
                                                          const ViewSet& views, const int tiles_u, const int64_t id,
                                                          OT* __restrict__ sino)
         {
-            constexpr int TU = (D == 3) ? kTU : kThreads;
-            constexpr int TV = kThreads / TU;
-            constexpr int WU = (D == 3) ? kWU : 64, WV = 64 / WU;
-            const int tu = (int) (id % tiles_u);
-            const int64_t bp = id / tiles_u;
-            const int pose = (int) (bp % dims.nposes);
-            const int band = (int) (bp / dims.nposes);
-            const int lane = (int) threadIdx.x % 64, w = (int) threadIdx.x / 64;
-            const int u = tu * TU + (w % (TU / WU)) * WU + lane % WU;
-            const int v = band * TV + (w / (TU / WU)) * WV + lane / WU;
+            int u, v, pose;
+            fwd_pixel<D>(dims, tiles_u, id, u, v, pose);
             const bool inside = u < dims.det[0] && v < dims.det[1];
             RayRec r{};
             int key = -1, tfirst = 0, tlast = 0, s0 = 0;
@@ -295,12 +253,9 @@ namespace ehp
             if (L.f == b2)
                 vs.fb2 |= 1 << a;
         }
-        const int TU = dims.dim == 3 ? kTU : kThreads;
-        const int TV = kThreads / TU;
-        const int tiles_u = (dims.det[0] + TU - 1) / TU;
-        const int bands = (dims.det[1] + TV - 1) / TV;
-        const int64_t total = (int64_t) tiles_u * bands * dims.nposes;
-        const int64_t per_xcd = (total + 7) / 8;
+        const FwdGrid fg = fwd_grid(dims);
+        const int tiles_u = fg.tiles_u;
+        const int64_t total = fg.total, per_xcd = fg.per_xcd;
         if (per_xcd * 8 > (int64_t) 0x7fffffff)
             return hipErrorInvalidValue;
         const dim3 grid((unsigned) (per_xcd * 8));
@@ -313,6 +268,8 @@ namespace ehp
         return hipGetLastError();
     }
 
+    template hipError_t joseph_view_copy<float>(const Dims&, const PadLayout&, const float*, float*, hipStream_t);
+    template hipError_t joseph_view_copy<bf16>(const Dims&, const PadLayout&, const bf16*, bf16*, hipStream_t);
     template hipError_t joseph_forward_planes<float>(const PoseGeom*, const Dims&, const PadLayout*, void* const*, int,
                                                      const float*, float*, hipStream_t);
     template hipError_t joseph_forward_planes<bf16>(const PoseGeom*, const Dims&, const PadLayout*, void* const*, int,
@@ -351,11 +308,9 @@ namespace ehp
                 if (L.f == b2)
                     fj.views.fb2 |= 1 << a;
             }
-            const int TU = it.dims.dim == 3 ? kTU : kThreads;
-            const int TV = kThreads / TU;
-            fj.tiles_u = (it.dims.det[0] + TU - 1) / TU;
-            const int bands = (it.dims.det[1] + TV - 1) / TV;
-            const int64_t nb = (int64_t) fj.tiles_u * bands * it.dims.nposes;
+            const FwdGrid fg = fwd_grid(it.dims);
+            fj.tiles_u = fg.tiles_u;
+            const int64_t nb = fg.total;
             fj.block0 = (int64_t) fblocks.size();
             fblocks.insert(fblocks.end(), (size_t) nb, (int) fjobs.size());
             fjobs.push_back(fj);

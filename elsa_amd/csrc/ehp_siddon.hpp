@@ -151,6 +151,76 @@ namespace ehp
         }
     }
 
+    // A Siddon traversal that can pause between the planes of one axis: the state of
+    // siddon_traverse after some of its steps (voxel, step counts, next crossing per axis, the
+    // last crossing).
+    struct SidWalk {
+        int cur[3];
+        int k[3];
+        float Tn[3];
+        float tExit;
+        bool alive;
+    };
+
+    template <int D>
+    EHP_HD inline void sid_walk_init(const SidRec& r, SidWalk& w)
+    {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            w.cur[i] = i < D ? r.cur0[i] : 0;
+            w.k[i] = 0;
+            w.Tn[i] = i < D ? r.T0[i] : kSidMax;
+        }
+        w.tExit = 0.0f;
+        w.alive = sid_hit(r);
+    }
+
+    // Steps of one plane: a ray crossing plane p of its driving axis A (the largest |d|) in it
+    // moves less than one voxel along every other axis, so it visits at most D voxels there.
+    constexpr int kSidPlaneSteps = 8;
+
+    // The steps of siddon_traverse whose voxel lies on plane p of axis A, fn(c0, c1, c2, w) for
+    // each in ray order, leaving the walk at the first voxel beyond that plane (or ended). The
+    // same arithmetic as siddon_traverse step by step, so walking plane after plane gives its
+    // crossings bit for bit. At most kSidPlaneSteps steps per call, which also stops a
+    // degenerate ray.
+    template <int D, class Fn>
+    EHP_HD inline void sid_walk_plane(const SidRec& r, const int* vol, int A, int p, SidWalk& w, Fn fn)
+    {
+        for (int g = 0; g < kSidPlaneSteps && w.alive && w.cur[A] == p; ++g) {
+            const float tEntry = w.tExit;
+            float tm = w.Tn[0];
+#pragma unroll
+            for (int i = 1; i < D; ++i)
+                tm = fminf(tm, w.Tn[i]);
+            w.tExit = tm;
+            bool mask[3];
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+                bool mi = true;
+#pragma unroll
+                for (int j = 0; j < D; ++j)
+                    if (j != i)
+                        mi = mi && (w.Tn[i] <= w.Tn[j]);
+                mask[i] = mi;
+            }
+            fn(w.cur[0], w.cur[1], w.cur[2], w.tExit - tEntry);
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+                if (mask[i]) {
+                    w.cur[i] += sid_step(r, i);
+                    w.k[i] += 1;
+                    w.Tn[i] = sid_T(r, i, w.k[i]);
+                }
+            }
+            bool in = true;
+#pragma unroll
+            for (int i = 0; i < D; ++i)
+                in = in && w.cur[i] < vol[i] && w.cur[i] >= 0;
+            w.alive = in;
+        }
+    }
+
     // The part of the traversal inside the voxel box lo[i] <= c[i] <= hi[i]: fn(c0, c1, c2, w)
     // for exactly the crossings of siddon_traverse whose voxel lies in the box, with the same
     // w. The ray is restarted at the last face crossing before the box: for each axis the step

@@ -298,3 +298,104 @@ extern "C" int siddon_tile_adjoint(int dim, const int64_t* vol_dims, const int64
     const TileCfg tc{LA, TB, dim == 3 ? TC : 1};
     return dim == 2 ? run_adjoint<2>(dims, geo, tc, sino, vol) : run_adjoint<3>(dims, geo, tc, sino, vol);
 }
+
+// The GPU forward over zero-padded plane-major views (fan[a]: the fastest axis of the view of
+// driving axis a), with its grouping: the pixels (u0 + l % wu, v0 + l / wu), l < 64, form a
+// wavefront; its rays fall into groups by driving axis and step direction along it, and each
+// group walks the planes of that axis together, every ray taking its own steps on the current
+// plane (sid_walk_plane). Every read is checked against its view and against the plane.
+extern "C" int siddon_plane_forward(int dim, const int64_t* vol_dims, const int64_t* det_dims, int nposes,
+                                    const float* P, const float* Pinv, const float* C, const int* fan,
+                                    const float* vol, float* sino, int wu)
+{
+    const Dims dims = make_dims(dim, vol_dims, det_dims, nposes);
+    const std::vector<PoseGeom> geo = poses(dim, nposes, P, Pinv, C);
+    if (dim == 2)
+        wu = 64;
+    if (wu < 1 || 64 % wu != 0)
+        return 23;
+    const int wv = 64 / wu;
+    std::vector<std::vector<float>> views(3);
+    PadLayout L[3];
+    for (int a = 0; a < dim; ++a) {
+        if (fan[a] == a || fan[a] < 0 || fan[a] > (dim == 3 ? 2 : 1))
+            return 20;
+        L[a] = pad_layout(dims, a, fan[a]);
+        views[a].assign(L[a].size, 0.0f);
+        for (int z = 0; z < dims.vol[2]; ++z)
+            for (int y = 0; y < dims.vol[1]; ++y)
+                for (int x = 0; x < dims.vol[0]; ++x)
+                    views[a][pad_index(L[a], x, y, z)] = vol[x + dims.strides[1] * y + dims.strides[2] * z];
+    }
+    bool bad = false;
+    for (int p = 0; p < nposes; ++p)
+        for (int v0 = 0; v0 < dims.det[1]; v0 += wv)
+            for (int u0 = 0; u0 < dims.det[0]; u0 += wu) {
+                SidRec rr[64];
+                int key[64];
+                for (int l = 0; l < 64; ++l) {
+                    key[l] = -1;
+                    const int u = u0 + l % wu, v = v0 + l / wu;
+                    if (u >= dims.det[0] || v >= dims.det[1])
+                        continue;
+                    rr[l] = dim == 2 ? ray_rec<2>(geo[p], dims, u, v) : ray_rec<3>(geo[p], dims, u, v);
+                    if (sid_hit(rr[l])) {
+                        const int a = sid_axis(rr[l]);
+                        key[l] = 2 * a + (sid_step(rr[l], a) < 0 ? 1 : 0);
+                    }
+                }
+                for (int k = 0; k < 2 * dim; ++k) {
+                    const int a = k / 2, dir = (k & 1) ? -1 : 1;
+                    int t0 = 0x7fffffff;
+                    for (int l = 0; l < 64; ++l)
+                        if (key[l] == k)
+                            t0 = dir * rr[l].cur0[a] < t0 ? dir * rr[l].cur0[a] : t0;
+                    if (t0 == 0x7fffffff)
+                        continue;
+                    SidWalk w[64];
+                    float acc[64];
+                    for (int l = 0; l < 64; ++l) {
+                        acc[l] = 0.0f;
+                        if (key[l] == k)
+                            dim == 2 ? sid_walk_init<2>(rr[l], w[l]) : sid_walk_init<3>(rr[l], w[l]);
+                    }
+                    for (int pl = dir * t0; pl >= 0 && pl < dims.vol[a]; pl += dir) {
+                        bool any = false;
+                        for (int l = 0; l < 64; ++l)
+                            any = any || (key[l] == k && w[l].alive);
+                        if (!any)
+                            break;
+                        for (int l = 0; l < 64; ++l) {
+                            if (key[l] != k)
+                                continue;
+                            auto fn = [&](int c0, int c1, int c2, float len) {
+                                const int c[3] = {c0, c1, c2};
+                                const int64_t o = pad_index(L[a], c0, c1, c2);
+                                if (c[a] != pl || o < 0 || o >= L[a].size) {
+                                    bad = true;
+                                    return;
+                                }
+                                acc[l] += views[a][o] * len;
+                            };
+                            if (dim == 2)
+                                sid_walk_plane<2>(rr[l], dims.vol, a, pl, w[l], fn);
+                            else
+                                sid_walk_plane<3>(rr[l], dims.vol, a, pl, w[l], fn);
+                        }
+                    }
+                    for (int l = 0; l < 64; ++l)
+                        if (key[l] == k && w[l].alive)
+                            return 24; // a ray left unfinished
+                    for (int l = 0; l < 64; ++l)
+                        if (key[l] == k)
+                            sino[(int64_t) p * dims.pix_per_pose + (int64_t) (v0 + l / wu) * dims.det[0] + u0 +
+                                 l % wu] = acc[l];
+                }
+                for (int l = 0; l < 64; ++l) {
+                    const int u = u0 + l % wu, v = v0 + l / wu;
+                    if (key[l] < 0 && u < dims.det[0] && v < dims.det[1])
+                        sino[(int64_t) p * dims.pix_per_pose + (int64_t) v * dims.det[0] + u] = 0.0f;
+                }
+            }
+    return bad ? 22 : 0;
+}
