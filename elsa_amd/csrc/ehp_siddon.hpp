@@ -152,11 +152,12 @@ namespace ehp
     }
 
     // A Siddon traversal that can pause between the planes of one axis: the state of
-    // siddon_traverse after some of its steps (voxel, step counts, next crossing per axis, the
-    // last crossing).
+    // siddon_traverse after some of its steps (voxel, step direction and count per axis as a
+    // float -- exact, as counts stay below 2^24 -- next crossing per axis, the last crossing).
     struct SidWalk {
         int cur[3];
-        int k[3];
+        int st[3];
+        float fk[3];
         float Tn[3];
         float tExit;
         bool alive;
@@ -168,7 +169,8 @@ namespace ehp
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             w.cur[i] = i < D ? r.cur0[i] : 0;
-            w.k[i] = 0;
+            w.st[i] = i < D ? sid_step(r, i) : 0;
+            w.fk[i] = 0.0f;
             w.Tn[i] = i < D ? r.T0[i] : kSidMax;
         }
         w.tExit = 0.0f;
@@ -182,8 +184,9 @@ namespace ehp
     // The steps of siddon_traverse whose voxel lies on plane p of axis A, fn(c0, c1, c2, w) for
     // each in ray order, leaving the walk at the first voxel beyond that plane (or ended). The
     // same arithmetic as siddon_traverse step by step, so walking plane after plane gives its
-    // crossings bit for bit. At most kSidPlaneSteps steps per call, which also stops a
-    // degenerate ray.
+    // crossings bit for bit. (Only an axis that moves can have the smallest T, as the driving
+    // axis moves, so the update needs no test for a non-moving one.) At most kSidPlaneSteps steps
+    // per call, which also stops a degenerate ray.
     template <int D, class Fn>
     EHP_HD inline void sid_walk_plane(const SidRec& r, const int* vol, int A, int p, SidWalk& w, Fn fn)
     {
@@ -205,18 +208,16 @@ namespace ehp
                 mask[i] = mi;
             }
             fn(w.cur[0], w.cur[1], w.cur[2], w.tExit - tEntry);
+            bool in = true;
 #pragma unroll
             for (int i = 0; i < D; ++i) {
                 if (mask[i]) {
-                    w.cur[i] += sid_step(r, i);
-                    w.k[i] += 1;
-                    w.Tn[i] = sid_T(r, i, w.k[i]);
+                    w.cur[i] += w.st[i];
+                    w.fk[i] += 1.0f;
+                    w.Tn[i] = __builtin_fmaf(w.fk[i], r.dt[i], r.T0[i]);
                 }
+                in = in && (uint32_t) w.cur[i] < (uint32_t) vol[i];
             }
-            bool in = true;
-#pragma unroll
-            for (int i = 0; i < D; ++i)
-                in = in && w.cur[i] < vol[i] && w.cur[i] >= 0;
             w.alive = in;
         }
     }
@@ -260,14 +261,16 @@ namespace ehp
                 t_lo = te > t_lo ? te : t_lo;
             }
         }
-        int k[3] = {0, 0, 0}, cur[3] = {0, 0, 0};
+        int cur[3] = {0, 0, 0};
+        float fk[3] = {0.0f, 0.0f, 0.0f}; // step counts (exact below 2^24)
         float Tn[3] = {kSidMax, kSidMax, kSidMax};
 #pragma unroll
         for (int i = 0; i < D; ++i) {
             int j = 0;
             if (st[i] != 0 && t_lo > 0.0f) {
-                // smallest j with T_i(j) > t_lo: estimate, then step to it
-                const float est = (t_lo - r.T0[i]) / r.dt[i];
+                // smallest j with T_i(j) > t_lo: estimate (an approximate reciprocal will do),
+                // then step to it
+                const float est = (t_lo - r.T0[i]) * ehp_rcp(r.dt[i]);
                 j = est < 0.0f ? 0 : (est > 1.0e7f ? 10000000 : (int) est);
                 j = j > a[i] ? a[i] : j;
                 while (sid_T(r, i, j) <= t_lo)
@@ -275,7 +278,7 @@ namespace ehp
                 while (j > 0 && sid_T(r, i, j - 1) > t_lo)
                     --j;
             }
-            k[i] = j;
+            fk[i] = (float) j;
             cur[i] = r.cur0[i] + st[i] * j;
             Tn[i] = sid_T(r, i, j);
         }
@@ -288,7 +291,7 @@ namespace ehp
             bool in = true;
 #pragma unroll
             for (int i = 0; i < D; ++i)
-                in = in && cur[i] >= lo[i] && cur[i] <= hi[i];
+                in = in && (uint32_t) (cur[i] - lo[i]) <= (uint32_t) (hi[i] - lo[i]);
             if (!in)
                 return;
             float tm = Tn[0];
@@ -307,12 +310,13 @@ namespace ehp
             }
             fn(cur[0], cur[1], cur[2], tm - tEntry);
             tEntry = tm;
+            // only an axis that moves can have the smallest T (the driving axis moves)
 #pragma unroll
             for (int i = 0; i < D; ++i) {
                 if (mask[i]) {
                     cur[i] += st[i];
-                    k[i] += 1;
-                    Tn[i] = sid_T(r, i, k[i]);
+                    fk[i] += 1.0f;
+                    Tn[i] = __builtin_fmaf(fk[i], r.dt[i], r.T0[i]);
                 }
             }
         }

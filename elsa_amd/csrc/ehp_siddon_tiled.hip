@@ -75,9 +75,14 @@ namespace ehp
     {
         constexpr int b1 = TileAxes<D, A>::b1, b2 = TileAxes<D, A>::b2;
         extern __shared__ long long lds[];
-        __shared__ int s_u0[kMaxPoseBatch], s_nu[kMaxPoseBatch], s_v0[kMaxPoseBatch], s_pose[kMaxPoseBatch];
+        // candidate rectangle of each pose of the batch: rnu = 1 / width, ray index of its first pixel
+        struct CandRect {
+            float rnu;
+            int nu;
+            long long base;
+        };
+        __shared__ CandRect s_rect[kMaxPoseBatch];
         __shared__ int s_cnt[kMaxPoseBatch], s_pref[kMaxPoseBatch + 1];
-        __shared__ float s_rnu[kMaxPoseBatch];
 
         // consecutive workgroups land on different XCDs: give each XCD a contiguous block of tiles
         const int tile = (int) (blockIdx.x % 8u) * tiles_per_xcd + (int) (blockIdx.x / 8u);
@@ -117,11 +122,11 @@ namespace ehp
                     if (box_bbox<D>(geo[p], flo, fhi, dims, u0, u1, v0, v1))
                         cnt = (u1 - u0 + 1) * (v1 - v0 + 1);
                 }
-                s_u0[tid] = u0;
-                s_nu[tid] = u1 - u0 + 1;
-                s_rnu[tid] = 1.0f / (float) (u1 - u0 + 1);
-                s_v0[tid] = v0;
-                s_pose[tid] = p;
+                CandRect cr;
+                cr.nu = u1 - u0 + 1;
+                cr.rnu = 1.0f / (float) (u1 - u0 + 1);
+                cr.base = (long long) p * dims.pix_per_pose + (long long) v0 * dims.det[0] + u0;
+                s_rect[tid] = cr;
                 s_cnt[tid] = cnt;
             }
             __syncthreads();
@@ -132,15 +137,16 @@ namespace ehp
             }
             __syncthreads();
             const int total = s_pref[batch];
+            // ray index of candidate k of this batch: pixel (col, row) of its pose's rectangle
             auto cand_ray = [&](int k) -> int64_t {
                 const int j = batch_pose_slot(s_pref, batch, k);
                 const int loc = k - s_pref[j];
+                const CandRect cr = s_rect[j];
                 // loc / nu through the reciprocal: exact, as loc + 0.5 is at least 0.5 away from
                 // a multiple of nu and loc < 2^22
-                const int row = (int) (((float) loc + 0.5f) * s_rnu[j]);
-                const int u = s_u0[j] + (loc - row * s_nu[j]);
-                const int v = s_v0[j] + row;
-                return (int64_t) s_pose[j] * dims.pix_per_pose + (int64_t) v * dims.det[0] + u;
+                const uint32_t row = (uint32_t) (((float) loc + 0.5f) * cr.rnu);
+                const uint32_t col = (uint32_t) loc - mul24(row, (uint32_t) cr.nu);
+                return cr.base + (int64_t) (mul24(row, (uint32_t) dims.det[0]) + col);
             };
             float yv_next = 0.0f;
             SidRec r_next{};
