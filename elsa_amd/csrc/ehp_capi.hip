@@ -636,8 +636,8 @@ static int apply(ehp_op* op, const void* in, void* out, void* stream, bool adjoi
                                                     op->lutscale, op->slot_bound, op->d_maxbits, op->d_fixexp,
                                                     (const ehp::bf16*) in, (ehp::bf16*) out, s);
         else
-            e = ehp::voxel_adjoint<float, ehp::bf16>(op->d_geo, dm, (const float*) op->d_lut, op->n_lut, op->support,
-                                                     op->differential, (const ehp::bf16*) in, (ehp::bf16*) out, s);
+            e = ehp::voxel_adjoint_fast<ehp::bf16>(op->d_geo, dm, (const float*) op->d_lut, op->n_lut, op->support,
+                                                   op->differential, (const ehp::bf16*) in, (ehp::bf16*) out, s);
     } else if (op->dtype == EHP_BF16) {
         // bfloat16 storage, FP32 arithmetic: the forward reads bfloat16 views and rounds each ray
         // sum once; the back projection sums in fixed point into FP32 and rounds each voxel once
@@ -672,6 +672,9 @@ static int apply(ehp_op* op, const void* in, void* out, void* stream, bool adjoi
             e = ehp::voxel_forward_tiled<float>(op->d_geo, dm, (const float*) op->d_lut, op->n_lut, op->support,
                                          op->differential, op->d_pose_axes, op->d_pose_margin, op->tile, op->lutscale,
                                          op->slot_bound, op->d_maxbits, op->d_fixexp, x, y, s);
+        else if (adjoint && op->tiled)
+            e = ehp::voxel_adjoint_fast<float>(op->d_geo, dm, (const float*) op->d_lut, op->n_lut, op->support,
+                                               op->differential, x, y, s);
         else
             e = adjoint ? ehp::voxel_adjoint<float>(op->d_geo, dm, (const float*) op->d_lut, op->n_lut, op->support,
                                                     op->differential, x, y, s)

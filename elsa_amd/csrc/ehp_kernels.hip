@@ -248,6 +248,60 @@ namespace ehp
         vol[vox] = down<ST>(acc);
     }
 
+    // The default FP32 gather: the taps of the tiled forward (one reciprocal of the scaling per
+    // voxel and pose instead of a division per tap, the table index as dist * (n - 1) / support),
+    // so forward and back projection weigh every tap alike. ST: float or bfloat16 storage.
+    template <int D, class ST>
+    __global__ void __launch_bounds__(256)
+        voxel_adj_fast_kernel(const PoseGeom* __restrict__ geo, const Dims dims, const float* __restrict__ lut,
+                              int n_lut, float support, const int diff, const ST* __restrict__ sino,
+                              ST* __restrict__ vol)
+    {
+        const int64_t vox = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+        if (vox >= dims.nvox)
+            return;
+        const int x = (int) (vox % dims.vol[0]);
+        const int y = (int) ((vox / dims.vol[0]) % dims.vol[1]);
+        const int z = (int) (vox / ((int64_t) dims.vol[0] * dims.vol[1]));
+        const float kidx = (float) (n_lut - 1) / support;
+        float acc = 0.0f;
+        for (int p = 0; p < dims.nposes; ++p) {
+            Footprint fp;
+            footprint<D>(geo[p], dims, x, y, z, support, fp);
+            const float rs = 1.0f / fp.scaling;
+            const ST* sp = sino + (int64_t) p * dims.pix_per_pose;
+            for (int j = fp.lo[1]; j <= fp.hi[1]; ++j) {
+                const float d1 = D == 3 ? (fp.center[1] - (float) j) * rs : 0.0f;
+                for (int i = fp.lo[0]; i <= fp.hi[0]; ++i) {
+                    const float d0 = (fp.center[0] - (float) i) * rs;
+                    const float dist = D == 2 ? sqrtf(d0 * d0) : sqrtf(d0 * d0 + d1 * d1);
+                    acc += tap_weight_scaled<D>(lut, n_lut, kidx, dist, d0, diff != 0) *
+                           up(sp[(int64_t) j * dims.det[0] + i]);
+                }
+            }
+        }
+        vol[vox] = down<ST>(acc);
+    }
+
+    template <class ST>
+    hipError_t voxel_adjoint_fast(const PoseGeom* geo, const Dims& dims, const float* lut, int n_lut, float support,
+                                  int diff, const ST* sino, ST* vol, hipStream_t s)
+    {
+        const dim3 grid((unsigned) ((dims.nvox + 255) / 256));
+        if (dims.dim == 2)
+            hipLaunchKernelGGL((voxel_adj_fast_kernel<2, ST>), grid, dim3(256), 0, s, geo, dims, lut, n_lut, support,
+                               diff, sino, vol);
+        else
+            hipLaunchKernelGGL((voxel_adj_fast_kernel<3, ST>), grid, dim3(256), 0, s, geo, dims, lut, n_lut, support,
+                               diff, sino, vol);
+        return hipGetLastError();
+    }
+
+    template hipError_t voxel_adjoint_fast<float>(const PoseGeom*, const Dims&, const float*, int, float, int,
+                                                  const float*, float*, hipStream_t);
+    template hipError_t voxel_adjoint_fast<bf16>(const PoseGeom*, const Dims&, const float*, int, float, int,
+                                                 const bf16*, bf16*, hipStream_t);
+
     // Forward: one thread per (voxel, pose) scatters into the detector.
     template <int D, class T>
     __global__ void __launch_bounds__(256)
@@ -429,8 +483,6 @@ namespace ehp
     EHP_INSTANTIATE(double)
     template hipError_t launch_fill<int>(int*, int64_t, int, hipStream_t);
     template hipError_t launch_fill<bf16>(bf16*, int64_t, bf16, hipStream_t);
-    template hipError_t voxel_adjoint<float, bf16>(const PoseGeom*, const Dims&, const float*, int, float, int,
-                                                   const bf16*, bf16*, hipStream_t);
     template hipError_t launch_fill<unsigned int>(unsigned int*, int64_t, unsigned int, hipStream_t);
 #undef EHP_INSTANTIATE
 } // namespace ehp

@@ -143,10 +143,14 @@ namespace ehp
     template <class T>
     hipError_t voxel_forward(const PoseGeom* geo, const Dims& dims, const T* lut, int n_lut, float support,
                              int diff, const T* vol, T* sino, hipStream_t s);
-    // ST: the storage type of sinogram and volume (T, or bfloat16 with T = float)
+    // elsa's tap arithmetic (a division per tap): the one-thread-per-voxel kernels and FP64
     template <class T, class ST = T>
     hipError_t voxel_adjoint(const PoseGeom* geo, const Dims& dims, const T* lut, int n_lut, float support,
                              int diff, const ST* sino, ST* vol, hipStream_t s);
+    // the default FP32 gather, with the tiled forward's tap arithmetic (ST: float or bfloat16)
+    template <class ST>
+    hipError_t voxel_adjoint_fast(const PoseGeom* geo, const Dims& dims, const float* lut, int n_lut, float support,
+                                  int diff, const ST* sino, ST* vol, hipStream_t s);
     hipError_t voxel_count(const PoseGeom* geo, const Dims& dims, float support, unsigned long long* total,
                            hipStream_t s);
 

@@ -111,6 +111,26 @@ namespace ehp
         return w * (T) d0;
     }
 
+    // tap_weight with the table index formed as distance * k, k = (n - 1) / support (the tiled
+    // forward's form: one multiplication instead of a division per tap)
+    template <int D>
+    EHP_HD inline float tap_weight_scaled(const float* __restrict__ lut, int n, float k, float dist, float d0,
+                                          bool diff)
+    {
+        const float index = dist * k;
+        float w = 0.0f;
+        if (index >= 0.0f && index <= (float) (n - 1)) {
+            const int a = (int) floorf(index);
+            const int b = (int) ceilf(index);
+            const float t = index - (float) a;
+            w = t * lut[b] + (1.0f - t) * lut[a];
+        }
+        if (!diff)
+            return w;
+        if (D == 2)
+            return d0 > 0.0f ? w : (d0 < 0.0f ? -w : 0.0f);
+        return w * d0;
+    }
 
     // Largest footprint scaling of a pose over the volume: sdd over the smallest distance from
     // the source to the volume box (scaling is sdd / |ext x| and |ext x| is the distance from

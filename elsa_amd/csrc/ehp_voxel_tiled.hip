@@ -54,6 +54,7 @@ namespace ehp
             s_lut[i] = lut[i];
         const int eg = fixexp[0];
         const float inv_scale = 1.0f / lutscale; // a power of two: exact
+        const float lut_index_scale = (float) (n_lut - 1) / support;
         const PoseGeom& g = geo[pose];
         const int A = pose_axis[pose];
         float cdir[12];
@@ -91,22 +92,21 @@ namespace ehp
                 float ys;
                 int sh;
                 ray_scale(xv * lutscale, eg, ys, sh);
-                for (int j = lo1; j <= hi1; ++j)
+                // the distance as visit() forms it, with one reciprocal of the scaling per voxel
+                // instead of a division per tap (and the table index as dist * (n - 1) / support):
+                // within a few ulp of the gather's weights
+                const float rs = 1.0f / fp.scaling;
+                for (int j = lo1; j <= hi1; ++j) {
+                    const float d1 = D == 3 ? (fp.center[1] - (float) j) * rs : 0.0f;
                     for (int i = lo0; i <= hi0; ++i) {
-                        // the distance as visit() forms it
-                        const float d0 = (fp.center[0] - (float) i) / fp.scaling;
-                        float dist;
-                        if (D == 2) {
-                            dist = sqrtf(d0 * d0);
-                        } else {
-                            const float d1 = (fp.center[1] - (float) j) / fp.scaling;
-                            dist = sqrtf(d0 * d0 + d1 * d1);
-                        }
-                        const float w = tap_weight<D, float>(s_lut, n_lut, support, dist, d0, diff != 0);
+                        const float d0 = (fp.center[0] - (float) i) * rs;
+                        const float dist = D == 2 ? sqrtf(d0 * d0) : sqrtf(d0 * d0 + d1 * d1);
+                        const float w = tap_weight_scaled<D>(s_lut, n_lut, lut_index_scale, dist, d0, diff != 0);
                         if (w != 0.0f)
                             atomicAdd(reinterpret_cast<unsigned long long*>(acc + (j - v0) * TU + (i - u0)),
                                       fixed_contrib(ys * (w * inv_scale), sh));
                     }
+                }
             }
         }
         __syncthreads();

@@ -121,17 +121,16 @@ namespace
                             float ys;
                             int sh;
                             ray_scale(xv * ls, eg, ys, sh);
+                            // the GPU's arithmetic: one reciprocal of the scaling per voxel, the
+                            // table index as dist * (n - 1) / support
+                            const float rs = 1.0f / fp.scaling;
+                            const float kidx = (float) (n_lut - 1) / support;
                             for (int j = lo1; j <= hi1; ++j)
                                 for (int i = lo0; i <= hi0; ++i) {
-                                    const float d0 = (fp.center[0] - (float) i) / fp.scaling;
-                                    float dist;
-                                    if (D == 2) {
-                                        dist = sqrtf(d0 * d0);
-                                    } else {
-                                        const float d1 = (fp.center[1] - (float) j) / fp.scaling;
-                                        dist = sqrtf(d0 * d0 + d1 * d1);
-                                    }
-                                    const float w = lut_eval<float>(lut, n_lut, support, dist);
+                                    const float d0 = (fp.center[0] - (float) i) * rs;
+                                    const float d1 = D == 3 ? (fp.center[1] - (float) j) * rs : 0.0f;
+                                    const float dist = D == 2 ? sqrtf(d0 * d0) : sqrtf(d0 * d0 + d1 * d1);
+                                    const float w = tap_weight_scaled<D>(lut, n_lut, kidx, dist, d0, false);
                                     if (w == 0.0f)
                                         continue;
                                     const float z = ys * (w * inv_scale);

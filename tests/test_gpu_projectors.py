@@ -187,7 +187,14 @@ def test_voxel_parity(name, kind, kernels, ops):
     else:
         op = ops("BSplineVoxelProjectorCUDA", name, **kw)
         lut, sup = oracle.bspline_lut(g.dim, 2, 100, np.float32), 1.5
-    np.testing.assert_array_equal(op.applyAdjoint(y), oracle.voxel(g, y, lut, sup, adjoint=True))
+    adj_ref = oracle.voxel(g, y, lut, sup, adjoint=True)
+    if kernels == "simple":
+        # elsa's tap arithmetic in elsa's order: the oracle bit for bit
+        np.testing.assert_array_equal(op.applyAdjoint(y), adj_ref)
+    else:
+        # the tiled forward's tap arithmetic (a reciprocal of the scaling per voxel and pose):
+        # within a few ulp per weight
+        assert rel_l2(op.applyAdjoint(y), adj_ref) <= 1e-6, rel_l2(op.applyAdjoint(y), adj_ref)
     fwd = op.apply(x)
     ref = oracle.voxel(g, x, lut, sup)
     assert rel_l2(fwd, ref) <= 1e-6, rel_l2(fwd, ref)
@@ -215,13 +222,21 @@ def test_phase_contrast_parity(name, kind, kernels, ops):
         op = ops("PhaseContrastBSplineVoxelProjectorCUDA", name, **kw)
         lut, sup = oracle.bspline_derivative_lut(g.dim, 2, 100, np.float32, normalized=norm), 1.5
     np.testing.assert_array_equal(op.lut, lut)
-    np.testing.assert_array_equal(op.applyAdjoint(y), oracle.voxel(g, y, lut, sup, adjoint=True, differential=True))
-    fwd = op.apply(x)
-    ref = oracle.voxel(g, x, lut, sup, differential=True)
+    adj = op.applyAdjoint(y)
+    adj_ref = oracle.voxel(g, y, lut, sup, adjoint=True, differential=True)
     # the differential weights change sign across the footprint, so the sums cancel: rounding
     # is measured against the sum of the terms' magnitudes (|weight| <= |table| in 2D and
     # <= |table| * support in 3D)
-    mag = oracle.voxel(g, np.abs(x), np.abs(lut), sup) * (sup if g.dim == 3 else 1.0)
+    scale = sup if g.dim == 3 else 1.0
+    if kernels == "simple":
+        np.testing.assert_array_equal(adj, adj_ref)
+    else:
+        adj_mag = oracle.voxel(g, np.abs(y), np.abs(lut), sup, adjoint=True) * scale
+        err = np.linalg.norm(np.asarray(adj, np.float64) - adj_ref) / np.linalg.norm(adj_mag)
+        assert err <= 1e-6, err
+    fwd = op.apply(x)
+    ref = oracle.voxel(g, x, lut, sup, differential=True)
+    mag = oracle.voxel(g, np.abs(x), np.abs(lut), sup) * scale
     err = np.linalg.norm(np.asarray(fwd, np.float64) - ref) / np.linalg.norm(mag)
     assert err <= 1e-6, err
     if kernels == "fast":
