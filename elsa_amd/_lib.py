@@ -27,6 +27,7 @@ EXPORTS = (
     "ehp_geometry_set_pixel_coords", "ehp_curved_detector_coords",
 )
 EHP_SOLVE_GRAPH = 1
+EHP_ADMM_FINITE_DIFFERENCES = 2
 EHP_DIFFERENTIAL = 2
 EHP_BF16 = 2
 

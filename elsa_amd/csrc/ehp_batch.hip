@@ -218,6 +218,11 @@ namespace
         A.nseg = (int) b->ops.size();
         A.doff = b->vol_off.data();
         A.roff = b->sino_off.data();
+        for (const ehp_op* op : b->ops) {
+            A.shapes.push_back(op->dims.dim);
+            for (int i = 0; i < 3; ++i)
+                A.shapes.push_back(op->dims.vol[i]);
+        }
         A.fwd = [b](const void* x, void* y, hipStream_t s) { return batch_apply(b, x, y, s, false); };
         A.adj = [b](const void* y, void* x, hipStream_t s) { return batch_apply(b, y, x, s, true); };
         return A;
@@ -360,16 +365,14 @@ int ehp_batch_admml2(ehp_batch* b, const void* d_b, void* d_x, int use_x0, int i
     int rc = solver_args(b, d_b, d_x, iterations);
     if (rc != EHP_OK)
         return rc;
-    if (flags & ~EHP_SOLVE_GRAPH)
+    if (flags & ~(EHP_SOLVE_GRAPH | EHP_ADMM_FINITE_DIFFERENCES))
         return bfail(EHP_ERR_ARGUMENT, "unknown solver flags 0x%x", flags);
     if (ninner < 0)
         return bfail(EHP_ERR_ARGUMENT, "negative inner iteration count %d", ninner);
-    const double t = tau > 0.0 ? tau : 0.9;
-    if (tau_used)
-        *tau_used = t;
     const ehp::SolveOps A = solve_ops(b);
-    return ehp::solve_admml2(A, b->dtype, d_b, d_x, use_x0, iterations, t, l1_sigma, ninner,
-                             (flags & EHP_SOLVE_GRAPH) != 0, residuals, (hipStream_t) stream);
+    return ehp::solve_admml2(A, b->dtype, d_b, d_x, use_x0, iterations, tau, l1_sigma, ninner,
+                             (flags & EHP_ADMM_FINITE_DIFFERENCES) != 0, (flags & EHP_SOLVE_GRAPH) != 0, residuals,
+                             tau_used, (hipStream_t) stream);
 }
 
 int ehp_batch_power_iterations(ehp_batch* b, int niters, double* L, void* stream)

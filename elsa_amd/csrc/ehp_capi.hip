@@ -805,6 +805,7 @@ namespace
             ops.nseg = 1;
             ops.doff = doff;
             ops.roff = roff;
+            ops.shapes = {op->dims.dim, op->dims.vol[0], op->dims.vol[1], op->dims.vol[2]};
             ops.fwd = [op](const void* x, void* y, hipStream_t s) { return ehp_forward(op, x, y, s); };
             ops.adj = [op](const void* y, void* x, hipStream_t s) { return ehp_adjoint(op, y, x, s); };
         }
@@ -866,17 +867,14 @@ int ehp_admml2(ehp_op* op, const void* d_b, void* d_x, int use_x0, int iteration
     int rc = solver_args(op, d_b, d_x, iterations);
     if (rc != EHP_OK)
         return rc;
-    if (flags & ~EHP_SOLVE_GRAPH)
+    if (flags & ~(EHP_SOLVE_GRAPH | EHP_ADMM_FINITE_DIFFERENCES))
         return fail(EHP_ERR_ARGUMENT, "unknown solver flags 0x%x", flags);
     if (ninner < 0)
         return fail(EHP_ERR_ARGUMENT, "negative inner iteration count %d", ninner);
-    // elsa's default tau = 0.9 / powerIterations(A^T A), A the identity here (ADMML2.cpp:30-43)
-    const double t = tau > 0.0 ? tau : 0.9;
-    if (tau_used)
-        *tau_used = t;
     SingleOps A(op);
-    return ehp::solve_admml2(A.ops, op->dtype, d_b, d_x, use_x0, iterations, t, l1_sigma, ninner,
-                             (flags & EHP_SOLVE_GRAPH) != 0, residuals, (hipStream_t) stream);
+    return ehp::solve_admml2(A.ops, op->dtype, d_b, d_x, use_x0, iterations, tau, l1_sigma, ninner,
+                             (flags & EHP_ADMM_FINITE_DIFFERENCES) != 0, (flags & EHP_SOLVE_GRAPH) != 0, residuals,
+                             tau_used, (hipStream_t) stream);
 }
 
 int ehp_cgls_refined(ehp_op* op_hi, ehp_op* op_lo, const void* d_b, void* d_x, int use_x0, int outer, int inner,

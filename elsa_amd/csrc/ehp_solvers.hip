@@ -740,8 +740,9 @@ namespace ehp
             }
             return EHP_OK;
         }
-        // ---- ADMML2 with the identity as regularisation operator (ADMML2.cpp:46-97): the
-        // x-update is CGNE (CGNE.cpp) on K = [op; lambda I], K^T w = op^T b + lambda (z - u)
+        // ---- ADMML2 (ADMML2.cpp:46-97) with the identity or forward finite differences as
+        // regularisation operator A: the x-update is CGNE (CGNE.cpp) on K = [op; lambda A],
+        // K^T w = op^T b + lambda A^T (z - u)
 
         // rhs = atb + lambda (z - u), as BlockLinearOperator's adjoint sums the blocks
         template <class T>
@@ -804,6 +805,38 @@ namespace ehp
             z[i] = zi;
             const T ux = u[i] + x[i];
             u[i] = ux - zi;
+        }
+
+        // out = a - b
+        template <class T>
+        __global__ void __launch_bounds__(kThreads)
+            admm_sub_kernel(const T* __restrict__ a, const T* __restrict__ b, T* __restrict__ out, int64_t n)
+        {
+            const int64_t i = (int64_t) blockIdx.x * kThreads + threadIdx.x;
+            if (i < n)
+                out[i] = a[i] - b[i];
+        }
+
+        // v = lambda v (elsa's scalar-times-operator: the scalar applied to the operator's result)
+        template <class T>
+        __global__ void __launch_bounds__(kThreads) admm_scale_kernel(T* __restrict__ v, T lambda, int64_t n)
+        {
+            const int64_t i = (int64_t) blockIdx.x * kThreads + threadIdx.x;
+            if (i < n)
+                v[i] = lambda * v[i];
+        }
+
+        // out = a + lambda v (a block operator's adjoint: the blocks' adjoints summed in order)
+        template <class T>
+        __global__ void __launch_bounds__(kThreads)
+            admm_axpy_kernel(const T* __restrict__ a, T lambda, const T* __restrict__ v, T* __restrict__ out,
+                             int64_t n)
+        {
+            const int64_t i = (int64_t) blockIdx.x * kThreads + threadIdx.x;
+            if (i < n) {
+                const T lv = lambda * v[i];
+                out[i] = a[i] + lv;
+            }
         }
 
         // r = b - q formed in FP64 and rounded to bfloat16, and partial sums of its square (the
@@ -894,10 +927,59 @@ namespace ehp
 
     namespace
     {
+        // frees the finite-difference plan when the solve (declared after it) has drained
+        struct FdGuard {
+            FdPlan plan;
+            ~FdGuard()
+            {
+                fd_plan_destroy(plan);
+            }
+        };
+
         template <class T>
         int admml2(const SolveOps& A, const T* b, T* x, int use_x0, int iterations, double tau, double sigma,
-                   int ninner, bool graph, double* residuals, hipStream_t caller)
+                   int ninner, bool fd, bool graph, double* residuals, double* tau_used, hipStream_t caller)
         {
+            FdGuard reg;
+            const int nseg0 = A.nseg;
+            if (fd) {
+                if ((int) A.shapes.size() != 4 * nseg0)
+                    return ehp_solver_fail(EHP_ERR_ARGUMENT, "finite differences need the volume shapes");
+                const int rc0 = fd_plan_create(A.shapes.data(), A.doff, nseg0, reg.plan);
+                if (rc0 != EHP_OK)
+                    return ehp_solver_fail(rc0, "finite-difference plan");
+            }
+            // elsa: tau = 0.9 * (1 / powerIterations(A^T A)) (ADMML2.cpp:30-43): 0.9 for the
+            // identity; for finite differences from 5 power iterations on the device
+            T ttau;
+            if (tau > 0.0) {
+                ttau = (T) tau;
+            } else if (!fd) {
+                ttau = (T) 0.9;
+            } else {
+                if (nseg0 > 1)
+                    return ehp_solver_fail(EHP_ERR_ARGUMENT,
+                                           "ADMML2 with finite differences on several problems needs tau > 0");
+                SolveOps D;
+                D.nseg = nseg0;
+                D.doff = A.doff;
+                D.roff = reg.plan.reg_off.data();
+                const FdPlan* pl = &reg.plan;
+                D.fwd = [pl](const void* in, void* out, hipStream_t s2) {
+                    return fd_apply<T>(*pl, (const T*) in, (T*) out, false, s2) == hipSuccess ? EHP_OK : EHP_ERR_HIP;
+                };
+                D.adj = [pl](const void* in, void* out, hipStream_t s2) {
+                    return fd_apply<T>(*pl, (const T*) in, (T*) out, true, s2) == hipSuccess ? EHP_OK : EHP_ERR_HIP;
+                };
+                double L = 0.0;
+                const int rc0 =
+                    solve_power_iterations(D, sizeof(T) == 8 ? EHP_F64 : EHP_F32, 5, &L, caller);
+                if (rc0 != EHP_OK)
+                    return rc0;
+                ttau = (T) (0.9 * (double) (T(1) / (T) L));
+            }
+            if (tau_used)
+                *tau_used = (double) ttau;
             Solve sv;
             int rc = sv.init(A, caller, "hipMalloc of the ADMML2 workspace failed");
             if (rc != EHP_OK)
@@ -905,41 +987,68 @@ namespace ehp
             hipStream_t s = sv.ss.s;
             const int nseg = sv.nseg;
             const int64_t nd = sv.nd, nr = sv.nr;
+            const int64_t nz = fd ? reg.plan.nreg : nd; // the regulariser's range: z, u
             const SegSpace &D = sv.D, &R = sv.R;
             Scratch& m = sv.m;
-            T *z, *u, *atb, *rhs, *r, *c, *ac, *q;
+            T *z, *u, *atb, *rhs, *r, *c, *ac, *q, *t = nullptr, *w = nullptr;
             const int nh = iterations + 1;
-            if (m.alloc(&z, nd * sizeof(T)) != hipSuccess || m.alloc(&u, nd * sizeof(T)) != hipSuccess ||
+            if (m.alloc(&z, nz * sizeof(T)) != hipSuccess || m.alloc(&u, nz * sizeof(T)) != hipSuccess ||
                 m.alloc(&atb, nd * sizeof(T)) != hipSuccess || m.alloc(&rhs, nd * sizeof(T)) != hipSuccess ||
                 m.alloc(&r, nd * sizeof(T)) != hipSuccess || m.alloc(&c, nd * sizeof(T)) != hipSuccess ||
-                m.alloc(&ac, nd * sizeof(T)) != hipSuccess || m.alloc(&q, nr * sizeof(T)) != hipSuccess)
+                m.alloc(&ac, nd * sizeof(T)) != hipSuccess || m.alloc(&q, nr * sizeof(T)) != hipSuccess ||
+                (fd && (m.alloc(&t, nz * sizeof(T)) != hipSuccess || m.alloc(&w, nd * sizeof(T)) != hipSuccess)))
                 return ehp_solver_fail(EHP_ERR_ALLOC, "hipMalloc of the ADMML2 workspace failed");
             // elsa: lambda = 1 / sqrt(tau) ("sqrttau", ADMML2.cpp:70), in the data type
-            const T ttau = (T) tau, lambda = T(1) / std::sqrt(ttau), tsig = sigma >= 0.0 ? (T) sigma : T(-1);
+            const T lambda = T(1) / std::sqrt(ttau), tsig = sigma >= 0.0 ? (T) sigma : T(-1);
             bool okf = true;
             if (!use_x0)
                 okf = launch_fill<T>(x, nd, T(0), s) == hipSuccess;
-            okf = okf && launch_fill<T>(z, nd, T(0), s) == hipSuccess && launch_fill<T>(u, nd, T(0), s) == hipSuccess;
+            okf = okf && launch_fill<T>(z, nz, T(0), s) == hipSuccess && launch_fill<T>(u, nz, T(0), s) == hipSuccess;
             if (!okf)
                 return ehp_solver_fail(EHP_ERR_HIP, "fill");
             rc = A.adj(b, atb, s); // op^T b, the same every iteration
             if (rc != EHP_OK)
                 return rc;
-            // K^T K v = op^T op v + lambda^2 v into out
+            auto regop = [&](const T* in, T* out, bool adjoint) -> int {
+                return fd_apply<T>(reg.plan, in, out, adjoint, s) == hipSuccess ? EHP_OK : EHP_ERR_HIP;
+            };
+            // K^T K v = op^T op v + lambda A^T (lambda A v) into out
             auto normal = [&](const T* v, T* out) -> int {
                 int r2 = A.fwd(v, q, s);
                 if (r2 == EHP_OK)
                     r2 = A.adj(q, out, s);
-                if (r2 == EHP_OK)
+                if (r2 != EHP_OK)
+                    return r2;
+                if (!fd) {
                     hipLaunchKernelGGL((admm_normal_kernel<T>), dim3(blocks(nd)), dim3(kThreads), 0, s, out, v, lambda,
                                        nd);
+                    return EHP_OK;
+                }
+                r2 = regop(v, t, false);
+                if (r2 != EHP_OK)
+                    return r2;
+                hipLaunchKernelGGL((admm_scale_kernel<T>), dim3(blocks(nz)), dim3(kThreads), 0, s, t, lambda, nz);
+                r2 = regop(t, w, true);
+                if (r2 == EHP_OK)
+                    hipLaunchKernelGGL((admm_axpy_kernel<T>), dim3(blocks(nd)), dim3(kThreads), 0, s, out, lambda, w,
+                                       out, nd);
                 return r2;
             };
             auto body = [&]() -> int {
                 // x = CGNE(K, [b; z - u], ninner, x) (RegularizedInversion.cpp, CGNE.cpp)
-                hipLaunchKernelGGL((admm_rhs_kernel<T>), dim3(blocks(nd)), dim3(kThreads), 0, s, atb, z, u, lambda, rhs,
-                                   nd);
-                int r2 = normal(x, ac);
+                int r2 = EHP_OK;
+                if (!fd) {
+                    hipLaunchKernelGGL((admm_rhs_kernel<T>), dim3(blocks(nd)), dim3(kThreads), 0, s, atb, z, u, lambda,
+                                       rhs, nd);
+                } else {
+                    hipLaunchKernelGGL((admm_sub_kernel<T>), dim3(blocks(nz)), dim3(kThreads), 0, s, z, u, t, nz);
+                    r2 = regop(t, w, true);
+                    if (r2 != EHP_OK)
+                        return r2;
+                    hipLaunchKernelGGL((admm_axpy_kernel<T>), dim3(blocks(nd)), dim3(kThreads), 0, s, atb, lambda, w,
+                                       rhs, nd);
+                }
+                r2 = normal(x, ac);
                 if (r2 != EHP_OK)
                     return r2;
                 hipLaunchKernelGGL((cgne_start_kernel<T>), dim3(blocks(nd)), dim3(kThreads), 0, s, rhs, ac, r, c, nd);
@@ -962,8 +1071,15 @@ namespace ehp
                     hipLaunchKernelGGL((cgls_direction_kernel<T>), dim3((unsigned) D.neb), dim3(kThreads), 0, s, r, c, D,
                                        sv.sc);
                 }
-                // z = prox(x + u, tau); u += x; u -= z (ADMML2.cpp:77-85)
-                hipLaunchKernelGGL((admm_zu_kernel<T>), dim3(blocks(nd)), dim3(kThreads), 0, s, x, z, u, ttau, tsig, nd);
+                // z = prox(A x + u, tau); u += A x; u -= z (ADMML2.cpp:74-85)
+                const T* ax = x;
+                if (fd) {
+                    r2 = regop(x, t, false);
+                    if (r2 != EHP_OK)
+                        return r2;
+                    ax = t;
+                }
+                hipLaunchKernelGGL((admm_zu_kernel<T>), dim3(blocks(nz)), dim3(kThreads), 0, s, ax, z, u, ttau, tsig, nz);
                 return hipGetLastError() == hipSuccess ? EHP_OK : EHP_ERR_HIP;
             };
             if (!residuals) {
@@ -998,14 +1114,14 @@ namespace ehp
     } // namespace
 
     int solve_admml2(const SolveOps& A, int dtype, const void* b, void* x, int use_x0, int iterations, double tau,
-                     double sigma, int ninner, bool graph, double* residuals, hipStream_t s)
+                     double sigma, int ninner, bool fd, bool graph, double* residuals, double* tau_used, hipStream_t s)
     {
         if (dtype == EHP_F64)
-            return admml2<double>(A, (const double*) b, (double*) x, use_x0, iterations, tau, sigma, ninner, graph,
-                                  residuals, s);
+            return admml2<double>(A, (const double*) b, (double*) x, use_x0, iterations, tau, sigma, ninner, fd,
+                                  graph, residuals, tau_used, s);
         if (dtype == EHP_F32)
-            return admml2<float>(A, (const float*) b, (float*) x, use_x0, iterations, tau, sigma, ninner, graph,
-                                 residuals, s);
+            return admml2<float>(A, (const float*) b, (float*) x, use_x0, iterations, tau, sigma, ninner, fd, graph,
+                                 residuals, tau_used, s);
         return ehp_solver_fail(EHP_ERR_UNSUPPORTED, "ADMML2 needs an FP32 or FP64 operator");
     }
 

@@ -149,17 +149,23 @@ class APGD:
 
 
 class ADMML2:
-    """ADMM with an L2 data term, 0.5 ||op x - b||^2 + g(x), for elsa's ADMML2 with the identity as
-    regularisation operator (ADMML2.h:62-64, ADMML2.cpp:46-97): ninneriters CGNE iterations per
-    x-update, prox ProximalL1(sigma) or ProximalIdentity(), tau None for elsa's default (0.9
-    with the identity). After solve(), `residuals` holds ||b - op x|| at the start and after each
-    iteration when record=True (the iterations then run without graph capture)."""
+    """ADMM with an L2 data term, 0.5 ||op x - b||^2 + g(A x), for elsa's ADMML2 (ADMML2.h:62-64,
+    ADMML2.cpp:20-97) with regularisation operator A the identity (regularizer=None) or forward
+    finite differences on every axis (regularizer="finite_differences", elsa's
+    FiniteDifferences(domain); with ProximalL1 the anisotropic TV problem): ninneriters CGNE
+    iterations per x-update, prox ProximalL1(sigma) or ProximalIdentity(), tau None for elsa's
+    default 0.9 / powerIterations(A^T A) (0.9 with the identity; a batch with finite differences
+    needs tau). After solve(), `tau_used` holds tau and `residuals` ||b - op x|| at the start and
+    after each iteration when record=True (the iterations then run without graph capture)."""
 
-    def __init__(self, op, b, prox=None, tau=None, ninneriters=5, graph=False, record=False):
+    def __init__(self, op, b, prox=None, tau=None, ninneriters=5, graph=False, record=False, regularizer=None):
         self.A, self.b, self.graph, self.record = op, b, bool(graph), bool(record)
         self.prox = prox if prox is not None else ProximalIdentity()
         self.tau = float(tau) if tau is not None else 0.0
         self.ninner = int(ninneriters)
+        if regularizer not in (None, "identity", "finite_differences"):
+            raise ValueError(f"unknown regularizer {regularizer!r} (None, 'identity' or 'finite_differences')")
+        self.fd = regularizer == "finite_differences"
         self.residuals = None
         self.tau_used = None
 
@@ -169,7 +175,7 @@ class ADMML2:
         nseg = len(self.A) if batch else 1
         res = np.empty((nseg, iterations + 1), np.float64) if self.record else None
         tau = ctypes.c_double()
-        flags = _lib.EHP_SOLVE_GRAPH if self.graph else 0
+        flags = (_lib.EHP_SOLVE_GRAPH if self.graph else 0) | (_lib.EHP_ADMM_FINITE_DIFFERENCES if self.fd else 0)
         fn = self.A._L.ehp_batch_admml2 if batch else self.A._L.ehp_admml2
         dp = ctypes.POINTER(ctypes.c_double)
         _lib.check(fn(self.A.handle, ctypes.c_void_p(bt.data_ptr()), ctypes.c_void_p(xt.data_ptr()),

@@ -163,7 +163,10 @@ int ehp_joseph_ray_info(const ehp_op* op, int32_t* axis, int64_t* nsteps);
  * `stream` and returns when the solve has finished. x is overwritten with the result
  * (it is the start value when use_x0 != 0). */
 enum ehp_solve_flags {
-    EHP_SOLVE_GRAPH = 1
+    EHP_SOLVE_GRAPH = 1,
+    /* ehp_admml2 / ehp_batch_admml2 only: the regularisation operator A is elsa's
+     * FiniteDifferences(domain) (forward differences on every axis) instead of the identity */
+    EHP_ADMM_FINITE_DIFFERENCES = 2
 };
 
 /* CGLS on min ||Ax - b||^2 + eps^2 ||x||^2 (elsa: CGLS(A, b, eps, tol).solve(iterations, x0),
@@ -183,13 +186,19 @@ int ehp_apgd(ehp_op* op, const void* d_b, void* d_x, int use_x0, int iterations,
              double epsilon, int flags, double* grad_sq_norms, int* iterations_done, double* mu_used, void* stream);
 
 /* ADMM with an L2 data term (elsa: ADMML2(op, b, A, proxg, tau, ninneriters).solve(iterations,
- * x0), ADMML2.cpp:20-97) for the regularisation operator A = identity: per iteration the x-update
- * runs ninner CGNE iterations (CGNE.cpp, via RegularizedInversion.cpp) on [op; lambda I] with
- * right-hand side [b; z - u] and lambda = 1 / sqrt(tau), starting from the current x; then
- * z = prox(x + u) with threshold tau * l1_sigma (ProximalL1; l1_sigma < 0: identity) and
- * u += x - z. tau <= 0 takes elsa's default 0.9 / powerIterations(A^T A) = 0.9; tau_used
- * receives it. residuals (host, iterations + 1 or NULL) receives ||b - op x|| (FP64) at the
- * start and after each iteration; asking for it runs the iterations without graph capture. */
+ * x0), ADMML2.cpp:20-97) for the regularisation operator A = identity, or with
+ * EHP_ADMM_FINITE_DIFFERENCES A = forward finite differences on every axis (elsa:
+ * FiniteDifferences(domain), FiniteDifferences.cpp:106-206; the anisotropic TV problem with
+ * ProximalL1). Per iteration the x-update runs ninner CGNE iterations (CGNE.cpp, via
+ * RegularizedInversion.cpp) on [op; lambda A] with right-hand side [b; z - u] and
+ * lambda = 1 / sqrt(tau), starting from the current x; then z = prox(A x + u) with threshold
+ * tau * l1_sigma (ProximalL1; l1_sigma < 0: identity) and u += A x - z. z and u live in A's
+ * range (dim * nvox values per problem for finite differences). tau <= 0 takes elsa's default
+ * 0.9 / powerIterations(A^T A): 0.9 for the identity, 5 power iterations on the device for
+ * finite differences (one problem only: a batch with finite differences needs tau > 0);
+ * tau_used receives it. residuals (host, iterations + 1 or NULL) receives ||b - op x|| (FP64)
+ * at the start and after each iteration; asking for it runs the iterations without graph
+ * capture. */
 int ehp_admml2(ehp_op* op, const void* d_b, void* d_x, int use_x0, int iterations, double tau, double l1_sigma,
                int ninner, int flags, double* residuals, double* tau_used, void* stream);
 

@@ -448,23 +448,65 @@ def power_iterations(A, At, shape, niters=5, dtype=np.float64):
     return _dot(u, At(A(u))) / np.sqrt(_dot(u, u))
 
 
-def admml2(A, At, b, shape, iterations, tau=0.9, sigma=None, ninner=5, x0=None, dtype=np.float64):
-    """ADMML2.cpp:46-97 with the identity as regularisation operator: x = CGNE(K = [A; lam I],
-    w = [b; z - u], ninner, x) (RegularizedInversion.cpp:11-61, CGNE.cpp:22-69) with lam =
-    1/sqrt(tau); z = prox(x + u) (soft thresholding of threshold tau * sigma, or the identity
-    when sigma is None); u += x; u -= z. Returns (x, ||b - A x|| at the start and after each
-    iteration)."""
+def finite_differences(x, dims, adjoint=False):
+    """elsa's FiniteDifferences(domain) with DiffType::FORWARD and every axis active.
+    dims: voxels per axis, axis 0 fastest (elsa's order); x flat. Forward
+    (FiniteDifferences.cpp:106-155): one block of nvox values per axis, axis 0 first, block d
+    at voxel i = -x[i] + x[i + e_d], -x[i] on the last plane of axis d. Adjoint: per voxel, axis
+    by axis in that order, + y_d[i - e_d] (not on the first plane) - y_d[i] -- the transpose of
+    the forward. (elsa's applyAdjointHelper, FiniteDifferences.cpp:157-206, reads block d at
+    offset d * _coordDelta[d], which is d * nvox only for the last axis: its 3D adjoint takes
+    axis 1's block from offset n0 * n1. This restatement uses nvox, the transpose.)"""
+    dims = tuple(int(n) for n in dims)
+    D, nvox = len(dims), int(np.prod(dims))
+    shp = dims[::-1]                                   # C order: axis 0 is the last numpy axis
+    if not adjoint:
+        v = np.asarray(x).reshape(shp)
+        out = np.empty((D,) + shp, v.dtype)
+        for d in range(D):
+            ax = D - 1 - d
+            y = -v                                     # :136
+            lo = [slice(None)] * D
+            hi = [slice(None)] * D
+            lo[ax], hi[ax] = slice(0, -1), slice(1, None)
+            y[tuple(lo)] += v[tuple(hi)]               # :137-138
+            out[d] = y
+        return out.reshape(-1)
+    y = np.asarray(x).reshape((D,) + shp)
+    acc = np.zeros(shp, y.dtype)
+    for d in range(D):
+        ax = D - 1 - d
+        lo = [slice(None)] * D
+        hi = [slice(None)] * D
+        lo[ax], hi[ax] = slice(0, -1), slice(1, None)
+        acc[tuple(hi)] += y[d][tuple(lo)]              # :185-186
+        acc -= y[d]                                    # :188
+    return acc.reshape(-1)
+
+
+def admml2(A, At, b, shape, iterations, tau=0.9, sigma=None, ninner=5, x0=None, dtype=np.float64, reg=None):
+    """ADMML2.cpp:46-97: x = CGNE(K = [A; lam R], w = [b; z - u], ninner, x)
+    (RegularizedInversion.cpp:11-61, CGNE.cpp:22-69) with lam = 1/sqrt(tau) and R the identity
+    (reg None) or reg = (R, Rt) callables on flat arrays; z = prox(R x + u) (soft thresholding of
+    threshold tau * sigma, or the identity when sigma is None); u += R x; u -= z. Returns
+    (x, ||b - A x|| at the start and after each iteration)."""
     t = dtype(tau)
     lam = dtype(1) / np.sqrt(t)
     b = np.asarray(b, dtype)
     x = np.zeros(shape, dtype) if x0 is None else np.array(x0, dtype).reshape(shape)
-    z = np.zeros(shape, dtype)
-    u = np.zeros(shape, dtype)
+    if reg is None:
+        R = Rt = lambda v: v
+        ktk = lambda v: At(A(v)) + lam * (lam * v)
+    else:
+        R = lambda v: reg[0](v.reshape(-1))
+        Rt = lambda y: reg[1](y).reshape(shape)
+        ktk = lambda v: At(A(v)) + lam * Rt(lam * R(v))
+    z = np.zeros_like(R(x))
+    u = np.zeros_like(z)
     atb = At(b)
-    ktk = lambda v: At(A(v)) + lam * (lam * v)
     res = [np.sqrt(_dot(b - A(x), b - A(x)))]
     for _ in range(iterations):
-        rhs = atb + lam * (z - u)
+        rhs = atb + lam * Rt(z - u)
         r = rhs - ktk(x)
         c = r.copy()
         kold = _dot(r, r)
@@ -476,14 +518,15 @@ def admml2(A, At, b, shape, iterations, tau=0.9, sigma=None, ninner=5, x0=None, 
             k = _dot(r, r)
             c = r + dtype(k / kold) * c
             kold = k
-        v = x + u
+        rx = R(x)
+        v = rx + u
         if sigma is None:
             z = v
         else:
             tmp = np.abs(v) - t * dtype(sigma)
             tmp = dtype(0.5) * (tmp + np.abs(tmp))
             z = np.sign(v) * tmp
-        u = (u + x) - z
+        u = (u + rx) - z
         res.append(np.sqrt(_dot(b - A(x), b - A(x))))
     return x, np.array(res)
 

@@ -502,3 +502,19 @@ def pc_bspline_single_pixel_2d(B):
 
 PHASE_CONTRAST_KATS = [pc_blob_two_pixels_2d, pc_bspline_two_pixels_2d, pc_blob_three_pixels_2d,
                        pc_bspline_three_pixels_2d, pc_blob_single_pixel_2d, pc_bspline_single_pixel_2d]
+
+
+# ---- FiniteDifferences, forward differences on every axis (elsa/operators/tests/
+# test_FiniteDifferences.cpp): (voxels per axis, axis 0 first; data; expected apply; expected
+# applyAdjoint of that apply's output, or None where elsa's test has none)
+FD_KATS = [
+    ((5,), [30, 3, 2, -1, 7], [-27, -1, -3, 8, -7], [27, -26, 2, -11, 15]),                 # :59-89
+    ((4, 4), [16, 5, 9, 4, 2, 11, 7, 14, 3, 10, 6, 15, 13, 8, 12, 1],                          # :139-161
+     [-11, 4, -5, -4, 9, -4, 7, -14, 7, -4, 9, -15, -5, 4, -11, -1,
+      -14, 6, -2, 10, 1, -1, -1, 1, 10, -2, 6, -14, -13, -8, -12, -1], None),
+    ((2, 2, 2), [1, 2, 3, 4, 5, 6, 7, 8],                                                       # :295-318
+     [1, -2, 1, -4, 1, -6, 1, -8, 2, 2, -3, -4, 2, 2, -7, -8, 4, 4, 4, 4, -5, -6, -7, -8], None),
+    ((2, 3, 2), [6, 3, 5, 8, 7, 5, 1, 0, 3, 10, -4, 1],                                         # :321-343
+     [-3, -3, 3, -8, -2, -5, -1, -0, 7, -10, 5, -1, -1, 5, 2, -3, -7, -5, 2, 10,
+      -7, -9, 4, -1, -5, -3, -2, 2, -11, -4, -1, -0, -3, -10, 4, -1], None),
+]

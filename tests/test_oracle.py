@@ -427,6 +427,62 @@ def test_admml2_closed_forms():
     np.testing.assert_array_equal(x2, x0)
 
 
+@pytest.mark.parametrize("case", range(len(kats.FD_KATS)))
+def test_finite_differences_kats(case):
+    """elsa's FiniteDifferences test vectors (forward differences, every axis active), and the
+    adjoint vector of its 1D test."""
+    dims, data, fwd, adj = kats.FD_KATS[case]
+    y = oracle.finite_differences(np.array(data, np.float64), dims)
+    np.testing.assert_array_equal(y, np.array(fwd, np.float64))
+    if adj is not None:
+        np.testing.assert_array_equal(oracle.finite_differences(y, dims, adjoint=True), np.array(adj, np.float64))
+
+
+@pytest.mark.parametrize("dims", [(7,), (4, 5), (3, 4, 5), (1, 6), (2, 1, 3)])
+def test_finite_differences_adjoint_is_transpose(dims):
+    """The adjoint is the transpose of the forward matrix, built column by column from unit
+    vectors (in 3D this is where elsa's adjoint, which reads axis 1's block at offset n0 * n1,
+    is not); constants lie in the null space; the forward is linear."""
+    n = int(np.prod(dims))
+    F = np.stack([oracle.finite_differences(np.eye(n)[i], dims) for i in range(n)], axis=1)
+    G = np.stack([oracle.finite_differences(np.eye(len(dims) * n)[i], dims, adjoint=True)
+                  for i in range(len(dims) * n)], axis=1)
+    np.testing.assert_array_equal(G, F.T)
+    assert set(np.unique(F)) <= {-1.0, 0.0, 1.0}
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(n)
+    np.testing.assert_array_equal(F @ x, oracle.finite_differences(x, dims))
+    # the last plane of each axis keeps -x alone: F @ 1 is -1 there and 0 elsewhere
+    ones = oracle.finite_differences(np.ones(n), dims).reshape((len(dims),) + tuple(dims[::-1]))
+    for d in range(len(dims)):
+        last = np.moveaxis(ones[d], len(dims) - 1 - d, 0)
+        assert np.all(last[-1] == -1.0) and np.all(last[:-1] == 0.0)
+
+
+def test_admml2_finite_differences_closed_form():
+    """ADMML2 with R = finite differences and no prox: u -> 0 and z = R x, so the fixed point
+    solves (A^T A + lam (lam - 1) R^T R) x = A^T b with lam = 1/sqrt(tau) -- least squares at
+    tau = 1. A transposed or unscaled regularisation block fails it."""
+    M, b, A, At = _dense(6)
+    dims = (5, 3)
+    R = lambda v: oracle.finite_differences(v, dims)
+    Rt = lambda y: oracle.finite_differences(y, dims, adjoint=True)
+    F = np.stack([R(np.eye(15)[i]) for i in range(15)], axis=1)
+    for tau in (0.5, 0.8):
+        lam = 1.0 / np.sqrt(tau)
+        ref = np.linalg.solve(M.T @ M + lam * (lam - 1.0) * F.T @ F, M.T @ b)
+        x, _ = oracle.admml2(A, At, b, (15,), 600, tau=tau, ninner=5, reg=(R, Rt))
+        assert np.linalg.norm(x - ref) <= 1e-9 * np.linalg.norm(ref), tau
+    x1, _ = oracle.admml2(A, At, b, (15,), 300, tau=1.0, ninner=5, reg=(R, Rt))
+    np.testing.assert_allclose(x1, np.linalg.lstsq(M, b, rcond=None)[0], rtol=1e-8, atol=1e-10)
+    # with the identity as R the regularised form is the identity form, step for step
+    I = lambda v: v
+    xa, ra = oracle.admml2(A, At, b, (15,), 5, tau=0.7, sigma=0.1, ninner=3, reg=(I, I))
+    xb, rb = oracle.admml2(A, At, b, (15,), 5, tau=0.7, sigma=0.1, ninner=3)
+    np.testing.assert_array_equal(xa, xb)
+    np.testing.assert_array_equal(ra, rb)
+
+
 # --------------------------------------------------------------------------------------
 # phase contrast: derivative tables and the differential weight
 
