@@ -11,8 +11,8 @@
 // Accumulation: LDS float atomics are slow on this part (a ds_add_f32 wave operation costs
 // about 15 times a ds_add_u64), so the tile is summed with 64-bit integer atomics in fixed
 // point: units of q = 2^(eg - 62), eg chosen per call from max|y| and a bound on what one slot
-// can receive, so that no sum can overflow. Each ray rounds its contributions to int32 in its
-// own power-of-two scale (relative error 2^-31 of y * len) and shifts them into units of q,
+// can receive, so that no sum can overflow. Each ray truncates its contributions to int32 in
+// its own power-of-two scale (error below 2^-29 of y * len) and shifts them into units of q,
 // which costs a few instructions instead of a float-to-int64 conversion. Integer sums do not
 // depend on the order of the additions, so the result is bitwise reproducible, and they are
 // more accurate than a sequential FP32 accumulation. A non-finite y makes the whole output
