@@ -542,39 +542,34 @@ namespace ehp
         s1 = s1 < r.nsteps ? s1 : r.nsteps;
     }
 
-    // Sample s (fs = s) of a ray (driving axis A) on the plane at byte offset poff of its padded
-    // view, added to acc; FB2 says that the fast in-plane axis of the view is b2 (else b1). The
-    // interpolation has the order and weights of elsa's doInterpolation (JosephsMethod.cpp:60-97):
-    // 2D taps (F0,C1)(C0,F1), 3D taps (F0,F1,F2)(F0,C1,C2)(C0,F1,C2)(C0,C1,F2), the driving axis'
-    // floor and ceil being its plane; each tap adds (value * len) * weight, elsa's association.
-    // C = F + 1 is read even where elsa's ceil gives C = F: the weight of that tap is 0 then.
-    // Positions are those of joseph_sample and tile_ray_adjoint.
+    // The taps of one sample and their weights (2D: the first two of each).
+    struct SampleTaps {
+        float v[4], w[4];
+    };
+
     template <int D, int A, bool FB2, bool V16>
-    EHP_HD inline float plane_sample_forward(const RayRec& r, const Dims& dims, const PadView& V, uint32_t poff,
-                                             float fs, float acc)
+    EHP_HD inline SampleTaps plane_sample_taps(const RayRec& r, const Dims& dims, const PadView& V, uint32_t poff,
+                                               float fs)
     {
         constexpr uint32_t ES = V16 ? 1u : 2u; // log2 of the element size
         constexpr int b1 = TileAxes<D, A>::b1, b2 = TileAxes<D, A>::b2;
         const int n1 = dims.vol[b1], n2 = (D == 3) ? dims.vol[b2] : 1;
-        const float len = r.len;
         const char* pb = (const char*) V.base + poff; // the plane (the same for the whole wavefront)
         const float q1 = __builtin_fmaf(fs, r.s1, r.p1);
         const float g1 = floorf(q1 - 0.5f);
         const float c1 = (q1 - g1) - 0.5f;
         const float w1 = 1.0f - c1;
         const int k1 = pad_clamp(g1, n1);
+        SampleTaps T;
         if (D == 2) {
             // offset of element k1 + 2 of the plane's only row
             const Pair fk = ld_pair<V16>(V, pb, ((uint32_t) k1 << ES) + (2u << ES));
-            const float vf = fk.a, vk = fk.b;
             // A = 0: (pa, C1) then (pa, F1); A = 1: (F0, pa) then (C0, pa)
-            if (A == 0) {
-                acc += vk * len * c1;
-                acc += vf * len * w1;
-            } else {
-                acc += vf * len * w1;
-                acc += vk * len * c1;
-            }
+            T.v[0] = A == 0 ? fk.b : fk.a;
+            T.w[0] = A == 0 ? c1 : w1;
+            T.v[1] = A == 0 ? fk.a : fk.b;
+            T.w[1] = A == 0 ? w1 : c1;
+            T.v[2] = T.v[3] = T.w[2] = T.w[3] = 0.0f;
         } else {
             const float q2 = __builtin_fmaf(fs, r.s2, r.p2);
             const float g2 = floorf(q2 - 0.5f);
@@ -593,22 +588,47 @@ namespace ehp
             const float vfk = FB2 ? r0.b : r1.a;
             const float vkk = r1.b;
             const float wff = w1 * w2, wkk = c1 * c2, wfk = w1 * c2, wkf = c1 * w2;
-            acc += vff * len * wff;
+            T.v[0] = vff;
+            T.w[0] = wff;
             if (A == 0) {
-                acc += vkk * len * wkk;
-                acc += vfk * len * wfk;
-                acc += vkf * len * wkf;
+                T.v[1] = vkk, T.w[1] = wkk;
+                T.v[2] = vfk, T.w[2] = wfk;
+                T.v[3] = vkf, T.w[3] = wkf;
             } else if (A == 1) {
-                acc += vfk * len * wfk;
-                acc += vkk * len * wkk;
-                acc += vkf * len * wkf;
+                T.v[1] = vfk, T.w[1] = wfk;
+                T.v[2] = vkk, T.w[2] = wkk;
+                T.v[3] = vkf, T.w[3] = wkf;
             } else {
-                acc += vfk * len * wfk;
-                acc += vkf * len * wkf;
-                acc += vkk * len * wkk;
+                T.v[1] = vfk, T.w[1] = wfk;
+                T.v[2] = vkf, T.w[2] = wkf;
+                T.v[3] = vkk, T.w[3] = wkk;
             }
         }
+        return T;
+    }
+
+    // acc plus the sample's taps in order, each as (value * len) * weight
+    template <int D>
+    EHP_HD inline float plane_taps_add(const SampleTaps& T, float len, float acc)
+    {
+#pragma unroll
+        for (int k = 0; k < (D == 2 ? 2 : 4); ++k)
+            acc += T.v[k] * len * T.w[k];
         return acc;
+    }
+
+    // Sample s (fs = s) of a ray (driving axis A) on the plane at byte offset poff of its padded
+    // view, added to acc; FB2 says that the fast in-plane axis of the view is b2 (else b1). The
+    // interpolation has the order and weights of elsa's doInterpolation (JosephsMethod.cpp:60-97):
+    // 2D taps (F0,C1)(C0,F1), 3D taps (F0,F1,F2)(F0,C1,C2)(C0,F1,C2)(C0,C1,F2), the driving axis'
+    // floor and ceil being its plane; each tap adds (value * len) * weight, elsa's association.
+    // C = F + 1 is read even where elsa's ceil gives C = F: the weight of that tap is 0 then.
+    // Positions are those of joseph_sample and tile_ray_adjoint.
+    template <int D, int A, bool FB2, bool V16>
+    EHP_HD inline float plane_sample_forward(const RayRec& r, const Dims& dims, const PadView& V, uint32_t poff,
+                                             float fs, float acc)
+    {
+        return plane_taps_add<D>(plane_sample_taps<D, A, FB2, V16>(r, dims, V, poff, fs), r.len, acc);
     }
 
     // Forward projection of one ray (driving axis A) through its padded view: the sum of
@@ -668,6 +688,14 @@ namespace ehp
         for (; it < full0; ++it, ++j, fs += 1.0f, poff += dp)
             if (j < n)
                 acc = plane_sample_forward<D, A, FB2, V16>(r, dims, V, poff, fs, acc);
+        // two samples per step, both fetched before either is added, so that each wavefront
+        // has four loads in flight
+        for (; it + 1 < full1; it += 2, fs += 2.0f, poff += 2u * dp) {
+            const SampleTaps Ta = plane_sample_taps<D, A, FB2, V16>(r, dims, V, poff, fs);
+            const SampleTaps Tb = plane_sample_taps<D, A, FB2, V16>(r, dims, V, poff + dp, fs + 1.0f);
+            acc = plane_taps_add<D>(Ta, r.len, acc);
+            acc = plane_taps_add<D>(Tb, r.len, acc);
+        }
         for (; it < full1; ++it, fs += 1.0f, poff += dp)
             acc = plane_sample_forward<D, A, FB2, V16>(r, dims, V, poff, fs, acc);
         j += (uint32_t) (full1 - full0);
