@@ -294,6 +294,32 @@ namespace ehp
     }
 
     constexpr int kMaxPoseBatch = 32;
+
+    // Candidate rectangle of one pose of a batch in the tiled back projections: rnu = 1 / width,
+    // nu = width, base = ray index of its first pixel.
+    struct CandRect {
+        float rnu;
+        int nu;
+        long long base;
+    };
+    // Dynamic LDS of a tiled back projection after the tile's 8-byte slots: the batch's
+    // rectangles, candidate counts and their prefix sums. Keeping them behind the slots puts the
+    // slots at LDS address 0, so that a slot's address is its byte offset.
+    constexpr size_t kBatchLdsBytes = kMaxPoseBatch * sizeof(CandRect) + (2 * kMaxPoseBatch + 1) * sizeof(int);
+
+    struct BatchLds {
+        CandRect* rect;
+        int *cnt, *pref;
+    };
+
+    EHP_HD inline BatchLds batch_lds(long long* lds, int nslots)
+    {
+        BatchLds b;
+        b.rect = reinterpret_cast<CandRect*>(lds + nslots);
+        b.cnt = reinterpret_cast<int*>(b.rect + kMaxPoseBatch);
+        b.pref = b.cnt + kMaxPoseBatch;
+        return b;
+    }
     // partial maxima of the fixed-point scale (one per block of the max-|y| launch)
     constexpr int kAbsmaxBlocks = 2048;
     constexpr int kPoseBatch = 8;

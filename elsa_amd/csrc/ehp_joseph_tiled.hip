@@ -139,18 +139,13 @@ namespace ehp
                                                     float* __restrict__ vol)
     {
         extern __shared__ long long lds[];
-        // candidate rectangle of each pose of the batch: rnu = 1 / width, ray index of its first pixel
-        struct CandRect {
-            float rnu;
-            int nu;
-            long long base;
-        };
-        __shared__ CandRect s_rect[kMaxPoseBatch];
-        __shared__ int s_cnt[kMaxPoseBatch], s_pref[kMaxPoseBatch + 1];
 
         const int tid = threadIdx.x;
         const TileBox t = make_tile<D, A>(dims, tc, tile % ntA, (tile / ntA) % ntB, tile / (ntA * ntB));
         long long* iacc = lds;
+        const BatchLds bl = batch_lds(lds, t.nslots);
+        CandRect* s_rect = bl.rect;
+        int *s_cnt = bl.cnt, *s_pref = bl.pref;
         for (int i = tid; i < t.nslots; i += kThreads)
             iacc[i] = 0;
         const int eg = fixexp[0];
@@ -324,7 +319,7 @@ namespace ehp
     size_t joseph_tile_lds_bytes(const Dims& dims, const TileCfg& tc)
     {
         const int PC = dims.dim == 3 ? tc.TC + 2 : 1;
-        return (size_t) tc.LA * PC * (tc.TB + 2) * sizeof(long long);
+        return (size_t) tc.LA * PC * (tc.TB + 2) * sizeof(long long) + kBatchLdsBytes;
     }
 
     // ST: the sinogram's storage type (float, or bf16 for bfloat16 operators); the volume is

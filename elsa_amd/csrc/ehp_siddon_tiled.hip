@@ -62,7 +62,7 @@ namespace ehp
 
     size_t siddon_tile_lds_bytes(const Dims& dims, const TileCfg& tc)
     {
-        return (size_t) tc.LA * tc.TB * (dims.dim == 3 ? tc.TC : 1) * sizeof(long long);
+        return (size_t) tc.LA * tc.TB * (dims.dim == 3 ? tc.TC : 1) * sizeof(long long) + kBatchLdsBytes;
     }
 
     template <int D, int A>
@@ -75,14 +75,6 @@ namespace ehp
     {
         constexpr int b1 = TileAxes<D, A>::b1, b2 = TileAxes<D, A>::b2;
         extern __shared__ long long lds[];
-        // candidate rectangle of each pose of the batch: rnu = 1 / width, ray index of its first pixel
-        struct CandRect {
-            float rnu;
-            int nu;
-            long long base;
-        };
-        __shared__ CandRect s_rect[kMaxPoseBatch];
-        __shared__ int s_cnt[kMaxPoseBatch], s_pref[kMaxPoseBatch + 1];
 
         // consecutive workgroups land on different XCDs: give each XCD a contiguous block of tiles
         const int tile = (int) (blockIdx.x % 8u) * tiles_per_xcd + (int) (blockIdx.x / 8u);
@@ -92,6 +84,9 @@ namespace ehp
         const TileBox t = make_tile<D, A>(dims, tc, tile % ntA, (tile / ntA) % ntB, tile / (ntA * ntB));
         const int nslots = tc.LA * tc.TB * (D == 3 ? tc.TC : 1);
         long long* iacc = lds;
+        const BatchLds bl = batch_lds(lds, nslots);
+        CandRect* s_rect = bl.rect;
+        int *s_cnt = bl.cnt, *s_pref = bl.pref;
         for (int i = tid; i < nslots; i += kThreads)
             iacc[i] = 0;
         // the tile's voxel box
