@@ -37,7 +37,7 @@ namespace ehp
             const int64_t r = i / dims.vol[0];
             const int y = (int) (r % dims.vol[1]);
             const int z = (int) (r / dims.vol[1]);
-            out[pad_index(L, x, y, z)] = in[i];
+            pad_store<E>(L, out, x, y, z, in[i]);
         }
 
         // View with f != x fastest: a 32 x 32 LDS tile over the (x, f) plane, reads coalesced
@@ -68,7 +68,7 @@ namespace ehp
                 const int cx = x0 + ty + j, cf = f0 + tx;
                 if (cx < nx && cf < nf) {
                     const int c1 = f == 1 ? cf : ct, c2 = f == 1 ? ct : cf;
-                    out[pad_index(L, cx, c1, c2)] = tile[tx][ty + j];
+                    pad_store<E>(L, out, cx, c1, c2, tile[tx][ty + j]);
                 }
             }
         }
@@ -77,6 +77,13 @@ namespace ehp
         __device__ inline float fwd_span(const RayRec& r, const Dims& dims, const ViewSet& vs, int t0, int niter,
                                          int s_first, int full0, int full1)
         {
+            if (!V16 && D == 3 && (vs.pairs >> A) & 1) {
+                if ((vs.fb2 >> A) & 1)
+                    return plane_span_forward<D, A, NEG, true, false, true>(r, dims, vs.v[A], t0, niter, s_first,
+                                                                            full0, full1);
+                return plane_span_forward<D, A, NEG, false, false, true>(r, dims, vs.v[A], t0, niter, s_first, full0,
+                                                                         full1);
+            }
             if (D == 3 && (vs.fb2 >> A) & 1)
                 return plane_span_forward<D, A, NEG, true, V16>(r, dims, vs.v[A], t0, niter, s_first, full0, full1);
             return plane_span_forward<D, A, NEG, false, V16>(r, dims, vs.v[A], t0, niter, s_first, full0, full1);
@@ -252,6 +259,11 @@ namespace ehp
             const int b2 = dims.dim == 3 ? (a == 2 ? 1 : 2) : -1;
             if (L.f == b2)
                 vs.fb2 |= 1 << a;
+            if (L.pairs) {
+                if (es != 4u)
+                    return hipErrorInvalidValue; // row-pair views hold FP32 elements
+                vs.pairs |= 1 << a;
+            }
         }
         const FwdGrid fg = fwd_grid(dims);
         const int tiles_u = fg.tiles_u;
@@ -307,6 +319,8 @@ namespace ehp
                 const int b2 = it.dims.dim == 3 ? (a == 2 ? 1 : 2) : -1;
                 if (L.f == b2)
                     fj.views.fb2 |= 1 << a;
+                if (L.pairs)
+                    fj.views.pairs |= 1 << a;
             }
             const FwdGrid fg = fwd_grid(it.dims);
             fj.tiles_u = fg.tiles_u;

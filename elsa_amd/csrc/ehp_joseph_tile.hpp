@@ -443,24 +443,31 @@ namespace ehp
     // c_A * PS + (c_m + mpad) * RS + (c_f + 2). A tap outside the volume then reads a zero
     // instead of needing a bounds test, which gives the same sum as elsa's zero weight for
     // finite volumes.
+    //
+    // Row-pair views (pairs = 1, FP32 3D Joseph forward): row m of a plane holds, for each f, the
+    // two elements (m, f) and (m + 1, f) side by side, so that the 2 x 2 footprint of a sample,
+    // (m, f), (m + 1, f), (m, f + 1), (m + 1, f + 1), is 16 contiguous bytes: one load instead
+    // of two. Each voxel is stored twice (slot 0 of its own row, slot 1 of the row before).
     struct PadLayout {
         int A, m, f;
         int mpad;      // 2 in 3D, 0 in 2D
-        uint32_t RS;   // row stride (n_f + 4, rounded up to even)
+        int pairs;     // 1: row-pair view
+        uint32_t RS;   // row stride in elements ((n_f + 4, rounded up to even) times 1 + pairs)
         uint32_t PS;   // plane stride
         int64_t size;  // elements in the view
     };
 
-    inline PadLayout pad_layout(const Dims& dims, int A, int f)
+    inline PadLayout pad_layout(const Dims& dims, int A, int f, bool pairs = false)
     {
         PadLayout L{};
         L.A = A;
         L.f = f;
         L.m = 3 - A - f; // in 2D the third axis has one voxel
         L.mpad = dims.dim == 3 ? 2 : 0;
+        L.pairs = pairs && dims.dim == 3 ? 1 : 0;
         // rows of an even number of elements, so that with bfloat16 elements every row and plane
         // starts on a 4-byte boundary
-        L.RS = (uint32_t) ((dims.vol[f] + 5) & ~1);
+        L.RS = (uint32_t) ((dims.vol[f] + 5) & ~1) << L.pairs;
         L.PS = L.RS * (uint32_t) (dims.vol[L.m] + 2 * L.mpad);
         L.size = (int64_t) L.PS * dims.vol[A];
         return L;
@@ -473,7 +480,18 @@ namespace ehp
         const int ca = L.A == 0 ? c[0] : (L.A == 1 ? c[1] : c[2]);
         const int cm = L.m == 0 ? c[0] : (L.m == 1 ? c[1] : c[2]);
         const int cf = L.f == 0 ? c[0] : (L.f == 1 ? c[1] : c[2]);
-        return (int64_t) ca * L.PS + (int64_t) (cm + L.mpad) * L.RS + (cf + 2);
+        return (int64_t) ca * L.PS + (int64_t) (cm + L.mpad) * L.RS + ((int64_t) (cf + 2) << L.pairs);
+    }
+
+    // Store voxel (c0, c1, c2) in its view: at pad_index, and in a row-pair view also in slot 1
+    // of the row before (which exists: rows start at m = -mpad = -2).
+    template <class E>
+    EHP_HD inline void pad_store(const PadLayout& L, E* out, int c0, int c1, int c2, E v)
+    {
+        const int64_t i = pad_index(L, c0, c1, c2);
+        out[i] = v;
+        if (L.pairs)
+            out[i - (int64_t) L.RS + 1] = v;
     }
 
     // Bytes allocated past the last element of a bfloat16 view (ld_view2_bf16 reads up to 4).
@@ -545,6 +563,26 @@ namespace ehp
         return V16 ? ld_view2_bf16(V, pb, off) : ld_view2(V, pb, off);
     }
 
+    // The 2 x 2 footprint of a row-pair view at byte offset off from pb: (m, f), (m + 1, f),
+    // (m, f + 1), (m + 1, f + 1). One 16-byte load; 8-byte alignment suffices.
+    struct __attribute__((aligned(8))) Quad {
+        float a, b, c, d;
+    };
+
+    EHP_HD inline Quad ld_view4(const PadView& V, const char* pb, uint32_t off)
+    {
+#if defined(EHP_VIEW_CHECK)
+        const int64_t o = (int64_t) (pb - (const char*) V.base) + off;
+        if ((o & 7) || o < 0 || (o >> 2) + 3 >= V.size) {
+            g_view_oob = true;
+            return Quad{0.0f, 0.0f, 0.0f, 0.0f};
+        }
+#endif
+        Quad q;
+        __builtin_memcpy(&q, pb + off, sizeof(Quad));
+        return q;
+    }
+
     // floor value g clamped to [-2, n]: taps outside the volume land in the zero padding
     EHP_HD inline int pad_clamp(float g, int n)
     {
@@ -573,7 +611,7 @@ namespace ehp
         float v[4], w[4];
     };
 
-    template <int D, int A, bool FB2, bool V16>
+    template <int D, int A, bool FB2, bool V16, bool PR = false>
     EHP_HD inline SampleTaps plane_sample_taps(const RayRec& r, const Dims& dims, const PadView& V, uint32_t poff,
                                                float fs)
     {
@@ -605,9 +643,20 @@ namespace ehp
             // the F and F + 1 taps along the fast axis are neighbours: two 8-byte loads, one per
             // row of the slow axis; (slow + 2) * RS + (fast + 2) elements in, the 2s being the padding
             const int ks = FB2 ? k1 : k2, kf = FB2 ? k2 : k1;
-            const uint32_t o =
-                (uint32_t) (smul24(ks, (int) V.RS4) + (int) (2u * V.RS4 + (2u << ES))) + ((uint32_t) kf << ES);
-            const Pair r0 = ld_pair<V16>(V, pb, o), r1 = ld_pair<V16>(V, pb + V.RS4, o);
+            Pair r0, r1;
+            if (PR) {
+                // row-pair view: element (ks, kf) holds rows ks and ks + 1, 8 bytes per element
+                const uint32_t o =
+                    (uint32_t) (smul24(ks, (int) V.RS4) + (int) (2u * V.RS4 + 16u)) + ((uint32_t) kf << 3);
+                const Quad q = ld_view4(V, pb, o);
+                r0 = Pair{q.a, q.c};
+                r1 = Pair{q.b, q.d};
+            } else {
+                const uint32_t o =
+                    (uint32_t) (smul24(ks, (int) V.RS4) + (int) (2u * V.RS4 + (2u << ES))) + ((uint32_t) kf << ES);
+                r0 = ld_pair<V16>(V, pb, o);
+                r1 = ld_pair<V16>(V, pb + V.RS4, o);
+            }
             // FB2: rows are b1 = F1, F1 + 1, pairs run along b2; else rows are b2, pairs along b1
             const float vff = r0.a;
             const float vkf = FB2 ? r1.a : r0.b;
@@ -650,16 +699,16 @@ namespace ehp
     // floor and ceil being its plane; each tap adds (value * len) * weight, elsa's association.
     // C = F + 1 is read even where elsa's ceil gives C = F: the weight of that tap is 0 then.
     // Positions are those of joseph_sample and tile_ray_adjoint.
-    template <int D, int A, bool FB2, bool V16>
+    template <int D, int A, bool FB2, bool V16, bool PR = false>
     EHP_HD inline float plane_sample_forward(const RayRec& r, const Dims& dims, const PadView& V, uint32_t poff,
                                              float fs, float acc)
     {
-        return plane_taps_add<D>(plane_sample_taps<D, A, FB2, V16>(r, dims, V, poff, fs), r.len, acc);
+        return plane_taps_add<D>(plane_sample_taps<D, A, FB2, V16, PR>(r, dims, V, poff, fs), r.len, acc);
     }
 
     // Forward projection of one ray (driving axis A) through its padded view: the sum of
     // plane_sample_forward over its samples in the volume, in traversal order.
-    template <int D, int A, bool FB2, bool V16 = false>
+    template <int D, int A, bool FB2, bool V16 = false, bool PR = false>
     EHP_HD inline float plane_ray_forward(const RayRec& r, const Dims& dims, const PadView& V)
     {
         int s0, s1;
@@ -671,7 +720,7 @@ namespace ehp
         uint32_t poff = (uint32_t) (neg ? r.P0 - s0 : r.P0 + s0) * V.PS4;
         const uint32_t dp = neg ? 0u - V.PS4 : V.PS4;
         for (int s = s0; s < s1; ++s, poff += dp)
-            acc = plane_sample_forward<D, A, FB2, V16>(r, dims, V, poff, (float) s, acc);
+            acc = plane_sample_forward<D, A, FB2, V16, PR>(r, dims, V, poff, (float) s, acc);
         return acc;
     }
 
@@ -698,7 +747,7 @@ namespace ehp
     // iterations [full0, full1) every ray of the group has a sample (full0 = max tfirst - t0,
     // full1 = max(full0, min tlast - t0 + 1)), so they need no test. Same samples, positions
     // and order as plane_ray_forward, so the same sum bit for bit.
-    template <int D, int A, bool NEG, bool FB2, bool V16 = false>
+    template <int D, int A, bool NEG, bool FB2, bool V16 = false, bool PR = false>
     EHP_HD inline float plane_span_forward(const RayRec& r, const Dims& dims, const PadView& V, int t0, int niter,
                                            int s_first, int full0, int full1)
     {
@@ -713,21 +762,21 @@ namespace ehp
         int it = 0;
         for (; it < full0; ++it, ++j, fs += 1.0f, poff += dp)
             if (j < n)
-                acc = plane_sample_forward<D, A, FB2, V16>(r, dims, V, poff, fs, acc);
+                acc = plane_sample_forward<D, A, FB2, V16, PR>(r, dims, V, poff, fs, acc);
         // two samples per step, both fetched before either is added, so that each wavefront
         // has four loads in flight
         for (; it + 1 < full1; it += 2, fs += 2.0f, poff += 2u * dp) {
-            const SampleTaps Ta = plane_sample_taps<D, A, FB2, V16>(r, dims, V, poff, fs);
-            const SampleTaps Tb = plane_sample_taps<D, A, FB2, V16>(r, dims, V, poff + dp, fs + 1.0f);
+            const SampleTaps Ta = plane_sample_taps<D, A, FB2, V16, PR>(r, dims, V, poff, fs);
+            const SampleTaps Tb = plane_sample_taps<D, A, FB2, V16, PR>(r, dims, V, poff + dp, fs + 1.0f);
             acc = plane_taps_add<D>(Ta, r.len, acc);
             acc = plane_taps_add<D>(Tb, r.len, acc);
         }
         for (; it < full1; ++it, fs += 1.0f, poff += dp)
-            acc = plane_sample_forward<D, A, FB2, V16>(r, dims, V, poff, fs, acc);
+            acc = plane_sample_forward<D, A, FB2, V16, PR>(r, dims, V, poff, fs, acc);
         j += (uint32_t) (full1 - full0);
         for (; it < niter; ++it, ++j, fs += 1.0f, poff += dp)
             if (j < n)
-                acc = plane_sample_forward<D, A, FB2, V16>(r, dims, V, poff, fs, acc);
+                acc = plane_sample_forward<D, A, FB2, V16, PR>(r, dims, V, poff, fs, acc);
         return acc;
     }
 } // namespace ehp

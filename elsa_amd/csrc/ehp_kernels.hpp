@@ -15,7 +15,8 @@ namespace ehp
     // the views of one plane-major forward
     struct ViewSet {
         PadView v[3];
-        int fb2; // bit a: the fast in-plane axis of view a is b2
+        int fb2;   // bit a: the fast in-plane axis of view a is b2
+        int pairs; // bit a: view a is a row-pair view (FP32, 3D)
     };
 
     // ---- batches of Joseph operators (FP32 fast kernels): one launch serves many operators.

@@ -109,6 +109,8 @@ namespace ehp
             if (!(axes_present & (1 << a)))
                 continue;
             const PadLayout& L = layouts[a];
+            if (L.pairs)
+                return hipErrorInvalidValue; // the Siddon walk reads single voxels of plain views
             hipError_t e = joseph_view_copy<float>(dims, L, vol, (float*) views[a], s);
             if (e != hipSuccess)
                 return e;

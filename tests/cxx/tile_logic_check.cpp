@@ -207,9 +207,14 @@ extern "C" int tile_adjoint(int dim, const int64_t* vol_dims, const int64_t* det
 // view of driving axis a. The views are filled by the index map of the view copy kernels
 // (pad_index), every offset checked against the view and written at most once; the padding
 // stays zero.
+static bool g_pairs = false; // row-pair views (FP32, 3D)
+
 template <int D, int A>
 static float fwd_one(const RayRec& r, const Dims& dims, const PadView& V, bool fb2, bool b16)
 {
+    if (D == 3 && g_pairs)
+        return fb2 ? plane_ray_forward<D, A, true, false, true>(r, dims, V)
+                   : plane_ray_forward<D, A, false, false, true>(r, dims, V);
     if (b16)
         return fb2 ? plane_ray_forward<D, A, true, true>(r, dims, V) : plane_ray_forward<D, A, false, true>(r, dims, V);
     return fb2 ? plane_ray_forward<D, A, true>(r, dims, V) : plane_ray_forward<D, A, false>(r, dims, V);
@@ -222,6 +227,11 @@ struct Span {
 template <int D, int A, bool NEG>
 static float span_one(const RayRec& r, const Dims& dims, const PadView& V, bool fb2, bool b16, const Span& q)
 {
+    if (D == 3 && g_pairs)
+        return fb2 ? plane_span_forward<D, A, NEG, true, false, true>(r, dims, V, q.t0, q.niter, q.s_first, q.full0,
+                                                                      q.full1)
+                   : plane_span_forward<D, A, NEG, false, false, true>(r, dims, V, q.t0, q.niter, q.s_first,
+                                                                       q.full0, q.full1);
     if (b16)
         return fb2 ? plane_span_forward<D, A, NEG, true, true>(r, dims, V, q.t0, q.niter, q.s_first, q.full0, q.full1)
                    : plane_span_forward<D, A, NEG, false, true>(r, dims, V, q.t0, q.niter, q.s_first, q.full0,
@@ -303,12 +313,17 @@ extern "C" int64_t plane_forward_mixed_waves()
 
 // flags: bit 0 bfloat16 views (the volume rounded to bfloat16), read by the bfloat16 variant of
 // the kernel's ray loop; bit 1 the GPU's wavefront grouping (wave_forward) instead of one loop
-// per ray, with wavefronts of wu = flags >> 8 pixels along u (3D; 0: 64) and 64 / wu along v
+// per ray, with wavefronts of wu = flags >> 8 pixels along u (3D; 0: 64) and 64 / wu along v;
+// bit 2 row-pair views (FP32, 3D: pad_store's two homes of every voxel, one 16-byte read per
+// sample)
 extern "C" int plane_forward(int dim, const int64_t* vol_dims, const int64_t* det_dims, int nposes, const float* P,
                              const float* Pinv, const float* C, const int* fan, const float* vol, float* sino,
                              int flags)
 {
     const bool b16 = flags & 1, waves = (flags & 2) != 0;
+    g_pairs = (flags & 4) != 0;
+    if (g_pairs && (b16 || dim != 3))
+        return 24;
     const Dims dims = make_dims(dim, vol_dims, det_dims, nposes);
     const int nm = dim * (dim + 1);
     std::vector<PoseGeom> geo(nposes);
@@ -330,17 +345,19 @@ extern "C" int plane_forward(int dim, const int64_t* vol_dims, const int64_t* de
     for (int a = 0; a < dim; ++a) {
         if (fan[a] == a || fan[a] < 0 || fan[a] > (dim == 3 ? 2 : 1))
             return 20;
-        const PadLayout L = pad_layout(dims, a, fan[a]);
+        const PadLayout L = pad_layout(dims, a, fan[a], g_pairs);
         views[a].assign(L.size, 0.0f);
         std::vector<char> seen(L.size, 0);
         for (int z = 0; z < dims.vol[2]; ++z)
             for (int y = 0; y < dims.vol[1]; ++y)
                 for (int x = 0; x < dims.vol[0]; ++x) {
                     const int64_t o = pad_index(L, x, y, z);
-                    if (o < 0 || o >= L.size || seen[o])
+                    const int64_t o2 = L.pairs ? o - (int64_t) L.RS + 1 : o; // pad_store's second home
+                    if (o < 0 || o >= L.size || seen[o] || o2 < 0 || o2 >= L.size || (L.pairs && seen[o2]))
                         return 21; // out of range or written twice
-                    seen[o] = 1;
-                    views[a][o] = vol[x + dims.strides[1] * y + dims.strides[2] * z];
+                    seen[o] = seen[o2] = 1;
+                    const float v = vol[x + dims.strides[1] * y + dims.strides[2] * z];
+                    pad_store<float>(L, views[a].data(), x, y, z, v);
                 }
         if (b16) {
             views16[a].resize(L.size);
