@@ -137,19 +137,19 @@ namespace ehp
         }
 
         // Block id of one pose-band tile of the forward: id = (band * nposes + pose) * tiles_u + tu.
-        // Each wavefront holds 8 x 8 neighbouring pixels (2D: 64 of the row). Its rays fall into
+        // Each wavefront holds WU x (64 / WU) neighbouring pixels (2D: 64 of the row). Its rays fall into
         // groups by driving axis and direction (in practice one or two), and each group runs one
         // plane loop (plane_span_forward), so that at every step all its lanes read one plane. A
         // per-lane loop would put rays that enter through a side face of the volume, each at its
         // own plane, on a cache line of their own at every step. Every lane of the block takes
         // part in the wavefront reductions; lanes past the detector edge join no group.
-        template <int D, class OT = float>
+        template <int D, class OT = float, int WU = kFwdWU>
         __device__ __forceinline__ void fwd_planes_block(const PoseGeom* __restrict__ geo, const Dims& dims,
                                                          const ViewSet& views, const int tiles_u, const int64_t id,
                                                          OT* __restrict__ sino)
         {
             int u, v, pose;
-            fwd_pixel<D>(dims, tiles_u, id, u, v, pose);
+            fwd_pixel<D, WU>(dims, tiles_u, id, u, v, pose);
             const bool inside = u < dims.det[0] && v < dims.det[1];
             RayRec r{};
             int key = -1, tfirst = 0, tlast = 0, s0 = 0;
@@ -177,7 +177,7 @@ namespace ehp
                 sino[(int64_t) pose * dims.pix_per_pose + (int64_t) v * dims.det[0] + u] = down<OT>(val);
         }
 
-        template <int D, class OT>
+        template <int D, class OT, int WU = kFwdWU>
         __global__ void __launch_bounds__(kThreads)
             joseph_fwd_planes_kernel(const PoseGeom* __restrict__ geo, const Dims dims, const ViewSet views,
                                      const int tiles_u, const int per_xcd, const int64_t total,
@@ -186,7 +186,7 @@ namespace ehp
             const int64_t id = (int64_t) (blockIdx.x % 8u) * per_xcd + (blockIdx.x / 8u);
             if (id >= total)
                 return;
-            fwd_planes_block<D, OT>(geo, dims, views, tiles_u, id, sino);
+            fwd_planes_block<D, OT, WU>(geo, dims, views, tiles_u, id, sino);
         }
 
         // ---- batches: one launch over the jobs of many operators; block b runs block
@@ -265,7 +265,9 @@ namespace ehp
                 vs.pairs |= 1 << a;
             }
         }
-        const FwdGrid fg = fwd_grid(dims);
+        // row-pair views (FP32, 3D) run with wider wavefronts
+        const bool pairs = dims.dim == 3 && vs.pairs != 0;
+        const FwdGrid fg = fwd_grid(dims, pairs ? kFwdWUPairs : kFwdWU);
         const int tiles_u = fg.tiles_u;
         const int64_t total = fg.total, per_xcd = fg.per_xcd;
         if (per_xcd * 8 > (int64_t) 0x7fffffff)
@@ -274,6 +276,9 @@ namespace ehp
         if (dims.dim == 2)
             hipLaunchKernelGGL((joseph_fwd_planes_kernel<2, T>), grid, dim3(kThreads), 0, s, geo, dims, vs, tiles_u,
                                (int) per_xcd, total, sino);
+        else if (pairs)
+            hipLaunchKernelGGL((joseph_fwd_planes_kernel<3, T, kFwdWUPairs>), grid, dim3(kThreads), 0, s, geo, dims,
+                               vs, tiles_u, (int) per_xcd, total, sino);
         else
             hipLaunchKernelGGL((joseph_fwd_planes_kernel<3, T>), grid, dim3(kThreads), 0, s, geo, dims, vs, tiles_u,
                                (int) per_xcd, total, sino);

@@ -7,16 +7,24 @@
 namespace ehp
 {
     constexpr int kFwdThreads = 256;
-    // 3D: a wavefront covers WU x WV pixels (u x v), a block TU x TV (four wavefronts side by
-    // side along u). 8 x 8: neighbouring pixels of a square have nearly the same entry and exit
-    // planes (96 % of a Joseph wavefront's lane-steps carry a sample at 256^3 / 384^2, against
-    // 77 % for a 64 x 1 row), at about twice the cache lines per step of a row. 2D: 64 pixels
-    // of the row per wavefront, 256 per block.
+    // 3D: a wavefront covers WU x WV pixels (u x v, WV = 64 / WU), a block TU x TV (four
+    // wavefronts side by side along u). Pixels of a compact patch have nearly the same entry and
+    // exit planes (96 % of a Joseph wavefront's lane-steps carry a sample at 256^3 / 384^2 with
+    // 8 x 8, against 77 % for a 64 x 1 row), while a wider patch reads fewer rows of a plane per
+    // step. 8 x 8 for views with one element per voxel (the Joseph forward over them: 7.84 ms
+    // at 256^3, 16 x 4: 7.95 ms); 16 x 4 for row-pair views, whose rows are twice as long (the
+    // same forward: 5.7 ms, 8 x 8: 6.9 ms, 32 x 2: 6.0 ms). 2D: 64 pixels of the row per
+    // wavefront, 256 per block.
     constexpr int kFwdWU = 8;
-    constexpr int kFwdWV = 64 / kFwdWU;
-    constexpr int kFwdTV = kFwdWV > 4 ? kFwdWV : 4;
-    constexpr int kFwdTU = kFwdThreads / kFwdTV;
-    static_assert(kFwdWU * kFwdWV == 64 && kFwdTU % kFwdWU == 0 && kFwdTV % kFwdWV == 0, "forward wavefront shape");
+    constexpr int kFwdWUPairs = 16;
+
+    template <int WU>
+    struct FwdShape {
+        static constexpr int WV = 64 / WU;
+        static constexpr int TV = WV > 4 ? WV : 4;
+        static constexpr int TU = kFwdThreads / TV;
+        static_assert(WU * WV == 64 && TU % WU == 0 && TV % WV == 0, "forward wavefront shape");
+    };
 
     // Pixels of a forward launch: blocks of TU x TV pixels of one pose, ordered detector-row band
     // first, then pose, then column: id = (band * nposes + pose) * tiles_u + tu.
@@ -26,9 +34,11 @@ namespace ehp
         int64_t per_xcd;
     };
 
-    inline FwdGrid fwd_grid(const Dims& dims)
+    // wu: the wavefront width WU of the launch (3D)
+    inline FwdGrid fwd_grid(const Dims& dims, int wu = kFwdWU)
     {
-        const int TU = dims.dim == 3 ? kFwdTU : kFwdThreads;
+        const int wv = 64 / wu;
+        const int TU = dims.dim == 3 ? kFwdThreads / (wv > 4 ? wv : 4) : kFwdThreads;
         const int TV = kFwdThreads / TU;
         FwdGrid g;
         g.tiles_u = (dims.det[0] + TU - 1) / TU;
@@ -38,13 +48,13 @@ namespace ehp
         return g;
     }
 
-    // Pixel (u, v) and pose of this lane in block id of a forward launch.
-    template <int D>
+    // Pixel (u, v) and pose of this lane in block id of a forward launch (3D wavefronts WU wide).
+    template <int D, int WU3 = kFwdWU>
     __device__ __forceinline__ void fwd_pixel(const Dims& dims, int tiles_u, int64_t id, int& u, int& v, int& pose)
     {
-        constexpr int TU = (D == 3) ? kFwdTU : kFwdThreads;
+        constexpr int TU = (D == 3) ? FwdShape<WU3>::TU : kFwdThreads;
         constexpr int TV = kFwdThreads / TU;
-        constexpr int WU = (D == 3) ? kFwdWU : 64, WV = 64 / WU;
+        constexpr int WU = (D == 3) ? WU3 : 64, WV = 64 / WU;
         const int tu = (int) (id % tiles_u);
         const int64_t bp = id / tiles_u;
         pose = (int) (bp % dims.nposes);
