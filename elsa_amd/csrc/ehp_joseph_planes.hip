@@ -1,17 +1,21 @@
 // ehp_joseph_planes.hip -- Joseph forward projection over plane-major volume views (FP32).
 //
 // The forward of elsa's Joseph method (elsa/projectors/JosephsMethod.cpp:140-198, forward
-// branch :187-194) with one lane per ray. A wavefront holds 8 x 8 neighbouring detector
-// pixels, and its rays step through the volume's planes together, so at each step its lanes
+// branch :187-194) with one lane per ray. A wavefront holds a patch of neighbouring detector
+// pixels (8 x 8, or 16 x 4 over row-pair views), and its rays step through the volume's
+// planes together, so at each step its lanes
 // read the same plane at neighbouring in-plane positions. Those reads coalesce only if the axis
 // along which the rays fan out along u is the fastest one in memory: rays driving along x fan
 // out along z (circular trajectories about y), which in the input's x-fastest layout puts
 // every lane on a different cache line.
 // So each driving axis reads its own view of the volume, made at the start of the call (one
 // read and one write of the volume): that axis slowest, the fan-out axis fastest, and two
-// zero voxels around each plane so that taps outside the volume need no bounds test.
+// zero voxels around each plane so that taps outside the volume need no bounds test. FP32 3D
+// operators keep row-pair views (each element holds a voxel and the one in the next row), so
+// that a sample's four taps are one 16-byte load.
 //
-// Blocks of 256 lanes cover 32 x 8 pixels (2D: 256 x 1) of one pose. Tiles are ordered
+// Blocks of 256 lanes cover 32 x 8 pixels (64 x 4 with 16 x 4 wavefronts; 2D: 256 x 1) of one
+// pose. Tiles are ordered
 // detector-row band first, then pose, then column, and each XCD (consecutive workgroups go
 // to different XCDs, so the block index is remapped) takes a contiguous run: the poses of a
 // band then run together on one XCD. For a rotation about y a band of rows sees the same
