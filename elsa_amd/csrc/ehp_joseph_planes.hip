@@ -15,10 +15,9 @@
 // that a sample's four taps are one 16-byte load.
 //
 // Blocks of 256 lanes cover 32 x 8 pixels (64 x 4 with 16 x 4 wavefronts; 2D: 256 x 1) of one
-// pose. Tiles are ordered
-// detector-row band first, then pose, then column, and each XCD (consecutive workgroups go
-// to different XCDs, so the block index is remapped) takes a contiguous run: the poses of a
-// band then run together on one XCD. For a rotation about y a band of rows sees the same
+// pose. Tiles are ordered detector-row band first, then pose, then column, and each XCD
+// (consecutive workgroups go to different XCDs, so the block index is remapped) takes a
+// contiguous run: the poses of a band then run together on one XCD. For a rotation about y a band of rows sees the same
 // slab of the volume from every pose, so that slab stays in the XCD's L2.
 #include "ehp_waves.hpp"
 
