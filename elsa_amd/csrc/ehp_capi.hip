@@ -284,8 +284,8 @@ namespace
     {
         const ehp::Dims& dm = op->dims;
         hipError_t e = hipSuccess;
-        // FP32 3D Joseph forwards read row-pair views (one 16-byte load per sample) when those fit
-        bool pairs = op->kind == KIND_JOSEPH && op->dtype == EHP_F32 && dm.dim == 3;
+        // 3D Joseph forwards (FP32, bfloat16) read row-pair views (one load per sample) when those fit
+        bool pairs = op->kind == KIND_JOSEPH && (op->dtype == EHP_F32 || op->dtype == EHP_BF16) && dm.dim == 3;
         for (int a = 0; pairs && a < dm.dim; ++a)
             if ((op->axes_present & (1 << a)) && ehp::pad_layout(dm, a, fan_axis(op, h, a), true).size >= ((int64_t) 1 << 30))
                 pairs = false;

@@ -10,9 +10,9 @@
 // every lane on a different cache line.
 // So each driving axis reads its own view of the volume, made at the start of the call (one
 // read and one write of the volume): that axis slowest, the fan-out axis fastest, and two
-// zero voxels around each plane so that taps outside the volume need no bounds test. FP32 3D
+// zero voxels around each plane so that taps outside the volume need no bounds test. 3D
 // operators keep row-pair views (each element holds a voxel and the one in the next row), so
-// that a sample's four taps are one 16-byte load.
+// that a sample's four taps are one load (16 bytes in FP32, 8 in bfloat16).
 //
 // Blocks of 256 lanes cover 32 x 8 pixels (64 x 4 with 16 x 4 wavefronts; 2D: 256 x 1) of one
 // pose. Tiles are ordered detector-row band first, then pose, then column, and each XCD
@@ -80,12 +80,12 @@ namespace ehp
         __device__ inline float fwd_span(const RayRec& r, const Dims& dims, const ViewSet& vs, int t0, int niter,
                                          int s_first, int full0, int full1)
         {
-            if (!V16 && D == 3 && (vs.pairs >> A) & 1) {
+            if (D == 3 && (vs.pairs >> A) & 1) {
                 if ((vs.fb2 >> A) & 1)
-                    return plane_span_forward<D, A, NEG, true, false, true>(r, dims, vs.v[A], t0, niter, s_first,
-                                                                            full0, full1);
-                return plane_span_forward<D, A, NEG, false, false, true>(r, dims, vs.v[A], t0, niter, s_first, full0,
-                                                                         full1);
+                    return plane_span_forward<D, A, NEG, true, V16, true>(r, dims, vs.v[A], t0, niter, s_first,
+                                                                          full0, full1);
+                return plane_span_forward<D, A, NEG, false, V16, true>(r, dims, vs.v[A], t0, niter, s_first, full0,
+                                                                       full1);
             }
             if (D == 3 && (vs.fb2 >> A) & 1)
                 return plane_span_forward<D, A, NEG, true, V16>(r, dims, vs.v[A], t0, niter, s_first, full0, full1);
@@ -262,13 +262,10 @@ namespace ehp
             const int b2 = dims.dim == 3 ? (a == 2 ? 1 : 2) : -1;
             if (L.f == b2)
                 vs.fb2 |= 1 << a;
-            if (L.pairs) {
-                if (es != 4u)
-                    return hipErrorInvalidValue; // row-pair views hold FP32 elements
+            if (L.pairs)
                 vs.pairs |= 1 << a;
-            }
         }
-        // row-pair views (FP32, 3D) run with wider wavefronts
+        // row-pair views (3D) run with wider wavefronts
         const bool pairs = dims.dim == 3 && vs.pairs != 0;
         const FwdGrid fg = fwd_grid(dims, pairs ? kFwdWUPairs : kFwdWU);
         const int tiles_u = fg.tiles_u;

@@ -444,10 +444,11 @@ namespace ehp
     // instead of needing a bounds test, which gives the same sum as elsa's zero weight for
     // finite volumes.
     //
-    // Row-pair views (pairs = 1, FP32 3D Joseph forward): row m of a plane holds, for each f, the
+    // Row-pair views (pairs = 1, the 3D Joseph forward): row m of a plane holds, for each f, the
     // two elements (m, f) and (m + 1, f) side by side, so that the 2 x 2 footprint of a sample,
-    // (m, f), (m + 1, f), (m, f + 1), (m + 1, f + 1), is 16 contiguous bytes: one load instead
-    // of two. Each voxel is stored twice (slot 0 of its own row, slot 1 of the row before).
+    // (m, f), (m + 1, f), (m, f + 1), (m + 1, f + 1), is 16 contiguous bytes (8 in bfloat16): one
+    // load instead of two. Each voxel is stored twice (slot 0 of its own row, slot 1 of the row
+    // before).
     struct PadLayout {
         int A, m, f;
         int mpad;      // 2 in 3D, 0 in 2D
@@ -569,6 +570,29 @@ namespace ehp
         float a, b, c, d;
     };
 
+    // The same footprint in a bfloat16 row-pair view: one 8-byte load (4-byte alignment suffices),
+    // widened to FP32.
+    EHP_HD inline Quad ld_view4_bf16(const PadView& V, const char* pb, uint32_t off)
+    {
+#if defined(EHP_VIEW_CHECK)
+        const int64_t o = (int64_t) (pb - (const char*) V.base) + off;
+        if ((o & 3) || o < 0 || (o >> 1) + 3 >= V.size) {
+            g_view_oob = true;
+            return Quad{0.0f, 0.0f, 0.0f, 0.0f};
+        }
+#endif
+        uint32_t w[2];
+        __builtin_memcpy(w, pb + off, sizeof(w));
+        // element (m, f) is the low half of w[0], (m + 1, f) its high half; w[1] holds f + 1
+#if defined(__HIP_DEVICE_COMPILE__)
+        return Quad{__uint_as_float(w[0] << 16), __uint_as_float(w[0] & 0xffff0000u), __uint_as_float(w[1] << 16),
+                    __uint_as_float(w[1] & 0xffff0000u)};
+#else
+        return Quad{bf16_to_float(bf16{(uint16_t) w[0]}), bf16_to_float(bf16{(uint16_t) (w[0] >> 16)}),
+                    bf16_to_float(bf16{(uint16_t) w[1]}), bf16_to_float(bf16{(uint16_t) (w[1] >> 16)})};
+#endif
+    }
+
     EHP_HD inline Quad ld_view4(const PadView& V, const char* pb, uint32_t off)
     {
 #if defined(EHP_VIEW_CHECK)
@@ -645,10 +669,12 @@ namespace ehp
             const int ks = FB2 ? k1 : k2, kf = FB2 ? k2 : k1;
             Pair r0, r1;
             if (PR) {
-                // row-pair view: element (ks, kf) holds rows ks and ks + 1, 8 bytes per element
-                const uint32_t o =
-                    (uint32_t) (smul24(ks, (int) V.RS4) + (int) (2u * V.RS4 + 16u)) + ((uint32_t) kf << 3);
-                const Quad q = ld_view4(V, pb, o);
+                // row-pair view: element (ks, kf) holds rows ks and ks + 1 (8 bytes per element in
+                // FP32, 4 in bfloat16)
+                constexpr uint32_t EP = ES + 1; // log2 of the pair element's size
+                const uint32_t o = (uint32_t) (smul24(ks, (int) V.RS4) + (int) (2u * V.RS4 + (2u << EP))) +
+                                   ((uint32_t) kf << EP);
+                const Quad q = V16 ? ld_view4_bf16(V, pb, o) : ld_view4(V, pb, o);
                 r0 = Pair{q.a, q.c};
                 r1 = Pair{q.b, q.d};
             } else {

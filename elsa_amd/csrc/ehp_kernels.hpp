@@ -16,7 +16,7 @@ namespace ehp
     struct ViewSet {
         PadView v[3];
         int fb2;   // bit a: the fast in-plane axis of view a is b2
-        int pairs; // bit a: view a is a row-pair view (FP32, 3D)
+        int pairs; // bit a: view a is a row-pair view (3D)
     };
 
     // ---- batches of Joseph operators (FP32 fast kernels): one launch serves many operators.

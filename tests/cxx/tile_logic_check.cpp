@@ -212,6 +212,9 @@ static bool g_pairs = false; // row-pair views (FP32, 3D)
 template <int D, int A>
 static float fwd_one(const RayRec& r, const Dims& dims, const PadView& V, bool fb2, bool b16)
 {
+    if (D == 3 && g_pairs && b16)
+        return fb2 ? plane_ray_forward<D, A, true, true, true>(r, dims, V)
+                   : plane_ray_forward<D, A, false, true, true>(r, dims, V);
     if (D == 3 && g_pairs)
         return fb2 ? plane_ray_forward<D, A, true, false, true>(r, dims, V)
                    : plane_ray_forward<D, A, false, false, true>(r, dims, V);
@@ -227,6 +230,11 @@ struct Span {
 template <int D, int A, bool NEG>
 static float span_one(const RayRec& r, const Dims& dims, const PadView& V, bool fb2, bool b16, const Span& q)
 {
+    if (D == 3 && g_pairs && b16)
+        return fb2 ? plane_span_forward<D, A, NEG, true, true, true>(r, dims, V, q.t0, q.niter, q.s_first, q.full0,
+                                                                     q.full1)
+                   : plane_span_forward<D, A, NEG, false, true, true>(r, dims, V, q.t0, q.niter, q.s_first,
+                                                                      q.full0, q.full1);
     if (D == 3 && g_pairs)
         return fb2 ? plane_span_forward<D, A, NEG, true, false, true>(r, dims, V, q.t0, q.niter, q.s_first, q.full0,
                                                                       q.full1)
@@ -314,15 +322,14 @@ extern "C" int64_t plane_forward_mixed_waves()
 // flags: bit 0 bfloat16 views (the volume rounded to bfloat16), read by the bfloat16 variant of
 // the kernel's ray loop; bit 1 the GPU's wavefront grouping (wave_forward) instead of one loop
 // per ray, with wavefronts of wu = flags >> 8 pixels along u (3D; 0: 64) and 64 / wu along v;
-// bit 2 row-pair views (FP32, 3D: pad_store's two homes of every voxel, one 16-byte read per
-// sample)
+// bit 2 row-pair views (3D: pad_store's two homes of every voxel, one read per sample)
 extern "C" int plane_forward(int dim, const int64_t* vol_dims, const int64_t* det_dims, int nposes, const float* P,
                              const float* Pinv, const float* C, const int* fan, const float* vol, float* sino,
                              int flags)
 {
     const bool b16 = flags & 1, waves = (flags & 2) != 0;
     g_pairs = (flags & 4) != 0;
-    if (g_pairs && (b16 || dim != 3))
+    if (g_pairs && dim != 3)
         return 24;
     const Dims dims = make_dims(dim, vol_dims, det_dims, nposes);
     const int nm = dim * (dim + 1);

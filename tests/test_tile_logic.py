@@ -113,21 +113,25 @@ def test_plane_forward_logic_matches_oracle_bitwise(lib, case, waves):
 
 @pytest.mark.parametrize("waves", WAVES, ids=WAVE_IDS)
 @pytest.mark.parametrize("case", [c for c in FWD_CASES if c[0] == 3], ids=lambda c: "x".join(map(str, c[1])))
-def test_plane_forward_row_pair_views_match_oracle_bitwise(lib, case, waves):
-    """Row-pair views (the FP32 3D forward's layout: every voxel stored twice, each sample's 2 x 2
-    footprint one 16-byte read) give the oracle's forward bit for bit, every read inside its view
-    and 8-byte aligned, every view element written at most once."""
+@pytest.mark.parametrize("b16", [False, True], ids=["f32", "bf16"])
+def test_plane_forward_row_pair_views_match_oracle_bitwise(lib, case, waves, b16):
+    """Row-pair views (the 3D forward's layout: every voxel stored twice, each sample's 2 x 2
+    footprint one read, 16 bytes in FP32 and 8 in bfloat16) give the oracle's forward of the
+    (rounded) volume bit for bit, every read inside its view and aligned, every view element
+    written at most once."""
     dim, vol, views, det, s2c, c2d, fans = case
     rng = np.random.default_rng(4)
     angles = np.sort(rng.uniform(0, 360, views)).astype(np.float32)
     g = oracle.circle_geometry(dim, vol, views, 360, s2c, c2d, det=det, angles_deg=angles)
     x = rng.standard_normal(g.vol_shape).astype(np.float32)
+    if b16:
+        x = _to_bf16(x)
     ref = oracle.joseph(g, x, mode=oracle.MODE_DIRECT)
     for fan in fans:
         out = np.full(g.sino_shape, np.nan, np.float32)
         rc = lib.plane_forward(dim, np.array(list(vol), np.int64), np.array(list(det), np.int64), views,
                                np.ascontiguousarray(g.P), np.ascontiguousarray(g.Pinv), np.ascontiguousarray(g.C),
-                               np.array(list(fan), np.int32), x, out, waves | 4)
+                               np.array(list(fan), np.int32), x, out, waves | 4 | int(b16))
         assert rc == 0, (fan, rc)
         np.testing.assert_array_equal(out, ref, err_msg=f"fan axes {fan}")
 
