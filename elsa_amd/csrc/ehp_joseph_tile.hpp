@@ -302,22 +302,26 @@ namespace ehp
         int nu;
         long long base;
     };
-    // Dynamic LDS of a tiled back projection after the tile's 8-byte slots: the batch's
-    // rectangles, candidate counts and their prefix sums. Keeping them behind the slots puts the
-    // slots at LDS address 0, so that a slot's address is its byte offset.
-    constexpr size_t kBatchLdsBytes = kMaxPoseBatch * sizeof(CandRect) + (2 * kMaxPoseBatch + 1) * sizeof(int);
+    // Dynamic LDS of a tiled back projection after the tile's 8-byte slots: the rectangles,
+    // candidate counts and their prefix sums of a batch of `batch` poses, sized by the launch's
+    // batch (at 8 x 17 x 17 tiles and 8 poses that keeps a workgroup under a seventh of the
+    // CU's 160 KB, where 32 poses' worth would allow only six workgroups per CU).
+    EHP_HD constexpr size_t batch_lds_bytes(int batch)
+    {
+        return ((size_t) batch * sizeof(CandRect) + (size_t) (2 * batch + 1) * sizeof(int) + 7) & ~(size_t) 7;
+    }
 
     struct BatchLds {
         CandRect* rect;
         int *cnt, *pref;
     };
 
-    EHP_HD inline BatchLds batch_lds(long long* lds, int nslots)
+    EHP_HD inline BatchLds batch_lds(long long* lds, int nslots, int batch)
     {
         BatchLds b;
         b.rect = reinterpret_cast<CandRect*>(lds + nslots);
-        b.cnt = reinterpret_cast<int*>(b.rect + kMaxPoseBatch);
-        b.pref = b.cnt + kMaxPoseBatch;
+        b.cnt = reinterpret_cast<int*>(b.rect + batch);
+        b.pref = b.cnt + batch;
         return b;
     }
     // partial maxima of the fixed-point scale (one per block of the max-|y| launch)

@@ -143,7 +143,7 @@ namespace ehp
         const int tid = threadIdx.x;
         const TileBox t = make_tile<D, A>(dims, tc, tile % ntA, (tile / ntA) % ntB, tile / (ntA * ntB));
         long long* iacc = lds;
-        const BatchLds bl = batch_lds(lds, t.nslots);
+        const BatchLds bl = batch_lds(lds, t.nslots, batch);
         CandRect* s_rect = bl.rect;
         int *s_cnt = bl.cnt, *s_pref = bl.pref;
         for (int i = tid; i < t.nslots; i += kThreads)
@@ -316,10 +316,10 @@ namespace ehp
         return hipGetLastError();
     }
 
-    size_t joseph_tile_lds_bytes(const Dims& dims, const TileCfg& tc)
+    size_t joseph_tile_lds_bytes(const Dims& dims, const TileCfg& tc, int batch)
     {
         const int PC = dims.dim == 3 ? tc.TC + 2 : 1;
-        return (size_t) tc.LA * PC * (tc.TB + 2) * sizeof(long long) + kBatchLdsBytes;
+        return (size_t) tc.LA * PC * (tc.TB + 2) * sizeof(long long) + batch_lds_bytes(batch);
     }
 
     // ST: the sinogram's storage type (float, or bf16 for bfloat16 operators); the volume is
@@ -337,7 +337,7 @@ namespace ehp
             e = fixed_point_exponent_launch(sino, dims.nrays, slot_bound, d_maxbits, d_fixexp, s);
         if (e != hipSuccess)
             return e;
-        const size_t lds = joseph_tile_lds_bytes(dims, tc);
+        const size_t lds = joseph_tile_lds_bytes(dims, tc, batch);
         for (int a = 0; a < dims.dim; ++a) {
             if (!(axes_present & (1 << a)))
                 continue;

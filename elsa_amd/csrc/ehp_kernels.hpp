@@ -100,7 +100,8 @@ namespace ehp
                            unsigned long long* total, hipStream_t s);
     hipError_t joseph_ray_table(const PoseGeom* geo, const Dims& dims, RayRec* table, hipStream_t s);
     hipError_t joseph_pose_axes(const RayRec* table, const Dims& dims, int* pose_axes, hipStream_t s);
-    size_t joseph_tile_lds_bytes(const Dims& dims, const TileCfg& tc);
+    // dynamic LDS of a tiled back projection with pose batches of `batch` (the largest by default)
+    size_t joseph_tile_lds_bytes(const Dims& dims, const TileCfg& tc, int batch = kMaxPoseBatch);
     template <class ST>
     hipError_t joseph_adjoint_tiled(const PoseGeom* geo, const RayRec* table, const Dims& dims, const int* pose_axes,
                                     int axes_present, const TileCfg& tc, int batch, double slot_bound,
@@ -112,7 +113,7 @@ namespace ehp
     // dst[i] = round-to-nearest-even bfloat16 of src[i]
     hipError_t launch_to_bf16(const float* src, bf16* dst, int64_t n, hipStream_t s);
     hipError_t siddon_ray_table(const PoseGeom* geo, const Dims& dims, SidRec* table, int* pose_axes, hipStream_t s);
-    size_t siddon_tile_lds_bytes(const Dims& dims, const TileCfg& tc);
+    size_t siddon_tile_lds_bytes(const Dims& dims, const TileCfg& tc, int batch = kMaxPoseBatch);
     hipError_t siddon_adjoint_tiled(const PoseGeom* geo, const SidRec* table, const Dims& dims, const int* pose_axes,
                                     int axes_present, const TileCfg& tc, int batch, double slot_bound,
                                     unsigned int* d_maxbits, int* d_fixexp, const float* sino, float* vol,

@@ -60,9 +60,9 @@ namespace ehp
         return hipGetLastError();
     }
 
-    size_t siddon_tile_lds_bytes(const Dims& dims, const TileCfg& tc)
+    size_t siddon_tile_lds_bytes(const Dims& dims, const TileCfg& tc, int batch)
     {
-        return (size_t) tc.LA * tc.TB * (dims.dim == 3 ? tc.TC : 1) * sizeof(long long) + kBatchLdsBytes;
+        return (size_t) tc.LA * tc.TB * (dims.dim == 3 ? tc.TC : 1) * sizeof(long long) + batch_lds_bytes(batch);
     }
 
     template <int D, int A>
@@ -84,7 +84,7 @@ namespace ehp
         const TileBox t = make_tile<D, A>(dims, tc, tile % ntA, (tile / ntA) % ntB, tile / (ntA * ntB));
         const int nslots = tc.LA * tc.TB * (D == 3 ? tc.TC : 1);
         long long* iacc = lds;
-        const BatchLds bl = batch_lds(lds, nslots);
+        const BatchLds bl = batch_lds(lds, nslots, batch);
         CandRect* s_rect = bl.rect;
         int *s_cnt = bl.cnt, *s_pref = bl.pref;
         for (int i = tid; i < nslots; i += kThreads)
@@ -208,7 +208,7 @@ namespace ehp
             e = fixed_point_exponent_launch(sino, dims.nrays, slot_bound, d_maxbits, d_fixexp, s);
         if (e != hipSuccess)
             return e;
-        const size_t lds = siddon_tile_lds_bytes(dims, tc);
+        const size_t lds = siddon_tile_lds_bytes(dims, tc, batch);
         for (int a = 0; a < dims.dim; ++a) {
             if (!(axes_present & (1 << a)))
                 continue;
