@@ -11,8 +11,10 @@
 #include "ehp_op.hpp"
 #include "ehp_solvers.hpp"
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <mutex>
 #include <vector>
 
 int ehp_solver_fail(int code, const char* msg);
@@ -42,6 +44,7 @@ struct ehp_batch {
     unsigned int* d_maxbits = nullptr;
     int* d_fixexp = nullptr;
     std::vector<void*> allocs;
+    ehp::OrderGuard order; // fused passes: ordered with each other and with the operators' own calls
 
     ~ehp_batch()
     {
@@ -182,7 +185,29 @@ namespace
             }
             return EHP_OK;
         }
-        hipError_t e;
+        // A fused pass rewrites the operators' views and the batch's scale: it runs after the
+        // previous call on the batch and on every operator of it, and the operators' next calls
+        // run after it. The mutexes are taken in address order, so two callers cannot deadlock.
+        std::vector<std::mutex*> mus{&b->order.mu};
+        for (ehp_op* op : b->ops)
+            mus.push_back(&op->order.mu);
+        std::sort(mus.begin(), mus.end());
+        std::vector<std::unique_lock<std::mutex>> locks;
+        locks.reserve(mus.size());
+        for (std::mutex* m : mus)
+            locks.emplace_back(*m);
+        const bool capturing = ehp::stream_capturing(s);
+        hipError_t e = ehp::order_begin(b->order, s, capturing);
+        std::vector<const ehp::SharedEvent*> waited;
+        for (size_t j = 0; e == hipSuccess && j < b->ops.size(); ++j) {
+            const ehp::OrderGuard& g = b->ops[j]->order;
+            if (!g.last || std::find(waited.begin(), waited.end(), g.last.get()) != waited.end())
+                continue;
+            waited.push_back(g.last.get());
+            e = ehp::order_begin(g, s, capturing);
+        }
+        if (e != hipSuccess)
+            return bfail(EHP_ERR_HIP, "ordering after the previous calls: %s", hipGetErrorString(e));
         if (!adjoint) {
             e = ehp::joseph_forward_batch(b->dim, b->d_cjobs, b->d_cblocks, b->ncblocks, b->d_fjobs, b->d_fblocks,
                                           b->nfblocks, (const float*) in, (float*) out, s);
@@ -196,6 +221,14 @@ namespace
         if (e != hipSuccess)
             return bfail(EHP_ERR_HIP, "%s launch: %s", adjoint ? "batch adjoint" : "batch forward",
                          hipGetErrorString(e));
+        e = ehp::order_end(b->order, s, capturing);
+        if (e != hipSuccess)
+            return bfail(EHP_ERR_HIP, "recording the end of the pass: %s", hipGetErrorString(e));
+        if (!capturing)
+            for (ehp_op* op : b->ops) {
+                op->order.last = b->order.last;
+                op->order.last_stream = s;
+            }
         return EHP_OK;
     }
 

@@ -12,6 +12,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -607,7 +608,7 @@ void ehp_destroy(ehp_op* op)
     for (void* p : {(void*) op->d_geo, op->d_lut, (void*) op->d_pose_axes, (void*) op->d_table,
                     (void*) op->d_sidtable, (void*) op->d_pose_margin, (void*) op->d_maxbits,
                     (void*) op->d_fixexp, op->d_view[0], op->d_view[1], op->d_view[2], (void*) op->d_acc32,
-                    (void*) op->d_pix})
+                    (void*) op->d_pix, op->d_stage})
         if (p)
             (void) hipFree(p);
     if (cur >= 0 && cur != op->device)
@@ -634,6 +635,33 @@ int ehp_sample_count(const ehp_op* op, uint64_t* samples)
     return EHP_OK;
 }
 
+static hipError_t launch(ehp_op* op, const void* in, void* out, hipStream_t s, bool adjoint);
+
+// Whether the kernels of this direction add into the output with global atomics (the
+// one-thread-per-ray / per-voxel scatters) rather than writing each element once.
+static bool scatters_into_output(const ehp_op* op, bool adjoint)
+{
+    if (op->dtype == EHP_BF16)
+        return false;
+    if (op->kind == KIND_VOXEL)
+        return !adjoint && !(op->dtype == EHP_F32 && op->tiled);
+    if (!adjoint)
+        return false;
+    return !(op->dtype == EHP_F32 && op->tiled);
+}
+
+// Memory from hipMalloc (not managed, not host memory mapped for the device).
+static bool plain_device_memory(const void* p)
+{
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void) hipGetLastError(); // clear the error of an unknown pointer
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice && !a.isManaged;
+}
+
+// Argument checks, then the launches in order after the previous call on the operator.
 static int apply(ehp_op* op, const void* in, void* out, void* stream, bool adjoint)
 {
     if (!op)
@@ -649,6 +677,51 @@ static int apply(ehp_op* op, const void* in, void* out, void* stream, bool adjoi
     if (cur != op->device)
         return fail(EHP_ERR_ARGUMENT, "operator lives on device %d, current device is %d", op->device, cur);
     hipStream_t s = (hipStream_t) stream;
+    std::lock_guard<std::mutex> lock(op->order.mu);
+    const bool capturing = ehp::stream_capturing(s);
+    e = ehp::order_begin(op->order, s, capturing);
+    if (e != hipSuccess)
+        return hip_fail(e, "ordering after the previous call on this operator");
+    // Atomic scatters go to plain device memory: into the output itself when it is that,
+    // else into the operator's staging buffer and then one copy (hardware float atomics are
+    // not guaranteed on managed or host-coherent memory).
+    void* dst = out;
+    size_t out_bytes = 0;
+    if (scatters_into_output(op, adjoint) && !plain_device_memory(out)) {
+        out_bytes = (size_t) (adjoint ? op->dims.nvox : op->dims.nrays) * dtype_size(op->dtype);
+        if (op->stage_bytes < out_bytes) {
+            if (capturing)
+                return fail(EHP_ERR_UNSUPPORTED, "an output outside plain device memory cannot be captured "
+                                                 "(the operator would allocate its staging buffer)");
+            (void) hipFree(op->d_stage);
+            op->d_stage = nullptr;
+            op->stage_bytes = 0;
+            e = hipMalloc(&op->d_stage, out_bytes);
+            if (e != hipSuccess)
+                return fail(EHP_ERR_ALLOC, "hipMalloc of a %zu-byte staging buffer failed: %s", out_bytes,
+                            hipGetErrorString(e));
+            op->stage_bytes = out_bytes;
+        }
+        dst = op->d_stage;
+    }
+    e = launch(op, in, dst, s, adjoint);
+    if (e != hipSuccess)
+        return hip_fail(e, adjoint ? "adjoint launch" : "forward launch");
+    if (dst != out) {
+        e = hipMemcpyAsync(out, dst, out_bytes, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess)
+            return hip_fail(e, "copy of the staged output");
+    }
+    e = ehp::order_end(op->order, s, capturing);
+    if (e != hipSuccess)
+        return hip_fail(e, "recording the end of the call");
+    return EHP_OK;
+}
+
+// The kernels of one forward or back projection, enqueued on s.
+static hipError_t launch(ehp_op* op, const void* in, void* out, hipStream_t s, bool adjoint)
+{
+    hipError_t e = hipSuccess;
     const ehp::Dims& dm = op->dims;
     if (op->dtype == EHP_BF16 && op->kind == KIND_VOXEL) {
         // bfloat16 storage, FP32 arithmetic: the tiled forward rounds each pixel's fixed-point
@@ -716,9 +789,7 @@ static int apply(ehp_op* op, const void* in, void* out, void* stream, bool adjoi
                         : ehp::voxel_forward<double>(op->d_geo, dm, (const double*) op->d_lut, op->n_lut, op->support,
                                                      op->differential, x, y, s);
     }
-    if (e != hipSuccess)
-        return hip_fail(e, adjoint ? "adjoint launch" : "forward launch");
-    return EHP_OK;
+    return e;
 }
 
 int ehp_forward(ehp_op* op, const void* d_vol, void* d_sino, void* stream)
@@ -737,6 +808,7 @@ int ehp_tile(ehp_op* op, int la, int tb, int tc, int* la_out, int* tb_out, int* 
         return fail(EHP_ERR_ARGUMENT, "operator is NULL");
     if (!op->tiled)
         return fail(EHP_ERR_ARGUMENT, "operator does not use a tiled kernel");
+    std::lock_guard<std::mutex> lock(op->order.mu);
     if (la > 0) {
         const bool vox = op->kind == KIND_VOXEL;
         if (tb <= 0 || (op->dims.dim == 3 && !vox && tc <= 0))
@@ -767,6 +839,7 @@ int ehp_pose_batch(ehp_op* op, int batch, int* batch_out)
         return fail(EHP_ERR_ARGUMENT, "operator is NULL");
     if (!op->tiled || op->kind == KIND_VOXEL)
         return fail(EHP_ERR_ARGUMENT, "operator has no tiled back projection");
+    std::lock_guard<std::mutex> lock(op->order.mu);
     if (batch > 0) {
         if (batch > ehp::kMaxPoseBatch)
             return fail(EHP_ERR_ARGUMENT, "pose batch %d exceeds %d", batch, ehp::kMaxPoseBatch);

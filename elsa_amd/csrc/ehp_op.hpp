@@ -3,6 +3,7 @@
 #pragma once
 
 #include "ehp_kernels.hpp"
+#include "ehp_order.hpp"
 
 #include <vector>
 
@@ -47,4 +48,7 @@ struct ehp_op {
     double slot_bound = 0.0;
     int batch = ehp::kPoseBatch;
     std::vector<ehp::PoseGeom> host_geo;
+    void* d_stage = nullptr; // atomic scatters whose output is not plain device memory go here first
+    size_t stage_bytes = 0;
+    ehp::OrderGuard order;   // calls on this operator run one after another on the device
 };

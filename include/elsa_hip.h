@@ -29,6 +29,14 @@
  *    begin pointer, never the parent storage). Outputs are overwritten, never accumulated.
  *  - Calls are asynchronous on the given HIP stream (hipStream_t passed as void*; NULL is
  *    the default stream). Inputs must stay valid until the stream reaches the call.
+ *  - Calls on one operator are ordered by the library, whatever stream or host thread they
+ *    come from: an operator owns device scratch that each call rewrites (volume views, the
+ *    fixed-point scale of a back projection), so a call enqueues under the operator's mutex
+ *    and its stream first waits for the event recorded after the operator's previous call
+ *    when that call went to another stream. A fused batch pass is ordered the same way with
+ *    the batch and every operator in it. Calls on different operators run concurrently.
+ *    Inside a stream capture (hipGraph) the capture's own order applies; replays of a
+ *    captured graph are ordered by the caller.
  *  - Host-only helpers (ehp_geometry_pose_*, ehp_*_lut) never touch the GPU.
  */
 #ifndef ELSA_HIP_H
@@ -227,8 +235,8 @@ int ehp_power_iterations(ehp_op* op, int niters, double* L, void* stream);
  * the default (fast) kernels, every projector pass is one launch over all problems (plus the
  * view copies, the zero fill and the fixed-point scale of the back projection); otherwise the
  * batch calls the operators one after another. The operators must outlive the batch; tile
- * shapes are taken at creation; a batch and its operators share device state, so calls on
- * either must not overlap. */
+ * shapes are taken at creation; a batch and its operators share device state, and the library
+ * orders calls on either (see Conventions). */
 int ehp_batch_create(ehp_op* const* ops, int n, ehp_batch** out);
 void ehp_batch_destroy(ehp_batch* b);
 

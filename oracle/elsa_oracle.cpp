@@ -537,6 +537,10 @@ namespace orc
             for (i64 g = 0; g < sh.nviews; ++g) {
                 for (i64 vox = 0; vox < sh.nvox; ++vox) {
                     const T xv = in[vox];
+                    // a zero voxel adds +-0 to each of its pixels, which leaves every sum as it
+                    // is: skipping it changes no bit and lets sparse volumes run fast
+                    if (xv == T(0))
+                        continue;
                     visit_footprint<R, D>(sh, P + g * pstride, ext + g * pstride, sdd[g], support,
                                           vox, g, [&](i64 pix, R dist, R d0) {
                                               out[pix] += xv * tap_weight<R, T, D>(lut, N, support, dist, d0, diff);
