@@ -28,6 +28,10 @@ namespace ehp
 #endif
     }
 
+    // Fixed-point exponent of a back projection whose largest |y| is not finite: the whole
+    // output becomes NaN (integer sums cannot carry a NaN to just the voxels it touches).
+    constexpr int kNonFinite = 1 << 20;
+
     // Tile of the LDS-privatised Joseph kernels: LA planes along the driving axis, TB x TC
     // voxels across (TC unused in 2D).
     struct TileCfg {
@@ -190,66 +194,198 @@ namespace ehp
         return box_bbox<D>(g, lo, hi, dims, u0, u1, v0, v1);
     }
 
-    // Back projection of sample s of one ray (driving axis A, transverse start p1, p2 and
-    // steps s1, s2) lying on tile plane pa: if its bilinear footprint touches the tile, hand
-    // ys * weight for each of its four (2D: two) slots to add(offset, value), offset being the
-    // slot's byte offset in the tile's array of 8-byte slots and ys the ray's y * len in the
-    // ray's own fixed-point scale (ray_scale). Weights are elsa's bilinear weights
-    // (doInterpolation, JosephsMethod.cpp:60-97), the driving weight being 1; a 3D tap's
-    // weight is applied as (ys * w2) * w1. The footprint's first slot is (pa * PC + i2) * PB + i1
-    // with i1 = f1 - (B0 - 1), i2 = f2 - (C0 - 1).
-    template <int D, class Add>
-    EHP_HD inline void tile_sample_adjoint(float p1, float p2, float s1, float s2, int s, int pa, const TileBox& t,
-                                           float ys, Add add)
+    // ---------------------------------------------------------------------------------
+    // back projection of one ray into a tile (the matched adjoint of the forward's samples)
+
+    // floor(x) as an int in one instruction (v_cvt_flr_i32_f32); |x| < 2^31
+    EHP_HD inline int cvt_flr(float x)
     {
-        const float fs = (float) s;
-        const float q1 = __builtin_fmaf(fs, s1, p1);
-        const float g1 = floorf(q1 - 0.5f);
-        // f1 - (B0 - 1), exact in float (both integers); negative or beyond the tile: rejected
-        const uint32_t i1 = (uint32_t) (int) (g1 - t.B0m1f);
-        if (i1 > (uint32_t) (t.B1 - t.B0)) // f1 outside [B0 - 1, B1 - 1]
+#if defined(__HIP_DEVICE_COMPILE__)
+        int r;
+        asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+        return r;
+#else
+        return (int) floorf(x);
+#endif
+    }
+
+    // x rounded to the nearest int, halves up, in one instruction (v_cvt_rpi_i32_f32); |x| < 2^30
+    EHP_HD inline int cvt_rpi(float x)
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        int r;
+        asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+        return r;
+#else
+        return (int) floorf(x + 0.5f);
+#endif
+    }
+
+    // x - floor(x) (v_fract_f32); exact for finite x >= 0
+    EHP_HD inline float fract_pos(float x)
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        return __builtin_amdgcn_fractf(x);
+#else
+        return x - floorf(x);
+#endif
+    }
+
+    // a + b + c in one instruction (v_add3_u32), c uniform; keeps the compiler from splitting the
+    // slot offset back into its parts
+    EHP_HD inline uint32_t add3_u32(uint32_t a, uint32_t b, uint32_t c)
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        uint32_t r;
+        asm("v_add3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(c));
+        return r;
+#else
+        return a + b + c;
+#endif
+    }
+
+    // a * b + c for |a|, |b| < 2^23 in one instruction (v_mad_i32_i24), b uniform, c in a
+    // vector register (one scalar operand per instruction)
+    EHP_HD inline int mad_i24(int a, int b, int c)
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        int r;
+        asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(b), "v"(c));
+        return r;
+#else
+        return a * b + c;
+#endif
+    }
+
+    // Whether p holds on every active lane of the wavefront (one lane on the CPU).
+    EHP_HD inline bool wave_all(bool p)
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        return __all(p);
+#else
+        return p;
+#endif
+    }
+
+    // Offsets of a tile's LDS slots (bytes; slot (pa, i2, i1) at base + pa * plane + i2 * pb8 + 8 i1) in
+    // the form the per-sample code adds them: a sample with floor indices g1, g2 touches the
+    // tile when g1 is in [B0 - 1, B1 - 1] and g2 in [C0 - 1, C1 - 1], that is when
+    // o1 = 8 g1 + nb1 <= max1 and o2 = g2 pb8 + nb2 <= max2 as unsigned numbers.
+    struct TileOff {
+        uint32_t nb1, max1;
+        int nb2;
+        uint32_t max2;
+        int pb8;
+        uint32_t plane;
+        uint32_t base; // byte address of slot 0 (the kernels: in LDS; 0 on the CPU)
+    };
+
+    EHP_HD inline TileOff tile_offsets(const TileBox& t, uint32_t base = 0)
+    {
+        TileOff o;
+        o.base = base;
+        o.pb8 = t.PB * 8;
+        o.nb1 = (uint32_t) (-8 * (t.B0 - 1));
+        o.max1 = (uint32_t) (8 * (t.B1 - t.B0));
+        o.nb2 = -(t.C0 - 1) * o.pb8;
+#if defined(__HIP_DEVICE_COMPILE__)
+        // mad_i24 takes nb2 from a vector register: copy it there once, not in every sample
+        asm("v_mov_b32 %0, %1" : "=v"(o.nb2) : "s"(o.nb2));
+#endif
+        o.max2 = (uint32_t) ((t.C1 - t.C0) * o.pb8);
+        o.plane = (uint32_t) (t.PC * o.pb8);
+        return o;
+    }
+
+    // Whether elsa's weight form must be evaluated as written in this tile: with B0 >= 1 (and
+    // C0 >= 1) every sample the tile keeps has q - 0.5 >= 0, where q - 0.5 is exact and
+    // c = (q - floor(q - 0.5)) - 0.5 equals fract(q - 0.5) bit for bit; samples of the tiles on
+    // the volume's low faces may have q < 0.5.
+    template <int D>
+    EHP_HD inline bool tile_low_face(const TileBox& t)
+    {
+        return t.B0 == 0 || (D == 3 && t.C0 == 0);
+    }
+
+    // Back projection of the sample at fs = s (as a float) of ray r lying on the tile plane at
+    // byte offset poff: if its bilinear footprint touches the tile, hand ys * weight for each of
+    // its four (2D: two) slots to add(offset, value), ys being y * len in the ray's own scale
+    // (ray_scale). The position
+    // and weights are elsa's (doInterpolation, JosephsMethod.cpp:60-97; driving weight 1) with the
+    // forward's position rule fma(s, step, pos0), so they equal the forward's bit for bit; a 3D
+    // tap's weight is applied as (ys * w2) * w1. LOW: evaluate c as elsa writes it (tiles on the
+    // low faces, tile_low_face); otherwise as fract(q - 0.5), the same value there.
+    template <int D, bool LOW, class Add>
+    EHP_HD inline void tile_sample_fx(const RayRec& r, float fs, uint32_t poff, const TileOff& to, float ys, Add add)
+    {
+        const float q1 = __builtin_fmaf(fs, r.s1, r.p1);
+        const float t1 = q1 - 0.5f;
+        const int g1 = cvt_flr(t1);
+        const uint32_t o1 = ((uint32_t) g1 << 3) + to.nb1;
+        if (o1 > to.max1)
             return;
-        const float c1 = (q1 - g1) - 0.5f;
-        const float w1 = 1.0f - c1;
-        const uint32_t pb8 = (uint32_t) t.PB * 8u;
         if (D == 2) {
-            const uint32_t off = (i1 << 3) + mul24((uint32_t) pa, pb8);
+            const float c1 = LOW ? (q1 - (float) g1) - 0.5f : fract_pos(t1);
+            const float w1 = 1.0f - c1;
+            const uint32_t off = o1 + poff;
             add(off, ys * w1);
             add(off + 8u, ys * c1);
         } else {
-            const float q2 = __builtin_fmaf(fs, s2, p2);
-            const float g2 = floorf(q2 - 0.5f);
-            const uint32_t i2 = (uint32_t) (int) (g2 - t.C0m1f);
-            if (i2 > (uint32_t) (t.C1 - t.C0)) // f2 outside [C0 - 1, C1 - 1]
+            const float q2 = __builtin_fmaf(fs, r.s2, r.p2);
+            const float t2 = q2 - 0.5f;
+            const int g2 = cvt_flr(t2);
+            const uint32_t o2 = (uint32_t) mad_i24(g2, to.pb8, to.nb2);
+            if (o2 > to.max2)
                 return;
-            const float c2 = (q2 - g2) - 0.5f;
-            const float w2 = 1.0f - c2;
-            const uint32_t off = mul24(i2, pb8) + mul24((uint32_t) pa, (uint32_t) t.PC * pb8) + (i1 << 3);
+            const float c1 = LOW ? (q1 - (float) g1) - 0.5f : fract_pos(t1);
+            const float c2 = LOW ? (q2 - (float) g2) - 0.5f : fract_pos(t2);
+            const float w1 = 1.0f - c1, w2 = 1.0f - c2;
+            const uint32_t off = add3_u32(o1, o2, poff);
+            const uint32_t off2 = off + (uint32_t) to.pb8;
             const float yw = ys * w2, yc = ys * c2;
             add(off, yw * w1);
             add(off + 8u, yw * c1);
-            add(off + pb8, yc * w1);
-            add(off + pb8 + 8u, yc * c1);
+            add(off2, yc * w1);
+            add(off2 + 8u, yc * c1);
         }
     }
 
-    // Back projection of one ray (driving axis A) into a tile: for each plane of the tile on
-    // which the ray has a sample (s in [0, nsteps)), tile_sample_adjoint, which keeps the
-    // samples whose footprint touches the tile's slots. Every ray visits the tile's planes in
-    // the same order, so that the lanes of a wavefront address the same LDS plane at each
-    // step; the slots hold integer sums, which the order of the additions does not change.
-    template <int D, int A, class Add>
-    EHP_HD inline void tile_ray_adjoint(const RayRec& r, const TileBox& t, float ys, Add add)
+    // Plane walk of one ray through the la planes of a tile: s0 is the ray's sample index on the
+    // tile's first plane and sg the step of the sample index from plane to plane. Each sample in
+    // [0, nsteps) goes to tile_sample_fx. FULL: the caller knows that all la samples lie in
+    // [0, nsteps) (no test per plane). Every ray visits the planes in the same order, so the
+    // lanes of a wavefront address one LDS plane at each step.
+    template <int D, bool LOW, bool FULL, class Add>
+    EHP_HD inline void tile_ray_fx(const RayRec& r, int s0, int sg, int la, const TileOff& to, float ys, Add add)
     {
-        if ((r.meta & 3) != A || r.nsteps == 0)
+        float fs = (float) s0;
+        const float fsg = (float) sg;
+        int s = s0;
+        uint32_t poff = to.base;
+        for (int pa = 0; pa < la; ++pa, fs += fsg, s += sg, poff += to.plane)
+            if (FULL || (uint32_t) s < (uint32_t) r.nsteps)
+                tile_sample_fx<D, LOW>(r, fs, poff, to, ys, add);
+    }
+
+    // Back projection of one candidate ray into a tile (planes [A0, A1) along A): rays of
+    // another driving axis, without samples or with y = 0 add nothing. ys: y * len in the ray's
+    // scale (ray_scale). Takes the test-free plane walk when every active lane of the wavefront
+    // has samples on all the tile's planes.
+    template <int D, int A, bool LOW, class Add>
+    EHP_HD inline void tile_candidate_fx(const RayRec& r, const TileBox& t, const TileOff& to, float ys, Add add)
+    {
+        if ((r.meta & 3) != A || r.nsteps == 0 || ys == 0.0f)
             return;
         const bool neg = (r.meta & 4) != 0;
         const int sg = neg ? -1 : 1;
-        int s = neg ? r.P0 - t.A0 : t.A0 - r.P0; // the ray's sample on the tile's first plane
+        const int s0 = neg ? r.P0 - t.A0 : t.A0 - r.P0; // the ray's sample on the tile's first plane
         const int la = t.A1 - t.A0;
-        for (int pa = 0; pa < la; ++pa, s += sg)
-            if ((uint32_t) s < (uint32_t) r.nsteps)
-                tile_sample_adjoint<D>(r.p1, r.p2, r.s1, r.s2, s, pa, t, ys, add);
+        const int s1 = s0 + (la - 1) * sg;
+        const int lo = s0 < s1 ? s0 : s1, hi = s0 < s1 ? s1 : s0;
+        if (wave_all(lo >= 0 && hi < r.nsteps))
+            tile_ray_fx<D, LOW, true>(r, s0, sg, la, to, ys, add);
+        else
+            tile_ray_fx<D, LOW, false>(r, s0, sg, la, to, ys, add);
     }
 
     // Linear volume index of LDS slot i if the tile owns that voxel, else -1.
@@ -394,7 +530,6 @@ namespace ehp
     // Fixed-point format of a back projection: slot sums are 64-bit integers in units of
     // q = 2^(eg - 62), where 2^eg > max|y| * bound, so no sum reaches 2^62 in magnitude.
     // Returns eg; 0 when max|y| is 0 (nothing to add), kNonFinite when it is not finite.
-    constexpr int kNonFinite = 1 << 20;
 
     EHP_HD inline int fixed_point_exponent(float maxabs, double bound)
     {
@@ -427,6 +562,12 @@ namespace ehp
     EHP_HD inline unsigned long long fixed_contrib(float t, int sh)
     {
         return (unsigned long long) (long long) (int) t << sh;
+    }
+
+    // The same, rounded to nearest (halves up) in one instruction (v_cvt_rpi_i32_f32).
+    EHP_HD inline unsigned long long fixed_contrib_rn(float t, int sh)
+    {
+        return (unsigned long long) (long long) cvt_rpi(t) << sh;
     }
 
     inline void tile_counts(const Dims& dims, const TileCfg& tc, int a, int& ntA, int& ntB, int& ntC)

@@ -11,12 +11,13 @@
 // Accumulation: LDS float atomics are slow on this part (a ds_add_f32 wave operation costs
 // about 15 times a ds_add_u64), so the tile is summed with 64-bit integer atomics in fixed
 // point: units of q = 2^(eg - 62), eg chosen per call from max|y| and a bound on what one slot
-// can receive, so that no sum can overflow. Each ray truncates its contributions to int32 in
-// its own power-of-two scale (error below 2^-29 of y * len) and shifts them into units of q,
-// which costs a few instructions instead of a float-to-int64 conversion. Integer sums do not
-// depend on the order of the additions, so the result is bitwise reproducible, and they are
-// more accurate than a sequential FP32 accumulation. A non-finite y makes the whole output
-// NaN (the fixed-point sums cannot carry it to just the voxels it touches).
+// can receive, so that no sum can overflow. Each ray rounds its contributions to int32 in its
+// own power-of-two scale (error at most 2^-30 of y * len) and shifts them into units of q: a
+// conversion, a sign extension and a 64-bit shift per tap instead of a float-to-int64
+// conversion. Integer sums do not depend on the order of the additions, so the result is
+// bitwise reproducible, whatever the tile shape, and more accurate than a sequential FP32
+// accumulation. A non-finite y makes the whole output NaN (the fixed-point sums cannot carry
+// it to just the voxels it touches).
 //
 // The per-ray state comes from a table built once per operator from the same functions as
 // the one-thread-per-ray kernels (make_ray, joseph_setup), and the per-sample weights use the
@@ -28,6 +29,9 @@ namespace ehp
     namespace
     {
         constexpr int kThreads = 256;
+        // LDS (address space 3) pointers: 32-bit byte addresses
+        typedef __attribute__((address_space(3))) unsigned long long lds_u64;
+        typedef __attribute__((address_space(3))) char lds_char;
     } // namespace
 
     // ---------------------------------------------------------------------------------
@@ -129,6 +133,63 @@ namespace ehp
     // ---------------------------------------------------------------------------------
     // the tiled back projection
 
+    // The candidate rays of one pose batch of one tile, all added into the tile's slots.
+    // Candidate k (k < total) is pixel k - pref[j] of pose j's rectangle, j the last pose with
+    // pref[j] <= k. Each lane walks its candidates k = tid, tid + 256, ... in order, so it keeps
+    // its pose j (and j's rectangle) in registers and moves on when k passes pref[j + 1]. The
+    // ray state and y of the next candidate are loaded while the current one is added (two
+    // register sets taken in turn).
+    template <int D, int A, bool LOW, class ST>
+    __device__ __forceinline__ void adj_tile_batch(const RayRec* __restrict__ table, const ST* __restrict__ sino,
+                                                   const int det0, const CandRect* s_rect, const int* s_pref,
+                                                   const int total, const TileBox& t, const TileOff& to,
+                                                   const int eg)
+    {
+        const int tid = threadIdx.x;
+        int j = 0, pj = 0, pn = s_pref[1];
+        CandRect cr = s_rect[0];
+        auto fetch = [&](int k, RayRec& r, float& yv) {
+            while (k >= pn) {
+                ++j;
+                pj = pn;
+                pn = s_pref[j + 1];
+                cr = s_rect[j];
+            }
+            const int loc = k - pj;
+            // loc / nu through the reciprocal: exact, as loc + 0.5 is at least 0.5 away from a
+            // multiple of nu and loc < 2^22 (the operator keeps pixels per pose below 2^22)
+            const uint32_t row = (uint32_t) (((float) loc + 0.5f) * cr.rnu);
+            const uint32_t col = (uint32_t) loc - mul24(row, (uint32_t) cr.nu);
+            const int64_t ray = cr.base + (int64_t) (mul24(row, (uint32_t) det0) + col);
+            yv = up(sino[ray]);
+            r = table[ray];
+        };
+        auto process = [&](const RayRec& r, float yv) {
+            float ys;
+            int sh;
+            ray_scale(yv * r.len, eg, ys, sh);
+            // off: the slot's LDS byte address (the tile's base included)
+            tile_candidate_fx<D, A, LOW>(r, t, to, ys, [&](uint32_t off, float val) {
+                __hip_atomic_fetch_add((lds_u64*) (uintptr_t) off, fixed_contrib_rn(val, sh), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+            });
+        };
+        RayRec ra{}, rb{};
+        float ya = 0.0f, yb = 0.0f;
+        if (tid < total)
+            fetch(tid, ra, ya);
+        for (int k = tid; k < total; k += 2 * kThreads) {
+            const int k1 = k + kThreads, k2 = k + 2 * kThreads;
+            if (k1 < total)
+                fetch(k1, rb, yb);
+            process(ra, ya);
+            if (k2 < total)
+                fetch(k2, ra, ya);
+            if (k1 < total)
+                process(rb, yb);
+        }
+    }
+
     // One tile of the back projection along driving axis A (tile < ntA * ntB * ntC).
     template <int D, int A, class ST = float>
     __device__ __forceinline__ void adj_tiled_block(const PoseGeom* __restrict__ geo,
@@ -142,6 +203,8 @@ namespace ehp
 
         const int tid = threadIdx.x;
         const TileBox t = make_tile<D, A>(dims, tc, tile % ntA, (tile / ntA) % ntB, tile / (ntA * ntB));
+        const TileOff to = tile_offsets(t, (uint32_t) (uintptr_t) (lds_char*) lds);
+        const bool low = tile_low_face<D>(t);
         long long* iacc = lds;
         const BatchLds bl = batch_lds(lds, t.nslots, batch);
         CandRect* s_rect = bl.rect;
@@ -175,44 +238,10 @@ namespace ehp
             }
             __syncthreads();
             const int total = s_pref[batch];
-            // ray index of candidate k of this batch: pixel (col, row) of its pose's rectangle
-            auto cand_ray = [&](int k) -> int64_t {
-                const int j = batch_pose_slot(s_pref, batch, k);
-                const int loc = k - s_pref[j];
-                const CandRect cr = s_rect[j];
-                // loc / nu through the reciprocal: exact, as loc + 0.5 is at least 0.5 away from
-                // a multiple of nu and loc < 2^22
-                const uint32_t row = (uint32_t) (((float) loc + 0.5f) * cr.rnu);
-                const uint32_t col = (uint32_t) loc - mul24(row, (uint32_t) cr.nu);
-                return cr.base + (int64_t) (mul24(row, (uint32_t) dims.det[0]) + col);
-            };
-            // the loads of the next candidate are issued before the current one is processed,
-            // so that their latency overlaps its samples
-            float yv_next = 0.0f;
-            RayRec r_next{};
-            if (tid < total) {
-                const int64_t ray = cand_ray(tid);
-                yv_next = up(sino[ray]);
-                r_next = table[ray];
-            }
-            for (int k = tid; k < total; k += kThreads) {
-                const float yv = yv_next;
-                const RayRec r = r_next;
-                if (k + kThreads < total) {
-                    const int64_t ray = cand_ray(k + kThreads);
-                    yv_next = up(sino[ray]);
-                    r_next = table[ray];
-                }
-                if (yv == 0.0f)
-                    continue;
-                float ys;
-                int sh;
-                ray_scale(yv * r.len, eg, ys, sh);
-                tile_ray_adjoint<D, A>(r, t, ys, [&](uint32_t off, float val) {
-                    atomicAdd(reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(iacc) + off),
-                              fixed_contrib(val, sh));
-                });
-            }
+            if (low)
+                adj_tile_batch<D, A, true>(table, sino, dims.det[0], s_rect, s_pref, total, t, to, eg);
+            else
+                adj_tile_batch<D, A, false>(table, sino, dims.det[0], s_rect, s_pref, total, t, to, eg);
             __syncthreads();
         }
 

@@ -2,7 +2,8 @@
 //
 // Built with g++ against elsa_amd/csrc/ehp_joseph_tile.hpp (the same functions the HIP
 // kernels call), it builds the per-ray table, walks every tile of every pass the way a
-// workgroup does -- pose batches, 64-bit fixed-point sums -- with bounds checks on every LDS
+// workgroup does -- pose batches, the low-face and test-free plane walks, contributions rounded
+// to int32 in each ray's scale, 64-bit fixed-point sums -- with bounds checks on every LDS
 // slot and volume index, a check that no sum leaves the int64 range, and a check that each
 // voxel is written by exactly one tile per pass. It also runs the plane-major forward
 // (plane_layout, plane_view, plane_ray_forward) over every layout a driving axis can get.
@@ -46,6 +47,16 @@ namespace
         double max_ratio = 0.0; // largest |slot sum| / 2^63 seen
     };
 
+    // one candidate ray of a tile, LOW chosen as the kernel chooses it per tile
+    template <int D, int A, class Add>
+    void candidate(const RayRec& r, const TileBox& t, const TileOff& to, float ys, Add add)
+    {
+        if (tile_low_face<D>(t))
+            tile_candidate_fx<D, A, true>(r, t, to, ys, add);
+        else
+            tile_candidate_fx<D, A, false>(r, t, to, ys, add);
+    }
+
     template <int D, int A>
     int run_pass(const Dims& dims, const std::vector<PoseGeom>& geo, const std::vector<RayRec>& table,
                  const std::vector<int>& axes, TileCfg tc, int eg, const float* sino, float* vol, Stats& st)
@@ -55,6 +66,7 @@ namespace
         tile_counts(dims, tc, A, ntA, ntB, ntC);
         for (int tile = 0; tile < ntA * ntB * ntC; ++tile) {
             const TileBox t = make_tile<D, A>(dims, tc, tile % ntA, (tile / ntA) % ntB, tile / (ntA * ntB));
+            const TileOff to = tile_offsets(t);
             std::vector<__int128> iacc(t.nslots, 0);
             for (int pbase = 0; pbase < dims.nposes; pbase += kPoseBatch) {
                 for (int p = pbase; p < dims.nposes && p < pbase + kPoseBatch; ++p) {
@@ -79,18 +91,18 @@ namespace
                             ray_scale(yv * table[ray].len, eg, ys, sh);
                             if (sh < 0 || sh > 32)
                                 bad = true;
-                            tile_ray_adjoint<D, A>(table[ray], t, ys, [&](uint32_t off, float val) {
+                            candidate<D, A>(table[ray], t, to, ys, [&](uint32_t off, float val) {
                                 const int64_t slot = off / 8;
                                 if (off % 8 != 0 || slot >= t.nslots) {
                                     bad = true;
                                     return;
                                 }
                                 // the contribution must round to an int32 of magnitude <= 2^30
-                                if (!(fabsf(rintf(val)) <= 1073741824.0f)) {
+                                if (!(fabsf(val) < 1073741824.0f)) {
                                     bad = true;
                                     return;
                                 }
-                                iacc[slot] += (__int128) (long long) fixed_contrib(val, sh);
+                                iacc[slot] += (__int128) (long long) fixed_contrib_rn(val, sh);
                             });
                             if (bad) {
                                 std::fprintf(stderr, "LDS slot or contribution out of range (pass %d tile %d pose %d u %d v %d)\n", A,
