@@ -17,7 +17,9 @@ from dataclasses import dataclass
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# ELSA_ORACLE_LIB names another build of the same source (the AddressSanitizer /
+# UndefinedBehaviorSanitizer build of `make -C oracle san`, run under LD_PRELOAD of the runtimes)
+_LIB_PATH = os.environ.get("ELSA_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 JOSEPH, SIDDON = 0, 1

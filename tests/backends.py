@@ -22,11 +22,11 @@ class OracleBackend:
         self.dtype = np.dtype(dtype)
         self.mode = mode
 
-    def setup(self, vol, det, poses):
+    def setup(self, vol, det, poses, spacing=None):
         dim = len(vol)
         ps = [oracle.geometry_pose(dim, p["s2c"], p["c2d"], vol, det, angle_rad=p.get("gamma", 0.0),
-                                   beta=p.get("beta", 0.0), alpha=p.get("alpha", 0.0), ppo=p.get("ppo"),
-                                   cor=p.get("cor"), dtype=self.dtype) for p in poses]
+                                   beta=p.get("beta", 0.0), alpha=p.get("alpha", 0.0), vol_spacing=spacing,
+                                   ppo=p.get("ppo"), cor=p.get("cor"), dtype=self.dtype) for p in poses]
         return oracle.geom_from_poses(dim, vol, det, ps, self.dtype)
 
     def joseph(self, g, x, adjoint=False):
@@ -58,14 +58,15 @@ class GpuBackend:
 
     name = "gpu"
 
-    def __init__(self, dtype=np.float32):
+    def __init__(self, dtype=np.float32, kernels="fast"):
         self.dtype = np.dtype(dtype)
+        self.kernels = kernels
         self._ops = {}
 
-    def setup(self, vol, det, poses):
+    def setup(self, vol, det, poses, spacing=None):
         import elsa_amd as E
 
-        vd = E.VolumeDescriptor(list(vol))
+        vd = E.VolumeDescriptor(list(vol), spacing)
         geoms = [E.Geometry(p["s2c"], p["c2d"], vd, list(det), gamma=p.get("gamma", 0.0), beta=p.get("beta", 0.0),
                             alpha=p.get("alpha", 0.0), principalPointOffset=p.get("ppo"),
                             centerOfRotationOffset=p.get("cor")) for p in poses]
@@ -77,7 +78,7 @@ class GpuBackend:
 
         key = (cls, id(g[1]), tuple(sorted(kw.items())))
         if key not in self._ops:
-            self._ops[key] = getattr(E, cls)(g[0], g[1], dtype=self.dtype, **kw)
+            self._ops[key] = getattr(E, cls)(g[0], g[1], dtype=self.dtype, kernels=self.kernels, **kw)
         return self._ops[key]
 
     def joseph(self, g, x, adjoint=False):

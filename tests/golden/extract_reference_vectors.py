@@ -12,6 +12,9 @@ It copies numbers only (expected outputs of the reference tests), never source t
                arrays of test_BlobGradientHelper.cpp are checked to be the same numbers);
                elsa/projectors/tests/test_BSplines.cpp  ProjectedBSpline derivative LUT,
                order 3, dim 2, 101 values at x = i / 50
+  blob_weights.json
+               elsa/projectors/tests/test_BlobVoxelProjector.cpp "Test weights": the projected
+               blob (a = 2, alpha = 10.83, m = 2, closed form) at distance i / 25, 51 values
 """
 import json
 import os
@@ -99,6 +102,13 @@ def main():
         "bspline_derivative_lut": {"source": "elsa/projectors/tests/test_BSplines.cpp", "order": 3, "dim": 2,
                                    "N": 101, "x": "i / 50", "expected": bsp_d},
     }, open(os.path.join(OUT, "derivatives.json"), "w"), indent=1)
+    bvp_src = open(os.path.join(REF, "elsa/projectors/tests/test_BlobVoxelProjector.cpp")).read()
+    at = bvp_src.index('TEST_CASE_TEMPLATE("BlobVoxelProjector: Test weights"')
+    weights = numbers(brace_block(bvp_src, bvp_src.index("expected", at)))
+    assert len(weights) == 51, len(weights)
+    json.dump({"source": "elsa/projectors/tests/test_BlobVoxelProjector.cpp", "radius": 2.0, "alpha": 10.83,
+               "order": 2, "distance": "i / 25", "epsilon": "doctest Approx default", "expected": weights},
+              open(os.path.join(OUT, "blob_weights.json"), "w"), indent=1)
     print("wrote", os.listdir(OUT))
 
 

@@ -407,7 +407,251 @@ def voxel_backward_2d(B):
             assert approx(x[s, col], w, 1e-4), (col, s, x[s, col], w)
 
 
-VOXEL_KATS = [voxel_centre_voxel_3d, voxel_centre_voxel_2d, voxel_backward_2d]
+# ---- the remaining 2D known answers of test_VoxelProjectorCUDA.cpp, for both basis functions
+# (blob :191-1278, B-spline :1280-2462; the two halves are the same tests with the B-spline's
+# table) and the CPU tests test_BlobVoxelProjector.cpp / test_BSplineVoxelProjector.cpp.
+
+
+def _basis(kind, dtype):
+    """op.blob(s) / op.bspline(s) of elsa (Blobs.h:266-272, BSplines.h:154-160): the 100-entry
+    table at |s|, in the back end's dtype (2D tables)."""
+    import oracle
+
+    if kind == "blob":
+        lut, sup = oracle.blob_lut(2.0, 10.83, 2, 100, dtype), 2.0
+    else:
+        lut, sup = oracle.bspline_lut(2, 2, 100, dtype), 1.5
+    return lambda s: float(oracle.lut_eval(lut, sup, abs(float(s))))
+
+
+def _projected_weight(kind, angle_deg, coord, spacing=None, dtype=np.float64):
+    """The weight elsa's tests expect for voxel (x, y) on the one-pixel detector of the
+    5^2 / s2c 100 / c2d 5 geometry: basis((u - 0.5) / scaling) with u and scaling from
+    projectAndScaleVoxelOnDetector (DetectorDescriptor.cpp:130-151), evaluated in FP64."""
+    import oracle
+
+    og = oracle.geom_from_poses(2, (5, 5), (1,), [oracle.geometry_pose(2, 100.0, 5.0, (5, 5), (1,),
+                                                                       float(np.radians(angle_deg)),
+                                                                       vol_spacing=spacing, dtype=np.float64)],
+                                np.float64)
+    P = og.P[0].reshape(3, 2).T
+    E = og.ext[0].reshape(3, 2).T
+    hv = np.array([coord[0] + 0.5, coord[1] + 0.5, 1.0])
+    pc = P @ hv
+    u = pc[0] / pc[1] - 0.5
+    return _basis(kind, dtype)(u / (og.sdd[0] / np.linalg.norm(E @ hv)))
+
+
+def _vol2d(coords):
+    """A 5^2 volume with the voxels elsa addresses as x(i, j) set to 1 (vol[j, i] here)."""
+    x = np.zeros((5, 5))
+    for i, j in coords:
+        x[j, i] = 1
+    return x
+
+
+def _centre_voxel_spacing(B, kind):
+    """:141-190 (blob) and :1325-1374 (B-spline): with volume spacing 0.5, 1 and 1.5 the centre
+    voxel of 5^2 projects to basis(0) on a one-pixel detector, poses 0, 4, ..., 356 deg."""
+    w0 = _basis(kind, B.dtype)(0.0)
+    for k in (1, 2, 3):
+        sp = (k / 2.0, k / 2.0)
+        for i in range(0, 360, 4):
+            g = B.setup((5, 5), (1,), [pose(100, 5, deg(i))], spacing=sp)
+            y = B.voxel(g, _vol2d([(2, 2)]), kind)
+            assert approx(w0, y.ravel()[0]), (sp, i, y)
+
+
+def _two_pixels(B, kind):
+    """:191-239 / :1375-1423: centre voxel, two-pixel detector: both pixels get
+    basis(0.4761878) (epsilon 0.01), poses 0, 4, ..., 356 deg."""
+    w = _basis(kind, B.dtype)(0.4761878)
+    for i in range(0, 360, 4):
+        g = B.setup((5, 5), (2,), [pose(100, 5, deg(i))])
+        y = B.voxel(g, _vol2d([(2, 2)]), kind).ravel()
+        assert approx(w, y[0], 0.01) and approx(w, y[1], 0.01), (i, y, w)
+
+
+def _three_pixels(B, kind):
+    """:240-296 / :1424-1480: three pixels, poses 0, 5, ..., 355 deg: the centre pixel gets
+    basis(0), the outer ones basis(0.95233786) (epsilon 0.01)."""
+    f = _basis(kind, B.dtype)
+    w0, w1 = f(0.0), f(0.95233786)
+    for i in range(0, 360, 5):
+        g = B.setup((5, 5), (3,), [pose(100, 5, deg(i))])
+        y = B.voxel(g, _vol2d([(2, 2)]), kind).ravel()
+        assert approx(w0, y[1]), (i, y)
+        assert approx(w1, y[0], 0.01) and approx(w1, y[2], 0.01), (i, y, w1)
+
+
+# :297-944 / :1481-2128, one-pixel detector at the given pose: for each voxel set (elsa's
+# x(i, j) coordinates) set to 1, the detector value equals the sum of the projected weights of
+# its voxels; the sets on the ray are elsa's "5 * weight of distance 0" cases.
+SINGLE_PIXEL_SETS = {
+    0: [[(2, s) for s in range(5)], [(3, s) for s in range(5)], [(1, s) for s in range(5)]],
+    45: [[(s, s) for s in range(5)], [(s, s + 1) for s in range(4)], [(s + 1, s) for s in range(4)],
+         [(s, s + 2) for s in range(3)], [(s + 2, s) for s in range(3)]],
+    90: [[(s, 2) for s in range(5)], [(s, 3) for s in range(5)], [(s, 1) for s in range(5)]],
+    135: [[(s, 4 - s) for s in range(5)], [(s, 5 - s) for s in range(1, 5)], [(s, 3 - s) for s in range(4)],
+          [(s, 6 - s) for s in range(2, 5)], [(s, 2 - s) for s in range(3)]],
+    180: [[(2, s) for s in range(5)], [(3, s) for s in range(5)], [(1, s) for s in range(5)]],
+    225: [[(s, s) for s in range(5)]],
+    270: [[(s, 2) for s in range(5)]],
+    315: [[(s, 4 - s) for s in range(5)]],
+}
+
+
+def _single_pixel(B, kind):
+    f = _basis(kind, B.dtype)
+    for a, sets in SINGLE_PIXEL_SETS.items():
+        g = B.setup((5, 5), (1,), [pose(100, 5, deg(a))])
+        for k, vs in enumerate(sets):
+            y = float(B.voxel(g, _vol2d(vs), kind).ravel()[0])
+            if k == 0:  # the voxels on the ray
+                assert approx(y, 5 * f(0.0)), (a, y)
+            exp = sum(_projected_weight(kind, a, v, dtype=B.dtype) for v in vs)
+            assert approx(y, exp), (a, k, y, exp)
+
+
+# :945-1278 / :2129-2462, back projection of a one-pixel detector holding 1: the voxels on the
+# ray get basis(0), the listed voxels beside it their projected weight. Every voxel of the
+# volume is checked against its projected weight as well (elsa lists 13-17 of them per pose).
+BACKWARD_SETS = {
+    0: [[(2, s) for s in range(5)], [(3, s) for s in range(5)], [(1, s) for s in range(5)]],
+    180: [[(2, s) for s in range(5)], [(3, s) for s in range(5)], [(1, s) for s in range(5)]],
+    90: [[(s, 2) for s in range(5)], [(s, 3) for s in range(5)], [(s, 1) for s in range(5)]],
+    270: [[(s, 2) for s in range(5)], [(s, 3) for s in range(5)], [(s, 1) for s in range(5)]],
+    45: [[(s, s) for s in range(5)], [(s, s + 1) for s in range(4)], [(s + 1, s) for s in range(4)],
+         [(s, 2 - s) for s in range(3)], [(2 - s, s) for s in range(3)]],
+}
+
+
+def _backward(B, kind):
+    f = _basis(kind, B.dtype)
+    for a, sets in BACKWARD_SETS.items():
+        g = B.setup((5, 5), (1,), [pose(100, 5, deg(a))])
+        x = B.voxel(g, np.ones((1, 1)), kind, adjoint=True)
+        for i, j in sets[0]:
+            assert approx(x[j, i], f(0.0)), (a, i, j, x[j, i])
+        for vs in sets[1:]:
+            for i, j in vs:
+                assert approx(x[j, i], _projected_weight(kind, a, (i, j), dtype=B.dtype)), (a, i, j, x[j, i])
+        for j in range(5):
+            for i in range(5):
+                assert approx(x[j, i], _projected_weight(kind, a, (i, j), dtype=B.dtype)), (a, i, j, x[j, i])
+
+
+def voxel_blob_centre_voxel_spacing_2d(B):
+    _centre_voxel_spacing(B, "blob")
+
+
+def voxel_bspline_centre_voxel_spacing_2d(B):
+    _centre_voxel_spacing(B, "bspline")
+
+
+def voxel_blob_two_pixels_2d(B):
+    _two_pixels(B, "blob")
+
+
+def voxel_bspline_two_pixels_2d(B):
+    _two_pixels(B, "bspline")
+
+
+def voxel_blob_three_pixels_2d(B):
+    _three_pixels(B, "blob")
+
+
+def voxel_bspline_three_pixels_2d(B):
+    _three_pixels(B, "bspline")
+
+
+def voxel_blob_single_pixel_2d(B):
+    _single_pixel(B, "blob")
+
+
+def voxel_bspline_single_pixel_2d(B):
+    _single_pixel(B, "bspline")
+
+
+def voxel_blob_backward_2d(B):
+    _backward(B, "blob")
+
+
+def voxel_bspline_backward_2d(B):
+    _backward(B, "bspline")
+
+
+def voxel_bspline_centre_voxel_3d(B):
+    """test_VoxelProjectorCUDA.cpp:1280-1323: the centre voxel of 5^3 projects to bspline(0)
+    (the 3D table) on a one-pixel detector, Gamma = 0, 4, 8, ... radians (epsilon 0.005)."""
+    import oracle
+
+    w0 = float(oracle.bspline_lut(3, 2, 100, B.dtype)[0])
+    x = np.zeros((5, 5, 5)); x[2, 2, 2] = 1
+    for i in range(0, 360, 4):
+        g = B.setup((5, 5, 5), (1, 1), [pose(100, 5, float(i))])
+        y = B.voxel(g, x, "bspline")
+        assert approx(w0, y.ravel()[0], 0.005), (i, y)
+
+
+def voxel_bspline_45_degree_cpu(B):
+    """test_BSplineVoxelProjector.cpp:1860-1938: at 45 deg the voxels on the ray give
+    5 bspline(0); the four voxels just above or just below it give 4 bspline(sqrt(1/2)) to
+    epsilon 0.001 (their projected distance is sqrt(1/2) up to the fan's divergence)."""
+    f = _basis("bspline", B.dtype)
+    g = B.setup((5, 5), (1,), [pose(100, 5, deg(45))])
+    y = float(B.voxel(g, _vol2d([(s, s) for s in range(5)]), "bspline").ravel()[0])
+    assert approx(y, 5 * f(0.0)), y
+    for vs in ([(s, s + 1) for s in range(4)], [(s + 1, s) for s in range(4)]):
+        y = float(B.voxel(g, _vol2d(vs), "bspline").ravel()[0])
+        assert approx(y, 4 * f(math.sqrt(0.5)), 0.001), (vs, y)
+
+
+def voxel_off_centre_3d(B):
+    """A closed form for the 3D footprint, independent of the projection matrices: one voxel
+    off centre along v (and u) in a non-cubic volume, a distant source, a non-square detector,
+    poses about y. With the pinhole model of Geometry.cpp:53-109 written out (camera
+    coordinates x_c = R(c - origin) + s2c e_z, pixel coordinate sdd x_c / z_c + det / 2),
+    every pixel (i, j) inside the footprint must hold blob(|(p - 0.5 - (i, j))| / s) with
+    s = sdd / |x_c|, and every other pixel 0; the back projection of each such pixel alone
+    gives the voxel that weight."""
+    import oracle
+
+    vol, det, s2c, c2d = (7, 9, 5), (13, 21), 400.0, 40.0
+    sdd = s2c + c2d
+    lut = oracle.blob_lut(2.0, 10.83, 2, 100, np.float64)
+    f = lambda d: float(oracle.lut_eval(lut, 2.0, d))
+    vox = (4, 7, 1)  # elsa's x(4, 7, 1): 1.5 voxels right of the axis, 3 above it
+    x = np.zeros(vol[::-1]); x[vox[2], vox[1], vox[0]] = 1
+    for gamma in (0.0, 0.5, 2.0, 4.0):
+        cg, sg = math.cos(gamma), math.sin(gamma)
+        R = np.array([[cg, 0, sg], [0, 1, 0], [-sg, 0, cg]])
+        xc = R @ (np.array(vox) + 0.5 - 0.5 * np.array(vol)) + np.array([0, 0, s2c])
+        p = sdd * xc[:2] / xc[2] + 0.5 * np.array(det) - 0.5
+        s = sdd / np.linalg.norm(xc)
+        exp = np.zeros((det[1], det[0]))
+        for j in range(det[1]):
+            for i in range(det[0]):
+                d = math.hypot(p[0] - i, p[1] - j) / s
+                if abs(p[0] - i) <= 2.0 * s and abs(p[1] - j) <= 2.0 * s:
+                    exp[j, i] = f(d)
+        assert np.count_nonzero(exp) >= 9, (gamma, p, s)
+        g = B.setup(vol, det, [pose(s2c, c2d, gamma)])
+        y = B.voxel(g, x, "blob").reshape(det[1], det[0])
+        for j in range(det[1]):
+            for i in range(det[0]):
+                assert approx(y[j, i], exp[j, i], 1e-4), (gamma, i, j, y[j, i], exp[j, i])
+        for j, i in zip(*np.nonzero(exp)):
+            e = np.zeros((1, det[1], det[0])); e[0, j, i] = 1
+            b = B.voxel(g, e, "blob", adjoint=True)
+            assert approx(b[vox[2], vox[1], vox[0]], exp[j, i], 1e-4), (gamma, i, j)
+
+
+VOXEL_KATS = [voxel_centre_voxel_3d, voxel_centre_voxel_2d, voxel_backward_2d, voxel_blob_centre_voxel_spacing_2d,
+              voxel_bspline_centre_voxel_spacing_2d, voxel_blob_two_pixels_2d, voxel_bspline_two_pixels_2d,
+              voxel_blob_three_pixels_2d, voxel_bspline_three_pixels_2d, voxel_blob_single_pixel_2d,
+              voxel_bspline_single_pixel_2d, voxel_blob_backward_2d, voxel_bspline_backward_2d,
+              voxel_bspline_centre_voxel_3d, voxel_bspline_45_degree_cpu, voxel_off_centre_3d]
 
 
 # --------------------------------------------------------------------------------------

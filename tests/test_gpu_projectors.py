@@ -55,9 +55,10 @@ def test_siddon_kats_gpu(kat, dt):
 
 
 @pytest.mark.parametrize("kat", kats.VOXEL_KATS, ids=lambda k: k.__name__)
-@pytest.mark.parametrize("dt", [np.float32, np.float64], ids=["f32", "f64"])
-def test_voxel_kats_gpu(kat, dt):
-    kat(GpuBackend(dt))
+@pytest.mark.parametrize("dt,kernels", [(np.float32, "fast"), (np.float32, "simple"), (np.float64, "fast")],
+                         ids=["f32", "f32-simple", "f64"])
+def test_voxel_kats_gpu(kat, dt, kernels):
+    kat(GpuBackend(dt, kernels))
 
 
 @pytest.mark.parametrize("kat", kats.PHASE_CONTRAST_KATS, ids=lambda k: k.__name__)

@@ -42,6 +42,17 @@ def test_projected_bspline_lut_golden():
             assert kats.approx(got, gold["expected"][i], 1e-6), (dt, i, got)
 
 
+def test_projected_blob_weights_golden():
+    """elsa/projectors/tests/test_BlobVoxelProjector.cpp:2289-2336 ("Test weights"): the
+    closed-form projected blob (a = 2, alpha = 10.83, m = 2) at distance i / 25 (doctest
+    Approx), in both of the oracle's precisions."""
+    gold = json.load(open(os.path.join(GOLDEN, "blob_weights.json")))
+    assert len(gold["expected"]) == 51
+    for i, e in enumerate(gold["expected"]):
+        got = oracle.blob_projected(i / 25.0, gold["radius"], gold["alpha"], gold["order"])
+        assert kats.approx(got, e), (i, got, e)
+
+
 def test_bessel_golden():
     """elsa/core/tests/test_Bessel.cpp:29-89 (std::cyl_bessel_i tables, doctest Approx)."""
     gold = json.load(open(os.path.join(GOLDEN, "bessel.json")))

@@ -11,15 +11,21 @@ import pytest
 import oracle
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+# ELSA_SANITIZE=1 builds the checks with AddressSanitizer and UndefinedBehaviorSanitizer under
+# their own names (tests/test_sanitizers.py runs this file that way, with the runtimes preloaded)
+SAN = os.environ.get("ELSA_SANITIZE") == "1"
+SFX = "_san" if SAN else ""
+CXXFLAGS = ["g++", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math"] + (
+    ["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined", "-fno-sanitize-recover=all"]
+    if SAN else ["-O2"])
 SRC = os.path.join(ROOT, "tests", "cxx", "tile_logic_check.cpp")
-OUT = os.path.join(ROOT, "tests", "cxx", "_build", "libtile_logic.so")
+OUT = os.path.join(ROOT, "tests", "cxx", "_build", f"libtile_logic{SFX}.so")
 
 
 @pytest.fixture(scope="module")
 def lib():
     os.makedirs(os.path.dirname(OUT), exist_ok=True)
-    subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
-                    "-I", os.path.join(ROOT, "elsa_amd", "csrc"), SRC, "-o", OUT], check=True)
+    subprocess.run([*CXXFLAGS, "-I", os.path.join(ROOT, "elsa_amd", "csrc"), SRC, "-o", OUT], check=True)
     L = ctypes.CDLL(OUT)
     f32 = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
     i64 = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
@@ -215,14 +221,13 @@ def test_tiled_adjoint_logic_high_dynamic_range(lib):
 # Siddon: the traversal and the tiled back projection of ehp_siddon.hpp
 
 SID_SRC = os.path.join(ROOT, "tests", "cxx", "siddon_tile_check.cpp")
-SID_OUT = os.path.join(ROOT, "tests", "cxx", "_build", "libsiddon_tile.so")
+SID_OUT = os.path.join(ROOT, "tests", "cxx", "_build", f"libsiddon_tile{SFX}.so")
 
 
 @pytest.fixture(scope="module")
 def sid():
     os.makedirs(os.path.dirname(SID_OUT), exist_ok=True)
-    subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
-                    "-I", os.path.join(ROOT, "elsa_amd", "csrc"), SID_SRC, "-o", SID_OUT], check=True)
+    subprocess.run([*CXXFLAGS, "-I", os.path.join(ROOT, "elsa_amd", "csrc"), SID_SRC, "-o", SID_OUT], check=True)
     L = ctypes.CDLL(SID_OUT)
     f32 = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
     i64 = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
@@ -316,14 +321,13 @@ def test_siddon_tiled_adjoint_logic_matches_oracle(sid, case):
 # voxel projectors: the detector-tiled forward of ehp_voxel.hpp
 
 VOX_SRC = os.path.join(ROOT, "tests", "cxx", "voxel_tile_check.cpp")
-VOX_OUT = os.path.join(ROOT, "tests", "cxx", "_build", "libvoxel_tile.so")
+VOX_OUT = os.path.join(ROOT, "tests", "cxx", "_build", f"libvoxel_tile{SFX}.so")
 
 
 @pytest.fixture(scope="module")
 def vox():
     os.makedirs(os.path.dirname(VOX_OUT), exist_ok=True)
-    subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
-                    "-I", os.path.join(ROOT, "elsa_amd", "csrc"), VOX_SRC, "-o", VOX_OUT], check=True)
+    subprocess.run([*CXXFLAGS, "-I", os.path.join(ROOT, "elsa_amd", "csrc"), VOX_SRC, "-o", VOX_OUT], check=True)
     L = ctypes.CDLL(VOX_OUT)
     f32 = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
     i64 = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
