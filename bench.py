@@ -13,10 +13,12 @@ Prints ONE JSON line:
               launch per driving axis; each is given the share of the pass's time that its
               samples have
   cpu_baseline  the CPU restatement (oracle/, OpenMP over rays) on a bounded sample of the same
-              workload (a subset of the poses), on this host's cores
-  cg          20 CGLS iterations (elsa's CGLS.cpp semantics) from b = A x_true, run on the device
-              eagerly and with one iteration captured in a hipGraph: wall times and the error
-              reached
+              workload (a subset of the poses spread over the orbit), on this host's cores: median
+              of 5 runs with every thread and with one thread
+  cg          CGLS (elsa's CGLS.cpp semantics) from b = A x_true on the device, eagerly and with
+              one iteration captured in a hipGraph: wall time to reach a relative error of 1e-2
+              against x* = the FP64 CGLS iterate after 100 iterations on the same b, and the
+              residual and errors after 20 iterations
 
 --projector siddon | blob | bspline times those projectors on the same workload instead
 (voxel crossings or footprint taps per second), without the CG and CPU parts.
@@ -73,8 +75,9 @@ def parse():
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--workload", default="cone3d_256", choices=sorted(WORKLOADS))
-    p.add_argument("--cpu-poses", type=int, default=240,
-                   help="poses of the bounded CPU sample (evenly spread over the trajectory)")
+    p.add_argument("--cpu-poses", type=int, nargs=2, default=[40, 2], metavar=("ALL", "ONE"),
+                   help="poses of the bounded CPU samples (evenly spread over the trajectory) timed with every "
+                        "host thread and with one thread")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--kernels", default="fast", choices=["fast", "simple"],
                    help="Joseph kernels: plane-major forward + LDS-tiled back projection (default) or one "
@@ -87,7 +90,11 @@ def parse():
     p.add_argument("--projector", default="joseph", choices=["joseph", "siddon", "blob", "bspline"],
                    help="projector to time (default: Joseph, the headline metric)")
     p.add_argument("--cg-iters", type=int, default=20,
-                   help="CGLS iterations of the reconstruction part (0 skips it)")
+                   help="CGLS iterations of the fixed-count reconstruction part (0 skips the CG part)")
+    p.add_argument("--cg-target", type=float, default=1e-2,
+                   help="relative error to the FP64 reference iterate that the timed CGLS must reach")
+    p.add_argument("--cg-ref-iters", type=int, default=100,
+                   help="FP64 CGLS iterations of the reference solution x*")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="only run a plain device copy of MIB MiB (3 launches), the reference kernel for "
                         "scaling FETCH_SIZE / WRITE_SIZE counters, and exit")
@@ -115,45 +122,101 @@ def copy_calibration(mib):
                       "copy_GBps": round(2 * n * 4 / t / 1e9, 1)}), flush=True)
 
 
-def cpu_baseline(wl, x_np, n_poses):
-    """Times the oracle (FP32, elsa's own position rule) on n_poses of the trajectory."""
-    import oracle
+def _cpu_run(wl, poses, threads, reps):
+    """One oracle timing (oracle/cpu_time.py) in a child process: OMP_NUM_THREADS and
+    OMP_PROC_BIND=close take effect there, not in this process whose OpenMP runtime is loaded."""
+    import subprocess
 
-    g = oracle.circle_geometry(wl["dim"], wl["vol"], wl["views"], 360, wl["s2c"], wl["c2d"], det=wl["det"])
-    idx = np.linspace(0, wl["views"] - 1, n_poses).round().astype(int)
-    sub = oracle.Geom(g.dim, g.vol, g.det, g.P[idx], g.Pinv[idx], g.C[idx], g.ext[idx], g.sdd[idx])
-    S = oracle.joseph_sample_count(sub)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    cmd = [sys.executable, "-m", "oracle.cpu_time", "--dim", str(wl["dim"]), "--vol", *map(str, wl["vol"]),
+           "--views", str(wl["views"]), "--det", *map(str, wl["det"]), "--s2c", str(wl["s2c"]),
+           "--c2d", str(wl["c2d"]), "--poses", str(poses), "--threads", str(threads), "--reps", str(reps)]
+    out = subprocess.run(cmd, cwd=ROOT, env=env, capture_output=True, text=True, check=True).stdout
+    return json.loads(out.strip().splitlines()[-1])
+
+
+def cpu_baseline(wl, poses_all, poses_one, reps=5):
+    """The oracle (FP32, elsa's own position rule, OpenMP over rays) on a subset of the poses:
+    median of `reps` runs after one warm-up, once with every host thread this process may use
+    and once with one thread, threads pinned to cores."""
     threads = len(os.sched_getaffinity(0))
     threads = min(threads, int(os.environ.get("OMP_NUM_THREADS", threads)))
-    t0 = time.perf_counter()
-    y = oracle.joseph(sub, x_np, nthreads=threads)
-    oracle.joseph(sub, y, adjoint=True, nthreads=threads)
-    dt = time.perf_counter() - t0
-    return {"value": round(2 * S / dt / 1e9, 5), "unit": "Gsamples/s", "cores": threads, "kind": "port",
-            "sample": f"{n_poses} of {wl['views']} poses of the same workload (Joseph fwd+adj, {2 * S} samples, "
-                      f"{dt:.1f} s), oracle/elsa_oracle.cpp with OpenMP over rays"}
+    allt = _cpu_run(wl, poses_all, threads, reps)
+    one = _cpu_run(wl, poses_one, 1, reps)
+    return {"value": allt["gsamples_per_s"], "unit": "Gsamples/s", "cores": threads, "kind": "port",
+            "sample": f"{poses_all} of {wl['views']} poses of the same workload spread over the orbit (Joseph fwd+adj, "
+                      f"{allt['samples']} samples per run), median of {reps} runs ({allt['median_s']} s) after a "
+                      f"warm-up; oracle/elsa_oracle.cpp, OpenMP over rays, OMP_PROC_BIND=close",
+            "cpu_model": allt["cpu_model"], "runs_s": allt["times_s"],
+            "one_thread": {"value": one["gsamples_per_s"], "unit": "Gsamples/s", "cores": 1,
+                           "sample": f"{poses_one} of {wl['views']} poses ({one['samples']} samples per run), "
+                                     f"median of {reps} runs ({one['median_s']} s)", "runs_s": one["times_s"]}}
 
 
-def cg_run(E, op, x, iters):
+def cg_run(E, op, x, iters, vd, dd, target, ref_iters):
     """CGLS (elsa's CGLS.cpp semantics, tol 0) from b = A x_true, whole iterations on the
-    device: wall time of solve() (workspace, graph capture and all iterations) eager and with
-    one iteration captured in a hipGraph, and the error reached."""
+    device.
+
+    Wall time to a fixed relative error: x* is the FP64 CGLS iterate after ref_iters
+    iterations on the same b (FP64 operator and reductions; an iteration-count proxy for the
+    FP64 solution of the normal equations). The FP32 iteration is deterministic (fixed-point
+    back projection), so solve(k) from scratch gives its k-th iterate: the smallest k with
+    ||x_k - x*|| / ||x*|| <= target is found by bisection, untimed, then solve(k) is timed eagerly
+    and with one iteration captured in a hipGraph. Also the error and residual after `iters`
+    iterations."""
     import torch
 
     b = op.apply(x)
     res = {"iterations": iters}
+    op64 = E.JosephsMethodCUDA(vd, dd, dtype=np.float64)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    xs = E.CGLS(op64, b.double(), tol=0.0, graph=True).solve(ref_iters)
+    torch.cuda.synchronize()
+    t_ref = time.perf_counter() - t0
+    nrm = float(torch.linalg.norm(xs))
+    del op64
+
+    def err(k):
+        xr = E.CGLS(op, b, tol=0.0, graph=True).solve(k)
+        return float(torch.linalg.norm(xr.double() - xs)) / nrm
+
+    errs = {}
+    lo, hi = 0, ref_iters
+    errs[hi] = err(hi)
+    if errs[hi] <= target:
+        while hi - lo > 1:
+            mid = (lo + hi) // 2
+            errs[mid] = err(mid)
+            if errs[mid] <= target:
+                hi = mid
+            else:
+                lo = mid
+        k = hi
+    else:
+        k = None
+    res.update({"target_rel_error": target,
+                "reference": f"FP64 CGLS iterate after {ref_iters} iterations on the same b (FP64 operator), "
+                             f"{t_ref:.2f} s",
+                "iterations_to_target": k, "rel_error_at_target": errs.get(k) if k else None,
+                "rel_error_trace": {str(kk): round(v, 6) for kk, v in sorted(errs.items())}})
     for graph in (False, True):
-        solver = E.CGLS(op, b, tol=0.0, graph=graph)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        xr = solver.solve(iters)
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        key = "graph" if graph else "eager"
-        res[f"wall_s_{key}"] = round(dt, 4)
-        res[f"ms_per_iter_{key}"] = round(dt / iters * 1e3, 3)
-    res["rel_residual"] = float(solver.residuals[solver.iterations_done] / solver.residuals[0])
-    res["rel_error_vs_phantom"] = float(torch.linalg.norm(xr - x) / torch.linalg.norm(x))
+        for n, key in ((k, "to_target"), (iters, "fixed")):
+            if n is None:
+                continue
+            solver = E.CGLS(op, b, tol=0.0, graph=graph)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            xr = solver.solve(n)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            mode = "graph" if graph else "eager"
+            res[f"wall_s_{key}_{mode}"] = round(dt, 4)
+            res[f"ms_per_iter_{key}_{mode}"] = round(dt / n * 1e3, 3)
+            if key == "fixed":
+                res["rel_residual_fixed"] = float(solver.residuals[solver.iterations_done] / solver.residuals[0])
+                res["rel_error_vs_phantom_fixed"] = float(torch.linalg.norm(xr - x) / torch.linalg.norm(x))
+                res["rel_error_vs_reference_fixed"] = float(torch.linalg.norm(xr.double() - xs)) / nrm
     return res
 
 
@@ -616,9 +679,9 @@ def main():
                                   "back_projection": len(adj_launches) if len(adj_launches) > 1 else "pass"}},
     }
     if a.cg_iters > 0 and a.projector == "joseph":
-        out["cg"] = cg_run(E, op, x, a.cg_iters)
+        out["cg"] = cg_run(E, op, x, a.cg_iters, vd, dd, a.cg_target, a.cg_ref_iters)
     if not a.no_cpu and a.projector == "joseph":
-        out["cpu_baseline"] = cpu_baseline(wl, x_np, a.cpu_poses)
+        out["cpu_baseline"] = cpu_baseline(wl, *a.cpu_poses)
     print(json.dumps(out), flush=True)
 
 

@@ -336,8 +336,9 @@ namespace
         return EHP_OK;
     }
 
-    // FP32 Joseph: per-ray table, driving axes per pose, plane-major views for the forward
-    // and, when the fixed-point bound exists, the LDS-tiled back projection.
+    // FP32 and bfloat16 Joseph: per-ray table, driving axes per pose, plane-major views for the
+    // forward and, when the fixed-point bound exists, the LDS-tiled back projection. FP64 Joseph:
+    // the same without the views.
     int setup_fast(ehp_op* op)
     {
         const ehp::Dims& dm = op->dims;
@@ -348,7 +349,7 @@ namespace
             return fail(EHP_ERR_ALLOC, "hipMalloc of the %zu-byte ray table failed: %s", tbytes, hipGetErrorString(e));
         e = hipMalloc((void**) &op->d_pose_axes, n * sizeof(int));
         if (e == hipSuccess)
-            e = hipMalloc((void**) &op->d_maxbits, ehp::kAbsmaxBlocks * sizeof(unsigned int));
+            e = hipMalloc((void**) &op->d_maxbits, ehp::kAbsmaxBlocks * sizeof(unsigned long long));
         if (e == hipSuccess)
             e = hipMalloc((void**) &op->d_fixexp, 2 * sizeof(int));
         if (e != hipSuccess)
@@ -365,7 +366,8 @@ namespace
         for (int v : h)
             op->axes_present |= v;
 
-        int rc = setup_views(op, h);
+        // FP64 operators keep the one-thread-per-ray forward (no plane-major views)
+        int rc = op->dtype == EHP_F64 ? EHP_OK : setup_views(op, h);
         if (rc != EHP_OK)
             return rc;
         const bool b16 = op->dtype == EHP_BF16;
@@ -519,7 +521,7 @@ int ehp_joseph_create(const ehp_geometry* g, int dtype, int flags, ehp_op** out)
     if (rc != EHP_OK)
         return rc;
     rc = count_samples(*out);
-    if (rc == EHP_OK && (dtype == EHP_F32 || dtype == EHP_BF16) && !(flags & EHP_SIMPLE_KERNELS))
+    if (rc == EHP_OK && !(flags & EHP_SIMPLE_KERNELS))
         rc = setup_fast(*out);
     if (rc == EHP_OK && dtype == EHP_BF16 && !((*out)->planes && (*out)->tiled))
         rc = fail(EHP_ERR_UNSUPPORTED, "this geometry has no plane-major / tiled Joseph kernels, which bfloat16 needs");
@@ -647,6 +649,8 @@ static bool scatters_into_output(const ehp_op* op, bool adjoint)
         return !adjoint && !(op->dtype == EHP_F32 && op->tiled);
     if (!adjoint)
         return false;
+    if (op->kind == KIND_JOSEPH && op->tiled)
+        return false; // FP32 and FP64: LDS tiles, each voxel written by one workgroup per pass
     return !(op->dtype == EHP_F32 && op->tiled);
 }
 
@@ -779,7 +783,11 @@ static hipError_t launch(ehp_op* op, const void* in, void* out, hipStream_t s, b
     } else {
         const double* x = (const double*) in;
         double* y = (double*) out;
-        if (op->kind == KIND_JOSEPH)
+        if (op->kind == KIND_JOSEPH && adjoint && op->tiled)
+            e = ehp::joseph_adjoint_tiled<double>(op->d_geo, op->d_table, dm, op->d_pose_axes, op->axes_present,
+                                                  op->tile, op->batch, op->slot_bound, op->d_maxbits, op->d_fixexp, x,
+                                                  y, s);
+        else if (op->kind == KIND_JOSEPH)
             e = adjoint ? ehp::joseph_adjoint<double>(op->d_geo, dm, x, y, s) : ehp::joseph_forward<double>(op->d_geo, dm, x, y, s);
         else if (op->kind == KIND_SIDDON)
             e = adjoint ? ehp::siddon_adjoint<double>(op->d_geo, dm, x, y, s) : ehp::siddon_forward<double>(op->d_geo, dm, x, y, s);

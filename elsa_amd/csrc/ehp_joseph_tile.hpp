@@ -310,12 +310,14 @@ namespace ehp
     // Back projection of the sample at fs = s (as a float) of ray r lying on the tile plane at
     // byte offset poff: if its bilinear footprint touches the tile, hand ys * weight for each of
     // its four (2D: two) slots to add(offset, value), ys being y * len in the ray's own scale
-    // (ray_scale). The position
+    // (ray_scale); with W, hand add(offset, weight) the tap's weight itself (3D: the product of
+    // the two transverse weights in FP32, elsa's value, the driving axis' weight being 1), for a
+    // caller that scales it in FP64. The position
     // and weights are elsa's (doInterpolation, JosephsMethod.cpp:60-97; driving weight 1) with the
     // forward's position rule fma(s, step, pos0), so they equal the forward's bit for bit; a 3D
     // tap's weight is applied as (ys * w2) * w1. LOW: evaluate c as elsa writes it (tiles on the
     // low faces, tile_low_face); otherwise as fract(q - 0.5), the same value there.
-    template <int D, bool LOW, class Add>
+    template <int D, bool LOW, bool W, class Add>
     EHP_HD inline void tile_sample_fx(const RayRec& r, float fs, uint32_t poff, const TileOff& to, float ys, Add add)
     {
         const float q1 = __builtin_fmaf(fs, r.s1, r.p1);
@@ -328,8 +330,8 @@ namespace ehp
             const float c1 = LOW ? (q1 - (float) g1) - 0.5f : fract_pos(t1);
             const float w1 = 1.0f - c1;
             const uint32_t off = o1 + poff;
-            add(off, ys * w1);
-            add(off + 8u, ys * c1);
+            add(off, W ? w1 : ys * w1);
+            add(off + 8u, W ? c1 : ys * c1);
         } else {
             const float q2 = __builtin_fmaf(fs, r.s2, r.p2);
             const float t2 = q2 - 0.5f;
@@ -342,11 +344,18 @@ namespace ehp
             const float w1 = 1.0f - c1, w2 = 1.0f - c2;
             const uint32_t off = add3_u32(o1, o2, poff);
             const uint32_t off2 = off + (uint32_t) to.pb8;
-            const float yw = ys * w2, yc = ys * c2;
-            add(off, yw * w1);
-            add(off + 8u, yw * c1);
-            add(off2, yc * w1);
-            add(off2 + 8u, yc * c1);
+            if (W) {
+                add(off, w1 * w2);
+                add(off + 8u, c1 * w2);
+                add(off2, w1 * c2);
+                add(off2 + 8u, c1 * c2);
+            } else {
+                const float yw = ys * w2, yc = ys * c2;
+                add(off, yw * w1);
+                add(off + 8u, yw * c1);
+                add(off2, yc * w1);
+                add(off2 + 8u, yc * c1);
+            }
         }
     }
 
@@ -355,7 +364,7 @@ namespace ehp
     // [0, nsteps) goes to tile_sample_fx. FULL: the caller knows that all la samples lie in
     // [0, nsteps) (no test per plane). Every ray visits the planes in the same order, so the
     // lanes of a wavefront address one LDS plane at each step.
-    template <int D, bool LOW, bool FULL, class Add>
+    template <int D, bool LOW, bool FULL, bool W, class Add>
     EHP_HD inline void tile_ray_fx(const RayRec& r, int s0, int sg, int la, const TileOff& to, float ys, Add add)
     {
         float fs = (float) s0;
@@ -364,14 +373,15 @@ namespace ehp
         uint32_t poff = to.base;
         for (int pa = 0; pa < la; ++pa, fs += fsg, s += sg, poff += to.plane)
             if (FULL || (uint32_t) s < (uint32_t) r.nsteps)
-                tile_sample_fx<D, LOW>(r, fs, poff, to, ys, add);
+                tile_sample_fx<D, LOW, W>(r, fs, poff, to, ys, add);
     }
 
     // Back projection of one candidate ray into a tile (planes [A0, A1) along A): rays of
     // another driving axis, without samples or with y = 0 add nothing. ys: y * len in the ray's
-    // scale (ray_scale). Takes the test-free plane walk when every active lane of the wavefront
-    // has samples on all the tile's planes.
-    template <int D, int A, bool LOW, class Add>
+    // scale (ray_scale); with W any value that is 0 exactly when y * len is, add() then
+    // receiving the weights (tile_sample_fx). Takes the test-free plane walk when every active
+    // lane of the wavefront has samples on all the tile's planes.
+    template <int D, int A, bool LOW, bool W = false, class Add>
     EHP_HD inline void tile_candidate_fx(const RayRec& r, const TileBox& t, const TileOff& to, float ys, Add add)
     {
         if ((r.meta & 3) != A || r.nsteps == 0 || ys == 0.0f)
@@ -383,9 +393,9 @@ namespace ehp
         const int s1 = s0 + (la - 1) * sg;
         const int lo = s0 < s1 ? s0 : s1, hi = s0 < s1 ? s1 : s0;
         if (wave_all(lo >= 0 && hi < r.nsteps))
-            tile_ray_fx<D, LOW, true>(r, s0, sg, la, to, ys, add);
+            tile_ray_fx<D, LOW, true, W>(r, s0, sg, la, to, ys, add);
         else
-            tile_ray_fx<D, LOW, false>(r, s0, sg, la, to, ys, add);
+            tile_ray_fx<D, LOW, false, W>(r, s0, sg, la, to, ys, add);
     }
 
     // Linear volume index of LDS slot i if the tile owns that voxel, else -1.
@@ -531,9 +541,9 @@ namespace ehp
     // q = 2^(eg - 62), where 2^eg > max|y| * bound, so no sum reaches 2^62 in magnitude.
     // Returns eg; 0 when max|y| is 0 (nothing to add), kNonFinite when it is not finite.
 
-    EHP_HD inline int fixed_point_exponent(float maxabs, double bound)
+    EHP_HD inline int fixed_point_exponent(double maxabs, double bound)
     {
-        const double X = (double) maxabs * bound;
+        const double X = maxabs * bound;
         if (X == 0.0)
             return 0;
         if (!(X <= 1.0e300))
@@ -541,6 +551,11 @@ namespace ehp
         int e;
         frexp(X, &e); // X = f 2^e, f in [0.5, 1)
         return e < -900 ? -900 : e;
+    }
+
+    EHP_HD inline int fixed_point_exponent(float maxabs, double bound)
+    {
+        return fixed_point_exponent((double) maxabs, bound);
     }
 
     // Per-ray scale of the contributions: with |yl| < 2^er (er raised to at least eg - 32),

@@ -70,13 +70,14 @@ namespace ehp
     // Largest |y| in two stages without atomics: each block writes the maximum of its share
     // as the bit pattern of |y| (for non-negative floats, and NaN above infinity, the integer
     // order of the bits is the order of the values), one block reduces those.
-    __device__ inline unsigned int block_max_bits(unsigned int m)
+    template <class U>
+    __device__ inline U block_max_bits(U m)
     {
         for (int off = 32; off > 0; off >>= 1) {
-            const unsigned int o = __shfl_down(m, off, 64);
+            const U o = __shfl_down(m, off, 64);
             m = o > m ? o : m;
         }
-        __shared__ unsigned int red[4];
+        __shared__ U red[4];
         if ((threadIdx.x & 63) == 0)
             red[threadIdx.x >> 6] = m;
         __syncthreads();
@@ -113,6 +114,35 @@ namespace ehp
             fixexp[0] = fixed_point_exponent(__uint_as_float(m), bound);
     }
 
+    // The same in FP64: the bit patterns of non-negative doubles order as the values do.
+    __global__ void __launch_bounds__(256)
+        absmax64_kernel(const double* __restrict__ x, int64_t n, unsigned long long* __restrict__ partial)
+    {
+        unsigned long long m = 0ull;
+        const int64_t stride = (int64_t) gridDim.x * blockDim.x;
+        for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+            const unsigned long long u = (unsigned long long) __double_as_longlong(fabs(x[i]));
+            m = u > m ? u : m;
+        }
+        m = block_max_bits(m);
+        if (threadIdx.x == 0)
+            partial[blockIdx.x] = m;
+    }
+
+    __global__ void __launch_bounds__(256)
+        scale64_kernel(const unsigned long long* __restrict__ partial, int nblocks, double bound,
+                       int* __restrict__ fixexp)
+    {
+        unsigned long long m = 0ull;
+        for (int i = threadIdx.x; i < nblocks; i += blockDim.x)
+            m = partial[i] > m ? partial[i] : m;
+        m = block_max_bits(m);
+        if (threadIdx.x == 0)
+            fixexp[0] = fixed_point_exponent(__longlong_as_double((long long) m), bound);
+    }
+
+    // d_maxbits holds kAbsmaxBlocks 64-bit partial maxima (FP32 and bfloat16 use its first half
+    // as 32-bit ones)
     template <class T>
     hipError_t fixed_point_exponent_launch(const T* y, int64_t n, double slot_bound, unsigned int* d_maxbits,
                                            int* d_fixexp, hipStream_t s)
@@ -120,8 +150,14 @@ namespace ehp
         // 16 elements per thread at least
         int64_t blocks = (n + 256 * 16 - 1) / (256 * 16);
         blocks = blocks > kAbsmaxBlocks ? kAbsmaxBlocks : (blocks < 1 ? 1 : blocks);
-        hipLaunchKernelGGL((absmax_kernel<T>), dim3((unsigned) blocks), dim3(256), 0, s, y, n, d_maxbits);
-        hipLaunchKernelGGL(scale_kernel, dim3(1), dim3(256), 0, s, d_maxbits, (int) blocks, slot_bound, d_fixexp);
+        if constexpr (sizeof(T) == 8) {
+            unsigned long long* part = reinterpret_cast<unsigned long long*>(d_maxbits);
+            hipLaunchKernelGGL(absmax64_kernel, dim3((unsigned) blocks), dim3(256), 0, s, y, n, part);
+            hipLaunchKernelGGL(scale64_kernel, dim3(1), dim3(256), 0, s, part, (int) blocks, slot_bound, d_fixexp);
+        } else {
+            hipLaunchKernelGGL((absmax_kernel<T>), dim3((unsigned) blocks), dim3(256), 0, s, y, n, d_maxbits);
+            hipLaunchKernelGGL(scale_kernel, dim3(1), dim3(256), 0, s, d_maxbits, (int) blocks, slot_bound, d_fixexp);
+        }
         return hipGetLastError();
     }
 
@@ -129,6 +165,8 @@ namespace ehp
                                                            hipStream_t);
     template hipError_t fixed_point_exponent_launch<bf16>(const bf16*, int64_t, double, unsigned int*, int*,
                                                           hipStream_t);
+    template hipError_t fixed_point_exponent_launch<double>(const double*, int64_t, double, unsigned int*, int*,
+                                                            hipStream_t);
 
     // ---------------------------------------------------------------------------------
     // the tiled back projection
@@ -145,10 +183,11 @@ namespace ehp
                                                    const int total, const TileBox& t, const TileOff& to,
                                                    const int eg)
     {
+        using YT = typename Acc<ST>::type; // FP64 for FP64 sinograms, else FP32
         const int tid = threadIdx.x;
         int j = 0, pj = 0, pn = s_pref[1];
         CandRect cr = s_rect[0];
-        auto fetch = [&](int k, RayRec& r, float& yv) {
+        auto fetch = [&](int k, RayRec& r, YT& yv) {
             while (k >= pn) {
                 ++j;
                 pj = pn;
@@ -164,18 +203,30 @@ namespace ehp
             yv = up(sino[ray]);
             r = table[ray];
         };
-        auto process = [&](const RayRec& r, float yv) {
-            float ys;
-            int sh;
-            ray_scale(yv * r.len, eg, ys, sh);
-            // off: the slot's LDS byte address (the tile's base included)
-            tile_candidate_fx<D, A, LOW>(r, t, to, ys, [&](uint32_t off, float val) {
-                __hip_atomic_fetch_add((lds_u64*) (uintptr_t) off, fixed_contrib_rn(val, sh), __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-            });
+        auto process = [&](const RayRec& r, YT yv) {
+            if constexpr (sizeof(YT) == 8) {
+                // FP64: each contribution is elsa's (y * len) * weight in FP64 (the weight in FP32,
+                // as the one-thread-per-ray FP64 kernel forms it), rounded to the nearest multiple of
+                // q; yq is y * len in units of q (a power-of-two scaling, exact)
+                const double yq = ldexp(yv * (double) r.len, 62 - eg);
+                tile_candidate_fx<D, A, LOW, true>(r, t, to, yq != 0.0 ? 1.0f : 0.0f, [&](uint32_t off, float w) {
+                    __hip_atomic_fetch_add((lds_u64*) (uintptr_t) off,
+                                           (unsigned long long) (long long) __builtin_rint(yq * (double) w),
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                });
+            } else {
+                float ys;
+                int sh;
+                ray_scale(yv * r.len, eg, ys, sh);
+                // off: the slot's LDS byte address (the tile's base included)
+                tile_candidate_fx<D, A, LOW>(r, t, to, ys, [&](uint32_t off, float val) {
+                    __hip_atomic_fetch_add((lds_u64*) (uintptr_t) off, fixed_contrib_rn(val, sh), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                });
+            }
         };
         RayRec ra{}, rb{};
-        float ya = 0.0f, yb = 0.0f;
+        YT ya = 0, yb = 0;
         if (tid < total)
             fetch(tid, ra, ya);
         for (int k = tid; k < total; k += 2 * kThreads) {
@@ -197,7 +248,7 @@ namespace ehp
                                                     const int* __restrict__ pose_axes, const TileCfg& tc,
                                                     const int ntA, const int ntB, const int tile, const int batch,
                                                     const int* __restrict__ fixexp, const ST* __restrict__ sino,
-                                                    float* __restrict__ vol)
+                                                    typename Acc<ST>::type* __restrict__ vol)
     {
         extern __shared__ long long lds[];
 
@@ -250,7 +301,7 @@ namespace ehp
         for (int i = tid; i < t.nslots; i += kThreads) {
             const int64_t lin = tile_slot_voxel<D, A>(t, dims, i);
             if (lin >= 0)
-                vol[lin] += (float) ((double) iacc[i] * q);
+                vol[lin] += (typename Acc<ST>::type) ((double) iacc[i] * q);
         }
     }
 
@@ -260,7 +311,7 @@ namespace ehp
                                 const int* __restrict__ pose_axes, const TileCfg tc, const int ntA, const int ntB,
                                 const int ntC, const int tiles_per_xcd, const int batch,
                                 const int* __restrict__ fixexp, const ST* __restrict__ sino,
-                                float* __restrict__ vol)
+                                typename Acc<ST>::type* __restrict__ vol)
     {
         // consecutive workgroups land on different XCDs: give each XCD a contiguous block of tiles
         const int tile = (int) (blockIdx.x % 8u) * tiles_per_xcd + (int) (blockIdx.x / 8u);
@@ -356,12 +407,13 @@ namespace ehp
     template <class ST>
     hipError_t joseph_adjoint_tiled(const PoseGeom* geo, const RayRec* table, const Dims& dims, const int* pose_axes,
                                     int axes_present, const TileCfg& tc, int batch, double slot_bound,
-                                    unsigned int* d_maxbits, int* d_fixexp, const ST* sino, float* vol,
-                                    hipStream_t s)
+                                    unsigned int* d_maxbits, int* d_fixexp, const ST* sino,
+                                    typename Acc<ST>::type* vol, hipStream_t s)
     {
+        using VT = typename Acc<ST>::type;
         if (batch < 1 || batch > kMaxPoseBatch)
             return hipErrorInvalidValue;
-        hipError_t e = launch_fill<float>(vol, dims.nvox, 0.0f, s);
+        hipError_t e = launch_fill<VT>(vol, dims.nvox, VT(0), s);
         if (e == hipSuccess)
             e = fixed_point_exponent_launch(sino, dims.nrays, slot_bound, d_maxbits, d_fixexp, s);
         if (e != hipSuccess)
@@ -404,6 +456,9 @@ namespace ehp
     template hipError_t joseph_adjoint_tiled<bf16>(const PoseGeom*, const RayRec*, const Dims&, const int*, int,
                                                    const TileCfg&, int, double, unsigned int*, int*, const bf16*,
                                                    float*, hipStream_t);
+    template hipError_t joseph_adjoint_tiled<double>(const PoseGeom*, const RayRec*, const Dims&, const int*, int,
+                                                     const TileCfg&, int, double, unsigned int*, int*, const double*,
+                                                     double*, hipStream_t);
 
     // Segments of a batch's sinograms (absmax) and the tiles of every operator per driving
     // axis (one job per operator that has rays of that axis).

@@ -102,11 +102,13 @@ namespace ehp
     hipError_t joseph_pose_axes(const RayRec* table, const Dims& dims, int* pose_axes, hipStream_t s);
     // dynamic LDS of a tiled back projection with pose batches of `batch` (the largest by default)
     size_t joseph_tile_lds_bytes(const Dims& dims, const TileCfg& tc, int batch = kMaxPoseBatch);
+    // ST: the sinogram's storage type; the volume is FP64 for FP64 sinograms, else FP32.
+    // d_maxbits: kAbsmaxBlocks 64-bit words.
     template <class ST>
     hipError_t joseph_adjoint_tiled(const PoseGeom* geo, const RayRec* table, const Dims& dims, const int* pose_axes,
                                     int axes_present, const TileCfg& tc, int batch, double slot_bound,
-                                    unsigned int* d_maxbits, int* d_fixexp, const ST* sino, float* vol,
-                                    hipStream_t s);
+                                    unsigned int* d_maxbits, int* d_fixexp, const ST* sino,
+                                    typename Acc<ST>::type* vol, hipStream_t s);
     template <class T>
     hipError_t fixed_point_exponent_launch(const T* y, int64_t n, double slot_bound, unsigned int* d_maxbits,
                                            int* d_fixexp, hipStream_t s);

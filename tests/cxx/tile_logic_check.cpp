@@ -47,19 +47,22 @@ namespace
         double max_ratio = 0.0; // largest |slot sum| / 2^63 seen
     };
 
-    // one candidate ray of a tile, LOW chosen as the kernel chooses it per tile
-    template <int D, int A, class Add>
+    // one candidate ray of a tile, LOW chosen as the kernel chooses it per tile; W: add() gets
+    // the weights (the FP64 back projection)
+    template <int D, int A, bool W = false, class Add>
     void candidate(const RayRec& r, const TileBox& t, const TileOff& to, float ys, Add add)
     {
         if (tile_low_face<D>(t))
-            tile_candidate_fx<D, A, true>(r, t, to, ys, add);
+            tile_candidate_fx<D, A, true, W>(r, t, to, ys, add);
         else
-            tile_candidate_fx<D, A, false>(r, t, to, ys, add);
+            tile_candidate_fx<D, A, false, W>(r, t, to, ys, add);
     }
 
-    template <int D, int A>
+    // T = float: the FP32 kernel's contributions (int32 in each ray's scale, shifted into units
+    // of q); T = double: the FP64 kernel's ((y * len) * weight in FP64, rounded to a multiple of q)
+    template <int D, int A, class T>
     int run_pass(const Dims& dims, const std::vector<PoseGeom>& geo, const std::vector<RayRec>& table,
-                 const std::vector<int>& axes, TileCfg tc, int eg, const float* sino, float* vol, Stats& st)
+                 const std::vector<int>& axes, TileCfg tc, int eg, const T* sino, T* vol, Stats& st)
     {
         std::vector<int> owner(dims.nvox, 0);
         int ntA, ntB, ntC;
@@ -82,28 +85,46 @@ namespace
                     for (int v = v0; v <= v1; ++v)
                         for (int u = u0; u <= u1; ++u) {
                             const int64_t ray = (int64_t) p * dims.pix_per_pose + (int64_t) v * dims.det[0] + u;
-                            const float yv = sino[ray];
-                            if (yv == 0.0f)
+                            const T yv = sino[ray];
+                            if (yv == T(0))
                                 continue;
                             bool bad = false;
-                            float ys;
-                            int sh;
-                            ray_scale(yv * table[ray].len, eg, ys, sh);
-                            if (sh < 0 || sh > 32)
-                                bad = true;
-                            candidate<D, A>(table[ray], t, to, ys, [&](uint32_t off, float val) {
-                                const int64_t slot = off / 8;
-                                if (off % 8 != 0 || slot >= t.nslots) {
+                            if (sizeof(T) == 8) {
+                                const double yq = ldexp((double) yv * (double) table[ray].len, 62 - eg);
+                                candidate<D, A, true>(table[ray], t, to, 1.0f, [&](uint32_t off, float w) {
+                                    const int64_t slot = off / 8;
+                                    if (off % 8 != 0 || slot >= t.nslots || !(w >= 0.0f && w <= 1.0f)) {
+                                        bad = true;
+                                        return;
+                                    }
+                                    // the contribution must convert to an int64 below 2^62
+                                    const double c = rint(yq * (double) w);
+                                    if (!(fabs(c) < 4611686018427387904.0)) {
+                                        bad = true;
+                                        return;
+                                    }
+                                    iacc[slot] += (__int128) (long long) c;
+                                });
+                            } else {
+                                float ys;
+                                int sh;
+                                ray_scale((float) yv * table[ray].len, eg, ys, sh);
+                                if (sh < 0 || sh > 32)
                                     bad = true;
-                                    return;
-                                }
-                                // the contribution must round to an int32 of magnitude <= 2^30
-                                if (!(fabsf(val) < 1073741824.0f)) {
-                                    bad = true;
-                                    return;
-                                }
-                                iacc[slot] += (__int128) (long long) fixed_contrib_rn(val, sh);
-                            });
+                                candidate<D, A>(table[ray], t, to, ys, [&](uint32_t off, float val) {
+                                    const int64_t slot = off / 8;
+                                    if (off % 8 != 0 || slot >= t.nslots) {
+                                        bad = true;
+                                        return;
+                                    }
+                                    // the contribution must round to an int32 of magnitude <= 2^30
+                                    if (!(fabsf(val) < 1073741824.0f)) {
+                                        bad = true;
+                                        return;
+                                    }
+                                    iacc[slot] += (__int128) (long long) fixed_contrib_rn(val, sh);
+                                });
+                            }
                             if (bad) {
                                 std::fprintf(stderr, "LDS slot or contribution out of range (pass %d tile %d pose %d u %d v %d)\n", A,
                                              tile, p, u, v);
@@ -129,7 +150,7 @@ namespace
                     return 4;
                 }
                 owner[lin] += 1;
-                vol[lin] += (float) ((double) (int64_t) iacc[i] * ldexp(1.0, eg - 62));
+                vol[lin] += (T) ((double) (int64_t) iacc[i] * ldexp(1.0, eg - 62));
             }
         }
         for (int64_t i = 0; i < dims.nvox; ++i) {
@@ -141,9 +162,8 @@ namespace
         return 0;
     }
 
-    template <int D>
-    int run(const Dims& dims, const std::vector<PoseGeom>& geo, TileCfg tc, const float* sino, float* vol,
-            double* info)
+    template <int D, class T>
+    int run(const Dims& dims, const std::vector<PoseGeom>& geo, TileCfg tc, const T* sino, T* vol, double* info)
     {
         std::vector<RayRec> table(dims.nrays);
         std::vector<int> axes(dims.nposes, 0);
@@ -166,24 +186,24 @@ namespace
         info[0] = bound;
         if (!(bound > 0.0))
             return 10;
-        float m = 0.0f;
+        T m = 0;
         for (int64_t i = 0; i < dims.nrays; ++i)
-            m = fabsf(sino[i]) > m ? fabsf(sino[i]) : m;
+            m = fabs(sino[i]) > m ? (T) fabs(sino[i]) : m;
         const int eg = fixed_point_exponent(m, bound);
         info[1] = ldexp(1.0, eg - 62);
         for (int64_t i = 0; i < dims.nvox; ++i)
-            vol[i] = 0.0f;
+            vol[i] = 0;
         Stats st;
         for (int a = 0; a < D; ++a) {
             if (!(present & (1 << a)))
                 continue;
             int rc = 0;
             if (a == 0)
-                rc = run_pass<D, 0>(dims, geo, table, axes, tc, eg, sino, vol, st);
+                rc = run_pass<D, 0, T>(dims, geo, table, axes, tc, eg, sino, vol, st);
             else if (a == 1)
-                rc = run_pass<D, 1>(dims, geo, table, axes, tc, eg, sino, vol, st);
+                rc = run_pass<D, 1, T>(dims, geo, table, axes, tc, eg, sino, vol, st);
             else
-                rc = run_pass<D, (D == 3 ? 2 : 1)>(dims, geo, table, axes, tc, eg, sino, vol, st);
+                rc = run_pass<D, (D == 3 ? 2 : 1), T>(dims, geo, table, axes, tc, eg, sino, vol, st);
             if (rc)
                 return rc;
             info[3 + a] = 1;
@@ -193,26 +213,44 @@ namespace
     }
 } // namespace
 
+namespace
+{
+    template <class T>
+    int tile_adjoint_t(int dim, const int64_t* vol_dims, const int64_t* det_dims, int nposes, const float* P,
+                       const float* Pinv, const float* C, int LA, int TB, int TC, const T* sino, T* vol, double* info)
+    {
+        const Dims dims = make_dims(dim, vol_dims, det_dims, nposes);
+        const int nm = dim * (dim + 1);
+        std::vector<PoseGeom> geo(nposes);
+        for (int p = 0; p < nposes; ++p) {
+            PoseGeom g{};
+            for (int i = 0; i < nm; ++i) {
+                g.P[i] = P[p * nm + i];
+                g.Pinv[i] = Pinv[p * nm + i];
+            }
+            for (int i = 0; i < dim; ++i)
+                g.C[i] = C[p * dim + i];
+            geo[p] = g;
+        }
+        const TileCfg tc{LA, TB, TC};
+        return dim == 2 ? run<2, T>(dims, geo, tc, sino, vol, info) : run<3, T>(dims, geo, tc, sino, vol, info);
+    }
+} // namespace
+
 // info: [0] slot bound, [1] q, [2] largest |slot sum| / 2^63, [3..5] passes run
 extern "C" int tile_adjoint(int dim, const int64_t* vol_dims, const int64_t* det_dims, int nposes, const float* P,
                             const float* Pinv, const float* C, int LA, int TB, int TC, const float* sino, float* vol,
                             double* info)
 {
-    const Dims dims = make_dims(dim, vol_dims, det_dims, nposes);
-    const int nm = dim * (dim + 1);
-    std::vector<PoseGeom> geo(nposes);
-    for (int p = 0; p < nposes; ++p) {
-        PoseGeom g{};
-        for (int i = 0; i < nm; ++i) {
-            g.P[i] = P[p * nm + i];
-            g.Pinv[i] = Pinv[p * nm + i];
-        }
-        for (int i = 0; i < dim; ++i)
-            g.C[i] = C[p * dim + i];
-        geo[p] = g;
-    }
-    const TileCfg tc{LA, TB, TC};
-    return dim == 2 ? run<2>(dims, geo, tc, sino, vol, info) : run<3>(dims, geo, tc, sino, vol, info);
+    return tile_adjoint_t<float>(dim, vol_dims, det_dims, nposes, P, Pinv, C, LA, TB, TC, sino, vol, info);
+}
+
+// The same for the FP64 back projection (FP32 geometry and weights, FP64 data).
+extern "C" int tile_adjoint64(int dim, const int64_t* vol_dims, const int64_t* det_dims, int nposes, const float* P,
+                              const float* Pinv, const float* C, int LA, int TB, int TC, const double* sino,
+                              double* vol, double* info)
+{
+    return tile_adjoint_t<double>(dim, vol_dims, det_dims, nposes, P, Pinv, C, LA, TB, TC, sino, vol, info);
 }
 
 // Forward projection through zero-padded plane-major views: fan[a] is the fastest axis of the

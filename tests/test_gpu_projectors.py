@@ -279,6 +279,24 @@ def test_fp64_data_projectors(name, ops):
     assert rel_l2(gpu_b, oracle.siddon(g, y.astype(np.float32), adjoint=True, mode=oracle.MODE_DIRECT)) <= 5e-6
 
 
+@pytest.mark.parametrize("name", list(CASES))
+def test_joseph_fp64_tiled_back_projection(name, ops):
+    """JosephsMethodCUDA<double>'s back projection (LDS tiles, 64-bit fixed-point sums of the
+    FP64 contributions): within a few units of FP64 rounding of the oracle's sequential FP64 sums
+    of the same FP32-geometry terms, equal to the one-thread-per-ray atomic kernel to the same
+    bar, and bitwise reproducible run to run (the atomic kernel is not)."""
+    g = make_case(name)
+    y = phantom(g.sino_shape, 21, np.float64)
+    op = ops("JosephsMethodCUDA", name, dtype=np.float64)
+    tiled = op.applyAdjoint(y)
+    ref = oracle.joseph(g, y, adjoint=True, mode=oracle.MODE_DIRECT, acc64=True)
+    assert rel_l2(tiled, ref) <= 1e-13, rel_l2(tiled, ref)
+    simple = ops("JosephsMethodCUDA", name, dtype=np.float64, kernels="simple").applyAdjoint(y)
+    assert rel_l2(tiled, simple) <= 1e-13, rel_l2(tiled, simple)
+    for _ in range(2):
+        np.testing.assert_array_equal(op.applyAdjoint(y), tiled)
+
+
 def test_outputs_overwritten_not_accumulated(ops):
     """elsa's projectors overwrite the output (TraverseJosephsCUDA.cu:69, SiddonsMethodCUDA.cpp:75)."""
     import torch

@@ -29,7 +29,8 @@ def test_oracle_and_tile_logic_under_asan_ubsan():
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "san"], check=True)
     env = dict(os.environ)
     env.update({
-        "LD_PRELOAD": f"{asan}:{ubsan}",
+        # the sanitizer runtimes go first; whatever the environment already preloads stays
+        "LD_PRELOAD": ":".join(p for p in (asan, ubsan, os.environ.get("LD_PRELOAD", "")) if p),
         # Python's own allocations are not instrumented and live until exit
         "ASAN_OPTIONS": "detect_leaks=0:abort_on_error=1:halt_on_error=1",
         "UBSAN_OPTIONS": "print_stacktrace=1:halt_on_error=1",

@@ -32,6 +32,8 @@ def lib():
     f64 = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
     L.tile_adjoint.argtypes = [ctypes.c_int, i64, i64, ctypes.c_int, f32, f32, f32, ctypes.c_int, ctypes.c_int,
                                ctypes.c_int, f32, f32, f64]
+    L.tile_adjoint64.argtypes = [ctypes.c_int, i64, i64, ctypes.c_int, f32, f32, f32, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_int, f64, f64, f64]
     L.plane_forward.argtypes = [ctypes.c_int, i64, i64, ctypes.c_int, f32, f32, f32,
                                 np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS"), f32, f32, ctypes.c_int]
     L.plane_forward_mixed_waves.restype = ctypes.c_int64
@@ -75,6 +77,35 @@ def test_tiled_adjoint_logic_matches_oracle(lib, case):
     assert e_fix <= e_f32, (e_fix, e_f32)
     assert info[3:].sum() >= 1
     # slot sums stay well inside int64 (the bound is a worst case)
+    assert 0 < info[2] < 0.5, info
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c[0]}d-{'x'.join(map(str, c[1]))}-t{c[6][0]}")
+def test_tiled_adjoint_fp64_logic_matches_oracle(lib, case):
+    """The FP64 tiled back projection's logic: each contribution (y * len) * weight in FP64, the
+    weight elsa's FP32 product, rounded to a multiple of q = 2^(eg - 62) and summed in 64-bit
+    integers. Against the oracle's FP64 sums of the same FP32-geometry terms (acc64), the
+    difference is at most half a unit q per contribution plus one FP64 rounding of the voxel."""
+    dim, vol, views, det, s2c, c2d, tile = case
+    rng = np.random.default_rng(2)
+    angles = np.sort(rng.uniform(0, 360, views)).astype(np.float32)
+    g = oracle.circle_geometry(dim, vol, views, 360, s2c, c2d, det=det, angles_deg=angles)
+    y = rng.standard_normal(g.sino_shape)
+    out = np.zeros(g.vol_shape, np.float64)
+    info = np.zeros(6, np.float64)
+    rc = lib.tile_adjoint64(dim, np.array(list(vol) + [1] * (3 - dim), np.int64),
+                            np.array(list(det) + [1] * (2 - len(det)), np.int64), views,
+                            np.ascontiguousarray(g.P), np.ascontiguousarray(g.Pinv), np.ascontiguousarray(g.C),
+                            tile[0], tile[1], tile[2], np.ascontiguousarray(y), out, info)
+    assert rc == 0, rc
+    exact = oracle.joseph(g, y, adjoint=True, mode=oracle.MODE_DIRECT, acc64=True)
+    q = info[1]
+    # contributions per voxel and pass: at most the slot bound info[0] (poses times the rays
+    # whose footprint can cover one slot, times |len| >= 1); one FP64 rounding per pass
+    bound = dim * (0.5 * q * info[0] + 2.0 ** -52 * np.abs(exact))
+    assert np.all(np.abs(out - exact) <= bound), np.max(np.abs(out - exact) / bound)
+    err = np.linalg.norm(out - exact) / np.linalg.norm(exact)
+    assert err <= 1e-14, err
     assert 0 < info[2] < 0.5, info
 
 
