@@ -24,7 +24,7 @@ EXPORTS = (
     "ehp_batch_create", "ehp_batch_destroy", "ehp_batch_layout", "ehp_batch_forward", "ehp_batch_adjoint",
     "ehp_batch_cgls", "ehp_batch_apgd", "ehp_batch_power_iterations", "ehp_blob_derivative_lut",
     "ehp_bspline_derivative_lut", "ehp_pose_batch", "ehp_cgls_refined", "ehp_admml2", "ehp_batch_admml2",
-    "ehp_geometry_set_pixel_coords", "ehp_curved_detector_coords",
+    "ehp_geometry_set_pixel_coords", "ehp_curved_detector_coords", "ehp_adjoint_noalloc",
 )
 EHP_SOLVE_GRAPH = 1
 EHP_ADMM_FINITE_DIFFERENCES = 2
@@ -92,6 +92,7 @@ def load():
     L.ehp_destroy.restype = None
     L.ehp_forward.argtypes = [vp, vp, vp, vp]
     L.ehp_adjoint.argtypes = [vp, vp, vp, vp]
+    L.ehp_adjoint_noalloc.argtypes = [vp, vp, vp, vp]
     L.ehp_op_sizes.argtypes = [vp, i64p, i64p]
     L.ehp_sample_count.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
     L.ehp_joseph_ray_info.argtypes = [vp, vp, vp]

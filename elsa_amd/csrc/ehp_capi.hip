@@ -81,6 +81,11 @@ namespace
             delete op;
             return hip_fail(e, "hipGetDevice");
         }
+        e = ehp::order_init(op->order);
+        if (e != hipSuccess) {
+            delete op;
+            return hip_fail(e, "event of the operator");
+        }
         const size_t bytes = g->host.size() * sizeof(ehp::PoseGeom);
         e = hipMalloc((void**) &op->d_geo, bytes);
         if (e != hipSuccess) {
@@ -616,6 +621,103 @@ void ehp_destroy(ehp_op* op)
     if (cur >= 0 && cur != op->device)
         (void) hipSetDevice(cur);
     delete op;
+}
+
+// ---------------------------------------------------------------------------------------
+// back projection without allocation
+
+namespace
+{
+    // The scratch of ehp_adjoint_noalloc: the fixed-point scale of one back projection, that is
+    // the per-block maxima of |y| (kAbsmaxBlocks 64-bit words) and, kScaleOffset bytes in, the
+    // exponent. Every kernel that reads a word of it writes that word first in the same call
+    // (absmax before scale, scale before the tiles), so nothing needs clearing. One copy per
+    // GPU, held until the process ends; in host code only its device address, looked up per
+    // device, is used.
+    constexpr size_t kScaleOffset = ehp::kAbsmaxBlocks * sizeof(unsigned long long);
+    constexpr size_t kScratchBytes = kScaleOffset + 256;
+    static_assert(kScaleOffset % 256 == 0, "the exponent must be 256-byte aligned");
+    static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+    constexpr int kMaxDevices = 64;
+    std::mutex g_scratch_mu;
+    void* g_scratch_ptr[kMaxDevices] = {};
+
+    // Device address of this GPU's scratch array (the current device), looked up once per
+    // device.
+    int scratch_address(int device, unsigned char** out)
+    {
+        if (device < 0 || device >= kMaxDevices)
+            return fail(EHP_ERR_UNSUPPORTED, "device %d is beyond the %d devices of the no-alloc scratch", device,
+                        kMaxDevices);
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (!g_scratch_ptr[device]) {
+            void* p = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !p)
+                return hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol, "address of the no-alloc scratch");
+            g_scratch_ptr[device] = p;
+        }
+        *out = (unsigned char*) g_scratch_ptr[device];
+        return EHP_OK;
+    }
+} // namespace
+
+int ehp_adjoint_noalloc(ehp_op* op, const void* d_sino, void* d_vol, void* stream)
+{
+    if (!op)
+        return fail(EHP_ERR_ARGUMENT, "operator is NULL");
+    if (!d_sino || !d_vol)
+        return fail(EHP_ERR_ARGUMENT, "NULL data pointer");
+    if (d_sino == d_vol)
+        return fail(EHP_ERR_ARGUMENT, "input and output must not alias");
+    const bool joseph = op->kind == KIND_JOSEPH && (op->dtype == EHP_F32 || op->dtype == EHP_F64);
+    const bool siddon = op->kind == KIND_SIDDON && op->dtype == EHP_F32;
+    if (!op->tiled || !(joseph || siddon))
+        return fail(EHP_ERR_UNSUPPORTED, "ehp_adjoint_noalloc takes FP32/FP64 Joseph and FP32 Siddon operators "
+                                         "with a tiled back projection only");
+    // the scratch this call needs: the partial maxima of at most kAbsmaxBlocks blocks and the exponent
+    const size_t need = kScaleOffset + 2 * sizeof(int);
+    if (need > kScratchBytes)
+        return fail(EHP_ERR_ARGUMENT, "the back projection needs %zu bytes of scratch, the no-alloc maximum is %zu",
+                    need, kScratchBytes);
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess)
+        return hip_fail(e, "hipGetDevice");
+    if (cur != op->device)
+        return fail(EHP_ERR_ARGUMENT, "operator lives on device %d, current device is %d", op->device, cur);
+    unsigned char* sp = nullptr;
+    int rc = scratch_address(cur, &sp);
+    if (rc != EHP_OK)
+        return rc;
+    unsigned int* d_maxbits = (unsigned int*) sp;
+    int* d_fixexp = (int*) (sp + kScaleOffset);
+    hipStream_t s = (hipStream_t) stream;
+    std::lock_guard<std::mutex> lock(op->order.mu);
+    const bool capturing = ehp::stream_capturing(s);
+    e = ehp::order_begin(op->order, s, capturing);
+    if (e != hipSuccess)
+        return hip_fail(e, "ordering after the previous call on this operator");
+    const ehp::Dims& dm = op->dims;
+    if (siddon)
+        e = ehp::siddon_adjoint_tiled(op->d_geo, op->d_sidtable, dm, op->d_pose_axes, op->axes_present, op->tile,
+                                      op->batch, op->slot_bound, d_maxbits, d_fixexp, (const float*) d_sino,
+                                      (float*) d_vol, s);
+    else if (op->dtype == EHP_F32)
+        e = ehp::joseph_adjoint_tiled<float>(op->d_geo, op->d_table, dm, op->d_pose_axes, op->axes_present, op->tile,
+                                             op->batch, op->slot_bound, d_maxbits, d_fixexp, (const float*) d_sino,
+                                             (float*) d_vol, s);
+    else
+        e = ehp::joseph_adjoint_tiled<double>(op->d_geo, op->d_table, dm, op->d_pose_axes, op->axes_present,
+                                              op->tile, op->batch, op->slot_bound, d_maxbits, d_fixexp,
+                                              (const double*) d_sino, (double*) d_vol, s);
+    if (e != hipSuccess)
+        return hip_fail(e, "adjoint launch");
+    e = ehp::order_end(op->order, s, capturing);
+    if (e != hipSuccess)
+        return hip_fail(e, "recording the end of the call");
+    return EHP_OK;
 }
 
 int ehp_op_sizes(const ehp_op* op, int64_t* n_vol, int64_t* n_sino)

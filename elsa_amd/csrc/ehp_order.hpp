@@ -57,19 +57,29 @@ namespace ehp
         return hipStreamWaitEvent(s, g.last->ev, 0);
     }
 
+    // The guard's event, created up front (an operator does this at create time, so that its
+    // calls create nothing).
+    inline hipError_t order_init(OrderGuard& g)
+    {
+        if (g.own)
+            return hipSuccess;
+        auto ev = std::make_shared<SharedEvent>();
+        const hipError_t e = hipEventCreateWithFlags(&ev->ev, hipEventDisableTiming);
+        if (e != hipSuccess)
+            return e;
+        g.own = ev;
+        return hipSuccess;
+    }
+
     // After the launches (mutex held): record the guard's event on s.
     inline hipError_t order_end(OrderGuard& g, hipStream_t s, bool capturing)
     {
         if (capturing)
             return hipSuccess;
-        if (!g.own) {
-            auto ev = std::make_shared<SharedEvent>();
-            const hipError_t e = hipEventCreateWithFlags(&ev->ev, hipEventDisableTiming);
-            if (e != hipSuccess)
-                return e;
-            g.own = ev;
-        }
-        const hipError_t e = hipEventRecord(g.own->ev, s);
+        hipError_t e = order_init(g);
+        if (e != hipSuccess)
+            return e;
+        e = hipEventRecord(g.own->ev, s);
         if (e != hipSuccess)
             return e;
         g.last = g.own;

@@ -136,6 +136,18 @@ int ehp_forward(ehp_op* op, const void* d_vol, void* d_sino, void* stream);
 /* Back projection A^T y of d_sino into d_vol (elsa: LinearOperator::applyAdjoint). */
 int ehp_adjoint(ehp_op* op, const void* d_sino, void* d_vol, void* stream);
 
+/* ehp_adjoint for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). No counterpart in elsa. The same
+ * kernels and result, bit for bit, but the scratch of the back projection (the fixed-point
+ * scale: per-block maxima of |y| and the exponent, 16.6 KB) lives in one static device array
+ * of the library, 16.6 KB per GPU held until the process ends, instead of the operator's; the
+ * call allocates nothing. Takes FP32 and FP64 Joseph and FP32 Siddon operators with a tiled
+ * back projection (EHP_ERR_UNSUPPORTED otherwise: bfloat16, voxel, EHP_SIMPLE_KERNELS or
+ * curved-detector operators). Every call on a GPU shares that GPU's array: calls that overlap
+ * on one GPU (different operators on different streams, or threads) race and must be ordered
+ * by the caller; calls on one operator are ordered as for ehp_adjoint. */
+int ehp_adjoint_noalloc(ehp_op* op, const void* d_sino, void* d_vol, void* stream);
+
 /* Number of elements of the domain (volume) and range (sinogram). */
 int ehp_op_sizes(const ehp_op* op, int64_t* n_vol, int64_t* n_sino);
 
