@@ -7,11 +7,13 @@ volume/sinogram, inputs resident in HBM.
 
 Prints ONE JSON line:
   value       whole-job Joseph samples per second, (S_fwd + S_adj) * steps / wall time, Gsamples/s
-  roofline    dominant kernel (the longest launch): algorithmic bytes per launch (16 B per
-              sample = 4 FP32 taps, plus 4 B per ray) / its mean duration from HIP events on
-              the launch stream, against MI355X's 8 TB/s HBM. The back projection runs one
-              launch per driving axis; each is given the share of the pass's time that its
-              samples have
+  roofline    dominant kernel (the longest launch), bound by its vector ALU: VALU lane operations
+              per second (SQ_INSTS_VALU x 64 per launch, from the committed counters in
+              profiles/, over its mean duration from HIP events on the launch stream) against
+              the VALU's published 78.6 T lane operations/s (157.3 TFLOP/s FP32); beside it the
+              counter-measured HBM bytes against 8 TB/s and the algorithmic tap bytes (16 B per
+              3D sample, served from L2 / LDS). The back projection runs one launch per driving
+              axis; each is given the share of the pass's time that its samples have
   cpu_baseline  the CPU restatement (oracle/, OpenMP over rays) on a bounded sample of the same
               workload (a subset of the poses spread over the orbit), on this host's cores: median
               of 5 runs with every thread and with one thread
@@ -46,6 +48,8 @@ if ROOT not in sys.path:
 
 METRIC = "Joseph fwd+adj Gsamples/s & %HBM roofline; CG wall time to fixed rel. error"
 HBM_PEAK_GBS = 8000.0
+# MI355X FP32 vector peak 157.3 TFLOP/s = 78.6 T lane operations per second (FMA = 2 FLOP)
+VALU_PEAK_TOPS = 78.64
 WORKLOADS = {
     "cone3d_256": dict(dim=3, vol=(256, 256, 256), views=360, det=(384, 384), s2c=25600.0, c2d=1280.0,
                        text="3D cone-beam Joseph fwd+adj: 256^3 f32 Shepp-Logan, 360 poses over 360 deg, "
@@ -531,6 +535,41 @@ def autograd_main(a, E, wl):
           checks={"dot_product_rel": dot_rel, "fd_fp64_h": h, "fd_directions": 64, "fd_max_rel_error": worst})
 
 
+def roofline(a, wl, name, launch_bytes, t_dom):
+    """Roofline of the dominant kernel (the longest launch), from its live HIP-event time and the
+    per-launch hardware counters committed under profiles/ for this projector and workload.
+
+    Its bound is the vector ALU: it issues a VALU instruction nearly every cycle it can
+    (VALUBusy ~100 %) while HBM is far from busy, because the taps are served from L2 (forward)
+    and LDS (back projection). achieved = VALU lane operations per second, SQ_INSTS_VALU x 64
+    lanes per launch over the launch's mean time; peak = the VALU's published capability,
+    MI355X's 157.3 TFLOP/s FP32 counted as 78.6 T lane operations (an FMA is 2 FLOP). Beside it:
+    the counter-measured HBM bytes per launch against 8 TB/s, and the algorithmic tap bytes of
+    the launch (16 B per 3D sample), which exceed any HBM rate since they never reach HBM.
+    Without a counter file for the launch, the HBM form of the algorithmic bytes is reported."""
+    out = {"kernel": name, "ms_per_launch": round(t_dom * 1e3, 3), "algorithmic_bytes_per_launch": launch_bytes,
+           "algorithmic_GBps": round(launch_bytes / t_dom / 1e9, 1)}
+    cfile = os.path.join(ROOT, "profiles", f"{a.projector}_{a.workload}_counters.json")
+    k = None
+    if a.kernels == "fast" and os.path.exists(cfile):
+        k = json.load(open(cfile))["kernels"].get(name)
+    if k is None:
+        achieved = launch_bytes / t_dom / 1e9
+        out.update({"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "note": "algorithmic bytes (no counter file for this launch)"})
+        return out
+    achieved = k["valu_insts"] * 64 / t_dom / 1e12
+    hbm = k["hbm_bytes"] / t_dom / 1e9
+    out.update({"bound": "valu", "achieved": round(achieved, 2), "peak": VALU_PEAK_TOPS, "unit": "Tops/s",
+                "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": k["hbm_bytes"],
+                "valu_busy_pct": k["valu_busy_pct"],
+                "hbm": {"achieved": round(hbm, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(hbm / HBM_PEAK_GBS, 4), "l2_hit_rate": k["l2_hit_rate"]},
+                "counters": cfile[len(ROOT) + 1:]})
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -645,14 +684,7 @@ def main():
         name, launch_bytes, t_dom = adj_launch
     else:
         name, launch_bytes, t_dom = fwd_name, alg_bytes, t_fwd
-    achieved = launch_bytes / t_dom / 1e9
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"{name.split('<')[0]}_{a.workload}_traffic.json")
-    if os.path.exists(tfile):
-        tj = json.load(open(tfile))
-        # a back-projection launch: the figure of its driving axis
-        traffic = tj.get("per_axis", {}).get(name[:-1].split(",")[-1]) if "<" in name else tj.get("hbm_bytes_per_launch")
-
+    roof = roofline(a, wl, name, launch_bytes, t_dom)
     out = {
         "metric": metric,
         "value": round(2 * S * a.steps / wall / 1e9, 3),
@@ -672,11 +704,8 @@ def main():
                    "volume": list(wl["vol"]), "poses": wl["views"],
                    "detector": list(wl["det"]), "samples_per_pass": S, "rays": R,
                    "fwd_ms": round(t_fwd * 1e3, 3), "adj_ms": round(t_adj * 1e3, 3)},
-        "roofline": {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "algorithmic_bytes_per_launch": launch_bytes, "ms_per_launch": round(t_dom * 1e3, 3),
-                     "launches": {"forward": 1,
-                                  "back_projection": len(adj_launches) if len(adj_launches) > 1 else "pass"}},
+        "roofline": dict(roof, launches={"forward": 1, "back_projection": len(adj_launches) if len(adj_launches) > 1
+                                                                             else "pass"}),
     }
     if a.cg_iters > 0 and a.projector == "joseph":
         out["cg"] = cg_run(E, op, x, a.cg_iters, vd, dd, a.cg_target, a.cg_ref_iters)
