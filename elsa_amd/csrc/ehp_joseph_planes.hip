@@ -1,7 +1,9 @@
 // ehp_joseph_planes.hip -- Joseph forward projection over plane-major volume views (FP32).
 //
 // The forward of elsa's Joseph method (elsa/projectors/JosephsMethod.cpp:140-198, forward
-// branch :187-194) with one lane per ray. A wavefront holds a patch of neighbouring detector
+// branch :187-194) with one lane per ray, each sample's bilinear interpolation in lerp form
+// (plane_taps_add) and the sum in FP64: elsa's samples, floors and weights, elsa's sums to FP32
+// rounding. A wavefront holds a patch of neighbouring detector
 // pixels (8 x 8, or 16 x 4 over row-pair views), and its rays step through the volume's
 // planes together, so at each step its lanes
 // read the same plane at neighbouring in-plane positions. Those reads coalesce only if the axis

@@ -314,7 +314,8 @@ namespace ehp
     // the two transverse weights in FP32, elsa's value, the driving axis' weight being 1), for a
     // caller that scales it in FP64. The position
     // and weights are elsa's (doInterpolation, JosephsMethod.cpp:60-97; driving weight 1) with the
-    // forward's position rule fma(s, step, pos0), so they equal the forward's bit for bit; a 3D
+    // forward's position rule fma(s, step, pos0), so positions and floors equal the forward's bit
+    // for bit (the forward interpolates in lerp form, the same weights up to FP32 rounding); a 3D
     // tap's weight is applied as (ys * w2) * w1. LOW: evaluate c as elsa writes it (tiles on the
     // low faces, tile_low_face); otherwise as fract(q - 0.5), the same value there.
     template <int D, bool LOW, bool W, class Add>
@@ -790,9 +791,12 @@ namespace ehp
         s1 = s1 < r.nsteps ? s1 : r.nsteps;
     }
 
-    // The taps of one sample and their weights (2D: the first two of each).
+    // The taps of one sample and its fractional positions: v00 = (F1, F2), v10 = (C1, F2),
+    // v01 = (F1, C2), v11 = (C1, C2) along the transverse axes b1, b2 of its driving axis, C = F + 1,
+    // and c1, c2 = position - 0.5 - F along b1, b2 (2D: v00, v10 and c1 only).
     struct SampleTaps {
-        float v[4], w[4];
+        float v00, v10, v01, v11;
+        float c1, c2;
     };
 
     template <int D, int A, bool FB2, bool V16, bool PR = false>
@@ -804,26 +808,23 @@ namespace ehp
         const int n1 = dims.vol[b1], n2 = (D == 3) ? dims.vol[b2] : 1;
         const char* pb = (const char*) V.base + poff; // the plane (the same for the whole wavefront)
         const float q1 = __builtin_fmaf(fs, r.s1, r.p1);
-        const float g1 = floorf(q1 - 0.5f);
-        const float c1 = (q1 - g1) - 0.5f;
-        const float w1 = 1.0f - c1;
+        const float t1 = q1 - 0.5f;
+        const float g1 = floorf(t1);
         const int k1 = pad_clamp(g1, n1);
         SampleTaps T;
+        T.c1 = t1 - g1;
         if (D == 2) {
             // offset of element k1 + 2 of the plane's only row
             const Pair fk = ld_pair<V16>(V, pb, ((uint32_t) k1 << ES) + (2u << ES));
-            // A = 0: (pa, C1) then (pa, F1); A = 1: (F0, pa) then (C0, pa)
-            T.v[0] = A == 0 ? fk.b : fk.a;
-            T.w[0] = A == 0 ? c1 : w1;
-            T.v[1] = A == 0 ? fk.a : fk.b;
-            T.w[1] = A == 0 ? w1 : c1;
-            T.v[2] = T.v[3] = T.w[2] = T.w[3] = 0.0f;
+            T.v00 = fk.a;
+            T.v10 = fk.b;
+            T.v01 = T.v11 = T.c2 = 0.0f;
         } else {
             const float q2 = __builtin_fmaf(fs, r.s2, r.p2);
-            const float g2 = floorf(q2 - 0.5f);
-            const float c2 = (q2 - g2) - 0.5f;
-            const float w2 = 1.0f - c2;
+            const float t2 = q2 - 0.5f;
+            const float g2 = floorf(t2);
             const int k2 = pad_clamp(g2, n2);
+            T.c2 = t2 - g2;
             // the F and F + 1 taps along the fast axis are neighbours: two 8-byte loads, one per
             // row of the slow axis; (slow + 2) * RS + (fast + 2) elements in, the 2s being the padding
             const int ks = FB2 ? k1 : k2, kf = FB2 ? k2 : k1;
@@ -844,52 +845,41 @@ namespace ehp
                 r1 = ld_pair<V16>(V, pb + V.RS4, o);
             }
             // FB2: rows are b1 = F1, F1 + 1, pairs run along b2; else rows are b2, pairs along b1
-            const float vff = r0.a;
-            const float vkf = FB2 ? r1.a : r0.b;
-            const float vfk = FB2 ? r0.b : r1.a;
-            const float vkk = r1.b;
-            const float wff = w1 * w2, wkk = c1 * c2, wfk = w1 * c2, wkf = c1 * w2;
-            T.v[0] = vff;
-            T.w[0] = wff;
-            if (A == 0) {
-                T.v[1] = vkk, T.w[1] = wkk;
-                T.v[2] = vfk, T.w[2] = wfk;
-                T.v[3] = vkf, T.w[3] = wkf;
-            } else if (A == 1) {
-                T.v[1] = vfk, T.w[1] = wfk;
-                T.v[2] = vkk, T.w[2] = wkk;
-                T.v[3] = vkf, T.w[3] = wkf;
-            } else {
-                T.v[1] = vfk, T.w[1] = wfk;
-                T.v[2] = vkf, T.w[2] = wkf;
-                T.v[3] = vkk, T.w[3] = wkk;
-            }
+            T.v00 = r0.a;
+            T.v10 = FB2 ? r1.a : r0.b;
+            T.v01 = FB2 ? r0.b : r1.a;
+            T.v11 = r1.b;
         }
         return T;
     }
 
-    // acc plus the sample's taps in order, each as (value * len) * weight
+    // acc plus the sample's bilinear (2D: linear) interpolation times len: the interpolation in
+    // FP32 lerp form (along b1 with c1, then in 3D along b2 with c2), the sum in FP64 (one
+    // conversion and one FMA per sample). An FP32 sum would round every step the same way where
+    // the volume is constant (equal increments), an error growing with the number of samples.
     template <int D>
-    EHP_HD inline float plane_taps_add(const SampleTaps& T, float len, float acc)
+    EHP_HD inline double plane_taps_add(const SampleTaps& T, double len, double acc)
     {
-#pragma unroll
-        for (int k = 0; k < (D == 2 ? 2 : 4); ++k)
-            acc += T.v[k] * len * T.w[k];
-        return acc;
+        const float a0 = __builtin_fmaf(T.c1, T.v10 - T.v00, T.v00);
+        if (D == 2)
+            return __builtin_fma((double) a0, len, acc);
+        const float a1 = __builtin_fmaf(T.c1, T.v11 - T.v01, T.v01);
+        return __builtin_fma((double) __builtin_fmaf(T.c2, a1 - a0, a0), len, acc);
     }
 
     // Sample s (fs = s) of a ray (driving axis A) on the plane at byte offset poff of its padded
     // view, added to acc; FB2 says that the fast in-plane axis of the view is b2 (else b1). The
-    // interpolation has the order and weights of elsa's doInterpolation (JosephsMethod.cpp:60-97):
-    // 2D taps (F0,C1)(C0,F1), 3D taps (F0,F1,F2)(F0,C1,C2)(C0,F1,C2)(C0,C1,F2), the driving axis'
-    // floor and ceil being its plane; each tap adds (value * len) * weight, elsa's association.
-    // C = F + 1 is read even where elsa's ceil gives C = F: the weight of that tap is 0 then.
-    // Positions are those of joseph_sample and tile_ray_adjoint.
+    // interpolation is elsa's doInterpolation (JosephsMethod.cpp:60-97: bilinear in the
+    // transverse axes with floor F = floor(q - 0.5) and fraction q - 0.5 - F, the driving axis on
+    // its plane, taps outside the volume 0) in lerp form, which needs 7 operations per 3D sample
+    // where elsa's weight products need 16, summed in FP64: elsa's sums to FP32 rounding (not
+    // bit for bit). C = F + 1 is read even where elsa's ceil gives C = F: its fraction is 0 then.
+    // Positions and floors are those of joseph_sample and tile_ray_adjoint, bit for bit.
     template <int D, int A, bool FB2, bool V16, bool PR = false>
-    EHP_HD inline float plane_sample_forward(const RayRec& r, const Dims& dims, const PadView& V, uint32_t poff,
-                                             float fs, float acc)
+    EHP_HD inline double plane_sample_forward(const RayRec& r, const Dims& dims, const PadView& V, uint32_t poff,
+                                              float fs, double len, double acc)
     {
-        return plane_taps_add<D>(plane_sample_taps<D, A, FB2, V16, PR>(r, dims, V, poff, fs), r.len, acc);
+        return plane_taps_add<D>(plane_sample_taps<D, A, FB2, V16, PR>(r, dims, V, poff, fs), len, acc);
     }
 
     // Forward projection of one ray (driving axis A) through its padded view: the sum of
@@ -899,15 +889,16 @@ namespace ehp
     {
         int s0, s1;
         plane_sample_range(r, dims.vol[A], s0, s1);
-        float acc = 0.0f;
+        double acc = 0.0;
         if (s0 >= s1)
-            return acc;
+            return 0.0f;
+        const double len = r.len;
         const bool neg = (r.meta & 4) != 0;
         uint32_t poff = (uint32_t) (neg ? r.P0 - s0 : r.P0 + s0) * V.PS4;
         const uint32_t dp = neg ? 0u - V.PS4 : V.PS4;
         for (int s = s0; s < s1; ++s, poff += dp)
-            acc = plane_sample_forward<D, A, FB2, V16, PR>(r, dims, V, poff, (float) s, acc);
-        return acc;
+            acc = plane_sample_forward<D, A, FB2, V16, PR>(r, dims, V, poff, (float) s, len, acc);
+        return (float) acc;
     }
 
     // A ray's samples in traversal coordinates, t = plane when it steps towards higher planes and
@@ -944,25 +935,26 @@ namespace ehp
         float fs = (float) s_first;            // its sample index (exact: |s| < 2^24)
         uint32_t poff = (uint32_t) (NEG ? -t0 : t0) * V.PS4;
         const uint32_t dp = NEG ? 0u - V.PS4 : V.PS4;
-        float acc = 0.0f;
+        const double len = r.len;
+        double acc = 0.0;
         int it = 0;
         for (; it < full0; ++it, ++j, fs += 1.0f, poff += dp)
             if (j < n)
-                acc = plane_sample_forward<D, A, FB2, V16, PR>(r, dims, V, poff, fs, acc);
+                acc = plane_sample_forward<D, A, FB2, V16, PR>(r, dims, V, poff, fs, len, acc);
         // two samples per step, both fetched before either is added, so that each wavefront
         // has four loads in flight
         for (; it + 1 < full1; it += 2, fs += 2.0f, poff += 2u * dp) {
             const SampleTaps Ta = plane_sample_taps<D, A, FB2, V16, PR>(r, dims, V, poff, fs);
             const SampleTaps Tb = plane_sample_taps<D, A, FB2, V16, PR>(r, dims, V, poff + dp, fs + 1.0f);
-            acc = plane_taps_add<D>(Ta, r.len, acc);
-            acc = plane_taps_add<D>(Tb, r.len, acc);
+            acc = plane_taps_add<D>(Ta, len, acc);
+            acc = plane_taps_add<D>(Tb, len, acc);
         }
         for (; it < full1; ++it, fs += 1.0f, poff += dp)
-            acc = plane_sample_forward<D, A, FB2, V16, PR>(r, dims, V, poff, fs, acc);
+            acc = plane_sample_forward<D, A, FB2, V16, PR>(r, dims, V, poff, fs, len, acc);
         j += (uint32_t) (full1 - full0);
         for (; it < niter; ++it, ++j, fs += 1.0f, poff += dp)
             if (j < n)
-                acc = plane_sample_forward<D, A, FB2, V16, PR>(r, dims, V, poff, fs, acc);
-        return acc;
+                acc = plane_sample_forward<D, A, FB2, V16, PR>(r, dims, V, poff, fs, len, acc);
+        return (float) acc;
     }
 } // namespace ehp

@@ -24,6 +24,9 @@ _lib = None
 
 JOSEPH, SIDDON = 0, 1
 MODE_ELSA, MODE_DIRECT = 0, 1
+# Joseph only: MODE_DIRECT's sample points, weights formed in the data type (with FP64 data, the
+# exact bilinear interpolation at the FP32 sample points)
+MODE_EXACT_WEIGHTS = 2
 
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")

@@ -190,14 +190,17 @@ namespace orc
         }
     }
 
-    enum { MODE_ELSA = 0, MODE_DIRECT = 1 };
+    enum { MODE_ELSA = 0, MODE_DIRECT = 1, MODE_EXACT_WEIGHTS = 2 };
 
     // elsa/projectors/JosephsMethod.cpp:140-198 with getLinearInterpolationWeights
     // (:37-52) and doInterpolation (:60-97). MODE_ELSA advances the position by repeated
     // addition of the step, exactly as DrivingDirectionTraversal::updateTraverse
     // (DrivingDirectionTraversal.cpp:59-64). MODE_DIRECT evaluates sample s as
     // pos0 + s*step with one fused multiply-add per transverse axis, which is what the
-    // HIP kernels compute.
+    // HIP kernels compute. MODE_EXACT_WEIGHTS takes MODE_DIRECT's positions and floors (in R)
+    // but forms the fractions, the weights 1 - c and their products in T: with T = double, the
+    // exact bilinear interpolation at the FP32 sample points (the target both elsa's FP32 weight
+    // products and a lerp-form evaluation approximate).
     template <class R, class T, int D, bool ADJ>
     void joseph_ray(const JRay<R>& jr, const i64* dims, const i64* strides, int mode,
                     const T* in, T* out, i64 ir)
@@ -207,7 +210,7 @@ namespace orc
         const T len = static_cast<T>(jr.len);
         R pos[3] = {jr.pos0[0], jr.pos0[1], jr.pos0[2]};
         for (i64 s = 0; s < jr.nsteps; ++s) {
-            if (mode == MODE_DIRECT) {
+            if (mode == MODE_DIRECT || mode == MODE_EXACT_WEIGHTS) {
                 for (int i = 0; i < D; ++i)
                     pos[i] = (i == a) ? jr.pos0[i] + R(s) * jr.step[i]
                                       : std::fma(R(s), jr.step[i], jr.pos0[i]);
@@ -223,7 +226,14 @@ namespace orc
             w[a] = R(1);
             cw[a] = R(1);
 
-            auto tap = [&](const i64* c, R wt) {
+            // the weights in T for MODE_EXACT_WEIGHTS, else in R as elsa forms them
+            T wT[3], cwT[3];
+            for (int i = 0; i < D; ++i) {
+                cwT[i] = (i == a) ? T(1) : (static_cast<T>(pos[i]) - static_cast<T>(fl[i])) - T(0.5);
+                wT[i] = (i == a) ? T(1) : T(1) - cwT[i];
+            }
+            const bool exactw = mode == MODE_EXACT_WEIGHTS;
+            auto tap = [&](const i64* c, R wt, T wt_exact) {
                 bool inbox = true;
                 i64 lin = 0;
                 for (int i = 0; i < D; ++i) {
@@ -231,28 +241,28 @@ namespace orc
                     const i64 cc = std::max<i64>(0, std::min<i64>(c[i], dims[i] - 1));
                     lin += cc * strides[i];
                 }
-                const R weight = R(inbox ? 1 : 0) * wt;
+                const T weight = exactw ? T(inbox ? 1 : 0) * wt_exact : static_cast<T>(R(inbox ? 1 : 0) * wt);
                 if (ADJ) {
 #pragma omp atomic
-                    out[lin] += in[ir] * len * static_cast<T>(weight);
+                    out[lin] += in[ir] * len * weight;
                 } else {
-                    acc += in[lin] * len * static_cast<T>(weight);
+                    acc += in[lin] * len * weight;
                 }
             };
             if (D == 2) {
                 const i64 c0[2] = {fl[0], cl[1]};
-                tap(c0, w[0] * cw[1]);
+                tap(c0, w[0] * cw[1], wT[0] * cwT[1]);
                 const i64 c1[2] = {cl[0], fl[1]};
-                tap(c1, cw[0] * w[1]);
+                tap(c1, cw[0] * w[1], cwT[0] * wT[1]);
             } else {
                 const i64 c0[3] = {fl[0], fl[1], fl[2]};
-                tap(c0, w[0] * w[1] * w[2]);
+                tap(c0, w[0] * w[1] * w[2], wT[0] * wT[1] * wT[2]);
                 const i64 c1[3] = {fl[0], cl[1], cl[2]};
-                tap(c1, w[0] * cw[1] * cw[2]);
+                tap(c1, w[0] * cw[1] * cw[2], wT[0] * cwT[1] * cwT[2]);
                 const i64 c2[3] = {cl[0], fl[1], cl[2]};
-                tap(c2, cw[0] * w[1] * cw[2]);
+                tap(c2, cw[0] * w[1] * cw[2], cwT[0] * wT[1] * cwT[2]);
                 const i64 c3[3] = {cl[0], cl[1], fl[2]};
-                tap(c3, cw[0] * cw[1] * w[2]);
+                tap(c3, cw[0] * cw[1] * w[2], cwT[0] * cwT[1] * wT[2]);
             }
             if (mode == MODE_ELSA)
                 for (int i = 0; i < D; ++i)

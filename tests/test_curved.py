@@ -74,9 +74,10 @@ def _curved_case(dim):
 @pytest.mark.parametrize("dim", [2, 3])
 @pytest.mark.parametrize("cls", ["JosephsMethodCUDA", "SiddonsMethodCUDA"])
 def test_curved_projectors_match_oracle(dim, cls):
-    """Rays through the planar coordinates: forward bit for bit (same rays, same traversal),
-    back projection (one thread per ray, atomics) within 1e-6; and different from the planar
-    detector of the same trajectory."""
+    """Rays through the planar coordinates: forward bit for bit for Siddon (same rays, same
+    traversal) and within 1e-6 for Joseph (lerp-form interpolation), back projection (one thread
+    per ray, atomics) within 1e-6; and different from the planar detector of the same
+    trajectory."""
     import elsa_amd as E
 
     vd, dd, g = _curved_case(dim)
@@ -87,7 +88,12 @@ def test_curved_projectors_match_oracle(dim, cls):
     y = rng.random(g.sino_shape, dtype=np.float32)
     fn = oracle.joseph if cls == "JosephsMethodCUDA" else oracle.siddon
     fwd = op.apply(x)
-    np.testing.assert_array_equal(fwd, fn(g, x, mode=oracle.MODE_DIRECT, pix=pix))
+    ref_fwd = fn(g, x, mode=oracle.MODE_DIRECT, pix=pix)
+    if cls == "JosephsMethodCUDA":
+        # the plane-major forward interpolates in lerp form: FP32 rounding of elsa's sums
+        assert np.linalg.norm(fwd - ref_fwd) <= 1e-6 * np.linalg.norm(ref_fwd)
+    else:
+        np.testing.assert_array_equal(fwd, ref_fwd)
     adj = op.applyAdjoint(y)
     ref = fn(g, y, adjoint=True, mode=oracle.MODE_DIRECT, pix=pix)
     assert np.linalg.norm(adj - ref) <= 1e-6 * np.linalg.norm(ref)

@@ -358,6 +358,50 @@ def test_joseph_fp64_brute_force_tiny():
         assert y[r] == pytest.approx(acc * ln[r], rel=1e-9, abs=1e-9), r
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_joseph_exact_weights_brute_force_fp32_points(dim):
+    """MODE_EXACT_WEIGHTS with FP64 data (the reference of the lerp-form forward): for an FP32
+    geometry, the value of each ray equals len * sum over its samples of the bilinear (2D:
+    linear) interpolation at the FP32 sample points fma(s, step, pos0), with the weights formed
+    exactly, recomputed here in FP64 from the oracle's own FP32 ray states; and it differs from
+    elsa's FP32 weight products (MODE_DIRECT, acc64) only by their rounding."""
+    n = 7
+    det = (9, 6) if dim == 3 else (10,)
+    g = _rand_geom(dim, n, 4, det, np.float32, seed=20, s2c=40.0)
+    rng = np.random.default_rng(21)
+    x = rng.standard_normal(g.vol_shape)
+    y = oracle.joseph(g, x, mode=oracle.MODE_EXACT_WEIGHTS, acc64=True).ravel()
+    elsa_w = oracle.joseph(g, x, mode=oracle.MODE_DIRECT, acc64=True).ravel()
+    assert np.linalg.norm(y - elsa_w) <= 1e-6 * np.linalg.norm(y)
+    axis, nsteps, pos0, step, ln = oracle.joseph_rays(g)
+
+    def val(c):
+        if all(0 <= c[i] < n for i in range(dim)):
+            return x[tuple(c[::-1])]
+        return 0.0
+
+    for r in range(y.size):
+        acc = 0.0
+        a = axis[r]
+        bs = [i for i in range(dim) if i != a]
+        for s in range(nsteps[r]):
+            # fma(s, step, pos0) in FP32: the product and sum are exact in FP64, rounded once
+            p = [np.float32(float(pos0[r, i]) + (s * float(step[r, i]) if i != a else 0.0)) if i != a
+                 else np.float32(np.float32(pos0[r, i]) + np.float32(np.float32(s) * step[r, i])) for i in range(dim)]
+            f = [int(math.floor(np.float32(p[i] - np.float32(0.5)))) for i in range(dim)]
+            t = [float(p[i]) - 0.5 - f[i] for i in range(dim)]
+            for d1 in (0, 1):
+                for d2 in ((0, 1) if dim == 3 else (0,)):
+                    c = list(f)
+                    c[bs[0]] += d1
+                    w = t[bs[0]] if d1 else 1 - t[bs[0]]
+                    if dim == 3:
+                        c[bs[1]] += d2
+                        w *= t[bs[1]] if d2 else 1 - t[bs[1]]
+                    acc += w * val(c)
+        assert y[r] == pytest.approx(acc * float(ln[r]), rel=1e-12, abs=1e-12), r
+
+
 # --------------------------------------------------------------------------------------
 # solver restatements, pinned by closed forms on small dense problems
 

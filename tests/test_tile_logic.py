@@ -1,6 +1,7 @@
 """CPU run of the LDS-tiled Joseph kernels' index logic (the same functions the HIP kernels
 call, built with g++): every LDS slot and volume index in range, every voxel owned by exactly
-one tile per pass, and the assembled back projection equal to the oracle's."""
+one tile per pass, and the assembled back projection equal to the oracle's; the plane-major
+forward's views and loops against the oracle (fp32_bounds) and each other."""
 import ctypes
 import os
 import subprocess
@@ -9,6 +10,7 @@ import numpy as np
 import pytest
 
 import oracle
+from fp32_bounds import assert_forward_close, oracle_forward_refs
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 # ELSA_SANITIZE=1 builds the checks with AddressSanitizer and UndefinedBehaviorSanitizer under
@@ -127,17 +129,20 @@ WAVE_IDS = ["per_ray", "wave64x1", "wave16x4", "wave8x8"]
 
 @pytest.mark.parametrize("waves", WAVES, ids=WAVE_IDS)
 @pytest.mark.parametrize("case", FWD_CASES, ids=lambda c: f"{c[0]}d-{'x'.join(map(str, c[1]))}")
-def test_plane_forward_logic_matches_oracle_bitwise(lib, case, waves):
-    """Every zero-padded plane-major view a driving axis can get gives the oracle's forward, bit
-    for bit (same positions, taps and summation order as the one-thread-per-ray rule), with
-    every read inside its view -- both with one plane loop per ray and with the GPU's grouping,
-    where the rays of a wavefront of one driving axis and direction share one plane loop."""
+def test_plane_forward_logic_matches_oracle(lib, case, waves):
+    """Every zero-padded plane-major view a driving axis can get gives the oracle's forward to
+    FP32 rounding (same positions, floors and taps, lerp-form interpolation; the bars of
+    fp32_bounds: at least as close to the exact sums as elsa's FP32 form), with every read inside
+    its view -- both with one plane loop per ray and with the GPU's grouping, where the rays of a
+    wavefront of one driving axis and direction share one plane loop. Every grouping and view
+    layout gives the same result bit for bit."""
     dim, vol, views, det, s2c, c2d, fans = case
     rng = np.random.default_rng(2)
     angles = np.sort(rng.uniform(0, 360, views)).astype(np.float32)
     g = oracle.circle_geometry(dim, vol, views, 360, s2c, c2d, det=det, angles_deg=angles)
     x = rng.standard_normal(g.vol_shape).astype(np.float32)
-    ref = oracle.joseph(g, x, mode=oracle.MODE_DIRECT)
+    refs = oracle_forward_refs(g, x)
+    first = None
     for fan in fans:
         out = np.full(g.sino_shape, np.nan, np.float32)
         rc = lib.plane_forward(dim, np.array(list(vol) + [1] * (3 - dim), np.int64),
@@ -145,17 +150,21 @@ def test_plane_forward_logic_matches_oracle_bitwise(lib, case, waves):
                                np.ascontiguousarray(g.P), np.ascontiguousarray(g.Pinv), np.ascontiguousarray(g.C),
                                np.array(list(fan) + [0] * (3 - len(fan)), np.int32), x, out, waves)
         assert rc == 0, (fan, rc)
-        np.testing.assert_array_equal(out, ref, err_msg=f"fan axes {fan}")
+        assert_forward_close(out, *refs, what=f"fan axes {fan}")
+        if first is None:
+            first = out
+        np.testing.assert_array_equal(out, first, err_msg=f"fan axes {fan}")
 
 
 @pytest.mark.parametrize("waves", WAVES, ids=WAVE_IDS)
 @pytest.mark.parametrize("case", [c for c in FWD_CASES if c[0] == 3], ids=lambda c: "x".join(map(str, c[1])))
 @pytest.mark.parametrize("b16", [False, True], ids=["f32", "bf16"])
-def test_plane_forward_row_pair_views_match_oracle_bitwise(lib, case, waves, b16):
+def test_plane_forward_row_pair_views_match_oracle(lib, case, waves, b16):
     """Row-pair views (the 3D forward's layout: every voxel stored twice, each sample's 2 x 2
     footprint one read, 16 bytes in FP32 and 8 in bfloat16) give the oracle's forward of the
-    (rounded) volume bit for bit, every read inside its view and aligned, every view element
-    written at most once."""
+    (rounded) volume to FP32 rounding (fp32_bounds), and the same sums bit for bit as views with
+    one element per voxel; every read inside its view and aligned, every view element written
+    at most once."""
     dim, vol, views, det, s2c, c2d, fans = case
     rng = np.random.default_rng(4)
     angles = np.sort(rng.uniform(0, 360, views)).astype(np.float32)
@@ -163,33 +172,40 @@ def test_plane_forward_row_pair_views_match_oracle_bitwise(lib, case, waves, b16
     x = rng.standard_normal(g.vol_shape).astype(np.float32)
     if b16:
         x = _to_bf16(x)
-    ref = oracle.joseph(g, x, mode=oracle.MODE_DIRECT)
+    refs = oracle_forward_refs(g, x)
     for fan in fans:
+        args = (dim, np.array(list(vol), np.int64), np.array(list(det), np.int64), views,
+                np.ascontiguousarray(g.P), np.ascontiguousarray(g.Pinv), np.ascontiguousarray(g.C),
+                np.array(list(fan), np.int32), x)
         out = np.full(g.sino_shape, np.nan, np.float32)
-        rc = lib.plane_forward(dim, np.array(list(vol), np.int64), np.array(list(det), np.int64), views,
-                               np.ascontiguousarray(g.P), np.ascontiguousarray(g.Pinv), np.ascontiguousarray(g.C),
-                               np.array(list(fan), np.int32), x, out, waves | 4 | int(b16))
+        rc = lib.plane_forward(*args, out, waves | 4 | int(b16))
         assert rc == 0, (fan, rc)
-        np.testing.assert_array_equal(out, ref, err_msg=f"fan axes {fan}")
+        assert_forward_close(out, *refs, what=f"fan axes {fan}")
+        single = np.full(g.sino_shape, np.nan, np.float32)
+        assert lib.plane_forward(*args, single, waves | int(b16)) == 0
+        np.testing.assert_array_equal(out, single, err_msg=f"fan axes {fan}")
 
 
 def test_plane_forward_wavefront_groups_cover_mixed_waves(lib):
     """A strongly divergent cone (source 60, detector 40 from the centre) puts rays of several
     driving axes and directions into one wavefront; each group's shared plane loop still gives
-    the oracle bit for bit."""
+    the oracle's forward (fp32_bounds) and the per-ray loop's sums bit for bit."""
     dim, vol, views, det = 3, (40, 24, 33), 13, (61, 29)
     rng = np.random.default_rng(7)
     angles = np.sort(rng.uniform(0, 360, views)).astype(np.float32)
     g = oracle.circle_geometry(dim, vol, views, 360, 20.0, 40.0, det=det, angles_deg=angles)
     x = rng.standard_normal(g.vol_shape).astype(np.float32)
-    ref = oracle.joseph(g, x, mode=oracle.MODE_DIRECT)
+    args = (dim, np.array(vol, np.int64), np.array(det, np.int64), views, np.ascontiguousarray(g.P),
+            np.ascontiguousarray(g.Pinv), np.ascontiguousarray(g.C), np.array([2, 2, 1], np.int32), x)
     out = np.full(g.sino_shape, np.nan, np.float32)
-    rc = lib.plane_forward(dim, np.array(vol, np.int64), np.array(det, np.int64), views,
-                           np.ascontiguousarray(g.P), np.ascontiguousarray(g.Pinv), np.ascontiguousarray(g.C),
-                           np.array([2, 2, 1], np.int32), x, out, 2 | (16 << 8))
+    rc = lib.plane_forward(*args, out, 2 | (16 << 8))
     assert rc == 0
     assert lib.plane_forward_mixed_waves() > 0
-    np.testing.assert_array_equal(out, ref)
+    check = oracle_forward_refs(g, x)
+    assert_forward_close(out, *check)
+    per_ray = np.full(g.sino_shape, np.nan, np.float32)
+    assert lib.plane_forward(*args, per_ray, 0) == 0
+    np.testing.assert_array_equal(out, per_ray)
 
 
 def _to_bf16(x):
@@ -201,23 +217,27 @@ def _to_bf16(x):
 
 @pytest.mark.parametrize("waves", WAVES, ids=WAVE_IDS)
 @pytest.mark.parametrize("case", FWD_CASES, ids=lambda c: f"{c[0]}d-{'x'.join(map(str, c[1]))}")
-def test_plane_forward_bf16_views_match_oracle_bitwise(lib, case, waves):
+def test_plane_forward_bf16_views_match_oracle(lib, case, waves):
     """bfloat16 views (half-width offsets, two-element loads) give the oracle's forward of the
-    bfloat16-rounded volume bit for bit, with every read inside its view."""
+    bfloat16-rounded volume (fp32_bounds) and the FP32 views' sums of it bit for bit, with every
+    read inside its view."""
     dim, vol, views, det, s2c, c2d, fans = case
     rng = np.random.default_rng(5)
     angles = np.sort(rng.uniform(0, 360, views)).astype(np.float32)
     g = oracle.circle_geometry(dim, vol, views, 360, s2c, c2d, det=det, angles_deg=angles)
     x = _to_bf16(rng.standard_normal(g.vol_shape).astype(np.float32))
-    ref = oracle.joseph(g, x, mode=oracle.MODE_DIRECT)
+    refs = oracle_forward_refs(g, x)
     for fan in fans:
+        args = (dim, np.array(list(vol) + [1] * (3 - dim), np.int64), np.array(list(det) + [1] * (2 - len(det)), np.int64),
+                views, np.ascontiguousarray(g.P), np.ascontiguousarray(g.Pinv), np.ascontiguousarray(g.C),
+                np.array(list(fan) + [0] * (3 - len(fan)), np.int32), x)
         out = np.full(g.sino_shape, np.nan, np.float32)
-        rc = lib.plane_forward(dim, np.array(list(vol) + [1] * (3 - dim), np.int64),
-                               np.array(list(det) + [1] * (2 - len(det)), np.int64), views,
-                               np.ascontiguousarray(g.P), np.ascontiguousarray(g.Pinv), np.ascontiguousarray(g.C),
-                               np.array(list(fan) + [0] * (3 - len(fan)), np.int32), x, out, waves | 1)
+        rc = lib.plane_forward(*args, out, waves | 1)
         assert rc == 0, (fan, rc)
-        np.testing.assert_array_equal(out, ref, err_msg=f"fan axes {fan}")
+        assert_forward_close(out, *refs, what=f"fan axes {fan}")
+        f32 = np.full(g.sino_shape, np.nan, np.float32)
+        assert lib.plane_forward(*args, f32, waves) == 0
+        np.testing.assert_array_equal(out, f32, err_msg=f"fan axes {fan}")
 
 
 def test_tiled_adjoint_logic_high_dynamic_range(lib):
