@@ -656,13 +656,13 @@ def main():
     elif a.projector == "siddon":
         # one FP32 voxel per crossing, one FP32 value per ray
         alg_bytes = 4 * S + 4 * R
-        fwd_name = "siddon_kernel"
+        fwd_name = "siddon_fwd_planes_kernel" if a.kernels == "fast" else "siddon_kernel"
         adj_name = "siddon_adj_tiled_kernel" if a.kernels == "fast" else "siddon_kernel"
         unit, metric = "Gcrossings/s", "Siddon fwd+adj voxel crossings/s & %HBM roofline"
     else:
         # one FP32 detector value per footprint tap, one FP32 voxel per voxel and pose
         alg_bytes = 4 * S + 4 * nvox * wl["views"]
-        adj_name = "voxel_adj_kernel"
+        adj_name = "voxel_adj_fast_kernel" if a.kernels == "fast" else "voxel_adj_kernel"
         fwd_name = "voxel_fwd_tiled_kernel" if a.kernels == "fast" else "voxel_fwd_kernel"
         unit, metric = "Gtaps/s", f"{a.projector} voxel projector fwd+adj footprint taps/s & %HBM roofline"
     # The tiled back projections run one launch per driving axis that occurs (a "pass"), each

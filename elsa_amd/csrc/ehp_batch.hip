@@ -1,12 +1,12 @@
 // ehp_batch.hip -- batches of independent problems (declared in include/elsa_hip.h).
 //
 // A batch applies many operators -- each with its own volume size, trajectory and detector --
-// as one block-diagonal operator over packed volumes and sinograms. For FP32 Joseph operators
-// with the fast kernels a projector pass is one launch over all problems: the launch's blocks
-// are the concatenation of the blocks every operator's own launch would run (job tables built
-// here once), so each problem's result is bit for bit its operator's, while the GPU is filled
-// by the batch as a whole instead of by one small problem at a time. Otherwise the batch calls
-// the operators one after another.
+// as one block-diagonal operator over packed volumes and sinograms. For FP32 Joseph or FP32
+// Siddon operators with the fast kernels (all of one kind) a projector pass is one launch over
+// all problems per kernel: the launch's blocks are the concatenation of the blocks every
+// operator's own launch would run (job tables built here once), so each problem's result is
+// bit for bit its operator's, while the GPU is filled by the batch as a whole instead of by one
+// small problem at a time. Otherwise the batch calls the operators one after another.
 #include "elsa_hip.h"
 #include "ehp_op.hpp"
 #include "ehp_solvers.hpp"
@@ -26,6 +26,10 @@ struct ehp_batch {
     std::vector<ehp_op*> ops;
     std::vector<int64_t> vol_off, sino_off; // n + 1
     bool fused = false;
+    int kind = KIND_JOSEPH; // of the fused passes
+    // fused Siddon passes (device tables; the copy, segment and block tables are shared)
+    ehp::SidFwdJob* d_sfjobs = nullptr;
+    ehp::SidAdjJob* d_sajobs[3] = {nullptr, nullptr, nullptr};
     // fused Joseph passes (device tables)
     ehp::CopyJob* d_cjobs = nullptr;
     int* d_cblocks = nullptr;
@@ -78,9 +82,10 @@ namespace
         return hipMemcpy(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
     }
 
-    bool fusable(const ehp_op* op, int dim)
+    bool fusable(const ehp_op* op, int kind, int dim)
     {
-        return op->kind == KIND_JOSEPH && op->dtype == EHP_F32 && op->planes && op->tiled && op->dims.dim == dim;
+        return (op->kind == KIND_JOSEPH || op->kind == KIND_SIDDON) && op->kind == kind && op->dtype == EHP_F32 &&
+               op->planes && op->tiled && op->dims.dim == dim;
     }
 
     int build_fused(ehp_batch* b)
@@ -93,6 +98,7 @@ namespace
             ehp::JosephBatchItem& it = items[j];
             it.geo = op->d_geo;
             it.table = op->d_table;
+            it.sidtable = op->d_sidtable;
             it.pose_axes = op->d_pose_axes;
             for (int a = 0; a < 3; ++a) {
                 it.views[a] = (float*) op->d_view[a];
@@ -108,12 +114,21 @@ namespace
         }
         std::vector<ehp::CopyJob> cj;
         std::vector<ehp::FwdJob> fj;
+        std::vector<ehp::SidFwdJob> sfj;
         std::vector<int> cb, fb, sb;
         std::vector<ehp::SegJob> sg;
         std::vector<ehp::AdjJob> aj[3];
+        std::vector<ehp::SidAdjJob> saj[3];
         std::vector<int> ab[3];
-        ehp::joseph_forward_batch_jobs(items.data(), n, cj, cb, fj, fb);
-        ehp::joseph_adjoint_batch_jobs(items.data(), n, sg, sb, aj, ab, b->lds);
+        const bool sid = b->kind == KIND_SIDDON;
+        if (sid) {
+            ehp::siddon_forward_batch_jobs(items.data(), n, cj, cb, sfj, fb);
+            ehp::batch_segments(items.data(), n, true, sg, sb);
+            ehp::siddon_adjoint_batch_jobs(items.data(), n, saj, ab, b->lds);
+        } else {
+            ehp::joseph_forward_batch_jobs(items.data(), n, cj, cb, fj, fb);
+            ehp::joseph_adjoint_batch_jobs(items.data(), n, sg, sb, aj, ab, b->lds);
+        }
         int max_lds = 65536;
         if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, b->device) != hipSuccess)
             max_lds = 65536;
@@ -125,7 +140,7 @@ namespace
         if (e == hipSuccess)
             e = upload(b, cb, &b->d_cblocks);
         if (e == hipSuccess)
-            e = upload(b, fj, &b->d_fjobs);
+            e = sid ? upload(b, sfj, &b->d_sfjobs) : upload(b, fj, &b->d_fjobs);
         if (e == hipSuccess)
             e = upload(b, fb, &b->d_fblocks);
         if (e == hipSuccess)
@@ -135,7 +150,7 @@ namespace
         if (e == hipSuccess)
             e = upload(b, bounds, &b->d_bounds);
         for (int a = 0; a < 3 && e == hipSuccess; ++a) {
-            e = upload(b, aj[a], &b->d_ajobs[a]);
+            e = sid ? upload(b, saj[a], &b->d_sajobs[a]) : upload(b, aj[a], &b->d_ajobs[a]);
             if (e == hipSuccess)
                 e = upload(b, ab[a], &b->d_ablocks[a]);
             b->nablocks[a] = (int64_t) ab[a].size();
@@ -208,9 +223,22 @@ namespace
         }
         if (e != hipSuccess)
             return bfail(EHP_ERR_HIP, "ordering after the previous calls: %s", hipGetErrorString(e));
-        if (!adjoint) {
+        if (!adjoint && b->kind == KIND_SIDDON) {
+            e = ehp::siddon_forward_batch(b->dim, b->d_cjobs, b->d_cblocks, b->ncblocks, b->d_sfjobs, b->d_fblocks,
+                                          b->nfblocks, (const float*) in, (float*) out, s);
+        } else if (!adjoint) {
             e = ehp::joseph_forward_batch(b->dim, b->d_cjobs, b->d_cblocks, b->ncblocks, b->d_fjobs, b->d_fblocks,
                                           b->nfblocks, (const float*) in, (float*) out, s);
+        } else if (b->kind == KIND_SIDDON) {
+            e = ehp::launch_fill<float>((float*) out, b->vol_off.back(), 0.0f, s);
+            if (e == hipSuccess)
+                e = ehp::fixed_point_exponent_batch((int) b->ops.size(), b->d_segs, b->d_sblocks, b->nsblocks,
+                                                    b->d_bounds, b->d_maxbits, b->d_fixexp, (const float*) in, s);
+            const ehp::SidAdjJob* sj[3] = {b->d_sajobs[0], b->d_sajobs[1], b->d_sajobs[2]};
+            const int* ab[3] = {b->d_ablocks[0], b->d_ablocks[1], b->d_ablocks[2]};
+            if (e == hipSuccess)
+                e = ehp::siddon_adjoint_batch(b->dim, sj, ab, b->nablocks, b->lds, b->d_fixexp, (const float*) in,
+                                              (float*) out, s);
         } else {
             const ehp::AdjJob* aj[3] = {b->d_ajobs[0], b->d_ajobs[1], b->d_ajobs[2]};
             const int* ab[3] = {b->d_ablocks[0], b->d_ablocks[1], b->d_ablocks[2]};
@@ -290,10 +318,11 @@ int ehp_batch_create(ehp_op* const* ops, int n, ehp_batch** out)
     b->vol_off.assign(n + 1, 0);
     b->sino_off.assign(n + 1, 0);
     bool fuse = true;
+    b->kind = ops[0]->kind;
     for (int j = 0; j < n; ++j) {
         b->vol_off[j + 1] = b->vol_off[j] + ops[j]->dims.nvox;
         b->sino_off[j + 1] = b->sino_off[j] + ops[j]->dims.nrays;
-        fuse = fuse && fusable(ops[j], b->dim);
+        fuse = fuse && fusable(ops[j], b->kind, b->dim);
     }
     int cur = -1;
     if (hipGetDevice(&cur) != hipSuccess || cur != b->device) {

@@ -128,19 +128,6 @@ namespace ehp
             view_copy_transpose<E>(in, out, dims, L, blockIdx.x, blockIdx.y, blockIdx.z);
         }
 
-        // Blocks of the view copy of one volume into its view (rows: a flat range; transpose:
-        // the grid (gx, gy, n_t) flattened with x fastest).
-        inline int64_t view_copy_blocks(const Dims& dims, const PadLayout& L, int& gx, int& gy)
-        {
-            if (L.f == 0) {
-                gx = gy = 0;
-                return (dims.nvox + 255) / 256;
-            }
-            gx = (dims.vol[0] + kTile - 1) / kTile;
-            gy = (dims.vol[L.f] + kTile - 1) / kTile;
-            return (int64_t) gx * gy * dims.vol[3 - L.f];
-        }
-
         // Block id of one pose-band tile of the forward: id = (band * nposes + pose) * tiles_u + tu.
         // Each wavefront holds WU x (64 / WU) neighbouring pixels (2D: 64 of the row). Its rays fall into
         // groups by driving axis and direction (in practice one or two), and each group runs one
@@ -338,14 +325,38 @@ namespace ehp
         }
     }
 
+    // Blocks of the view copy of one volume into its view (rows: a flat range; transpose:
+    // the grid (gx, gy, n_t) flattened with x fastest).
+    int64_t view_copy_blocks(const Dims& dims, const PadLayout& L, int& gx, int& gy)
+    {
+        if (L.f == 0) {
+            gx = gy = 0;
+            return (dims.nvox + 255) / 256;
+        }
+        gx = (dims.vol[0] + kTile - 1) / kTile;
+        gy = (dims.vol[L.f] + kTile - 1) / kTile;
+        return (int64_t) gx * gy * dims.vol[3 - L.f];
+    }
+
+    hipError_t view_copy_batch(const CopyJob* cjobs, const int* cblocks, int64_t ncblocks, const float* vols,
+                               hipStream_t s)
+    {
+        if (ncblocks > 0x7fffffff)
+            return hipErrorInvalidValue;
+        if (ncblocks > 0)
+            hipLaunchKernelGGL(view_copy_batch_kernel, dim3((unsigned) ncblocks), dim3(256), 0, s, cjobs, cblocks, vols);
+        return hipGetLastError();
+    }
+
     hipError_t joseph_forward_batch(int dim, const CopyJob* cjobs, const int* cblocks, int64_t ncblocks,
                                     const FwdJob* fjobs, const int* fblocks, int64_t nfblocks, const float* vols,
                                     float* sinos, hipStream_t s)
     {
-        if (ncblocks > 0x7fffffff || nfblocks > 0x7fffffff)
+        if (nfblocks > 0x7fffffff)
             return hipErrorInvalidValue;
-        if (ncblocks > 0)
-            hipLaunchKernelGGL(view_copy_batch_kernel, dim3((unsigned) ncblocks), dim3(256), 0, s, cjobs, cblocks, vols);
+        hipError_t e = view_copy_batch(cjobs, cblocks, ncblocks, vols, s);
+        if (e != hipSuccess)
+            return e;
         const int64_t per_xcd = (nfblocks + 7) / 8;
         if (per_xcd * 8 > (int64_t) 0x7fffffff)
             return hipErrorInvalidValue;

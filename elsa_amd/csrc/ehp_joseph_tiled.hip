@@ -462,12 +462,45 @@ namespace ehp
 
     // Segments of a batch's sinograms (absmax) and the tiles of every operator per driving
     // axis (one job per operator that has rays of that axis).
+    // One segment per operator of the packed sinograms (sinogram) or volumes, each reduced by
+    // up to 256 blocks of the batched absmax.
+    void batch_segments(const JosephBatchItem* items, int n, bool sinogram, std::vector<SegJob>& segs,
+                        std::vector<int>& sblocks)
+    {
+        segs.clear();
+        sblocks.clear();
+        for (int j = 0; j < n; ++j) {
+            const JosephBatchItem& it = items[j];
+            SegJob sg{};
+            sg.off = sinogram ? it.sino_off : it.vol_off;
+            sg.n = sinogram ? it.dims.nrays : it.dims.nvox;
+            int64_t nb = (sg.n + 256 * 16 - 1) / (256 * 16);
+            nb = nb > 256 ? 256 : (nb < 1 ? 1 : nb);
+            sg.nblocks = (int) nb;
+            sg.block0 = (int64_t) sblocks.size();
+            sblocks.insert(sblocks.end(), (size_t) nb, j);
+            segs.push_back(sg);
+        }
+    }
+
+    hipError_t fixed_point_exponent_batch(int nitems, const SegJob* segs, const int* sblocks, int64_t nsblocks,
+                                          const double* bounds, unsigned int* d_maxbits, int* d_fixexp,
+                                          const float* y, hipStream_t s)
+    {
+        if (nsblocks > 0x7fffffff)
+            return hipErrorInvalidValue;
+        hipLaunchKernelGGL(absmax_batch_kernel, dim3((unsigned) nsblocks), dim3(256), 0, s, y, segs, sblocks,
+                           d_maxbits);
+        hipLaunchKernelGGL(scale_batch_kernel, dim3((unsigned) nitems), dim3(256), 0, s, d_maxbits, segs, bounds,
+                           d_fixexp);
+        return hipGetLastError();
+    }
+
     void joseph_adjoint_batch_jobs(const JosephBatchItem* items, int n, std::vector<SegJob>& segs,
                                    std::vector<int>& sblocks, std::vector<AdjJob> ajobs[3],
                                    std::vector<int> ablocks[3], size_t lds[3])
     {
-        segs.clear();
-        sblocks.clear();
+        batch_segments(items, n, true, segs, sblocks);
         for (int a = 0; a < 3; ++a) {
             ajobs[a].clear();
             ablocks[a].clear();
@@ -475,15 +508,6 @@ namespace ehp
         }
         for (int j = 0; j < n; ++j) {
             const JosephBatchItem& it = items[j];
-            SegJob sg{};
-            sg.off = it.sino_off;
-            sg.n = it.dims.nrays;
-            int64_t nb = (sg.n + 256 * 16 - 1) / (256 * 16);
-            nb = nb > 256 ? 256 : (nb < 1 ? 1 : nb);
-            sg.nblocks = (int) nb;
-            sg.block0 = (int64_t) sblocks.size();
-            sblocks.insert(sblocks.end(), (size_t) nb, j);
-            segs.push_back(sg);
             for (int a = 0; a < it.dims.dim; ++a) {
                 if (!(it.axes_present & (1 << a)))
                     continue;
@@ -514,15 +538,11 @@ namespace ehp
                                     unsigned int* d_maxbits, int* d_fixexp, const float* sinos, float* vols,
                                     hipStream_t s)
     {
-        if (nsblocks > 0x7fffffff)
-            return hipErrorInvalidValue;
         hipError_t e = launch_fill<float>(vols, nvol_total, 0.0f, s);
+        if (e == hipSuccess)
+            e = fixed_point_exponent_batch(nitems, segs, sblocks, nsblocks, bounds, d_maxbits, d_fixexp, sinos, s);
         if (e != hipSuccess)
             return e;
-        hipLaunchKernelGGL(absmax_batch_kernel, dim3((unsigned) nsblocks), dim3(256), 0, s, sinos, segs, sblocks,
-                           d_maxbits);
-        hipLaunchKernelGGL(scale_batch_kernel, dim3((unsigned) nitems), dim3(256), 0, s, d_maxbits, segs, bounds,
-                           d_fixexp);
         for (int a = 0; a < dim; ++a) {
             const int64_t total = nablocks[a];
             if (total == 0)

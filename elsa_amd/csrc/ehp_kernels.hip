@@ -306,39 +306,60 @@ namespace ehp
     template <int D, class T>
     __global__ void __launch_bounds__(256)
         voxel_fwd_kernel(const PoseGeom* __restrict__ geo, const Dims dims, const T* __restrict__ lut, int n_lut,
-                         float support, const int diff, const T* __restrict__ vol, T* __restrict__ sino,
-                         unsigned long long* count)
+                         float support, const int diff, const T* __restrict__ vol, T* __restrict__ sino)
     {
         const int p = blockIdx.y;
+        const int64_t vox = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+        if (vox >= dims.nvox)
+            return;
+        const int x = (int) (vox % dims.vol[0]);
+        const int y = (int) ((vox / dims.vol[0]) % dims.vol[1]);
+        const int z = (int) (vox / ((int64_t) dims.vol[0] * dims.vol[1]));
+        const T xv = vol[vox];
+        if (xv == T(0))
+            return;
+        Footprint fp;
+        footprint<D>(geo[p], dims, x, y, z, support, fp);
+        T* sp = sino + (int64_t) p * dims.pix_per_pose;
+        visit<D, T>(fp, dims, [&](int64_t pix, float dist, float d0) {
+            const T w = tap_weight<D, T>(lut, n_lut, support, dist, d0, diff != 0);
+            if (w != T(0))
+                atomic_add(sp + pix, xv * w);
+        });
+    }
+
+    // Footprint taps of the voxel projector (the roofline's unit): the clamped footprint
+    // rectangle of every voxel and pose, counted per voxel over all poses, reduced per block,
+    // one atomic per block.
+    template <int D>
+    __global__ void __launch_bounds__(256)
+        voxel_count_kernel(const PoseGeom* __restrict__ geo, const Dims dims, float support,
+                           unsigned long long* __restrict__ count)
+    {
         const int64_t vox = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
         unsigned long long taps = 0;
         if (vox < dims.nvox) {
             const int x = (int) (vox % dims.vol[0]);
             const int y = (int) ((vox / dims.vol[0]) % dims.vol[1]);
             const int z = (int) (vox / ((int64_t) dims.vol[0] * dims.vol[1]));
-            Footprint fp;
-            footprint<D>(geo[p], dims, x, y, z, support, fp);
-            if (count) {
+            for (int p = 0; p < dims.nposes; ++p) {
+                Footprint fp;
+                footprint<D>(geo[p], dims, x, y, z, support, fp);
                 const int nu = fp.hi[0] - fp.lo[0] + 1;
                 const int nv = fp.hi[1] - fp.lo[1] + 1;
-                taps = (nu > 0 && nv > 0) ? (unsigned long long) nu * nv : 0;
-            } else {
-                const T xv = vol[vox];
-                if (xv != T(0)) {
-                    T* sp = sino + (int64_t) p * dims.pix_per_pose;
-                    visit<D, T>(fp, dims, [&](int64_t pix, float dist, float d0) {
-                        const T w = tap_weight<D, T>(lut, n_lut, support, dist, d0, diff != 0);
-                        if (w != T(0))
-                            atomic_add(sp + pix, xv * w);
-                    });
-                }
+                taps += (nu > 0 && nv > 0) ? (unsigned long long) nu * nv : 0;
             }
         }
-        if (count) {
-            for (int off = 32; off > 0; off >>= 1)
-                taps += __shfl_down(taps, off, 64);
-            if ((threadIdx.x & 63) == 0 && taps)
-                atomicAdd(count, taps);
+        for (int off = 32; off > 0; off >>= 1)
+            taps += __shfl_down(taps, off, 64);
+        __shared__ unsigned long long part[4];
+        if ((threadIdx.x & 63) == 0)
+            part[threadIdx.x >> 6] = taps;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const unsigned long long t = part[0] + part[1] + part[2] + part[3];
+            if (t)
+                atomicAdd(count, t);
         }
     }
 
@@ -435,10 +456,10 @@ namespace ehp
             return e;
         if (dims.dim == 2)
             hipLaunchKernelGGL((voxel_fwd_kernel<2, T>), vox_pose_grid(dims), dim3(256), 0, s, geo, dims, lut, n_lut,
-                               support, diff, vol, sino, nullptr);
+                               support, diff, vol, sino);
         else
             hipLaunchKernelGGL((voxel_fwd_kernel<3, T>), vox_pose_grid(dims), dim3(256), 0, s, geo, dims, lut, n_lut,
-                               support, diff, vol, sino, nullptr);
+                               support, diff, vol, sino);
         return hipGetLastError();
     }
 
@@ -459,12 +480,11 @@ namespace ehp
     hipError_t voxel_count(const PoseGeom* geo, const Dims& dims, float support, unsigned long long* total,
                            hipStream_t s)
     {
+        const dim3 grid((unsigned) ((dims.nvox + 255) / 256));
         if (dims.dim == 2)
-            hipLaunchKernelGGL((voxel_fwd_kernel<2, float>), vox_pose_grid(dims), dim3(256), 0, s, geo, dims,
-                               (const float*) nullptr, 0, support, 0, (const float*) nullptr, (float*) nullptr, total);
+            hipLaunchKernelGGL((voxel_count_kernel<2>), grid, dim3(256), 0, s, geo, dims, support, total);
         else
-            hipLaunchKernelGGL((voxel_fwd_kernel<3, float>), vox_pose_grid(dims), dim3(256), 0, s, geo, dims,
-                               (const float*) nullptr, 0, support, 0, (const float*) nullptr, (float*) nullptr, total);
+            hipLaunchKernelGGL((voxel_count_kernel<3>), grid, dim3(256), 0, s, geo, dims, support, total);
         return hipGetLastError();
     }
 

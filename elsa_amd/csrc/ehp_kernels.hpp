@@ -24,10 +24,11 @@ namespace ehp
     // b - job.block0 of job block_job[b], exactly as that operator's own launch would. Volumes
     // and sinograms of the batch are packed back to back (offsets in elements).
 
-    // what the batch needs of one operator
+    // what the batch needs of one operator (Joseph: table; Siddon: sidtable)
     struct JosephBatchItem {
         const PoseGeom* geo;
         const RayRec* table;
+        const SidRec* sidtable;
         const int* pose_axes;
         float* views[3];
         PadLayout layouts[3];
@@ -88,6 +89,51 @@ namespace ehp
                                     const int64_t nablocks[3], const size_t lds[3], int64_t nvol_total,
                                     unsigned int* d_maxbits, int* d_fixexp, const float* sinos, float* vols,
                                     hipStream_t s);
+
+    // the view copies of a batched forward (one launch) and the fixed-point exponents of a batched
+    // back projection (absmax over every operator's segment, then one exponent per operator)
+    int64_t view_copy_blocks(const Dims& dims, const PadLayout& L, int& gx, int& gy);
+    hipError_t view_copy_batch(const CopyJob* cjobs, const int* cblocks, int64_t ncblocks, const float* vols,
+                               hipStream_t s);
+    hipError_t fixed_point_exponent_batch(int nitems, const SegJob* segs, const int* sblocks, int64_t nsblocks,
+                                          const double* bounds, unsigned int* d_maxbits, int* d_fixexp,
+                                          const float* y, hipStream_t s);
+    void batch_segments(const JosephBatchItem* items, int n, bool sinogram, std::vector<SegJob>& segs,
+                        std::vector<int>& sblocks);
+
+    // ---- batches of Siddon operators (FP32 fast kernels), on the same scheme
+    struct SidFwdJob { // the forward of one operator
+        const SidRec* table;
+        ViewSet views;
+        Dims dims;
+        int tiles_u;
+        int64_t sino_off;
+        int64_t block0;
+    };
+
+    struct SidAdjJob { // the tiles of one operator along one driving axis
+        const PoseGeom* geo;
+        const SidRec* table;
+        const int* pose_axes;
+        Dims dims;
+        TileCfg tc;
+        int ntA, ntB;
+        int batch;
+        int item;
+        int64_t sino_off, vol_off;
+        int64_t block0;
+    };
+
+    void siddon_forward_batch_jobs(const JosephBatchItem* items, int n, std::vector<CopyJob>& cjobs,
+                                   std::vector<int>& cblocks, std::vector<SidFwdJob>& fjobs, std::vector<int>& fblocks);
+    hipError_t siddon_forward_batch(int dim, const CopyJob* cjobs, const int* cblocks, int64_t ncblocks,
+                                    const SidFwdJob* fjobs, const int* fblocks, int64_t nfblocks, const float* vols,
+                                    float* sinos, hipStream_t s);
+    void siddon_adjoint_batch_jobs(const JosephBatchItem* items, int n, std::vector<SidAdjJob> ajobs[3],
+                                   std::vector<int> ablocks[3], size_t lds[3]);
+    hipError_t siddon_adjoint_batch(int dim, const SidAdjJob* const ajobs[3], const int* const ablocks[3],
+                                    const int64_t nablocks[3], const size_t lds[3], const int* d_fixexp,
+                                    const float* sinos, float* vols, hipStream_t s);
 
     template <class T>
     hipError_t launch_fill(T* p, int64_t n, T value, hipStream_t s);

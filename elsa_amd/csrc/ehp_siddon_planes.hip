@@ -61,15 +61,13 @@ namespace ehp
                                                       : sid_planes<D, A, NEG, false>(r, dims, vs.v[A], t0);
         }
 
+        // Block id of the forward's tiles (fwd_pixel's order): the rays of its wavefronts, grouped
+        // by driving axis and direction, each group walking its planes together.
         template <int D>
-        __global__ void __launch_bounds__(kFwdThreads)
-            siddon_fwd_planes_kernel(const SidRec* __restrict__ table, const Dims dims, const ViewSet views,
-                                     const int tiles_u, const int per_xcd, const int64_t total,
-                                     float* __restrict__ sino)
+        __device__ __forceinline__ void sid_fwd_block(const SidRec* __restrict__ table, const Dims& dims,
+                                                      const ViewSet& views, const int tiles_u, const int64_t id,
+                                                      float* __restrict__ sino)
         {
-            const int64_t id = (int64_t) (blockIdx.x % 8u) * per_xcd + (blockIdx.x / 8u);
-            if (id >= total)
-                return;
             int u, v, pose;
             fwd_pixel<D>(dims, tiles_u, id, u, v, pose);
             const bool inside = u < dims.det[0] && v < dims.det[1];
@@ -97,6 +95,32 @@ namespace ehp
             }
             if (inside)
                 sino[ray] = val;
+        }
+
+        template <int D>
+        __global__ void __launch_bounds__(kFwdThreads)
+            siddon_fwd_planes_kernel(const SidRec* __restrict__ table, const Dims dims, const ViewSet views,
+                                     const int tiles_u, const int per_xcd, const int64_t total,
+                                     float* __restrict__ sino)
+        {
+            const int64_t id = (int64_t) (blockIdx.x % 8u) * per_xcd + (blockIdx.x / 8u);
+            if (id >= total)
+                return;
+            sid_fwd_block<D>(table, dims, views, tiles_u, id, sino);
+        }
+
+        // the forwards of many operators in one launch (block b: block b - job.block0 of job
+        // block_job[b], as that operator's own launch would run it)
+        template <int D>
+        __global__ void __launch_bounds__(kFwdThreads)
+            siddon_fwd_planes_batch_kernel(const SidFwdJob* __restrict__ jobs, const int* __restrict__ block_job,
+                                           const int per_xcd, const int64_t total, float* __restrict__ sinos)
+        {
+            const int64_t id = (int64_t) (blockIdx.x % 8u) * per_xcd + (blockIdx.x / 8u);
+            if (id >= total)
+                return;
+            const SidFwdJob& J = jobs[block_job[id]];
+            sid_fwd_block<D>(J.table, J.dims, J.views, J.tiles_u, id - J.block0, sinos + J.sino_off);
         }
     } // namespace
 
@@ -129,6 +153,69 @@ namespace ehp
         else
             hipLaunchKernelGGL((siddon_fwd_planes_kernel<3>), grid, dim3(kFwdThreads), 0, s, table, dims, vs,
                                fg.tiles_u, (int) fg.per_xcd, fg.total, sino);
+        return hipGetLastError();
+    }
+
+    // Jobs of a batched Siddon forward: the view copies of every operator (one job per view)
+    // and the forward blocks (one job per operator).
+    void siddon_forward_batch_jobs(const JosephBatchItem* items, int n, std::vector<CopyJob>& cjobs,
+                                   std::vector<int>& cblocks, std::vector<SidFwdJob>& fjobs, std::vector<int>& fblocks)
+    {
+        cjobs.clear();
+        cblocks.clear();
+        fjobs.clear();
+        fblocks.clear();
+        for (int j = 0; j < n; ++j) {
+            const JosephBatchItem& it = items[j];
+            SidFwdJob fj{};
+            fj.table = it.sidtable;
+            fj.dims = it.dims;
+            fj.sino_off = it.sino_off;
+            for (int a = 0; a < it.dims.dim; ++a) {
+                if (!(it.axes_present & (1 << a)))
+                    continue;
+                const PadLayout& L = it.layouts[a];
+                CopyJob cj{};
+                cj.view = it.views[a];
+                cj.dims = it.dims;
+                cj.L = L;
+                cj.vol_off = it.vol_off;
+                cj.block0 = (int64_t) cblocks.size();
+                const int64_t nb = view_copy_blocks(it.dims, L, cj.gx, cj.gy);
+                cblocks.insert(cblocks.end(), (size_t) nb, (int) cjobs.size());
+                cjobs.push_back(cj);
+                fj.views.v[a] = PadView{it.views[a], L.PS * 4u, L.RS * 4u, L.size};
+                const int b2 = it.dims.dim == 3 ? (a == 2 ? 1 : 2) : -1;
+                if (L.f == b2)
+                    fj.views.fb2 |= 1 << a;
+            }
+            const FwdGrid fg = fwd_grid(it.dims);
+            fj.tiles_u = fg.tiles_u;
+            fj.block0 = (int64_t) fblocks.size();
+            fblocks.insert(fblocks.end(), (size_t) fg.total, (int) fjobs.size());
+            fjobs.push_back(fj);
+        }
+    }
+
+    hipError_t siddon_forward_batch(int dim, const CopyJob* cjobs, const int* cblocks, int64_t ncblocks,
+                                    const SidFwdJob* fjobs, const int* fblocks, int64_t nfblocks, const float* vols,
+                                    float* sinos, hipStream_t s)
+    {
+        hipError_t e = view_copy_batch(cjobs, cblocks, ncblocks, vols, s);
+        if (e != hipSuccess)
+            return e;
+        const int64_t per_xcd = (nfblocks + 7) / 8;
+        if (per_xcd * 8 > (int64_t) 0x7fffffff)
+            return hipErrorInvalidValue;
+        if (nfblocks > 0) {
+            const dim3 grid((unsigned) (per_xcd * 8));
+            if (dim == 2)
+                hipLaunchKernelGGL((siddon_fwd_planes_batch_kernel<2>), grid, dim3(kFwdThreads), 0, s, fjobs, fblocks,
+                                   (int) per_xcd, nfblocks, sinos);
+            else
+                hipLaunchKernelGGL((siddon_fwd_planes_batch_kernel<3>), grid, dim3(kFwdThreads), 0, s, fjobs, fblocks,
+                                   (int) per_xcd, nfblocks, sinos);
+        }
         return hipGetLastError();
     }
 } // namespace ehp

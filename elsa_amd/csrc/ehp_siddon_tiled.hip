@@ -65,21 +65,18 @@ namespace ehp
         return (size_t) tc.LA * tc.TB * (dims.dim == 3 ? tc.TC : 1) * sizeof(long long) + batch_lds_bytes(batch);
     }
 
+    // One tile of the back projection along driving axis A (tile < ntA * ntB * ntC).
     template <int D, int A>
-    __global__ void __launch_bounds__(kThreads)
-        siddon_adj_tiled_kernel(const PoseGeom* __restrict__ geo, const SidRec* __restrict__ table, const Dims dims,
-                                const int* __restrict__ pose_axes, const TileCfg tc, const int ntA, const int ntB,
-                                const int ntC, const int tiles_per_xcd, const int batch,
-                                const int* __restrict__ fixexp, const float* __restrict__ sino,
-                                float* __restrict__ vol)
+    __device__ __forceinline__ void sid_adj_tiled_block(const PoseGeom* __restrict__ geo,
+                                                        const SidRec* __restrict__ table, const Dims& dims,
+                                                        const int* __restrict__ pose_axes, const TileCfg& tc,
+                                                        const int ntA, const int ntB, const int tile, const int batch,
+                                                        const int* __restrict__ fixexp,
+                                                        const float* __restrict__ sino, float* __restrict__ vol)
     {
         constexpr int b1 = TileAxes<D, A>::b1, b2 = TileAxes<D, A>::b2;
         extern __shared__ long long lds[];
 
-        // consecutive workgroups land on different XCDs: give each XCD a contiguous block of tiles
-        const int tile = (int) (blockIdx.x % 8u) * tiles_per_xcd + (int) (blockIdx.x / 8u);
-        if (tile >= ntA * ntB * ntC)
-            return;
         const int tid = threadIdx.x;
         const TileBox t = make_tile<D, A>(dims, tc, tile % ntA, (tile / ntA) % ntB, tile / (ntA * ntB));
         const int nslots = tc.LA * tc.TB * (D == 3 ? tc.TC : 1);
@@ -196,6 +193,37 @@ namespace ehp
         }
     }
 
+    template <int D, int A>
+    __global__ void __launch_bounds__(kThreads)
+        siddon_adj_tiled_kernel(const PoseGeom* __restrict__ geo, const SidRec* __restrict__ table, const Dims dims,
+                                const int* __restrict__ pose_axes, const TileCfg tc, const int ntA, const int ntB,
+                                const int ntC, const int tiles_per_xcd, const int batch,
+                                const int* __restrict__ fixexp, const float* __restrict__ sino,
+                                float* __restrict__ vol)
+    {
+        // consecutive workgroups land on different XCDs: give each XCD a contiguous block of tiles
+        const int tile = (int) (blockIdx.x % 8u) * tiles_per_xcd + (int) (blockIdx.x / 8u);
+        if (tile >= ntA * ntB * ntC)
+            return;
+        sid_adj_tiled_block<D, A>(geo, table, dims, pose_axes, tc, ntA, ntB, tile, batch, fixexp, sino, vol);
+    }
+
+    // the tiles of axis A of many operators in one launch (block b: tile b - job.block0 of job
+    // block_job[b])
+    template <int D, int A>
+    __global__ void __launch_bounds__(kThreads)
+        siddon_adj_tiled_batch_kernel(const SidAdjJob* __restrict__ jobs, const int* __restrict__ block_job,
+                                      const int per_xcd, const int64_t total, const int* __restrict__ fixexp,
+                                      const float* __restrict__ sinos, float* __restrict__ vols)
+    {
+        const int64_t id = (int64_t) (blockIdx.x % 8u) * per_xcd + (blockIdx.x / 8u);
+        if (id >= total)
+            return;
+        const SidAdjJob& J = jobs[block_job[id]];
+        sid_adj_tiled_block<D, A>(J.geo, J.table, J.dims, J.pose_axes, J.tc, J.ntA, J.ntB, (int) (id - J.block0),
+                                  J.batch, fixexp + J.item, sinos + J.sino_off, vols + J.vol_off);
+    }
+
     hipError_t siddon_adjoint_tiled(const PoseGeom* geo, const SidRec* table, const Dims& dims, const int* pose_axes,
                                     int axes_present, const TileCfg& tc, int batch, double slot_bound,
                                     unsigned int* d_maxbits, int* d_fixexp, const float* sino, float* vol,
@@ -234,6 +262,80 @@ namespace ehp
             }
 #undef EHP_SID_TILED
             e = hipGetLastError();
+            if (e != hipSuccess)
+                return e;
+        }
+        return hipSuccess;
+    }
+
+    // The tiles of every operator per driving axis (one job per operator that has rays of that
+    // axis).
+    void siddon_adjoint_batch_jobs(const JosephBatchItem* items, int n, std::vector<SidAdjJob> ajobs[3],
+                                   std::vector<int> ablocks[3], size_t lds[3])
+    {
+        for (int a = 0; a < 3; ++a) {
+            ajobs[a].clear();
+            ablocks[a].clear();
+            lds[a] = 0;
+        }
+        for (int j = 0; j < n; ++j) {
+            const JosephBatchItem& it = items[j];
+            for (int a = 0; a < it.dims.dim; ++a) {
+                if (!(it.axes_present & (1 << a)))
+                    continue;
+                SidAdjJob aj{};
+                aj.geo = it.geo;
+                aj.table = it.sidtable;
+                aj.pose_axes = it.pose_axes;
+                aj.dims = it.dims;
+                aj.tc = it.tile;
+                aj.batch = it.batch;
+                aj.item = j;
+                aj.sino_off = it.sino_off;
+                aj.vol_off = it.vol_off;
+                int ntC;
+                tile_counts(it.dims, it.tile, a, aj.ntA, aj.ntB, ntC);
+                aj.block0 = (int64_t) ablocks[a].size();
+                ablocks[a].insert(ablocks[a].end(), (size_t) aj.ntA * aj.ntB * ntC, (int) ajobs[a].size());
+                ajobs[a].push_back(aj);
+                const size_t need = siddon_tile_lds_bytes(it.dims, it.tile, it.batch);
+                lds[a] = need > lds[a] ? need : lds[a];
+            }
+        }
+    }
+
+    // The tile launches of a batched back projection (the volumes already cleared and the
+    // exponents d_fixexp[item] computed by fixed_point_exponent_batch).
+    hipError_t siddon_adjoint_batch(int dim, const SidAdjJob* const ajobs[3], const int* const ablocks[3],
+                                    const int64_t nablocks[3], const size_t lds[3], const int* d_fixexp,
+                                    const float* sinos, float* vols, hipStream_t s)
+    {
+        for (int a = 0; a < dim; ++a) {
+            const int64_t total = nablocks[a];
+            if (total == 0)
+                continue;
+            const int64_t per_xcd = (total + 7) / 8;
+            if (per_xcd * 8 > (int64_t) 0x7fffffff)
+                return hipErrorInvalidValue;
+            const dim3 grid((unsigned) (per_xcd * 8));
+#define EHP_SID_BATCH(Dv, Av)                                                                                      \
+    hipLaunchKernelGGL((siddon_adj_tiled_batch_kernel<Dv, Av>), grid, dim3(kThreads), lds[a], s, ajobs[a], ablocks[a], \
+                       (int) per_xcd, total, d_fixexp, sinos, vols)
+            if (dim == 2) {
+                if (a == 0)
+                    EHP_SID_BATCH(2, 0);
+                else
+                    EHP_SID_BATCH(2, 1);
+            } else {
+                if (a == 0)
+                    EHP_SID_BATCH(3, 0);
+                else if (a == 1)
+                    EHP_SID_BATCH(3, 1);
+                else
+                    EHP_SID_BATCH(3, 2);
+            }
+#undef EHP_SID_BATCH
+            const hipError_t e = hipGetLastError();
             if (e != hipSuccess)
                 return e;
         }

@@ -1,6 +1,7 @@
 """Batches of independent problems (ProjectorBatch): every problem's forward, back
 projection and solver run equal what its own operator gives, bit for bit, whether the batch
-fuses the passes into single launches (FP32 Joseph) or calls the operators in turn."""
+fuses the passes into single launches (FP32 Joseph, FP32 Siddon) or calls the operators in
+turn."""
 import numpy as np
 import pytest
 
@@ -77,6 +78,38 @@ def test_fused_joseph_batch_equals_each_operator(E, dim, n):
     geoms = problems(dim, n, seed=dim)
     xs, ys = data(geoms, 7)
     check_passes(E, make_ops(E, geoms), xs, ys, fused=True)
+
+
+@pytest.mark.parametrize("dim,n", [(2, 12), (3, 5)])
+def test_fused_siddon_batch_equals_each_operator(E, dim, n):
+    geoms = problems(dim, n, seed=10 + dim)
+    xs, ys = data(geoms, 8)
+    check_passes(E, make_ops(E, geoms, "SiddonsMethodCUDA"), xs, ys, fused=True)
+
+
+def test_fused_siddon_batch_zero_nonfinite_and_cgls(E):
+    """Per-problem fixed-point scales in a fused Siddon batch (a zero and a NaN sinogram stay in
+    their own problems), and a batched CGLS equal to each problem's own, bit for bit."""
+    import torch
+
+    geoms = problems(2, 5, seed=13)
+    ops = make_ops(E, geoms, "SiddonsMethodCUDA")
+    xs, ys = data(geoms, 4)
+    ys[1][:] = 0.0
+    ys[2][0, 0] = np.nan
+    B = E.ProjectorBatch(ops)
+    assert B.fused
+    by = B.unpack(B.applyAdjoint(B.pack(ys, adjoint_space=True)))
+    assert np.all(by[1].cpu().numpy() == 0.0)
+    assert np.isnan(by[2].cpu().numpy()).all()
+    for j in (0, 3, 4):
+        np.testing.assert_array_equal(by[j].cpu().numpy(), ops[j].applyAdjoint(ys[j]))
+    bs = [op.apply(x) for op, x in zip(ops, phantoms(geoms, 5))]
+    cg = E.CGLS(B, B.pack(bs, adjoint_space=True), graph=True)
+    x = B.unpack(cg.solve(4))
+    torch.cuda.synchronize()
+    for j, op in enumerate(ops):
+        np.testing.assert_array_equal(x[j].cpu().numpy(), E.CGLS(op, bs[j], graph=True).solve(4))
 
 
 def test_fused_batch_with_zero_and_nonfinite_problems(E):
