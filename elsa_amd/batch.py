@@ -6,9 +6,9 @@ Each op is a projector of elsa_amd.projectors with its own volume, trajectory an
 (all of one dtype). The batch's volume is the problems' volumes flattened and packed back to
 back (problem j at vol_offsets[j]), its sinogram likewise; pack() / unpack() convert between
 the packed vector and per-problem arrays. apply / applyAdjoint give, for every problem, what
-the problem's own operator gives, bit for bit; for FP32 Joseph operators, or FP32 Siddon
-operators, with the default kernels every pass is one kernel launch over all problems per
-kernel. The solvers of elsa_amd.solvers
+the problem's own operator gives, bit for bit; for FP32 Joseph operators, FP32 Siddon
+operators or FP32 voxel projectors (one of these kinds per batch) with the default kernels
+every pass is one kernel launch over all problems per kernel. The solvers of elsa_amd.solvers
 accept a batch in place of an operator and then solve every problem independently (its own
 step lengths and convergence), with per-problem outputs as rows.
 

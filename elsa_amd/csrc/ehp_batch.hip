@@ -30,6 +30,11 @@ struct ehp_batch {
     // fused Siddon passes (device tables; the copy, segment and block tables are shared)
     ehp::SidFwdJob* d_sfjobs = nullptr;
     ehp::SidAdjJob* d_sajobs[3] = {nullptr, nullptr, nullptr};
+    // fused voxel passes (the forward's blocks in d_fblocks, its segments over the volumes)
+    ehp::VoxFwdJob* d_vfjobs = nullptr;
+    ehp::VoxAdjJob* d_vajobs = nullptr;
+    int* d_vablocks = nullptr;
+    int64_t nvablocks = 0;
     // fused Joseph passes (device tables)
     ehp::CopyJob* d_cjobs = nullptr;
     int* d_cblocks = nullptr;
@@ -84,8 +89,9 @@ namespace
 
     bool fusable(const ehp_op* op, int kind, int dim)
     {
-        return (op->kind == KIND_JOSEPH || op->kind == KIND_SIDDON) && op->kind == kind && op->dtype == EHP_F32 &&
-               op->planes && op->tiled && op->dims.dim == dim;
+        if (op->kind != kind || op->dtype != EHP_F32 || !op->tiled || op->dims.dim != dim)
+            return false;
+        return op->kind == KIND_VOXEL || op->planes;
     }
 
     int build_fused(ehp_batch* b)
@@ -120,10 +126,43 @@ namespace
         std::vector<ehp::AdjJob> aj[3];
         std::vector<ehp::SidAdjJob> saj[3];
         std::vector<int> ab[3];
-        const bool sid = b->kind == KIND_SIDDON;
-        if (sid) {
+        const bool sid = b->kind == KIND_SIDDON, vox = b->kind == KIND_VOXEL;
+        std::vector<ehp::VoxFwdJob> vfj;
+        std::vector<ehp::VoxAdjJob> vaj;
+        std::vector<int> vab;
+        if (vox) {
+            // the forward's fixed-point scale comes from each problem's volume
+            std::vector<ehp::VoxBatchItem> vitems(n);
+            std::vector<int64_t> offs(n), sizes(n);
+            for (int j = 0; j < n; ++j) {
+                const ehp_op* op = b->ops[j];
+                ehp::VoxBatchItem& v = vitems[j];
+                v.geo = op->d_geo;
+                v.dims = op->dims;
+                v.lut = (const float*) op->d_lut;
+                v.n_lut = op->n_lut;
+                v.support = op->support;
+                v.diff = op->differential;
+                v.pose_axes = op->d_pose_axes;
+                v.pose_margin = op->d_pose_margin;
+                v.tile = op->tile;
+                v.lutscale = op->lutscale;
+                v.vol_off = b->vol_off[j];
+                v.sino_off = b->sino_off[j];
+                offs[j] = b->vol_off[j];
+                sizes[j] = op->dims.nvox;
+            }
+            ehp::voxel_forward_batch_jobs(vitems.data(), n, vfj, fb, b->lds[0]);
+            ehp::batch_segments(offs.data(), sizes.data(), n, sg, sb);
+            ehp::voxel_adjoint_batch_jobs(vitems.data(), n, vaj, vab);
+        } else if (sid) {
+            std::vector<int64_t> offs(n), sizes(n);
+            for (int j = 0; j < n; ++j) {
+                offs[j] = b->sino_off[j];
+                sizes[j] = b->ops[j]->dims.nrays;
+            }
             ehp::siddon_forward_batch_jobs(items.data(), n, cj, cb, sfj, fb);
-            ehp::batch_segments(items.data(), n, true, sg, sb);
+            ehp::batch_segments(offs.data(), sizes.data(), n, sg, sb);
             ehp::siddon_adjoint_batch_jobs(items.data(), n, saj, ab, b->lds);
         } else {
             ehp::joseph_forward_batch_jobs(items.data(), n, cj, cb, fj, fb);
@@ -140,7 +179,13 @@ namespace
         if (e == hipSuccess)
             e = upload(b, cb, &b->d_cblocks);
         if (e == hipSuccess)
-            e = sid ? upload(b, sfj, &b->d_sfjobs) : upload(b, fj, &b->d_fjobs);
+            e = vox ? upload(b, vfj, &b->d_vfjobs) : (sid ? upload(b, sfj, &b->d_sfjobs) : upload(b, fj, &b->d_fjobs));
+        if (e == hipSuccess && vox) {
+            e = upload(b, vaj, &b->d_vajobs);
+            if (e == hipSuccess)
+                e = upload(b, vab, &b->d_vablocks);
+            b->nvablocks = (int64_t) vab.size();
+        }
         if (e == hipSuccess)
             e = upload(b, fb, &b->d_fblocks);
         if (e == hipSuccess)
@@ -223,7 +268,16 @@ namespace
         }
         if (e != hipSuccess)
             return bfail(EHP_ERR_HIP, "ordering after the previous calls: %s", hipGetErrorString(e));
-        if (!adjoint && b->kind == KIND_SIDDON) {
+        if (!adjoint && b->kind == KIND_VOXEL) {
+            e = ehp::fixed_point_exponent_batch((int) b->ops.size(), b->d_segs, b->d_sblocks, b->nsblocks,
+                                                b->d_bounds, b->d_maxbits, b->d_fixexp, (const float*) in, s);
+            if (e == hipSuccess)
+                e = ehp::voxel_forward_batch(b->dim, b->d_vfjobs, b->d_fblocks, b->nfblocks, b->lds[0], b->d_fixexp,
+                                             (const float*) in, (float*) out, s);
+        } else if (adjoint && b->kind == KIND_VOXEL) {
+            e = ehp::voxel_adjoint_batch(b->dim, b->d_vajobs, b->d_vablocks, b->nvablocks, (const float*) in,
+                                         (float*) out, s);
+        } else if (!adjoint && b->kind == KIND_SIDDON) {
             e = ehp::siddon_forward_batch(b->dim, b->d_cjobs, b->d_cblocks, b->ncblocks, b->d_sfjobs, b->d_fblocks,
                                           b->nfblocks, (const float*) in, (float*) out, s);
         } else if (!adjoint) {

@@ -462,18 +462,17 @@ namespace ehp
 
     // Segments of a batch's sinograms (absmax) and the tiles of every operator per driving
     // axis (one job per operator that has rays of that axis).
-    // One segment per operator of the packed sinograms (sinogram) or volumes, each reduced by
-    // up to 256 blocks of the batched absmax.
-    void batch_segments(const JosephBatchItem* items, int n, bool sinogram, std::vector<SegJob>& segs,
+    // One segment per operator (offset and size in a packed vector), each reduced by up to 256
+    // blocks of the batched absmax.
+    void batch_segments(const int64_t* offs, const int64_t* sizes, int n, std::vector<SegJob>& segs,
                         std::vector<int>& sblocks)
     {
         segs.clear();
         sblocks.clear();
         for (int j = 0; j < n; ++j) {
-            const JosephBatchItem& it = items[j];
             SegJob sg{};
-            sg.off = sinogram ? it.sino_off : it.vol_off;
-            sg.n = sinogram ? it.dims.nrays : it.dims.nvox;
+            sg.off = offs[j];
+            sg.n = sizes[j];
             int64_t nb = (sg.n + 256 * 16 - 1) / (256 * 16);
             nb = nb > 256 ? 256 : (nb < 1 ? 1 : nb);
             sg.nblocks = (int) nb;
@@ -500,7 +499,12 @@ namespace ehp
                                    std::vector<int>& sblocks, std::vector<AdjJob> ajobs[3],
                                    std::vector<int> ablocks[3], size_t lds[3])
     {
-        batch_segments(items, n, true, segs, sblocks);
+        std::vector<int64_t> offs(n), sizes(n);
+        for (int j = 0; j < n; ++j) {
+            offs[j] = items[j].sino_off;
+            sizes[j] = items[j].dims.nrays;
+        }
+        batch_segments(offs.data(), sizes.data(), n, segs, sblocks);
         for (int a = 0; a < 3; ++a) {
             ajobs[a].clear();
             ablocks[a].clear();

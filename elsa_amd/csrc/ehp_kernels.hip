@@ -252,12 +252,11 @@ namespace ehp
     // voxel and pose instead of a division per tap, the table index as dist * (n - 1) / support),
     // so forward and back projection weigh every tap alike. ST: float or bfloat16 storage.
     template <int D, class ST>
-    __global__ void __launch_bounds__(256)
-        voxel_adj_fast_kernel(const PoseGeom* __restrict__ geo, const Dims dims, const float* __restrict__ lut,
-                              int n_lut, float support, const int diff, const ST* __restrict__ sino,
-                              ST* __restrict__ vol)
+    __device__ __forceinline__ void vox_adj_fast(const PoseGeom* __restrict__ geo, const Dims& dims,
+                                                 const float* __restrict__ lut, int n_lut, float support,
+                                                 const int diff, const ST* __restrict__ sino, ST* __restrict__ vol,
+                                                 const int64_t vox)
     {
-        const int64_t vox = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
         if (vox >= dims.nvox)
             return;
         const int x = (int) (vox % dims.vol[0]);
@@ -281,6 +280,66 @@ namespace ehp
             }
         }
         vol[vox] = down<ST>(acc);
+    }
+
+    template <int D, class ST>
+    __global__ void __launch_bounds__(256)
+        voxel_adj_fast_kernel(const PoseGeom* __restrict__ geo, const Dims dims, const float* __restrict__ lut,
+                              int n_lut, float support, const int diff, const ST* __restrict__ sino,
+                              ST* __restrict__ vol)
+    {
+        vox_adj_fast<D, ST>(geo, dims, lut, n_lut, support, diff, sino, vol,
+                            (int64_t) blockIdx.x * blockDim.x + threadIdx.x);
+    }
+
+    // the gathers of many operators in one launch (block b: block b - job.block0 of job
+    // block_job[b])
+    template <int D>
+    __global__ void __launch_bounds__(256)
+        voxel_adj_fast_batch_kernel(const VoxAdjJob* __restrict__ jobs, const int* __restrict__ block_job,
+                                    const float* __restrict__ sinos, float* __restrict__ vols)
+    {
+        const VoxAdjJob& J = jobs[block_job[blockIdx.x]];
+        vox_adj_fast<D, float>(J.geo, J.dims, J.lut, J.n_lut, J.support, J.diff, sinos + J.sino_off,
+                               vols + J.vol_off, ((int64_t) blockIdx.x - J.block0) * blockDim.x + threadIdx.x);
+    }
+
+    void voxel_adjoint_batch_jobs(const VoxBatchItem* items, int n, std::vector<VoxAdjJob>& jobs,
+                                  std::vector<int>& blocks)
+    {
+        jobs.clear();
+        blocks.clear();
+        for (int j = 0; j < n; ++j) {
+            const VoxBatchItem& it = items[j];
+            VoxAdjJob J{};
+            J.geo = it.geo;
+            J.dims = it.dims;
+            J.lut = it.lut;
+            J.n_lut = it.n_lut;
+            J.support = it.support;
+            J.diff = it.diff;
+            J.vol_off = it.vol_off;
+            J.sino_off = it.sino_off;
+            J.block0 = (int64_t) blocks.size();
+            blocks.insert(blocks.end(), (size_t) ((it.dims.nvox + 255) / 256), (int) jobs.size());
+            jobs.push_back(J);
+        }
+    }
+
+    hipError_t voxel_adjoint_batch(int dim, const VoxAdjJob* jobs, const int* blocks, int64_t nblocks,
+                                   const float* sinos, float* vols, hipStream_t s)
+    {
+        if (nblocks > 0x7fffffff)
+            return hipErrorInvalidValue;
+        if (nblocks == 0)
+            return hipSuccess;
+        if (dim == 2)
+            hipLaunchKernelGGL((voxel_adj_fast_batch_kernel<2>), dim3((unsigned) nblocks), dim3(256), 0, s, jobs,
+                               blocks, sinos, vols);
+        else
+            hipLaunchKernelGGL((voxel_adj_fast_batch_kernel<3>), dim3((unsigned) nblocks), dim3(256), 0, s, jobs,
+                               blocks, sinos, vols);
+        return hipGetLastError();
     }
 
     template <class ST>

@@ -98,7 +98,7 @@ namespace ehp
     hipError_t fixed_point_exponent_batch(int nitems, const SegJob* segs, const int* sblocks, int64_t nsblocks,
                                           const double* bounds, unsigned int* d_maxbits, int* d_fixexp,
                                           const float* y, hipStream_t s);
-    void batch_segments(const JosephBatchItem* items, int n, bool sinogram, std::vector<SegJob>& segs,
+    void batch_segments(const int64_t* offs, const int64_t* sizes, int n, std::vector<SegJob>& segs,
                         std::vector<int>& sblocks);
 
     // ---- batches of Siddon operators (FP32 fast kernels), on the same scheme
@@ -134,6 +134,57 @@ namespace ehp
     hipError_t siddon_adjoint_batch(int dim, const SidAdjJob* const ajobs[3], const int* const ablocks[3],
                                     const int64_t nablocks[3], const size_t lds[3], const int* d_fixexp,
                                     const float* sinos, float* vols, hipStream_t s);
+
+    // ---- batches of voxel projectors (FP32 fast kernels): the tiled forward and the gather
+    struct VoxBatchItem {
+        const PoseGeom* geo;
+        Dims dims;
+        const float* lut;
+        int n_lut;
+        float support;
+        int diff;
+        const int* pose_axes;
+        const float* pose_margin;
+        TileCfg tile;
+        float lutscale;
+        int64_t vol_off, sino_off;
+    };
+
+    struct VoxFwdJob { // the tile-pose blocks of one operator's forward
+        const PoseGeom* geo;
+        Dims dims;
+        const float* lut;
+        int n_lut;
+        float support;
+        int diff;
+        const int* pose_axis;
+        const float* pose_margin;
+        int TU, TV, ntu;
+        float lutscale;
+        int item;
+        int64_t vol_off, sino_off;
+        int64_t block0;
+    };
+
+    struct VoxAdjJob { // the voxel blocks of one operator's gather
+        const PoseGeom* geo;
+        Dims dims;
+        const float* lut;
+        int n_lut;
+        float support;
+        int diff;
+        int64_t vol_off, sino_off;
+        int64_t block0;
+    };
+
+    void voxel_forward_batch_jobs(const VoxBatchItem* items, int n, std::vector<VoxFwdJob>& jobs,
+                                  std::vector<int>& blocks, size_t& lds);
+    hipError_t voxel_forward_batch(int dim, const VoxFwdJob* jobs, const int* blocks, int64_t nblocks, size_t lds,
+                                   const int* d_fixexp, const float* vols, float* sinos, hipStream_t s);
+    void voxel_adjoint_batch_jobs(const VoxBatchItem* items, int n, std::vector<VoxAdjJob>& jobs,
+                                  std::vector<int>& blocks);
+    hipError_t voxel_adjoint_batch(int dim, const VoxAdjJob* jobs, const int* blocks, int64_t nblocks,
+                                   const float* sinos, float* vols, hipStream_t s);
 
     template <class T>
     hipError_t launch_fill(T* p, int64_t n, T value, hipStream_t s);

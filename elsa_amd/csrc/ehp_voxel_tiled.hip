@@ -26,19 +26,19 @@ namespace ehp
         constexpr int kMaxLut = 1024;
     } // namespace
 
+    // Tile-pose id (tile * nposes + pose) of one operator's forward.
     template <int D, class ST>
-    __global__ void __launch_bounds__(kThreads)
-        voxel_fwd_tiled_kernel(const PoseGeom* __restrict__ geo, const Dims dims, const float* __restrict__ lut,
-                               const int n_lut, const float support, const int diff, const int* __restrict__ pose_axis,
-                               const float* __restrict__ pose_margin, const int TU, const int TV, const int ntu,
-                               const int per_xcd, const int64_t total, const float lutscale,
-                               const int* __restrict__ fixexp, const ST* __restrict__ vol, ST* __restrict__ sino)
+    __device__ __forceinline__ void vox_fwd_tiled_block(const PoseGeom* __restrict__ geo, const Dims& dims,
+                                                        const float* __restrict__ lut, const int n_lut,
+                                                        const float support, const int diff,
+                                                        const int* __restrict__ pose_axis,
+                                                        const float* __restrict__ pose_margin, const int TU,
+                                                        const int TV, const int ntu, const int64_t id,
+                                                        const float lutscale, const int* __restrict__ fixexp,
+                                                        const ST* __restrict__ vol, ST* __restrict__ sino)
     {
         extern __shared__ long long lds[];
         __shared__ float s_lut[kMaxLut];
-        const int64_t id = (int64_t) (blockIdx.x % 8u) * per_xcd + (blockIdx.x / 8u);
-        if (id >= total)
-            return;
         const int tid = threadIdx.x;
         // id = tile * nposes + pose
         const int pose = (int) (id % dims.nposes);
@@ -119,6 +119,38 @@ namespace ehp
         }
     }
 
+    template <int D, class ST>
+    __global__ void __launch_bounds__(kThreads)
+        voxel_fwd_tiled_kernel(const PoseGeom* __restrict__ geo, const Dims dims, const float* __restrict__ lut,
+                               const int n_lut, const float support, const int diff, const int* __restrict__ pose_axis,
+                               const float* __restrict__ pose_margin, const int TU, const int TV, const int ntu,
+                               const int per_xcd, const int64_t total, const float lutscale,
+                               const int* __restrict__ fixexp, const ST* __restrict__ vol, ST* __restrict__ sino)
+    {
+        const int64_t id = (int64_t) (blockIdx.x % 8u) * per_xcd + (blockIdx.x / 8u);
+        if (id >= total)
+            return;
+        vox_fwd_tiled_block<D, ST>(geo, dims, lut, n_lut, support, diff, pose_axis, pose_margin, TU, TV, ntu, id,
+                                   lutscale, fixexp, vol, sino);
+    }
+
+    // the forwards of many operators in one launch (block b: tile-pose b - job.block0 of job
+    // block_job[b], as that operator's own launch would run it)
+    template <int D>
+    __global__ void __launch_bounds__(kThreads)
+        voxel_fwd_tiled_batch_kernel(const VoxFwdJob* __restrict__ jobs, const int* __restrict__ block_job,
+                                     const int per_xcd, const int64_t total, const int* __restrict__ fixexp,
+                                     const float* __restrict__ vols, float* __restrict__ sinos)
+    {
+        const int64_t id = (int64_t) (blockIdx.x % 8u) * per_xcd + (blockIdx.x / 8u);
+        if (id >= total)
+            return;
+        const VoxFwdJob& J = jobs[block_job[id]];
+        vox_fwd_tiled_block<D, float>(J.geo, J.dims, J.lut, J.n_lut, J.support, J.diff, J.pose_axis, J.pose_margin,
+                                      J.TU, J.TV, J.ntu, id - J.block0, J.lutscale, fixexp + J.item,
+                                      vols + J.vol_off, sinos + J.sino_off);
+    }
+
     // dynamic LDS (the tile's sums) plus the table
     size_t voxel_tile_lds_bytes(const TileCfg& tc)
     {
@@ -167,4 +199,59 @@ namespace ehp
     template hipError_t voxel_forward_tiled<bf16>(const PoseGeom*, const Dims&, const float*, int, float, int,
                                                   const int*, const float*, const TileCfg&, float, double,
                                                   unsigned int*, int*, const bf16*, bf16*, hipStream_t);
+
+    // Jobs of a batched voxel forward: one per operator, its tile-pose blocks back to back; the
+    // launch's dynamic LDS is the largest tile's.
+    void voxel_forward_batch_jobs(const VoxBatchItem* items, int n, std::vector<VoxFwdJob>& jobs,
+                                  std::vector<int>& blocks, size_t& lds)
+    {
+        jobs.clear();
+        blocks.clear();
+        lds = 0;
+        for (int j = 0; j < n; ++j) {
+            const VoxBatchItem& it = items[j];
+            VoxFwdJob J{};
+            J.geo = it.geo;
+            J.dims = it.dims;
+            J.lut = it.lut;
+            J.n_lut = it.n_lut;
+            J.support = it.support;
+            J.diff = it.diff;
+            J.pose_axis = it.pose_axes;
+            J.pose_margin = it.pose_margin;
+            J.TU = it.tile.LA;
+            J.TV = it.dims.dim == 3 ? it.tile.TB : 1;
+            J.ntu = (it.dims.det[0] + J.TU - 1) / J.TU;
+            const int ntv = (it.dims.det[1] + J.TV - 1) / J.TV;
+            J.lutscale = it.lutscale;
+            J.item = j;
+            J.vol_off = it.vol_off;
+            J.sino_off = it.sino_off;
+            J.block0 = (int64_t) blocks.size();
+            blocks.insert(blocks.end(), (size_t) J.ntu * ntv * it.dims.nposes, (int) jobs.size());
+            jobs.push_back(J);
+            const size_t need = (size_t) J.TU * J.TV * sizeof(long long);
+            lds = need > lds ? need : lds;
+        }
+    }
+
+    // The tile launch of a batched voxel forward (the exponents d_fixexp[item] computed from
+    // each operator's volume by fixed_point_exponent_batch).
+    hipError_t voxel_forward_batch(int dim, const VoxFwdJob* jobs, const int* blocks, int64_t nblocks, size_t lds,
+                                   const int* d_fixexp, const float* vols, float* sinos, hipStream_t s)
+    {
+        const int64_t per_xcd = (nblocks + 7) / 8;
+        if (per_xcd * 8 > (int64_t) 0x7fffffff)
+            return hipErrorInvalidValue;
+        if (nblocks == 0)
+            return hipSuccess;
+        const dim3 grid((unsigned) (per_xcd * 8));
+        if (dim == 2)
+            hipLaunchKernelGGL((voxel_fwd_tiled_batch_kernel<2>), grid, dim3(kThreads), lds, s, jobs, blocks,
+                               (int) per_xcd, nblocks, d_fixexp, vols, sinos);
+        else
+            hipLaunchKernelGGL((voxel_fwd_tiled_batch_kernel<3>), grid, dim3(kThreads), lds, s, jobs, blocks,
+                               (int) per_xcd, nblocks, d_fixexp, vols, sinos);
+        return hipGetLastError();
+    }
 } // namespace ehp

@@ -131,6 +131,23 @@ class _HipProjector(LinearOperator):
     def handle(self) -> ctypes.c_void_p:
         return self._op
 
+    def _ctor_kwargs(self):
+        return {}
+
+    def clone(self):
+        """A new operator on the same descriptors with the same settings (elsa:
+        LinearOperator::clone / cloneImpl): its own device state, the same results bit for bit;
+        the tile and poses per step of its tiled kernels are copied."""
+        dt = BF16 if self._dtype == BF16 else self._dtype
+        c = type(self)(self._domain, self._range, dtype=dt, **self._ctor_kwargs())
+        for knob in ("tile", "pose_batch"):
+            try:
+                cur = getattr(self, knob)()
+            except ValueError:
+                continue  # no tiled kernel of that kind
+            getattr(c, knob)(*cur) if knob == "tile" else c.pose_batch(cur)
+        return c
+
     def tile(self, la: int = 0, tb: int = 0, tc: int = 0):
         """Set (la > 0) and return the tile of the LDS-tiled kernel (FP32): (LA, TB, TC) of the
         Joseph / Siddon back projection, (width, height, -) in pixels of the voxel forward."""
@@ -207,6 +224,9 @@ class JosephsMethodCUDA(_HipProjector):
         self.kernels = kernels
         super().__init__(domainDescriptor, rangeDescriptor, dtype)
 
+    def _ctor_kwargs(self):
+        return {"fast": self.fast, "kernels": self.kernels}
+
     def _create(self, geo):
         flags = _lib.EHP_SIMPLE_KERNELS if self.kernels == "simple" else 0
         _lib.check(self._L.ehp_joseph_create(geo, _DT[self._dtype], flags, ctypes.byref(self._op)),
@@ -233,6 +253,9 @@ class SiddonsMethodCUDA(_HipProjector):
             raise ValueError("kernels must be 'fast' or 'simple'")
         self.kernels = kernels
         super().__init__(domainDescriptor, rangeDescriptor, dtype)
+
+    def _ctor_kwargs(self):
+        return {"kernels": self.kernels}
 
     def _create(self, geo):
         flags = _lib.EHP_SIMPLE_KERNELS if self.kernels == "simple" else 0
@@ -298,6 +321,12 @@ class _VoxelProjector(_HipProjector):
         self.kernels = kernels
 
     _differential = False
+
+    def _ctor_kwargs(self):
+        kw = {"order": self.order, "kernels": self.kernels}
+        if hasattr(self, "radius"):
+            kw.update(radius=self.radius, alpha=self.alpha)
+        return kw
 
     def _create(self, geo):
         flags = _lib.EHP_SIMPLE_KERNELS if self.kernels == "simple" else 0

@@ -244,9 +244,10 @@ int ehp_power_iterations(ehp_op* op, int niters, double* L, void* stream);
  * batch gives exactly what ehp_forward / ehp_adjoint of A_j give (bit for bit), and its
  * solvers run elsa's iteration on every problem independently, each with its own scalars and
  * its own convergence test. When all operators are FP32 Joseph operators, or all are FP32
- * Siddon operators, of one dimension with the default (fast) kernels, every projector pass is
- * one launch over all problems per kernel (plus the view copies, the zero fill and the
- * fixed-point scale of the back projection); otherwise the
+ * Siddon operators, or all are FP32 voxel projectors (blob, B-spline, phase contrast), of one
+ * dimension with the default (fast) kernels, every projector pass is one launch over all
+ * problems per kernel (plus the view copies, the zero fill and the fixed-point scales);
+ * otherwise the
  * batch calls the operators one after another. The operators must outlive the batch; tile
  * shapes are taken at creation; a batch and its operators share device state, and the library
  * orders calls on either (see Conventions). */

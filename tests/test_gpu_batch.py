@@ -1,7 +1,7 @@
 """Batches of independent problems (ProjectorBatch): every problem's forward, back
 projection and solver run equal what its own operator gives, bit for bit, whether the batch
-fuses the passes into single launches (FP32 Joseph, FP32 Siddon) or calls the operators in
-turn."""
+fuses the passes into single launches (FP32 Joseph, FP32 Siddon, FP32 voxel projectors) or
+calls the operators in turn."""
 import numpy as np
 import pytest
 
@@ -87,6 +87,19 @@ def test_fused_siddon_batch_equals_each_operator(E, dim, n):
     check_passes(E, make_ops(E, geoms, "SiddonsMethodCUDA"), xs, ys, fused=True)
 
 
+@pytest.mark.parametrize("dim,n", [(2, 9), (3, 4)])
+def test_fused_voxel_batch_equals_each_operator(E, dim, n):
+    """Blob, B-spline and phase-contrast blob operators in one batch: the tiled forward and the
+    gather run as one launch each over all problems (the forward's fixed-point scale per
+    problem, from its own volume), bit for bit each operator's own result."""
+    geoms = problems(dim, n, seed=20 + dim)
+    kinds = ["BlobVoxelProjectorCUDA", "BSplineVoxelProjectorCUDA", "PhaseContrastBlobVoxelProjectorCUDA"]
+    ops = [getattr(E, kinds[j % 3])(*geom_to_descriptors(g)) for j, g in enumerate(geoms)]
+    xs, ys = data(geoms, 9)
+    xs[1][:] = 0.0
+    check_passes(E, ops, xs, ys, fused=True)
+
+
 def test_fused_siddon_batch_zero_nonfinite_and_cgls(E):
     """Per-problem fixed-point scales in a fused Siddon batch (a zero and a NaN sinogram stay in
     their own problems), and a batched CGLS equal to each problem's own, bit for bit."""
@@ -150,6 +163,21 @@ def test_unfused_batches_equal_each_operator(E, kind):
         xs, ys = data(geoms, 1)
     rtol = {"fp64": 1e-13, "simple": 1e-6}.get(kind, 0.0)
     check_passes(E, ops, xs, ys, fused=False, adj_rtol=rtol)
+
+
+@pytest.mark.parametrize("cls", ["JosephsMethodCUDA", "SiddonsMethodCUDA", "BlobVoxelProjectorCUDA",
+                                 "PhaseContrastBSplineVoxelProjectorCUDA"])
+def test_clone_equals_the_original(E, cls):
+    """clone() (elsa's LinearOperator::clone) makes a new operator with its own device state and
+    the same results bit for bit; its tiled kernels take the original's tile."""
+    g = problems(3, 1, seed=41)[0]
+    op = getattr(E, cls)(*geom_to_descriptors(g))
+    c = op.clone()
+    assert c.handle.value != op.handle.value and type(c) is type(op)
+    xs, ys = data([g], 2)
+    np.testing.assert_array_equal(c.apply(xs[0]), op.apply(xs[0]))
+    np.testing.assert_array_equal(c.applyAdjoint(ys[0]), op.applyAdjoint(ys[0]))
+    assert c.tile() == op.tile()
 
 
 def test_batch_rejects_bad_operator_lists(E):
