@@ -111,10 +111,15 @@ namespace
                 return fail(EHP_ERR_ALLOC, "detector coordinates: %s", hipGetErrorString(e));
             }
             op->dims.pix = op->d_pix;
+            op->host_pix = g->pix;
+            op->curved_tiles = ehp::curved_candidates(op->dims, g->pix.data());
         }
         *out = op;
         return EHP_OK;
     }
+
+    // host copy of a curved detector's planar centres (nullptr for a planar detector)
+    const float* curved_pix(const ehp_op* op) { return op->host_pix.empty() ? nullptr : op->host_pix.data(); }
 
     // Sample count of one pass, computed once on the device.
     int count_samples(ehp_op* op)
@@ -393,10 +398,10 @@ namespace
             return hip_fail(e, "volume view");
 
         // back projection: the bound keeps the 64-bit fixed-point sums in range; without it,
-        // and for curved detectors (the tiles find their rays through planar pixel centres),
-        // the one-thread-per-ray kernel runs
-        const double bound = ehp::fixed_point_slot_bound(dm, op->host_geo.data());
-        if (!(bound > 0.0) || bound > 1.0e18 || op->d_pix) {
+        // and for curved detectors whose planar centres lack the structure the tiles' candidate
+        // search needs, the one-thread-per-ray kernel runs
+        const double bound = ehp::fixed_point_slot_bound(dm, op->host_geo.data(), curved_pix(op));
+        if (!(bound > 0.0) || bound > 1.0e18 || (op->d_pix && !op->curved_tiles) || !ehp::tiled_detector_fits(dm)) {
             (void) hipFree(op->d_table);
             op->d_table = nullptr;
             return EHP_OK;
@@ -416,9 +421,9 @@ namespace
     int setup_siddon_tiled(ehp_op* op)
     {
         const ehp::Dims& dm = op->dims;
-        const double bound = ehp::fixed_point_slot_bound(dm, op->host_geo.data());
-        if (!(bound > 0.0) || bound > 1.0e18 || op->d_pix)
-            return EHP_OK; // keep the one-thread-per-ray kernel (also for curved detectors)
+        const double bound = ehp::fixed_point_slot_bound(dm, op->host_geo.data(), curved_pix(op));
+        if (!(bound > 0.0) || bound > 1.0e18 || (op->d_pix && !op->curved_tiles) || !ehp::tiled_detector_fits(dm))
+            return EHP_OK; // keep the one-thread-per-ray kernel
         for (int i = 0; i < dm.dim; ++i)
             if (dm.vol[i] > 32767)
                 return EHP_OK; // the table stores voxel indices as 16-bit values

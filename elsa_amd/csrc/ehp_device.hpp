@@ -112,6 +112,11 @@ namespace ehp
         // planar detector coordinates of the pixel centres (dim - 1 per pixel, u fastest) of a
         // curved detector; nullptr for a planar detector (centre of pixel u at u + 0.5)
         const float* pix;
+        // curved detectors in the tiled back projections (3D): over any column range [a, b] the
+        // planar v of a row stays within pix_margin of its values at columns a, b and
+        // clamp(pix_uc, a, b)
+        int pix_uc;
+        float pix_margin;
     };
 
     EHP_HD inline float minNum(float x, float y)

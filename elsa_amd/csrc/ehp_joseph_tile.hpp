@@ -111,10 +111,94 @@ namespace ehp
         return t;
     }
 
+    // Number of leading entries k of the increasing sequence f(0..n-1) with f(k) < x (LE:
+    // f(k) <= x), by binary lifting: pos + step only moves while the predicate holds, over the
+    // bits of the answer from the highest.
+    template <bool LE, class F>
+    EHP_HD inline int count_below(int n, float x, F f)
+    {
+        int pos = 0;
+        for (int step = n > 0 ? 1 << (31 - __builtin_clz((unsigned) n)) : 0; step > 0; step >>= 1) {
+            if (pos + step <= n) {
+                const float y = f(pos + step - 1);
+                if (LE ? y <= x : y < x)
+                    pos += step;
+            }
+        }
+        return pos;
+    }
+
+    // Row and column of entry loc of a row-major rectangle nu wide: loc / nu through the
+    // reciprocal rnu = 1 / nu, then one correction step. The float quotient has a relative error
+    // below 2^-22, so for rows below 2^21 it is within one of the true row; exact for loc < 2^31
+    // and nu < 2^23 (the tiled operators keep detectors within these sizes).
+    EHP_HD inline void rect_split(int loc, int nu, float rnu, int& row, int& col)
+    {
+        int r = (int) (((float) loc + 0.5f) * rnu);
+        // loc - r * nu lies in [-nu, 2 nu); modulo 2^32 (r * nu itself may pass 2^31)
+        int c = (int) ((uint32_t) loc - mul24((uint32_t) r, (uint32_t) nu));
+        if (c < 0) {
+            r -= 1;
+            c += nu;
+        } else if (c >= nu) {
+            r += 1;
+            c -= nu;
+        }
+        row = r;
+        col = c;
+    }
+
+    // Detectors the tiled back projections take: rect_split's limits, and a batch of
+    // kMaxPoseBatch whole-detector rectangles still counted in an int.
+    EHP_HD inline bool tiled_detector_fits(const Dims& dims)
+    {
+        return dims.det[0] < (1 << 23) && dims.det[1] < (1 << 21) && dims.pix_per_pose < (int64_t) (1 << 26);
+    }
+
+    // Candidate pixels of a curved detector whose rays can pass through a region projecting
+    // into [umin, umax] x [vmin, vmax] on the planar detector (the ray of pixel (u, v) runs
+    // through its planar centre (pu(u), pv(u, v))). Requires what the operator checks when it
+    // is created: pu the same on every row and increasing along it, pv increasing down every
+    // column, and dims.pix_uc / pix_margin set. Columns: pu within a quarter pixel of [umin,
+    // umax]; rows: those where pv at some candidate column can be within a quarter pixel of
+    // [vmin, vmax] -- on row v, pv over the columns [u0, u1] lies within pix_margin of its
+    // values at u0, u1 and clamp(pix_uc, u0, u1), whose maximum (minimum) increases with v.
+    template <int D>
+    EHP_HD inline bool curved_rect(const Dims& dims, float umin, float umax, float vmin, float vmax, int& u0,
+                                   int& u1, int& v0, int& v1)
+    {
+        const float* px = dims.pix;
+        const int W = dims.det[0];
+        const float alo = umin - 0.25f, ahi = umax + 0.25f;
+        u0 = count_below<false>(W, alo, [&](int u) { return px[(int64_t) u * (D - 1)]; });
+        u1 = count_below<true>(W, ahi, [&](int u) { return px[(int64_t) u * (D - 1)]; }) - 1;
+        v0 = v1 = 0;
+        if (u0 > u1)
+            return false;
+        if (D == 3) {
+            const int c2 = dims.pix_uc < u0 ? u0 : (dims.pix_uc > u1 ? u1 : dims.pix_uc);
+            const float m = 0.25f + dims.pix_margin;
+            const float blo = vmin - m, bhi = vmax + m;
+            const int64_t W2 = 2 * (int64_t) W;
+            const float* c0 = px + 2 * (int64_t) u0 + 1;
+            const float* c1 = px + 2 * (int64_t) u1 + 1;
+            const float* cc = px + 2 * (int64_t) c2 + 1;
+            v0 = count_below<false>(dims.det[1], blo, [&](int v) {
+                const int64_t o = v * W2;
+                return fmaxf(fmaxf(c0[o], c1[o]), cc[o]);
+            });
+            v1 = count_below<true>(dims.det[1], bhi, [&](int v) {
+                     const int64_t o = v * W2;
+                     return fminf(fminf(c0[o], c1[o]), cc[o]);
+                 }) - 1;
+        }
+        return u0 <= u1 && v0 <= v1;
+    }
+
     // Candidate pixels of one pose for the box [lo, hi]: the bounding rectangle of the box's
     // projected corners (a point X lies on the ray of pixel u when P X, homogeneous, maps to
-    // u + 0.5), widened by a quarter pixel for rounding. Falls back to the whole detector when
-    // a corner is not in front of the source.
+    // u + 0.5, or to its planar centre on a curved detector), widened by a quarter pixel for
+    // rounding. Falls back to the whole detector when a corner is not in front of the source.
     template <int D>
     EHP_HD inline bool box_bbox(const PoseGeom& g, const float* lo, const float* hi, const Dims& dims, int& u0,
                                 int& u1, int& v0, int& v1)
@@ -152,6 +236,8 @@ namespace ehp
             v1 = dims.det[1] - 1;
             return true;
         }
+        if (dims.pix)
+            return curved_rect<D>(dims, umin, umax, vmin, vmax, u0, u1, v0, v1);
         const float W = (float) dims.det[0] + 4.0f, H = (float) dims.det[1] + 4.0f;
         umin = umin < -4.0f ? -4.0f : (umin > W ? W : umin);
         umax = umax < -4.0f ? -4.0f : (umax > W ? W : umax);
@@ -481,11 +567,40 @@ namespace ehp
     // neighbouring rays at least h apart there are at most (2/h + 2)^(dim-1) of them, each with
     // weight <= 1. h is bounded below by half the smallest angle between neighbouring pixel
     // rays (sampled at the detector corners and centre) times the smallest distance from the
-    // source to the volume. Returns a negative value when no useful bound exists (source inside
-    // or touching the volume); the caller then uses the one-thread-per-ray kernels.
-    inline double fixed_point_slot_bound(const Dims& dims, const PoseGeom* poses)
+    // source to the volume. On a curved detector (pix: its planar centres, host memory, with
+    // the structure curved_candidates checks) neighbouring rays run through planar points at
+    // least the smallest planar spacing apart, and the corners are those of the planar centres'
+    // extent. Returns a negative value when no useful bound exists (source inside or touching
+    // the volume); the caller then uses the one-thread-per-ray kernels.
+    inline double fixed_point_slot_bound(const Dims& dims, const PoseGeom* poses, const float* pix = nullptr)
     {
         const int D = dims.dim;
+        const int W = dims.det[0], H = dims.det[1];
+        double su = 1.0, sv = 1.0, ulo = 0.5, uhi = W - 0.5, vlo = 0.5, vhi = H - 0.5;
+        if (pix) {
+            const int s = D - 1;
+            ulo = pix[0];
+            uhi = pix[(int64_t) (W - 1) * s];
+            for (int u = 0; u + 1 < W; ++u) {
+                const double d = (double) pix[(int64_t) (u + 1) * s] - (double) pix[(int64_t) u * s];
+                su = (u == 0 || d < su) ? d : su;
+            }
+            if (D == 3) {
+                auto pv = [&](int u, int v) { return (double) pix[((int64_t) v * W + u) * 2 + 1]; };
+                vlo = 1.0e300;
+                vhi = -1.0e300;
+                bool first = true;
+                for (int u = 0; u < W; ++u) {
+                    vlo = pv(u, 0) < vlo ? pv(u, 0) : vlo;
+                    vhi = pv(u, H - 1) > vhi ? pv(u, H - 1) : vhi;
+                    for (int v = 0; v + 1 < H; ++v) {
+                        const double d = pv(u, v + 1) - pv(u, v);
+                        sv = (first || d < sv) ? d : sv;
+                        first = false;
+                    }
+                }
+            }
+        }
         double theta = 1.0e300, dmin = 1.0e300;
         for (int p = 0; p < dims.nposes; ++p) {
             const PoseGeom& g = poses[p];
@@ -501,20 +616,19 @@ namespace ehp
                 for (int i = 0; i < D; ++i)
                     n[i] = y[i] * nn;
             };
-            const double W = dims.det[0], H = dims.det[1];
-            const double us[3] = {0.5, W - 0.5, 0.5 * W};
-            const double vs[3] = {0.5, H - 0.5, 0.5 * H};
+            const double us[3] = {ulo, uhi, 0.5 * (ulo + uhi)};
+            const double vs[3] = {vlo, vhi, 0.5 * (vlo + vhi)};
             for (int a = 0; a < 3; ++a)
                 for (int b = 0; b < (D == 3 ? 3 : 1); ++b) {
                     double n0[3], n1[3], n2[3];
                     dir(us[a], vs[b], n0);
-                    dir(us[a] + 1.0, vs[b], n1);
+                    dir(us[a] + su, vs[b], n1);
                     double d1 = 0, d2 = 0;
                     for (int i = 0; i < D; ++i)
                         d1 += (n1[i] - n0[i]) * (n1[i] - n0[i]);
                     theta = sqrt(d1) < theta ? sqrt(d1) : theta;
                     if (D == 3) {
-                        dir(us[a], vs[b] + 1.0, n2);
+                        dir(us[a], vs[b] + sv, n2);
                         for (int i = 0; i < D; ++i)
                             d2 += (n2[i] - n0[i]) * (n2[i] - n0[i]);
                         theta = sqrt(d2) < theta ? sqrt(d2) : theta;
@@ -536,6 +650,67 @@ namespace ehp
         const double per_pose = D == 3 ? (2.0 / h + 2.0) * (2.0 / h + 2.0) : (2.0 / h + 2.0);
         const double len_max = D == 3 ? 1.7320509 : 1.4142137; // |step| with one component 1
         return dims.nposes * per_pose * len_max;
+    }
+
+    // Checks that a curved detector's planar centres pix (host memory, dim - 1 per pixel, u
+    // fastest) have the structure curved_rect relies on -- all finite, planar u the same on
+    // every row and increasing along it, planar v increasing down every column -- and sets
+    // dims.pix_uc to the column whose planar v spans the least (the middle of a curved
+    // detector) and dims.pix_margin to the least distance that keeps, on every row, pv over any
+    // column range within reach of its values at the range's ends and at clamp(pix_uc, .):
+    // within [0, uc] (and within [uc, W - 1]) a row that rises by at most r and falls by at
+    // most f (later minus earlier entries) stays within min(r, f) of its two ends. Returns
+    // false without that structure; the caller then keeps the one-thread-per-ray kernels.
+    inline bool curved_candidates(Dims& dims, const float* pix)
+    {
+        const int D = dims.dim, W = dims.det[0], H = dims.det[1];
+        const int s = D - 1;
+        for (int u = 0; u < W; ++u) {
+            const float a = pix[(int64_t) u * s];
+            if (!(a - a == 0.0f) || (u > 0 && !(a > pix[(int64_t) (u - 1) * s])))
+                return false;
+            for (int v = 1; v < (D == 3 ? H : 1); ++v)
+                if (!(pix[((int64_t) v * W + u) * 2] == a))
+                    return false;
+        }
+        dims.pix_uc = 0;
+        dims.pix_margin = 0.0f;
+        if (D == 2)
+            return true;
+        auto pv = [&](int u, int v) { return (double) pix[((int64_t) v * W + u) * 2 + 1]; };
+        double least = 0.0;
+        int uc = 0;
+        for (int u = 0; u < W; ++u) {
+            for (int v = 0; v < H; ++v)
+                if (!(pv(u, v) - pv(u, v) == 0.0) || (v > 0 && !(pv(u, v) > pv(u, v - 1))))
+                    return false;
+            const double span = pv(u, H - 1) - pv(u, 0);
+            if (u == 0 || span < least) {
+                least = span;
+                uc = u;
+            }
+        }
+        double margin = 0.0;
+        for (int v = 0; v < H; ++v)
+            for (int part = 0; part < 2; ++part) {
+                const int a = part == 0 ? 0 : uc, b = part == 0 ? uc : W - 1;
+                double lo = pv(a, v), hi = lo, rise = 0.0, fall = 0.0;
+                for (int u = a + 1; u <= b; ++u) {
+                    const double x = pv(u, v);
+                    rise = x - lo > rise ? x - lo : rise;
+                    fall = hi - x > fall ? hi - x : fall;
+                    lo = x < lo ? x : lo;
+                    hi = x > hi ? x : hi;
+                }
+                const double m = rise < fall ? rise : fall;
+                margin = m > margin ? m : margin;
+            }
+        float mf = (float) margin;
+        if ((double) mf < margin)
+            mf = nextafterf(mf, 3.0e38f);
+        dims.pix_uc = uc;
+        dims.pix_margin = mf;
+        return true;
     }
 
     // Fixed-point format of a back projection: slot sums are 64-bit integers in units of

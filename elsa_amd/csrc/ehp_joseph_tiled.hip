@@ -194,12 +194,9 @@ namespace ehp
                 pn = s_pref[j + 1];
                 cr = s_rect[j];
             }
-            const int loc = k - pj;
-            // loc / nu through the reciprocal: exact, as loc + 0.5 is at least 0.5 away from a
-            // multiple of nu and loc < 2^22 (the operator keeps pixels per pose below 2^22)
-            const uint32_t row = (uint32_t) (((float) loc + 0.5f) * cr.rnu);
-            const uint32_t col = (uint32_t) loc - mul24(row, (uint32_t) cr.nu);
-            const int64_t ray = cr.base + (int64_t) (mul24(row, (uint32_t) det0) + col);
+            int row, col;
+            rect_split(k - pj, cr.nu, cr.rnu, row, col);
+            const int64_t ray = cr.base + (int64_t) (mul24((uint32_t) row, (uint32_t) det0) + (uint32_t) col);
             yv = up(sino[ray]);
             r = table[ray];
         };

@@ -29,6 +29,8 @@ struct ehp_op {
     float support = 0.0f;
     int differential = 0; // voxel projectors: phase-contrast weight
     float* d_pix = nullptr; // curved detectors: planar pixel coordinates (dims.pix points here)
+    std::vector<float> host_pix;    // the same in host memory
+    bool curved_tiles = false;      // curved detectors: the coordinates allow the tiled back projections
     uint64_t samples = 0;
     // Joseph, FP32: plane-major forward and LDS-tiled back projection
     bool planes = false;

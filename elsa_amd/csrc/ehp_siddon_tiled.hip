@@ -132,13 +132,10 @@ namespace ehp
             // ray index of candidate k of this batch: pixel (col, row) of its pose's rectangle
             auto cand_ray = [&](int k) -> int64_t {
                 const int j = batch_pose_slot(s_pref, batch, k);
-                const int loc = k - s_pref[j];
                 const CandRect cr = s_rect[j];
-                // loc / nu through the reciprocal: exact, as loc + 0.5 is at least 0.5 away from
-                // a multiple of nu and loc < 2^22
-                const uint32_t row = (uint32_t) (((float) loc + 0.5f) * cr.rnu);
-                const uint32_t col = (uint32_t) loc - mul24(row, (uint32_t) cr.nu);
-                return cr.base + (int64_t) (mul24(row, (uint32_t) dims.det[0]) + col);
+                int row, col;
+                rect_split(k - s_pref[j], cr.nu, cr.rnu, row, col);
+                return cr.base + (int64_t) (mul24((uint32_t) row, (uint32_t) dims.det[0]) + (uint32_t) col);
             };
             float yv_next = 0.0f;
             SidRec r_next{};

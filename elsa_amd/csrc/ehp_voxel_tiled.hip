@@ -71,12 +71,11 @@ namespace ehp
             const int cnt = nb * (chi - clo + 1);
             const float rnb = 1.0f / (float) nb;
             for (int k = tid; k < cnt; k += kThreads) {
-                // k / nb through the reciprocal: exact, as k + 0.5 is at least 0.5 away from a
-                // multiple of nb and k < 2^22
-                const int row = (int) (((float) k + 0.5f) * rnb);
+                int row, col;
+                rect_split(k, nb, rnb, row, col);
                 int c[3] = {0, 0, 0};
                 c[A] = pa;
-                c[b1] = blo + (k - row * nb);
+                c[b1] = blo + col;
                 if (D == 3)
                     c[b2] = clo + row;
                 const int64_t lin = c[0] + c[1] * dims.strides[1] + (int64_t) c[2] * dims.strides[2];

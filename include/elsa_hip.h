@@ -100,9 +100,12 @@ void ehp_geometry_destroy(ehp_geometry* g);
  * the planar detector coordinate of each pixel centre, det_dims[0] x det_dims[1] pixels (u
  * fastest), dim - 1 floats each (host memory; NULL returns to a planar detector). Operators
  * created from the geometry afterwards trace the ray of each pixel through that coordinate
- * instead of u + 0.5 (v + 0.5). Joseph and Siddon take it (their back projections then run one
- * thread per ray); the voxel projectors, like elsa's, take planar detectors only. elsa's GPU
- * projectors ignore curved detectors; its CPU projectors honour them, as these do. */
+ * instead of u + 0.5 (v + 0.5). Joseph and Siddon take it: their tiled back projections find
+ * each tile's candidate pixels by binary search over the planar coordinates when these have a
+ * curved detector's structure (planar u the same on every row and increasing along it, planar
+ * v increasing down every column; otherwise one thread per ray). The voxel projectors, like
+ * elsa's, take planar detectors only. elsa's GPU projectors ignore curved detectors; its CPU
+ * projectors honour them, as these do. */
 int ehp_geometry_set_pixel_coords(ehp_geometry* g, const float* coords);
 
 /* Joseph's method (elsa: JosephsMethodCUDA(volume, detector, fast), JosephsMethodCUDA.h:78-79,

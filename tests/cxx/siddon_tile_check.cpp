@@ -12,6 +12,9 @@ using namespace ehp;
 
 namespace
 {
+    // planar centres of a curved detector's pixels for the next call (nullptr: planar)
+    const float* g_pix = nullptr;
+
     Dims make_dims(int dim, const int64_t* vol, const int64_t* det, int nposes)
     {
         Dims d{};
@@ -27,6 +30,11 @@ namespace
         d.nvox = (int64_t) d.vol[0] * d.vol[1] * d.vol[2];
         d.pix_per_pose = (int64_t) d.det[0] * d.det[1];
         d.nrays = d.pix_per_pose * nposes;
+        if (g_pix) {
+            d.pix = g_pix;
+            if (!curved_candidates(d, g_pix))
+                d.dim = 0; // refused: the caller reports it
+        }
         return d;
     }
 
@@ -51,7 +59,7 @@ namespace
     SidRec ray_rec(const PoseGeom& g, const Dims& dims, int u, int v)
     {
         float o[3], d[3];
-        make_ray<D>(g, u, v, o, d);
+        make_ray_px<D>(g, dims, u, v, o, d);
         return siddon_setup<D>(o, d, dims.vol);
     }
 
@@ -286,6 +294,20 @@ extern "C" int siddon_tile_cover(int dim, const int64_t* vol_dims, const int64_t
         if (!rc)
             rc = check_cover_pass<3, 2>(dims, geo, tc, stride, stats);
     }
+    return rc;
+}
+
+// The same for a curved detector whose pixels have the planar centres pix (dim - 1 per pixel,
+// u fastest).
+extern "C" int siddon_tile_cover_px(int dim, const int64_t* vol_dims, const int64_t* det_dims, int nposes,
+                                    const float* P, const float* Pinv, const float* C, int LA, int TB, int TC,
+                                    int stride, const float* pix, double* stats)
+{
+    g_pix = pix;
+    const int rc = make_dims(dim, vol_dims, det_dims, nposes).dim == 0
+                       ? 11
+                       : siddon_tile_cover(dim, vol_dims, det_dims, nposes, P, Pinv, C, LA, TB, TC, stride, stats);
+    g_pix = nullptr;
     return rc;
 }
 

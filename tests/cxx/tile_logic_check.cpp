@@ -43,6 +43,8 @@ namespace
         return d;
     }
 
+    bool g_check_outside = false;
+
     struct Stats {
         double max_ratio = 0.0; // largest |slot sum| / 2^63 seen
     };
@@ -75,12 +77,40 @@ namespace
                 for (int p = pbase; p < dims.nposes && p < pbase + kPoseBatch; ++p) {
                     if (!(axes[p] & (1 << A)))
                         continue;
-                    int u0, u1, v0, v1;
-                    if (!tile_bbox<D, A>(geo[p], t, dims, u0, u1, v0, v1))
+                    int u0 = 0, u1 = -1, v0 = 0, v1 = -1;
+                    const bool any = tile_bbox<D, A>(geo[p], t, dims, u0, u1, v0, v1);
+                    if (g_check_outside) {
+                        // no ray outside the candidate rectangle may reach a slot of the tile
+                        for (int v = 0; v < dims.det[1]; ++v)
+                            for (int u = 0; u < dims.det[0]; ++u) {
+                                if (any && u >= u0 && u <= u1 && v >= v0 && v <= v1)
+                                    continue;
+                                const int64_t ray = (int64_t) p * dims.pix_per_pose + (int64_t) v * dims.det[0] + u;
+                                bool hit = false;
+                                candidate<D, A>(table[ray], t, to, 1.0f, [&](uint32_t, float) { hit = true; });
+                                if (hit) {
+                                    std::fprintf(stderr, "ray outside the candidates (pass %d tile %d pose %d u %d v %d)\n",
+                                                 A, tile, p, u, v);
+                                    return 7;
+                                }
+                            }
+                    }
+                    if (!any)
                         continue;
                     if (u0 < 0 || u1 >= dims.det[0] || v0 < 0 || v1 >= dims.det[1]) {
                         std::fprintf(stderr, "bbox out of the detector\n");
                         return 2;
+                    }
+                    // the kernels' split of a rectangle entry into row and column
+                    const int nu = u1 - u0 + 1, nv = v1 - v0 + 1;
+                    const float rnu = 1.0f / (float) nu;
+                    for (int k = 0; k < nu * nv; k += 1 + (nu * nv) / 4096) {
+                        int row, col;
+                        rect_split(k, nu, rnu, row, col);
+                        if (row != k / nu || col != k % nu) {
+                            std::fprintf(stderr, "rect_split(%d, %d) wrong\n", k, nu);
+                            return 8;
+                        }
                     }
                     for (int v = v0; v <= v1; ++v)
                         for (int u = u0; u <= u1; ++u) {
@@ -163,7 +193,8 @@ namespace
     }
 
     template <int D, class T>
-    int run(const Dims& dims, const std::vector<PoseGeom>& geo, TileCfg tc, const T* sino, T* vol, double* info)
+    int run(const Dims& dims, const std::vector<PoseGeom>& geo, TileCfg tc, const T* sino, T* vol, double* info,
+            const float* pix)
     {
         std::vector<RayRec> table(dims.nrays);
         std::vector<int> axes(dims.nposes, 0);
@@ -171,7 +202,7 @@ namespace
             for (int v = 0; v < dims.det[1]; ++v)
                 for (int u = 0; u < dims.det[0]; ++u) {
                     float o[3], d[3];
-                    make_ray<D>(geo[p], u, v, o, d);
+                    make_ray_px<D>(geo[p], dims, u, v, o, d);
                     JRay jr;
                     joseph_setup<D>(o, d, dims.vol, jr);
                     const RayRec r = make_ray_rec<D>(jr);
@@ -182,7 +213,7 @@ namespace
         int present = 0;
         for (int p = 0; p < dims.nposes; ++p)
             present |= axes[p];
-        const double bound = fixed_point_slot_bound(dims, geo.data());
+        const double bound = fixed_point_slot_bound(dims, geo.data(), pix);
         info[0] = bound;
         if (!(bound > 0.0))
             return 10;
@@ -217,9 +248,17 @@ namespace
 {
     template <class T>
     int tile_adjoint_t(int dim, const int64_t* vol_dims, const int64_t* det_dims, int nposes, const float* P,
-                       const float* Pinv, const float* C, int LA, int TB, int TC, const T* sino, T* vol, double* info)
+                       const float* Pinv, const float* C, int LA, int TB, int TC, const T* sino, T* vol, double* info,
+                       const float* pix = nullptr)
     {
-        const Dims dims = make_dims(dim, vol_dims, det_dims, nposes);
+        Dims dims = make_dims(dim, vol_dims, det_dims, nposes);
+        if (pix) {
+            dims.pix = pix;
+            if (!curved_candidates(dims, pix))
+                return 11;
+            info[6] = dims.pix_uc;
+            info[7] = dims.pix_margin;
+        }
         const int nm = dim * (dim + 1);
         std::vector<PoseGeom> geo(nposes);
         for (int p = 0; p < nposes; ++p) {
@@ -233,7 +272,7 @@ namespace
             geo[p] = g;
         }
         const TileCfg tc{LA, TB, TC};
-        return dim == 2 ? run<2, T>(dims, geo, tc, sino, vol, info) : run<3, T>(dims, geo, tc, sino, vol, info);
+        return dim == 2 ? run<2, T>(dims, geo, tc, sino, vol, info, pix) : run<3, T>(dims, geo, tc, sino, vol, info, pix);
     }
 } // namespace
 
@@ -243,6 +282,25 @@ extern "C" int tile_adjoint(int dim, const int64_t* vol_dims, const int64_t* det
                             double* info)
 {
     return tile_adjoint_t<float>(dim, vol_dims, det_dims, nposes, P, Pinv, C, LA, TB, TC, sino, vol, info);
+}
+
+// A curved detector: pix holds the planar centres of its pixels (dim - 1 per pixel, u fastest);
+// check_outside also traces every ray outside each tile's candidate rectangle and fails when one
+// reaches the tile. info[6], info[7]: the column and margin of the row search.
+extern "C" int tile_adjoint_px(int dim, const int64_t* vol_dims, const int64_t* det_dims, int nposes, const float* P,
+                               const float* Pinv, const float* C, int LA, int TB, int TC, const float* pix,
+                               int check_outside, const float* sino, float* vol, double* info)
+{
+    g_check_outside = check_outside != 0;
+    const int rc = tile_adjoint_t<float>(dim, vol_dims, det_dims, nposes, P, Pinv, C, LA, TB, TC, sino, vol, info, pix);
+    g_check_outside = false;
+    return rc;
+}
+
+// rect_split as the kernels call it (rnu = 1 / nu in FP32)
+extern "C" void rect_split_host(int loc, int nu, int* row, int* col)
+{
+    rect_split(loc, nu, 1.0f / (float) nu, *row, *col);
 }
 
 // The same for the FP64 back projection (FP32 geometry and weights, FP64 data).

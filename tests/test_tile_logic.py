@@ -36,6 +36,8 @@ def lib():
                                ctypes.c_int, f32, f32, f64]
     L.tile_adjoint64.argtypes = [ctypes.c_int, i64, i64, ctypes.c_int, f32, f32, f32, ctypes.c_int, ctypes.c_int,
                                  ctypes.c_int, f64, f64, f64]
+    L.tile_adjoint_px.argtypes = [ctypes.c_int, i64, i64, ctypes.c_int, f32, f32, f32, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_int, f32, ctypes.c_int, f32, f32, f64]
     L.plane_forward.argtypes = [ctypes.c_int, i64, i64, ctypes.c_int, f32, f32, f32,
                                 np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS"), f32, f32, ctypes.c_int]
     L.plane_forward_mixed_waves.restype = ctypes.c_int64
@@ -109,6 +111,86 @@ def test_tiled_adjoint_fp64_logic_matches_oracle(lib, case):
     err = np.linalg.norm(out - exact) / np.linalg.norm(exact)
     assert err <= 1e-14, err
     assert 0 < info[2] < 0.5, info
+
+
+CURVED_CASES = [
+    # dim, vol, views, det, s2c, c2d, fan angle, tile (LA, TB, TC), trace the rays outside
+    (3, (24, 20, 22), 13, (40, 34), 48.0, 36.0, 1.2, (8, 7, 5), True),
+    (3, (32, 32, 32), 9, (48, 40), 60.0, 60.0, 0.9, (16, 7, 9), True),
+    (3, (40, 24, 33), 13, (61, 29), 60.0, 40.0, 1.5, (32, 15, 15), False),
+    # one tile over the whole volume, magnified onto a wide fan: rows bend by pixels across it;
+    # at 45 degrees an edge of the volume projects to the middle column, where the lowest
+    # and highest rows of the projection then lie
+    (3, (16, 16, 16), 5, (64, 128), 20.0, 40.0, 2.0, (16, 16, 16), True),
+    (2, (31, 17), 11, (80,), 40.0, 30.0, 1.0, (8, 5, 1), True),
+]
+
+
+@pytest.mark.parametrize("case", CURVED_CASES, ids=lambda c: f"{c[0]}d-{'x'.join(map(str, c[1]))}-a{c[6]}")
+def test_tiled_adjoint_curved_detector(lib, case):
+    """Curved detectors (rays through the planar centres of CurvedDetectorDescriptor): the tiles'
+    candidate rectangles, found by binary search over the planar centres, hold every ray that
+    reaches the tile (checked by tracing every other ray, where enabled), and the assembled back
+    projection matches the oracle's."""
+    dim, vol, views, det, s2c, c2d, angle, tile, outside = case
+    rng = np.random.default_rng(5)
+    angles = np.sort(rng.uniform(0, 360, views)).astype(np.float32)
+    if tile[0] == vol[0]:
+        angles = np.array([45.0, 135.0, 225.0, 315.0, 10.0][:views], np.float32)
+    g = oracle.circle_geometry(dim, vol, views, 360, s2c, c2d, det=det, angles_deg=angles)
+    pix = np.ascontiguousarray(oracle.curved_planar_coords(det, angle, s2c + c2d).reshape(-1))
+    y = rng.standard_normal(g.sino_shape).astype(np.float32)
+    out = np.zeros(g.vol_shape, np.float32)
+    info = np.zeros(8, np.float64)
+    rc = lib.tile_adjoint_px(dim, np.array(list(vol) + [1] * (3 - dim), np.int64),
+                             np.array(list(det) + [1] * (2 - len(det)), np.int64), views,
+                             np.ascontiguousarray(g.P), np.ascontiguousarray(g.Pinv), np.ascontiguousarray(g.C),
+                             tile[0], tile[1], tile[2], pix, int(outside), y, out, info)
+    assert rc == 0, rc
+    ref = oracle.joseph(g, y, adjoint=True, mode=oracle.MODE_DIRECT, pix=pix.reshape(-1, dim - 1))
+    err = np.linalg.norm(out - ref) / np.linalg.norm(ref)
+    assert err <= 1e-6, err
+    assert info[3:6].sum() >= 1 and 0 < info[2] < 0.5, info
+    if dim == 3:
+        # the row search's column is the middle one (least spread of planar v), and the
+        # rows of a curved detector bend monotonically on either side of it: no extra margin
+        assert info[6] in ((det[0] - 1) // 2, det[0] // 2) and info[7] == 0.0, info
+
+
+def test_curved_candidates_reject_unstructured_coords(lib):
+    """Planar centres without the structure of a curved detector (planar u varying down a
+    column) are refused by the tiled path's check (the operator then keeps the one-thread-per-
+    ray kernels)."""
+    dim, vol, det = 3, (12, 12, 12), (16, 12)
+    g = oracle.circle_geometry(dim, vol, 3, 360, 40.0, 30.0, det=det)
+    pix = oracle.curved_planar_coords(det, 0.8, 70.0).copy()
+    pix[5 * det[0] + 3, 0] += 0.01
+    y = np.ones(g.sino_shape, np.float32)
+    out = np.zeros(g.vol_shape, np.float32)
+    rc = lib.tile_adjoint_px(dim, np.array(vol, np.int64), np.array(det, np.int64), 3, np.ascontiguousarray(g.P),
+                             np.ascontiguousarray(g.Pinv), np.ascontiguousarray(g.C), 4, 5, 5,
+                             np.ascontiguousarray(pix.reshape(-1)), 0, y, out, np.zeros(8, np.float64))
+    assert rc == 11
+
+
+def test_rect_split_exact_at_large_rectangles(lib):
+    """The tiled kernels split a candidate index into (row, column) through a float reciprocal
+    and one correction step; exact for rectangles far beyond 2^22 entries (here up to 2^31 - 1),
+    where the reciprocal alone goes wrong."""
+    L = lib
+    for nu in (1, 3, 7, 640, 1023, 2048, 4095, 8191, 65535, (1 << 22) + 1):
+        for loc in np.unique(np.concatenate([np.arange(0, 5000), rng_locs(nu)])):
+            row, col = ctypes.c_int(), ctypes.c_int()
+            L.rect_split_host(int(loc), nu, ctypes.byref(row), ctypes.byref(col))
+            assert (row.value, col.value) == divmod(int(loc), nu), (loc, nu, row.value, col.value)
+
+
+def rng_locs(nu):
+    rng = np.random.default_rng(nu)
+    hi = min(2 ** 31 - 1, nu * ((1 << 21) - 1))
+    loc = rng.integers(0, hi, 3000, dtype=np.int64)
+    near = (loc // nu) * nu
+    return np.concatenate([loc, near, np.maximum(near - 1, 0), np.minimum(near + 1, hi - 1), [hi - 1]])
 
 
 FWD_CASES = [
@@ -286,6 +368,8 @@ def sid():
     L.siddon_forward.argtypes = [ctypes.c_int, i64, i64, ctypes.c_int, f32, f32, f32, f32, f32]
     L.siddon_tile_cover.argtypes = [ctypes.c_int, i64, i64, ctypes.c_int, f32, f32, f32, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_int, f64]
+    L.siddon_tile_cover_px.argtypes = [ctypes.c_int, i64, i64, ctypes.c_int, f32, f32, f32, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_int, ctypes.c_int, f32, f64]
     L.siddon_tile_adjoint.argtypes = [ctypes.c_int, i64, i64, ctypes.c_int, f32, f32, f32, ctypes.c_int,
                                       ctypes.c_int, ctypes.c_int, f32, f32]
     L.siddon_plane_forward.argtypes = [ctypes.c_int, i64, i64, ctypes.c_int, f32, f32, f32,
@@ -350,6 +434,19 @@ def test_siddon_tiles_restart_rays_exactly(sid, case):
     tile = case[6]
     stats = np.zeros(2)
     rc = sid.siddon_tile_cover(*head, tile[0], tile[1], tile[2], 3, stats)
+    assert rc == 0, rc
+    assert stats[0] > 0 and stats[1] > stats[0]
+
+
+@pytest.mark.parametrize("case", CURVED_CASES, ids=lambda c: f"{c[0]}d-{'x'.join(map(str, c[1]))}-a{c[6]}")
+def test_siddon_tiles_curved_detector(sid, case):
+    """Curved detectors: every ray through a planar centre lies in the candidate rectangle of
+    every tile it crosses, and the tiles' crossings are those of the whole traversal."""
+    dim, vol, views, det, s2c, c2d, angle, tile, _ = case
+    g, head, rng = _sid_args((dim, vol, views, det, s2c, c2d, tile), 4)
+    pix = np.ascontiguousarray(oracle.curved_planar_coords(det, angle, s2c + c2d).reshape(-1))
+    stats = np.zeros(2)
+    rc = sid.siddon_tile_cover_px(*head, max(tile[0] // 2, 1), tile[1], tile[2], 1, pix, stats)
     assert rc == 0, rc
     assert stats[0] > 0 and stats[1] > stats[0]
 
