@@ -616,7 +616,8 @@ void ehp_destroy(ehp_op* op)
     if (!op)
         return;
     int cur = -1;
-    hipGetDevice(&cur);
+    if (hipGetDevice(&cur) != hipSuccess)
+        cur = -1; // restore nothing afterwards
     if (cur != op->device)
         (void) hipSetDevice(op->device);
     for (void* p : {(void*) op->d_geo, op->d_lut, (void*) op->d_pose_axes, (void*) op->d_table,

@@ -28,6 +28,25 @@ namespace ehp
 #endif
     }
 
+    // The two's-complement 64-bit integer nearest to a * b, for |a * b| < 2^51, in one FMA:
+    // a * b + 1.5 * 2^52 is rounded once to an integer (its unit in the last place is 1), which
+    // then sits in the low mantissa bits, offset by the constant's bit pattern; that pattern's
+    // low word is 0, so removing it takes one 32-bit subtraction from the high word.
+    constexpr double kRoundMagic = 6755399441055744.0; // 1.5 * 2^52, bits 0x4338000000000000
+    EHP_HD inline unsigned long long fixed_round(double a, double b)
+    {
+        const double d = __builtin_fma(a, b, kRoundMagic);
+        unsigned long long u;
+        __builtin_memcpy(&u, &d, 8);
+        return u - 0x4338000000000000ull;
+    }
+
+    // What the back projection's tap code hands to add(offset, value): the FP32 weight of the tap
+    // (TAP_WEIGHT, for a caller that scales it itself), or the tap's 64-bit fixed-point
+    // contribution (TAP_FIXED): round(yq * w1 * w2) with the exact bilinear weights 1 - c and c
+    // of the FP32 fractions, formed in FP64.
+    enum { TAP_WEIGHT = 1, TAP_FIXED = 2 };
+
     // Fixed-point exponent of a back projection whose largest |y| is not finite: the whole
     // output becomes NaN (integer sums cannot carry a NaN to just the voxels it touches).
     constexpr int kNonFinite = 1 << 20;
@@ -295,18 +314,6 @@ namespace ehp
 #endif
     }
 
-    // x rounded to the nearest int, halves up, in one instruction (v_cvt_rpi_i32_f32); |x| < 2^30
-    EHP_HD inline int cvt_rpi(float x)
-    {
-#if defined(__HIP_DEVICE_COMPILE__)
-        int r;
-        asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(x));
-        return r;
-#else
-        return (int) floorf(x + 0.5f);
-#endif
-    }
-
     // x - floor(x) (v_fract_f32); exact for finite x >= 0
     EHP_HD inline float fract_pos(float x)
     {
@@ -394,18 +401,20 @@ namespace ehp
     }
 
     // Back projection of the sample at fs = s (as a float) of ray r lying on the tile plane at
-    // byte offset poff: if its bilinear footprint touches the tile, hand ys * weight for each of
-    // its four (2D: two) slots to add(offset, value), ys being y * len in the ray's own scale
-    // (ray_scale); with W, hand add(offset, weight) the tap's weight itself (3D: the product of
-    // the two transverse weights in FP32, elsa's value, the driving axis' weight being 1), for a
-    // caller that scales it in FP64. The position
-    // and weights are elsa's (doInterpolation, JosephsMethod.cpp:60-97; driving weight 1) with the
-    // forward's position rule fma(s, step, pos0), so positions and floors equal the forward's bit
-    // for bit (the forward interpolates in lerp form, the same weights up to FP32 rounding); a 3D
-    // tap's weight is applied as (ys * w2) * w1. LOW: evaluate c as elsa writes it (tiles on the
-    // low faces, tile_low_face); otherwise as fract(q - 0.5), the same value there.
-    template <int D, bool LOW, bool W, class Add>
-    EHP_HD inline void tile_sample_fx(const RayRec& r, float fs, uint32_t poff, const TileOff& to, float ys, Add add)
+    // byte offset poff: if its bilinear footprint touches the tile, hand add(offset, value) one
+    // value for each of its four (2D: two) slots. M = TAP_FIXED: yq is y * len in units of the
+    // fixed-point unit q (a power-of-two scaling of elsa's FP32 product y * len) and the value is
+    // round(yq * w2 * w1) as a 64-bit integer, with the weights 1 - c and c of the FP32
+    // fractions c taken exactly (FP64; the products are exact, only the final rounding is not).
+    // M = TAP_WEIGHT: the value is the tap's FP32 weight as elsa forms it (3D: the product of
+    // the two transverse weights, the driving axis' weight being 1), for a caller that scales it
+    // in FP64. The position and fractions are elsa's (doInterpolation, JosephsMethod.cpp:60-97)
+    // with the forward's position rule fma(s, step, pos0), so positions and floors equal the
+    // forward's bit for bit (the forward interpolates in lerp form: the exact weights up to FP32
+    // rounding). LOW: evaluate c as elsa writes it (tiles on the low faces, tile_low_face);
+    // otherwise as fract(q - 0.5), the same value there.
+    template <int D, bool LOW, int M, class Y, class Add>
+    EHP_HD inline void tile_sample_fx(const RayRec& r, float fs, uint32_t poff, const TileOff& to, Y yq, Add add)
     {
         const float q1 = __builtin_fmaf(fs, r.s1, r.p1);
         const float t1 = q1 - 0.5f;
@@ -415,10 +424,15 @@ namespace ehp
             return;
         if (D == 2) {
             const float c1 = LOW ? (q1 - (float) g1) - 0.5f : fract_pos(t1);
-            const float w1 = 1.0f - c1;
             const uint32_t off = o1 + poff;
-            add(off, W ? w1 : ys * w1);
-            add(off + 8u, W ? c1 : ys * c1);
+            if constexpr (M == TAP_FIXED) {
+                const double c1d = c1;
+                add(off, fixed_round(yq, 1.0 - c1d));
+                add(off + 8u, fixed_round(yq, c1d));
+            } else {
+                add(off, 1.0f - c1);
+                add(off + 8u, c1);
+            }
         } else {
             const float q2 = __builtin_fmaf(fs, r.s2, r.p2);
             const float t2 = q2 - 0.5f;
@@ -428,20 +442,22 @@ namespace ehp
                 return;
             const float c1 = LOW ? (q1 - (float) g1) - 0.5f : fract_pos(t1);
             const float c2 = LOW ? (q2 - (float) g2) - 0.5f : fract_pos(t2);
-            const float w1 = 1.0f - c1, w2 = 1.0f - c2;
             const uint32_t off = add3_u32(o1, o2, poff);
             const uint32_t off2 = off + (uint32_t) to.pb8;
-            if (W) {
+            if constexpr (M == TAP_FIXED) {
+                const double c1d = c1, c2d = c2;
+                const double w1d = 1.0 - c1d;
+                const double a = yq * (1.0 - c2d), b = yq * c2d;
+                add(off, fixed_round(a, w1d));
+                add(off + 8u, fixed_round(a, c1d));
+                add(off2, fixed_round(b, w1d));
+                add(off2 + 8u, fixed_round(b, c1d));
+            } else {
+                const float w1 = 1.0f - c1, w2 = 1.0f - c2;
                 add(off, w1 * w2);
                 add(off + 8u, c1 * w2);
                 add(off2, w1 * c2);
                 add(off2 + 8u, c1 * c2);
-            } else {
-                const float yw = ys * w2, yc = ys * c2;
-                add(off, yw * w1);
-                add(off + 8u, yw * c1);
-                add(off2, yc * w1);
-                add(off2 + 8u, yc * c1);
             }
         }
     }
@@ -451,8 +467,8 @@ namespace ehp
     // [0, nsteps) goes to tile_sample_fx. FULL: the caller knows that all la samples lie in
     // [0, nsteps) (no test per plane). Every ray visits the planes in the same order, so the
     // lanes of a wavefront address one LDS plane at each step.
-    template <int D, bool LOW, bool FULL, bool W, class Add>
-    EHP_HD inline void tile_ray_fx(const RayRec& r, int s0, int sg, int la, const TileOff& to, float ys, Add add)
+    template <int D, bool LOW, bool FULL, int M, class Y, class Add>
+    EHP_HD inline void tile_ray_fx(const RayRec& r, int s0, int sg, int la, const TileOff& to, Y yq, Add add)
     {
         float fs = (float) s0;
         const float fsg = (float) sg;
@@ -460,18 +476,17 @@ namespace ehp
         uint32_t poff = to.base;
         for (int pa = 0; pa < la; ++pa, fs += fsg, s += sg, poff += to.plane)
             if (FULL || (uint32_t) s < (uint32_t) r.nsteps)
-                tile_sample_fx<D, LOW, W>(r, fs, poff, to, ys, add);
+                tile_sample_fx<D, LOW, M>(r, fs, poff, to, yq, add);
     }
 
     // Back projection of one candidate ray into a tile (planes [A0, A1) along A): rays of
-    // another driving axis, without samples or with y = 0 add nothing. ys: y * len in the ray's
-    // scale (ray_scale); with W any value that is 0 exactly when y * len is, add() then
-    // receiving the weights (tile_sample_fx). Takes the test-free plane walk when every active
-    // lane of the wavefront has samples on all the tile's planes.
-    template <int D, int A, bool LOW, bool W = false, class Add>
-    EHP_HD inline void tile_candidate_fx(const RayRec& r, const TileBox& t, const TileOff& to, float ys, Add add)
+    // another driving axis, without samples or with yq = 0 add nothing (tile_sample_fx; with
+    // TAP_WEIGHT, yq is any value that is 0 exactly when y * len is). Takes the test-free plane
+    // walk when every active lane of the wavefront has samples on all the tile's planes.
+    template <int D, int A, bool LOW, int M, class Y, class Add>
+    EHP_HD inline void tile_candidate_fx(const RayRec& r, const TileBox& t, const TileOff& to, Y yq, Add add)
     {
-        if ((r.meta & 3) != A || r.nsteps == 0 || ys == 0.0f)
+        if ((r.meta & 3) != A || r.nsteps == 0 || yq == Y(0))
             return;
         const bool neg = (r.meta & 4) != 0;
         const int sg = neg ? -1 : 1;
@@ -480,9 +495,9 @@ namespace ehp
         const int s1 = s0 + (la - 1) * sg;
         const int lo = s0 < s1 ? s0 : s1, hi = s0 < s1 ? s1 : s0;
         if (wave_all(lo >= 0 && hi < r.nsteps))
-            tile_ray_fx<D, LOW, true, W>(r, s0, sg, la, to, ys, add);
+            tile_ray_fx<D, LOW, true, M>(r, s0, sg, la, to, yq, add);
         else
-            tile_ray_fx<D, LOW, false, W>(r, s0, sg, la, to, ys, add);
+            tile_ray_fx<D, LOW, false, M>(r, s0, sg, la, to, yq, add);
     }
 
     // Linear volume index of LDS slot i if the tile owns that voxel, else -1.
@@ -649,7 +664,10 @@ namespace ehp
             return -1.0;
         const double per_pose = D == 3 ? (2.0 / h + 2.0) * (2.0 / h + 2.0) : (2.0 / h + 2.0);
         const double len_max = D == 3 ? 1.7320509 : 1.4142137; // |step| with one component 1
-        return dims.nposes * per_pose * len_max;
+        // at least 4096 (> sqrt(3) 2^11): a single term, at most max|y| len_max, then stays below
+        // 2^51 units, the range of fixed_round
+        const double b = dims.nposes * per_pose * len_max;
+        return b > 4096.0 ? b : 4096.0;
     }
 
     // Checks that a curved detector's planar centres pix (host memory, dim - 1 per pixel, u
@@ -755,11 +773,6 @@ namespace ehp
         return (unsigned long long) (long long) (int) t << sh;
     }
 
-    // The same, rounded to nearest (halves up) in one instruction (v_cvt_rpi_i32_f32).
-    EHP_HD inline unsigned long long fixed_contrib_rn(float t, int sh)
-    {
-        return (unsigned long long) (long long) cvt_rpi(t) << sh;
-    }
 
     inline void tile_counts(const Dims& dims, const TileCfg& tc, int a, int& ntA, int& ntB, int& ntC)
     {

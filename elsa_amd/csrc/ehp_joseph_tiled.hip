@@ -206,18 +206,17 @@ namespace ehp
                 // as the one-thread-per-ray FP64 kernel forms it), rounded to the nearest multiple of
                 // q; yq is y * len in units of q (a power-of-two scaling, exact)
                 const double yq = ldexp(yv * (double) r.len, 62 - eg);
-                tile_candidate_fx<D, A, LOW, true>(r, t, to, yq != 0.0 ? 1.0f : 0.0f, [&](uint32_t off, float w) {
-                    __hip_atomic_fetch_add((lds_u64*) (uintptr_t) off,
-                                           (unsigned long long) (long long) __builtin_rint(yq * (double) w),
-                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                tile_candidate_fx<D, A, LOW, TAP_WEIGHT>(r, t, to, yq, [&](uint32_t off, float w) {
+                    __hip_atomic_fetch_add((lds_u64*) (uintptr_t) off, fixed_round(yq, (double) w), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
                 });
             } else {
-                float ys;
-                int sh;
-                ray_scale(yv * r.len, eg, ys, sh);
+                // FP32 (and bfloat16, widened): elsa's FP32 product y * len in units of q (exact),
+                // times the exact weights, rounded once to the nearest multiple of q
+                const double yq = ldexp((double) (yv * r.len), 62 - eg);
                 // off: the slot's LDS byte address (the tile's base included)
-                tile_candidate_fx<D, A, LOW>(r, t, to, ys, [&](uint32_t off, float val) {
-                    __hip_atomic_fetch_add((lds_u64*) (uintptr_t) off, fixed_contrib_rn(val, sh), __ATOMIC_RELAXED,
+                tile_candidate_fx<D, A, LOW, TAP_FIXED>(r, t, to, yq, [&](uint32_t off, unsigned long long v) {
+                    __hip_atomic_fetch_add((lds_u64*) (uintptr_t) off, v, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_WORKGROUP);
                 });
             }

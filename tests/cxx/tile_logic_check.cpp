@@ -49,15 +49,16 @@ namespace
         double max_ratio = 0.0; // largest |slot sum| / 2^63 seen
     };
 
-    // one candidate ray of a tile, LOW chosen as the kernel chooses it per tile; W: add() gets
-    // the weights (the FP64 back projection)
-    template <int D, int A, bool W = false, class Add>
-    void candidate(const RayRec& r, const TileBox& t, const TileOff& to, float ys, Add add)
+    // one candidate ray of a tile, LOW chosen as the kernel chooses it per tile; M: what add()
+    // gets (TAP_FIXED: the FP32 back projection's 64-bit contributions, TAP_WEIGHT: the weights,
+    // which the FP64 back projection scales)
+    template <int D, int A, int M, class Y, class Add>
+    void candidate(const RayRec& r, const TileBox& t, const TileOff& to, Y yq, Add add)
     {
         if (tile_low_face<D>(t))
-            tile_candidate_fx<D, A, true, W>(r, t, to, ys, add);
+            tile_candidate_fx<D, A, true, M>(r, t, to, yq, add);
         else
-            tile_candidate_fx<D, A, false, W>(r, t, to, ys, add);
+            tile_candidate_fx<D, A, false, M>(r, t, to, yq, add);
     }
 
     // T = float: the FP32 kernel's contributions (int32 in each ray's scale, shifted into units
@@ -87,7 +88,7 @@ namespace
                                     continue;
                                 const int64_t ray = (int64_t) p * dims.pix_per_pose + (int64_t) v * dims.det[0] + u;
                                 bool hit = false;
-                                candidate<D, A>(table[ray], t, to, 1.0f, [&](uint32_t, float) { hit = true; });
+                                candidate<D, A, TAP_WEIGHT>(table[ray], t, to, 1.0f, [&](uint32_t, float) { hit = true; });
                                 if (hit) {
                                     std::fprintf(stderr, "ray outside the candidates (pass %d tile %d pose %d u %d v %d)\n",
                                                  A, tile, p, u, v);
@@ -119,41 +120,30 @@ namespace
                             if (yv == T(0))
                                 continue;
                             bool bad = false;
+                            // the kernels' y * len in units of q: FP64 y times the FP32 length, or
+                            // the FP32 product of the FP32 back projection; fixed_round needs
+                            // |yq * w| < 2^51
+                            const double yq = sizeof(T) == 8
+                                                  ? ldexp((double) yv * (double) table[ray].len, 62 - eg)
+                                                  : ldexp((double) ((float) yv * table[ray].len), 62 - eg);
+                            if (!(fabs(yq) < 2251799813685248.0))
+                                bad = true;
+                            auto put = [&](uint32_t off, unsigned long long v) {
+                                const int64_t slot = off / 8;
+                                if (off % 8 != 0 || slot >= t.nslots) {
+                                    bad = true;
+                                    return;
+                                }
+                                iacc[slot] += (__int128) (long long) v;
+                            };
                             if (sizeof(T) == 8) {
-                                const double yq = ldexp((double) yv * (double) table[ray].len, 62 - eg);
-                                candidate<D, A, true>(table[ray], t, to, 1.0f, [&](uint32_t off, float w) {
-                                    const int64_t slot = off / 8;
-                                    if (off % 8 != 0 || slot >= t.nslots || !(w >= 0.0f && w <= 1.0f)) {
+                                candidate<D, A, TAP_WEIGHT>(table[ray], t, to, yq, [&](uint32_t off, float w) {
+                                    if (!(w >= 0.0f && w <= 1.0f))
                                         bad = true;
-                                        return;
-                                    }
-                                    // the contribution must convert to an int64 below 2^62
-                                    const double c = rint(yq * (double) w);
-                                    if (!(fabs(c) < 4611686018427387904.0)) {
-                                        bad = true;
-                                        return;
-                                    }
-                                    iacc[slot] += (__int128) (long long) c;
+                                    put(off, fixed_round(yq, (double) w));
                                 });
                             } else {
-                                float ys;
-                                int sh;
-                                ray_scale((float) yv * table[ray].len, eg, ys, sh);
-                                if (sh < 0 || sh > 32)
-                                    bad = true;
-                                candidate<D, A>(table[ray], t, to, ys, [&](uint32_t off, float val) {
-                                    const int64_t slot = off / 8;
-                                    if (off % 8 != 0 || slot >= t.nslots) {
-                                        bad = true;
-                                        return;
-                                    }
-                                    // the contribution must round to an int32 of magnitude <= 2^30
-                                    if (!(fabsf(val) < 1073741824.0f)) {
-                                        bad = true;
-                                        return;
-                                    }
-                                    iacc[slot] += (__int128) (long long) fixed_contrib_rn(val, sh);
-                                });
+                                candidate<D, A, TAP_FIXED>(table[ray], t, to, yq, put);
                             }
                             if (bad) {
                                 std::fprintf(stderr, "LDS slot or contribution out of range (pass %d tile %d pose %d u %d v %d)\n", A,
