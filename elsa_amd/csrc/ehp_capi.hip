@@ -308,30 +308,33 @@ namespace
     }
 
     // Zero-padded plane-major views of the volume for the plane-walking forwards, one per
-    // driving axis that occurs (pose_axes: the axes per pose): 32-bit byte offsets inside a view,
-    // so below 4 GiB each; the padding is zeroed once here and never written again. Sets
-    // op->planes when every view fits.
+    // driving axis that occurs (pose_axes: the axes per pose), in the operator's storage type:
+    // 32-bit byte offsets inside a view, so below 4 GiB each; the padding is zeroed once here and
+    // never written again. Sets op->planes when every view fits.
     int setup_views(ehp_op* op, const std::vector<int>& h)
     {
         const ehp::Dims& dm = op->dims;
         hipError_t e = hipSuccess;
+        // elements per view below 2^32 bytes: 2^30 FP32 / bfloat16 (the latter's byte offsets
+        // would allow more; one limit keeps the FP32 and bfloat16 views alike), 2^29 FP64
+        const int64_t max_elems = (int64_t) 1 << (op->dtype == EHP_F64 ? 29 : 30);
         // 3D Joseph forwards (FP32, bfloat16) read row-pair views (one load per sample) when those fit
         bool pairs = op->kind == KIND_JOSEPH && (op->dtype == EHP_F32 || op->dtype == EHP_BF16) && dm.dim == 3;
         for (int a = 0; pairs && a < dm.dim; ++a)
-            if ((op->axes_present & (1 << a)) && ehp::pad_layout(dm, a, fan_axis(op, h, a), true).size >= ((int64_t) 1 << 30))
+            if ((op->axes_present & (1 << a)) && ehp::pad_layout(dm, a, fan_axis(op, h, a), true).size >= max_elems)
                 pairs = false;
         bool planes = true;
         for (int a = 0; a < dm.dim; ++a) {
             op->layouts[a] = ehp::pad_layout(dm, a, fan_axis(op, h, a), pairs);
-            if ((op->axes_present & (1 << a)) && op->layouts[a].size >= ((int64_t) 1 << 30))
+            if ((op->axes_present & (1 << a)) && op->layouts[a].size >= max_elems)
                 planes = false;
         }
         const bool b16 = op->dtype == EHP_BF16;
+        const size_t esize = b16 ? sizeof(ehp::bf16) : (op->dtype == EHP_F64 ? sizeof(double) : sizeof(float));
         for (int a = 0; planes && a < dm.dim; ++a) {
             if (!(op->axes_present & (1 << a)))
                 continue;
-            const size_t vbytes =
-                (size_t) op->layouts[a].size * (b16 ? sizeof(ehp::bf16) : sizeof(float)) + (b16 ? ehp::kViewSlack : 0);
+            const size_t vbytes = (size_t) op->layouts[a].size * esize + (b16 ? ehp::kViewSlack : 0);
             e = hipMalloc((void**) &op->d_view[a], vbytes);
             if (e != hipSuccess)
                 return fail(EHP_ERR_ALLOC, "hipMalloc of a %zu-byte volume view failed: %s", vbytes,
@@ -339,6 +342,8 @@ namespace
             if (b16)
                 e = ehp::launch_fill<ehp::bf16>((ehp::bf16*) op->d_view[a], op->layouts[a].size + ehp::kViewSlack / 2,
                                                 ehp::bf16{0}, nullptr);
+            else if (op->dtype == EHP_F64)
+                e = ehp::launch_fill<double>((double*) op->d_view[a], op->layouts[a].size, 0.0, nullptr);
             else
                 e = ehp::launch_fill<float>((float*) op->d_view[a], op->layouts[a].size, 0.0f, nullptr);
             if (e != hipSuccess)
@@ -348,9 +353,9 @@ namespace
         return EHP_OK;
     }
 
-    // FP32 and bfloat16 Joseph: per-ray table, driving axes per pose, plane-major views for the
-    // forward and, when the fixed-point bound exists, the LDS-tiled back projection. FP64 Joseph:
-    // the same without the views.
+    // Joseph (FP32, bfloat16, FP64): per-ray table, driving axes per pose, plane-major views for
+    // the forward in the operator's storage type and, when the fixed-point bound exists, the
+    // LDS-tiled back projection.
     int setup_fast(ehp_op* op)
     {
         const ehp::Dims& dm = op->dims;
@@ -378,8 +383,7 @@ namespace
         for (int v : h)
             op->axes_present |= v;
 
-        // FP64 operators keep the one-thread-per-ray forward (no plane-major views)
-        int rc = op->dtype == EHP_F64 ? EHP_OK : setup_views(op, h);
+        int rc = setup_views(op, h);
         if (rc != EHP_OK)
             return rc;
         const bool b16 = op->dtype == EHP_BF16;
@@ -897,6 +901,8 @@ static hipError_t launch(ehp_op* op, const void* in, void* out, hipStream_t s, b
             e = ehp::joseph_adjoint_tiled<double>(op->d_geo, op->d_table, dm, op->d_pose_axes, op->axes_present,
                                                   op->tile, op->batch, op->slot_bound, op->d_maxbits, op->d_fixexp, x,
                                                   y, s);
+        else if (op->kind == KIND_JOSEPH && !adjoint && op->planes)
+            e = ehp::joseph_forward_planes<double>(op->d_geo, dm, op->layouts, op->d_view, op->axes_present, x, y, s);
         else if (op->kind == KIND_JOSEPH)
             e = adjoint ? ehp::joseph_adjoint<double>(op->d_geo, dm, x, y, s) : ehp::joseph_forward<double>(op->d_geo, dm, x, y, s);
         else if (op->kind == KIND_SIDDON)

@@ -1,9 +1,10 @@
-// ehp_joseph_planes.hip -- Joseph forward projection over plane-major volume views (FP32).
+// ehp_joseph_planes.hip -- Joseph forward projection over plane-major volume views.
 //
 // The forward of elsa's Joseph method (elsa/projectors/JosephsMethod.cpp:140-198, forward
-// branch :187-194) with one lane per ray, each sample's bilinear interpolation in lerp form
-// (plane_taps_add) and the sum in FP64: elsa's samples, floors and weights, elsa's sums to FP32
-// rounding. A wavefront holds a patch of neighbouring detector
+// branch :187-194) with one lane per ray. FP32 and bfloat16: each sample's bilinear
+// interpolation in lerp form (plane_taps_add) and the sum in FP64, elsa's samples, floors and
+// weights, elsa's sums to FP32 rounding. FP64: elsa's weights and sums over FP64 views
+// (plane_taps_add64). A wavefront holds a patch of neighbouring detector
 // pixels (8 x 8, or 16 x 4 over row-pair views), and its rays step through the volume's
 // planes together, so at each step its lanes
 // read the same plane at neighbouring in-plane positions. Those reads coalesce only if the axis
@@ -78,10 +79,18 @@ namespace ehp
             }
         }
 
-        template <int D, int A, bool NEG, bool V16>
-        __device__ inline float fwd_span(const RayRec& r, const Dims& dims, const ViewSet& vs, int t0, int niter,
-                                         int s_first, int full0, int full1)
+        template <int D, int A, bool NEG, bool V16, bool F64>
+        __device__ inline double fwd_span(const RayRec& r, const Dims& dims, const ViewSet& vs, int t0, int niter,
+                                          int s_first, int full0, int full1)
         {
+            if constexpr (F64) {
+                // FP64 views hold one element per voxel
+                if (D == 3 && (vs.fb2 >> A) & 1)
+                    return plane_span_forward<D, A, NEG, true, false, false, true>(r, dims, vs.v[A], t0, niter,
+                                                                                   s_first, full0, full1);
+                return plane_span_forward<D, A, NEG, false, false, false, true>(r, dims, vs.v[A], t0, niter, s_first,
+                                                                                full0, full1);
+            }
             if (D == 3 && (vs.pairs >> A) & 1) {
                 if ((vs.fb2 >> A) & 1)
                     return plane_span_forward<D, A, NEG, true, V16, true>(r, dims, vs.v[A], t0, niter, s_first,
@@ -96,9 +105,9 @@ namespace ehp
 
         // The rays of group K = 2 * axis + direction of a wavefront share one plane loop over the
         // group's span. Called with every lane active; `keys` holds the groups present.
-        template <int D, int K, bool V16>
+        template <int D, int K, bool V16, bool F64>
         __device__ __forceinline__ void fwd_group(int keys, int key, const RayRec& r, const Dims& dims,
-                                                  const ViewSet& vs, int tfirst, int tlast, int s0, float& val)
+                                                  const ViewSet& vs, int tfirst, int tlast, int s0, double& val)
         {
             if (!((keys >> K) & 1))
                 return;
@@ -109,8 +118,8 @@ namespace ehp
             const int tb = wave_min(mine ? tlast : 0x7fffffff);   // in [ta, tb]
             const int full0 = ta - t0, full1 = tb - t0 + 1 > full0 ? tb - t0 + 1 : full0;
             if (mine)
-                val = fwd_span<D, K / 2, (K & 1) != 0, V16>(r, dims, vs, t0, t1 - t0 + 1, s0 + t0 - tfirst, full0,
-                                                            full1);
+                val = fwd_span<D, K / 2, (K & 1) != 0, V16, F64>(r, dims, vs, t0, t1 - t0 + 1, s0 + t0 - tfirst,
+                                                                 full0, full1);
         }
 
         template <class E>
@@ -155,18 +164,24 @@ namespace ehp
                     key = 2 * jr.axis + ((r.meta & 4) ? 1 : 0);
             }
             constexpr bool V16 = sizeof(OT) == 2; // bfloat16 operators read bfloat16 views
+            constexpr bool F64 = sizeof(OT) == 8; // FP64 operators read FP64 views
             const int keys = wave_or(key >= 0 ? 1 << key : 0);
-            float val = 0.0f;
-            fwd_group<D, 0, V16>(keys, key, r, dims, views, tfirst, tlast, s0, val);
-            fwd_group<D, 1, V16>(keys, key, r, dims, views, tfirst, tlast, s0, val);
-            fwd_group<D, 2, V16>(keys, key, r, dims, views, tfirst, tlast, s0, val);
-            fwd_group<D, 3, V16>(keys, key, r, dims, views, tfirst, tlast, s0, val);
+            double val = 0.0;
+            fwd_group<D, 0, V16, F64>(keys, key, r, dims, views, tfirst, tlast, s0, val);
+            fwd_group<D, 1, V16, F64>(keys, key, r, dims, views, tfirst, tlast, s0, val);
+            fwd_group<D, 2, V16, F64>(keys, key, r, dims, views, tfirst, tlast, s0, val);
+            fwd_group<D, 3, V16, F64>(keys, key, r, dims, views, tfirst, tlast, s0, val);
             if (D == 3) {
-                fwd_group<D, 2 * D - 2, V16>(keys, key, r, dims, views, tfirst, tlast, s0, val);
-                fwd_group<D, 2 * D - 1, V16>(keys, key, r, dims, views, tfirst, tlast, s0, val);
+                fwd_group<D, 2 * D - 2, V16, F64>(keys, key, r, dims, views, tfirst, tlast, s0, val);
+                fwd_group<D, 2 * D - 1, V16, F64>(keys, key, r, dims, views, tfirst, tlast, s0, val);
             }
-            if (inside)
-                sino[(int64_t) pose * dims.pix_per_pose + (int64_t) v * dims.det[0] + u] = down<OT>(val);
+            if (inside) {
+                OT* out = sino + (int64_t) pose * dims.pix_per_pose + (int64_t) v * dims.det[0] + u;
+                if constexpr (F64)
+                    *out = val;
+                else
+                    *out = down<OT>((float) val);
+            }
         }
 
         template <int D, class OT, int WU = kFwdWU>
@@ -232,8 +247,8 @@ namespace ehp
         return hipGetLastError();
     }
 
-    // T = float (FP32 operators) or bf16 (bfloat16 storage: bfloat16 volume, views and
-    // sinogram, FP32 arithmetic)
+    // T = float (FP32 operators), bf16 (bfloat16 storage: bfloat16 volume, views and sinogram,
+    // FP32 arithmetic) or double (FP64 operators: FP64 views, elsa's FP64 sums)
     template <class T>
     hipError_t joseph_forward_planes(const PoseGeom* geo, const Dims& dims, const PadLayout* layouts,
                                      void* const* views, int axes_present, const T* vol, T* sino, hipStream_t s)
@@ -276,10 +291,13 @@ namespace ehp
 
     template hipError_t joseph_view_copy<float>(const Dims&, const PadLayout&, const float*, float*, hipStream_t);
     template hipError_t joseph_view_copy<bf16>(const Dims&, const PadLayout&, const bf16*, bf16*, hipStream_t);
+    template hipError_t joseph_view_copy<double>(const Dims&, const PadLayout&, const double*, double*, hipStream_t);
     template hipError_t joseph_forward_planes<float>(const PoseGeom*, const Dims&, const PadLayout*, void* const*, int,
                                                      const float*, float*, hipStream_t);
     template hipError_t joseph_forward_planes<bf16>(const PoseGeom*, const Dims&, const PadLayout*, void* const*, int,
                                                     const bf16*, bf16*, hipStream_t);
+    template hipError_t joseph_forward_planes<double>(const PoseGeom*, const Dims&, const PadLayout*, void* const*,
+                                                      int, const double*, double*, hipStream_t);
 
     // Jobs of a batched forward: the view copies of every operator (one job per view) and the
     // forward blocks (one job per operator); block_job maps each block to its job.

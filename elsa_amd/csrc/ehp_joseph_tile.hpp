@@ -848,7 +848,7 @@ namespace ehp
     constexpr int kViewSlack = 8;
 
     // What the kernels read of a view; strides in bytes (a view is below 4 GiB). Views hold
-    // FP32 or bfloat16 elements.
+    // FP32, bfloat16 or FP64 elements.
     struct PadView {
         const void* base;
         uint32_t PS4, RS4;
@@ -956,6 +956,25 @@ namespace ehp
         return q;
     }
 
+    // Two neighbouring doubles of an FP64 view (one 16-byte load; 8-byte alignment suffices).
+    struct __attribute__((aligned(8))) Pair64 {
+        double a, b;
+    };
+
+    EHP_HD inline Pair64 ld_view2_f64(const PadView& V, const char* pb, uint32_t off)
+    {
+#if defined(EHP_VIEW_CHECK)
+        const int64_t o = (int64_t) (pb - (const char*) V.base) + off;
+        if ((o & 7) || o < 0 || (o >> 3) + 1 >= V.size) {
+            g_view_oob = true;
+            return Pair64{0.0, 0.0};
+        }
+#endif
+        Pair64 p;
+        __builtin_memcpy(&p, pb + off, sizeof(Pair64));
+        return p;
+    }
+
     // floor value g clamped to [-2, n]: taps outside the volume land in the zero padding
     EHP_HD inline int pad_clamp(float g, int n)
     {
@@ -1055,38 +1074,125 @@ namespace ehp
         return __builtin_fma((double) __builtin_fmaf(T.c2, a1 - a0, a0), len, acc);
     }
 
+    // The taps of one sample of an FP64 view (one element per voxel) and elsa's fractions
+    // c = (q - F) - 0.5 (doInterpolation's form, also for q < 0.5 at the low faces).
+    struct SampleTaps64 {
+        double v00, v10, v01, v11;
+        float c1, c2;
+    };
+
+    template <int D, int A, bool FB2>
+    EHP_HD inline SampleTaps64 plane_sample_taps64(const RayRec& r, const Dims& dims, const PadView& V,
+                                                   uint32_t poff, float fs)
+    {
+        constexpr int b1 = TileAxes<D, A>::b1, b2 = TileAxes<D, A>::b2;
+        const int n1 = dims.vol[b1], n2 = (D == 3) ? dims.vol[b2] : 1;
+        const char* pb = (const char*) V.base + poff;
+        const float q1 = __builtin_fmaf(fs, r.s1, r.p1);
+        const float g1 = floorf(q1 - 0.5f);
+        const int k1 = pad_clamp(g1, n1);
+        SampleTaps64 T;
+        T.c1 = (q1 - g1) - 0.5f;
+        if (D == 2) {
+            const Pair64 fk = ld_view2_f64(V, pb, ((uint32_t) k1 << 3) + 16u);
+            T.v00 = fk.a;
+            T.v10 = fk.b;
+            T.v01 = T.v11 = 0.0;
+            T.c2 = 0.0f;
+        } else {
+            const float q2 = __builtin_fmaf(fs, r.s2, r.p2);
+            const float g2 = floorf(q2 - 0.5f);
+            const int k2 = pad_clamp(g2, n2);
+            T.c2 = (q2 - g2) - 0.5f;
+            const int ks = FB2 ? k1 : k2, kf = FB2 ? k2 : k1;
+            const uint32_t o = (uint32_t) (smul24(ks, (int) V.RS4) + (int) (2u * V.RS4 + 16u)) + ((uint32_t) kf << 3);
+            const Pair64 r0 = ld_view2_f64(V, pb, o);
+            const Pair64 r1 = ld_view2_f64(V, pb + V.RS4, o);
+            T.v00 = r0.a;
+            T.v10 = FB2 ? r1.a : r0.b;
+            T.v01 = FB2 ? r0.b : r1.a;
+            T.v11 = r1.b;
+        }
+        return T;
+    }
+
+    // acc plus one FP64 sample as elsa sums it (JosephsMethod.cpp:60-97 with data_t = double):
+    // (value * len) * w per tap in doInterpolation's tap order, w the FP32 product of the
+    // transverse weights (the driving axis' weight is 1). 2D taps (F0, C1), (C0, F1); 3D (F0,
+    // F1, F2), (F0, C1, C2), (C0, F1, C2), (C0, C1, F2), the driving axis' F = C being its plane.
+    template <int D, int A>
+    EHP_HD inline double plane_taps_add64(const SampleTaps64& T, double len, double acc)
+    {
+        const float w1 = 1.0f - T.c1;
+        auto tap = [&](double v, float w) { acc = acc + (v * len) * (double) w; };
+        if (D == 2) {
+            if (A == 0) { // (plane, C1), (plane, F1)
+                tap(T.v10, T.c1);
+                tap(T.v00, w1);
+            } else { // (F0, plane), (C0, plane)
+                tap(T.v00, w1);
+                tap(T.v10, T.c1);
+            }
+            return acc;
+        }
+        const float w2 = 1.0f - T.c2;
+        if (A == 0) { // b1 = 1, b2 = 2: (F1, F2), (C1, C2), (F1, C2), (C1, F2)
+            tap(T.v00, w1 * w2);
+            tap(T.v11, T.c1 * T.c2);
+            tap(T.v01, w1 * T.c2);
+            tap(T.v10, T.c1 * w2);
+        } else if (A == 1) { // b1 = 0, b2 = 2: (F0, F2), (F0, C2), (C0, C2), (C0, F2)
+            tap(T.v00, w1 * w2);
+            tap(T.v01, w1 * T.c2);
+            tap(T.v11, T.c1 * T.c2);
+            tap(T.v10, T.c1 * w2);
+        } else { // b1 = 0, b2 = 1: (F0, F1), (F0, C1), (C0, F1), (C0, C1)
+            tap(T.v00, w1 * w2);
+            tap(T.v01, w1 * T.c2);
+            tap(T.v10, T.c1 * w2);
+            tap(T.v11, T.c1 * T.c2);
+        }
+        return acc;
+    }
+
     // Sample s (fs = s) of a ray (driving axis A) on the plane at byte offset poff of its padded
     // view, added to acc; FB2 says that the fast in-plane axis of the view is b2 (else b1). The
     // interpolation is elsa's doInterpolation (JosephsMethod.cpp:60-97: bilinear in the
     // transverse axes with floor F = floor(q - 0.5) and fraction q - 0.5 - F, the driving axis on
-    // its plane, taps outside the volume 0) in lerp form, which needs 7 operations per 3D sample
-    // where elsa's weight products need 16, summed in FP64: elsa's sums to FP32 rounding (not
-    // bit for bit). C = F + 1 is read even where elsa's ceil gives C = F: its fraction is 0 then.
-    // Positions and floors are those of joseph_sample and tile_ray_adjoint, bit for bit.
-    template <int D, int A, bool FB2, bool V16, bool PR = false>
-    EHP_HD inline double plane_sample_forward(const RayRec& r, const Dims& dims, const PadView& V, uint32_t poff,
-                                              float fs, double len, double acc)
+    // its plane, taps outside the volume 0). FP32 and bfloat16 views (F64 = false): in lerp form,
+    // which needs 7 operations per 3D sample where elsa's weight products need 16, summed in
+    // FP64: elsa's sums to FP32 rounding (not bit for bit). FP64 views (F64): elsa's weights and
+    // sums, plane_taps_add64. C = F + 1 is read even where elsa's ceil gives C = F: its weight is
+    // 0 then. Positions and floors are those of joseph_sample and the tiled back projection, bit
+    // for bit.
+    template <int D, int A, bool FB2, bool V16, bool PR, bool F64>
+    EHP_HD inline double plane_sample_add(const RayRec& r, const Dims& dims, const PadView& V, uint32_t poff,
+                                          float fs, double len, double acc)
     {
-        return plane_taps_add<D>(plane_sample_taps<D, A, FB2, V16, PR>(r, dims, V, poff, fs), len, acc);
+        if constexpr (F64)
+            return plane_taps_add64<D, A>(plane_sample_taps64<D, A, FB2>(r, dims, V, poff, fs), len, acc);
+        else
+            return plane_taps_add<D>(plane_sample_taps<D, A, FB2, V16, PR>(r, dims, V, poff, fs), len, acc);
     }
 
-    // Forward projection of one ray (driving axis A) through its padded view: the sum of
-    // plane_sample_forward over its samples in the volume, in traversal order.
-    template <int D, int A, bool FB2, bool V16 = false, bool PR = false>
-    EHP_HD inline float plane_ray_forward(const RayRec& r, const Dims& dims, const PadView& V)
+    // Forward projection of one ray (driving axis A) through its padded view: the sum of its
+    // samples in the volume (plane_sample_add), in traversal order, in FP64 (the FP32 and
+    // bfloat16 operators round it once).
+    template <int D, int A, bool FB2, bool V16 = false, bool PR = false, bool F64 = false>
+    EHP_HD inline double plane_ray_forward(const RayRec& r, const Dims& dims, const PadView& V)
     {
         int s0, s1;
         plane_sample_range(r, dims.vol[A], s0, s1);
         double acc = 0.0;
         if (s0 >= s1)
-            return 0.0f;
+            return 0.0;
         const double len = r.len;
         const bool neg = (r.meta & 4) != 0;
         uint32_t poff = (uint32_t) (neg ? r.P0 - s0 : r.P0 + s0) * V.PS4;
         const uint32_t dp = neg ? 0u - V.PS4 : V.PS4;
         for (int s = s0; s < s1; ++s, poff += dp)
-            acc = plane_sample_forward<D, A, FB2, V16, PR>(r, dims, V, poff, (float) s, len, acc);
-        return (float) acc;
+            acc = plane_sample_add<D, A, FB2, V16, PR, F64>(r, dims, V, poff, (float) s, len, acc);
+        return acc;
     }
 
     // A ray's samples in traversal coordinates, t = plane when it steps towards higher planes and
@@ -1112,9 +1218,9 @@ namespace ehp
     // iterations [full0, full1) every ray of the group has a sample (full0 = max tfirst - t0,
     // full1 = max(full0, min tlast - t0 + 1)), so they need no test. Same samples, positions
     // and order as plane_ray_forward, so the same sum bit for bit.
-    template <int D, int A, bool NEG, bool FB2, bool V16 = false, bool PR = false>
-    EHP_HD inline float plane_span_forward(const RayRec& r, const Dims& dims, const PadView& V, int t0, int niter,
-                                           int s_first, int full0, int full1)
+    template <int D, int A, bool NEG, bool FB2, bool V16 = false, bool PR = false, bool F64 = false>
+    EHP_HD inline double plane_span_forward(const RayRec& r, const Dims& dims, const PadView& V, int t0, int niter,
+                                            int s_first, int full0, int full1)
     {
         int s0, s1;
         plane_sample_range(r, dims.vol[A], s0, s1);
@@ -1128,21 +1234,28 @@ namespace ehp
         int it = 0;
         for (; it < full0; ++it, ++j, fs += 1.0f, poff += dp)
             if (j < n)
-                acc = plane_sample_forward<D, A, FB2, V16, PR>(r, dims, V, poff, fs, len, acc);
+                acc = plane_sample_add<D, A, FB2, V16, PR, F64>(r, dims, V, poff, fs, len, acc);
         // two samples per step, both fetched before either is added, so that each wavefront
         // has four loads in flight
         for (; it + 1 < full1; it += 2, fs += 2.0f, poff += 2u * dp) {
-            const SampleTaps Ta = plane_sample_taps<D, A, FB2, V16, PR>(r, dims, V, poff, fs);
-            const SampleTaps Tb = plane_sample_taps<D, A, FB2, V16, PR>(r, dims, V, poff + dp, fs + 1.0f);
-            acc = plane_taps_add<D>(Ta, len, acc);
-            acc = plane_taps_add<D>(Tb, len, acc);
+            if constexpr (F64) {
+                const SampleTaps64 Ta = plane_sample_taps64<D, A, FB2>(r, dims, V, poff, fs);
+                const SampleTaps64 Tb = plane_sample_taps64<D, A, FB2>(r, dims, V, poff + dp, fs + 1.0f);
+                acc = plane_taps_add64<D, A>(Ta, len, acc);
+                acc = plane_taps_add64<D, A>(Tb, len, acc);
+            } else {
+                const SampleTaps Ta = plane_sample_taps<D, A, FB2, V16, PR>(r, dims, V, poff, fs);
+                const SampleTaps Tb = plane_sample_taps<D, A, FB2, V16, PR>(r, dims, V, poff + dp, fs + 1.0f);
+                acc = plane_taps_add<D>(Ta, len, acc);
+                acc = plane_taps_add<D>(Tb, len, acc);
+            }
         }
         for (; it < full1; ++it, fs += 1.0f, poff += dp)
-            acc = plane_sample_forward<D, A, FB2, V16, PR>(r, dims, V, poff, fs, len, acc);
+            acc = plane_sample_add<D, A, FB2, V16, PR, F64>(r, dims, V, poff, fs, len, acc);
         j += (uint32_t) (full1 - full0);
         for (; it < niter; ++it, ++j, fs += 1.0f, poff += dp)
             if (j < n)
-                acc = plane_sample_forward<D, A, FB2, V16, PR>(r, dims, V, poff, fs, len, acc);
-        return (float) acc;
+                acc = plane_sample_add<D, A, FB2, V16, PR, F64>(r, dims, V, poff, fs, len, acc);
+        return acc;
     }
 } // namespace ehp

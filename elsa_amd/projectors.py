@@ -213,9 +213,10 @@ class JosephsMethodCUDA(_HipProjector):
     ``kernels="fast"`` (FP32) runs the forward over plane-major volume views and the back
     projection over LDS-privatised tiles with 64-bit fixed-point sums; ``"simple"`` runs one
     thread per ray for both (the back projection with float atomics). FP64 with
-    ``kernels="fast"`` runs the one-thread-per-ray forward and the LDS-tiled back projection,
-    each FP64 contribution rounded to a multiple of a per-call unit and summed in 64-bit
-    integers (bitwise reproducible); ``"simple"`` its FP64-atomic back projection."""
+    ``kernels="fast"`` runs the forward over FP64 plane-major views (elsa's weights and sums, bit
+    for bit) and the LDS-tiled back projection, each FP64 contribution rounded to a multiple of a
+    per-call unit and summed in 64-bit integers (bitwise reproducible); ``"simple"`` one thread
+    per ray, with an FP64-atomic back projection."""
 
     def __init__(self, domainDescriptor, rangeDescriptor, fast: bool = True, dtype=np.float32, kernels: str = "fast"):
         self.fast = bool(fast)

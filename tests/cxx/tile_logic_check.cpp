@@ -325,9 +325,15 @@ struct Span {
     int t0, niter, s_first, full0, full1;
 };
 
-template <int D, int A, bool NEG>
-static float span_one(const RayRec& r, const Dims& dims, const PadView& V, bool fb2, bool b16, const Span& q)
+// F64: an FP64 view (one element per voxel) and elsa's FP64 sums (plane_taps_add64)
+template <int D, int A, bool NEG, bool F64>
+static double span_one(const RayRec& r, const Dims& dims, const PadView& V, bool fb2, bool b16, const Span& q)
 {
+    if (F64)
+        return fb2 ? plane_span_forward<D, A, NEG, true, false, false, true>(r, dims, V, q.t0, q.niter, q.s_first,
+                                                                             q.full0, q.full1)
+                   : plane_span_forward<D, A, NEG, false, false, false, true>(r, dims, V, q.t0, q.niter, q.s_first,
+                                                                              q.full0, q.full1);
     if (D == 3 && g_pairs && b16)
         return fb2 ? plane_span_forward<D, A, NEG, true, true, true>(r, dims, V, q.t0, q.niter, q.s_first, q.full0,
                                                                      q.full1)
@@ -346,23 +352,23 @@ static float span_one(const RayRec& r, const Dims& dims, const PadView& V, bool 
                : plane_span_forward<D, A, NEG, false>(r, dims, V, q.t0, q.niter, q.s_first, q.full0, q.full1);
 }
 
-template <int D>
-static float span_key(int key, const RayRec& r, const Dims& dims, const PadView* pv, const bool* fb2, bool b16,
-                      const Span& q)
+template <int D, bool F64>
+static double span_key(int key, const RayRec& r, const Dims& dims, const PadView* pv, const bool* fb2, bool b16,
+                       const Span& q)
 {
     switch (key) {
     case 0:
-        return span_one<D, 0, false>(r, dims, pv[0], fb2[0], b16, q);
+        return span_one<D, 0, false, F64>(r, dims, pv[0], fb2[0], b16, q);
     case 1:
-        return span_one<D, 0, true>(r, dims, pv[0], fb2[0], b16, q);
+        return span_one<D, 0, true, F64>(r, dims, pv[0], fb2[0], b16, q);
     case 2:
-        return span_one<D, 1, false>(r, dims, pv[1], fb2[1], b16, q);
+        return span_one<D, 1, false, F64>(r, dims, pv[1], fb2[1], b16, q);
     case 3:
-        return span_one<D, 1, true>(r, dims, pv[1], fb2[1], b16, q);
+        return span_one<D, 1, true, F64>(r, dims, pv[1], fb2[1], b16, q);
     case 4:
-        return span_one<D, D - 1, false>(r, dims, pv[D - 1], fb2[D - 1], b16, q);
+        return span_one<D, D - 1, false, F64>(r, dims, pv[D - 1], fb2[D - 1], b16, q);
     default:
-        return span_one<D, D - 1, true>(r, dims, pv[D - 1], fb2[D - 1], b16, q);
+        return span_one<D, D - 1, true, F64>(r, dims, pv[D - 1], fb2[D - 1], b16, q);
     }
 }
 
@@ -371,9 +377,9 @@ static float span_key(int key, const RayRec& r, const Dims& dims, const PadView*
 // group's span (plane_ray_span, plane_span_forward). Pixels past the detector edge join no group.
 static int64_t g_mixed_waves = 0; // wavefronts of the last plane_forward call with more than one group
 
-template <int D>
+template <int D, bool F64 = false>
 static void wave_forward(const Dims& dims, const PoseGeom& g, int u0, int v0, int wu, const PadView* pv,
-                         const bool* fb2, bool b16, float* out)
+                         const bool* fb2, bool b16, double* out)
 {
     RayRec rr[64];
     int key[64], tf[64], tl[64], s0[64];
@@ -389,7 +395,7 @@ static void wave_forward(const Dims& dims, const PoseGeom& g, int u0, int v0, in
         rr[l] = make_ray_rec<D>(jr);
         if (plane_ray_span(rr[l], dims.vol[jr.axis], tf[l], tl[l], s0[l]))
             key[l] = 2 * jr.axis + ((rr[l].meta & 4) ? 1 : 0);
-        out[l] = 0.0f;
+        out[l] = 0.0;
     }
     int groups = 0;
     for (int k = 0; k < 2 * D; ++k) {
@@ -407,7 +413,8 @@ static void wave_forward(const Dims& dims, const PoseGeom& g, int u0, int v0, in
         const int full0 = ta - t0, full1 = tb - t0 + 1 > full0 ? tb - t0 + 1 : full0;
         for (int l = 0; l < 64; ++l)
             if (key[l] == k)
-                out[l] = span_key<D>(k, rr[l], dims, pv, fb2, b16, Span{t0, t1 - t0 + 1, s0[l] + t0 - tf[l], full0, full1});
+                out[l] = span_key<D, F64>(k, rr[l], dims, pv, fb2, b16,
+                                          Span{t0, t1 - t0 + 1, s0[l] + t0 - tf[l], full0, full1});
     }
     g_mixed_waves += groups > 1;
 }
@@ -480,7 +487,7 @@ extern "C" int plane_forward(int dim, const int64_t* vol_dims, const int64_t* de
         if (wu < 1 || 64 % wu != 0)
             return 23;
         const int wv = 64 / wu;
-        float lane[64];
+        double lane[64];
         for (int p = 0; p < nposes; ++p)
             for (int v0 = 0; v0 < dims.det[1]; v0 += wv)
                 for (int u0 = 0; u0 < dims.det[0]; u0 += wu) {
@@ -491,7 +498,7 @@ extern "C" int plane_forward(int dim, const int64_t* vol_dims, const int64_t* de
                     for (int l = 0; l < 64; ++l) {
                         const int u = u0 + l % wu, v = v0 + l / wu;
                         if (u < dims.det[0] && v < dims.det[1])
-                            sino[(int64_t) p * dims.pix_per_pose + (int64_t) v * dims.det[0] + u] = lane[l];
+                            sino[(int64_t) p * dims.pix_per_pose + (int64_t) v * dims.det[0] + u] = (float) lane[l];
                     }
                 }
         return g_view_oob ? 22 : 0;
@@ -515,6 +522,102 @@ extern "C" int plane_forward(int dim, const int64_t* vol_dims, const int64_t* de
                     val = jr.axis == 0   ? fwd_one<3, 0>(r, dims, pv[0], fb2[0], b16)
                           : jr.axis == 1 ? fwd_one<3, 1>(r, dims, pv[1], fb2[1], b16)
                                          : fwd_one<3, 2>(r, dims, pv[2], fb2[2], b16);
+                }
+                sino[(int64_t) p * dims.pix_per_pose + (int64_t) v * dims.det[0] + u] = val;
+            }
+    return g_view_oob ? 22 : 0;
+}
+
+// FP64 operators: the forward through FP64 views (one element per voxel, pad_store's layout),
+// per ray (waves = 0) or with the GPU's wavefront grouping of wu x (64 / wu) pixels (waves = wu),
+// with every 16-byte read checked against its view.
+extern "C" int plane_forward64(int dim, const int64_t* vol_dims, const int64_t* det_dims, int nposes, const float* P,
+                               const float* Pinv, const float* C, const int* fan, const double* vol, double* sino,
+                               int waves)
+{
+    const Dims dims = make_dims(dim, vol_dims, det_dims, nposes);
+    const int nm = dim * (dim + 1);
+    std::vector<PoseGeom> geo(nposes);
+    for (int p = 0; p < nposes; ++p) {
+        PoseGeom g{};
+        for (int i = 0; i < nm; ++i) {
+            g.P[i] = P[p * nm + i];
+            g.Pinv[i] = Pinv[p * nm + i];
+        }
+        for (int i = 0; i < dim; ++i)
+            g.C[i] = C[p * dim + i];
+        geo[p] = g;
+    }
+    g_view_oob = false;
+    std::vector<double> views[3];
+    PadView pv[3];
+    bool fb2[3] = {false, false, false};
+    for (int a = 0; a < dim; ++a) {
+        if (fan[a] == a || fan[a] < 0 || fan[a] > (dim == 3 ? 2 : 1))
+            return 20;
+        const PadLayout L = pad_layout(dims, a, fan[a], false);
+        views[a].assign(L.size, 0.0);
+        std::vector<char> seen(L.size, 0);
+        for (int z = 0; z < dims.vol[2]; ++z)
+            for (int y = 0; y < dims.vol[1]; ++y)
+                for (int x = 0; x < dims.vol[0]; ++x) {
+                    const int64_t o = pad_index(L, x, y, z);
+                    if (o < 0 || o >= L.size || seen[o])
+                        return 21;
+                    seen[o] = 1;
+                    pad_store<double>(L, views[a].data(), x, y, z, vol[x + dims.strides[1] * y + dims.strides[2] * z]);
+                }
+        pv[a] = PadView{views[a].data(), L.PS * 8u, L.RS * 8u, L.size};
+        fb2[a] = dim == 3 && fan[a] == (a == 2 ? 1 : 2);
+    }
+    if (waves) {
+        const int wu = dim == 2 ? 64 : waves;
+        if (wu < 1 || 64 % wu != 0)
+            return 23;
+        const int wv = 64 / wu;
+        double lane[64];
+        for (int p = 0; p < nposes; ++p)
+            for (int v0 = 0; v0 < dims.det[1]; v0 += wv)
+                for (int u0 = 0; u0 < dims.det[0]; u0 += wu) {
+                    if (dim == 2)
+                        wave_forward<2, true>(dims, geo[p], u0, v0, wu, pv, fb2, false, lane);
+                    else
+                        wave_forward<3, true>(dims, geo[p], u0, v0, wu, pv, fb2, false, lane);
+                    for (int l = 0; l < 64; ++l) {
+                        const int u = u0 + l % wu, v = v0 + l / wu;
+                        if (u < dims.det[0] && v < dims.det[1])
+                            sino[(int64_t) p * dims.pix_per_pose + (int64_t) v * dims.det[0] + u] = lane[l];
+                    }
+                }
+        return g_view_oob ? 22 : 0;
+    }
+    for (int p = 0; p < nposes; ++p)
+        for (int v = 0; v < dims.det[1]; ++v)
+            for (int u = 0; u < dims.det[0]; ++u) {
+                float o[3], d[3];
+                JRay jr;
+                double val;
+                if (dim == 2) {
+                    make_ray<2>(geo[p], u, v, o, d);
+                    joseph_setup<2>(o, d, dims.vol, jr);
+                    const RayRec r = make_ray_rec<2>(jr);
+                    val = jr.axis == 0 ? plane_ray_forward<2, 0, false, false, false, true>(r, dims, pv[0])
+                                       : plane_ray_forward<2, 1, false, false, false, true>(r, dims, pv[1]);
+                } else {
+                    make_ray<3>(geo[p], u, v, o, d);
+                    joseph_setup<3>(o, d, dims.vol, jr);
+                    const RayRec r = make_ray_rec<3>(jr);
+                    const int a = jr.axis;
+                    const PadView& V = pv[a];
+                    if (a == 0)
+                        val = fb2[0] ? plane_ray_forward<3, 0, true, false, false, true>(r, dims, V)
+                                     : plane_ray_forward<3, 0, false, false, false, true>(r, dims, V);
+                    else if (a == 1)
+                        val = fb2[1] ? plane_ray_forward<3, 1, true, false, false, true>(r, dims, V)
+                                     : plane_ray_forward<3, 1, false, false, false, true>(r, dims, V);
+                    else
+                        val = fb2[2] ? plane_ray_forward<3, 2, true, false, false, true>(r, dims, V)
+                                     : plane_ray_forward<3, 2, false, false, false, true>(r, dims, V);
                 }
                 sino[(int64_t) p * dims.pix_per_pose + (int64_t) v * dims.det[0] + u] = val;
             }
