@@ -420,8 +420,8 @@ namespace
         return EHP_OK;
     }
 
-    // FP32 Siddon: when the fixed-point bound exists, the per-ray table, the driving axes per
-    // pose, plane-major views for the forward and the LDS-tiled back projection.
+    // Siddon (FP32, FP64): when the fixed-point bound exists, the per-ray table, the driving axes
+    // per pose and the LDS-tiled back projection; FP32 also the plane-major views of the forward.
     int setup_siddon_tiled(ehp_op* op)
     {
         const ehp::Dims& dm = op->dims;
@@ -438,7 +438,7 @@ namespace
             return fail(EHP_ERR_ALLOC, "hipMalloc of the %zu-byte ray table failed: %s", tbytes, hipGetErrorString(e));
         e = hipMalloc((void**) &op->d_pose_axes, n * sizeof(int));
         if (e == hipSuccess)
-            e = hipMalloc((void**) &op->d_maxbits, ehp::kAbsmaxBlocks * sizeof(unsigned int));
+            e = hipMalloc((void**) &op->d_maxbits, ehp::kAbsmaxBlocks * sizeof(unsigned long long));
         if (e == hipSuccess)
             e = hipMalloc((void**) &op->d_fixexp, 2 * sizeof(int));
         if (e != hipSuccess)
@@ -452,13 +452,16 @@ namespace
         op->axes_present = 0;
         for (int v : h)
             op->axes_present |= v;
-        // forward: the plane-major views of the Joseph forward, walked plane by plane
-        const int rc = setup_views(op, h);
-        if (rc != EHP_OK)
-            return rc;
-        e = hipDeviceSynchronize(); // the views' padding is cleared on the null stream
-        if (e != hipSuccess)
-            return hip_fail(e, "volume view");
+        // FP32 forward: the plane-major views of the Joseph forward, walked plane by plane (FP64
+        // keeps the one-thread-per-ray forward)
+        if (op->dtype == EHP_F32) {
+            const int rc = setup_views(op, h);
+            if (rc != EHP_OK)
+                return rc;
+            e = hipDeviceSynchronize(); // the views' padding is cleared on the null stream
+            if (e != hipSuccess)
+                return hip_fail(e, "volume view");
+        }
         op->slot_bound = bound;
         int max_lds = 65536;
         if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, op->device) != hipSuccess)
@@ -511,7 +514,7 @@ namespace
         if (e == hipSuccess)
             e = hipMalloc((void**) &op->d_pose_margin, n * sizeof(float));
         if (e == hipSuccess)
-            e = hipMalloc((void**) &op->d_maxbits, ehp::kAbsmaxBlocks * sizeof(unsigned int));
+            e = hipMalloc((void**) &op->d_maxbits, ehp::kAbsmaxBlocks * sizeof(unsigned long long));
         if (e == hipSuccess)
             e = hipMalloc((void**) &op->d_fixexp, 2 * sizeof(int));
         if (e != hipSuccess)
@@ -558,7 +561,7 @@ int ehp_siddon_create(const ehp_geometry* g, int dtype, int flags, ehp_op** out)
     if (rc != EHP_OK)
         return rc;
     rc = count_samples(*out);
-    if (rc == EHP_OK && dtype == EHP_F32 && !(flags & EHP_SIMPLE_KERNELS))
+    if (rc == EHP_OK && (dtype == EHP_F32 || dtype == EHP_F64) && !(flags & EHP_SIMPLE_KERNELS))
         rc = setup_siddon_tiled(*out);
     if (rc != EHP_OK) {
         ehp_destroy(*out);
@@ -684,9 +687,9 @@ int ehp_adjoint_noalloc(ehp_op* op, const void* d_sino, void* d_vol, void* strea
     if (d_sino == d_vol)
         return fail(EHP_ERR_ARGUMENT, "input and output must not alias");
     const bool joseph = op->kind == KIND_JOSEPH && (op->dtype == EHP_F32 || op->dtype == EHP_F64);
-    const bool siddon = op->kind == KIND_SIDDON && op->dtype == EHP_F32;
+    const bool siddon = op->kind == KIND_SIDDON && (op->dtype == EHP_F32 || op->dtype == EHP_F64);
     if (!op->tiled || !(joseph || siddon))
-        return fail(EHP_ERR_UNSUPPORTED, "ehp_adjoint_noalloc takes FP32/FP64 Joseph and FP32 Siddon operators "
+        return fail(EHP_ERR_UNSUPPORTED, "ehp_adjoint_noalloc takes FP32/FP64 Joseph and Siddon operators "
                                          "with a tiled back projection only");
     // the scratch this call needs: the partial maxima of at most kAbsmaxBlocks blocks and the exponent
     const size_t need = kScaleOffset + 2 * sizeof(int);
@@ -712,10 +715,14 @@ int ehp_adjoint_noalloc(ehp_op* op, const void* d_sino, void* d_vol, void* strea
     if (e != hipSuccess)
         return hip_fail(e, "ordering after the previous call on this operator");
     const ehp::Dims& dm = op->dims;
-    if (siddon)
-        e = ehp::siddon_adjoint_tiled(op->d_geo, op->d_sidtable, dm, op->d_pose_axes, op->axes_present, op->tile,
-                                      op->batch, op->slot_bound, d_maxbits, d_fixexp, (const float*) d_sino,
-                                      (float*) d_vol, s);
+    if (siddon && op->dtype == EHP_F32)
+        e = ehp::siddon_adjoint_tiled<float>(op->d_geo, op->d_sidtable, dm, op->d_pose_axes, op->axes_present,
+                                             op->tile, op->batch, op->slot_bound, d_maxbits, d_fixexp,
+                                             (const float*) d_sino, (float*) d_vol, s);
+    else if (siddon)
+        e = ehp::siddon_adjoint_tiled<double>(op->d_geo, op->d_sidtable, dm, op->d_pose_axes, op->axes_present,
+                                              op->tile, op->batch, op->slot_bound, d_maxbits, d_fixexp,
+                                              (const double*) d_sino, (double*) d_vol, s);
     else if (op->dtype == EHP_F32)
         e = ehp::joseph_adjoint_tiled<float>(op->d_geo, op->d_table, dm, op->d_pose_axes, op->axes_present, op->tile,
                                              op->batch, op->slot_bound, d_maxbits, d_fixexp, (const float*) d_sino,
@@ -905,6 +912,10 @@ static hipError_t launch(ehp_op* op, const void* in, void* out, hipStream_t s, b
             e = ehp::joseph_forward_planes<double>(op->d_geo, dm, op->layouts, op->d_view, op->axes_present, x, y, s);
         else if (op->kind == KIND_JOSEPH)
             e = adjoint ? ehp::joseph_adjoint<double>(op->d_geo, dm, x, y, s) : ehp::joseph_forward<double>(op->d_geo, dm, x, y, s);
+        else if (op->kind == KIND_SIDDON && adjoint && op->tiled)
+            e = ehp::siddon_adjoint_tiled<double>(op->d_geo, op->d_sidtable, dm, op->d_pose_axes, op->axes_present,
+                                                  op->tile, op->batch, op->slot_bound, op->d_maxbits, op->d_fixexp, x,
+                                                  y, s);
         else if (op->kind == KIND_SIDDON)
             e = adjoint ? ehp::siddon_adjoint<double>(op->d_geo, dm, x, y, s) : ehp::siddon_forward<double>(op->d_geo, dm, x, y, s);
         else

@@ -213,10 +213,10 @@ namespace ehp
     hipError_t launch_to_bf16(const float* src, bf16* dst, int64_t n, hipStream_t s);
     hipError_t siddon_ray_table(const PoseGeom* geo, const Dims& dims, SidRec* table, int* pose_axes, hipStream_t s);
     size_t siddon_tile_lds_bytes(const Dims& dims, const TileCfg& tc, int batch = kMaxPoseBatch);
+    template <class T>
     hipError_t siddon_adjoint_tiled(const PoseGeom* geo, const SidRec* table, const Dims& dims, const int* pose_axes,
                                     int axes_present, const TileCfg& tc, int batch, double slot_bound,
-                                    unsigned int* d_maxbits, int* d_fixexp, const float* sino, float* vol,
-                                    hipStream_t s);
+                                    unsigned int* d_maxbits, int* d_fixexp, const T* sino, T* vol, hipStream_t s);
     size_t voxel_tile_lds_bytes(const TileCfg& tc);
     int voxel_max_lut();
     template <class ST>

@@ -1,4 +1,4 @@
-// ehp_siddon_tiled.hip -- Siddon back projection with LDS-privatised volume tiles (FP32).
+// ehp_siddon_tiled.hip -- Siddon back projection with LDS-privatised volume tiles (FP32, FP64).
 //
 // The adjoint of Siddon's method (elsa/projectors/SiddonsMethod.cpp:76-124, adjoint
 // :119-120) without global atomics, on the scheme of the tiled Joseph back projection: the
@@ -9,7 +9,8 @@
 // boundary, so every crossing -- voxel and length -- is the one the forward visits.
 //
 // Sums are 64-bit fixed point as in the Joseph kernel: one exponent per call from max|y|
-// and the slot bound, each ray's contributions rounded to int32 in the ray's own scale.
+// and the slot bound, each contribution y * length rounded once to the unit (fixed_round; in
+// FP32 the product is exact in FP64).
 #include "ehp_kernels.hpp"
 
 namespace ehp
@@ -65,14 +66,15 @@ namespace ehp
         return (size_t) tc.LA * tc.TB * (dims.dim == 3 ? tc.TC : 1) * sizeof(long long) + batch_lds_bytes(batch);
     }
 
-    // One tile of the back projection along driving axis A (tile < ntA * ntB * ntC).
-    template <int D, int A>
+    // One tile of the back projection along driving axis A (tile < ntA * ntB * ntC); T = float or
+    // double (sinogram and volume).
+    template <int D, int A, class T>
     __device__ __forceinline__ void sid_adj_tiled_block(const PoseGeom* __restrict__ geo,
                                                         const SidRec* __restrict__ table, const Dims& dims,
                                                         const int* __restrict__ pose_axes, const TileCfg& tc,
                                                         const int ntA, const int ntB, const int tile, const int batch,
-                                                        const int* __restrict__ fixexp,
-                                                        const float* __restrict__ sino, float* __restrict__ vol)
+                                                        const int* __restrict__ fixexp, const T* __restrict__ sino,
+                                                        T* __restrict__ vol)
     {
         constexpr int b1 = TileAxes<D, A>::b1, b2 = TileAxes<D, A>::b2;
         extern __shared__ long long lds[];
@@ -137,7 +139,7 @@ namespace ehp
                 rect_split(k - s_pref[j], cr.nu, cr.rnu, row, col);
                 return cr.base + (int64_t) (mul24((uint32_t) row, (uint32_t) dims.det[0]) + (uint32_t) col);
             };
-            float yv_next = 0.0f;
+            T yv_next = T(0);
             SidRec r_next{};
             if (tid < total) {
                 const int64_t ray = cand_ray(tid);
@@ -145,24 +147,23 @@ namespace ehp
                 r_next = table[ray];
             }
             for (int k = tid; k < total; k += kThreads) {
-                const float yv = yv_next;
+                const T yv = yv_next;
                 const SidRec r = r_next;
                 if (k + kThreads < total) {
                     const int64_t ray = cand_ray(k + kThreads);
                     yv_next = sino[ray];
                     r_next = table[ray];
                 }
-                if (yv == 0.0f || sid_axis(r) != A)
+                if (yv == T(0) || sid_axis(r) != A)
                     continue;
-                // lengths are at most sqrt(3) < 2: scale 2 y, contribute (ys) (w / 2)
-                float ys;
-                int sh;
-                ray_scale(yv * 2.0f, eg, ys, sh);
+                // y in units of q (a power-of-two scaling, exact); each crossing adds
+                // y * length rounded once to an integer (|y * length| < 2^51 units: the slot
+                // bound is at least 4096 and lengths are below 2)
+                const double yq = ldexp((double) yv, 62 - eg);
                 siddon_box<D>(r, lo, hi, [&](int c0, int c1, int c2, float w) {
                     if (w != 0.0f) {
                         const int slot = sid_slot<D, A>(t, tc, c0, c1, c2);
-                        atomicAdd(reinterpret_cast<unsigned long long*>(iacc + slot),
-                                  fixed_contrib(ys * (w * 0.5f), sh));
+                        atomicAdd(reinterpret_cast<unsigned long long*>(iacc + slot), fixed_round(yq, (double) w));
                     }
                 });
             }
@@ -186,23 +187,22 @@ namespace ehp
             if (D == 3)
                 c[b2] = cc;
             const int64_t lin = c[0] + c[1] * dims.strides[1] + (int64_t) c[2] * dims.strides[2];
-            vol[lin] += (float) ((double) iacc[i] * q);
+            vol[lin] += (T) ((double) iacc[i] * q);
         }
     }
 
-    template <int D, int A>
+    template <int D, int A, class T>
     __global__ void __launch_bounds__(kThreads)
         siddon_adj_tiled_kernel(const PoseGeom* __restrict__ geo, const SidRec* __restrict__ table, const Dims dims,
                                 const int* __restrict__ pose_axes, const TileCfg tc, const int ntA, const int ntB,
                                 const int ntC, const int tiles_per_xcd, const int batch,
-                                const int* __restrict__ fixexp, const float* __restrict__ sino,
-                                float* __restrict__ vol)
+                                const int* __restrict__ fixexp, const T* __restrict__ sino, T* __restrict__ vol)
     {
         // consecutive workgroups land on different XCDs: give each XCD a contiguous block of tiles
         const int tile = (int) (blockIdx.x % 8u) * tiles_per_xcd + (int) (blockIdx.x / 8u);
         if (tile >= ntA * ntB * ntC)
             return;
-        sid_adj_tiled_block<D, A>(geo, table, dims, pose_axes, tc, ntA, ntB, tile, batch, fixexp, sino, vol);
+        sid_adj_tiled_block<D, A, T>(geo, table, dims, pose_axes, tc, ntA, ntB, tile, batch, fixexp, sino, vol);
     }
 
     // the tiles of axis A of many operators in one launch (block b: tile b - job.block0 of job
@@ -217,18 +217,19 @@ namespace ehp
         if (id >= total)
             return;
         const SidAdjJob& J = jobs[block_job[id]];
-        sid_adj_tiled_block<D, A>(J.geo, J.table, J.dims, J.pose_axes, J.tc, J.ntA, J.ntB, (int) (id - J.block0),
-                                  J.batch, fixexp + J.item, sinos + J.sino_off, vols + J.vol_off);
+        sid_adj_tiled_block<D, A, float>(J.geo, J.table, J.dims, J.pose_axes, J.tc, J.ntA, J.ntB,
+                                         (int) (id - J.block0), J.batch, fixexp + J.item, sinos + J.sino_off,
+                                         vols + J.vol_off);
     }
 
+    template <class T>
     hipError_t siddon_adjoint_tiled(const PoseGeom* geo, const SidRec* table, const Dims& dims, const int* pose_axes,
                                     int axes_present, const TileCfg& tc, int batch, double slot_bound,
-                                    unsigned int* d_maxbits, int* d_fixexp, const float* sino, float* vol,
-                                    hipStream_t s)
+                                    unsigned int* d_maxbits, int* d_fixexp, const T* sino, T* vol, hipStream_t s)
     {
         if (batch < 1 || batch > kMaxPoseBatch)
             return hipErrorInvalidValue;
-        hipError_t e = launch_fill<float>(vol, dims.nvox, 0.0f, s);
+        hipError_t e = launch_fill<T>(vol, dims.nvox, T(0), s);
         if (e == hipSuccess)
             e = fixed_point_exponent_launch(sino, dims.nrays, slot_bound, d_maxbits, d_fixexp, s);
         if (e != hipSuccess)
@@ -242,8 +243,8 @@ namespace ehp
             const int per_xcd = (ntA * ntB * ntC + 7) / 8;
             const dim3 grid((unsigned) (per_xcd * 8));
 #define EHP_SID_TILED(Dv, Av)                                                                                      \
-    hipLaunchKernelGGL((siddon_adj_tiled_kernel<Dv, Av>), grid, dim3(kThreads), lds, s, geo, table, dims, pose_axes, \
-                       tc, ntA, ntB, ntC, per_xcd, batch, d_fixexp, sino, vol)
+    hipLaunchKernelGGL((siddon_adj_tiled_kernel<Dv, Av, T>), grid, dim3(kThreads), lds, s, geo, table, dims,         \
+                       pose_axes, tc, ntA, ntB, ntC, per_xcd, batch, d_fixexp, sino, vol)
             if (dims.dim == 2) {
                 if (a == 0)
                     EHP_SID_TILED(2, 0);
@@ -264,6 +265,13 @@ namespace ehp
         }
         return hipSuccess;
     }
+
+    template hipError_t siddon_adjoint_tiled<float>(const PoseGeom*, const SidRec*, const Dims&, const int*, int,
+                                                    const TileCfg&, int, double, unsigned int*, int*, const float*,
+                                                    float*, hipStream_t);
+    template hipError_t siddon_adjoint_tiled<double>(const PoseGeom*, const SidRec*, const Dims&, const int*, int,
+                                                     const TileCfg&, int, double, unsigned int*, int*,
+                                                     const double*, double*, hipStream_t);
 
     // The tiles of every operator per driving axis (one job per operator that has rays of that
     // axis).

@@ -144,9 +144,9 @@ int ehp_adjoint(ehp_op* op, const void* d_sino, void* d_vol, void* stream);
  * kernels and result, bit for bit, but the scratch of the back projection (the fixed-point
  * scale: per-block maxima of |y| and the exponent, 16.6 KB) lives in one static device array
  * of the library, 16.6 KB per GPU held until the process ends, instead of the operator's; the
- * call allocates nothing. Takes FP32 and FP64 Joseph and FP32 Siddon operators with a tiled
- * back projection (EHP_ERR_UNSUPPORTED otherwise: bfloat16, voxel, EHP_SIMPLE_KERNELS or
- * curved-detector operators). Every call on a GPU shares that GPU's array: calls that overlap
+ * call allocates nothing. Takes FP32 and FP64 Joseph and Siddon operators with a tiled back
+ * projection (EHP_ERR_UNSUPPORTED otherwise: bfloat16, voxel or EHP_SIMPLE_KERNELS operators,
+ * and geometries without a tiled back projection). Every call on a GPU shares that GPU's array: calls that overlap
  * on one GPU (different operators on different streams, or threads) race and must be ordered
  * by the caller; calls on one operator are ordered as for ehp_adjoint. */
 int ehp_adjoint_noalloc(ehp_op* op, const void* d_sino, void* d_vol, void* stream);

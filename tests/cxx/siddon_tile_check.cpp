@@ -147,9 +147,11 @@ namespace
         return 0;
     }
 
-    template <int D, int A>
+    // T = float or double (the FP32 and FP64 back projections: the same contributions,
+    // round(y * length) in units of q by fixed_round)
+    template <int D, int A, class T>
     int adj_pass(const Dims& dims, const std::vector<PoseGeom>& geo, const std::vector<SidRec>& table,
-                 const std::vector<int>& axes, const TileCfg& tc, int eg, const float* sino, float* vol)
+                 const std::vector<int>& axes, const TileCfg& tc, int eg, const T* sino, T* vol)
     {
         int ntA, ntB, ntC;
         tile_counts(dims, tc, A, ntA, ntB, ntC);
@@ -169,14 +171,12 @@ namespace
                 for (int v = v0; v <= v1; ++v)
                     for (int u = u0; u <= u1; ++u) {
                         const int64_t ray = (int64_t) p * dims.pix_per_pose + (int64_t) v * dims.det[0] + u;
-                        const float yv = sino[ray];
+                        const T yv = sino[ray];
                         const SidRec& r = table[ray];
-                        if (yv == 0.0f || sid_axis(r) != A)
+                        if (yv == T(0) || sid_axis(r) != A)
                             continue;
-                        float ys;
-                        int sh;
-                        ray_scale(yv * 2.0f, eg, ys, sh);
-                        bool bad = sh < 0 || sh > 32;
+                        const double yq = ldexp((double) yv, 62 - eg);
+                        bool bad = false;
                         siddon_box<D>(r, lo, hi, [&](int c0, int c1, int c2, float w) {
                             if (w == 0.0f)
                                 return;
@@ -185,12 +185,12 @@ namespace
                                 bad = true;
                                 return;
                             }
-                            const float z = ys * (w * 0.5f);
-                            if (!(fabsf(rintf(z)) <= 1073741824.0f)) {
+                            // fixed_round's range: |y * length| below 2^51 units
+                            if (!(fabs(yq * (double) w) < 2251799813685248.0)) {
                                 bad = true;
                                 return;
                             }
-                            iacc[slot] += (__int128) (long long) fixed_contrib(z, sh);
+                            iacc[slot] += (__int128) (long long) fixed_round(yq, (double) w);
                         });
                         if (bad)
                             return 41;
@@ -214,15 +214,14 @@ namespace
                 const double r = (double) (iacc[i] < 0 ? -iacc[i] : iacc[i]) / 9223372036854775808.0;
                 if (r >= 1.0)
                     return 43;
-                vol[lin] += (float) ((double) (int64_t) iacc[i] * ldexp(1.0, eg - 62));
+                vol[lin] += (T) ((double) (int64_t) iacc[i] * ldexp(1.0, eg - 62));
             }
         }
         return 0;
     }
 
-    template <int D>
-    int run_adjoint(const Dims& dims, const std::vector<PoseGeom>& geo, const TileCfg& tc, const float* sino,
-                    float* vol)
+    template <int D, class T>
+    int run_adjoint(const Dims& dims, const std::vector<PoseGeom>& geo, const TileCfg& tc, const T* sino, T* vol)
     {
         std::vector<SidRec> table(dims.nrays);
         std::vector<int> axes(dims.nposes, 0);
@@ -237,17 +236,17 @@ namespace
         const double bound = fixed_point_slot_bound(dims, geo.data());
         if (!(bound > 0.0))
             return 10;
-        float m = 0.0f;
+        double m = 0.0;
         for (int64_t i = 0; i < dims.nrays; ++i)
-            m = fabsf(sino[i]) > m ? fabsf(sino[i]) : m;
+            m = fabs((double) sino[i]) > m ? fabs((double) sino[i]) : m;
         const int eg = fixed_point_exponent(m, bound);
         for (int64_t i = 0; i < dims.nvox; ++i)
-            vol[i] = 0.0f;
-        int rc = adj_pass<D, 0>(dims, geo, table, axes, tc, eg, sino, vol);
+            vol[i] = T(0);
+        int rc = adj_pass<D, 0, T>(dims, geo, table, axes, tc, eg, sino, vol);
         if (!rc)
-            rc = adj_pass<D, 1>(dims, geo, table, axes, tc, eg, sino, vol);
+            rc = adj_pass<D, 1, T>(dims, geo, table, axes, tc, eg, sino, vol);
         if (!rc && D == 3)
-            rc = adj_pass<D, (D == 3 ? 2 : 1)>(dims, geo, table, axes, tc, eg, sino, vol);
+            rc = adj_pass<D, (D == 3 ? 2 : 1), T>(dims, geo, table, axes, tc, eg, sino, vol);
         return rc;
     }
 } // namespace
@@ -318,7 +317,19 @@ extern "C" int siddon_tile_adjoint(int dim, const int64_t* vol_dims, const int64
     const Dims dims = make_dims(dim, vol_dims, det_dims, nposes);
     const std::vector<PoseGeom> geo = poses(dim, nposes, P, Pinv, C);
     const TileCfg tc{LA, TB, dim == 3 ? TC : 1};
-    return dim == 2 ? run_adjoint<2>(dims, geo, tc, sino, vol) : run_adjoint<3>(dims, geo, tc, sino, vol);
+    return dim == 2 ? run_adjoint<2, float>(dims, geo, tc, sino, vol) : run_adjoint<3, float>(dims, geo, tc, sino, vol);
+}
+
+// The FP64 back projection (FP32 geometry and lengths, FP64 data).
+extern "C" int siddon_tile_adjoint64(int dim, const int64_t* vol_dims, const int64_t* det_dims, int nposes,
+                                     const float* P, const float* Pinv, const float* C, int LA, int TB, int TC,
+                                     const double* sino, double* vol)
+{
+    const Dims dims = make_dims(dim, vol_dims, det_dims, nposes);
+    const std::vector<PoseGeom> geo = poses(dim, nposes, P, Pinv, C);
+    const TileCfg tc{LA, TB, dim == 3 ? TC : 1};
+    return dim == 2 ? run_adjoint<2, double>(dims, geo, tc, sino, vol)
+                    : run_adjoint<3, double>(dims, geo, tc, sino, vol);
 }
 
 // The GPU forward over zero-padded plane-major views (fan[a]: the fastest axis of the view of

@@ -316,6 +316,24 @@ def test_joseph_fp64_tiled_back_projection(name, ops):
         np.testing.assert_array_equal(op.applyAdjoint(y), tiled)
 
 
+@pytest.mark.parametrize("name", list(CASES))
+def test_siddon_fp64_tiled_back_projection(name, ops):
+    """SiddonsMethodCUDA<double>'s back projection (LDS tiles, each crossing's FP64 y * length
+    rounded once to the fixed-point unit): within a few units of FP64 rounding of the oracle's
+    FP64 sums and of the one-thread-per-ray atomic kernel, bitwise reproducible run to run."""
+    g = make_case(name)
+    y = phantom(g.sino_shape, 22, np.float64)
+    op = ops("SiddonsMethodCUDA", name, dtype=np.float64)
+    assert op.tile()[0] > 0  # the tiled kernel runs
+    tiled = op.applyAdjoint(y)
+    ref = oracle.siddon(g, y, adjoint=True, mode=oracle.MODE_DIRECT, acc64=True)
+    assert rel_l2(tiled, ref) <= 1e-13, rel_l2(tiled, ref)
+    simple = ops("SiddonsMethodCUDA", name, dtype=np.float64, kernels="simple").applyAdjoint(y)
+    assert rel_l2(tiled, simple) <= 1e-13, rel_l2(tiled, simple)
+    for _ in range(2):
+        np.testing.assert_array_equal(op.applyAdjoint(y), tiled)
+
+
 def test_outputs_overwritten_not_accumulated(ops):
     """elsa's projectors overwrite the output (TraverseJosephsCUDA.cu:69, SiddonsMethodCUDA.cpp:75)."""
     import torch

@@ -49,8 +49,8 @@ def seeded_sino(g, seed, dtype):
 
 
 @pytest.mark.parametrize("cls,dtype", [("JosephsMethodCUDA", np.float32), ("JosephsMethodCUDA", np.float64),
-                                       ("SiddonsMethodCUDA", np.float32)], ids=["joseph-f32", "joseph-f64",
-                                                                                "siddon-f32"])
+                                       ("SiddonsMethodCUDA", np.float32), ("SiddonsMethodCUDA", np.float64)],
+                         ids=["joseph-f32", "joseph-f64", "siddon-f32", "siddon-f64"])
 def test_noalloc_equals_adjoint_large_then_small(cls, dtype):
     import torch
 
