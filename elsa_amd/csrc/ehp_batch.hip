@@ -146,7 +146,6 @@ namespace
                 v.pose_axes = op->d_pose_axes;
                 v.pose_margin = op->d_pose_margin;
                 v.tile = op->tile;
-                v.lutscale = op->lutscale;
                 v.vol_off = b->vol_off[j];
                 v.sino_off = b->sino_off[j];
                 offs[j] = b->vol_off[j];

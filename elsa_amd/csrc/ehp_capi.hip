@@ -238,10 +238,10 @@ namespace
     // per workgroup. Joseph 3D: 8 x 17 x 17 (9.15 ms against 9.3 ms for 8 x 15 x 15 and 9.7 ms for
     // 8 x 19 x 19 at 256^3 / 360 poses; odd widths, as an even slot row stride costs LDS bank
     // conflicts: 12.6 ms for 8 x 18 x 18).
-    size_t tile_lds_bytes(int kind, const ehp::Dims& dims, const ehp::TileCfg& tc)
+    size_t tile_lds_bytes(int kind, const ehp::Dims& dims, const ehp::TileCfg& tc, int dtype = EHP_F32)
     {
         if (kind == KIND_VOXEL)
-            return ehp::voxel_tile_lds_bytes(tc);
+            return ehp::voxel_tile_lds_bytes(tc, dtype == EHP_F64 ? sizeof(double) : sizeof(float));
         return kind == KIND_SIDDON ? ehp::siddon_tile_lds_bytes(dims, tc) : ehp::joseph_tile_lds_bytes(dims, tc);
     }
 
@@ -471,9 +471,9 @@ namespace
         op->tiled = true;
         return EHP_OK;
     }
-    // FP32 voxel projectors: the detector-tiled forward, when every source lies outside the
-    // volume (the footprint scaling is then bounded).
-    int setup_voxel_tiled(ehp_op* op, const float* host_lut)
+    // Voxel projectors (FP32, bfloat16, FP64): the detector-tiled forward, when every source lies
+    // outside the volume (the footprint scaling is then bounded).
+    int setup_voxel_tiled(ehp_op* op, const void* host_lut)
     {
         const ehp::Dims& dm = op->dims;
         const int n = dm.nposes;
@@ -490,26 +490,22 @@ namespace
             margin[p] = (float) (op->support * smax + 1.0);
             axis[p] = dm.dim == 3 ? ehp::pose_axis<3>(op->host_geo[p], dm) : ehp::pose_axis<2>(op->host_geo[p], dm);
         }
-        float lutmax = 0.0f;
-        for (int i = 0; i < op->n_lut; ++i)
-            lutmax = std::fabs(host_lut[i]) > lutmax ? std::fabs(host_lut[i]) : lutmax;
+        // the table (FP64 for FP64 operators, else FP32)
+        double lutmax = 0.0;
+        for (int i = 0; i < op->n_lut; ++i) {
+            const double v = op->dtype == EHP_F64 ? ((const double*) host_lut)[i] : ((const float*) host_lut)[i];
+            lutmax = std::fabs(v) > lutmax ? std::fabs(v) : lutmax;
+        }
         // the 3D phase-contrast weight is the table times d_u, |d_u| <= |d| <= support where the
         // table is not zero (with a little room for the rounding of |d|)
         if (op->differential && dm.dim == 3)
-            lutmax = (float) ((double) lutmax * (op->support > 1.0f ? op->support : 1.0f) * 1.001);
-        if (!(lutmax < 1.0e30f))
+            lutmax = lutmax * (op->support > 1.0f ? op->support : 1.0f) * 1.001;
+        if (!(lutmax < 1.0e30))
             return EHP_OK;
-        float ls = 1.0f;
+        double ls = 1.0;
         while (ls < lutmax)
-            ls *= 2.0f;
-        op->lutscale = ls;
-        // what one pixel can receive: at most (2 r s + 2)^(dim-1) voxels per plane, a weight of at
-        // most the table's maximum each, over the longest axis of planes
-        int maxdim = 0;
-        for (int i = 0; i < dm.dim; ++i)
-            maxdim = dm.vol[i] > maxdim ? dm.vol[i] : maxdim;
-        const double per_plane = 2.0 * op->support * smax_all + 2.0;
-        op->slot_bound = (double) maxdim * (dm.dim == 3 ? per_plane * per_plane : per_plane) * ls;
+            ls *= 2.0;
+        op->slot_bound = ehp::voxel_slot_bound(dm, op->support, smax_all, ls);
         hipError_t e = hipMalloc((void**) &op->d_pose_axes, n * sizeof(int));
         if (e == hipSuccess)
             e = hipMalloc((void**) &op->d_pose_margin, n * sizeof(float));
@@ -605,8 +601,8 @@ int ehp_voxel_create(const ehp_geometry* g, int dtype, const void* lut, int n_lu
         return hip_fail(e, "basis-function table upload");
     }
     rc = count_samples(op);
-    if (rc == EHP_OK && (dtype == EHP_F32 || dtype == EHP_BF16) && !(flags & EHP_SIMPLE_KERNELS))
-        rc = setup_voxel_tiled(op, (const float*) lut);
+    if (rc == EHP_OK && !(flags & EHP_SIMPLE_KERNELS))
+        rc = setup_voxel_tiled(op, lut);
     if (rc == EHP_OK && dtype == EHP_BF16 && !op->tiled)
         rc = fail(EHP_ERR_UNSUPPORTED, "bfloat16 storage needs the tiled voxel forward, which this geometry or "
                                        "table does not allow (source inside the volume, or a table over %d entries)",
@@ -852,10 +848,10 @@ static hipError_t launch(ehp_op* op, const void* in, void* out, hipStream_t s, b
         // bfloat16 storage, FP32 arithmetic: the tiled forward rounds each pixel's fixed-point
         // sum once, the gather each voxel's FP32 sum
         if (!adjoint)
-            e = ehp::voxel_forward_tiled<ehp::bf16>(op->d_geo, dm, (const float*) op->d_lut, op->n_lut, op->support,
-                                                    op->differential, op->d_pose_axes, op->d_pose_margin, op->tile,
-                                                    op->lutscale, op->slot_bound, op->d_maxbits, op->d_fixexp,
-                                                    (const ehp::bf16*) in, (ehp::bf16*) out, s);
+            e = ehp::voxel_forward_tiled<ehp::bf16, float>(op->d_geo, dm, (const float*) op->d_lut, op->n_lut,
+                                                           op->support, op->differential, op->d_pose_axes,
+                                                           op->d_pose_margin, op->tile, op->slot_bound, op->d_maxbits,
+                                                           op->d_fixexp, (const ehp::bf16*) in, (ehp::bf16*) out, s);
         else
             e = ehp::voxel_adjoint_fast<ehp::bf16>(op->d_geo, dm, (const float*) op->d_lut, op->n_lut, op->support,
                                                    op->differential, (const ehp::bf16*) in, (ehp::bf16*) out, s);
@@ -890,9 +886,10 @@ static hipError_t launch(ehp_op* op, const void* in, void* out, hipStream_t s, b
         else if (op->kind == KIND_SIDDON)
             e = adjoint ? ehp::siddon_adjoint<float>(op->d_geo, dm, x, y, s) : ehp::siddon_forward<float>(op->d_geo, dm, x, y, s);
         else if (!adjoint && op->tiled)
-            e = ehp::voxel_forward_tiled<float>(op->d_geo, dm, (const float*) op->d_lut, op->n_lut, op->support,
-                                         op->differential, op->d_pose_axes, op->d_pose_margin, op->tile, op->lutscale,
-                                         op->slot_bound, op->d_maxbits, op->d_fixexp, x, y, s);
+            e = ehp::voxel_forward_tiled<float, float>(op->d_geo, dm, (const float*) op->d_lut, op->n_lut,
+                                                       op->support, op->differential, op->d_pose_axes,
+                                                       op->d_pose_margin, op->tile, op->slot_bound, op->d_maxbits,
+                                                       op->d_fixexp, x, y, s);
         else if (adjoint && op->tiled)
             e = ehp::voxel_adjoint_fast<float>(op->d_geo, dm, (const float*) op->d_lut, op->n_lut, op->support,
                                                op->differential, x, y, s);
@@ -918,6 +915,11 @@ static hipError_t launch(ehp_op* op, const void* in, void* out, hipStream_t s, b
                                                   y, s);
         else if (op->kind == KIND_SIDDON)
             e = adjoint ? ehp::siddon_adjoint<double>(op->d_geo, dm, x, y, s) : ehp::siddon_forward<double>(op->d_geo, dm, x, y, s);
+        else if (!adjoint && op->tiled)
+            e = ehp::voxel_forward_tiled<double, double>(op->d_geo, dm, (const double*) op->d_lut, op->n_lut,
+                                                         op->support, op->differential, op->d_pose_axes,
+                                                         op->d_pose_margin, op->tile, op->slot_bound, op->d_maxbits,
+                                                         op->d_fixexp, x, y, s);
         else
             e = adjoint ? ehp::voxel_adjoint<double>(op->d_geo, dm, (const double*) op->d_lut, op->n_lut, op->support,
                                                      op->differential, x, y, s)
@@ -954,7 +956,7 @@ int ehp_tile(ehp_op* op, int la, int tb, int tc, int* la_out, int* tb_out, int* 
         int max_lds = 65536;
         if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, op->device) != hipSuccess)
             max_lds = 65536;
-        const size_t need = tile_lds_bytes(op->kind, op->dims, t);
+        const size_t need = tile_lds_bytes(op->kind, op->dims, t, op->dtype);
         if ((int64_t) need > max_lds)
             return fail(EHP_ERR_ARGUMENT, "tile needs %zu bytes of LDS, the device allows %d", need, max_lds);
         op->tile = t;

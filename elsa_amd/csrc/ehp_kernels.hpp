@@ -146,7 +146,6 @@ namespace ehp
         const int* pose_axes;
         const float* pose_margin;
         TileCfg tile;
-        float lutscale;
         int64_t vol_off, sino_off;
     };
 
@@ -160,7 +159,6 @@ namespace ehp
         const int* pose_axis;
         const float* pose_margin;
         int TU, TV, ntu;
-        float lutscale;
         int item;
         int64_t vol_off, sino_off;
         int64_t block0;
@@ -217,13 +215,13 @@ namespace ehp
     hipError_t siddon_adjoint_tiled(const PoseGeom* geo, const SidRec* table, const Dims& dims, const int* pose_axes,
                                     int axes_present, const TileCfg& tc, int batch, double slot_bound,
                                     unsigned int* d_maxbits, int* d_fixexp, const T* sino, T* vol, hipStream_t s);
-    size_t voxel_tile_lds_bytes(const TileCfg& tc);
+    size_t voxel_tile_lds_bytes(const TileCfg& tc, size_t lut_bytes = sizeof(float));
     int voxel_max_lut();
-    template <class ST>
-    hipError_t voxel_forward_tiled(const PoseGeom* geo, const Dims& dims, const float* lut, int n_lut, float support,
+    template <class ST, class LT>
+    hipError_t voxel_forward_tiled(const PoseGeom* geo, const Dims& dims, const LT* lut, int n_lut, float support,
                                    int diff, const int* pose_axis, const float* pose_margin, const TileCfg& tc,
-                                   float lutscale, double slot_bound, unsigned int* d_maxbits, int* d_fixexp,
-                                   const ST* vol, ST* sino, hipStream_t s);
+                                   double slot_bound, unsigned int* d_maxbits, int* d_fixexp, const ST* vol, ST* sino,
+                                   hipStream_t s);
     template <class T>
     hipError_t joseph_forward_planes(const PoseGeom* geo, const Dims& dims, const PadLayout* layouts,
                                      void* const* views, int axes_present, const T* vol, T* sino, hipStream_t s);

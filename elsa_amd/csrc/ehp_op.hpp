@@ -44,7 +44,6 @@ struct ehp_op {
     ehp::RayRec* d_table = nullptr;   // per-ray traversal state (Joseph)
     ehp::SidRec* d_sidtable = nullptr; // per-ray traversal state (Siddon)
     float* d_pose_margin = nullptr;    // voxel forward: frustum margin per pose (pixels)
-    float lutscale = 1.0f;             // voxel forward: power of two >= max |table|
     unsigned int* d_maxbits = nullptr; // per-block maxima of |y| of the current back projection (kAbsmaxBlocks)
     int* d_fixexp = nullptr;           // fixed-point exponent of the current back projection
     double slot_bound = 0.0;

@@ -132,6 +132,21 @@ namespace ehp
         return w * d0;
     }
 
+    // What one detector pixel of the tiled voxel forward can receive, in units of max|x|: at most
+    // (2 r s + 2)^(dim-1) voxels per plane (s the largest footprint scaling), each with a
+    // weight of at most ls (a power of two at least the largest |weight|), over the longest
+    // axis of planes; at least 4096 ls, so that one term x * w stays below 2^51 units (the range
+    // of fixed_round).
+    inline double voxel_slot_bound(const Dims& dims, double support, double smax, double ls)
+    {
+        int maxdim = 0;
+        for (int i = 0; i < dims.dim; ++i)
+            maxdim = dims.vol[i] > maxdim ? dims.vol[i] : maxdim;
+        const double per_plane = 2.0 * support * smax + 2.0;
+        const double b = (double) maxdim * (dims.dim == 3 ? per_plane * per_plane : per_plane) * ls;
+        return b > 4096.0 * ls ? b : 4096.0 * ls;
+    }
+
     // Largest footprint scaling of a pose over the volume: sdd over the smallest distance from
     // the source to the volume box (scaling is sdd / |ext x| and |ext x| is the distance from
     // the source to the voxel centre x).

@@ -1,5 +1,5 @@
 // ehp_voxel_tiled.hip -- voxel-driven forward projection with LDS-privatised detector tiles
-// (FP32 arithmetic; volume and sinogram stored in FP32 or bfloat16).
+// (FP32 arithmetic with the volume and sinogram stored in FP32 or bfloat16; FP64).
 //
 // The forward of elsa's voxel projectors (VoxelComputation.h:110-152: every voxel adds
 // x * basis(distance) to the detector pixels of its footprint) without global atomics. One
@@ -12,10 +12,12 @@
 // volume, run together on one XCD.
 //
 // Sums are 64-bit fixed point as in the tiled back projections: one exponent per call from
-// max|x| and a bound on what one pixel can receive, each voxel's contributions rounded to
-// int32 in the voxel's own scale. With bfloat16 storage the voxels are widened exactly and each
-// pixel's sum is rounded to bfloat16 once, so the result is the FP32 operator's on the same
-// (bfloat16) volume, rounded once.
+// max|x| and a bound on what one pixel can receive, each tap's x * weight rounded once to the
+// unit (fixed_round; exact product in FP64 for FP32 data). With bfloat16 storage the voxels
+// are widened exactly and each pixel's sum is rounded to bfloat16 once, so the result is the
+// FP32 operator's on the same (bfloat16) volume, rounded once. FP64 operators take an FP64
+// table and form each weight as the one-thread-per-voxel FP64 kernel does (tap_weight, the
+// divisions of VoxelComputation.h), so their terms are that kernel's.
 #include "ehp_kernels.hpp"
 
 namespace ehp
@@ -26,19 +28,20 @@ namespace ehp
         constexpr int kMaxLut = 1024;
     } // namespace
 
-    // Tile-pose id (tile * nposes + pose) of one operator's forward.
-    template <int D, class ST>
+    // Tile-pose id (tile * nposes + pose) of one operator's forward; LT = float (FP32 and
+    // bfloat16 storage) or double (FP64) is the table's type.
+    template <int D, class ST, class LT = float>
     __device__ __forceinline__ void vox_fwd_tiled_block(const PoseGeom* __restrict__ geo, const Dims& dims,
-                                                        const float* __restrict__ lut, const int n_lut,
+                                                        const LT* __restrict__ lut, const int n_lut,
                                                         const float support, const int diff,
                                                         const int* __restrict__ pose_axis,
                                                         const float* __restrict__ pose_margin, const int TU,
                                                         const int TV, const int ntu, const int64_t id,
-                                                        const float lutscale, const int* __restrict__ fixexp,
+                                                        const int* __restrict__ fixexp,
                                                         const ST* __restrict__ vol, ST* __restrict__ sino)
     {
         extern __shared__ long long lds[];
-        __shared__ float s_lut[kMaxLut];
+        __shared__ LT s_lut[kMaxLut];
         const int tid = threadIdx.x;
         // id = tile * nposes + pose
         const int pose = (int) (id % dims.nposes);
@@ -53,7 +56,6 @@ namespace ehp
         for (int i = tid; i < n_lut; i += kThreads)
             s_lut[i] = lut[i];
         const int eg = fixexp[0];
-        const float inv_scale = 1.0f / lutscale; // a power of two: exact
         const float lut_index_scale = (float) (n_lut - 1) / support;
         const PoseGeom& g = geo[pose];
         const int A = pose_axis[pose];
@@ -79,8 +81,8 @@ namespace ehp
                 if (D == 3)
                     c[b2] = clo + row;
                 const int64_t lin = c[0] + c[1] * dims.strides[1] + (int64_t) c[2] * dims.strides[2];
-                const float xv = up(vol[lin]);
-                if (xv == 0.0f)
+                const auto xv = up(vol[lin]);
+                if (xv == 0)
                     continue;
                 Footprint fp;
                 footprint<D>(g, dims, c[0], c[1], c[2], support, fp);
@@ -88,22 +90,38 @@ namespace ehp
                 const int lo1 = fp.lo[1] > v0 ? fp.lo[1] : v0, hi1 = fp.hi[1] < v1 ? fp.hi[1] : v1;
                 if (lo0 > hi0 || lo1 > hi1)
                     continue;
-                float ys;
-                int sh;
-                ray_scale(xv * lutscale, eg, ys, sh);
-                // the distance as visit() forms it, with one reciprocal of the scaling per voxel
-                // instead of a division per tap (and the table index as dist * (n - 1) / support):
-                // within a few ulp of the gather's weights
-                const float rs = 1.0f / fp.scaling;
-                for (int j = lo1; j <= hi1; ++j) {
-                    const float d1 = D == 3 ? (fp.center[1] - (float) j) * rs : 0.0f;
-                    for (int i = lo0; i <= hi0; ++i) {
-                        const float d0 = (fp.center[0] - (float) i) * rs;
-                        const float dist = D == 2 ? sqrtf(d0 * d0) : sqrtf(d0 * d0 + d1 * d1);
-                        const float w = tap_weight_scaled<D>(s_lut, n_lut, lut_index_scale, dist, d0, diff != 0);
-                        if (w != 0.0f)
-                            atomicAdd(reinterpret_cast<unsigned long long*>(acc + (j - v0) * TU + (i - u0)),
-                                      fixed_contrib(ys * (w * inv_scale), sh));
+                // x in units of q (a power-of-two scaling, exact); each tap adds x * w rounded once
+                // (|x w| < 2^51 units: the pixel bound is at least 4096 times the table's bound)
+                const double xq = ldexp((double) xv, 62 - eg);
+                if constexpr (sizeof(LT) == 8) {
+                    // FP64: the weights of visit() and tap_weight, as the one-thread-per-voxel kernel
+                    for (int j = lo1; j <= hi1; ++j) {
+                        const float d1 = D == 3 ? (fp.center[1] - (float) j) / fp.scaling : 0.0f;
+                        for (int i = lo0; i <= hi0; ++i) {
+                            const float d0 = (fp.center[0] - (float) i) / fp.scaling;
+                            const float dist = D == 2 ? sqrtf(d0 * d0) : sqrtf(d0 * d0 + d1 * d1);
+                            const double w = tap_weight<D, double>(s_lut, n_lut, support, dist, d0, diff != 0);
+                            if (w != 0.0)
+                                atomicAdd(reinterpret_cast<unsigned long long*>(acc + (j - v0) * TU + (i - u0)),
+                                          fixed_round(xq, w));
+                        }
+                    }
+                } else {
+                    // the distance as visit() forms it, with one reciprocal of the scaling per voxel
+                    // instead of a division per tap (and the table index as dist * (n - 1) /
+                    // support): within a few ulp of the gather's weights
+                    const float rs = 1.0f / fp.scaling;
+                    for (int j = lo1; j <= hi1; ++j) {
+                        const float d1 = D == 3 ? (fp.center[1] - (float) j) * rs : 0.0f;
+                        for (int i = lo0; i <= hi0; ++i) {
+                            const float d0 = (fp.center[0] - (float) i) * rs;
+                            const float dist = D == 2 ? sqrtf(d0 * d0) : sqrtf(d0 * d0 + d1 * d1);
+                            const float w =
+                                tap_weight_scaled<D>(s_lut, n_lut, lut_index_scale, dist, d0, diff != 0);
+                            if (w != 0.0f)
+                                atomicAdd(reinterpret_cast<unsigned long long*>(acc + (j - v0) * TU + (i - u0)),
+                                          fixed_round(xq, (double) w));
+                        }
                     }
                 }
             }
@@ -114,23 +132,23 @@ namespace ehp
         for (int i = tid; i < TU * TV; i += kThreads) {
             const int u = u0 + i % TU, v = v0 + i / TU;
             if (u <= u1 && v <= v1)
-                sp[(int64_t) v * dims.det[0] + u] = down<ST>((float) ((double) acc[i] * q));
+                sp[(int64_t) v * dims.det[0] + u] = down<ST>((typename Acc<ST>::type) ((double) acc[i] * q));
         }
     }
 
-    template <int D, class ST>
+    template <int D, class ST, class LT>
     __global__ void __launch_bounds__(kThreads)
-        voxel_fwd_tiled_kernel(const PoseGeom* __restrict__ geo, const Dims dims, const float* __restrict__ lut,
+        voxel_fwd_tiled_kernel(const PoseGeom* __restrict__ geo, const Dims dims, const LT* __restrict__ lut,
                                const int n_lut, const float support, const int diff, const int* __restrict__ pose_axis,
                                const float* __restrict__ pose_margin, const int TU, const int TV, const int ntu,
-                               const int per_xcd, const int64_t total, const float lutscale,
-                               const int* __restrict__ fixexp, const ST* __restrict__ vol, ST* __restrict__ sino)
+                               const int per_xcd, const int64_t total, const int* __restrict__ fixexp,
+                               const ST* __restrict__ vol, ST* __restrict__ sino)
     {
         const int64_t id = (int64_t) (blockIdx.x % 8u) * per_xcd + (blockIdx.x / 8u);
         if (id >= total)
             return;
-        vox_fwd_tiled_block<D, ST>(geo, dims, lut, n_lut, support, diff, pose_axis, pose_margin, TU, TV, ntu, id,
-                                   lutscale, fixexp, vol, sino);
+        vox_fwd_tiled_block<D, ST, LT>(geo, dims, lut, n_lut, support, diff, pose_axis, pose_margin, TU, TV, ntu, id,
+                                       fixexp, vol, sino);
     }
 
     // the forwards of many operators in one launch (block b: tile-pose b - job.block0 of job
@@ -146,14 +164,14 @@ namespace ehp
             return;
         const VoxFwdJob& J = jobs[block_job[id]];
         vox_fwd_tiled_block<D, float>(J.geo, J.dims, J.lut, J.n_lut, J.support, J.diff, J.pose_axis, J.pose_margin,
-                                      J.TU, J.TV, J.ntu, id - J.block0, J.lutscale, fixexp + J.item,
+                                      J.TU, J.TV, J.ntu, id - J.block0, fixexp + J.item,
                                       vols + J.vol_off, sinos + J.sino_off);
     }
 
-    // dynamic LDS (the tile's sums) plus the table
-    size_t voxel_tile_lds_bytes(const TileCfg& tc)
+    // dynamic LDS (the tile's sums) plus the table (lut_bytes per entry: 4, or 8 for FP64)
+    size_t voxel_tile_lds_bytes(const TileCfg& tc, size_t lut_bytes)
     {
-        return (size_t) tc.LA * tc.TB * sizeof(long long) + kMaxLut * sizeof(float);
+        return (size_t) tc.LA * tc.TB * sizeof(long long) + kMaxLut * lut_bytes;
     }
 
     int voxel_max_lut()
@@ -161,12 +179,13 @@ namespace ehp
         return kMaxLut;
     }
 
-    // tc.LA x tc.TB: tile width x height in pixels (height 1 in 2D)
-    template <class ST>
-    hipError_t voxel_forward_tiled(const PoseGeom* geo, const Dims& dims, const float* lut, int n_lut, float support,
+    // tc.LA x tc.TB: tile width x height in pixels (height 1 in 2D); LT: the table's type (FP64
+    // for FP64 operators, else FP32)
+    template <class ST, class LT>
+    hipError_t voxel_forward_tiled(const PoseGeom* geo, const Dims& dims, const LT* lut, int n_lut, float support,
                                    int diff, const int* pose_axis, const float* pose_margin, const TileCfg& tc,
-                                   float lutscale, double slot_bound, unsigned int* d_maxbits, int* d_fixexp,
-                                   const ST* vol, ST* sino, hipStream_t s)
+                                   double slot_bound, unsigned int* d_maxbits, int* d_fixexp, const ST* vol, ST* sino,
+                                   hipStream_t s)
     {
         if (n_lut < 2 || n_lut > kMaxLut)
             return hipErrorInvalidValue;
@@ -182,22 +201,26 @@ namespace ehp
         const dim3 grid((unsigned) (per_xcd * 8));
         const size_t lds = (size_t) TU * TV * sizeof(long long);
         if (dims.dim == 2)
-            hipLaunchKernelGGL((voxel_fwd_tiled_kernel<2, ST>), grid, dim3(kThreads), lds, s, geo, dims, lut, n_lut,
-                               support, diff, pose_axis, pose_margin, TU, TV, ntu, (int) per_xcd, total, lutscale,
+            hipLaunchKernelGGL((voxel_fwd_tiled_kernel<2, ST, LT>), grid, dim3(kThreads), lds, s, geo, dims, lut,
+                               n_lut, support, diff, pose_axis, pose_margin, TU, TV, ntu, (int) per_xcd, total,
                                d_fixexp, vol, sino);
         else
-            hipLaunchKernelGGL((voxel_fwd_tiled_kernel<3, ST>), grid, dim3(kThreads), lds, s, geo, dims, lut, n_lut,
-                               support, diff, pose_axis, pose_margin, TU, TV, ntu, (int) per_xcd, total, lutscale,
+            hipLaunchKernelGGL((voxel_fwd_tiled_kernel<3, ST, LT>), grid, dim3(kThreads), lds, s, geo, dims, lut,
+                               n_lut, support, diff, pose_axis, pose_margin, TU, TV, ntu, (int) per_xcd, total,
                                d_fixexp, vol, sino);
         return hipGetLastError();
     }
 
-    template hipError_t voxel_forward_tiled<float>(const PoseGeom*, const Dims&, const float*, int, float, int,
-                                                   const int*, const float*, const TileCfg&, float, double,
-                                                   unsigned int*, int*, const float*, float*, hipStream_t);
-    template hipError_t voxel_forward_tiled<bf16>(const PoseGeom*, const Dims&, const float*, int, float, int,
-                                                  const int*, const float*, const TileCfg&, float, double,
-                                                  unsigned int*, int*, const bf16*, bf16*, hipStream_t);
+    template hipError_t voxel_forward_tiled<float, float>(const PoseGeom*, const Dims&, const float*, int, float,
+                                                          int, const int*, const float*, const TileCfg&, double,
+                                                          unsigned int*, int*, const float*, float*, hipStream_t);
+    template hipError_t voxel_forward_tiled<bf16, float>(const PoseGeom*, const Dims&, const float*, int, float,
+                                                         int, const int*, const float*, const TileCfg&, double,
+                                                         unsigned int*, int*, const bf16*, bf16*, hipStream_t);
+    template hipError_t voxel_forward_tiled<double, double>(const PoseGeom*, const Dims&, const double*, int, float,
+                                                            int, const int*, const float*, const TileCfg&, double,
+                                                            unsigned int*, int*, const double*, double*,
+                                                            hipStream_t);
 
     // Jobs of a batched voxel forward: one per operator, its tile-pose blocks back to back; the
     // launch's dynamic LDS is the largest tile's.
@@ -222,7 +245,6 @@ namespace ehp
             J.TV = it.dims.dim == 3 ? it.tile.TB : 1;
             J.ntu = (it.dims.det[0] + J.TU - 1) / J.TU;
             const int ntv = (it.dims.det[1] + J.TV - 1) / J.TV;
-            J.lutscale = it.lutscale;
             J.item = j;
             J.vol_off = it.vol_off;
             J.sino_off = it.sino_off;

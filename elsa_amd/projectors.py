@@ -313,10 +313,11 @@ def _table_dtype(dtype):
 
 
 class _VoxelProjector(_HipProjector):
-    """``kernels="fast"`` (FP32 and bfloat16 storage) runs the forward projection over
-    LDS-privatised detector tiles with 64-bit fixed-point sums; ``"simple"`` runs one thread per
-    voxel and pose with float atomics. The back projection is a gather in both. FP64 always uses
-    the simple kernels; bfloat16 operators the fast ones only."""
+    """``kernels="fast"`` runs the forward projection over LDS-privatised detector tiles with
+    64-bit fixed-point sums (FP64: with the simple kernels' weights, each term rounded once to the
+    fixed-point unit); ``"simple"`` runs one thread per voxel and pose with float atomics. The
+    back projection is a gather in both (FP64: the simple kernels' form). bfloat16 operators run
+    the fast kernels only."""
 
     def _set_kernels(self, kernels):
         if kernels not in ("fast", "simple"):

@@ -60,6 +60,8 @@ def lib():
                                 c_int, _f32p, _f32p, c_int]
     L.orc_voxel_f64.argtypes = [c_int, c_int, _i64p, _i64p, c_i64, _f64p, _f64p, _f64p, _f64p, c_int, c_d,
                                 c_int, _f64p, _f64p, c_int]
+    L.orc_voxel_f32_acc64.argtypes = [c_int, c_int, _i64p, _i64p, c_i64, _f32p, _f32p, _f32p, _f64p, c_int, c_f,
+                                      c_int, _f64p, _f64p, c_int]
     vp = ctypes.c_void_p
     L.orc_joseph_rays_f32.argtypes = [c_int, _i64p, _i64p, c_i64, _f32p, _f32p, vp, vp, vp, vp, vp]
     L.orc_joseph_rays_f64.argtypes = [c_int, _i64p, _i64p, c_i64, _f64p, _f64p, vp, vp, vp, vp, vp]
@@ -280,13 +282,25 @@ def curved_planar_coords(det, angle, s2d, dtype=np.float32):
     return np.array(out, dtype=dt)
 
 
-def voxel(g: Geom, data, lut, support, adjoint=False, nthreads=0, differential=False):
+def voxel(g: Geom, data, lut, support, adjoint=False, nthreads=0, differential=False, acc64=False):
     """Blob / B-spline voxel projector (VoxelComputation.h:40-185, full-rectangle footprint);
     differential=True gives the phase-contrast weight (VoxelComputation.h:26-38, used by
-    PhaseContrastProjector.cpp with the derivative LUT in 2D, the normalized-gradient LUT in 3D)."""
+    PhaseContrastProjector.cpp with the derivative LUT in 2D, the normalized-gradient LUT in 3D).
+    acc64 with an FP32 geometry: FP32 footprints with an FP64 table, data and sums (elsa's
+    VoxelProjectorCUDA<double>, real_t geometry)."""
     L = lib()
     dt = np.float64 if g.P.dtype == np.float64 else np.float32
     f = L.orc_voxel_f64 if dt == np.float64 else L.orc_voxel_f32
+    if acc64 and dt == np.float32:
+        vol, det = _dims(g)
+        data = np.ascontiguousarray(data, dtype=np.float64)
+        lut = np.ascontiguousarray(lut, dtype=np.float64)
+        out = np.empty(g.vol_shape if adjoint else g.sino_shape, np.float64)
+        rc = L.orc_voxel_f32_acc64(g.dim, int(adjoint), vol, det, g.nviews, np.ascontiguousarray(g.P),
+                                   np.ascontiguousarray(g.ext), np.ascontiguousarray(g.sdd), lut, lut.shape[0],
+                                   support, int(differential), data.reshape(-1), out.reshape(-1), nthreads)
+        assert rc == 0
+        return out
     vol, det = _dims(g)
     data = np.ascontiguousarray(data, dtype=dt)
     lut = np.ascontiguousarray(lut, dtype=dt)

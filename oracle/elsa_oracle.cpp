@@ -1112,6 +1112,17 @@ int orc_voxel_f32(int dim, int adjoint, const int64_t* vol, const int64_t* det, 
                                              support, diff, in, out);
 }
 
+// FP32 geometry with an FP64 table and FP64 data (elsa's VoxelProjectorCUDA<double>: real_t
+// geometry, data_t = double sums and table).
+int orc_voxel_f32_acc64(int dim, int adjoint, const int64_t* vol, const int64_t* det, int64_t nviews,
+                        const float* P, const float* ext, const float* sdd, const double* lut, int N,
+                        float support, int diff, const double* in, double* out, int nthreads)
+{
+    orc::set_threads(nthreads);
+    return orc::dispatch_voxel<float, double>(dim, adjoint, vol, det, nviews, P, ext, sdd, lut, N,
+                                              support, diff, in, out);
+}
+
 int orc_voxel_f64(int dim, int adjoint, const int64_t* vol, const int64_t* det, int64_t nviews,
                   const double* P, const double* ext, const double* sdd, const double* lut, int N,
                   double support, int diff, const double* in, double* out, int nthreads)

@@ -31,9 +31,11 @@ namespace
         return d;
     }
 
-    template <int D>
-    int run(const Dims& dims, const std::vector<PoseGeom>& geo, const float* lut, int n_lut, float support, int TU,
-            int TV, const float* vol, float* sino, double* stats)
+    // T = float (FP32 table and weights in the forward's reciprocal form) or double (FP64 table,
+    // the weights of visit() and tap_weight, as the FP64 kernels form them)
+    template <int D, class T>
+    int run(const Dims& dims, const std::vector<PoseGeom>& geo, const T* lut, int n_lut, float support, int TU,
+            int TV, const T* vol, T* sino, double* stats)
     {
         // host set-up as the library does it
         double smax_all = 0.0;
@@ -47,22 +49,17 @@ namespace
             margin[p] = (float) (support * smax + 1.0);
             axis[p] = pose_axis<D>(geo[p], dims);
         }
-        float lutmax = 0.0f;
+        double lutmax = 0.0;
         for (int i = 0; i < n_lut; ++i)
-            lutmax = std::fabs(lut[i]) > lutmax ? std::fabs(lut[i]) : lutmax;
-        float ls = 1.0f;
+            lutmax = std::fabs((double) lut[i]) > lutmax ? std::fabs((double) lut[i]) : lutmax;
+        double ls = 1.0;
         while (ls < lutmax)
-            ls *= 2.0f;
-        int maxdim = 0;
-        for (int i = 0; i < D; ++i)
-            maxdim = dims.vol[i] > maxdim ? dims.vol[i] : maxdim;
-        const double per_plane = 2.0 * support * smax_all + 2.0;
-        const double bound = (double) maxdim * (D == 3 ? per_plane * per_plane : per_plane) * ls;
-        float m = 0.0f;
+            ls *= 2.0;
+        const double bound = voxel_slot_bound(dims, support, smax_all, ls);
+        double m = 0.0;
         for (int64_t i = 0; i < dims.nvox; ++i)
-            m = std::fabs(vol[i]) > m ? std::fabs(vol[i]) : m;
+            m = std::fabs((double) vol[i]) > m ? std::fabs((double) vol[i]) : m;
         const int eg = fixed_point_exponent(m, bound);
-        const float inv_scale = 1.0f / ls;
         if (TV > dims.det[1])
             TV = dims.det[1];
         const int ntu = (dims.det[0] + TU - 1) / TU, ntv = (dims.det[1] + TV - 1) / TV;
@@ -115,28 +112,35 @@ namespace
                             if (lo0 > hi0 || lo1 > hi1)
                                 continue;
                             taps_tiled += (double) (hi0 - lo0 + 1) * (hi1 - lo1 + 1);
-                            const float xv = vol[lin];
-                            if (xv == 0.0f)
+                            const T xv = vol[lin];
+                            if (xv == T(0))
                                 continue;
-                            float ys;
-                            int sh;
-                            ray_scale(xv * ls, eg, ys, sh);
-                            // the GPU's arithmetic: one reciprocal of the scaling per voxel, the
-                            // table index as dist * (n - 1) / support
+                            const double xq = ldexp((double) xv, 62 - eg);
+                            // the GPU's arithmetic: FP32, one reciprocal of the scaling per voxel and
+                            // the table index as dist * (n - 1) / support; FP64, visit()'s divisions
                             const float rs = 1.0f / fp.scaling;
                             const float kidx = (float) (n_lut - 1) / support;
                             for (int j = lo1; j <= hi1; ++j)
                                 for (int i = lo0; i <= hi0; ++i) {
-                                    const float d0 = (fp.center[0] - (float) i) * rs;
-                                    const float d1 = D == 3 ? (fp.center[1] - (float) j) * rs : 0.0f;
-                                    const float dist = D == 2 ? sqrtf(d0 * d0) : sqrtf(d0 * d0 + d1 * d1);
-                                    const float w = tap_weight_scaled<D>(lut, n_lut, kidx, dist, d0, false);
-                                    if (w == 0.0f)
+                                    double w;
+                                    if (sizeof(T) == 8) {
+                                        const float d0 = (fp.center[0] - (float) i) / fp.scaling;
+                                        const float d1 = D == 3 ? (fp.center[1] - (float) j) / fp.scaling : 0.0f;
+                                        const float dist = D == 2 ? sqrtf(d0 * d0) : sqrtf(d0 * d0 + d1 * d1);
+                                        w = (double) tap_weight<D, T>(lut, n_lut, support, dist, d0, false);
+                                    } else {
+                                        const float d0 = (fp.center[0] - (float) i) * rs;
+                                        const float d1 = D == 3 ? (fp.center[1] - (float) j) * rs : 0.0f;
+                                        const float dist = D == 2 ? sqrtf(d0 * d0) : sqrtf(d0 * d0 + d1 * d1);
+                                        w = (double) tap_weight_scaled<D>((const float*) lut, n_lut, kidx, dist, d0,
+                                                                          false);
+                                    }
+                                    if (w == 0.0)
                                         continue;
-                                    const float z = ys * (w * inv_scale);
-                                    if (!(std::fabs(rintf(z)) <= 1073741824.0f))
+                                    // fixed_round's range: |x w| below 2^51 units
+                                    if (!(std::fabs(xq * w) < 2251799813685248.0))
                                         return 52;
-                                    acc[(size_t) (j - v0) * TU + (i - u0)] += (__int128) (long long) fixed_contrib(z, sh);
+                                    acc[(size_t) (j - v0) * TU + (i - u0)] += (__int128) (long long) fixed_round(xq, w);
                                 }
                         }
                 }
@@ -146,7 +150,7 @@ namespace
                         if ((double) (a < 0 ? -a : a) >= 9.2e18)
                             return 53;
                         sino[(int64_t) p * dims.pix_per_pose + (int64_t) v * dims.det[0] + u] =
-                            (float) ((double) (int64_t) a * ldexp(1.0, eg - 62));
+                            (T) ((double) (int64_t) a * ldexp(1.0, eg - 62));
                     }
             }
             if (taps_tiled != taps_full) {
@@ -179,6 +183,31 @@ extern "C" int voxel_tile_forward(int dim, const int64_t* vol_dims, const int64_
         g.sdd = sdd[p];
         geo[p] = g;
     }
-    return dim == 2 ? run<2>(dims, geo, lut, n_lut, support, TU, TV, vol, sino, stats)
-                    : run<3>(dims, geo, lut, n_lut, support, TU, TV, vol, sino, stats);
+    return dim == 2 ? run<2, float>(dims, geo, lut, n_lut, support, TU, TV, vol, sino, stats)
+                    : run<3, float>(dims, geo, lut, n_lut, support, TU, TV, vol, sino, stats);
+}
+
+// The FP64 forward (FP64 table and data, FP32 geometry).
+extern "C" int voxel_tile_forward64(int dim, const int64_t* vol_dims, const int64_t* det_dims, int nposes,
+                                    const float* P, const float* ext, const float* C, const float* sdd,
+                                    const float* Pinv, const double* lut, int n_lut, float support, int TU, int TV,
+                                    const double* vol, double* sino, double* stats)
+{
+    const Dims dims = make_dims(dim, vol_dims, det_dims, nposes);
+    const int nm = dim * (dim + 1);
+    std::vector<PoseGeom> geo(nposes);
+    for (int p = 0; p < nposes; ++p) {
+        PoseGeom g{};
+        for (int i = 0; i < nm; ++i) {
+            g.P[i] = P[p * nm + i];
+            g.Pinv[i] = Pinv[p * nm + i];
+            g.ext[i] = ext[p * nm + i];
+        }
+        for (int i = 0; i < dim; ++i)
+            g.C[i] = C[p * dim + i];
+        g.sdd = sdd[p];
+        geo[p] = g;
+    }
+    return dim == 2 ? run<2, double>(dims, geo, lut, n_lut, support, TU, TV, vol, sino, stats)
+                    : run<3, double>(dims, geo, lut, n_lut, support, TU, TV, vol, sino, stats);
 }

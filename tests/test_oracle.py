@@ -254,6 +254,31 @@ def test_voxel_kats_oracle(kat, dt):
     kat(OracleBackend(dt))
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("adjoint", [False, True], ids=["fwd", "adj"])
+def test_voxel_fp32_geometry_fp64_data(dim, adjoint):
+    """The FP32-geometry / FP64-data voxel projector (elsa's VoxelProjectorCUDA<double>) visits
+    the FP32 projector's footprints with the same weights: on FP32-valued data and table it
+    equals the all-FP32 result to FP32 rounding of the sums, it is linear to FP64 rounding, and
+    the all-FP64 geometry differs from it by the FP32 geometry's rounding only."""
+    det = (23,) if dim == 2 else (19, 15)
+    g = _rand_geom(dim, 12, 7, det, np.float32, seed=5, s2c=40.0)
+    g64 = _rand_geom(dim, 12, 7, det, np.float64, seed=5, s2c=40.0)
+    lut = oracle.blob_lut(2.0, 10.83, 2, 100, np.float32)
+    rng = np.random.default_rng(6)
+    x = rng.standard_normal(g.sino_shape if adjoint else g.vol_shape).astype(np.float32)
+    f32 = oracle.voxel(g, x, lut, 2.0, adjoint=adjoint)
+    f64 = oracle.voxel(g, x.astype(np.float64), lut.astype(np.float64), 2.0, adjoint=adjoint, acc64=True)
+    rel = lambda u, v: np.linalg.norm(u - v) / np.linalg.norm(v)
+    assert rel(f32, f64) <= 1e-6
+    x2 = rng.standard_normal(x.shape)
+    f2 = oracle.voxel(g, x2, lut.astype(np.float64), 2.0, adjoint=adjoint, acc64=True)
+    f12 = oracle.voxel(g, x.astype(np.float64) + 3.0 * x2, lut.astype(np.float64), 2.0, adjoint=adjoint, acc64=True)
+    assert rel(f12, f64 + 3.0 * f2) <= 1e-14
+    full64 = oracle.voxel(g64, x.astype(np.float64), lut.astype(np.float64), 2.0, adjoint=adjoint)
+    assert 0 < rel(f64, full64) <= 1e-4
+
+
 def _rand_geom(dim, n, nviews, det, dtype, seed=0, s2c=None):
     rng = np.random.default_rng(seed)
     angles = np.sort(rng.uniform(0, 360, nviews)).astype(np.float32)

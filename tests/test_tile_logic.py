@@ -532,6 +532,8 @@ def vox():
     f64 = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
     L.voxel_tile_forward.argtypes = [ctypes.c_int, i64, i64, ctypes.c_int, f32, f32, f32, f32, f32, f32, ctypes.c_int,
                                      ctypes.c_float, ctypes.c_int, ctypes.c_int, f32, f32, f64]
+    L.voxel_tile_forward64.argtypes = [ctypes.c_int, i64, i64, ctypes.c_int, f32, f32, f32, f32, f32, f64,
+                                       ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int, f64, f64, f64]
     return L
 
 
@@ -570,3 +572,32 @@ def test_voxel_tiled_forward_logic_matches_oracle(vox, case):
     assert stats[0] > 0
     ref = oracle.voxel(g, x, lut, support)
     assert np.linalg.norm(out - ref) <= 1e-6 * np.linalg.norm(ref), np.linalg.norm(out - ref) / np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("case", VOX_CASES, ids=lambda c: f"{c[0]}d-{'x'.join(map(str, c[1]))}-{c[7]}")
+def test_voxel_tiled_forward_fp64_logic_matches_oracle(vox, case):
+    """The FP64 tiled forward (FP64 table, the one-thread-per-voxel FP64 kernel's weights, each
+    tap's x * w rounded once to the fixed-point unit): the oracle's FP64 forward to a few units
+    of FP64 rounding, and every tap visited by exactly one (pose, tile)."""
+    dim, vol, views, det, s2c, c2d, tile, basis = case
+    rng = np.random.default_rng(8)
+    angles = np.sort(rng.uniform(0, 360, views)).astype(np.float32)
+    g = oracle.circle_geometry(dim, vol, views, 360, s2c, c2d, det=det, angles_deg=angles)
+    if basis == "blob":
+        lut, support = oracle.blob_lut(2.0, 10.83, 2, 100, np.float64), 2.0
+    else:
+        lut, support = oracle.bspline_lut(dim, 2, 100, np.float64), 1.5
+    x = rng.standard_normal(g.vol_shape)
+    out = np.full(g.sino_shape, np.nan, np.float64)
+    stats = np.zeros(2)
+    rc = vox.voxel_tile_forward64(dim, np.array(list(vol) + [1] * (3 - dim), np.int64),
+                                  np.array(list(det) + [1] * (2 - len(det)), np.int64), views,
+                                  np.ascontiguousarray(g.P), np.ascontiguousarray(g.ext), np.ascontiguousarray(g.C),
+                                  np.ascontiguousarray(g.sdd), np.ascontiguousarray(g.Pinv), np.ascontiguousarray(lut),
+                                  lut.shape[0], support, tile[0], tile[1], x, out, stats)
+    assert rc == 0, rc
+    assert stats[0] > 0
+    # the oracle with FP32 geometry and the FP64 table and data
+    ref = oracle.voxel(g, x, lut, support, acc64=True)
+    err = np.linalg.norm(out - ref) / np.linalg.norm(ref)
+    assert err <= 1e-14, err
