@@ -99,6 +99,9 @@ def parse():
                    help="relative error to the FP64 reference iterate that the timed CGLS must reach")
     p.add_argument("--cg-ref-iters", type=int, default=100,
                    help="FP64 CGLS iterations of the reference solution x*")
+    p.add_argument("--dtype", default="f32", choices=["f32", "f64"],
+                   help="data type of the operator (f64: elsa's data_t = double classes; the CG leg and the "
+                        "CPU baseline are measured for f32 only)")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="only run a plain device copy of MIB MiB (3 launches), the reference kernel for "
                         "scaling FETCH_SIZE / WRITE_SIZE counters, and exit")
@@ -549,7 +552,8 @@ def roofline(a, wl, name, launch_bytes, t_dom):
     Without a counter file for the launch, the HBM form of the algorithmic bytes is reported."""
     out = {"kernel": name, "ms_per_launch": round(t_dom * 1e3, 3), "algorithmic_bytes_per_launch": launch_bytes,
            "algorithmic_GBps": round(launch_bytes / t_dom / 1e9, 1)}
-    cfile = os.path.join(ROOT, "profiles", f"{a.projector}_{a.workload}_counters.json")
+    key = a.projector + ("_f64" if a.dtype == "f64" else "")
+    cfile = os.path.join(ROOT, "profiles", f"{key}_{a.workload}_counters.json")
     k = None
     if a.kernels == "fast" and os.path.exists(cfile):
         k = json.load(open(cfile))["kernels"].get(name)
@@ -599,19 +603,21 @@ def main():
     vd = E.VolumeDescriptor(list(wl["vol"]))
     dd = E.CircleTrajectoryGenerator.createTrajectory(wl["views"], vd, 360, wl["s2c"], wl["c2d"],
                                                       detectorSize=list(wl["det"]))
+    ndt = np.float64 if a.dtype == "f64" else np.float32
     if a.projector == "joseph":
-        op = E.JosephsMethodCUDA(vd, dd, kernels=a.kernels)
+        op = E.JosephsMethodCUDA(vd, dd, kernels=a.kernels, dtype=ndt)
     elif a.projector == "siddon":
-        op = E.SiddonsMethodCUDA(vd, dd, kernels=a.kernels)
+        op = E.SiddonsMethodCUDA(vd, dd, kernels=a.kernels, dtype=ndt)
     elif a.projector == "blob":
-        op = E.BlobVoxelProjectorCUDA(vd, dd, kernels=a.kernels)
+        op = E.BlobVoxelProjectorCUDA(vd, dd, kernels=a.kernels, dtype=ndt)
     else:
-        op = E.BSplineVoxelProjectorCUDA(vd, dd, kernels=a.kernels)
+        op = E.BSplineVoxelProjectorCUDA(vd, dd, kernels=a.kernels, dtype=ndt)
     tile = None
     S = op.sample_count()
     R = int(np.prod(wl["det"])) * wl["views"]
-    x_np = modified_shepp_logan(wl["vol"][::-1])
+    x_np = modified_shepp_logan(wl["vol"][::-1]).astype(ndt)
     x = torch.from_numpy(x_np).to(dev)
+    tdt = torch.float64 if a.dtype == "f64" else torch.float32
     if a.kernels == "fast":
         if a.tile == "auto":
             # tune on the workload's own data: the phantom for the voxel forward, its sinogram
@@ -623,8 +629,8 @@ def main():
             tile = op.tile(*[int(v) for v in a.tile.split(",")]) if a.tile else op.tile()
             if a.pose_batch and a.projector in ("joseph", "siddon"):
                 op.pose_batch(a.pose_batch)
-    sino = torch.empty(dd.shape, dtype=torch.float32, device=dev)
-    back = torch.empty(vd.shape, dtype=torch.float32, device=dev)
+    sino = torch.empty(dd.shape, dtype=tdt, device=dev)
+    back = torch.empty(vd.shape, dtype=tdt, device=dev)
 
     for _ in range(a.warmup):
         op.apply(x, sino)
@@ -647,22 +653,25 @@ def main():
     t_fwd = float(np.mean([e0.elapsed_time(e1) for e0, e1, _ in evs])) * 1e-3
     t_adj = float(np.mean([e1.elapsed_time(e2) for _, e1, e2 in evs])) * 1e-3
     nvox = int(np.prod(wl["vol"]))
+    eb = 8 if a.dtype == "f64" else 4  # bytes per value
     if a.projector == "joseph":
-        # 4 (2D: 2) FP32 taps per sample, one FP32 value per ray
-        alg_bytes = (16 if wl["dim"] == 3 else 8) * S + 4 * R
+        # 4 (2D: 2) taps per sample, one value per ray
+        alg_bytes = (4 if wl["dim"] == 3 else 2) * eb * S + eb * R
         adj_name = "joseph_adj_tiled_kernel" if a.kernels == "fast" else "joseph_adj_kernel"
         fwd_name = "joseph_fwd_planes_kernel" if a.kernels == "fast" else "joseph_fwd_kernel"
         unit, metric = "Gsamples/s", METRIC
     elif a.projector == "siddon":
-        # one FP32 voxel per crossing, one FP32 value per ray
-        alg_bytes = 4 * S + 4 * R
-        fwd_name = "siddon_fwd_planes_kernel" if a.kernels == "fast" else "siddon_kernel"
+        # one voxel per crossing, one value per ray
+        alg_bytes = eb * S + eb * R
+        # FP64: the one-thread-per-ray forward and the tiled back projection
+        fwd_name = "siddon_fwd_planes_kernel" if a.kernels == "fast" and a.dtype == "f32" else "siddon_kernel"
         adj_name = "siddon_adj_tiled_kernel" if a.kernels == "fast" else "siddon_kernel"
         unit, metric = "Gcrossings/s", "Siddon fwd+adj voxel crossings/s & %HBM roofline"
     else:
-        # one FP32 detector value per footprint tap, one FP32 voxel per voxel and pose
-        alg_bytes = 4 * S + 4 * nvox * wl["views"]
-        adj_name = "voxel_adj_fast_kernel" if a.kernels == "fast" else "voxel_adj_kernel"
+        # one detector value per footprint tap, one voxel per voxel and pose
+        alg_bytes = eb * S + eb * nvox * wl["views"]
+        # FP64: the tiled forward and the simple kernels' gather
+        adj_name = "voxel_adj_fast_kernel" if a.kernels == "fast" and a.dtype == "f32" else "voxel_adj_kernel"
         fwd_name = "voxel_fwd_tiled_kernel" if a.kernels == "fast" else "voxel_fwd_kernel"
         unit, metric = "Gtaps/s", f"{a.projector} voxel projector fwd+adj footprint taps/s & %HBM roofline"
     # The tiled back projections run one launch per driving axis that occurs (a "pass"), each
@@ -676,8 +685,8 @@ def main():
                     for ax in range(wl["dim"])]
         per_axis = [(ax, s_a, r_a) for ax, s_a, r_a in per_axis if s_a > 0]
         s_tot = sum(s_a for _, s_a, _ in per_axis)
-        b_tap = 16 if wl["dim"] == 3 else 8
-        adj_launches = [(f"{adj_name}<{wl['dim']},{ax}>", b_tap * s_a + 4 * r_a, t_adj * s_a / s_tot)
+        b_tap = (4 if wl["dim"] == 3 else 2) * eb
+        adj_launches = [(f"{adj_name}<{wl['dim']},{ax}>", b_tap * s_a + eb * r_a, t_adj * s_a / s_tot)
                         for ax, s_a, r_a in per_axis]
     adj_launch = max(adj_launches, key=lambda x: x[2])
     if adj_launch[2] >= t_fwd:
@@ -696,7 +705,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": a.dtype,
         "data": "synthetic (modified Shepp-Logan phantom)",
         "config": {"workload": wl["text"], "projector": a.projector, "kernels": a.kernels, "tile": tile,
                    "pose_batch": op.pose_batch() if a.kernels == "fast" and a.projector in ("joseph", "siddon")
@@ -707,9 +716,9 @@ def main():
         "roofline": dict(roof, launches={"forward": 1, "back_projection": len(adj_launches) if len(adj_launches) > 1
                                                                              else "pass"}),
     }
-    if a.cg_iters > 0 and a.projector == "joseph":
+    if a.cg_iters > 0 and a.projector == "joseph" and a.dtype == "f32":
         out["cg"] = cg_run(E, op, x, a.cg_iters, vd, dd, a.cg_target, a.cg_ref_iters)
-    if not a.no_cpu and a.projector == "joseph":
+    if not a.no_cpu and a.projector == "joseph" and a.dtype == "f32":
         out["cpu_baseline"] = cpu_baseline(wl, *a.cpu_poses)
     print(json.dumps(out), flush=True)
 
