@@ -663,8 +663,7 @@ def main():
     elif a.projector == "siddon":
         # one voxel per crossing, one value per ray
         alg_bytes = eb * S + eb * R
-        # FP64: the one-thread-per-ray forward and the tiled back projection
-        fwd_name = "siddon_fwd_planes_kernel" if a.kernels == "fast" and a.dtype == "f32" else "siddon_kernel"
+        fwd_name = "siddon_fwd_planes_kernel" if a.kernels == "fast" else "siddon_kernel"
         adj_name = "siddon_adj_tiled_kernel" if a.kernels == "fast" else "siddon_kernel"
         unit, metric = "Gcrossings/s", "Siddon fwd+adj voxel crossings/s & %HBM roofline"
     else:

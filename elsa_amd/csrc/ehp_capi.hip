@@ -421,7 +421,7 @@ namespace
     }
 
     // Siddon (FP32, FP64): when the fixed-point bound exists, the per-ray table, the driving axes
-    // per pose and the LDS-tiled back projection; FP32 also the plane-major views of the forward.
+    // per pose, the LDS-tiled back projection and the plane-major views of the forward.
     int setup_siddon_tiled(ehp_op* op)
     {
         const ehp::Dims& dm = op->dims;
@@ -452,16 +452,13 @@ namespace
         op->axes_present = 0;
         for (int v : h)
             op->axes_present |= v;
-        // FP32 forward: the plane-major views of the Joseph forward, walked plane by plane (FP64
-        // keeps the one-thread-per-ray forward)
-        if (op->dtype == EHP_F32) {
-            const int rc = setup_views(op, h);
-            if (rc != EHP_OK)
-                return rc;
-            e = hipDeviceSynchronize(); // the views' padding is cleared on the null stream
-            if (e != hipSuccess)
-                return hip_fail(e, "volume view");
-        }
+        // the forward walks plane-major views (one element per voxel) plane by plane
+        const int rc = setup_views(op, h);
+        if (rc != EHP_OK)
+            return rc;
+        e = hipDeviceSynchronize(); // the views' padding is cleared on the null stream
+        if (e != hipSuccess)
+            return hip_fail(e, "volume view");
         op->slot_bound = bound;
         int max_lds = 65536;
         if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, op->device) != hipSuccess)
@@ -762,13 +759,11 @@ static bool scatters_into_output(const ehp_op* op, bool adjoint)
 {
     if (op->dtype == EHP_BF16)
         return false;
+    // FP32 and FP64 alike: the detector-tiled voxel forward and the LDS-tiled Joseph and Siddon
+    // back projections write each element from one workgroup
     if (op->kind == KIND_VOXEL)
-        return !adjoint && !(op->dtype == EHP_F32 && op->tiled);
-    if (!adjoint)
-        return false;
-    if (op->kind == KIND_JOSEPH && op->tiled)
-        return false; // FP32 and FP64: LDS tiles, each voxel written by one workgroup per pass
-    return !(op->dtype == EHP_F32 && op->tiled);
+        return !adjoint && !op->tiled;
+    return adjoint && !op->tiled;
 }
 
 // Memory from hipMalloc (not managed, not host memory mapped for the device).
@@ -879,7 +874,8 @@ static hipError_t launch(ehp_op* op, const void* in, void* out, hipStream_t s, b
         else if (op->kind == KIND_JOSEPH)
             e = adjoint ? ehp::joseph_adjoint<float>(op->d_geo, dm, x, y, s) : ehp::joseph_forward<float>(op->d_geo, dm, x, y, s);
         else if (op->kind == KIND_SIDDON && !adjoint && op->planes)
-            e = ehp::siddon_forward_planes(op->d_sidtable, dm, op->layouts, op->d_view, op->axes_present, x, y, s);
+            e = ehp::siddon_forward_planes<float>(op->d_sidtable, dm, op->layouts, op->d_view, op->axes_present, x, y,
+                                                  s);
         else if (op->kind == KIND_SIDDON && adjoint && op->tiled)
             e = ehp::siddon_adjoint_tiled(op->d_geo, op->d_sidtable, dm, op->d_pose_axes, op->axes_present, op->tile,
                                           op->batch, op->slot_bound, op->d_maxbits, op->d_fixexp, x, y, s);
@@ -909,6 +905,9 @@ static hipError_t launch(ehp_op* op, const void* in, void* out, hipStream_t s, b
             e = ehp::joseph_forward_planes<double>(op->d_geo, dm, op->layouts, op->d_view, op->axes_present, x, y, s);
         else if (op->kind == KIND_JOSEPH)
             e = adjoint ? ehp::joseph_adjoint<double>(op->d_geo, dm, x, y, s) : ehp::joseph_forward<double>(op->d_geo, dm, x, y, s);
+        else if (op->kind == KIND_SIDDON && !adjoint && op->planes)
+            e = ehp::siddon_forward_planes<double>(op->d_sidtable, dm, op->layouts, op->d_view, op->axes_present, x,
+                                                   y, s);
         else if (op->kind == KIND_SIDDON && adjoint && op->tiled)
             e = ehp::siddon_adjoint_tiled<double>(op->d_geo, op->d_sidtable, dm, op->d_pose_axes, op->axes_present,
                                                   op->tile, op->batch, op->slot_bound, op->d_maxbits, op->d_fixexp, x,

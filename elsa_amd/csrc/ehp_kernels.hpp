@@ -228,10 +228,10 @@ namespace ehp
     // the volume copied into the interior of its plane-major view (float or bfloat16 elements)
     template <class E>
     hipError_t joseph_view_copy(const Dims& dims, const PadLayout& L, const E* vol, E* view, hipStream_t s);
-    // Siddon forward over plane-major views (FP32), from the per-ray table
+    // Siddon forward over plane-major views (FP32, FP64), from the per-ray table
+    template <class T>
     hipError_t siddon_forward_planes(const SidRec* table, const Dims& dims, const PadLayout* layouts,
-                                     void* const* views, int axes_present, const float* vol, float* sino,
-                                     hipStream_t s);
+                                     void* const* views, int axes_present, const T* vol, T* sino, hipStream_t s);
 
     template <class T>
     hipError_t siddon_forward(const PoseGeom* geo, const Dims& dims, const T* vol, T* sino, hipStream_t s);

@@ -1,4 +1,4 @@
-// ehp_siddon_planes.hip -- Siddon forward projection over plane-major volume views (FP32).
+// ehp_siddon_planes.hip -- Siddon forward projection over plane-major volume views (FP32, FP64).
 //
 // The forward of Siddon's method (elsa/projectors/SiddonsMethod.cpp:76-124, forward :115-117;
 // traversal of TraverseAABB.cpp:6-184) with one lane per ray, reading the zero-padded
@@ -9,8 +9,9 @@
 // at most 3), so all lanes read the same plane at neighbouring positions. One loop per lane
 // would put rays that enter through a side face, each at its own plane, on a cache line of
 // their own at every step. The steps are siddon_traverse's, in its order, so the sum is the
-// one-thread-per-ray kernel's bit for bit. The traversal state of every ray comes from the
-// per-operator table of the tiled back projection.
+// one-thread-per-ray kernel's bit for bit, in FP32 and in FP64 (SiddonsMethodCUDA<double>: FP64
+// views and sums, the FP32 lengths of the FP32 traversal, as that kernel forms them). The
+// traversal state of every ray comes from the per-operator table of the tiled back projection.
 #include "ehp_waves.hpp"
 
 namespace ehp
@@ -19,16 +20,17 @@ namespace ehp
     {
         // The group's rays walk the planes of axis A from traversal coordinate t0 (the plane, or
         // its negative when NEG), until none is left inside the volume. FB2: the view's fastest
-        // axis is b2 (else b1).
-        template <int D, int A, bool NEG, bool FB2>
-        __device__ inline float sid_planes(const SidRec& r, const Dims& dims, const PadView& V, int t0)
+        // axis is b2 (else b1). T: the view's element type, in which the sum runs.
+        template <int D, int A, bool NEG, bool FB2, class T>
+        __device__ inline T sid_planes(const SidRec& r, const Dims& dims, const PadView& V, int t0)
         {
+            constexpr uint32_t lg = sizeof(T) == 8 ? 3u : 2u; // log2 of the element size
             constexpr int b1 = TileAxes<D, A>::b1, b2 = TileAxes<D, A>::b2;
             constexpr int f = (D == 3 && FB2) ? b2 : b1; // fastest axis of the view
             constexpr int m = (D == 3) ? (FB2 ? b1 : b2) : 0; // rows (3D)
             SidWalk w;
             sid_walk_init<D>(r, w);
-            float acc = 0.0f;
+            T acc = T(0);
             constexpr int dir = NEG ? -1 : 1;
             for (int p = NEG ? -t0 : t0; p >= 0 && p < dims.vol[A]; p += dir) {
                 if (!__any(w.alive))
@@ -37,18 +39,18 @@ namespace ehp
                 sid_walk_plane<D>(r, dims.vol, A, p, w, [&](int c0, int c1, int c2, float len) {
                     const int c[3] = {c0, c1, c2};
                     const uint32_t off =
-                        (D == 3 ? mul24((uint32_t) (c[m] + 2), V.RS4) : 0u) + ((uint32_t) (c[f] + 2) << 2);
-                    float v;
-                    __builtin_memcpy(&v, pb + off, sizeof(float));
-                    acc += v * len;
+                        (D == 3 ? mul24((uint32_t) (c[m] + 2), V.RS4) : 0u) + ((uint32_t) (c[f] + 2) << lg);
+                    T v;
+                    __builtin_memcpy(&v, pb + off, sizeof(T));
+                    acc += v * (T) len;
                 });
             }
             return acc;
         }
 
-        template <int D, int K>
+        template <int D, int K, class T>
         __device__ __forceinline__ void sid_group(int keys, int key, const SidRec& r, const Dims& dims,
-                                                  const ViewSet& vs, int t, float& val)
+                                                  const ViewSet& vs, int t, T& val)
         {
             if (!((keys >> K) & 1))
                 return;
@@ -57,16 +59,16 @@ namespace ehp
             const bool mine = key == K;
             const int t0 = wave_min(mine ? t : 0x7fffffff);
             if (mine)
-                val = (D == 3 && ((vs.fb2 >> A) & 1)) ? sid_planes<D, A, NEG, true>(r, dims, vs.v[A], t0)
-                                                      : sid_planes<D, A, NEG, false>(r, dims, vs.v[A], t0);
+                val = (D == 3 && ((vs.fb2 >> A) & 1)) ? sid_planes<D, A, NEG, true, T>(r, dims, vs.v[A], t0)
+                                                      : sid_planes<D, A, NEG, false, T>(r, dims, vs.v[A], t0);
         }
 
         // Block id of the forward's tiles (fwd_pixel's order): the rays of its wavefronts, grouped
         // by driving axis and direction, each group walking its planes together.
-        template <int D>
+        template <int D, class T>
         __device__ __forceinline__ void sid_fwd_block(const SidRec* __restrict__ table, const Dims& dims,
                                                       const ViewSet& views, const int tiles_u, const int64_t id,
-                                                      float* __restrict__ sino)
+                                                      T* __restrict__ sino)
         {
             int u, v, pose;
             fwd_pixel<D>(dims, tiles_u, id, u, v, pose);
@@ -84,7 +86,7 @@ namespace ehp
                 }
             }
             const int keys = wave_or(key >= 0 ? 1 << key : 0);
-            float val = 0.0f;
+            T val = T(0);
             sid_group<D, 0>(keys, key, r, dims, views, t, val);
             sid_group<D, 1>(keys, key, r, dims, views, t, val);
             sid_group<D, 2>(keys, key, r, dims, views, t, val);
@@ -97,16 +99,16 @@ namespace ehp
                 sino[ray] = val;
         }
 
-        template <int D>
+        template <int D, class T>
         __global__ void __launch_bounds__(kFwdThreads)
             siddon_fwd_planes_kernel(const SidRec* __restrict__ table, const Dims dims, const ViewSet views,
                                      const int tiles_u, const int per_xcd, const int64_t total,
-                                     float* __restrict__ sino)
+                                     T* __restrict__ sino)
         {
             const int64_t id = (int64_t) (blockIdx.x % 8u) * per_xcd + (blockIdx.x / 8u);
             if (id >= total)
                 return;
-            sid_fwd_block<D>(table, dims, views, tiles_u, id, sino);
+            sid_fwd_block<D, T>(table, dims, views, tiles_u, id, sino);
         }
 
         // the forwards of many operators in one launch (block b: block b - job.block0 of job
@@ -120,14 +122,15 @@ namespace ehp
             if (id >= total)
                 return;
             const SidFwdJob& J = jobs[block_job[id]];
-            sid_fwd_block<D>(J.table, J.dims, J.views, J.tiles_u, id - J.block0, sinos + J.sino_off);
+            sid_fwd_block<D, float>(J.table, J.dims, J.views, J.tiles_u, id - J.block0, sinos + J.sino_off);
         }
     } // namespace
 
+    template <class T>
     hipError_t siddon_forward_planes(const SidRec* table, const Dims& dims, const PadLayout* layouts,
-                                     void* const* views, int axes_present, const float* vol, float* sino,
-                                     hipStream_t s)
+                                     void* const* views, int axes_present, const T* vol, T* sino, hipStream_t s)
     {
+        constexpr uint32_t es = sizeof(T);
         ViewSet vs{};
         for (int a = 0; a < dims.dim; ++a) {
             if (!(axes_present & (1 << a)))
@@ -135,10 +138,10 @@ namespace ehp
             const PadLayout& L = layouts[a];
             if (L.pairs)
                 return hipErrorInvalidValue; // the Siddon walk reads single voxels of plain views
-            hipError_t e = joseph_view_copy<float>(dims, L, vol, (float*) views[a], s);
+            hipError_t e = joseph_view_copy<T>(dims, L, vol, (T*) views[a], s);
             if (e != hipSuccess)
                 return e;
-            vs.v[a] = PadView{views[a], L.PS * 4u, L.RS * 4u, L.size};
+            vs.v[a] = PadView{views[a], L.PS * es, L.RS * es, L.size};
             const int b2 = dims.dim == 3 ? (a == 2 ? 1 : 2) : -1;
             if (L.f == b2)
                 vs.fb2 |= 1 << a;
@@ -148,13 +151,18 @@ namespace ehp
             return hipErrorInvalidValue;
         const dim3 grid((unsigned) (fg.per_xcd * 8));
         if (dims.dim == 2)
-            hipLaunchKernelGGL((siddon_fwd_planes_kernel<2>), grid, dim3(kFwdThreads), 0, s, table, dims, vs,
+            hipLaunchKernelGGL((siddon_fwd_planes_kernel<2, T>), grid, dim3(kFwdThreads), 0, s, table, dims, vs,
                                fg.tiles_u, (int) fg.per_xcd, fg.total, sino);
         else
-            hipLaunchKernelGGL((siddon_fwd_planes_kernel<3>), grid, dim3(kFwdThreads), 0, s, table, dims, vs,
+            hipLaunchKernelGGL((siddon_fwd_planes_kernel<3, T>), grid, dim3(kFwdThreads), 0, s, table, dims, vs,
                                fg.tiles_u, (int) fg.per_xcd, fg.total, sino);
         return hipGetLastError();
     }
+
+    template hipError_t siddon_forward_planes<float>(const SidRec*, const Dims&, const PadLayout*, void* const*, int,
+                                                     const float*, float*, hipStream_t);
+    template hipError_t siddon_forward_planes<double>(const SidRec*, const Dims&, const PadLayout*, void* const*, int,
+                                                      const double*, double*, hipStream_t);
 
     // Jobs of a batched Siddon forward: the view copies of every operator (one job per view)
     // and the forward blocks (one job per operator).

@@ -245,11 +245,11 @@ class JosephsMethodCUDA(_HipProjector):
 class SiddonsMethodCUDA(_HipProjector):
     """Siddon's method (traversal of TraverseAABB.cpp:6-184, crossing parameters T0 + k tDelta).
 
-    ``kernels="fast"`` (FP32) runs the back projection over LDS-privatised tiles with 64-bit
-    fixed-point sums; ``"simple"`` runs one thread per ray with float atomics. FP64 with
-    ``kernels="fast"`` runs the one-thread-per-ray forward and the same tiled back projection
-    (each FP64 contribution rounded once to the fixed-point unit, bitwise reproducible);
-    ``"simple"`` one thread per ray with FP64 atomics."""
+    ``kernels="fast"`` runs the forward over plane-major volume views (each ray's crossings in
+    traversal order, the one-thread-per-ray sums bit for bit) and the back projection over
+    LDS-privatised tiles with 64-bit fixed-point sums, in FP32 and in FP64 (each FP64
+    contribution rounded once to the fixed-point unit, bitwise reproducible); ``"simple"`` runs
+    one thread per ray with float (FP64) atomics."""
 
     def __init__(self, domainDescriptor, rangeDescriptor, dtype=np.float32, kernels: str = "fast"):
         if kernels not in ("fast", "simple"):

@@ -336,10 +336,12 @@ extern "C" int siddon_tile_adjoint64(int dim, const int64_t* vol_dims, const int
 // driving axis a), with its grouping: the pixels (u0 + l % wu, v0 + l / wu), l < 64, form a
 // wavefront; its rays fall into groups by driving axis and step direction along it, and each
 // group walks the planes of that axis together, every ray taking its own steps on the current
-// plane (sid_walk_plane). Every read is checked against its view and against the plane.
-extern "C" int siddon_plane_forward(int dim, const int64_t* vol_dims, const int64_t* det_dims, int nposes,
-                                    const float* P, const float* Pinv, const float* C, const int* fan,
-                                    const float* vol, float* sino, int wu)
+// plane (sid_walk_plane). Every read is checked against its view and against the plane. T: the
+// element type of the views and of the sums (FP64: SiddonsMethodCUDA<double>, the FP32 lengths
+// widened).
+template <class T>
+static int plane_forward(int dim, const int64_t* vol_dims, const int64_t* det_dims, int nposes, const float* P,
+                         const float* Pinv, const float* C, const int* fan, const T* vol, T* sino, int wu)
 {
     const Dims dims = make_dims(dim, vol_dims, det_dims, nposes);
     const std::vector<PoseGeom> geo = poses(dim, nposes, P, Pinv, C);
@@ -348,13 +350,13 @@ extern "C" int siddon_plane_forward(int dim, const int64_t* vol_dims, const int6
     if (wu < 1 || 64 % wu != 0)
         return 23;
     const int wv = 64 / wu;
-    std::vector<std::vector<float>> views(3);
+    std::vector<std::vector<T>> views(3);
     PadLayout L[3];
     for (int a = 0; a < dim; ++a) {
         if (fan[a] == a || fan[a] < 0 || fan[a] > (dim == 3 ? 2 : 1))
             return 20;
         L[a] = pad_layout(dims, a, fan[a]);
-        views[a].assign(L[a].size, 0.0f);
+        views[a].assign(L[a].size, T(0));
         for (int z = 0; z < dims.vol[2]; ++z)
             for (int y = 0; y < dims.vol[1]; ++y)
                 for (int x = 0; x < dims.vol[0]; ++x)
@@ -386,9 +388,9 @@ extern "C" int siddon_plane_forward(int dim, const int64_t* vol_dims, const int6
                     if (t0 == 0x7fffffff)
                         continue;
                     SidWalk w[64];
-                    float acc[64];
+                    T acc[64];
                     for (int l = 0; l < 64; ++l) {
-                        acc[l] = 0.0f;
+                        acc[l] = T(0);
                         if (key[l] == k)
                             dim == 2 ? sid_walk_init<2>(rr[l], w[l]) : sid_walk_init<3>(rr[l], w[l]);
                     }
@@ -408,7 +410,7 @@ extern "C" int siddon_plane_forward(int dim, const int64_t* vol_dims, const int6
                                     bad = true;
                                     return;
                                 }
-                                acc[l] += views[a][o] * len;
+                                acc[l] += views[a][o] * (T) len;
                             };
                             if (dim == 2)
                                 sid_walk_plane<2>(rr[l], dims.vol, a, pl, w[l], fn);
@@ -427,8 +429,22 @@ extern "C" int siddon_plane_forward(int dim, const int64_t* vol_dims, const int6
                 for (int l = 0; l < 64; ++l) {
                     const int u = u0 + l % wu, v = v0 + l / wu;
                     if (key[l] < 0 && u < dims.det[0] && v < dims.det[1])
-                        sino[(int64_t) p * dims.pix_per_pose + (int64_t) v * dims.det[0] + u] = 0.0f;
+                        sino[(int64_t) p * dims.pix_per_pose + (int64_t) v * dims.det[0] + u] = T(0);
                 }
             }
     return bad ? 22 : 0;
+}
+
+extern "C" int siddon_plane_forward(int dim, const int64_t* vol_dims, const int64_t* det_dims, int nposes,
+                                    const float* P, const float* Pinv, const float* C, const int* fan,
+                                    const float* vol, float* sino, int wu)
+{
+    return plane_forward<float>(dim, vol_dims, det_dims, nposes, P, Pinv, C, fan, vol, sino, wu);
+}
+
+extern "C" int siddon_plane_forward64(int dim, const int64_t* vol_dims, const int64_t* det_dims, int nposes,
+                                      const float* P, const float* Pinv, const float* C, const int* fan,
+                                      const double* vol, double* sino, int wu)
+{
+    return plane_forward<double>(dim, vol_dims, det_dims, nposes, P, Pinv, C, fan, vol, sino, wu);
 }

@@ -327,6 +327,18 @@ def test_joseph_fp64_plane_forward_bitwise(name, ops):
 
 
 @pytest.mark.parametrize("name", list(CASES))
+def test_siddon_fp64_plane_forward_bitwise(name, ops):
+    """SiddonsMethodCUDA<double>'s forward over FP64 plane-major views (FP32 traversal and
+    lengths, FP64 sums in traversal order): the oracle's FP32-geometry / FP64-data forward bit for
+    bit, and the one-thread-per-ray FP64 kernel's bit for bit."""
+    g = make_case(name)
+    x = phantom(g.vol_shape, 24, np.float64)
+    fast = ops("SiddonsMethodCUDA", name, dtype=np.float64).apply(x)
+    np.testing.assert_array_equal(fast, oracle.siddon(g, x, mode=oracle.MODE_DIRECT, acc64=True))
+    np.testing.assert_array_equal(fast, ops("SiddonsMethodCUDA", name, dtype=np.float64, kernels="simple").apply(x))
+
+
+@pytest.mark.parametrize("name", list(CASES))
 def test_joseph_fp64_tiled_back_projection(name, ops):
     """JosephsMethodCUDA<double>'s back projection (LDS tiles, 64-bit fixed-point sums of the
     FP64 contributions): within a few units of FP64 rounding of the oracle's sequential FP64 sums
