@@ -249,14 +249,19 @@ namespace ehp
     }
 
     // The default FP32 gather: the taps of the tiled forward (one reciprocal of the scaling per
-    // voxel and pose instead of a division per tap, the table index as dist * (n - 1) / support),
-    // so forward and back projection weigh every tap alike. ST: float or bfloat16 storage.
+    // voxel and pose instead of a division per tap, tap_weight_fast's square root and table
+    // index), so forward and back projection weigh every tap alike. The table (n_lut <=
+    // kVoxMaxLut entries) and its zero entry are read from LDS. ST: float or bfloat16 storage.
     template <int D, class ST>
     __device__ __forceinline__ void vox_adj_fast(const PoseGeom* __restrict__ geo, const Dims& dims,
                                                  const float* __restrict__ lut, int n_lut, float support,
                                                  const int diff, const ST* __restrict__ sino, ST* __restrict__ vol,
                                                  const int64_t vox)
     {
+        __shared__ float s_lut[kVoxMaxLut + 1];
+        for (int i = threadIdx.x; i <= n_lut; i += blockDim.x)
+            s_lut[i] = i < n_lut ? lut[i] : 0.0f;
+        __syncthreads();
         if (vox >= dims.nvox)
             return;
         const int x = (int) (vox % dims.vol[0]);
@@ -271,10 +276,10 @@ namespace ehp
             const ST* sp = sino + (int64_t) p * dims.pix_per_pose;
             for (int j = fp.lo[1]; j <= fp.hi[1]; ++j) {
                 const float d1 = D == 3 ? (fp.center[1] - (float) j) * rs : 0.0f;
+                const float d1s = d1 * d1;
                 for (int i = fp.lo[0]; i <= fp.hi[0]; ++i) {
                     const float d0 = (fp.center[0] - (float) i) * rs;
-                    const float dist = D == 2 ? sqrtf(d0 * d0) : sqrtf(d0 * d0 + d1 * d1);
-                    acc += tap_weight_scaled<D>(lut, n_lut, kidx, dist, d0, diff != 0) *
+                    acc += tap_weight_fast<D>(s_lut, n_lut, kidx, d0 * d0 + d1s, d0, diff != 0) *
                            up(sp[(int64_t) j * dims.det[0] + i]);
                 }
             }

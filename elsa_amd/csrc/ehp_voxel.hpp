@@ -13,6 +13,9 @@
 
 namespace ehp
 {
+    // Largest table of the LDS-table kernels (tiled forward, FP32 gather back projection)
+    constexpr int kVoxMaxLut = 1024;
+
     template <class T>
     EHP_HD inline T lut_eval(const T* __restrict__ lut, int n, float support, float distance)
     {
@@ -111,20 +114,33 @@ namespace ehp
         return w * (T) d0;
     }
 
-    // tap_weight with the table index formed as distance * k, k = (n - 1) / support (the tiled
-    // forward's form: one multiplication instead of a division per tap)
-    template <int D>
-    EHP_HD inline float tap_weight_scaled(const float* __restrict__ lut, int n, float k, float dist, float d0,
-                                          bool diff)
+    // Square root for the FP32 taps: the hardware's v_sqrt_f32 (within 1 ulp; the correctly
+    // rounded sqrtf adds a dozen instructions per tap on this part); sqrtf on the CPU.
+    EHP_HD inline float tap_sqrt(float x)
     {
-        const float index = dist * k;
-        float w = 0.0f;
-        if (index >= 0.0f && index <= (float) (n - 1)) {
-            const int a = (int) floorf(index);
-            const int b = (int) ceilf(index);
-            const float t = index - (float) a;
-            w = t * lut[b] + (1.0f - t) * lut[a];
-        }
+#if defined(__HIP_DEVICE_COMPILE__)
+        return __builtin_amdgcn_sqrtf(x);
+#else
+        return sqrtf(x);
+#endif
+    }
+
+    // The FP32 kernels' tap weight (tiled forward, gather back projection) from the squared
+    // distance d2 = |d|^2: the table index as |d| * k, k = (n - 1) / support (one multiplication
+    // instead of a division per tap), then lut_eval's interpolation t * lut[b] + (1 - t) * lut[a]
+    // with a = floor(index) and b = a + 1, which is lut_eval's b = ceil(index) wherever they
+    // differ in value (at an integer index t = 0 and the b term is 0), and 0 outside [0, n - 1].
+    // Branch-free: lut holds n + 1 entries, lut[n] = 0, so that b = n stays in the table. In 2D
+    // |d| is |d0|, which the correctly rounded sqrt(d0 * d0) equals (d0 * d0 normal).
+    template <int D>
+    EHP_HD inline float tap_weight_fast(const float* __restrict__ lut, int n, float k, float d2, float d0, bool diff)
+    {
+        const float index = (D == 2 ? fabsf(d0) : tap_sqrt(d2)) * k; // >= 0
+        const float last = (float) (n - 1);
+        const int a = (int) (index < last ? index : last); // floor(index) where index <= n - 1
+        const float t = index - (float) a;
+        float w = t * lut[a + 1] + (1.0f - t) * lut[a];
+        w = index <= last ? w : 0.0f;
         if (!diff)
             return w;
         if (D == 2)

@@ -25,7 +25,7 @@ namespace ehp
     namespace
     {
         constexpr int kThreads = 256;
-        constexpr int kMaxLut = 1024;
+        constexpr int kMaxLut = kVoxMaxLut;
     } // namespace
 
     // Tile-pose id (tile * nposes + pose) of one operator's forward; LT = float (FP32 and
@@ -41,7 +41,7 @@ namespace ehp
                                                         const ST* __restrict__ vol, ST* __restrict__ sino)
     {
         extern __shared__ long long lds[];
-        __shared__ LT s_lut[kMaxLut];
+        __shared__ LT s_lut[kMaxLut + 1]; // the table and one zero entry (tap_weight_fast)
         const int tid = threadIdx.x;
         // id = tile * nposes + pose
         const int pose = (int) (id % dims.nposes);
@@ -53,8 +53,8 @@ namespace ehp
         long long* acc = lds;
         for (int i = tid; i < TU * TV; i += kThreads)
             acc[i] = 0;
-        for (int i = tid; i < n_lut; i += kThreads)
-            s_lut[i] = lut[i];
+        for (int i = tid; i <= n_lut; i += kThreads)
+            s_lut[i] = i < n_lut ? lut[i] : LT(0);
         const int eg = fixexp[0];
         const float lut_index_scale = (float) (n_lut - 1) / support;
         const PoseGeom& g = geo[pose];
@@ -108,16 +108,17 @@ namespace ehp
                     }
                 } else {
                     // the distance as visit() forms it, with one reciprocal of the scaling per voxel
-                    // instead of a division per tap (and the table index as dist * (n - 1) /
-                    // support): within a few ulp of the gather's weights
+                    // instead of a division per tap (and tap_weight_fast's square root and table
+                    // index): within a few ulp of the one-thread-per-voxel kernels' weights, and
+                    // the gather back projection's weights bit for bit
                     const float rs = 1.0f / fp.scaling;
                     for (int j = lo1; j <= hi1; ++j) {
                         const float d1 = D == 3 ? (fp.center[1] - (float) j) * rs : 0.0f;
+                        const float d1s = d1 * d1;
                         for (int i = lo0; i <= hi0; ++i) {
                             const float d0 = (fp.center[0] - (float) i) * rs;
-                            const float dist = D == 2 ? sqrtf(d0 * d0) : sqrtf(d0 * d0 + d1 * d1);
-                            const float w =
-                                tap_weight_scaled<D>(s_lut, n_lut, lut_index_scale, dist, d0, diff != 0);
+                            const float w = tap_weight_fast<D>(s_lut, n_lut, lut_index_scale, d0 * d0 + d1s, d0,
+                                                               diff != 0);
                             if (w != 0.0f)
                                 atomicAdd(reinterpret_cast<unsigned long long*>(acc + (j - v0) * TU + (i - u0)),
                                           fixed_round(xq, (double) w));
@@ -168,10 +169,11 @@ namespace ehp
                                       vols + J.vol_off, sinos + J.sino_off);
     }
 
-    // dynamic LDS (the tile's sums) plus the table (lut_bytes per entry: 4, or 8 for FP64)
+    // dynamic LDS (the tile's sums) plus the table and its zero entry (lut_bytes per entry: 4,
+    // or 8 for FP64)
     size_t voxel_tile_lds_bytes(const TileCfg& tc, size_t lut_bytes)
     {
-        return (size_t) tc.LA * tc.TB * sizeof(long long) + kMaxLut * lut_bytes;
+        return (size_t) tc.LA * tc.TB * sizeof(long long) + (kMaxLut + 1) * lut_bytes;
     }
 
     int voxel_max_lut()

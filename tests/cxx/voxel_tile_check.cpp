@@ -52,6 +52,10 @@ namespace
         double lutmax = 0.0;
         for (int i = 0; i < n_lut; ++i)
             lutmax = std::fabs((double) lut[i]) > lutmax ? std::fabs((double) lut[i]) : lutmax;
+        // the FP32 kernels' LDS table: the entries and one zero entry (tap_weight_fast)
+        std::vector<float> lut_pad(n_lut + 1, 0.0f);
+        for (int i = 0; i < n_lut; ++i)
+            lut_pad[i] = (float) lut[i];
         double ls = 1.0;
         while (ls < lutmax)
             ls *= 2.0;
@@ -131,9 +135,8 @@ namespace
                                     } else {
                                         const float d0 = (fp.center[0] - (float) i) * rs;
                                         const float d1 = D == 3 ? (fp.center[1] - (float) j) * rs : 0.0f;
-                                        const float dist = D == 2 ? sqrtf(d0 * d0) : sqrtf(d0 * d0 + d1 * d1);
-                                        w = (double) tap_weight_scaled<D>((const float*) lut, n_lut, kidx, dist, d0,
-                                                                          false);
+                                        w = (double) tap_weight_fast<D>(lut_pad.data(), n_lut, kidx,
+                                                                        d0 * d0 + d1 * d1, d0, false);
                                     }
                                     if (w == 0.0)
                                         continue;
