@@ -287,13 +287,14 @@ namespace ehp
         vol[vox] = down<ST>(acc);
     }
 
-    template <int D, class ST>
+    // DIFF: the phase-contrast projectors' weights (a compile-time flag, so that the taps of the
+    // others carry no select)
+    template <int D, class ST, bool DIFF>
     __global__ void __launch_bounds__(256)
         voxel_adj_fast_kernel(const PoseGeom* __restrict__ geo, const Dims dims, const float* __restrict__ lut,
-                              int n_lut, float support, const int diff, const ST* __restrict__ sino,
-                              ST* __restrict__ vol)
+                              int n_lut, float support, const ST* __restrict__ sino, ST* __restrict__ vol)
     {
-        vox_adj_fast<D, ST>(geo, dims, lut, n_lut, support, diff, sino, vol,
+        vox_adj_fast<D, ST>(geo, dims, lut, n_lut, support, DIFF ? 1 : 0, sino, vol,
                             (int64_t) blockIdx.x * blockDim.x + threadIdx.x);
     }
 
@@ -352,12 +353,13 @@ namespace ehp
                                   int diff, const ST* sino, ST* vol, hipStream_t s)
     {
         const dim3 grid((unsigned) ((dims.nvox + 255) / 256));
+        auto launch = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, grid, dim3(256), 0, s, geo, dims, lut, n_lut, support, sino, vol);
+        };
         if (dims.dim == 2)
-            hipLaunchKernelGGL((voxel_adj_fast_kernel<2, ST>), grid, dim3(256), 0, s, geo, dims, lut, n_lut, support,
-                               diff, sino, vol);
+            diff ? launch(voxel_adj_fast_kernel<2, ST, true>) : launch(voxel_adj_fast_kernel<2, ST, false>);
         else
-            hipLaunchKernelGGL((voxel_adj_fast_kernel<3, ST>), grid, dim3(256), 0, s, geo, dims, lut, n_lut, support,
-                               diff, sino, vol);
+            diff ? launch(voxel_adj_fast_kernel<3, ST, true>) : launch(voxel_adj_fast_kernel<3, ST, false>);
         return hipGetLastError();
     }
 

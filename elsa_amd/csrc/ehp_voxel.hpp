@@ -135,10 +135,13 @@ namespace ehp
     template <int D>
     EHP_HD inline float tap_weight_fast(const float* __restrict__ lut, int n, float k, float d2, float d0, bool diff)
     {
-        const float index = (D == 2 ? fabsf(d0) : tap_sqrt(d2)) * k; // >= 0
+        // index >= 0, and below sqrt(2) (n - 1) for a tap of the footprint rectangle (|d0|, |d1| at
+        // most the radius in table units), so its integer part fits an int
+        const float index = (D == 2 ? fabsf(d0) : tap_sqrt(d2)) * k;
         const float last = (float) (n - 1);
-        const int a = (int) (index < last ? index : last); // floor(index) where index <= n - 1
-        const float t = index - (float) a;
+        const int ia = (int) index;
+        const int a = ia < n - 1 ? ia : n - 1; // floor(index) where index <= n - 1
+        const float t = fract_pos(index);      // index - a there (exact)
         float w = t * lut[a + 1] + (1.0f - t) * lut[a];
         w = index <= last ? w : 0.0f;
         if (!diff)

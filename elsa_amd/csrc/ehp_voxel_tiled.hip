@@ -137,10 +137,12 @@ namespace ehp
         }
     }
 
-    template <int D, class ST, class LT>
+    // DIFF: the phase-contrast projectors' weights (a compile-time flag, so that the taps of the
+    // others carry no select)
+    template <int D, class ST, class LT, bool DIFF>
     __global__ void __launch_bounds__(kThreads)
         voxel_fwd_tiled_kernel(const PoseGeom* __restrict__ geo, const Dims dims, const LT* __restrict__ lut,
-                               const int n_lut, const float support, const int diff, const int* __restrict__ pose_axis,
+                               const int n_lut, const float support, const int* __restrict__ pose_axis,
                                const float* __restrict__ pose_margin, const int TU, const int TV, const int ntu,
                                const int per_xcd, const int64_t total, const int* __restrict__ fixexp,
                                const ST* __restrict__ vol, ST* __restrict__ sino)
@@ -148,8 +150,8 @@ namespace ehp
         const int64_t id = (int64_t) (blockIdx.x % 8u) * per_xcd + (blockIdx.x / 8u);
         if (id >= total)
             return;
-        vox_fwd_tiled_block<D, ST, LT>(geo, dims, lut, n_lut, support, diff, pose_axis, pose_margin, TU, TV, ntu, id,
-                                       fixexp, vol, sino);
+        vox_fwd_tiled_block<D, ST, LT>(geo, dims, lut, n_lut, support, DIFF ? 1 : 0, pose_axis, pose_margin, TU, TV,
+                                       ntu, id, fixexp, vol, sino);
     }
 
     // the forwards of many operators in one launch (block b: tile-pose b - job.block0 of job
@@ -202,14 +204,14 @@ namespace ehp
             return hipErrorInvalidValue;
         const dim3 grid((unsigned) (per_xcd * 8));
         const size_t lds = (size_t) TU * TV * sizeof(long long);
+        auto launch = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, grid, dim3(kThreads), lds, s, geo, dims, lut, n_lut, support, pose_axis,
+                               pose_margin, TU, TV, ntu, (int) per_xcd, total, d_fixexp, vol, sino);
+        };
         if (dims.dim == 2)
-            hipLaunchKernelGGL((voxel_fwd_tiled_kernel<2, ST, LT>), grid, dim3(kThreads), lds, s, geo, dims, lut,
-                               n_lut, support, diff, pose_axis, pose_margin, TU, TV, ntu, (int) per_xcd, total,
-                               d_fixexp, vol, sino);
+            diff ? launch(voxel_fwd_tiled_kernel<2, ST, LT, true>) : launch(voxel_fwd_tiled_kernel<2, ST, LT, false>);
         else
-            hipLaunchKernelGGL((voxel_fwd_tiled_kernel<3, ST, LT>), grid, dim3(kThreads), lds, s, geo, dims, lut,
-                               n_lut, support, diff, pose_axis, pose_margin, TU, TV, ntu, (int) per_xcd, total,
-                               d_fixexp, vol, sino);
+            diff ? launch(voxel_fwd_tiled_kernel<3, ST, LT, true>) : launch(voxel_fwd_tiled_kernel<3, ST, LT, false>);
         return hipGetLastError();
     }
 

@@ -24,9 +24,14 @@ def printed():
     if not os.path.exists(os.path.join(LIBDIR, "libelsahip.so")):
         pytest.fail("libelsahip.so is not built (python -m elsa_amd.build)")
     os.makedirs(os.path.dirname(OUT), exist_ok=True)
+    # a program of this process (test processes of pytest -n may build it at the same time)
+    exe = f"{OUT}.{os.getpid()}"
     subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror", "-I", os.path.join(ROOT, "include"),
-                    SRC, "-o", OUT, "-L", LIBDIR, "-lelsahip", f"-Wl,-rpath,{LIBDIR}"], check=True)
-    res = subprocess.run([OUT], capture_output=True, text=True, timeout=120)
+                    SRC, "-o", exe, "-L", LIBDIR, "-lelsahip", f"-Wl,-rpath,{LIBDIR}"], check=True)
+    try:
+        res = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    finally:
+        os.remove(exe)
     assert res.returncode == 0, res.stderr
     lines = {}
     for line in res.stdout.splitlines():

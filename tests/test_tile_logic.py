@@ -24,10 +24,18 @@ SRC = os.path.join(ROOT, "tests", "cxx", "tile_logic_check.cpp")
 OUT = os.path.join(ROOT, "tests", "cxx", "_build", f"libtile_logic{SFX}.so")
 
 
+def _build(src, out):
+    """Compile one check into out, through a file of this process renamed into place, so that
+    test processes building the same check at once (pytest -n) never load a partial library."""
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    tmp = f"{out}.{os.getpid()}.tmp"
+    subprocess.run([*CXXFLAGS, "-I", os.path.join(ROOT, "elsa_amd", "csrc"), src, "-o", tmp], check=True)
+    os.replace(tmp, out)
+
+
 @pytest.fixture(scope="module")
 def lib():
-    os.makedirs(os.path.dirname(OUT), exist_ok=True)
-    subprocess.run([*CXXFLAGS, "-I", os.path.join(ROOT, "elsa_amd", "csrc"), SRC, "-o", OUT], check=True)
+    _build(SRC, OUT)
     L = ctypes.CDLL(OUT)
     f32 = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
     i64 = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
@@ -389,8 +397,7 @@ SID_OUT = os.path.join(ROOT, "tests", "cxx", "_build", f"libsiddon_tile{SFX}.so"
 
 @pytest.fixture(scope="module")
 def sid():
-    os.makedirs(os.path.dirname(SID_OUT), exist_ok=True)
-    subprocess.run([*CXXFLAGS, "-I", os.path.join(ROOT, "elsa_amd", "csrc"), SID_SRC, "-o", SID_OUT], check=True)
+    _build(SID_SRC, SID_OUT)
     L = ctypes.CDLL(SID_OUT)
     f32 = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
     i64 = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
@@ -548,8 +555,7 @@ VOX_OUT = os.path.join(ROOT, "tests", "cxx", "_build", f"libvoxel_tile{SFX}.so")
 
 @pytest.fixture(scope="module")
 def vox():
-    os.makedirs(os.path.dirname(VOX_OUT), exist_ok=True)
-    subprocess.run([*CXXFLAGS, "-I", os.path.join(ROOT, "elsa_amd", "csrc"), VOX_SRC, "-o", VOX_OUT], check=True)
+    _build(VOX_SRC, VOX_OUT)
     L = ctypes.CDLL(VOX_OUT)
     f32 = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
     i64 = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
