@@ -252,7 +252,9 @@ namespace ehp
     // voxel and pose instead of a division per tap, tap_weight_fast's square root and table
     // index), so forward and back projection weigh every tap alike. The table (n_lut <=
     // kVoxMaxLut entries) and its zero entry are read from LDS. ST: float or bfloat16 storage.
-    template <int D, class ST>
+    // YSLOW: the thread's index vox counts the voxels with y slowest (x fastest, then z), else in
+    // the volume's order.
+    template <int D, class ST, bool YSLOW = false>
     __device__ __forceinline__ void vox_adj_fast(const PoseGeom* __restrict__ geo, const Dims& dims,
                                                  const float* __restrict__ lut, int n_lut, float support,
                                                  const int diff, const ST* __restrict__ sino, ST* __restrict__ vol,
@@ -265,8 +267,14 @@ namespace ehp
         if (vox >= dims.nvox)
             return;
         const int x = (int) (vox % dims.vol[0]);
-        const int y = (int) ((vox / dims.vol[0]) % dims.vol[1]);
-        const int z = (int) (vox / ((int64_t) dims.vol[0] * dims.vol[1]));
+        int y, z;
+        if (YSLOW) {
+            z = (int) ((vox / dims.vol[0]) % dims.vol[2]);
+            y = (int) (vox / ((int64_t) dims.vol[0] * dims.vol[2]));
+        } else {
+            y = (int) ((vox / dims.vol[0]) % dims.vol[1]);
+            z = (int) (vox / ((int64_t) dims.vol[0] * dims.vol[1]));
+        }
         const float kidx = (float) (n_lut - 1) / support;
         float acc = 0.0f;
         for (int p = 0; p < dims.nposes; ++p) {
@@ -284,18 +292,24 @@ namespace ehp
                 }
             }
         }
-        vol[vox] = down<ST>(acc);
+        vol[x + (int64_t) y * dims.strides[1] + (int64_t) z * dims.strides[2]] = down<ST>(acc);
     }
 
     // DIFF: the phase-contrast projectors' weights (a compile-time flag, so that the taps of the
-    // others carry no select)
+    // others carry no select). In 3D the voxels are taken with y slowest and each XCD gets a
+    // contiguous run of blocks (consecutive blocks land on different XCDs): a slab of y, whose
+    // detector rows over all poses stay in that XCD's L2 when the trajectory turns about y.
     template <int D, class ST, bool DIFF>
     __global__ void __launch_bounds__(256)
         voxel_adj_fast_kernel(const PoseGeom* __restrict__ geo, const Dims dims, const float* __restrict__ lut,
-                              int n_lut, float support, const ST* __restrict__ sino, ST* __restrict__ vol)
+                              int n_lut, float support, const int per_xcd, const ST* __restrict__ sino,
+                              ST* __restrict__ vol)
     {
-        vox_adj_fast<D, ST>(geo, dims, lut, n_lut, support, DIFF ? 1 : 0, sino, vol,
-                            (int64_t) blockIdx.x * blockDim.x + threadIdx.x);
+        const int64_t blk = (int64_t) (blockIdx.x % 8u) * per_xcd + (blockIdx.x / 8u);
+        if (blk * blockDim.x >= dims.nvox)
+            return; // the whole block
+        vox_adj_fast<D, ST, D == 3>(geo, dims, lut, n_lut, support, DIFF ? 1 : 0, sino, vol,
+                                    blk * blockDim.x + threadIdx.x);
     }
 
     // the gathers of many operators in one launch (block b: block b - job.block0 of job
@@ -352,9 +366,13 @@ namespace ehp
     hipError_t voxel_adjoint_fast(const PoseGeom* geo, const Dims& dims, const float* lut, int n_lut, float support,
                                   int diff, const ST* sino, ST* vol, hipStream_t s)
     {
-        const dim3 grid((unsigned) ((dims.nvox + 255) / 256));
+        const int64_t per_xcd = ((dims.nvox + 255) / 256 + 7) / 8;
+        if (per_xcd * 8 > (int64_t) 0x7fffffff)
+            return hipErrorInvalidValue;
+        const dim3 grid((unsigned) (per_xcd * 8));
         auto launch = [&](auto kernel) {
-            hipLaunchKernelGGL(kernel, grid, dim3(256), 0, s, geo, dims, lut, n_lut, support, sino, vol);
+            hipLaunchKernelGGL(kernel, grid, dim3(256), 0, s, geo, dims, lut, n_lut, support, (int) per_xcd, sino,
+                               vol);
         };
         if (dims.dim == 2)
             diff ? launch(voxel_adj_fast_kernel<2, ST, true>) : launch(voxel_adj_fast_kernel<2, ST, false>);
