@@ -250,8 +250,8 @@ namespace ehp
 
     // The default FP32 gather: the taps of the tiled forward (one reciprocal of the scaling per
     // voxel and pose instead of a division per tap, tap_weight_fast's square root and table
-    // index), so forward and back projection weigh every tap alike. The table (n_lut <=
-    // kVoxMaxLut entries) and its zero entry are read from LDS. ST: float or bfloat16 storage.
+    // index), so forward and back projection weigh every tap alike. The pair table of the n_lut <=
+    // kVoxMaxLut entries (lerp_table_entry) is read from LDS. ST: float or bfloat16 storage.
     // YSLOW: the thread's index vox counts the voxels with y slowest (x fastest, then z), else in
     // the volume's order.
     template <int D, class ST, bool YSLOW = false>
@@ -260,9 +260,9 @@ namespace ehp
                                                  const int diff, const ST* __restrict__ sino, ST* __restrict__ vol,
                                                  const int64_t vox)
     {
-        __shared__ float s_lut[kVoxMaxLut + 1];
-        for (int i = threadIdx.x; i <= n_lut; i += blockDim.x)
-            s_lut[i] = i < n_lut ? lut[i] : 0.0f;
+        __shared__ float s_lut[2 * kVoxMaxLut];
+        for (int i = threadIdx.x; i < n_lut; i += blockDim.x)
+            lerp_table_entry(lut, n_lut, i, s_lut);
         __syncthreads();
         if (vox >= dims.nvox)
             return;
@@ -285,9 +285,10 @@ namespace ehp
             for (int j = fp.lo[1]; j <= fp.hi[1]; ++j) {
                 const float d1 = D == 3 ? (fp.center[1] - (float) j) * rs : 0.0f;
                 const float d1s = d1 * d1;
-                for (int i = fp.lo[0]; i <= fp.hi[0]; ++i) {
-                    const float d0 = (fp.center[0] - (float) i) * rs;
-                    acc += tap_weight_fast<D>(s_lut, n_lut, kidx, d0 * d0 + d1s, d0, diff != 0) *
+                float fi = (float) fp.lo[0]; // i as a float (exact), without a conversion per tap
+                for (int i = fp.lo[0]; i <= fp.hi[0]; ++i, fi += 1.0f) {
+                    const float d0 = (fp.center[0] - fi) * rs;
+                    acc += tap_weight_fast<D>(s_lut, n_lut, kidx, __builtin_fmaf(d0, d0, d1s), d0, diff != 0) *
                            up(sp[(int64_t) j * dims.det[0] + i]);
                 }
             }

@@ -125,24 +125,34 @@ namespace ehp
 #endif
     }
 
-    // The FP32 kernels' tap weight (tiled forward, gather back projection) from the squared
-    // distance d2 = |d|^2: the table index as |d| * k, k = (n - 1) / support (one multiplication
-    // instead of a division per tap), then lut_eval's interpolation t * lut[b] + (1 - t) * lut[a]
-    // with a = floor(index) and b = a + 1, which is lut_eval's b = ceil(index) wherever they
-    // differ in value (at an integer index t = 0 and the b term is 0), and 0 outside [0, n - 1].
-    // Branch-free: lut holds n + 1 entries, lut[n] = 0, so that b = n stays in the table. In 2D
-    // |d| is |d0|, which the correctly rounded sqrt(d0 * d0) equals (d0 * d0 normal).
+    // The FP32 kernels' table (tiled forward, gather back projection): entry i of a table of n
+    // values as the pair (lut[i], lut[i + 1] - lut[i]) in tab[2 i], tab[2 i + 1], with lut[n] = 0,
+    // so that one aligned 8-byte read gives an interpolation's two operands.
+    EHP_HD inline void lerp_table_entry(const float* __restrict__ lut, int n, int i, float* __restrict__ tab)
+    {
+        const float next = i + 1 < n ? lut[i + 1] : 0.0f;
+        tab[2 * i] = lut[i];
+        tab[2 * i + 1] = next - lut[i];
+    }
+
+    // The FP32 kernels' tap weight from the squared distance d2 = |d|^2 (d0 * d0 + d1 * d1 with one
+    // rounding): the table index as |d| * k, k = (n - 1) / support (one multiplication instead of
+    // a division per tap), and with a = floor(index), t = index - a the interpolation
+    // lut[a] + t * (lut[a + 1] - lut[a]) in one FMA over the pair table (lerp_table_entry; lut_eval
+    // forms t * lut[b] + (1 - t) * lut[a], b = ceil(index): the same up to FP32 rounding, and
+    // exactly lut[a] at an integer index), 0 outside [0, n - 1]. Branch-free.
     template <int D>
-    EHP_HD inline float tap_weight_fast(const float* __restrict__ lut, int n, float k, float d2, float d0, bool diff)
+    EHP_HD inline float tap_weight_fast(const float* __restrict__ tab, int n, float k, float d2, float d0, bool diff)
     {
         // index >= 0, and below sqrt(2) (n - 1) for a tap of the footprint rectangle (|d0|, |d1| at
-        // most the radius in table units), so its integer part fits an int
+        // most the radius in table units), so its integer part fits an int. In 2D |d| is |d0|,
+        // which the correctly rounded sqrt(d0 * d0) equals (d0 * d0 normal).
         const float index = (D == 2 ? fabsf(d0) : tap_sqrt(d2)) * k;
         const float last = (float) (n - 1);
         const int ia = (int) index;
         const int a = ia < n - 1 ? ia : n - 1; // floor(index) where index <= n - 1
         const float t = fract_pos(index);      // index - a there (exact)
-        float w = t * lut[a + 1] + (1.0f - t) * lut[a];
+        float w = __builtin_fmaf(t, tab[2 * a + 1], tab[2 * a]);
         w = index <= last ? w : 0.0f;
         if (!diff)
             return w;

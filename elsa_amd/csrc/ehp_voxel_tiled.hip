@@ -41,7 +41,8 @@ namespace ehp
                                                         const ST* __restrict__ vol, ST* __restrict__ sino)
     {
         extern __shared__ long long lds[];
-        __shared__ LT s_lut[kMaxLut + 1]; // the table and one zero entry (tap_weight_fast)
+        // FP64: the table; FP32: its pair table (lerp_table_entry, tap_weight_fast)
+        __shared__ LT s_lut[sizeof(LT) == 8 ? kMaxLut : 2 * kMaxLut];
         const int tid = threadIdx.x;
         // id = tile * nposes + pose
         const int pose = (int) (id % dims.nposes);
@@ -53,8 +54,12 @@ namespace ehp
         long long* acc = lds;
         for (int i = tid; i < TU * TV; i += kThreads)
             acc[i] = 0;
-        for (int i = tid; i <= n_lut; i += kThreads)
-            s_lut[i] = i < n_lut ? lut[i] : LT(0);
+        for (int i = tid; i < n_lut; i += kThreads) {
+            if constexpr (sizeof(LT) == 8)
+                s_lut[i] = lut[i];
+            else
+                lerp_table_entry(lut, n_lut, i, s_lut);
+        }
         const int eg = fixexp[0];
         const float lut_index_scale = (float) (n_lut - 1) / support;
         const PoseGeom& g = geo[pose];
@@ -115,10 +120,11 @@ namespace ehp
                     for (int j = lo1; j <= hi1; ++j) {
                         const float d1 = D == 3 ? (fp.center[1] - (float) j) * rs : 0.0f;
                         const float d1s = d1 * d1;
-                        for (int i = lo0; i <= hi0; ++i) {
-                            const float d0 = (fp.center[0] - (float) i) * rs;
-                            const float w = tap_weight_fast<D>(s_lut, n_lut, lut_index_scale, d0 * d0 + d1s, d0,
-                                                               diff != 0);
+                        float fi = (float) lo0; // i as a float (exact), without a conversion per tap
+                        for (int i = lo0; i <= hi0; ++i, fi += 1.0f) {
+                            const float d0 = (fp.center[0] - fi) * rs;
+                            const float w = tap_weight_fast<D>(s_lut, n_lut, lut_index_scale,
+                                                               __builtin_fmaf(d0, d0, d1s), d0, diff != 0);
                             if (w != 0.0f)
                                 atomicAdd(reinterpret_cast<unsigned long long*>(acc + (j - v0) * TU + (i - u0)),
                                           fixed_round(xq, (double) w));
@@ -171,11 +177,11 @@ namespace ehp
                                       vols + J.vol_off, sinos + J.sino_off);
     }
 
-    // dynamic LDS (the tile's sums) plus the table and its zero entry (lut_bytes per entry: 4,
-    // or 8 for FP64)
+    // dynamic LDS (the tile's sums) plus the table: FP64 entries (lut_bytes 8) or the FP32 pair
+    // table (lut_bytes 4), 8 bytes per entry either way
     size_t voxel_tile_lds_bytes(const TileCfg& tc, size_t lut_bytes)
     {
-        return (size_t) tc.LA * tc.TB * sizeof(long long) + (kMaxLut + 1) * lut_bytes;
+        return (size_t) tc.LA * tc.TB * sizeof(long long) + (size_t) kMaxLut * (lut_bytes == 8 ? 8 : 2 * lut_bytes);
     }
 
     int voxel_max_lut()

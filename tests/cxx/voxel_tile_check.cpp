@@ -52,10 +52,12 @@ namespace
         double lutmax = 0.0;
         for (int i = 0; i < n_lut; ++i)
             lutmax = std::fabs((double) lut[i]) > lutmax ? std::fabs((double) lut[i]) : lutmax;
-        // the FP32 kernels' LDS table: the entries and one zero entry (tap_weight_fast)
-        std::vector<float> lut_pad(n_lut + 1, 0.0f);
+        // the FP32 kernels' LDS pair table (lerp_table_entry, tap_weight_fast)
+        std::vector<float> lut_f(n_lut), tab(2 * (size_t) n_lut);
         for (int i = 0; i < n_lut; ++i)
-            lut_pad[i] = (float) lut[i];
+            lut_f[i] = (float) lut[i];
+        for (int i = 0; i < n_lut; ++i)
+            lerp_table_entry(lut_f.data(), n_lut, i, tab.data());
         double ls = 1.0;
         while (ls < lutmax)
             ls *= 2.0;
@@ -135,8 +137,8 @@ namespace
                                     } else {
                                         const float d0 = (fp.center[0] - (float) i) * rs;
                                         const float d1 = D == 3 ? (fp.center[1] - (float) j) * rs : 0.0f;
-                                        w = (double) tap_weight_fast<D>(lut_pad.data(), n_lut, kidx,
-                                                                        d0 * d0 + d1 * d1, d0, false);
+                                        w = (double) tap_weight_fast<D>(tab.data(), n_lut, kidx,
+                                                                        __builtin_fmaf(d0, d0, d1 * d1), d0, false);
                                     }
                                     if (w == 0.0)
                                         continue;
