@@ -1006,6 +1006,29 @@ namespace ehp
         float c1, c2;
     };
 
+    // floor(t) clamped to the padded range [-2, n] of a view (t finite), as floor(med3(t, -2, n)):
+    // the same integer, since -2 and n are integers and floor is monotone (two instructions)
+    EHP_HD inline int pad_floor(float t, int n)
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        return cvt_flr(__builtin_amdgcn_fmed3f(t, -2.0f, (float) n));
+#else
+        const float c = t < -2.0f ? -2.0f : (t > (float) n ? (float) n : t);
+        return (int) floorf(c);
+#endif
+    }
+
+    // The fraction t - floor(t) of a sample position t = q - 0.5 in one instruction (v_fract_f32):
+    // exact for t >= 0; for t < 0 (q below 0.5, at the volume's low faces) within one rounding.
+    EHP_HD inline float pad_fract(float t)
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        return __builtin_amdgcn_fractf(t);
+#else
+        return t - floorf(t);
+#endif
+    }
+
     template <int D, int A, bool FB2, bool V16, bool PR = false>
     EHP_HD inline SampleTaps plane_sample_taps(const RayRec& r, const Dims& dims, const PadView& V, uint32_t poff,
                                                float fs)
@@ -1016,10 +1039,9 @@ namespace ehp
         const char* pb = (const char*) V.base + poff; // the plane (the same for the whole wavefront)
         const float q1 = __builtin_fmaf(fs, r.s1, r.p1);
         const float t1 = q1 - 0.5f;
-        const float g1 = floorf(t1);
-        const int k1 = pad_clamp(g1, n1);
+        const int k1 = pad_floor(t1, n1);
         SampleTaps T;
-        T.c1 = t1 - g1;
+        T.c1 = pad_fract(t1);
         if (D == 2) {
             // offset of element k1 + 2 of the plane's only row
             const Pair fk = ld_pair<V16>(V, pb, ((uint32_t) k1 << ES) + (2u << ES));
@@ -1029,9 +1051,8 @@ namespace ehp
         } else {
             const float q2 = __builtin_fmaf(fs, r.s2, r.p2);
             const float t2 = q2 - 0.5f;
-            const float g2 = floorf(t2);
-            const int k2 = pad_clamp(g2, n2);
-            T.c2 = t2 - g2;
+            const int k2 = pad_floor(t2, n2);
+            T.c2 = pad_fract(t2);
             // the F and F + 1 taps along the fast axis are neighbours: two 8-byte loads, one per
             // row of the slow axis; (slow + 2) * RS + (fast + 2) elements in, the 2s being the padding
             const int ks = FB2 ? k1 : k2, kf = FB2 ? k2 : k1;
@@ -1040,8 +1061,8 @@ namespace ehp
                 // row-pair view: element (ks, kf) holds rows ks and ks + 1 (8 bytes per element in
                 // FP32, 4 in bfloat16)
                 constexpr uint32_t EP = ES + 1; // log2 of the pair element's size
-                const uint32_t o = (uint32_t) (smul24(ks, (int) V.RS4) + (int) (2u * V.RS4 + (2u << EP))) +
-                                   ((uint32_t) kf << EP);
+                const uint32_t o = (uint32_t) mad_i24(ks, (int) V.RS4, (int) (((uint32_t) kf << EP) +
+                                                                              (2u * V.RS4 + (2u << EP))));
                 const Quad q = V16 ? ld_view4_bf16(V, pb, o) : ld_view4(V, pb, o);
                 r0 = Pair{q.a, q.c};
                 r1 = Pair{q.b, q.d};

@@ -161,6 +161,13 @@ namespace ehp
         return w * d0;
     }
 
+    // Row stride (slots) of a detector tile's LDS sums in the tiled voxel forward: odd in 3D, so
+    // that the 8-byte slots of neighbouring detector rows fall on different LDS banks.
+    EHP_HD inline int vox_acc_stride(int TU, int TV)
+    {
+        return TV > 1 ? (TU | 1) : TU;
+    }
+
     // What one detector pixel of the tiled voxel forward can receive, in units of max|x|: at most
     // (2 r s + 2)^(dim-1) voxels per plane (s the largest footprint scaling), each with a
     // weight of at most ls (a power of two at least the largest |weight|), over the longest

@@ -52,7 +52,8 @@ namespace ehp
         const int u1 = (u0 + TU < dims.det[0] ? u0 + TU : dims.det[0]) - 1;
         const int v1 = (v0 + TV < dims.det[1] ? v0 + TV : dims.det[1]) - 1;
         long long* acc = lds;
-        for (int i = tid; i < TU * TV; i += kThreads)
+        const int RS = vox_acc_stride(TU, TV); // slot (v - v0, u - u0) at (v - v0) * RS + u - u0
+        for (int i = tid; i < RS * TV; i += kThreads)
             acc[i] = 0;
         for (int i = tid; i < n_lut; i += kThreads) {
             if constexpr (sizeof(LT) == 8)
@@ -107,7 +108,7 @@ namespace ehp
                             const float dist = D == 2 ? sqrtf(d0 * d0) : sqrtf(d0 * d0 + d1 * d1);
                             const double w = tap_weight<D, double>(s_lut, n_lut, support, dist, d0, diff != 0);
                             if (w != 0.0)
-                                atomicAdd(reinterpret_cast<unsigned long long*>(acc + (j - v0) * TU + (i - u0)),
+                                atomicAdd(reinterpret_cast<unsigned long long*>(acc + (j - v0) * RS + (i - u0)),
                                           fixed_round(xq, w));
                         }
                     }
@@ -115,19 +116,25 @@ namespace ehp
                     // the distance as visit() forms it, with one reciprocal of the scaling per voxel
                     // instead of a division per tap (and tap_weight_fast's square root and table
                     // index): within a few ulp of the one-thread-per-voxel kernels' weights, and
-                    // the gather back projection's weights bit for bit
+                    // the gather back projection's weights bit for bit. Two columns per step, so that
+                    // their table reads are in flight together (the second is skipped past hi0).
                     const float rs = 1.0f / fp.scaling;
                     for (int j = lo1; j <= hi1; ++j) {
                         const float d1 = D == 3 ? (fp.center[1] - (float) j) * rs : 0.0f;
                         const float d1s = d1 * d1;
+                        unsigned long long* row = reinterpret_cast<unsigned long long*>(acc + (j - v0) * RS);
                         float fi = (float) lo0; // i as a float (exact), without a conversion per tap
-                        for (int i = lo0; i <= hi0; ++i, fi += 1.0f) {
-                            const float d0 = (fp.center[0] - fi) * rs;
-                            const float w = tap_weight_fast<D>(s_lut, n_lut, lut_index_scale,
-                                                               __builtin_fmaf(d0, d0, d1s), d0, diff != 0);
-                            if (w != 0.0f)
-                                atomicAdd(reinterpret_cast<unsigned long long*>(acc + (j - v0) * TU + (i - u0)),
-                                          fixed_round(xq, (double) w));
+                        for (int i = lo0; i <= hi0; i += 2, fi += 2.0f) {
+                            const float d0a = (fp.center[0] - fi) * rs;
+                            const float d0b = (fp.center[0] - (fi + 1.0f)) * rs;
+                            const float wa = tap_weight_fast<D>(s_lut, n_lut, lut_index_scale,
+                                                                __builtin_fmaf(d0a, d0a, d1s), d0a, diff != 0);
+                            const float wb = tap_weight_fast<D>(s_lut, n_lut, lut_index_scale,
+                                                                __builtin_fmaf(d0b, d0b, d1s), d0b, diff != 0);
+                            if (wa != 0.0f)
+                                atomicAdd(row + (i - u0), fixed_round(xq, (double) wa));
+                            if (i < hi0 && wb != 0.0f)
+                                atomicAdd(row + (i + 1 - u0), fixed_round(xq, (double) wb));
                         }
                     }
                 }
@@ -139,7 +146,8 @@ namespace ehp
         for (int i = tid; i < TU * TV; i += kThreads) {
             const int u = u0 + i % TU, v = v0 + i / TU;
             if (u <= u1 && v <= v1)
-                sp[(int64_t) v * dims.det[0] + u] = down<ST>((typename Acc<ST>::type) ((double) acc[i] * q));
+                sp[(int64_t) v * dims.det[0] + u] =
+                    down<ST>((typename Acc<ST>::type) ((double) acc[(v - v0) * RS + (u - u0)] * q));
         }
     }
 
@@ -181,7 +189,8 @@ namespace ehp
     // table (lut_bytes 4), 8 bytes per entry either way
     size_t voxel_tile_lds_bytes(const TileCfg& tc, size_t lut_bytes)
     {
-        return (size_t) tc.LA * tc.TB * sizeof(long long) + (size_t) kMaxLut * (lut_bytes == 8 ? 8 : 2 * lut_bytes);
+        return (size_t) vox_acc_stride(tc.LA, tc.TB) * tc.TB * sizeof(long long) +
+               (size_t) kMaxLut * (lut_bytes == 8 ? 8 : 2 * lut_bytes);
     }
 
     int voxel_max_lut()
@@ -209,7 +218,7 @@ namespace ehp
         if (per_xcd * 8 > (int64_t) 0x7fffffff)
             return hipErrorInvalidValue;
         const dim3 grid((unsigned) (per_xcd * 8));
-        const size_t lds = (size_t) TU * TV * sizeof(long long);
+        const size_t lds = (size_t) vox_acc_stride(TU, TV) * TV * sizeof(long long);
         auto launch = [&](auto kernel) {
             hipLaunchKernelGGL(kernel, grid, dim3(kThreads), lds, s, geo, dims, lut, n_lut, support, pose_axis,
                                pose_margin, TU, TV, ntu, (int) per_xcd, total, d_fixexp, vol, sino);
@@ -261,7 +270,7 @@ namespace ehp
             J.block0 = (int64_t) blocks.size();
             blocks.insert(blocks.end(), (size_t) J.ntu * ntv * it.dims.nposes, (int) jobs.size());
             jobs.push_back(J);
-            const size_t need = (size_t) J.TU * J.TV * sizeof(long long);
+            const size_t need = (size_t) vox_acc_stride(J.TU, J.TV) * J.TV * sizeof(long long);
             lds = need > lds ? need : lds;
         }
     }
