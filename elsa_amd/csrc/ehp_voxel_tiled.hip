@@ -26,6 +26,15 @@ namespace ehp
     {
         constexpr int kThreads = 256;
         constexpr int kMaxLut = kVoxMaxLut;
+
+        // The frustum rectangle of one plane: first voxel along b1 and b2, width along b1 and its
+        // reciprocal (rect_split)
+        struct PlaneRect {
+            int blo, clo, nb;
+            float rnb;
+        };
+        // LDS of the plane tables of a chunk of kThreads planes: rectangles, counts and their prefix
+        constexpr size_t kPlaneLds = kThreads * (sizeof(PlaneRect) + sizeof(int)) + (kThreads + 1) * sizeof(int);
     } // namespace
 
     // Tile-pose id (tile * nposes + pose) of one operator's forward; LT = float (FP32 and
@@ -43,6 +52,9 @@ namespace ehp
         extern __shared__ long long lds[];
         // FP64: the table; FP32: its pair table (lerp_table_entry, tap_weight_fast)
         __shared__ LT s_lut[sizeof(LT) == 8 ? kMaxLut : 2 * kMaxLut];
+        __shared__ PlaneRect s_pr[kThreads];
+        __shared__ int s_cnt[kThreads];
+        __shared__ int s_pref[kThreads + 1];
         const int tid = threadIdx.x;
         // id = tile * nposes + pose
         const int pose = (int) (id % dims.nposes);
@@ -71,21 +83,49 @@ namespace ehp
 
         const int b1 = (D == 2) ? 1 - A : (A == 0 ? 1 : 0);
         const int b2 = (D == 2) ? 2 : (A == 2 ? 1 : 2);
-        for (int pa = 0; eg != kNonFinite && pa < dims.vol[A]; ++pa) {
-            int blo, bhi, clo, chi;
-            if (!frustum_plane_range<D>(g, dims, A, pa, cdir, ncorner, blo, bhi, clo, chi))
-                continue;
-            const int nb = bhi - blo + 1;
-            const int cnt = nb * (chi - clo + 1);
-            const float rnb = 1.0f / (float) nb;
-            for (int k = tid; k < cnt; k += kThreads) {
+        // The planes in chunks of up to kThreads: one thread per plane computes the plane's frustum
+        // rectangle (frustum_plane_range) into LDS, then every thread walks its candidates
+        // k = tid, tid + kThreads, ... over all planes of the chunk in order, keeping its plane jp in
+        // registers (candidate k is entry k - pref[jp] of plane jp's rectangle, jp the last plane with
+        // pref[jp] <= k). A chunk holds fewer planes when its candidates could pass 2^31 - 1.
+        const int64_t psz = (int64_t) dims.vol[b1] * (D == 3 ? dims.vol[b2] : 1);
+        int pch = kThreads;
+        while (pch > 1 && (int64_t) pch * psz > 0x7fffffffll)
+            pch >>= 1;
+        for (int pc0 = 0; eg != kNonFinite && pc0 < dims.vol[A]; pc0 += pch) {
+            const int npl = dims.vol[A] - pc0 < pch ? dims.vol[A] - pc0 : pch;
+            if (tid < npl) {
+                int blo, bhi, clo, chi, cnt = 0;
+                if (frustum_plane_range<D>(g, dims, A, pc0 + tid, cdir, ncorner, blo, bhi, clo, chi))
+                    cnt = (bhi - blo + 1) * (chi - clo + 1);
+                const int nb = cnt > 0 ? bhi - blo + 1 : 1;
+                s_pr[tid] = PlaneRect{blo, clo, nb, 1.0f / (float) nb};
+                s_cnt[tid] = cnt;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                s_pref[0] = 0;
+                for (int q = 0; q < npl; ++q)
+                    s_pref[q + 1] = s_pref[q] + s_cnt[q];
+            }
+            __syncthreads();
+            const int total = s_pref[npl];
+            int jp = 0, pj = 0, pn = s_pref[1];
+            PlaneRect pr = s_pr[0];
+            for (int k = tid; k < total; k += kThreads) {
+                while (k >= pn) {
+                    ++jp;
+                    pj = pn;
+                    pn = s_pref[jp + 1];
+                    pr = s_pr[jp];
+                }
                 int row, col;
-                rect_split(k, nb, rnb, row, col);
+                rect_split(k - pj, pr.nb, pr.rnb, row, col);
                 int c[3] = {0, 0, 0};
-                c[A] = pa;
-                c[b1] = blo + col;
+                c[A] = pc0 + jp;
+                c[b1] = pr.blo + col;
                 if (D == 3)
-                    c[b2] = clo + row;
+                    c[b2] = pr.clo + row;
                 const int64_t lin = c[0] + c[1] * dims.strides[1] + (int64_t) c[2] * dims.strides[2];
                 const auto xv = up(vol[lin]);
                 if (xv == 0)
@@ -139,8 +179,8 @@ namespace ehp
                     }
                 }
             }
+            __syncthreads(); // the next chunk rewrites the plane tables
         }
-        __syncthreads();
         const double q = eg == kNonFinite ? __longlong_as_double(0x7ff8000000000000ll) : ldexp(1.0, eg - 62);
         ST* sp = sino + (int64_t) pose * dims.pix_per_pose;
         for (int i = tid; i < TU * TV; i += kThreads) {
@@ -190,7 +230,7 @@ namespace ehp
     size_t voxel_tile_lds_bytes(const TileCfg& tc, size_t lut_bytes)
     {
         return (size_t) vox_acc_stride(tc.LA, tc.TB) * tc.TB * sizeof(long long) +
-               (size_t) kMaxLut * (lut_bytes == 8 ? 8 : 2 * lut_bytes);
+               (size_t) kMaxLut * (lut_bytes == 8 ? 8 : 2 * lut_bytes) + kPlaneLds;
     }
 
     int voxel_max_lut()

@@ -96,17 +96,43 @@ namespace
                 float cdir[12];
                 const int nc = frustum_corners<D>(g, u0, u1, v0, v1, margin[p], cdir);
                 std::vector<char> seen(dims.nvox, 0);
-                for (int pa = 0; pa < dims.vol[A]; ++pa) {
-                    int blo, bhi, clo, chi;
-                    if (!frustum_plane_range<D>(g, dims, A, pa, cdir, nc, blo, bhi, clo, chi))
-                        continue;
-                    for (int cc = clo; cc <= chi; ++cc)
-                        for (int bb = blo; bb <= bhi; ++bb) {
+                // the kernel's walk: the planes in chunks (5 here, so that a tile spans many chunks,
+                // empty planes included), each chunk's candidates taken by kSimThreads threads as
+                // k = t, t + kSimThreads, ... through the prefix of the planes' rectangle sizes
+                constexpr int kChunk = 5, kSimThreads = 64;
+                for (int pc0 = 0; pc0 < dims.vol[A]; pc0 += kChunk) {
+                    const int npl = dims.vol[A] - pc0 < kChunk ? dims.vol[A] - pc0 : kChunk;
+                    int rb[kChunk], rc[kChunk], rn[kChunk], pref[kChunk + 1];
+                    float rr[kChunk];
+                    pref[0] = 0;
+                    for (int q = 0; q < npl; ++q) {
+                        int blo, bhi, clo, chi, cnt = 0;
+                        if (frustum_plane_range<D>(g, dims, A, pc0 + q, cdir, nc, blo, bhi, clo, chi))
+                            cnt = (bhi - blo + 1) * (chi - clo + 1);
+                        rn[q] = cnt > 0 ? bhi - blo + 1 : 1;
+                        rb[q] = blo;
+                        rc[q] = clo;
+                        rr[q] = 1.0f / (float) rn[q];
+                        pref[q + 1] = pref[q] + cnt;
+                    }
+                    const int total = pref[npl];
+                    for (int t = 0; t < kSimThreads; ++t) {
+                        int jp = 0, pj = 0, pn = pref[1];
+                        for (int k = t; k < total; k += kSimThreads) {
+                            while (k >= pn) {
+                                ++jp;
+                                pj = pn;
+                                pn = pref[jp + 1];
+                            }
+                            int row, col;
+                            rect_split(k - pj, rn[jp], rr[jp], row, col);
+                            if (row < 0 || col < 0 || col >= rn[jp])
+                                return 55;
                             int c[3] = {0, 0, 0};
-                            c[A] = pa;
-                            c[b1] = bb;
+                            c[A] = pc0 + jp;
+                            c[b1] = rb[jp] + col;
                             if (D == 3)
-                                c[b2] = cc;
+                                c[b2] = rc[jp] + row;
                             const int64_t lin = c[0] + c[1] * dims.strides[1] + (int64_t) c[2] * dims.strides[2];
                             if (lin < 0 || lin >= dims.nvox || seen[lin])
                                 return 51; // out of range or enumerated twice for one tile
@@ -148,6 +174,7 @@ namespace
                                     acc[(size_t) (j - v0) * TU + (i - u0)] += (__int128) (long long) fixed_round(xq, w);
                                 }
                         }
+                    }
                 }
                 for (int v = v0; v <= v1; ++v)
                     for (int u = u0; u <= u1; ++u) {
