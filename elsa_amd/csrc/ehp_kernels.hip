@@ -248,9 +248,10 @@ namespace ehp
         vol[vox] = down<ST>(acc);
     }
 
-    // The default FP32 gather: the taps of the tiled forward (one reciprocal of the scaling per
-    // voxel and pose instead of a division per tap, tap_weight_fast's square root and table
-    // index), so forward and back projection weigh every tap alike. The pair table of the n_lut <=
+    // The default FP32 gather: the footprints and taps of the tiled forward (footprint_fast, one
+    // reciprocal of the scaling per voxel and pose instead of a division per tap,
+    // tap_weight_fast's square root and table index), so forward and back projection weigh every
+    // tap alike. The pair table of the n_lut <=
     // kVoxMaxLut entries (lerp_table_entry) is read from LDS. ST: float or bfloat16 storage.
     // YSLOW: the thread's index vox counts the voxels with y slowest (x fastest, then z), else in
     // the volume's order.
@@ -279,8 +280,8 @@ namespace ehp
         float acc = 0.0f;
         for (int p = 0; p < dims.nposes; ++p) {
             Footprint fp;
-            footprint<D>(geo[p], dims, x, y, z, support, fp);
-            const float rs = 1.0f / fp.scaling;
+            float rs;
+            footprint_fast<D>(geo[p], dims, x, y, z, support, fp, rs);
             const ST* sp = sino + (int64_t) p * dims.pix_per_pose;
             for (int j = fp.lo[1]; j <= fp.hi[1]; ++j) {
                 const float d1 = D == 3 ? (fp.center[1] - (float) j) * rs : 0.0f;

@@ -130,8 +130,14 @@ namespace ehp
                 const auto xv = up(vol[lin]);
                 if (xv == 0)
                     continue;
+                // FP64: footprint()'s divisions, as the one-thread-per-voxel FP64 kernel; FP32:
+                // footprint_fast, as the gather back projection
                 Footprint fp;
-                footprint<D>(g, dims, c[0], c[1], c[2], support, fp);
+                float rs = 0.0f;
+                if constexpr (sizeof(LT) == 8)
+                    footprint<D>(g, dims, c[0], c[1], c[2], support, fp);
+                else
+                    footprint_fast<D>(g, dims, c[0], c[1], c[2], support, fp, rs);
                 const int lo0 = fp.lo[0] > u0 ? fp.lo[0] : u0, hi0 = fp.hi[0] < u1 ? fp.hi[0] : u1;
                 const int lo1 = fp.lo[1] > v0 ? fp.lo[1] : v0, hi1 = fp.hi[1] < v1 ? fp.hi[1] : v1;
                 if (lo0 > hi0 || lo1 > hi1)
@@ -153,12 +159,11 @@ namespace ehp
                         }
                     }
                 } else {
-                    // the distance as visit() forms it, with one reciprocal of the scaling per voxel
-                    // instead of a division per tap (and tap_weight_fast's square root and table
-                    // index): within a few ulp of the one-thread-per-voxel kernels' weights, and
-                    // the gather back projection's weights bit for bit. Two columns per step, so that
-                    // their table reads are in flight together (the second is skipped past hi0).
-                    const float rs = 1.0f / fp.scaling;
+                    // the distance as visit() forms it, with the reciprocal rs of the scaling per
+                    // voxel instead of a division per tap (and tap_weight_fast's square root and
+                    // table index): within a few ulp of the one-thread-per-voxel kernels' weights,
+                    // and the gather back projection's weights bit for bit. Two columns per step, so
+                    // that their table reads are in flight together (the second is skipped past hi0).
                     for (int j = lo1; j <= hi1; ++j) {
                         const float d1 = D == 3 ? (fp.center[1] - (float) j) * rs : 0.0f;
                         const float d1s = d1 * d1;

@@ -78,7 +78,11 @@ namespace
                 const int y = (int) ((lin / dims.vol[0]) % dims.vol[1]);
                 const int z = (int) (lin / ((int64_t) dims.vol[0] * dims.vol[1]));
                 Footprint fp;
-                footprint<D>(g, dims, x, y, z, support, fp);
+                float rs_unused;
+                if (sizeof(T) == 8)
+                    footprint<D>(g, dims, x, y, z, support, fp);
+                else
+                    footprint_fast<D>(g, dims, x, y, z, support, fp, rs_unused);
                 const int nu = fp.hi[0] - fp.lo[0] + 1, nv = fp.hi[1] - fp.lo[1] + 1;
                 if (nu > 0 && nv > 0)
                     taps_full += (double) nu * nv;
@@ -138,7 +142,12 @@ namespace
                                 return 51; // out of range or enumerated twice for one tile
                             seen[lin] = 1;
                             Footprint fp;
-                            footprint<D>(g, dims, c[0], c[1], c[2], support, fp);
+                            // the kernel's footprints: footprint() for FP64, footprint_fast for FP32
+                            float rs = 0.0f;
+                            if (sizeof(T) == 8)
+                                footprint<D>(g, dims, c[0], c[1], c[2], support, fp);
+                            else
+                                footprint_fast<D>(g, dims, c[0], c[1], c[2], support, fp, rs);
                             const int lo0 = fp.lo[0] > u0 ? fp.lo[0] : u0, hi0 = fp.hi[0] < u1 ? fp.hi[0] : u1;
                             const int lo1 = fp.lo[1] > v0 ? fp.lo[1] : v0, hi1 = fp.hi[1] < v1 ? fp.hi[1] : v1;
                             if (lo0 > hi0 || lo1 > hi1)
@@ -150,7 +159,6 @@ namespace
                             const double xq = ldexp((double) xv, 62 - eg);
                             // the GPU's arithmetic: FP32, one reciprocal of the scaling per voxel and
                             // the table index as dist * (n - 1) / support; FP64, visit()'s divisions
-                            const float rs = 1.0f / fp.scaling;
                             const float kidx = (float) (n_lut - 1) / support;
                             for (int j = lo1; j <= hi1; ++j)
                                 for (int i = lo0; i <= hi0; ++i) {
