@@ -28,33 +28,45 @@ namespace ehp
         constexpr int kMaxLut = kVoxMaxLut;
 
         // The frustum rectangle of one plane: first voxel along b1 and b2, width along b1 and its
-        // reciprocal (rect_split)
+        // reciprocal (rect_split), and the linear index of its first voxel (volumes below 2^31
+        // voxels with strides below 2^24, whose indices are formed with 24-bit multiplications)
         struct PlaneRect {
             int blo, clo, nb;
             float rnb;
+            uint32_t base;
         };
         // LDS of the plane tables of a chunk of kThreads planes: rectangles, counts and their prefix
         constexpr size_t kPlaneLds = kThreads * (sizeof(PlaneRect) + sizeof(int)) + (kThreads + 1) * sizeof(int);
     } // namespace
 
-    // Tile-pose id (tile * nposes + pose) of one operator's forward; LT = float (FP32 and
-    // bfloat16 storage) or double (FP64) is the table's type.
-    template <int D, class ST, class LT = float>
-    __device__ __forceinline__ void vox_fwd_tiled_block(const PoseGeom* __restrict__ geo, const Dims& dims,
-                                                        const LT* __restrict__ lut, const int n_lut,
-                                                        const float support, const int diff,
-                                                        const int* __restrict__ pose_axis,
-                                                        const float* __restrict__ pose_margin, const int TU,
-                                                        const int TV, const int ntu, const int64_t id,
-                                                        const int* __restrict__ fixexp,
-                                                        const ST* __restrict__ vol, ST* __restrict__ sino)
+    // Tile-pose id (tile * nposes + pose) of one operator's forward, whose pose has driving axis
+    // A; LT = float (FP32 and bfloat16 storage) or double (FP64) is the table's type.
+    // The static LDS of the forward, declared once per kernel (each instantiation of a function
+    // declaring its own would get its own copy): FP64 the table, FP32 its pair table
+    // (lerp_table_entry, tap_weight_fast); the plane tables of a chunk.
+    template <class LT>
+    struct VoxFwdLds {
+        LT* lut;
+        PlaneRect* pr;
+        int* cnt;
+        int* pref;
+    };
+
+    template <int D, int A, class ST, class LT>
+    __device__ __forceinline__ void vox_fwd_tiled_axis(const PoseGeom* __restrict__ geo, const Dims& dims,
+                                                       const LT* __restrict__ lut, const int n_lut,
+                                                       const float support, const int diff,
+                                                       const float* __restrict__ pose_margin, const int TU,
+                                                       const int TV, const int ntu, const int64_t id,
+                                                       const int* __restrict__ fixexp,
+                                                       const ST* __restrict__ vol, ST* __restrict__ sino,
+                                                       const VoxFwdLds<LT>& L)
     {
         extern __shared__ long long lds[];
-        // FP64: the table; FP32: its pair table (lerp_table_entry, tap_weight_fast)
-        __shared__ LT s_lut[sizeof(LT) == 8 ? kMaxLut : 2 * kMaxLut];
-        __shared__ PlaneRect s_pr[kThreads];
-        __shared__ int s_cnt[kThreads];
-        __shared__ int s_pref[kThreads + 1];
+        LT* s_lut = L.lut;
+        PlaneRect* s_pr = L.pr;
+        int* s_cnt = L.cnt;
+        int* s_pref = L.pref;
         const int tid = threadIdx.x;
         // id = tile * nposes + pose
         const int pose = (int) (id % dims.nposes);
@@ -76,13 +88,15 @@ namespace ehp
         const int eg = fixexp[0];
         const float lut_index_scale = (float) (n_lut - 1) / support;
         const PoseGeom& g = geo[pose];
-        const int A = pose_axis[pose];
         float cdir[12];
         const int ncorner = frustum_corners<D>(g, u0, u1, v0, v1, pose_margin[pose], cdir);
         __syncthreads();
 
-        const int b1 = (D == 2) ? 1 - A : (A == 0 ? 1 : 0);
-        const int b2 = (D == 2) ? 2 : (A == 2 ? 1 : 2);
+        constexpr int b1 = (D == 2) ? 1 - A : (A == 0 ? 1 : 0);
+        constexpr int b2 = (D == 2) ? 2 : (A == 2 ? 1 : 2);
+        const bool small = dims.nvox < (1ll << 31) && dims.strides[1] < (1 << 24) && dims.strides[2] < (1 << 24);
+        const uint32_t sA = (uint32_t) dims.strides[A], s1 = (uint32_t) dims.strides[b1];
+        const uint32_t s2 = D == 3 ? (uint32_t) dims.strides[b2] : 0u;
         // The planes in chunks of up to kThreads: one thread per plane computes the plane's frustum
         // rectangle (frustum_plane_range) into LDS, then every thread walks its candidates
         // k = tid, tid + kThreads, ... over all planes of the chunk in order, keeping its plane jp in
@@ -99,7 +113,10 @@ namespace ehp
                 if (frustum_plane_range<D>(g, dims, A, pc0 + tid, cdir, ncorner, blo, bhi, clo, chi))
                     cnt = (bhi - blo + 1) * (chi - clo + 1);
                 const int nb = cnt > 0 ? bhi - blo + 1 : 1;
-                s_pr[tid] = PlaneRect{blo, clo, nb, 1.0f / (float) nb};
+                const uint32_t base = small && cnt > 0
+                                          ? (uint32_t) (pc0 + tid) * sA + (uint32_t) blo * s1 + (uint32_t) clo * s2
+                                          : 0u;
+                s_pr[tid] = PlaneRect{blo, clo, nb, 1.0f / (float) nb, base};
                 s_cnt[tid] = cnt;
             }
             __syncthreads();
@@ -126,7 +143,8 @@ namespace ehp
                 c[b1] = pr.blo + col;
                 if (D == 3)
                     c[b2] = pr.clo + row;
-                const int64_t lin = c[0] + c[1] * dims.strides[1] + (int64_t) c[2] * dims.strides[2];
+                const int64_t lin = small ? (int64_t) (pr.base + mul24((uint32_t) col, s1) + mul24((uint32_t) row, s2))
+                                          : c[0] + c[1] * dims.strides[1] + (int64_t) c[2] * dims.strides[2];
                 const auto xv = up(vol[lin]);
                 if (xv == 0)
                     continue;
@@ -194,6 +212,34 @@ namespace ehp
                 sp[(int64_t) v * dims.det[0] + u] =
                     down<ST>((typename Acc<ST>::type) ((double) acc[(v - v0) * RS + (u - u0)] * q));
         }
+    }
+
+    // Tile-pose id of one operator's forward: the pose's driving axis as a template parameter
+    template <int D, class ST, class LT = float>
+    __device__ __forceinline__ void vox_fwd_tiled_block(const PoseGeom* __restrict__ geo, const Dims& dims,
+                                                        const LT* __restrict__ lut, const int n_lut,
+                                                        const float support, const int diff,
+                                                        const int* __restrict__ pose_axis,
+                                                        const float* __restrict__ pose_margin, const int TU,
+                                                        const int TV, const int ntu, const int64_t id,
+                                                        const int* __restrict__ fixexp,
+                                                        const ST* __restrict__ vol, ST* __restrict__ sino)
+    {
+        __shared__ LT s_lut[sizeof(LT) == 8 ? kMaxLut : 2 * kMaxLut];
+        __shared__ PlaneRect s_pr[kThreads];
+        __shared__ int s_cnt[kThreads];
+        __shared__ int s_pref[kThreads + 1];
+        const VoxFwdLds<LT> L{s_lut, s_pr, s_cnt, s_pref};
+        const int A = pose_axis[(int) (id % dims.nposes)];
+        if (A == 0)
+            vox_fwd_tiled_axis<D, 0, ST, LT>(geo, dims, lut, n_lut, support, diff, pose_margin, TU, TV, ntu, id,
+                                             fixexp, vol, sino, L);
+        else if (D == 2 || A == 1)
+            vox_fwd_tiled_axis<D, 1, ST, LT>(geo, dims, lut, n_lut, support, diff, pose_margin, TU, TV, ntu, id,
+                                             fixexp, vol, sino, L);
+        else
+            vox_fwd_tiled_axis<D, D == 3 ? 2 : 1, ST, LT>(geo, dims, lut, n_lut, support, diff, pose_margin, TU, TV,
+                                                          ntu, id, fixexp, vol, sino, L);
     }
 
     // DIFF: the phase-contrast projectors' weights (a compile-time flag, so that the taps of the
