@@ -116,15 +116,16 @@ int ehp_joseph_create(const ehp_geometry* g, int dtype, int flags, ehp_op** out)
 
 /* Siddon's method (elsa: SiddonsMethodCUDA(volume, detector), SiddonsMethodCUDA.h:71-72,
  * traversal of TraverseAABB.cpp:6-184, crossing parameters T0 + k * tDelta with one rounding
- * each). FP32 back projections run LDS-tiled; flags: 0 or EHP_SIMPLE_KERNELS. */
+ * each). FP32 and FP64 forward projections walk plane-major views of the volume and back
+ * projections run LDS-tiled; flags: 0 or EHP_SIMPLE_KERNELS. */
 int ehp_siddon_create(const ehp_geometry* g, int dtype, int flags, ehp_op** out);
 
 /* Voxel-driven projector with a radially symmetric basis function sampled in a table of
  * n_lut values over [0, support] (elsa: BlobVoxelProjectorCUDA / BSplineVoxelProjectorCUDA,
  * VoxelProjectorCUDA.h:39-43, :82-84; the table is ProjectedBlob::get_lut() or
  * ProjectedBSpline::get_lut(), Blobs.h:249-317, BSplines.h:132-208). lut is host memory of
- * the op's dtype (FP32 for EHP_BF16 operators, which compute in FP32). FP32 and bfloat16
- * forward projections run on LDS-tiled detector tiles; flags: 0,
+ * the op's dtype (FP32 for EHP_BF16 operators, which compute in FP32). Forward projections
+ * (FP32, bfloat16, FP64) run on LDS-tiled detector tiles; flags: 0,
  * EHP_SIMPLE_KERNELS (one thread per voxel and pose, atomic splat) and EHP_DIFFERENTIAL (the
  * phase-contrast projectors PhaseContrastBlobVoxelProjectorCUDA /
  * PhaseContrastBSplineVoxelProjectorCUDA, VoxelProjectorCUDA.h:109-191). */
