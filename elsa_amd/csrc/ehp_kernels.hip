@@ -9,6 +9,8 @@
 // instead of accumulating the step, so the forward and the adjoint see identical weights.
 #include "ehp_kernels.hpp"
 
+#include <type_traits>
+
 namespace ehp
 {
     // ---------------------------------------------------------------------------------
@@ -261,8 +263,8 @@ namespace ehp
                                                  const int diff, const ST* __restrict__ sino, ST* __restrict__ vol,
                                                  const int64_t vox)
     {
-        __shared__ float s_lut[2 * kVoxMaxLut];
-        for (int i = threadIdx.x; i < n_lut; i += blockDim.x)
+        __shared__ float s_lut[2 * kVoxLutSpan];
+        for (int i = threadIdx.x; i < lerp_table_size(n_lut); i += blockDim.x)
             lerp_table_entry(lut, n_lut, i, s_lut);
         __syncthreads();
         if (vox >= dims.nvox)
@@ -277,23 +279,30 @@ namespace ehp
             z = (int) (vox / ((int64_t) dims.vol[0] * dims.vol[1]));
         }
         const float kidx = (float) (n_lut - 1) / support;
-        float acc = 0.0f;
-        for (int p = 0; p < dims.nposes; ++p) {
-            Footprint fp;
-            float rs;
-            footprint_fast<D>(geo[p], dims, x, y, z, support, fp, rs);
-            const ST* sp = sino + (int64_t) p * dims.pix_per_pose;
-            for (int j = fp.lo[1]; j <= fp.hi[1]; ++j) {
-                const float d1 = D == 3 ? (fp.center[1] - (float) j) * rs : 0.0f;
-                const float d1s = d1 * d1;
-                float fi = (float) fp.lo[0]; // i as a float (exact), without a conversion per tap
-                for (int i = fp.lo[0]; i <= fp.hi[0]; ++i, fi += 1.0f) {
-                    const float d0 = (fp.center[0] - fi) * rs;
-                    acc += tap_weight_fast<D>(s_lut, n_lut, kidx, __builtin_fmaf(d0, d0, d1s), d0, diff != 0) *
-                           up(sp[(int64_t) j * dims.det[0] + i]);
+        // a table ending in 0 takes tap_weight_fast's TZ form
+        auto gather = [&](auto tzc) {
+            constexpr bool TZ = decltype(tzc)::value;
+            float acc = 0.0f;
+            for (int p = 0; p < dims.nposes; ++p) {
+                Footprint fp;
+                float rs;
+                footprint_fast<D>(geo[p], dims, x, y, z, support, fp, rs);
+                const ST* sp = sino + (int64_t) p * dims.pix_per_pose;
+                for (int j = fp.lo[1]; j <= fp.hi[1]; ++j) {
+                    const float d1 = D == 3 ? (fp.center[1] - (float) j) * rs : 0.0f;
+                    const float d1s = d1 * d1;
+                    float fi = (float) fp.lo[0]; // i as a float (exact), without a conversion per tap
+                    for (int i = fp.lo[0]; i <= fp.hi[0]; ++i, fi += 1.0f) {
+                        const float d0 = (fp.center[0] - fi) * rs;
+                        acc += tap_weight_fast<D, TZ>(s_lut, n_lut, kidx, __builtin_fmaf(d0, d0, d1s), d0,
+                                                      diff != 0) *
+                               up(sp[(int64_t) j * dims.det[0] + i]);
+                    }
                 }
             }
-        }
+            return acc;
+        };
+        const float acc = lut[n_lut - 1] == 0.0f ? gather(std::true_type{}) : gather(std::false_type{});
         vol[x + (int64_t) y * dims.strides[1] + (int64_t) z * dims.strides[2]] = down<ST>(acc);
     }
 

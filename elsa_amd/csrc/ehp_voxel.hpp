@@ -16,6 +16,15 @@ namespace ehp
     // Largest table of the LDS-table kernels (tiled forward, FP32 gather back projection)
     constexpr int kVoxMaxLut = 1024;
 
+    // Entries of the FP32 kernels' pair table of an n-entry table: the entries and zeros past
+    // sqrt(2) (n - 1), the largest index a tap of a footprint rectangle reaches (|d0|, |d1| at most
+    // the radius, up to rounding), plus the entry after it.
+    EHP_HD inline int lerp_table_size(int n)
+    {
+        return (int) (1.4143f * (float) (n - 1)) + 3;
+    }
+    constexpr int kVoxLutSpan = (int) (1.4143f * (float) (kVoxMaxLut - 1)) + 3;
+
     template <class T>
     EHP_HD inline T lut_eval(const T* __restrict__ lut, int n, float support, float distance)
     {
@@ -190,14 +199,16 @@ namespace ehp
 #endif
     }
 
-    // The FP32 kernels' table (tiled forward, gather back projection): entry i of a table of n
-    // values as the pair (lut[i], lut[i + 1] - lut[i]) in tab[2 i], tab[2 i + 1], with lut[n] = 0,
-    // so that one aligned 8-byte read gives an interpolation's two operands.
+    // The FP32 kernels' table (tiled forward, gather back projection): entry i < lerp_table_size(n)
+    // of a table of n values as the pair (lut[i], lut[i + 1] - lut[i]) in tab[2 i], tab[2 i + 1],
+    // with lut[i] = 0 for i >= n, so that one aligned 8-byte read gives an interpolation's two
+    // operands.
     EHP_HD inline void lerp_table_entry(const float* __restrict__ lut, int n, int i, float* __restrict__ tab)
     {
+        const float cur = i < n ? lut[i] : 0.0f;
         const float next = i + 1 < n ? lut[i + 1] : 0.0f;
-        tab[2 * i] = lut[i];
-        tab[2 * i + 1] = next - lut[i];
+        tab[2 * i] = cur;
+        tab[2 * i + 1] = next - cur;
     }
 
     // The FP32 kernels' tap weight from the squared distance d2 = |d|^2 (d0 * d0 + d1 * d1 with one
@@ -205,8 +216,10 @@ namespace ehp
     // a division per tap), and with a = floor(index), t = index - a the interpolation
     // lut[a] + t * (lut[a + 1] - lut[a]) in one FMA over the pair table (lerp_table_entry; lut_eval
     // forms t * lut[b] + (1 - t) * lut[a], b = ceil(index): the same up to FP32 rounding, and
-    // exactly lut[a] at an integer index), 0 outside [0, n - 1]. Branch-free.
-    template <int D>
+    // exactly lut[a] at an integer index), 0 outside [0, n - 1]. Branch-free. TZ: the table's last
+    // entry is 0 (blob and B-spline tables vanish at the support), so that past n - 1 the zero
+    // pairs of the padded table give 0 by themselves and the clamp and range test drop out.
+    template <int D, bool TZ = false>
     EHP_HD inline float tap_weight_fast(const float* __restrict__ tab, int n, float k, float d2, float d0, bool diff)
     {
         // index >= 0, and below sqrt(2) (n - 1) for a tap of the footprint rectangle (|d0|, |d1| at
@@ -215,10 +228,11 @@ namespace ehp
         const float index = (D == 2 ? fabsf(d0) : tap_sqrt(d2)) * k;
         const float last = (float) (n - 1);
         const int ia = (int) index;
-        const int a = ia < n - 1 ? ia : n - 1; // floor(index) where index <= n - 1
-        const float t = fract_pos(index);      // index - a there (exact)
+        const int a = TZ ? ia : (ia < n - 1 ? ia : n - 1); // floor(index) where index <= n - 1
+        const float t = fract_pos(index);                  // index - a there (exact)
         float w = __builtin_fmaf(t, tab[2 * a + 1], tab[2 * a]);
-        w = index <= last ? w : 0.0f;
+        if (!TZ)
+            w = index <= last ? w : 0.0f;
         if (!diff)
             return w;
         if (D == 2)

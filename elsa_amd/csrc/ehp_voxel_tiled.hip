@@ -20,6 +20,8 @@
 // divisions of VoxelComputation.h), so their terms are that kernel's.
 #include "ehp_kernels.hpp"
 
+#include <type_traits>
+
 namespace ehp
 {
     namespace
@@ -79,7 +81,7 @@ namespace ehp
         const int RS = vox_acc_stride(TU, TV); // slot (v - v0, u - u0) at (v - v0) * RS + u - u0
         for (int i = tid; i < RS * TV; i += kThreads)
             acc[i] = 0;
-        for (int i = tid; i < n_lut; i += kThreads) {
+        for (int i = tid; i < (sizeof(LT) == 8 ? n_lut : lerp_table_size(n_lut)); i += kThreads) {
             if constexpr (sizeof(LT) == 8)
                 s_lut[i] = lut[i];
             else
@@ -87,6 +89,7 @@ namespace ehp
         }
         const int eg = fixexp[0];
         const float lut_index_scale = (float) (n_lut - 1) / support;
+        const bool lut_tail_zero = lut[n_lut - 1] == LT(0); // tap_weight_fast's TZ (FP32)
         const PoseGeom& g = geo[pose];
         float cdir[12];
         const int ncorner = frustum_corners<D>(g, u0, u1, v0, v1, pose_margin[pose], cdir);
@@ -182,24 +185,32 @@ namespace ehp
                     // table index): within a few ulp of the one-thread-per-voxel kernels' weights,
                     // and the gather back projection's weights bit for bit. Two columns per step, so
                     // that their table reads are in flight together (the second is skipped past hi0).
-                    for (int j = lo1; j <= hi1; ++j) {
-                        const float d1 = D == 3 ? (fp.center[1] - (float) j) * rs : 0.0f;
-                        const float d1s = d1 * d1;
-                        unsigned long long* row = reinterpret_cast<unsigned long long*>(acc + (j - v0) * RS);
-                        float fi = (float) lo0; // i as a float (exact), without a conversion per tap
-                        for (int i = lo0; i <= hi0; i += 2, fi += 2.0f) {
-                            const float d0a = (fp.center[0] - fi) * rs;
-                            const float d0b = (fp.center[0] - (fi + 1.0f)) * rs;
-                            const float wa = tap_weight_fast<D>(s_lut, n_lut, lut_index_scale,
-                                                                __builtin_fmaf(d0a, d0a, d1s), d0a, diff != 0);
-                            const float wb = tap_weight_fast<D>(s_lut, n_lut, lut_index_scale,
-                                                                __builtin_fmaf(d0b, d0b, d1s), d0b, diff != 0);
-                            if (wa != 0.0f)
-                                atomicAdd(row + (i - u0), fixed_round(xq, (double) wa));
-                            if (i < hi0 && wb != 0.0f)
-                                atomicAdd(row + (i + 1 - u0), fixed_round(xq, (double) wb));
+                    // A table ending in 0 takes tap_weight_fast's TZ form.
+                    auto taps = [&](auto tzc) {
+                        constexpr bool TZ = decltype(tzc)::value;
+                        for (int j = lo1; j <= hi1; ++j) {
+                            const float d1 = D == 3 ? (fp.center[1] - (float) j) * rs : 0.0f;
+                            const float d1s = d1 * d1;
+                            unsigned long long* row = reinterpret_cast<unsigned long long*>(acc + (j - v0) * RS);
+                            float fi = (float) lo0; // i as a float (exact), without a conversion per tap
+                            for (int i = lo0; i <= hi0; i += 2, fi += 2.0f) {
+                                const float d0a = (fp.center[0] - fi) * rs;
+                                const float d0b = (fp.center[0] - (fi + 1.0f)) * rs;
+                                const float wa = tap_weight_fast<D, TZ>(s_lut, n_lut, lut_index_scale,
+                                                                        __builtin_fmaf(d0a, d0a, d1s), d0a, diff != 0);
+                                const float wb = tap_weight_fast<D, TZ>(s_lut, n_lut, lut_index_scale,
+                                                                        __builtin_fmaf(d0b, d0b, d1s), d0b, diff != 0);
+                                if (wa != 0.0f)
+                                    atomicAdd(row + (i - u0), fixed_round(xq, (double) wa));
+                                if (i < hi0 && wb != 0.0f)
+                                    atomicAdd(row + (i + 1 - u0), fixed_round(xq, (double) wb));
+                            }
                         }
-                    }
+                    };
+                    if (lut_tail_zero)
+                        taps(std::true_type{});
+                    else
+                        taps(std::false_type{});
                 }
             }
             __syncthreads(); // the next chunk rewrites the plane tables
@@ -225,7 +236,7 @@ namespace ehp
                                                         const int* __restrict__ fixexp,
                                                         const ST* __restrict__ vol, ST* __restrict__ sino)
     {
-        __shared__ LT s_lut[sizeof(LT) == 8 ? kMaxLut : 2 * kMaxLut];
+        __shared__ LT s_lut[sizeof(LT) == 8 ? kMaxLut : 2 * kVoxLutSpan];
         __shared__ PlaneRect s_pr[kThreads];
         __shared__ int s_cnt[kThreads];
         __shared__ int s_pref[kThreads + 1];
@@ -281,7 +292,7 @@ namespace ehp
     size_t voxel_tile_lds_bytes(const TileCfg& tc, size_t lut_bytes)
     {
         return (size_t) vox_acc_stride(tc.LA, tc.TB) * tc.TB * sizeof(long long) +
-               (size_t) kMaxLut * (lut_bytes == 8 ? 8 : 2 * lut_bytes) + kPlaneLds;
+               (lut_bytes == 8 ? (size_t) kMaxLut * 8 : (size_t) kVoxLutSpan * 2 * lut_bytes) + kPlaneLds;
     }
 
     int voxel_max_lut()

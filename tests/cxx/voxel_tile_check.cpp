@@ -53,11 +53,12 @@ namespace
         for (int i = 0; i < n_lut; ++i)
             lutmax = std::fabs((double) lut[i]) > lutmax ? std::fabs((double) lut[i]) : lutmax;
         // the FP32 kernels' LDS pair table (lerp_table_entry, tap_weight_fast)
-        std::vector<float> lut_f(n_lut), tab(2 * (size_t) n_lut);
+        std::vector<float> lut_f(n_lut), tab(2 * (size_t) lerp_table_size(n_lut));
         for (int i = 0; i < n_lut; ++i)
             lut_f[i] = (float) lut[i];
-        for (int i = 0; i < n_lut; ++i)
+        for (int i = 0; i < lerp_table_size(n_lut); ++i)
             lerp_table_entry(lut_f.data(), n_lut, i, tab.data());
+        const bool tail_zero = lut_f[n_lut - 1] == 0.0f; // the kernels' tap_weight_fast<D, true>
         double ls = 1.0;
         while (ls < lutmax)
             ls *= 2.0;
@@ -171,8 +172,11 @@ namespace
                                     } else {
                                         const float d0 = (fp.center[0] - (float) i) * rs;
                                         const float d1 = D == 3 ? (fp.center[1] - (float) j) * rs : 0.0f;
-                                        w = (double) tap_weight_fast<D>(tab.data(), n_lut, kidx,
-                                                                        __builtin_fmaf(d0, d0, d1 * d1), d0, false);
+                                        const float d2 = __builtin_fmaf(d0, d0, d1 * d1);
+                                        w = (double) (tail_zero
+                                                          ? tap_weight_fast<D, true>(tab.data(), n_lut, kidx, d2, d0, false)
+                                                          : tap_weight_fast<D, false>(tab.data(), n_lut, kidx, d2, d0,
+                                                                                      false));
                                     }
                                     if (w == 0.0)
                                         continue;

@@ -240,6 +240,32 @@ def test_voxel_parity(name, kind, kernels, ops):
         np.testing.assert_array_equal(op.apply(x), fwd)
 
 
+@pytest.mark.parametrize("name", ["fan2d_256", "cone3d_wide"])
+def test_voxel_table_with_nonzero_tail(name):
+    """A table whose last entry is not 0 (elsa's blob and B-spline tables vanish at the support,
+    and the fast kernels then drop their range test): the tiled forward and the gather weigh taps
+    past the table's end with 0, as the oracle does."""
+    import elsa_amd as E
+    from elsa_amd.projectors import _VoxelProjector
+
+    class RaisedBlob(E.BlobVoxelProjectorCUDA):
+        def __init__(self, vd, dd):
+            self.radius, self.alpha, self.order = 2.0, 10.83, 2
+            lut, self.support = E.blob_lut(2.0, 10.83, 2, 100, np.float32)
+            self.lut = (lut + np.float32(0.05)).astype(np.float32)
+            self._set_kernels("fast")
+            _VoxelProjector.__init__(self, vd, dd, np.float32)
+
+    g = make_case(name)
+    vd, dd = geom_to_descriptors(g)
+    op = RaisedBlob(vd, dd)
+    assert op.lut[-1] != 0.0
+    x = phantom(g.vol_shape, 9)
+    y = phantom(g.sino_shape, 10)
+    assert rel_l2(op.apply(x), oracle.voxel(g, x, op.lut, 2.0)) <= 1e-6
+    assert rel_l2(op.applyAdjoint(y), oracle.voxel(g, y, op.lut, 2.0, adjoint=True)) <= 1e-6
+
+
 @pytest.mark.parametrize("name", ["fan2d_256", "cone3d_wide", "fan2d_wide"])
 @pytest.mark.parametrize("kind", ["blob", "bspline"])
 @pytest.mark.parametrize("kernels", ["fast", "simple"])

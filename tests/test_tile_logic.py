@@ -574,6 +574,9 @@ VOX_CASES = [
     (3, (12, 9, 11), 5, (21, 13), 25.0, 25.0, (7, 5), "blob"),
     (2, (31, 17), 11, (80,), 40.0, 30.0, (16, 1), "blob"),
     (2, (48, 48), 20, (70,), 4800.0, 240.0, (32, 1), "bspline"),
+    # a table whose last entry is not 0 (the kernels' range-tested taps)
+    (3, (20, 14, 17), 9, (40, 26), 60.0, 40.0, (16, 8), "raised"),
+    (2, (31, 17), 11, (80,), 40.0, 30.0, (16, 1), "raised"),
 ]
 
 
@@ -588,6 +591,9 @@ def test_voxel_tiled_forward_logic_matches_oracle(vox, case):
     g = oracle.circle_geometry(dim, vol, views, 360, s2c, c2d, det=det, angles_deg=angles)
     if basis == "blob":
         lut, support = oracle.blob_lut(2.0, 10.83, 2, 100, np.float32), 2.0
+    elif basis == "raised":
+        lut, support = (oracle.blob_lut(2.0, 10.83, 2, 100, np.float32) + np.float32(0.05)).astype(np.float32), 2.0
+        assert lut[-1] != 0.0
     else:
         lut, support = oracle.bspline_lut(dim, 2, 100, np.float32), 1.5
     x = rng.standard_normal(g.vol_shape).astype(np.float32)
@@ -615,6 +621,8 @@ def test_voxel_tiled_forward_fp64_logic_matches_oracle(vox, case):
     g = oracle.circle_geometry(dim, vol, views, 360, s2c, c2d, det=det, angles_deg=angles)
     if basis == "blob":
         lut, support = oracle.blob_lut(2.0, 10.83, 2, 100, np.float64), 2.0
+    elif basis == "raised":
+        lut, support = oracle.blob_lut(2.0, 10.83, 2, 100, np.float64) + 0.05, 2.0
     else:
         lut, support = oracle.bspline_lut(dim, 2, 100, np.float64), 1.5
     x = rng.standard_normal(g.vol_shape)
