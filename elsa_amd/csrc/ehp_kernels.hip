@@ -294,9 +294,11 @@ namespace ehp
                     float fi = (float) fp.lo[0]; // i as a float (exact), without a conversion per tap
                     for (int i = fp.lo[0]; i <= fp.hi[0]; ++i, fi += 1.0f) {
                         const float d0 = (fp.center[0] - fi) * rs;
-                        acc += tap_weight_fast<D, TZ>(s_lut, n_lut, kidx, __builtin_fmaf(d0, d0, d1s), d0,
-                                                      diff != 0) *
-                               up(sp[(int64_t) j * dims.det[0] + i]);
+                        // one FMA per tap: the sum of the taps in pose and pixel order, each product
+                        // unrounded
+                        acc = __builtin_fmaf(tap_weight_fast<D, TZ>(s_lut, n_lut, kidx, __builtin_fmaf(d0, d0, d1s),
+                                                                    d0, diff != 0),
+                                             up(sp[(int64_t) j * dims.det[0] + i]), acc);
                     }
                 }
             }
