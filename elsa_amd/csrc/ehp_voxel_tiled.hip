@@ -132,7 +132,8 @@ namespace ehp
             const int total = s_pref[npl];
             int jp = 0, pj = 0, pn = s_pref[1];
             PlaneRect pr = s_pr[0];
-            for (int k = tid; k < total; k += kThreads) {
+            // candidate k's voxel c and its linear index (k increasing from call to call)
+            auto locate = [&](int k, int* c) -> int64_t {
                 while (k >= pn) {
                     ++jp;
                     pj = pn;
@@ -141,16 +142,18 @@ namespace ehp
                 }
                 int row, col;
                 rect_split(k - pj, pr.nb, pr.rnb, row, col);
-                int c[3] = {0, 0, 0};
                 c[A] = pc0 + jp;
                 c[b1] = pr.blo + col;
                 if (D == 3)
                     c[b2] = pr.clo + row;
-                const int64_t lin = small ? (int64_t) (pr.base + mul24((uint32_t) col, s1) + mul24((uint32_t) row, s2))
-                                          : c[0] + c[1] * dims.strides[1] + (int64_t) c[2] * dims.strides[2];
-                const auto xv = up(vol[lin]);
+                return small ? (int64_t) (pr.base + mul24((uint32_t) col, s1) + mul24((uint32_t) row, s2))
+                             : c[0] + c[1] * dims.strides[1] + (int64_t) c[2] * dims.strides[2];
+            };
+            using XT = decltype(up(vol[0]));
+            // the taps of voxel c with value xv
+            auto visit = [&](const int* c, const XT xv) {
                 if (xv == 0)
-                    continue;
+                    return;
                 // FP64: footprint()'s divisions, as the one-thread-per-voxel FP64 kernel; FP32:
                 // footprint_fast, as the gather back projection
                 Footprint fp;
@@ -162,7 +165,7 @@ namespace ehp
                 const int lo0 = fp.lo[0] > u0 ? fp.lo[0] : u0, hi0 = fp.hi[0] < u1 ? fp.hi[0] : u1;
                 const int lo1 = fp.lo[1] > v0 ? fp.lo[1] : v0, hi1 = fp.hi[1] < v1 ? fp.hi[1] : v1;
                 if (lo0 > hi0 || lo1 > hi1)
-                    continue;
+                    return;
                 // x in units of q (a power-of-two scaling, exact); each tap adds x * w rounded once
                 // (|x w| < 2^51 units: the pixel bound is at least 4096 times the table's bound)
                 const double xq = ldexp((double) xv, 62 - eg);
@@ -212,6 +215,23 @@ namespace ehp
                     else
                         taps(std::false_type{});
                 }
+            };
+            // each lane loads its next candidate's value before it visits the current one, so that
+            // the load's latency is spent on that visit
+            int cur[3] = {0, 0, 0};
+            XT xcur = 0;
+            if (tid < total)
+                xcur = up(vol[locate(tid, cur)]);
+            for (int k = tid; k < total; k += kThreads) {
+                int nxt[3] = {0, 0, 0};
+                XT xnxt = 0;
+                if (k + kThreads < total)
+                    xnxt = up(vol[locate(k + kThreads, nxt)]);
+                visit(cur, xcur);
+                cur[0] = nxt[0];
+                cur[1] = nxt[1];
+                cur[2] = nxt[2];
+                xcur = xnxt;
             }
             __syncthreads(); // the next chunk rewrites the plane tables
         }
