@@ -230,14 +230,24 @@ namespace ehp
     template <int D, class T, class ST>
     __global__ void __launch_bounds__(256)
         voxel_adj_kernel(const PoseGeom* __restrict__ geo, const Dims dims, const T* __restrict__ lut, int n_lut,
-                         float support, const int diff, const ST* __restrict__ sino, ST* __restrict__ vol)
+                         float support, const int diff, const int per_xcd, const ST* __restrict__ sino,
+                         ST* __restrict__ vol)
     {
-        const int64_t vox = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+        // the FP32 gather's order (voxel_adj_fast_kernel): in 3D y slowest and a contiguous run of
+        // blocks per XCD, so that an XCD's detector rows stay in its L2; each voxel's sum is the same
+        const int64_t blk = (int64_t) (blockIdx.x % 8u) * per_xcd + (blockIdx.x / 8u);
+        const int64_t vox = blk * blockDim.x + threadIdx.x;
         if (vox >= dims.nvox)
             return;
         const int x = (int) (vox % dims.vol[0]);
-        const int y = (int) ((vox / dims.vol[0]) % dims.vol[1]);
-        const int z = (int) (vox / ((int64_t) dims.vol[0] * dims.vol[1]));
+        int y, z;
+        if (D == 3) {
+            z = (int) ((vox / dims.vol[0]) % dims.vol[2]);
+            y = (int) (vox / ((int64_t) dims.vol[0] * dims.vol[2]));
+        } else {
+            y = (int) ((vox / dims.vol[0]) % dims.vol[1]);
+            z = 0;
+        }
         T acc = T(0);
         for (int p = 0; p < dims.nposes; ++p) {
             Footprint fp;
@@ -247,7 +257,7 @@ namespace ehp
                 acc += tap_weight<D, T>(lut, n_lut, support, dist, d0, diff != 0) * up(sp[pix]);
             });
         }
-        vol[vox] = down<ST>(acc);
+        vol[x + (int64_t) y * dims.strides[1] + (int64_t) z * dims.strides[2]] = down<ST>(acc);
     }
 
     // The default FP32 gather: the footprints and taps of the tiled forward (footprint_fast, one
@@ -564,13 +574,16 @@ namespace ehp
     hipError_t voxel_adjoint(const PoseGeom* geo, const Dims& dims, const T* lut, int n_lut, float support,
                              int diff, const ST* sino, ST* vol, hipStream_t s)
     {
-        const dim3 grid((unsigned) ((dims.nvox + 255) / 256));
+        const int64_t per_xcd = ((dims.nvox + 255) / 256 + 7) / 8;
+        if (per_xcd * 8 > (int64_t) 0x7fffffff)
+            return hipErrorInvalidValue;
+        const dim3 grid((unsigned) (per_xcd * 8));
         if (dims.dim == 2)
             hipLaunchKernelGGL((voxel_adj_kernel<2, T, ST>), grid, dim3(256), 0, s, geo, dims, lut, n_lut, support,
-                               diff, sino, vol);
+                               diff, (int) per_xcd, sino, vol);
         else
             hipLaunchKernelGGL((voxel_adj_kernel<3, T, ST>), grid, dim3(256), 0, s, geo, dims, lut, n_lut, support,
-                               diff, sino, vol);
+                               diff, (int) per_xcd, sino, vol);
         return hipGetLastError();
     }
 
